@@ -1,0 +1,282 @@
+/*
+ * ps.h — C-ABI of the MI355X (gfx950) unified particle solver hot path.
+ *
+ * One opaque handle (ps_system) owns every device buffer and one HIP stream.
+ * It replaces the reference's process-global thrust::device_vectors and the
+ * per-stage extern "C" functions in gpu/src/cuda/wrappers.cuh:12-97,
+ * gpu/src/cuda/util.cuh:6-25 and gpu/src/cuda/shared_variables.cuh:13-36,
+ * whose only caller is ParticleSystem (gpu/src/particlesystem.cpp:88-245).
+ *
+ * Conventions
+ *  - Every function returns 0 (PS_OK) or a negative ps_error code; nothing
+ *    calls exit().  ps_last_error() gives a message for the last failure.
+ *  - Host arrays passed in are copied; none is retained after the call.
+ *  - Particles are addressed by insertion index ("original index"), as the
+ *    reference's VBO order is (gpu/src/particlesystem.cpp:338-342).  The
+ *    solver keeps its own storage in cell-sorted order; readbacks return
+ *    insertion order unless the name says "sorted".
+ *  - Work is enqueued on the handle's stream; ps_step and the stage calls
+ *    return before the GPU finishes.  Readbacks synchronise the stream.
+ *  - One handle per host thread.  Two handles never share device state.
+ *  - Phases follow gpu/src/cuda/shared_variables.cuh:4-9: FLUID 0, GAS 1,
+ *    CLOTH 2, SOLID 3, RIGID+k (k >= 0) for bodies that must not self-collide.
+ */
+#ifndef PS_H
+#define PS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PS_ABI_VERSION 1u
+
+/* Particle phases (gpu/src/cuda/shared_variables.cuh:4-9). */
+#define PS_PHASE_FLUID 0
+#define PS_PHASE_GAS 1
+#define PS_PHASE_CLOTH 2
+#define PS_PHASE_SOLID 3
+#define PS_PHASE_RIGID 4
+
+/* Error codes. */
+typedef enum ps_error {
+    PS_OK = 0,
+    PS_ERR_INVALID_ARGUMENT = -1, /* bad pointer, size or parameter value      */
+    PS_ERR_CAPACITY = -2,         /* would exceed max_particles or a table size */
+    PS_ERR_DEVICE = -3,           /* a HIP runtime call failed                  */
+    PS_ERR_NO_DEVICE = -4,        /* no usable GPU                              */
+    PS_ERR_STATE = -5,            /* call out of order (e.g. iterate w/o begin) */
+    PS_ERR_OUT_OF_MEMORY = -6,    /* device allocation failed                   */
+    PS_ERR_NOT_FINITE = -7        /* non-finite input values                    */
+} ps_error;
+
+/* Flags for ps_params.flags. */
+#define PS_FLAG_REBUILD_EVERY_ITERATION 0x1u /* reference order: grid per iteration
+                                                (gpu/src/particlesystem.cpp:171-197) */
+#define PS_FLAG_SLEEPING 0x2u /* zero v and keep x when |x_new - x| < 1e-4
+                                 (cpu/src/particle.h:60-65)                     */
+
+/* Status bits raised by the kernels, read with ps_get_status. */
+#define PS_STATUS_NEIGHBOR_OVERFLOW 0x1u /* a particle had more than max_neighbors
+                                            candidates within the search radius */
+#define PS_STATUS_NONFINITE 0x2u         /* a predicted position was NaN/Inf      */
+#define PS_STATUS_OUTSIDE_GRID 0x4u      /* a predicted position hashed outside the
+                                            grid and was clamped to an edge cell   */
+#define PS_STATUS_INTERNAL 0x80000000u   /* a kernel's own bounds check tripped
+                                            (inconsistent grid tables); the write
+                                            was skipped                             */
+
+/*
+ * Simulation parameters.  The reference splits these between SimParams
+ * (gpu/src/cuda/kernel.cuh:9-22, uploaded by setParameters
+ * gpu/src/cuda/integration.cu:96-100), the ParticleSystem constructor
+ * (gpu/src/particlesystem.cpp:36-69) and compile-time #defines
+ * (gpu/src/cuda/integration_kernel.cuh:20-42).  Here they are all runtime.
+ */
+typedef struct ps_params {
+    uint32_t abi_version;   /* must be PS_ABI_VERSION                              */
+    uint32_t dims;          /* 3, or 2 (z held fixed, grid[2] == 1)                */
+    float radius;           /* particle radius r                                   */
+    float h;                /* SPH smoothing radius                                */
+    float cell;             /* grid cell edge; must be >= max(h, collide_dist)     */
+    float collide_dist;     /* particle contact distance (reference 2.001 r)       */
+    float gravity[3];
+    float box_min[3];       /* world box; particles are kept r inside it           */
+    float box_max[3];
+    float origin[3];        /* grid origin (world position of cell (0,0,0))        */
+    uint32_t grid[3];       /* cells per axis; positions outside clamp to the edge */
+    uint32_t iterations;    /* solver iterations per substep                       */
+    float relaxation;       /* lambda relaxation epsilon                           */
+    float k_corr;           /* tensile correction s_corr = -k h^2 (W(r)/W(dq h))^n */
+    uint32_t n_corr;
+    float dq_corr;          /* dq as a fraction of h                               */
+    float xsph_c;           /* XSPH viscosity c (0 disables)                       */
+    float s_solid;          /* density weight of non-fluid neighbours of a fluid   */
+    float mu_s;             /* particle-particle static friction                   */
+    float mu_k;             /* particle-particle kinetic friction                  */
+    float wall_mu_s;        /* wall friction (reference uses sqrt of these)        */
+    float wall_mu_k;
+    float wall_jitter;      /* wall push-out jitter as a fraction of r             */
+    float shape_stiffness;  /* default shape-matching stiffness                    */
+    uint32_t max_particles; /* capacity fixed at ps_create                         */
+    uint32_t max_neighbors; /* neighbour-table width per particle                  */
+    uint64_t seed;          /* counter-based RNG seed for wall jitter              */
+    uint32_t flags;         /* PS_FLAG_*                                           */
+    uint32_t reserved[7];
+} ps_params;
+
+/* Constants derived from ps_params on the host (FP64, rounded once to FP32). */
+typedef struct ps_derived {
+    float h2;          /* h^2                                                     */
+    float poly6;       /* 315 / (64 pi h^9)        (integration_kernel.cuh:29)    */
+    float spiky;       /* 45 / (pi h^6)            (integration_kernel.cuh:30)    */
+    float w_zero;      /* poly6 * h^6 = W(0)                                      */
+    float w_dq;        /* poly6 * (h^2 - (dq h)^2)^3 = W(dq h)                    */
+    float inv_w_dq;    /* 1 / W(dq h)                                             */
+    float tensile_k;   /* k_corr * h^2: the tensile term in lambda's units (length^2) */
+    float search_r2;   /* neighbour search radius^2 = max(h, collide_dist)^2      */
+    float collide_r2;  /* collide_dist^2                                          */
+    float wall_s;      /* sqrt(wall_mu_s) * r                                     */
+    float wall_k;      /* sqrt(wall_mu_k) * r                                     */
+    float jitter;      /* wall_jitter * r                                         */
+    uint32_t num_cells;
+    uint32_t reserved[3];
+} ps_derived;
+
+typedef struct ps_status {
+    uint32_t flags;            /* PS_STATUS_* bits since the last ps_clear_status */
+    uint32_t max_neighbors;    /* largest candidate count seen                    */
+    uint32_t num_particles;
+    uint32_t num_cells;
+    uint64_t substeps;         /* substeps completed                              */
+    uint64_t iterations;       /* solver iterations completed                     */
+} ps_status;
+
+typedef struct ps_system ps_system;
+
+/* ---- parameters (host only; no GPU needed) ------------------------------ */
+
+/* Fill with the reference's GPU-demo constants (radius 0.25, 64^3 grid,
+ * box (-50,0,-50)-(50,200,50), 5 iterations: gpu/src/particleapp.cpp:65-80;
+ * fluid constants gpu/src/cuda/integration_kernel.cuh:20-42). */
+int ps_params_default(ps_params *params);
+/* Check a parameter set; returns PS_OK or PS_ERR_INVALID_ARGUMENT. */
+int ps_params_validate(const ps_params *params);
+/* Derived constants exactly as the kernels use them. */
+int ps_derive(const ps_params *params, ps_derived *out);
+
+/* ---- lifecycle ---------------------------------------------------------- */
+
+/* Replaces ParticleSystem::ParticleSystem/_init (gpu/src/particlesystem.cpp:36-112),
+ * initIntegration (gpu/src/cuda/integration.cu:43-48), allocateArray
+ * (gpu/src/cuda/util.cu:12-15) and setParameters (integration.cu:96-100).
+ * Allocates every buffer for max_particles up front. */
+int ps_create(const ps_params *params, ps_system **out);
+/* Replaces _finalize (gpu/src/particlesystem.cpp:115-135) and the
+ * free*Vectors calls (integration.cu:74-94, solver.cu:158-178,
+ * shared_variables.cu:12-23). */
+int ps_destroy(ps_system *sys);
+/* Replaces setParameters (integration.cu:96-100).  Capacities, grid and
+ * dims cannot change after ps_create. */
+int ps_set_params(ps_system *sys, const ps_params *params);
+int ps_get_params(const ps_system *sys, ps_params *out);
+const char *ps_last_error(const ps_system *sys);
+
+/* ---- scene building ------------------------------------------------------ */
+
+/* Replaces addParticleMultiple (gpu/src/particlesystem.cpp:333-348), i.e.
+ * appendIntegrationParticle (integration.cu:50-72), appendPhaseAndMass
+ * (shared_variables.cu:32-50) and appendSolverParticle (solver.cu:64-70).
+ * pos, vel: n*3 floats (vel may be NULL for zero); inv_mass, rest_density:
+ * n floats; phase: n ints.  *first_out (may be NULL) receives the index of
+ * the first new particle.  Exceeding max_particles is PS_ERR_CAPACITY (the
+ * reference silently drops the batch, particlesystem.cpp:335-336). */
+int ps_add_particles(ps_system *sys, uint32_t n, const float *pos, const float *vel,
+                     const float *inv_mass, const int32_t *phase,
+                     const float *rest_density, uint32_t *first_out);
+
+/* Replaces addDistanceConstraint + updateOccurences (solver.cu:72-106,125-156).
+ * pairs: 2*m particle indices; rest: m lengths; stiffness: m values in (0,1]
+ * (NULL means 1). */
+int ps_add_distance_constraints(ps_system *sys, uint32_t m, const uint32_t *pairs,
+                                const float *rest, const float *stiffness);
+
+/* Replaces addPointConstraint (solver.cu:108-123): particle idx[k] is held
+ * at xyz[3k..3k+2] after every iteration. */
+int ps_add_point_constraints(ps_system *sys, uint32_t m, const uint32_t *idx,
+                             const float *xyz);
+
+/* Shape-matching cluster over particles idx[0..n) (rest shape = their
+ * current positions).  The GPU reference has only a stub
+ * (gpu/src/cuda/solver_kernel.cuh:289-312); the semantics follow the CPU
+ * createRigidBody + TotalShapeConstraint (cpu/src/simulation.cpp:371-406,
+ * cpu/src/constraint/totalshapeconstraint.cpp:14-24) in 3D.
+ * stiffness <= 0 uses params.shape_stiffness. */
+int ps_add_rigid_body(ps_system *sys, uint32_t n, const uint32_t *idx, float stiffness,
+                      uint32_t *body_out);
+
+/* Overwrite velocities of particles [first, first+n) (n*3 floats). */
+int ps_set_velocities(ps_system *sys, uint32_t first, uint32_t n, const float *vel);
+
+/* ---- the substep (hot path) --------------------------------------------- */
+
+/* ParticleSystem::update (gpu/src/particlesystem.cpp:144-246) for
+ * `substeps` substeps of dt each: predict, grid, neighbours, `iterations`
+ * Jacobi iterations, velocity + XSPH.  No dt clamp (the reference clamps to
+ * 0.05, :149).  Asynchronous. */
+int ps_step(ps_system *sys, float dt, uint32_t substeps);
+
+/* The same substep in its stages, for callers that interleave work and for
+ * stage-level parity tests:
+ *   begin   = integrateSystem (integration.cu:122-135) + calcHash,
+ *             sortParticles, reorderDataAndFindCellStart (integration.cu:161-275)
+ *             + neighbour collection (integration_kernel.cuh:303-356,481-559)
+ *   iterate = one pass of the loop body (particlesystem.cpp:171-235): particle
+ *             contacts, PBF lambda/delta-p, distance, shape, walls, pins
+ *   end     = calcVelocity (integration.cu:416-426) + XSPH, positions committed */
+int ps_substep_begin(ps_system *sys, float dt);
+int ps_substep_iterate(ps_system *sys);
+int ps_substep_end(ps_system *sys);
+
+/* Finer stages (begin = predict + build_grid + find_neighbors). */
+int ps_predict(ps_system *sys, float dt);
+int ps_build_grid(ps_system *sys);
+int ps_find_neighbors(ps_system *sys);
+
+int ps_synchronize(ps_system *sys);
+
+/* ---- readback (synchronous) ---------------------------------------------- */
+
+uint32_t ps_num_particles(const ps_system *sys); /* getNumParticles, particlesystem.h:48 */
+/* Positions / velocities of particles [first, first+n) in insertion order,
+ * n*3 floats.  The reference keeps positions in a GL VBO
+ * (particlesystem.h:47); copyArrayFromDevice (util.cu:64-67) is its readback. */
+int ps_read_positions(ps_system *sys, uint32_t first, uint32_t n, float *xyz);
+int ps_read_velocities(ps_system *sys, uint32_t first, uint32_t n, float *xyz);
+
+/* Internal sorted-order state, for parity tests and tools.
+ * sorted_index[t] = insertion index of the particle in sorted slot t;
+ * cell[t] = its cell key; cell_offsets: num_cells+1 entries, particles of
+ * cell c occupy slots [off[c], off[c+1]) (reference cellStart/cellEnd,
+ * integration_kernel.cuh:229-299). Any pointer may be NULL. */
+int ps_read_grid(ps_system *sys, uint32_t *sorted_index, uint32_t *cell,
+                 uint32_t *cell_offsets);
+/* counts: n entries; table: n*max_neighbors entries, row t holds the sorted
+ * slots of slot t's neighbours in ascending order (entries past the count
+ * are unspecified). */
+int ps_read_neighbors(ps_system *sys, uint32_t *counts, uint32_t *table);
+/* Current iterate in sorted order: n*4 floats (x, y, z, lambda).
+ * start: positions at the start of the substep, sorted order, n*3 floats. */
+int ps_read_sorted_state(ps_system *sys, float *iterate_xyzl, float *start_xyz);
+
+int ps_get_status(ps_system *sys, ps_status *out);
+int ps_clear_status(ps_system *sys);
+
+/* ---- per-kernel timing with HIP events on the handle's stream ----------- */
+
+typedef enum ps_kernel_id {
+    PS_K_PREDICT_HASH = 0,
+    PS_K_SCAN,
+    PS_K_SCATTER,
+    PS_K_REORDER,
+    PS_K_NEIGHBORS,
+    PS_K_LAMBDA,
+    PS_K_SOLVE,
+    PS_K_SHAPE,
+    PS_K_FINALIZE,
+    PS_K_COUNT
+} ps_kernel_id;
+
+int ps_timing_enable(ps_system *sys, int enable);
+int ps_timing_reset(ps_system *sys);
+/* Total milliseconds and launch count for a kernel since the last reset. */
+int ps_timing_read(ps_system *sys, int kernel_id, double *total_ms, uint64_t *launches);
+const char *ps_kernel_name(int kernel_id);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PS_H */

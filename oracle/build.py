@@ -1,0 +1,34 @@
+"""Build recipe of the CPU oracle (test infrastructure only).
+
+ps_oracle.c is compiled twice with gcc: FP32 (liboracle_f32.so) and FP64
+(liboracle_f64.so).  -ffp-contract=off and no -march/-ffast-math keep the
+FP32 build to exactly the IEEE operations written in the source.  OpenMP
+parallelises the per-particle loops (each particle is a pure gather, so the
+results do not depend on the thread count).
+"""
+import os
+import subprocess
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SRC = os.path.join(HERE, "ps_oracle.c")
+
+CFLAGS = ["-std=c99", "-O2", "-fPIC", "-shared", "-ffp-contract=off", "-fno-fast-math",
+          "-fopenmp", "-Wall", "-Wno-unknown-pragmas"]
+
+
+def build_oracle(verbose=True):
+    outs = []
+    for dbl, name in ((0, "liboracle_f32.so"), (1, "liboracle_f64.so")):
+        out = os.path.join(HERE, name)
+        tmp = out + ".tmp"
+        cmd = ["gcc"] + CFLAGS + ["-DORC_DOUBLE=%d" % dbl, SRC, "-o", tmp, "-lm"]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+        os.replace(tmp, out)
+        outs.append(out)
+    return outs
+
+
+if __name__ == "__main__":
+    build_oracle()

@@ -1,0 +1,358 @@
+"""ctypes binding of the C-ABI declared in include/ps.h.
+
+The library (libparticlesolver.so, built in-tree by __graft_entry__.build())
+is the product: every call here goes straight into the HIP implementation.
+There is no CPU fallback; if the library is missing, loading raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libparticlesolver.so")
+
+PS_ABI_VERSION = 1
+
+PHASE_FLUID = 0
+PHASE_GAS = 1
+PHASE_CLOTH = 2
+PHASE_SOLID = 3
+PHASE_RIGID = 4
+
+PS_OK = 0
+PS_ERR_INVALID_ARGUMENT = -1
+PS_ERR_CAPACITY = -2
+PS_ERR_DEVICE = -3
+PS_ERR_NO_DEVICE = -4
+PS_ERR_STATE = -5
+PS_ERR_OUT_OF_MEMORY = -6
+PS_ERR_NOT_FINITE = -7
+
+PS_FLAG_REBUILD_EVERY_ITERATION = 0x1
+PS_FLAG_SLEEPING = 0x2
+
+PS_STATUS_NEIGHBOR_OVERFLOW = 0x1
+PS_STATUS_NONFINITE = 0x2
+PS_STATUS_OUTSIDE_GRID = 0x4
+PS_STATUS_INTERNAL = 0x80000000
+
+KERNELS = ["predict_hash", "scan", "scatter", "reorder", "neighbors", "lambda", "solve",
+           "shape", "finalize"]
+K_PREDICT_HASH, K_SCAN, K_SCATTER, K_REORDER, K_NEIGHBORS, K_LAMBDA, K_SOLVE, K_SHAPE, \
+    K_FINALIZE = range(9)
+
+
+class PsParams(ctypes.Structure):
+    _fields_ = [
+        ("abi_version", ctypes.c_uint32),
+        ("dims", ctypes.c_uint32),
+        ("radius", ctypes.c_float),
+        ("h", ctypes.c_float),
+        ("cell", ctypes.c_float),
+        ("collide_dist", ctypes.c_float),
+        ("gravity", ctypes.c_float * 3),
+        ("box_min", ctypes.c_float * 3),
+        ("box_max", ctypes.c_float * 3),
+        ("origin", ctypes.c_float * 3),
+        ("grid", ctypes.c_uint32 * 3),
+        ("iterations", ctypes.c_uint32),
+        ("relaxation", ctypes.c_float),
+        ("k_corr", ctypes.c_float),
+        ("n_corr", ctypes.c_uint32),
+        ("dq_corr", ctypes.c_float),
+        ("xsph_c", ctypes.c_float),
+        ("s_solid", ctypes.c_float),
+        ("mu_s", ctypes.c_float),
+        ("mu_k", ctypes.c_float),
+        ("wall_mu_s", ctypes.c_float),
+        ("wall_mu_k", ctypes.c_float),
+        ("wall_jitter", ctypes.c_float),
+        ("shape_stiffness", ctypes.c_float),
+        ("max_particles", ctypes.c_uint32),
+        ("max_neighbors", ctypes.c_uint32),
+        ("seed", ctypes.c_uint64),
+        ("flags", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32 * 7),
+    ]
+
+    def copy(self):
+        out = PsParams()
+        ctypes.memmove(ctypes.byref(out), ctypes.byref(self), ctypes.sizeof(PsParams))
+        return out
+
+    def as_dict(self):
+        d = {}
+        for name, _ in self._fields_:
+            v = getattr(self, name)
+            d[name] = list(v) if hasattr(v, "__len__") else v
+        return d
+
+
+class PsDerived(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_float) for n in
+                ("h2", "poly6", "spiky", "w_zero", "w_dq", "inv_w_dq", "tensile_k", "search_r2",
+                 "collide_r2", "wall_s", "wall_k", "jitter")] + [
+        ("num_cells", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 3)]
+
+
+class PsStatus(ctypes.Structure):
+    _fields_ = [("flags", ctypes.c_uint32), ("max_neighbors", ctypes.c_uint32),
+                ("num_particles", ctypes.c_uint32), ("num_cells", ctypes.c_uint32),
+                ("substeps", ctypes.c_uint64), ("iterations", ctypes.c_uint64)]
+
+
+_P = ctypes.c_void_p
+_u32 = ctypes.c_uint32
+_f32 = ctypes.c_float
+_int = ctypes.c_int
+
+# name -> (restype, argtypes); the full exported surface of include/ps.h
+PROTOTYPES = {
+    "ps_params_default": (_int, [ctypes.POINTER(PsParams)]),
+    "ps_params_validate": (_int, [ctypes.POINTER(PsParams)]),
+    "ps_derive": (_int, [ctypes.POINTER(PsParams), ctypes.POINTER(PsDerived)]),
+    "ps_create": (_int, [ctypes.POINTER(PsParams), ctypes.POINTER(_P)]),
+    "ps_destroy": (_int, [_P]),
+    "ps_set_params": (_int, [_P, ctypes.POINTER(PsParams)]),
+    "ps_get_params": (_int, [_P, ctypes.POINTER(PsParams)]),
+    "ps_last_error": (ctypes.c_char_p, [_P]),
+    "ps_add_particles": (_int, [_P, _u32, _P, _P, _P, _P, _P, ctypes.POINTER(_u32)]),
+    "ps_add_distance_constraints": (_int, [_P, _u32, _P, _P, _P]),
+    "ps_add_point_constraints": (_int, [_P, _u32, _P, _P]),
+    "ps_add_rigid_body": (_int, [_P, _u32, _P, _f32, ctypes.POINTER(_u32)]),
+    "ps_set_velocities": (_int, [_P, _u32, _u32, _P]),
+    "ps_step": (_int, [_P, _f32, _u32]),
+    "ps_substep_begin": (_int, [_P, _f32]),
+    "ps_substep_iterate": (_int, [_P]),
+    "ps_substep_end": (_int, [_P]),
+    "ps_predict": (_int, [_P, _f32]),
+    "ps_build_grid": (_int, [_P]),
+    "ps_find_neighbors": (_int, [_P]),
+    "ps_synchronize": (_int, [_P]),
+    "ps_num_particles": (_u32, [_P]),
+    "ps_read_positions": (_int, [_P, _u32, _u32, _P]),
+    "ps_read_velocities": (_int, [_P, _u32, _u32, _P]),
+    "ps_read_grid": (_int, [_P, _P, _P, _P]),
+    "ps_read_neighbors": (_int, [_P, _P, _P]),
+    "ps_read_sorted_state": (_int, [_P, _P, _P]),
+    "ps_get_status": (_int, [_P, ctypes.POINTER(PsStatus)]),
+    "ps_clear_status": (_int, [_P]),
+    "ps_timing_enable": (_int, [_P, _int]),
+    "ps_timing_reset": (_int, [_P]),
+    "ps_timing_read": (_int, [_P, _int, ctypes.POINTER(ctypes.c_double),
+                              ctypes.POINTER(ctypes.c_uint64)]),
+    "ps_kernel_name": (ctypes.c_char_p, [_int]),
+}
+
+_lib = None
+
+
+def load(path=None):
+    """Load the native library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise RuntimeError("native library %s is missing: run __graft_entry__.build()" % p)
+    lib = ctypes.CDLL(p)
+    for name, (res, args) in PROTOTYPES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+class PsError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s (code %d)" % (msg, code))
+        self.code = code
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def default_params():
+    lib = load()
+    p = PsParams()
+    rc = lib.ps_params_default(ctypes.byref(p))
+    assert rc == 0
+    return p
+
+
+def derive(params):
+    lib = load()
+    d = PsDerived()
+    rc = lib.ps_derive(ctypes.byref(params), ctypes.byref(d))
+    if rc:
+        raise PsError(rc, lib.ps_last_error(None).decode())
+    return d
+
+
+class System:
+    """One particle system on the GPU (wraps a ps_system handle)."""
+
+    def __init__(self, params):
+        self.lib = load()
+        self.params = params.copy()
+        h = ctypes.c_void_p()
+        rc = self.lib.ps_create(ctypes.byref(self.params), ctypes.byref(h))
+        if rc:
+            raise PsError(rc, self.lib.ps_last_error(None).decode())
+        self.h = h
+
+    # -- plumbing
+    def _check(self, rc):
+        if rc:
+            raise PsError(rc, self.lib.ps_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.ps_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- scene
+    def add_particles(self, pos, vel=None, inv_mass=None, phase=None, rest_density=None):
+        pos = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1, 3)
+        n = pos.shape[0]
+        vel = None if vel is None else np.ascontiguousarray(vel, dtype=np.float32).reshape(n, 3)
+        w = np.ascontiguousarray(np.broadcast_to(1.0 if inv_mass is None else inv_mass, (n,)),
+                                 dtype=np.float32)
+        ph = np.ascontiguousarray(np.broadcast_to(PHASE_FLUID if phase is None else phase, (n,)),
+                                  dtype=np.int32)
+        r0 = np.ascontiguousarray(np.broadcast_to(1.0 if rest_density is None else rest_density,
+                                                  (n,)), dtype=np.float32)
+        first = ctypes.c_uint32()
+        self._check(self.lib.ps_add_particles(self.h, n, _ptr(pos), _ptr(vel), _ptr(w), _ptr(ph),
+                                              _ptr(r0), ctypes.byref(first)))
+        return first.value
+
+    def add_distance_constraints(self, pairs, rest, stiffness=None):
+        pairs = np.ascontiguousarray(pairs, dtype=np.uint32).reshape(-1, 2)
+        rest = np.ascontiguousarray(rest, dtype=np.float32).reshape(-1)
+        st = None if stiffness is None else np.ascontiguousarray(stiffness, dtype=np.float32)
+        self._check(self.lib.ps_add_distance_constraints(self.h, pairs.shape[0], _ptr(pairs),
+                                                         _ptr(rest), _ptr(st)))
+
+    def add_point_constraints(self, idx, xyz):
+        idx = np.ascontiguousarray(idx, dtype=np.uint32).reshape(-1)
+        xyz = np.ascontiguousarray(xyz, dtype=np.float32).reshape(-1, 3)
+        self._check(self.lib.ps_add_point_constraints(self.h, idx.shape[0], _ptr(idx), _ptr(xyz)))
+
+    def add_rigid_body(self, idx, stiffness=0.0):
+        idx = np.ascontiguousarray(idx, dtype=np.uint32).reshape(-1)
+        b = ctypes.c_uint32()
+        self._check(self.lib.ps_add_rigid_body(self.h, idx.shape[0], _ptr(idx), stiffness,
+                                               ctypes.byref(b)))
+        return b.value
+
+    def set_velocities(self, vel, first=0):
+        vel = np.ascontiguousarray(vel, dtype=np.float32).reshape(-1, 3)
+        self._check(self.lib.ps_set_velocities(self.h, first, vel.shape[0], _ptr(vel)))
+
+    def set_params(self, params):
+        self._check(self.lib.ps_set_params(self.h, ctypes.byref(params)))
+        self.params = params.copy()
+
+    # -- stepping
+    def step(self, dt, substeps=1):
+        self._check(self.lib.ps_step(self.h, dt, substeps))
+
+    def begin(self, dt):
+        self._check(self.lib.ps_substep_begin(self.h, dt))
+
+    def iterate(self):
+        self._check(self.lib.ps_substep_iterate(self.h))
+
+    def end(self):
+        self._check(self.lib.ps_substep_end(self.h))
+
+    def predict(self, dt):
+        self._check(self.lib.ps_predict(self.h, dt))
+
+    def build_grid(self):
+        self._check(self.lib.ps_build_grid(self.h))
+
+    def find_neighbors(self):
+        self._check(self.lib.ps_find_neighbors(self.h))
+
+    def synchronize(self):
+        self._check(self.lib.ps_synchronize(self.h))
+
+    # -- readback
+    @property
+    def n(self):
+        return self.lib.ps_num_particles(self.h)
+
+    def positions(self):
+        out = np.empty((self.n, 3), dtype=np.float32)
+        self._check(self.lib.ps_read_positions(self.h, 0, self.n, _ptr(out)))
+        return out
+
+    def velocities(self):
+        out = np.empty((self.n, 3), dtype=np.float32)
+        self._check(self.lib.ps_read_velocities(self.h, 0, self.n, _ptr(out)))
+        return out
+
+    def grid(self):
+        n = self.n
+        nc = self.params.grid[0] * self.params.grid[1] * self.params.grid[2]
+        idx = np.empty(n, np.uint32)
+        cell = np.empty(n, np.uint32)
+        off = np.empty(nc + 1, np.uint32)
+        self._check(self.lib.ps_read_grid(self.h, _ptr(idx), _ptr(cell), _ptr(off)))
+        return idx, cell, off
+
+    def neighbors(self):
+        n, k = self.n, self.params.max_neighbors
+        cnt = np.empty(n, np.uint32)
+        tab = np.empty((n, k), np.uint32)
+        self._check(self.lib.ps_read_neighbors(self.h, _ptr(cnt), _ptr(tab)))
+        return cnt, tab
+
+    def sorted_state(self):
+        n = self.n
+        it = np.empty((n, 4), np.float32)
+        st = np.empty((n, 3), np.float32)
+        self._check(self.lib.ps_read_sorted_state(self.h, _ptr(it), _ptr(st)))
+        return it, st
+
+    def status(self):
+        s = PsStatus()
+        self._check(self.lib.ps_get_status(self.h, ctypes.byref(s)))
+        return s
+
+    def clear_status(self):
+        self._check(self.lib.ps_clear_status(self.h))
+
+    # -- timing
+    def timing(self, enable=True):
+        self._check(self.lib.ps_timing_enable(self.h, 1 if enable else 0))
+
+    def timing_reset(self):
+        self._check(self.lib.ps_timing_reset(self.h))
+
+    def timing_read(self):
+        out = {}
+        for k, name in enumerate(KERNELS):
+            ms = ctypes.c_double()
+            cnt = ctypes.c_uint64()
+            self._check(self.lib.ps_timing_read(self.h, k, ctypes.byref(ms), ctypes.byref(cnt)))
+            out[name] = (ms.value, cnt.value)
+        return out

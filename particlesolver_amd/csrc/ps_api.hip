@@ -1,0 +1,1123 @@
+// ps_api.hip — host implementation of the C-ABI in include/ps.h.
+//
+// Owns the device buffers of one particle system and enqueues the substep
+// kernels on the system's own stream.  No process globals: every handle is
+// independent (the reference's state is global thrust::device_vectors,
+// gpu/src/cuda/integration.cu:23-34, solver.cu:32-41,
+// shared_variables.cu:8-10).
+
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "ps_internal.hpp"
+
+using ps::Attr;
+using ps::DevBuffers;
+using ps::KConst;
+
+namespace {
+
+struct TimedLaunch {
+    int id;
+    hipEvent_t start, stop;
+};
+
+}  // namespace
+
+struct ps_system {
+    ps_params prm;
+    ps_derived der;
+    KConst kc;
+    DevBuffers b;
+    hipStream_t stream = nullptr;
+    uint32_t n = 0, cap = 0, ncell = 0, kmax = 0, tiles = 0;
+    // scene composition
+    bool uniform = true;     // every particle fluid with one inverse mass
+    float wu = 1.0f;
+    bool any_contact_phase = false;
+    // host copies of the constraint sets (insertion indices)
+    std::vector<uint32_t> dist_pairs;
+    std::vector<float> dist_rest, dist_stiff;
+    std::vector<uint32_t> occ;
+    std::vector<uint32_t> pin_of;
+    std::vector<float> pin_xyz;
+    std::vector<uint32_t> body_off{0}, body_idx, body_of, member_of;
+    std::vector<float> body_rest, body_stiff, body_mass;
+    size_t dist_cap = 0, pin_cap = 0, body_cap = 0, member_cap = 0;
+    // substep state
+    int stage = 0;          // 1 between predict and end
+    bool hashed = false;    // keys/ranks of the current storage are valid
+    bool nbr_valid = false; // neighbour table matches the current storage
+    float dt = 0.0f;
+    uint64_t substeps = 0, iters = 0;
+    // timing
+    bool timing = false;
+    std::vector<TimedLaunch> pending;
+    std::vector<hipEvent_t> free_events;
+    double total_ms[PS_K_COUNT] = {};
+    uint64_t launches[PS_K_COUNT] = {};
+    std::string err;
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(ps_system *s, int code, const std::string &msg)
+{
+    if (s) s->err = msg;
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(sys, call)                                                                  \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return fail((sys), e_ == hipErrorOutOfMemory ? PS_ERR_OUT_OF_MEMORY : PS_ERR_DEVICE, \
+                        std::string(#call) + ": " + hipGetErrorString(e_));                \
+    } while (0)
+
+bool is_finite(float x) { return isfinite(x); }
+
+void derive_constants(const ps_params &p, ps_derived &d)
+{
+    // POLY6 = 315/(64 pi h^9), SPIKEY = 45/(pi h^6)
+    // (gpu/src/cuda/integration_kernel.cuh:29-30), in FP64, rounded once.
+    const double pi = 3.14159265358979323846;
+    const double h = p.h, h2 = h * h, h3 = h2 * h, h6 = h3 * h3, h9 = h6 * h3;
+    const double poly6 = 315.0 / (64.0 * pi * h9);
+    const double spiky = 45.0 / (pi * h6);
+    const double dq = (double)p.dq_corr * h;
+    const double tq = h2 - dq * dq;
+    const double sr = p.h > p.collide_dist ? (double)p.h : (double)p.collide_dist;
+    memset(&d, 0, sizeof(d));
+    d.h2 = (float)h2;
+    d.poly6 = (float)poly6;
+    d.spiky = (float)spiky;
+    d.w_zero = (float)(poly6 * h6);
+    d.w_dq = (float)(poly6 * tq * tq * tq);
+    d.inv_w_dq = (float)(1.0 / (poly6 * tq * tq * tq));
+    // The tensile term is added to lambda (units length^2); the paper's
+    // dimensionless k measures lengths in units of h, hence k h^2.
+    d.tensile_k = (float)((double)p.k_corr * h2);
+    d.search_r2 = (float)(sr * sr);
+    d.collide_r2 = (float)((double)p.collide_dist * (double)p.collide_dist);
+    d.wall_s = (float)(sqrt((double)p.wall_mu_s) * (double)p.radius);
+    d.wall_k = (float)(sqrt((double)p.wall_mu_k) * (double)p.radius);
+    d.jitter = (float)((double)p.wall_jitter * (double)p.radius);
+    d.num_cells = p.grid[0] * p.grid[1] * p.grid[2];
+}
+
+void fill_kconst(const ps_params &p, const ps_derived &d, KConst &k)
+{
+    memset(&k, 0, sizeof(k));
+    k.h = p.h;
+    k.h2 = d.h2;
+    k.poly6 = d.poly6;
+    k.spiky = d.spiky;
+    k.w_zero = d.w_zero;
+    k.w_dq = d.w_dq;
+    k.inv_w_dq = d.inv_w_dq;
+    k.search_r2 = d.search_r2;
+    k.collide_r2 = d.collide_r2;
+    k.collide_dist = p.collide_dist;
+    k.wall_s = d.wall_s;
+    k.wall_k = d.wall_k;
+    k.jitter = d.jitter;
+    k.radius = p.radius;
+    k.relax = p.relaxation;
+    k.k_corr = d.tensile_k;
+    k.xsph_c = p.xsph_c;
+    k.s_solid = p.s_solid;
+    k.mu_s = p.mu_s;
+    k.mu_k = p.mu_k;
+    for (int a = 0; a < 3; a++) {
+        k.gravity[a] = p.gravity[a];
+        k.box_min[a] = p.box_min[a];
+        k.box_max[a] = p.box_max[a];
+        k.origin[a] = p.origin[a];
+        k.grid[a] = p.grid[a];
+    }
+    k.cell = p.cell;
+    k.n_corr = p.n_corr;
+    k.dims = p.dims;
+    k.seed = p.seed;
+}
+
+int validate(const ps_params *p, std::string &why)
+{
+    if (!p) {
+        why = "params is NULL";
+        return PS_ERR_INVALID_ARGUMENT;
+    }
+    if (p->abi_version != PS_ABI_VERSION) {
+        why = "abi_version mismatch";
+        return PS_ERR_INVALID_ARGUMENT;
+    }
+    if (p->dims != 2 && p->dims != 3) {
+        why = "dims must be 2 or 3";
+        return PS_ERR_INVALID_ARGUMENT;
+    }
+    const float pos[] = {p->radius, p->h, p->cell, p->collide_dist};
+    for (float v : pos)
+        if (!(v > 0.0f) || !is_finite(v)) {
+            why = "radius, h, cell and collide_dist must be positive and finite";
+            return PS_ERR_INVALID_ARGUMENT;
+        }
+    if (p->cell < p->h || p->cell < p->collide_dist) {
+        why = "cell must be >= max(h, collide_dist) (one-cell neighbour stencil)";
+        return PS_ERR_INVALID_ARGUMENT;
+    }
+    for (int a = 0; a < 3; a++) {
+        if (!is_finite(p->gravity[a]) || !is_finite(p->box_min[a]) || !is_finite(p->box_max[a]) ||
+            !is_finite(p->origin[a])) {
+            why = "gravity, box and origin must be finite";
+            return PS_ERR_INVALID_ARGUMENT;
+        }
+        if (p->grid[a] == 0 || p->grid[a] > (1u << 20)) {
+            why = "grid dimensions must be in [1, 2^20]";
+            return PS_ERR_INVALID_ARGUMENT;
+        }
+    }
+    if (p->dims == 2 && p->grid[2] != 1) {
+        why = "2D systems need grid[2] == 1";
+        return PS_ERR_INVALID_ARGUMENT;
+    }
+    const uint64_t cells = (uint64_t)p->grid[0] * p->grid[1] * p->grid[2];
+    if (cells >= (1ull << 31)) {
+        why = "too many grid cells";
+        return PS_ERR_INVALID_ARGUMENT;
+    }
+    if (p->iterations == 0 || p->iterations > 1000) {
+        why = "iterations must be in [1, 1000]";
+        return PS_ERR_INVALID_ARGUMENT;
+    }
+    if (p->max_particles == 0 || p->max_particles > (1u << 30)) {
+        why = "max_particles must be in [1, 2^30]";
+        return PS_ERR_INVALID_ARGUMENT;
+    }
+    if (p->max_neighbors == 0 || p->max_neighbors > 4096) {
+        why = "max_neighbors must be in [1, 4096]";
+        return PS_ERR_INVALID_ARGUMENT;
+    }
+    const float nonneg[] = {p->relaxation, p->k_corr, p->dq_corr, p->xsph_c, p->s_solid,
+                            p->mu_s, p->mu_k, p->wall_mu_s, p->wall_mu_k, p->wall_jitter,
+                            p->shape_stiffness};
+    for (float v : nonneg)
+        if (!(v >= 0.0f) || !is_finite(v)) {
+            why = "solver coefficients must be finite and non-negative";
+            return PS_ERR_INVALID_ARGUMENT;
+        }
+    if (!(p->dq_corr < 1.0f)) {
+        why = "dq_corr must be < 1";
+        return PS_ERR_INVALID_ARGUMENT;
+    }
+    for (int a = 0; a < 3; a++)
+        if (!(p->box_max[a] > p->box_min[a])) {
+            why = "box_max must exceed box_min";
+            return PS_ERR_INVALID_ARGUMENT;
+        }
+    return PS_OK;
+}
+
+template <typename T>
+int dev_alloc(ps_system *s, T **ptr, size_t count)
+{
+    void *p = nullptr;
+    HIP_TRY(s, hipMalloc(&p, sizeof(T) * (count ? count : 1)));
+    *ptr = (T *)p;
+    return PS_OK;
+}
+
+template <typename T>
+void dev_free(T *&p)
+{
+    if (p) (void)hipFree((void *)p);
+    p = nullptr;
+}
+
+void free_all(ps_system *s)
+{
+    DevBuffers &b = s->b;
+    dev_free(b.X); dev_free(b.X2); dev_free(b.P); dev_free(b.Q); dev_free(b.V); dev_free(b.V2);
+    dev_free(b.A); dev_free(b.A2); dev_free(b.key); dev_free(b.key2); dev_free(b.rank);
+    dev_free(b.tmp_orig); dev_free(b.slot_of); dev_free(b.cell_count); dev_free(b.cell_off);
+    dev_free(b.scan_part); dev_free(b.nbr); dev_free(b.ncnt); dev_free(b.dist_off);
+    dev_free(b.dist_adj); dev_free(b.dist_pairs); dev_free(b.dist_rs); dev_free(b.occ);
+    dev_free(b.pin_of); dev_free(b.pin_xyz); dev_free(b.body_off); dev_free(b.body_idx);
+    dev_free(b.body_rest); dev_free(b.body_stiff); dev_free(b.body_mass); dev_free(b.body_q);
+    dev_free(b.body_goal); dev_free(b.body_of); dev_free(b.member_of); dev_free(b.status);
+}
+
+// ---- timing -------------------------------------------------------------
+
+int timed_begin(ps_system *s, int id, TimedLaunch &tl)
+{
+    tl.id = id;
+    tl.start = tl.stop = nullptr;
+    if (!s->timing) return PS_OK;
+    for (hipEvent_t *e : {&tl.start, &tl.stop}) {
+        if (!s->free_events.empty()) {
+            *e = s->free_events.back();
+            s->free_events.pop_back();
+        } else {
+            HIP_TRY(s, hipEventCreate(e));
+        }
+    }
+    HIP_TRY(s, hipEventRecord(tl.start, s->stream));
+    return PS_OK;
+}
+
+int timed_end(ps_system *s, TimedLaunch &tl)
+{
+    if (!s->timing) return PS_OK;
+    HIP_TRY(s, hipEventRecord(tl.stop, s->stream));
+    s->pending.push_back(tl);
+    return PS_OK;
+}
+
+int timing_collect(ps_system *s)
+{
+    if (s->pending.empty()) return PS_OK;
+    HIP_TRY(s, hipStreamSynchronize(s->stream));
+    for (TimedLaunch &tl : s->pending) {
+        float ms = 0.0f;
+        HIP_TRY(s, hipEventElapsedTime(&ms, tl.start, tl.stop));
+        s->total_ms[tl.id] += ms;
+        s->launches[tl.id] += 1;
+        s->free_events.push_back(tl.start);
+        s->free_events.push_back(tl.stop);
+    }
+    s->pending.clear();
+    return PS_OK;
+}
+
+#define LAUNCH(sys, id, expr)                                                              \
+    do {                                                                                   \
+        TimedLaunch tl_;                                                                   \
+        int rc_ = timed_begin((sys), (id), tl_);                                           \
+        if (rc_) return rc_;                                                               \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return fail((sys), PS_ERR_DEVICE, std::string("launch ") + ps_kernel_name(id) + \
+                                                  ": " + hipGetErrorString(e_));          \
+        rc_ = timed_end((sys), tl_);                                                       \
+        if (rc_) return rc_;                                                               \
+    } while (0)
+
+// ---- stages -------------------------------------------------------------
+
+uint32_t features(const ps_system *s)
+{
+    uint32_t f = ps::F_FLUID;
+    if (s->any_contact_phase) f |= ps::F_CONTACT;
+    if (!s->dist_pairs.empty()) f |= ps::F_DIST;
+    if (s->body_off.size() > 1) f |= ps::F_SHAPE;
+    if (!s->pin_xyz.empty()) f |= ps::F_PIN;
+    return f;
+}
+
+bool uniform_fluid(const ps_system *s) { return s->uniform && features(s) == ps::F_FLUID; }
+
+int stage_hash(ps_system *s, bool predict, float dt)
+{
+    LAUNCH(s, PS_K_PREDICT_HASH, ps::launch_predict_hash(s->stream, s->b, s->kc, s->n, dt, predict, true));
+    s->hashed = true;
+    return PS_OK;
+}
+
+// scan + scatter + reorder on keys already computed; swaps the buffers
+int stage_sort(ps_system *s)
+{
+    LAUNCH(s, PS_K_SCAN, ps::launch_scan(s->stream, s->b, s->ncell));
+    LAUNCH(s, PS_K_SCATTER, ps::launch_scatter(s->stream, s->b, s->n));
+    LAUNCH(s, PS_K_REORDER, ps::launch_reorder(s->stream, s->b, s->n));
+    DevBuffers &b = s->b;
+    std::swap(b.X, b.X2);
+    std::swap(b.P, b.Q);
+    std::swap(b.V, b.V2);
+    std::swap(b.A, b.A2);
+    std::swap(b.key, b.key2);
+    s->hashed = false;
+    s->nbr_valid = false;
+    return PS_OK;
+}
+
+int stage_neighbors(ps_system *s)
+{
+    LAUNCH(s, PS_K_NEIGHBORS, ps::launch_neighbors(s->stream, s->b, s->kc, s->n, s->kmax));
+    s->nbr_valid = true;
+    return PS_OK;
+}
+
+int stage_iterate(ps_system *s)
+{
+    int rc;
+    if (s->prm.flags & PS_FLAG_REBUILD_EVERY_ITERATION) {
+        if ((rc = stage_hash(s, false, 0.0f))) return rc;
+        if ((rc = stage_sort(s))) return rc;
+        if ((rc = stage_neighbors(s))) return rc;
+    }
+    if (!s->nbr_valid) return fail(s, PS_ERR_STATE, "neighbour table is stale: call ps_find_neighbors");
+    const bool uni = uniform_fluid(s);
+    LAUNCH(s, PS_K_LAMBDA, ps::launch_lambda(s->stream, s->b, s->kc, s->n, uni, s->wu));
+    const uint32_t nbody = (uint32_t)s->body_off.size() - 1;
+    if (nbody) LAUNCH(s, PS_K_SHAPE, ps::launch_shape(s->stream, s->b, nbody));
+    LAUNCH(s, PS_K_SOLVE,
+           ps::launch_solve(s->stream, s->b, s->kc, s->n, features(s), uni, s->wu, s->iters));
+    std::swap(s->b.P, s->b.Q);
+    s->iters++;
+    return PS_OK;
+}
+
+int stage_end(ps_system *s)
+{
+    const bool uni = uniform_fluid(s);
+    LAUNCH(s, PS_K_FINALIZE,
+           ps::launch_finalize(s->stream, s->b, s->kc, s->n, s->dt,
+                               (s->prm.flags & PS_FLAG_SLEEPING) != 0, uni, s->wu));
+    std::swap(s->b.X, s->b.P);
+    s->substeps++;
+    s->stage = 0;
+    return PS_OK;
+}
+
+int check_sys(ps_system *s)
+{
+    if (!s) return fail(nullptr, PS_ERR_INVALID_ARGUMENT, "system is NULL");
+    return PS_OK;
+}
+
+// Upload the whole constraint incidence structure (insertion indices).
+int upload_constraints(ps_system *s)
+{
+    DevBuffers &b = s->b;
+    const size_t m = s->dist_rest.size();
+    if (m > s->dist_cap) {
+        size_t nc = m * 2;
+        dev_free(b.dist_adj);
+        dev_free(b.dist_pairs);
+        dev_free(b.dist_rs);
+        int rc;
+        if ((rc = dev_alloc(s, &b.dist_adj, 2 * nc))) return rc;
+        if ((rc = dev_alloc(s, &b.dist_pairs, nc))) return rc;
+        if ((rc = dev_alloc(s, &b.dist_rs, nc))) return rc;
+        s->dist_cap = nc;
+    }
+    std::vector<uint32_t> off(s->cap + 1, 0), adj(2 * m), fill(s->cap, 0);
+    for (size_t c = 0; c < m; c++) {
+        off[s->dist_pairs[2 * c] + 1]++;
+        off[s->dist_pairs[2 * c + 1] + 1]++;
+    }
+    for (uint32_t i = 0; i < s->cap; i++) off[i + 1] += off[i];
+    for (size_t c = 0; c < m; c++) {
+        uint32_t a = s->dist_pairs[2 * c], bb = s->dist_pairs[2 * c + 1];
+        adj[off[a] + fill[a]++] = (uint32_t)c;
+        adj[off[bb] + fill[bb]++] = (uint32_t)c;
+    }
+    std::vector<float> rs(2 * m);
+    for (size_t c = 0; c < m; c++) {
+        rs[2 * c] = s->dist_rest[c];
+        rs[2 * c + 1] = s->dist_stiff[c];
+    }
+    HIP_TRY(s, hipMemcpy(b.dist_off, off.data(), sizeof(uint32_t) * off.size(), hipMemcpyHostToDevice));
+    if (m) {
+        HIP_TRY(s, hipMemcpy(b.dist_adj, adj.data(), sizeof(uint32_t) * adj.size(), hipMemcpyHostToDevice));
+        HIP_TRY(s, hipMemcpy(b.dist_pairs, s->dist_pairs.data(), sizeof(uint32_t) * 2 * m,
+                             hipMemcpyHostToDevice));
+        HIP_TRY(s, hipMemcpy(b.dist_rs, rs.data(), sizeof(float) * 2 * m, hipMemcpyHostToDevice));
+    }
+    HIP_TRY(s, hipMemcpy(b.occ, s->occ.data(), sizeof(uint32_t) * s->cap, hipMemcpyHostToDevice));
+    return PS_OK;
+}
+
+int upload_pins(ps_system *s)
+{
+    DevBuffers &b = s->b;
+    const size_t np = s->pin_xyz.size() / 3;
+    if (np > s->pin_cap) {
+        dev_free(b.pin_xyz);
+        int rc = dev_alloc(s, &b.pin_xyz, np * 2);
+        if (rc) return rc;
+        s->pin_cap = np * 2;
+    }
+    std::vector<float> p4(4 * np);
+    for (size_t k = 0; k < np; k++) {
+        p4[4 * k] = s->pin_xyz[3 * k];
+        p4[4 * k + 1] = s->pin_xyz[3 * k + 1];
+        p4[4 * k + 2] = s->pin_xyz[3 * k + 2];
+        p4[4 * k + 3] = 0.0f;
+    }
+    if (np)
+        HIP_TRY(s, hipMemcpy(b.pin_xyz, p4.data(), sizeof(float) * 4 * np, hipMemcpyHostToDevice));
+    HIP_TRY(s, hipMemcpy(b.pin_of, s->pin_of.data(), sizeof(uint32_t) * s->cap, hipMemcpyHostToDevice));
+    HIP_TRY(s, hipMemcpy(b.occ, s->occ.data(), sizeof(uint32_t) * s->cap, hipMemcpyHostToDevice));
+    return PS_OK;
+}
+
+// Copy the sorted positions and insertion indices to the host.
+int read_sorted_positions(ps_system *s, std::vector<float4> &X, std::vector<Attr> &A)
+{
+    X.resize(s->n);
+    A.resize(s->n);
+    if (s->n == 0) return PS_OK;
+    HIP_TRY(s, hipStreamSynchronize(s->stream));
+    HIP_TRY(s, hipMemcpy(X.data(), s->b.X, sizeof(float4) * s->n, hipMemcpyDeviceToHost));
+    HIP_TRY(s, hipMemcpy(A.data(), s->b.A, sizeof(Attr) * s->n, hipMemcpyDeviceToHost));
+    return PS_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+
+extern "C" {
+
+int ps_params_default(ps_params *p)
+{
+    if (!p) return fail(nullptr, PS_ERR_INVALID_ARGUMENT, "params is NULL");
+    memset(p, 0, sizeof(*p));
+    p->abi_version = PS_ABI_VERSION;
+    p->dims = 3;
+    // gpu/src/particleapp.cpp:65-80: radius 0.25, 64^3 grid, 15000 particles,
+    // box (-50,0,-50)..(50,200,50), 5 iterations; cell = 2r
+    // (particlesystem.cpp:62).  H = 2 (integration_kernel.cuh:25) exceeds the
+    // one-cell stencil at cell 0.5, so the default grid uses cell = h = 2.
+    p->radius = 0.25f;
+    p->h = 2.0f;
+    p->cell = 2.0f;
+    p->collide_dist = 0.25f * 2.001f;   // integration_kernel.cuh:319
+    p->gravity[0] = 0.0f;
+    p->gravity[1] = -9.8f;
+    p->gravity[2] = 0.0f;
+    p->box_min[0] = -50.0f; p->box_min[1] = 0.0f; p->box_min[2] = -50.0f;
+    p->box_max[0] = 50.0f; p->box_max[1] = 200.0f; p->box_max[2] = 50.0f;
+    p->origin[0] = -50.0f; p->origin[1] = 0.0f; p->origin[2] = -50.0f;
+    p->grid[0] = 50; p->grid[1] = 100; p->grid[2] = 50;
+    p->iterations = 5;
+    p->relaxation = 0.01f;  // FLUID_RELAXATION :32
+    p->k_corr = 0.1f;       // K_P :33
+    p->n_corr = 4;          // E_P :34
+    p->dq_corr = 0.2f;      // DQ_P :35
+    p->xsph_c = 0.0f;       // the reference has no XSPH
+    p->s_solid = 1.0f;      // solids count fully in density (:577)
+    p->mu_s = 0.005f;       // S_FRICTION :39
+    p->mu_k = 0.0002f;      // K_FRICTION :40
+    p->wall_mu_s = 0.005f;
+    p->wall_mu_k = 0.0002f;
+    p->wall_jitter = 0.01f; // eps = d * 0.01 (:83,91)
+    p->shape_stiffness = 1.0f;
+    p->max_particles = 15000;
+    p->max_neighbors = 128;
+    p->seed = 1234;         // integration.cu:47
+    p->flags = 0;
+    return PS_OK;
+}
+
+int ps_params_validate(const ps_params *p)
+{
+    std::string why;
+    int rc = validate(p, why);
+    if (rc) return fail(nullptr, rc, why);
+    return PS_OK;
+}
+
+int ps_derive(const ps_params *p, ps_derived *out)
+{
+    std::string why;
+    int rc = validate(p, why);
+    if (rc) return fail(nullptr, rc, why);
+    if (!out) return fail(nullptr, PS_ERR_INVALID_ARGUMENT, "out is NULL");
+    derive_constants(*p, *out);
+    return PS_OK;
+}
+
+int ps_create(const ps_params *params, ps_system **out)
+{
+    std::string why;
+    int rc = validate(params, why);
+    if (rc) return fail(nullptr, rc, why);
+    if (!out) return fail(nullptr, PS_ERR_INVALID_ARGUMENT, "out is NULL");
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+        return fail(nullptr, PS_ERR_NO_DEVICE, "no HIP device available");
+    ps_system *s = new ps_system();
+    s->prm = *params;
+    derive_constants(s->prm, s->der);
+    fill_kconst(s->prm, s->der, s->kc);
+    s->cap = params->max_particles;
+    s->ncell = s->der.num_cells;
+    s->kmax = params->max_neighbors;
+    s->tiles = ps::scan_tiles(s->ncell);
+    s->occ.assign(s->cap, 0);
+    s->pin_of.assign(s->cap, 0);
+    s->body_of.assign(s->cap, 0);
+    s->member_of.assign(s->cap, 0);
+    const size_t c = s->cap;
+    DevBuffers &b = s->b;
+    b.stride = (uint32_t)((c + 63) / 64 * 64);
+#define ALLOC(ptr, cnt)                                 \
+    do {                                                \
+        int r_ = dev_alloc(s, &(ptr), (cnt));           \
+        if (r_) {                                       \
+            free_all(s);                                \
+            delete s;                                   \
+            return r_;                                  \
+        }                                               \
+    } while (0)
+    ALLOC(b.X, c); ALLOC(b.X2, c); ALLOC(b.P, c); ALLOC(b.Q, c); ALLOC(b.V, c); ALLOC(b.V2, c);
+    ALLOC(b.A, c); ALLOC(b.A2, c); ALLOC(b.key, c); ALLOC(b.key2, c); ALLOC(b.rank, c);
+    ALLOC(b.tmp_orig, c); ALLOC(b.slot_of, c);
+    ALLOC(b.cell_count, s->ncell); ALLOC(b.cell_off, (size_t)s->ncell + 1);
+    ALLOC(b.scan_part, s->tiles);
+    ALLOC(b.nbr, (size_t)s->kmax * b.stride); ALLOC(b.ncnt, c);
+    ALLOC(b.dist_off, c + 1); ALLOC(b.occ, c); ALLOC(b.pin_of, c);
+    ALLOC(b.body_of, c); ALLOC(b.member_of, c); ALLOC(b.status, 1);
+#undef ALLOC
+    hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipMemset(b.cell_count, 0, sizeof(uint32_t) * s->ncell);
+    if (e == hipSuccess) e = hipMemset(b.cell_off, 0, sizeof(uint32_t) * ((size_t)s->ncell + 1));
+    if (e == hipSuccess) e = hipMemset(b.ncnt, 0, sizeof(uint32_t) * c);
+    if (e == hipSuccess) e = hipMemset(b.dist_off, 0, sizeof(uint32_t) * (c + 1));
+    if (e == hipSuccess) e = hipMemset(b.occ, 0, sizeof(uint32_t) * c);
+    if (e == hipSuccess) e = hipMemset(b.pin_of, 0, sizeof(uint32_t) * c);
+    if (e == hipSuccess) e = hipMemset(b.body_of, 0, sizeof(uint32_t) * c);
+    if (e == hipSuccess) e = hipMemset(b.member_of, 0, sizeof(uint32_t) * c);
+    if (e == hipSuccess) e = hipMemset(b.status, 0, sizeof(ps::DevStatus));
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+        std::string msg = std::string("device setup: ") + hipGetErrorString(e);
+        if (s->stream) (void)hipStreamDestroy(s->stream);
+        free_all(s);
+        delete s;
+        return fail(nullptr, PS_ERR_DEVICE, msg);
+    }
+    *out = s;
+    return PS_OK;
+}
+
+int ps_destroy(ps_system *s)
+{
+    if (!s) return PS_OK;
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    for (TimedLaunch &tl : s->pending) {
+        (void)hipEventDestroy(tl.start);
+        (void)hipEventDestroy(tl.stop);
+    }
+    for (hipEvent_t e : s->free_events) (void)hipEventDestroy(e);
+    free_all(s);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    delete s;
+    return PS_OK;
+}
+
+int ps_set_params(ps_system *s, const ps_params *p)
+{
+    if (int rc = check_sys(s)) return rc;
+    std::string why;
+    int rc = validate(p, why);
+    if (rc) return fail(s, rc, why);
+    if (p->max_particles != s->prm.max_particles || p->max_neighbors != s->prm.max_neighbors ||
+        p->dims != s->prm.dims || memcmp(p->grid, s->prm.grid, sizeof(p->grid)) != 0)
+        return fail(s, PS_ERR_INVALID_ARGUMENT,
+                    "capacities, grid and dims are fixed at ps_create");
+    s->prm = *p;
+    derive_constants(s->prm, s->der);
+    fill_kconst(s->prm, s->der, s->kc);
+    return PS_OK;
+}
+
+int ps_get_params(const ps_system *s, ps_params *out)
+{
+    if (!s || !out) return fail(nullptr, PS_ERR_INVALID_ARGUMENT, "NULL argument");
+    *out = s->prm;
+    return PS_OK;
+}
+
+const char *ps_last_error(const ps_system *s)
+{
+    if (s && !s->err.empty()) return s->err.c_str();
+    return g_err.c_str();
+}
+
+int ps_add_particles(ps_system *s, uint32_t n, const float *pos, const float *vel,
+                     const float *inv_mass, const int32_t *phase, const float *rest_density,
+                     uint32_t *first_out)
+{
+    if (int rc = check_sys(s)) return rc;
+    if (s->stage) return fail(s, PS_ERR_STATE, "cannot add particles inside a substep");
+    if (n == 0) {
+        if (first_out) *first_out = s->n;
+        return PS_OK;
+    }
+    if (!pos || !inv_mass || !phase || !rest_density)
+        return fail(s, PS_ERR_INVALID_ARGUMENT, "NULL input array");
+    if ((uint64_t)s->n + n > s->cap)
+        return fail(s, PS_ERR_CAPACITY, "max_particles exceeded");
+    std::vector<float4> X(n), V(n);
+    std::vector<Attr> A(n);
+    std::vector<uint32_t> slot(n);
+    bool uni = s->uniform;
+    float wu = s->n ? s->wu : inv_mass[0];
+    bool contact = s->any_contact_phase;
+    for (uint32_t k = 0; k < n; k++) {
+        for (int a = 0; a < 3; a++) {
+            if (!is_finite(pos[3 * k + a]) || (vel && !is_finite(vel[3 * k + a])))
+                return fail(s, PS_ERR_NOT_FINITE, "non-finite position or velocity");
+        }
+        if (!(inv_mass[k] >= 0.0f) || !is_finite(inv_mass[k]))
+            return fail(s, PS_ERR_INVALID_ARGUMENT, "inverse mass must be finite and >= 0");
+        if (phase[k] < 0) return fail(s, PS_ERR_INVALID_ARGUMENT, "phase must be >= 0");
+        if (phase[k] == PS_PHASE_FLUID && inv_mass[k] == 0.0f)
+            // cpu/src/simulation.cpp:436-439 rejects fluids of infinite mass
+            return fail(s, PS_ERR_INVALID_ARGUMENT, "a fluid particle cannot have infinite mass");
+        if (phase[k] == PS_PHASE_FLUID && !(rest_density[k] > 0.0f && is_finite(rest_density[k])))
+            return fail(s, PS_ERR_INVALID_ARGUMENT, "fluid rest density must be > 0");
+        X[k] = make_float4(pos[3 * k], pos[3 * k + 1], pos[3 * k + 2], 0.0f);
+        V[k] = vel ? make_float4(vel[3 * k], vel[3 * k + 1], vel[3 * k + 2], 0.0f)
+                   : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        A[k].w = inv_mass[k];
+        A[k].phase = phase[k];
+        A[k].rho0 = rest_density[k];
+        A[k].orig = s->n + k;
+        slot[k] = s->n + k;
+        if (phase[k] != PS_PHASE_FLUID || inv_mass[k] != wu) uni = false;
+        if (phase[k] >= PS_PHASE_CLOTH) contact = true;
+    }
+    const size_t o = s->n;
+    DevBuffers &b = s->b;
+    HIP_TRY(s, hipStreamSynchronize(s->stream));
+    HIP_TRY(s, hipMemcpy(b.X + o, X.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
+    // the iterate starts equal to the positions (lambda 0)
+    HIP_TRY(s, hipMemcpy(b.P + o, X.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
+    HIP_TRY(s, hipMemcpy(b.V + o, V.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
+    HIP_TRY(s, hipMemcpy(b.A + o, A.data(), sizeof(Attr) * n, hipMemcpyHostToDevice));
+    HIP_TRY(s, hipMemcpy(b.slot_of + o, slot.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice));
+    if (first_out) *first_out = s->n;
+    s->n += n;
+    s->uniform = uni;
+    s->wu = wu;
+    s->any_contact_phase = contact;
+    s->nbr_valid = false;
+    return PS_OK;
+}
+
+int ps_add_distance_constraints(ps_system *s, uint32_t m, const uint32_t *pairs,
+                                const float *rest, const float *stiffness)
+{
+    if (int rc = check_sys(s)) return rc;
+    if (s->stage) return fail(s, PS_ERR_STATE, "cannot add constraints inside a substep");
+    if (m == 0) return PS_OK;
+    if (!pairs || !rest) return fail(s, PS_ERR_INVALID_ARGUMENT, "NULL input array");
+    for (uint32_t k = 0; k < m; k++) {
+        if (pairs[2 * k] >= s->n || pairs[2 * k + 1] >= s->n || pairs[2 * k] == pairs[2 * k + 1])
+            return fail(s, PS_ERR_INVALID_ARGUMENT, "constraint index out of range or degenerate");
+        if (!(rest[k] >= 0.0f) || !is_finite(rest[k]))
+            return fail(s, PS_ERR_INVALID_ARGUMENT, "rest length must be finite and >= 0");
+        if (stiffness && !(stiffness[k] > 0.0f && stiffness[k] <= 1.0f))
+            return fail(s, PS_ERR_INVALID_ARGUMENT, "stiffness must be in (0, 1]");
+    }
+    for (uint32_t k = 0; k < m; k++) {
+        s->dist_pairs.push_back(pairs[2 * k]);
+        s->dist_pairs.push_back(pairs[2 * k + 1]);
+        s->dist_rest.push_back(rest[k]);
+        s->dist_stiff.push_back(stiffness ? stiffness[k] : 1.0f);
+        s->occ[pairs[2 * k]]++;
+        s->occ[pairs[2 * k + 1]]++;
+    }
+    HIP_TRY(s, hipStreamSynchronize(s->stream));
+    return upload_constraints(s);
+}
+
+int ps_add_point_constraints(ps_system *s, uint32_t m, const uint32_t *idx, const float *xyz)
+{
+    if (int rc = check_sys(s)) return rc;
+    if (s->stage) return fail(s, PS_ERR_STATE, "cannot add constraints inside a substep");
+    if (m == 0) return PS_OK;
+    if (!idx || !xyz) return fail(s, PS_ERR_INVALID_ARGUMENT, "NULL input array");
+    for (uint32_t k = 0; k < m; k++) {
+        if (idx[k] >= s->n) return fail(s, PS_ERR_INVALID_ARGUMENT, "pin index out of range");
+        for (int a = 0; a < 3; a++)
+            if (!is_finite(xyz[3 * k + a])) return fail(s, PS_ERR_NOT_FINITE, "non-finite pin");
+    }
+    for (uint32_t k = 0; k < m; k++) {
+        const uint32_t pid = (uint32_t)(s->pin_xyz.size() / 3);
+        s->pin_xyz.push_back(xyz[3 * k]);
+        s->pin_xyz.push_back(xyz[3 * k + 1]);
+        s->pin_xyz.push_back(xyz[3 * k + 2]);
+        s->pin_of[idx[k]] = pid + 1;
+        s->occ[idx[k]]++;
+    }
+    HIP_TRY(s, hipStreamSynchronize(s->stream));
+    return upload_pins(s);
+}
+
+int ps_add_rigid_body(ps_system *s, uint32_t n, const uint32_t *idx, float stiffness,
+                      uint32_t *body_out)
+{
+    if (int rc = check_sys(s)) return rc;
+    if (s->stage) return fail(s, PS_ERR_STATE, "cannot add bodies inside a substep");
+    if (n < 2 || !idx)
+        // cpu/src/simulation.cpp:373-376: rigid bodies need at least 2 points
+        return fail(s, PS_ERR_INVALID_ARGUMENT, "a rigid body needs at least 2 particles");
+    if (stiffness > 1.0f || !is_finite(stiffness))
+        return fail(s, PS_ERR_INVALID_ARGUMENT, "stiffness must be <= 1");
+    std::vector<float4> X;
+    std::vector<Attr> A;
+    if (int rc = read_sorted_positions(s, X, A)) return rc;
+    std::vector<uint32_t> slot_of(s->n);
+    for (uint32_t t = 0; t < s->n; t++) slot_of[A[t].orig] = t;
+    for (uint32_t k = 0; k < n; k++) {
+        if (idx[k] >= s->n || s->body_of[idx[k]])
+            return fail(s, PS_ERR_INVALID_ARGUMENT, "body member out of range or already in a body");
+        if (A[slot_of[idx[k]]].w == 0.0f)
+            // cpu/src/simulation.cpp:387-390
+            return fail(s, PS_ERR_INVALID_ARGUMENT, "a rigid body cannot have a static particle");
+    }
+    for (uint32_t k = 0; k < n; k++)
+        for (uint32_t q = k + 1; q < n; q++)
+            if (idx[k] == idx[q]) return fail(s, PS_ERR_INVALID_ARGUMENT, "duplicate body member");
+    const uint32_t bnew = (uint32_t)s->body_off.size() - 1;
+    const uint32_t base = s->body_off.back();
+    // centre of mass and rest offsets (same float operations as the kernels)
+    float cx = 0.0f, cy = 0.0f, cz = 0.0f, M = 0.0f;
+    for (uint32_t k = 0; k < n; k++) {
+        const uint32_t t = slot_of[idx[k]];
+        const float m = 1.0f / A[t].w;
+        cx = cx + (m * X[t].x);
+        cy = cy + (m * X[t].y);
+        cz = cz + (m * X[t].z);
+        M = M + m;
+    }
+    cx = cx / M;
+    cy = cy / M;
+    cz = cz / M;
+    for (uint32_t k = 0; k < n; k++) {
+        const uint32_t t = slot_of[idx[k]];
+        s->body_idx.push_back(idx[k]);
+        s->body_rest.push_back(X[t].x - cx);
+        s->body_rest.push_back(X[t].y - cy);
+        s->body_rest.push_back(X[t].z - cz);
+        s->body_of[idx[k]] = bnew + 1;
+        s->member_of[idx[k]] = base + k;
+    }
+    s->body_off.push_back(base + n);
+    s->body_stiff.push_back(stiffness > 0.0f ? stiffness : s->prm.shape_stiffness);
+    s->body_mass.push_back(M);
+    // upload everything (bodies are added at scene-build time)
+    DevBuffers &b = s->b;
+    const size_t nb = s->body_stiff.size(), nm = s->body_idx.size();
+    if (nb > s->body_cap || nm > s->member_cap) {
+        // keep the quaternions of existing bodies
+        std::vector<float4> q(nb, make_float4(1.f, 0.f, 0.f, 0.f));
+        if (s->body_cap && nb > 1)
+            HIP_TRY(s, hipMemcpy(q.data(), b.body_q, sizeof(float4) * (nb - 1), hipMemcpyDeviceToHost));
+        size_t bc = nb * 2, mc = nm * 2;
+        dev_free(b.body_off); dev_free(b.body_idx); dev_free(b.body_rest); dev_free(b.body_stiff);
+        dev_free(b.body_mass); dev_free(b.body_q); dev_free(b.body_goal);
+        int rc;
+        if ((rc = dev_alloc(s, &b.body_off, bc + 1))) return rc;
+        if ((rc = dev_alloc(s, &b.body_idx, mc))) return rc;
+        if ((rc = dev_alloc(s, &b.body_rest, mc))) return rc;
+        if ((rc = dev_alloc(s, &b.body_stiff, bc))) return rc;
+        if ((rc = dev_alloc(s, &b.body_mass, bc))) return rc;
+        if ((rc = dev_alloc(s, &b.body_q, bc))) return rc;
+        if ((rc = dev_alloc(s, &b.body_goal, mc))) return rc;
+        HIP_TRY(s, hipMemcpy(b.body_q, q.data(), sizeof(float4) * nb, hipMemcpyHostToDevice));
+        s->body_cap = bc;
+        s->member_cap = mc;
+    } else {
+        float4 q1 = make_float4(1.f, 0.f, 0.f, 0.f);
+        HIP_TRY(s, hipMemcpy(b.body_q + bnew, &q1, sizeof(float4), hipMemcpyHostToDevice));
+    }
+    std::vector<float4> rest(nm);
+    for (size_t k = 0; k < nm; k++)
+        rest[k] = make_float4(s->body_rest[3 * k], s->body_rest[3 * k + 1], s->body_rest[3 * k + 2], 0.f);
+    HIP_TRY(s, hipMemcpy(b.body_off, s->body_off.data(), sizeof(uint32_t) * s->body_off.size(), hipMemcpyHostToDevice));
+    HIP_TRY(s, hipMemcpy(b.body_idx, s->body_idx.data(), sizeof(uint32_t) * nm, hipMemcpyHostToDevice));
+    HIP_TRY(s, hipMemcpy(b.body_rest, rest.data(), sizeof(float4) * nm, hipMemcpyHostToDevice));
+    HIP_TRY(s, hipMemcpy(b.body_stiff, s->body_stiff.data(), sizeof(float) * nb, hipMemcpyHostToDevice));
+    HIP_TRY(s, hipMemcpy(b.body_mass, s->body_mass.data(), sizeof(float) * nb, hipMemcpyHostToDevice));
+    HIP_TRY(s, hipMemcpy(b.body_of, s->body_of.data(), sizeof(uint32_t) * s->cap, hipMemcpyHostToDevice));
+    HIP_TRY(s, hipMemcpy(b.member_of, s->member_of.data(), sizeof(uint32_t) * s->cap, hipMemcpyHostToDevice));
+    if (body_out) *body_out = bnew;
+    return PS_OK;
+}
+
+int ps_set_velocities(ps_system *s, uint32_t first, uint32_t n, const float *vel)
+{
+    if (int rc = check_sys(s)) return rc;
+    if (s->stage) return fail(s, PS_ERR_STATE, "cannot set velocities inside a substep");
+    if ((uint64_t)first + n > s->n || (n && !vel))
+        return fail(s, PS_ERR_INVALID_ARGUMENT, "range out of bounds");
+    for (uint32_t k = 0; k < 3 * n; k++)
+        if (!is_finite(vel[k])) return fail(s, PS_ERR_NOT_FINITE, "non-finite velocity");
+    std::vector<Attr> A(s->n);
+    std::vector<float4> V(s->n);
+    HIP_TRY(s, hipStreamSynchronize(s->stream));
+    HIP_TRY(s, hipMemcpy(A.data(), s->b.A, sizeof(Attr) * s->n, hipMemcpyDeviceToHost));
+    HIP_TRY(s, hipMemcpy(V.data(), s->b.V, sizeof(float4) * s->n, hipMemcpyDeviceToHost));
+    for (uint32_t t = 0; t < s->n; t++) {
+        const uint32_t o = A[t].orig;
+        if (o >= first && o < first + n) {
+            const uint32_t k = o - first;
+            V[t] = make_float4(vel[3 * k], vel[3 * k + 1], vel[3 * k + 2], 0.0f);
+        }
+    }
+    HIP_TRY(s, hipMemcpy(s->b.V, V.data(), sizeof(float4) * s->n, hipMemcpyHostToDevice));
+    return PS_OK;
+}
+
+// ---- substep --------------------------------------------------------------
+
+int ps_predict(ps_system *s, float dt)
+{
+    if (int rc = check_sys(s)) return rc;
+    if (!is_finite(dt) || dt < 0.0f) return fail(s, PS_ERR_INVALID_ARGUMENT, "dt must be finite and >= 0");
+    LAUNCH(s, PS_K_PREDICT_HASH, ps::launch_predict_hash(s->stream, s->b, s->kc, s->n, dt, true, false));
+    s->dt = dt;
+    s->stage = 1;
+    s->hashed = false;
+    s->nbr_valid = false;
+    return PS_OK;
+}
+
+int ps_build_grid(ps_system *s)
+{
+    if (int rc = check_sys(s)) return rc;
+    if (s->stage != 1) return fail(s, PS_ERR_STATE, "ps_build_grid before ps_predict");
+    int rc;
+    if (!s->hashed && (rc = stage_hash(s, false, 0.0f))) return rc;
+    return stage_sort(s);
+}
+
+int ps_find_neighbors(ps_system *s)
+{
+    if (int rc = check_sys(s)) return rc;
+    if (s->stage != 1) return fail(s, PS_ERR_STATE, "ps_find_neighbors before ps_predict");
+    if (s->hashed) return fail(s, PS_ERR_STATE, "ps_find_neighbors before ps_build_grid");
+    return stage_neighbors(s);
+}
+
+int ps_substep_begin(ps_system *s, float dt)
+{
+    if (int rc = check_sys(s)) return rc;
+    if (!(dt > 0.0f) || !is_finite(dt)) return fail(s, PS_ERR_INVALID_ARGUMENT, "dt must be > 0");
+    int rc;
+    s->dt = dt;
+    if ((rc = stage_hash(s, true, dt))) return rc;  // fused predict + hash
+    s->stage = 1;
+    if ((rc = stage_sort(s))) return rc;
+    return stage_neighbors(s);
+}
+
+int ps_substep_iterate(ps_system *s)
+{
+    if (int rc = check_sys(s)) return rc;
+    if (s->stage != 1) return fail(s, PS_ERR_STATE, "ps_substep_iterate before ps_substep_begin");
+    return stage_iterate(s);
+}
+
+int ps_substep_end(ps_system *s)
+{
+    if (int rc = check_sys(s)) return rc;
+    if (s->stage != 1) return fail(s, PS_ERR_STATE, "ps_substep_end before ps_substep_begin");
+    if (!(s->dt > 0.0f)) return fail(s, PS_ERR_STATE, "substep has dt = 0");
+    return stage_end(s);
+}
+
+int ps_step(ps_system *s, float dt, uint32_t substeps)
+{
+    if (int rc = check_sys(s)) return rc;
+    if (s->stage) return fail(s, PS_ERR_STATE, "ps_step inside an open substep");
+    if (!(dt > 0.0f) || !is_finite(dt)) return fail(s, PS_ERR_INVALID_ARGUMENT, "dt must be > 0");
+    if (s->n == 0) return PS_OK;
+    for (uint32_t k = 0; k < substeps; k++) {
+        int rc;
+        if ((rc = ps_substep_begin(s, dt))) return rc;
+        for (uint32_t it = 0; it < s->prm.iterations; it++)
+            if ((rc = stage_iterate(s))) return rc;
+        if ((rc = stage_end(s))) return rc;
+    }
+    return PS_OK;
+}
+
+int ps_synchronize(ps_system *s)
+{
+    if (int rc = check_sys(s)) return rc;
+    HIP_TRY(s, hipStreamSynchronize(s->stream));
+    return PS_OK;
+}
+
+// ---- readback -------------------------------------------------------------
+
+uint32_t ps_num_particles(const ps_system *s) { return s ? s->n : 0; }
+
+int ps_read_positions(ps_system *s, uint32_t first, uint32_t n, float *xyz)
+{
+    if (int rc = check_sys(s)) return rc;
+    if ((uint64_t)first + n > s->n || (n && !xyz))
+        return fail(s, PS_ERR_INVALID_ARGUMENT, "range out of bounds");
+    std::vector<float4> X;
+    std::vector<Attr> A;
+    if (int rc = read_sorted_positions(s, X, A)) return rc;
+    for (uint32_t t = 0; t < s->n; t++) {
+        const uint32_t o = A[t].orig;
+        if (o >= first && o < first + n) {
+            xyz[3 * (o - first)] = X[t].x;
+            xyz[3 * (o - first) + 1] = X[t].y;
+            xyz[3 * (o - first) + 2] = X[t].z;
+        }
+    }
+    return PS_OK;
+}
+
+int ps_read_velocities(ps_system *s, uint32_t first, uint32_t n, float *xyz)
+{
+    if (int rc = check_sys(s)) return rc;
+    if ((uint64_t)first + n > s->n || (n && !xyz))
+        return fail(s, PS_ERR_INVALID_ARGUMENT, "range out of bounds");
+    if (s->n == 0) return PS_OK;
+    std::vector<float4> V(s->n);
+    std::vector<Attr> A(s->n);
+    HIP_TRY(s, hipStreamSynchronize(s->stream));
+    HIP_TRY(s, hipMemcpy(V.data(), s->b.V, sizeof(float4) * s->n, hipMemcpyDeviceToHost));
+    HIP_TRY(s, hipMemcpy(A.data(), s->b.A, sizeof(Attr) * s->n, hipMemcpyDeviceToHost));
+    for (uint32_t t = 0; t < s->n; t++) {
+        const uint32_t o = A[t].orig;
+        if (o >= first && o < first + n) {
+            xyz[3 * (o - first)] = V[t].x;
+            xyz[3 * (o - first) + 1] = V[t].y;
+            xyz[3 * (o - first) + 2] = V[t].z;
+        }
+    }
+    return PS_OK;
+}
+
+int ps_read_grid(ps_system *s, uint32_t *sorted_index, uint32_t *cell, uint32_t *cell_offsets)
+{
+    if (int rc = check_sys(s)) return rc;
+    HIP_TRY(s, hipStreamSynchronize(s->stream));
+    if (sorted_index && s->n) {
+        std::vector<Attr> A(s->n);
+        HIP_TRY(s, hipMemcpy(A.data(), s->b.A, sizeof(Attr) * s->n, hipMemcpyDeviceToHost));
+        for (uint32_t t = 0; t < s->n; t++) sorted_index[t] = A[t].orig;
+    }
+    if (cell && s->n)
+        HIP_TRY(s, hipMemcpy(cell, s->b.key, sizeof(uint32_t) * s->n, hipMemcpyDeviceToHost));
+    if (cell_offsets)
+        HIP_TRY(s, hipMemcpy(cell_offsets, s->b.cell_off, sizeof(uint32_t) * ((size_t)s->ncell + 1),
+                             hipMemcpyDeviceToHost));
+    return PS_OK;
+}
+
+int ps_read_neighbors(ps_system *s, uint32_t *counts, uint32_t *table)
+{
+    if (int rc = check_sys(s)) return rc;
+    if (s->n == 0) return PS_OK;
+    HIP_TRY(s, hipStreamSynchronize(s->stream));
+    std::vector<uint32_t> cnt(s->n);
+    HIP_TRY(s, hipMemcpy(cnt.data(), s->b.ncnt, sizeof(uint32_t) * s->n, hipMemcpyDeviceToHost));
+    if (counts) memcpy(counts, cnt.data(), sizeof(uint32_t) * s->n);
+    if (table) {
+        const uint32_t stride = s->b.stride;
+        std::vector<uint32_t> col((size_t)s->kmax * stride);
+        HIP_TRY(s, hipMemcpy(col.data(), s->b.nbr, sizeof(uint32_t) * col.size(), hipMemcpyDeviceToHost));
+        for (uint32_t i = 0; i < s->n; i++)
+            for (uint32_t k = 0; k < s->kmax; k++)
+                table[(size_t)i * s->kmax + k] = k < cnt[i] ? col[(size_t)k * stride + i] : 0u;
+    }
+    return PS_OK;
+}
+
+int ps_read_sorted_state(ps_system *s, float *iterate_xyzl, float *start_xyz)
+{
+    if (int rc = check_sys(s)) return rc;
+    if (s->n == 0) return PS_OK;
+    HIP_TRY(s, hipStreamSynchronize(s->stream));
+    if (iterate_xyzl)
+        HIP_TRY(s, hipMemcpy(iterate_xyzl, s->b.P, sizeof(float4) * s->n, hipMemcpyDeviceToHost));
+    if (start_xyz) {
+        std::vector<float4> X(s->n);
+        HIP_TRY(s, hipMemcpy(X.data(), s->b.X, sizeof(float4) * s->n, hipMemcpyDeviceToHost));
+        for (uint32_t t = 0; t < s->n; t++) {
+            start_xyz[3 * t] = X[t].x;
+            start_xyz[3 * t + 1] = X[t].y;
+            start_xyz[3 * t + 2] = X[t].z;
+        }
+    }
+    return PS_OK;
+}
+
+int ps_get_status(ps_system *s, ps_status *out)
+{
+    if (int rc = check_sys(s)) return rc;
+    if (!out) return fail(s, PS_ERR_INVALID_ARGUMENT, "out is NULL");
+    ps::DevStatus ds;
+    HIP_TRY(s, hipStreamSynchronize(s->stream));
+    HIP_TRY(s, hipMemcpy(&ds, s->b.status, sizeof(ds), hipMemcpyDeviceToHost));
+    memset(out, 0, sizeof(*out));
+    out->flags = ds.flags;
+    out->max_neighbors = ds.max_neighbors;
+    out->num_particles = s->n;
+    out->num_cells = s->ncell;
+    out->substeps = s->substeps;
+    out->iterations = s->iters;
+    return PS_OK;
+}
+
+int ps_clear_status(ps_system *s)
+{
+    if (int rc = check_sys(s)) return rc;
+    HIP_TRY(s, hipStreamSynchronize(s->stream));
+    HIP_TRY(s, hipMemset(s->b.status, 0, sizeof(ps::DevStatus)));
+    return PS_OK;
+}
+
+// ---- timing ---------------------------------------------------------------
+
+int ps_timing_enable(ps_system *s, int enable)
+{
+    if (int rc = check_sys(s)) return rc;
+    if (int rc = timing_collect(s)) return rc;
+    s->timing = enable != 0;
+    return PS_OK;
+}
+
+int ps_timing_reset(ps_system *s)
+{
+    if (int rc = check_sys(s)) return rc;
+    if (int rc = timing_collect(s)) return rc;
+    for (int k = 0; k < PS_K_COUNT; k++) {
+        s->total_ms[k] = 0.0;
+        s->launches[k] = 0;
+    }
+    return PS_OK;
+}
+
+int ps_timing_read(ps_system *s, int kernel_id, double *total_ms, uint64_t *launches)
+{
+    if (int rc = check_sys(s)) return rc;
+    if (kernel_id < 0 || kernel_id >= PS_K_COUNT)
+        return fail(s, PS_ERR_INVALID_ARGUMENT, "kernel id out of range");
+    if (int rc = timing_collect(s)) return rc;
+    if (total_ms) *total_ms = s->total_ms[kernel_id];
+    if (launches) *launches = s->launches[kernel_id];
+    return PS_OK;
+}
+
+const char *ps_kernel_name(int kernel_id)
+{
+    static const char *names[PS_K_COUNT] = {"predict_hash", "scan", "scatter", "reorder",
+                                            "neighbors", "lambda", "solve", "shape", "finalize"};
+    if (kernel_id < 0 || kernel_id >= PS_K_COUNT) return "unknown";
+    return names[kernel_id];
+}
+
+}  // extern "C"

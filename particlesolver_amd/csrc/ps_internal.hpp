@@ -1,0 +1,110 @@
+// ps_internal.hpp — shared between the host API (ps_api.hip) and the
+// kernels (ps_kernels.hip).  Device-state layout and kernel launchers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/ps.h"
+
+namespace ps {
+
+// Per-particle attributes that travel with a particle through every reorder.
+struct __align__(16) Attr {
+    float w;        // inverse mass
+    int32_t phase;  // PS_PHASE_*
+    float rho0;     // rest density
+    uint32_t orig;  // insertion index
+};
+
+// Device-resident status word (read by ps_get_status).
+struct DevStatus {
+    uint32_t flags;
+    uint32_t max_neighbors;
+};
+
+// Constants handed to every kernel by value (kernarg segment).
+struct KConst {
+    float h, h2, poly6, spiky, w_zero, w_dq, inv_w_dq, search_r2, collide_r2, collide_dist;
+    float wall_s, wall_k, jitter, radius, relax, k_corr, xsph_c, s_solid, mu_s, mu_k;
+    float gravity[3];
+    float box_min[3], box_max[3];
+    float origin[3];
+    float cell;
+    uint32_t grid[3];
+    uint32_t n_corr;
+    uint32_t dims;
+    uint64_t seed;
+};
+
+// Scene-composition switches chosen on the host per launch; each selects a
+// separately compiled variant of the solve kernel, so a pure-fluid scene runs
+// without the contact/distance/shape code paths.
+enum SolveFeature : uint32_t {
+    F_FLUID = 1u,
+    F_CONTACT = 2u,
+    F_DIST = 4u,
+    F_SHAPE = 8u,
+    F_PIN = 16u,
+};
+
+struct DevBuffers {
+    // particle state, sorted storage order
+    float4 *X, *X2;  // positions at substep start (x*), .w unused
+    float4 *P, *Q;   // iterate (x, y, z, lambda) ping-pong
+    float4 *V, *V2;  // velocities
+    Attr *A, *A2;
+    uint32_t *key, *key2;  // cell of each slot
+    uint32_t *rank;        // arrival rank inside the cell (from the hash atomics)
+    uint32_t *tmp_orig;    // insertion index by provisional sorted slot
+    uint32_t *slot_of;     // insertion index -> current slot
+    // grid
+    uint32_t *cell_count;  // num_cells, zero between builds
+    uint32_t *cell_off;    // num_cells + 1
+    uint32_t *scan_part;   // per scan tile
+    // neighbours, column-major: nbr[k * stride + i]
+    uint32_t *nbr;
+    uint32_t *ncnt;
+    uint32_t stride;
+    // distance constraints (insertion-index CSR)
+    uint32_t *dist_off;    // max_particles + 1
+    uint32_t *dist_adj;    // 2*m: constraint ids
+    uint2 *dist_pairs;     // m
+    float2 *dist_rs;       // m: (rest, stiffness)
+    uint32_t *occ;         // per particle incidence count (distance + pins)
+    // pins
+    uint32_t *pin_of;      // per particle: pin + 1 (0 = none)
+    float4 *pin_xyz;
+    // shape matching
+    uint32_t *body_off;    // nbody + 1
+    uint32_t *body_idx;    // insertion indices
+    float4 *body_rest;     // rest offsets
+    float *body_stiff;
+    float *body_mass;
+    float4 *body_q;        // rotation quaternion (w, x, y, z)
+    float4 *body_goal;     // per member
+    uint32_t *body_of;     // per particle: body + 1
+    uint32_t *member_of;   // per particle: member index
+    DevStatus *status;
+};
+
+// Kernel launchers (ps_kernels.hip).  All enqueue on `st` and return the
+// launch error.
+hipError_t launch_predict_hash(hipStream_t st, const DevBuffers &b, const KConst &k, uint32_t n,
+                               float dt, bool predict, bool hash);
+hipError_t launch_scan(hipStream_t st, const DevBuffers &b, uint32_t num_cells);
+hipError_t launch_scatter(hipStream_t st, const DevBuffers &b, uint32_t n);
+hipError_t launch_reorder(hipStream_t st, const DevBuffers &b, uint32_t n);
+hipError_t launch_neighbors(hipStream_t st, const DevBuffers &b, const KConst &k, uint32_t n,
+                            uint32_t kmax);
+hipError_t launch_lambda(hipStream_t st, const DevBuffers &b, const KConst &k, uint32_t n,
+                         bool uniform, float wu);
+hipError_t launch_shape(hipStream_t st, const DevBuffers &b, uint32_t nbody);
+hipError_t launch_solve(hipStream_t st, const DevBuffers &b, const KConst &k, uint32_t n,
+                        uint32_t features, bool uniform, float wu, uint64_t iter);
+hipError_t launch_finalize(hipStream_t st, const DevBuffers &b, const KConst &k, uint32_t n,
+                           float dt, bool sleeping, bool uniform, float wu);
+
+uint32_t scan_tiles(uint32_t num_cells);
+
+}  // namespace ps

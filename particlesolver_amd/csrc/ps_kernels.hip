@@ -1,0 +1,934 @@
+// ps_kernels.hip — gfx950 kernels of the particle-solver substep.
+//
+// Layout: every per-particle array is in cell-sorted storage order; a
+// particle's insertion index travels in Attr::orig.  Positions and the PBF
+// multiplier share one float4 (x, y, z, lambda), so each neighbour visit in
+// the lambda and delta-p kernels is a single 16-byte gather.  The neighbour
+// table is column-major (entry k of particle i at nbr[k*stride + i]), so a
+// wavefront reading entry k of 64 consecutive particles issues one coalesced
+// 256-byte load.
+//
+// Arithmetic: built with -ffp-contract=off and IEEE division/sqrt, and every
+// expression is parenthesised as in the CPU oracle, so FP32 results are
+// reproducible operation by operation.  Accumulations run over the neighbour
+// table in ascending slot order; no floating-point atomics anywhere.
+//
+// Launch mapping: blocks of the neighbour-gathering kernels are remapped so
+// that each of the 8 XCDs (which receive workgroups round-robin) owns one
+// contiguous range of sorted particles, keeping a spatial region's
+// neighbour data inside one XCD's L2.
+
+#include "ps_internal.hpp"
+
+namespace ps {
+
+static constexpr uint32_t kBlock = 256;
+static constexpr uint32_t kScanPerThread = 16;
+static constexpr uint32_t kScanTile = kBlock * kScanPerThread;
+
+__device__ __forceinline__ uint32_t xcd_block()
+{
+    const uint32_t nb = gridDim.x, b = blockIdx.x;
+    const uint32_t x = b & 7u, q = nb >> 3, r = nb & 7u;
+    return x * q + (x < r ? x : r) + (b >> 3);
+}
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v)
+{
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        uint32_t t = __shfl_up(v, d, 64);
+        if (lane >= d) v += t;
+    }
+    return v;
+}
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v)
+{
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
+    return v;
+}
+
+__device__ __forceinline__ uint32_t wave_max(uint32_t v)
+{
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) {
+        uint32_t t = __shfl_xor(v, d, 64);
+        v = t > v ? t : v;
+    }
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// a1 + a2: predict (integrate_functor, integration_kernel.cuh:159-184) and
+// cell hash (calcGridPos/calcGridHash/calcHashD :187-225) with a dense,
+// clamped grid.  The atomic's return value is the particle's arrival rank in
+// its cell, used only to place it provisionally; the final in-cell order is
+// fixed by insertion index in reorder_k.
+
+__device__ __forceinline__ uint32_t cell_axis(float p, float o, float cell, uint32_t g,
+                                              uint32_t &flags)
+{
+    float f = floorf((p - o) / cell);
+    if (!(f >= 0.0f)) {
+        flags |= (f != f) ? PS_STATUS_NONFINITE : PS_STATUS_OUTSIDE_GRID;
+        return 0u;
+    }
+    if (f >= (float)g) {
+        flags |= PS_STATUS_OUTSIDE_GRID;
+        return g - 1u;
+    }
+    return (uint32_t)f;
+}
+
+template <bool PRED, bool HASH>
+__global__ __launch_bounds__(kBlock) void predict_hash_k(DevBuffers b, KConst k, uint32_t n,
+                                                         float dt)
+{
+    const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+    if (s >= n) return;
+    float px, py, pz;
+    if (PRED) {
+        const float4 x = b.X[s];
+        const Attr a = b.A[s];
+        px = x.x;
+        py = x.y;
+        pz = x.z;
+        if (a.w != 0.0f) {
+            const float4 v = b.V[s];
+            float vx = v.x + (k.gravity[0] * dt);
+            float vy = v.y + (k.gravity[1] * dt);
+            float vz = v.z + (k.gravity[2] * dt);
+            px = x.x + (vx * dt);
+            py = x.y + (vy * dt);
+            pz = k.dims == 2u ? x.z : (x.z + (vz * dt));
+        }
+        b.P[s] = make_float4(px, py, pz, 0.0f);
+    } else {
+        const float4 p = b.P[s];
+        px = p.x;
+        py = p.y;
+        pz = p.z;
+    }
+    if (HASH) {
+        uint32_t flags = 0;
+        uint32_t cx = cell_axis(px, k.origin[0], k.cell, k.grid[0], flags);
+        uint32_t cy = cell_axis(py, k.origin[1], k.cell, k.grid[1], flags);
+        uint32_t cz = cell_axis(pz, k.origin[2], k.cell, k.grid[2], flags);
+        uint32_t key = (cz * k.grid[1] + cy) * k.grid[0] + cx;
+        b.key[s] = key;
+        b.rank[s] = atomicAdd(&b.cell_count[key], 1u);
+        if (flags) atomicOr(&b.status->flags, flags);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// a4 part 1: exclusive scan of the per-cell counts into cell offsets
+// (cell c = sorted slots [off[c], off[c+1]); the reference's cellStart/End,
+// reorderDataAndFindCellStartD integration_kernel.cuh:229-299), two passes
+// over tiles of 4096 cells; the counts are cleared for the next build.
+
+__device__ __forceinline__ uint32_t block_sum(uint32_t v, uint32_t *lds4)
+{
+    v = wave_sum(v);
+    const uint32_t w = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) lds4[w] = v;
+    __syncthreads();
+    return lds4[0] + lds4[1] + lds4[2] + lds4[3];
+}
+
+__global__ __launch_bounds__(kBlock) void scan_partials_k(const uint32_t *cnt, uint32_t *part,
+                                                          uint32_t C)
+{
+    __shared__ uint32_t lds4[4];
+    const uint32_t base = blockIdx.x * kScanTile;
+    uint32_t sum = 0;
+#pragma unroll
+    for (uint32_t q = 0; q < kScanPerThread; q++) {
+        uint32_t c = base + q * kBlock + threadIdx.x;
+        if (c < C) sum += cnt[c];
+    }
+    sum = block_sum(sum, lds4);
+    if (threadIdx.x == 0) part[blockIdx.x] = sum;
+}
+
+__global__ __launch_bounds__(kBlock) void scan_final_k(uint32_t *cnt, const uint32_t *part,
+                                                       uint32_t *off, uint32_t C)
+{
+    __shared__ uint32_t tile[kScanTile + kScanTile / 16];
+    __shared__ uint32_t lds4[4];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t base = blockIdx.x * kScanTile;
+    uint32_t pre = 0;
+    for (uint32_t t = tid; t < blockIdx.x; t += kBlock) pre += part[t];
+    pre = block_sum(pre, lds4);
+    // coalesced load (and clear) into LDS; padded index avoids bank conflicts
+#pragma unroll
+    for (uint32_t q = 0; q < kScanPerThread; q++) {
+        uint32_t idx = q * kBlock + tid;
+        uint32_t c = base + idx;
+        uint32_t v = 0;
+        if (c < C) {
+            v = cnt[c];
+            cnt[c] = 0;
+        }
+        tile[idx + (idx >> 4)] = v;
+    }
+    __syncthreads();
+    uint32_t v[kScanPerThread];
+    uint32_t local = 0;
+#pragma unroll
+    for (uint32_t q = 0; q < kScanPerThread; q++) {
+        uint32_t idx = tid * kScanPerThread + q;
+        v[q] = tile[idx + (idx >> 4)];
+        local += v[q];
+    }
+    // block exclusive scan of the per-thread totals
+    uint32_t incl = wave_incl_scan(local);
+    const uint32_t w = tid >> 6;
+    __shared__ uint32_t wtot[4];
+    if ((tid & 63) == 63) wtot[w] = incl;
+    __syncthreads();
+    uint32_t wpre = 0;
+    for (uint32_t q = 0; q < w; q++) wpre += wtot[q];
+    uint32_t run = pre + wpre + (incl - local);
+#pragma unroll
+    for (uint32_t q = 0; q < kScanPerThread; q++) {
+        uint32_t idx = tid * kScanPerThread + q;
+        tile[idx + (idx >> 4)] = run;
+        run += v[q];
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t q = 0; q < kScanPerThread; q++) {
+        uint32_t idx = q * kBlock + tid;
+        uint32_t c = base + idx;
+        if (c < C) off[c] = tile[idx + (idx >> 4)];
+    }
+    if (blockIdx.x == gridDim.x - 1 && tid == kBlock - 1) off[C] = run;
+}
+
+// a4 part 2: provisional placement by arrival rank (insertion indices only)
+__global__ __launch_bounds__(kBlock) void scatter_k(DevBuffers b, uint32_t n)
+{
+    const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+    if (s >= n) return;
+    const uint32_t t = b.cell_off[b.key[s]] + b.rank[s];
+    if (t < n)
+        b.tmp_orig[t] = b.A[s].orig;
+    else
+        atomicOr(&b.status->flags, PS_STATUS_INTERNAL);
+}
+
+// a3 + a4 part 3: final slot = cell start + number of particles in the same
+// cell with a smaller insertion index, i.e. a sort by (cell, insertion
+// index): a stable sort of insertion-ordered input, as thrust::sort_by_key
+// of (hash, index) in sortParticles (integration.cu:270-275) is meant to
+// give.  Every per-particle array moves with the particle.
+__global__ __launch_bounds__(kBlock) void reorder_k(DevBuffers b, uint32_t n)
+{
+    const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+    if (s >= n) return;
+    const uint32_t c = b.key[s];
+    const Attr a = b.A[s];
+    const uint32_t b0 = b.cell_off[c], b1 = b.cell_off[c + 1];
+    if (b1 > n || b0 >= b1 || a.orig >= n) {
+        atomicOr(&b.status->flags, PS_STATUS_INTERNAL);
+        return;
+    }
+    uint32_t r = 0;
+    for (uint32_t u = b0; u < b1; u++) r += b.tmp_orig[u] < a.orig ? 1u : 0u;
+    const uint32_t d = b0 + r;
+    const float4 p = b.P[s];
+    b.X2[d] = b.X[s];
+    b.Q[d] = make_float4(p.x, p.y, p.z, 0.0f);
+    b.V2[d] = b.V[s];
+    b.A2[d] = a;
+    b.key2[d] = c;
+    b.slot_of[a.orig] = d;
+}
+
+// ---------------------------------------------------------------------------
+// Neighbour table (collideCell/collideCellRadius, integration_kernel.cuh:
+// 303-356,481-518): every j != i with r^2 < search_r2, scanning the 3x3 rows
+// of x-adjacent cells; a row's three cells are one contiguous slot range.
+
+__global__ __launch_bounds__(kBlock) void neighbors_k(DevBuffers b, KConst k, uint32_t n,
+                                                      uint32_t kmax)
+{
+    __shared__ uint32_t lds4[4];
+    const uint32_t i = xcd_block() * kBlock + threadIdx.x;
+    uint32_t cnt = 0;
+    if (i < n) {
+        const uint32_t gx = k.grid[0], gy = k.grid[1], gz = k.grid[2];
+        const uint32_t key = b.key[i];
+        const uint32_t cx = key % gx, cy = (key / gx) % gy, cz = key / (gx * gy);
+        const uint32_t x0 = cx > 0 ? cx - 1 : 0, x1 = cx + 1 < gx ? cx + 1 : gx - 1;
+        const float4 p = b.P[i];
+        const float r2max = k.search_r2;
+        const uint32_t stride = b.stride;
+        for (int dz = -1; dz <= 1; dz++) {
+            const int z = (int)cz + dz;
+            if (z < 0 || z >= (int)gz) continue;
+            for (int dy = -1; dy <= 1; dy++) {
+                const int y = (int)cy + dy;
+                if (y < 0 || y >= (int)gy) continue;
+                const uint32_t row = ((uint32_t)z * gy + (uint32_t)y) * gx;
+                const uint32_t j0 = b.cell_off[row + x0];
+                const uint32_t j1 = min(b.cell_off[row + x1 + 1], n);
+                for (uint32_t j = j0; j < j1; j++) {
+                    if (j == i) continue;
+                    const float4 q = b.P[j];
+                    float dx = p.x - q.x, dy_ = p.y - q.y, dz_ = p.z - q.z;
+                    float r2 = ((dx * dx) + (dy_ * dy_)) + (dz_ * dz_);
+                    if (r2 < r2max) {
+                        if (cnt < kmax) b.nbr[(size_t)cnt * stride + i] = j;
+                        cnt++;
+                    }
+                }
+            }
+        }
+        b.ncnt[i] = cnt < kmax ? cnt : kmax;
+    }
+    // one status update per block
+    uint32_t m = wave_max(cnt);
+    const uint32_t w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) lds4[w] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t mm = lds4[0];
+        for (int q = 1; q < 4; q++) mm = lds4[q] > mm ? lds4[q] : mm;
+        if (mm > 0) atomicMax(&b.status->max_neighbors, mm);
+        if (mm > kmax) atomicOr(&b.status->flags, (uint32_t)PS_STATUS_NEIGHBOR_OVERFLOW);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// a5: PBF multiplier (findLambdasD, integration_kernel.cuh:521-593)
+//   rho_i = W(0)/w_i + sum_j s_j W(r)/w_j,  grad_j = (spiky (h-r)^2 / r)/rho0_i d
+//   lambda_i = -(rho_i/rho0_i - 1) / (sum |grad_j|^2 + |sum grad_j|^2 + eps)
+// UNIFORM: every particle is fluid with inverse mass wu (no Attr gathers).
+
+template <bool UNIFORM>
+__global__ __launch_bounds__(kBlock) void lambda_k(DevBuffers b, KConst k, uint32_t n, float mu)
+{
+    const uint32_t i = xcd_block() * kBlock + threadIdx.x;
+    if (i >= n) return;
+    float *Pw = reinterpret_cast<float *>(b.P);
+    const Attr a = b.A[i];
+    if (!UNIFORM && a.phase != PS_PHASE_FLUID) {
+        Pw[4 * (size_t)i + 3] = 0.0f;
+        return;
+    }
+    const float4 p = b.P[i];
+    const float r0 = a.rho0;
+    const float ir0 = 1.0f / r0;
+    float rho = k.w_zero * (UNIFORM ? mu : (1.0f / a.w));
+    float gix = 0.0f, giy = 0.0f, giz = 0.0f, den = 0.0f;
+    const uint32_t c = b.ncnt[i];
+    const uint32_t stride = b.stride;
+    for (uint32_t q = 0; q < c; q++) {
+        const uint32_t j = b.nbr[(size_t)q * stride + i];
+        float wj = 1.0f;
+        int32_t phj = PS_PHASE_FLUID;
+        if (!UNIFORM) {
+            const Attr aj = b.A[j];
+            wj = aj.w;
+            phj = aj.phase;
+            if (wj == 0.0f) continue;
+        }
+        const float4 pj = b.P[j];
+        float dx = p.x - pj.x, dy = p.y - pj.y, dz = p.z - pj.z;
+        float r2 = ((dx * dx) + (dy * dy)) + (dz * dz);
+        if (!(r2 < k.h2)) continue;
+        float t = k.h2 - r2;
+        float W = k.poly6 * ((t * t) * t);
+        float cc = W * (UNIFORM ? mu : (1.0f / wj));
+        if (phj != PS_PHASE_FLUID) cc = cc * k.s_solid;
+        rho = rho + cc;
+        float r = sqrtf(r2);
+        if (r < 1e-4f) continue;
+        float qq = k.h - r;
+        float aa = ((k.spiky * (qq * qq)) / r) * ir0;
+        float gx = aa * dx, gy = aa * dy, gz = aa * dz;
+        den = den + (((gx * gx) + (gy * gy)) + (gz * gz));
+        gix = gix + gx;
+        giy = giy + gy;
+        giz = giz + gz;
+    }
+    den = den + (((gix * gix) + (giy * giy)) + (giz * giz));
+    float C = (rho / r0) - 1.0f;
+    Pw[4 * (size_t)i + 3] = -C / (den + k.relax);
+}
+
+// ---------------------------------------------------------------------------
+// helpers of the solve kernel
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z)
+{
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+// Wall-jitter uniform in [0,1): SplitMix64 finaliser of seed + golden*counter,
+// counter = (iteration, insertion index, wall), top 24 bits.
+__device__ __forceinline__ float wall_uniform(uint64_t seed, uint64_t iter, uint32_t orig,
+                                              uint32_t wall)
+{
+    uint64_t ctr = (iter << 35) | ((uint64_t)orig << 3) | (uint64_t)wall;
+    uint64_t z = mix64(seed + ctr * 0x9E3779B97F4A7C15ull);
+    return (float)(uint32_t)(z >> 40) * (1.0f / 16777216.0f);
+}
+
+// e^x: Cody-Waite reduction x = k ln2 + r, degree-7 Taylor polynomial of
+// e^r (Horner), exact scaling by 2^k.
+__device__ __forceinline__ float ps_exp(float x)
+{
+    if (x > 80.0f) x = 80.0f;
+    if (x < -80.0f) x = -80.0f;
+    float kk = floorf((x * 1.44269504088896341f) + 0.5f);
+    float r = (x - (kk * 0.693145751953125f)) - (kk * 1.42860682030941723e-6f);
+    float p = 1.98412698412698413e-4f;
+    p = (p * r) + 1.38888888888888889e-3f;
+    p = (p * r) + 8.33333333333333333e-3f;
+    p = (p * r) + 4.16666666666666667e-2f;
+    p = (p * r) + 1.66666666666666667e-1f;
+    p = (p * r) + 0.5f;
+    p = (p * r) + 1.0f;
+    p = (p * r) + 1.0f;
+    return ldexpf(p, (int)kk);
+}
+
+__device__ __forceinline__ float ipow(float x, uint32_t n)
+{
+    float r = 1.0f, bb = x;
+    bool first = true;
+    while (n) {
+        if (n & 1u) {
+            r = first ? bb : (r * bb);
+            first = false;
+        }
+        n >>= 1;
+        if (n) bb = bb * bb;
+    }
+    return r;
+}
+
+// a8: world box (collide_world_functor, integration_kernel.cuh:57-157).
+__device__ __forceinline__ void walls(const KConst &k, int32_t ph, uint32_t o, float4 xs,
+                                      float &px, float &py, float &pz, uint64_t iter)
+{
+    const float r = k.radius;
+    float nx = 0.0f, ny = 0.0f, nz = 0.0f;
+    const float efl = ph < PS_PHASE_SOLID ? k.jitter : 0.0f;
+    if (py < k.box_min[1] + r) {
+        py = (k.box_min[1] + r) + (wall_uniform(k.seed, iter, o, 5) * efl);
+        ny = ny + 1.0f;
+    }
+    if (px > k.box_max[0] - r) {
+        px = k.box_max[0] - (r + (wall_uniform(k.seed, iter, o, 0) * k.jitter));
+        nx = nx - 1.0f;
+    }
+    if (px < k.box_min[0] + r) {
+        px = k.box_min[0] + (r + (wall_uniform(k.seed, iter, o, 1) * k.jitter));
+        nx = nx + 1.0f;
+    }
+    if (py > k.box_max[1] - r) {
+        py = k.box_max[1] - (r + (wall_uniform(k.seed, iter, o, 2) * k.jitter));
+        ny = ny - 1.0f;
+    }
+    if (k.dims != 2u) {
+        if (pz > k.box_max[2] - r) {
+            pz = k.box_max[2] - (r + (wall_uniform(k.seed, iter, o, 3) * k.jitter));
+            nz = nz - 1.0f;
+        }
+        if (pz < k.box_min[2] + r) {
+            pz = k.box_min[2] + (r + (wall_uniform(k.seed, iter, o, 4) * k.jitter));
+            nz = nz + 1.0f;
+        }
+    }
+    if (ph < PS_PHASE_CLOTH) return;
+    if (nx == 0.0f && ny == 0.0f && nz == 0.0f) return;
+    float ex = px - xs.x, ey = py - xs.y, ez = pz - xs.z;
+    float dn = ((ex * nx) + (ey * ny)) + (ez * nz);
+    float tx = ex - (dn * nx), ty = ey - (dn * ny), tz = ez - (dn * nz);
+    float l = sqrtf(((tx * tx) + (ty * ty)) + (tz * tz));
+    if (l < 0.001f) return;
+    if (l < k.wall_s) {
+        px = px - tx;
+        py = py - ty;
+        pz = pz - tz;
+    } else {
+        float f = k.wall_k / l;
+        if (f > 1.0f) f = 1.0f;
+        px = px - (tx * f);
+        py = py - (ty * f);
+        pz = pz - (tz * f);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// One Jacobi iteration for every particle: contacts (a7), fluid delta-p
+// (a6), distance (a9), shape goal (a12) summed in that order from the same
+// snapshot P, then walls (a8) and pins (a10); result to Q.
+
+template <uint32_t F, bool UNIFORM>
+__global__ __launch_bounds__(kBlock) void solve_k(DevBuffers b, KConst k, uint32_t n, float wu,
+                                                  uint64_t iter)
+{
+    const uint32_t i = xcd_block() * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const float4 p = b.P[i];
+    const Attr a = b.A[i];
+    const float px = p.x, py = p.y, pz = p.z;
+    const int32_t ph = UNIFORM ? PS_PHASE_FLUID : a.phase;
+    const float wi = UNIFORM ? wu : a.w;
+    const uint32_t c = b.ncnt[i];
+    const uint32_t stride = b.stride;
+    float ax = 0.0f, ay = 0.0f, az = 0.0f;
+
+    // a7: particle contacts + friction (collideD, integration_kernel.cuh:358-462)
+    if (F & F_CONTACT) {
+        if (ph >= PS_PHASE_CLOTH && wi != 0.0f) {
+            uint32_t nc = 0;
+            for (uint32_t q = 0; q < c; q++) {
+                const uint32_t j = b.nbr[(size_t)q * stride + i];
+                const int32_t pj = b.A[j].phase;
+                if (ph > PS_PHASE_SOLID && ph == pj) continue;
+                const float4 qj = b.P[j];
+                float dx = px - qj.x, dy = py - qj.y, dz = pz - qj.z;
+                float r2 = ((dx * dx) + (dy * dy)) + (dz * dz);
+                if (r2 < k.collide_r2) nc++;
+            }
+            if (nc > 0) {
+                const float fnc = (float)nc;
+                const float ewi = wi * ps_exp(py);
+                const float4 xi = b.X[i];
+                float cx = 0.0f, cy = 0.0f, cz = 0.0f;
+                for (uint32_t q = 0; q < c; q++) {
+                    const uint32_t j = b.nbr[(size_t)q * stride + i];
+                    const Attr aj = b.A[j];
+                    const int32_t pj = aj.phase;
+                    if (ph > PS_PHASE_SOLID && ph == pj) continue;
+                    const float4 qj = b.P[j];
+                    float dx = px - qj.x, dy = py - qj.y, dz = pz - qj.z;
+                    float r2 = ((dx * dx) + (dy * dy)) + (dz * dz);
+                    if (!(r2 < k.collide_r2)) continue;
+                    float dist = sqrtf(r2);
+                    if (dist < 1e-6f) continue;
+                    const float wj = aj.w;
+                    float cw = wi, cw2 = wj;
+                    const bool solid = ph >= PS_PHASE_SOLID && pj >= PS_PHASE_SOLID;
+                    if (solid) {
+                        cw = ewi;
+                        cw2 = wj * ps_exp(qj.y);
+                    }
+                    float wsum = cw + cw2;
+                    if (wsum == 0.0f) continue;
+                    float mag = dist - k.collide_dist;
+                    float sc = (((mag / wsum) / dist) * cw) / fnc;
+                    float ex = -(sc * dx), ey = -(sc * dy), ez = -(sc * dz);
+                    cx = cx + ex;
+                    cy = cy + ey;
+                    cz = cz + ez;
+                    if (!solid) continue;
+                    float sc2 = (((mag / wsum) / dist) * cw2) / fnc;
+                    float fx = sc2 * dx, fy = sc2 * dy, fz = sc2 * dz;
+                    const float4 xj = b.X[j];
+                    float rx = ((px + ex) - xi.x) - ((qj.x + fx) - xj.x);
+                    float ry = ((py + ey) - xi.y) - ((qj.y + fy) - xj.y);
+                    float rz = ((pz + ez) - xi.z) - ((qj.z + fz) - xj.z);
+                    float nx = dx / dist, ny = dy / dist, nz = dz / dist;
+                    float dn = ((rx * nx) + (ry * ny)) + (rz * nz);
+                    float tx = rx - (dn * nx), ty = ry - (dn * ny), tz = rz - (dn * nz);
+                    float lt = sqrtf(((tx * tx) + (ty * ty)) + (tz * tz));
+                    if (lt < 0.001f) continue;
+                    float share = cw / wsum;
+                    if (lt < (k.mu_s * dist)) {
+                        cx = cx - (tx * share);
+                        cy = cy - (ty * share);
+                        cz = cz - (tz * share);
+                    } else {
+                        float f = (k.mu_k * dist) / lt;
+                        if (f > 1.0f) f = 1.0f;
+                        cx = cx - ((tx * f) * share);
+                        cy = cy - ((ty * f) * share);
+                        cz = cz - ((tz * f) * share);
+                    }
+                }
+                ax = ax + cx;
+                ay = ay + cy;
+                az = az + cz;
+            }
+        }
+    }
+
+    // a6: PBF delta-p with tensile correction s_corr = -k h^2 (W/W(dq h))^n
+    // (solveFluidsD :596-642); k.k_corr already holds k h^2
+    if (F & F_FLUID) {
+        if (ph == PS_PHASE_FLUID) {
+            const float li = p.w, ir0 = 1.0f / a.rho0;
+            float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+            for (uint32_t q = 0; q < c; q++) {
+                const uint32_t j = b.nbr[(size_t)q * stride + i];
+                if (!UNIFORM && b.A[j].w == 0.0f) continue;
+                const float4 pj = b.P[j];
+                float dx = px - pj.x, dy = py - pj.y, dz = pz - pj.z;
+                float r2 = ((dx * dx) + (dy * dy)) + (dz * dz);
+                if (!(r2 < k.h2)) continue;
+                float t = k.h2 - r2;
+                float W = k.poly6 * ((t * t) * t);
+                float corr = -(k.k_corr * ipow(W * k.inv_w_dq, k.n_corr));
+                float coef = (li + pj.w) + corr;
+                float r = sqrtf(r2);
+                float qq = k.h - r;
+                float g = k.spiky * (qq * qq);
+                if (r < 1e-4f) {
+                    fy = fy - (coef * (g * 0.001f));
+                } else {
+                    float aa = g / r;
+                    fx = fx - (coef * (aa * dx));
+                    fy = fy - (coef * (aa * dy));
+                    fz = fz - (coef * (aa * dz));
+                }
+            }
+            ax = ax + (fx * ir0);
+            ay = ay + (fy * ir0);
+            az = az + (fz * ir0);
+        }
+    }
+
+    // a9: distance constraints, gathered over this particle's incident list
+    if (F & F_DIST) {
+        const uint32_t o = a.orig;
+        const uint32_t a0 = b.dist_off[o], a1 = b.dist_off[o + 1];
+        if (a1 > a0 && wi != 0.0f) {
+            float cx = 0.0f, cy = 0.0f, cz = 0.0f;
+            for (uint32_t q = a0; q < a1; q++) {
+                const uint32_t cid = b.dist_adj[q];
+                const uint2 pr = b.dist_pairs[cid];
+                const uint32_t other = pr.x == o ? pr.y : pr.x;
+                const uint32_t j = b.slot_of[other];
+                const float4 pj = b.P[j];
+                float dx = px - pj.x, dy = py - pj.y, dz = pz - pj.z;
+                float dist = sqrtf(((dx * dx) + (dy * dy)) + (dz * dz));
+                float wsum = wi + b.A[j].w;
+                if (dist < 1e-6f || wsum == 0.0f) continue;
+                const float2 rs = b.dist_rs[cid];
+                float sc = ((((dist - rs.x) / wsum) / dist) * wi) * rs.y;
+                cx = cx - (sc * dx);
+                cy = cy - (sc * dy);
+                cz = cz - (sc * dz);
+            }
+            const float occ = (float)b.occ[o];
+            ax = ax + (cx / occ);
+            ay = ay + (cy / occ);
+            az = az + (cz / occ);
+        }
+    }
+
+    // a12: shape-matching goal pull
+    if (F & F_SHAPE) {
+        const uint32_t o = a.orig;
+        const uint32_t bo = b.body_of[o];
+        if (bo) {
+            const float4 g = b.body_goal[b.member_of[o]];
+            const float st = b.body_stiff[bo - 1];
+            ax = ax + ((g.x - px) * st);
+            ay = ay + ((g.y - py) * st);
+            az = az + ((g.z - pz) * st);
+        }
+    }
+
+    float nxp = px + ax, nyp = py + ay, nzp = pz + az;
+    if (k.dims == 2u) nzp = pz;
+    const float4 xs = (ph >= PS_PHASE_CLOTH) ? b.X[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    walls(k, ph, a.orig, xs, nxp, nyp, nzp, iter);
+    if (F & F_PIN) {
+        const uint32_t pin = b.pin_of[a.orig];
+        if (pin) {
+            const float4 t = b.pin_xyz[pin - 1];
+            nxp = t.x;
+            nyp = t.y;
+            nzp = t.z;
+        }
+    }
+    b.Q[i] = make_float4(nxp, nyp, nzp, 0.0f);
+}
+
+// ---------------------------------------------------------------------------
+// a12: shape matching per body (one thread per body): centre of mass, A_pq,
+// rotation by the fixed-point quaternion iteration (Mueller, Bender,
+// Chentanez, Macklin, MIG 2016) with a first-order quaternion step, goals.
+
+__device__ __forceinline__ void quat_to_mat(float w, float x, float y, float z, float *m)
+{
+    m[0] = 1.0f - (2.0f * ((y * y) + (z * z)));
+    m[1] = 2.0f * ((x * y) - (w * z));
+    m[2] = 2.0f * ((x * z) + (w * y));
+    m[3] = 2.0f * ((x * y) + (w * z));
+    m[4] = 1.0f - (2.0f * ((x * x) + (z * z)));
+    m[5] = 2.0f * ((y * z) - (w * x));
+    m[6] = 2.0f * ((x * z) - (w * y));
+    m[7] = 2.0f * ((y * z) + (w * x));
+    m[8] = 1.0f - (2.0f * ((x * x) + (y * y)));
+}
+
+__global__ __launch_bounds__(kBlock) void shape_k(DevBuffers b, uint32_t nbody)
+{
+    const uint32_t bd = blockIdx.x * kBlock + threadIdx.x;
+    if (bd >= nbody) return;
+    const uint32_t a0 = b.body_off[bd], a1 = b.body_off[bd + 1];
+    const float M = b.body_mass[bd];
+    float cx = 0.0f, cy = 0.0f, cz = 0.0f;
+    for (uint32_t q = a0; q < a1; q++) {
+        const uint32_t t = b.slot_of[b.body_idx[q]];
+        const float m = 1.0f / b.A[t].w;
+        const float4 p = b.P[t];
+        cx = cx + (m * p.x);
+        cy = cy + (m * p.y);
+        cz = cz + (m * p.z);
+    }
+    cx = cx / M;
+    cy = cy / M;
+    cz = cz / M;
+    float A[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    for (uint32_t q = a0; q < a1; q++) {
+        const uint32_t t = b.slot_of[b.body_idx[q]];
+        const float m = 1.0f / b.A[t].w;
+        const float4 p = b.P[t];
+        const float ux = m * (p.x - cx), uy = m * (p.y - cy), uz = m * (p.z - cz);
+        const float4 r = b.body_rest[q];
+        A[0] = A[0] + (ux * r.x); A[1] = A[1] + (ux * r.y); A[2] = A[2] + (ux * r.z);
+        A[3] = A[3] + (uy * r.x); A[4] = A[4] + (uy * r.y); A[5] = A[5] + (uy * r.z);
+        A[6] = A[6] + (uz * r.x); A[7] = A[7] + (uz * r.y); A[8] = A[8] + (uz * r.z);
+    }
+    float4 qt = b.body_q[bd];
+    float qw = qt.x, qx = qt.y, qy = qt.z, qz = qt.w;
+    for (int it = 0; it < 10; it++) {
+        float m[9];
+        quat_to_mat(qw, qx, qy, qz, m);
+        float ox = 0.0f, oy = 0.0f, oz = 0.0f, dot = 0.0f;
+        for (int col = 0; col < 3; col++) {
+            float rx = m[col], ry = m[3 + col], rz = m[6 + col];
+            float ax_ = A[col], ay_ = A[3 + col], az_ = A[6 + col];
+            ox = ox + ((ry * az_) - (rz * ay_));
+            oy = oy + ((rz * ax_) - (rx * az_));
+            oz = oz + ((rx * ay_) - (ry * ax_));
+            dot = dot + (((rx * ax_) + (ry * ay_)) + (rz * az_));
+        }
+        float den = (dot < 0.0f ? -dot : dot) + 1e-9f;
+        ox = ox / den;
+        oy = oy / den;
+        oz = oz / den;
+        float wl2 = ((ox * ox) + (oy * oy)) + (oz * oz);
+        if (wl2 < 1e-18f) break;
+        float dw = 1.0f, dx = ox * 0.5f, dy = oy * 0.5f, dz = oz * 0.5f;
+        float nw = (((dw * qw) - (dx * qx)) - (dy * qy)) - (dz * qz);
+        float nx = (((dw * qx) + (dx * qw)) + (dy * qz)) - (dz * qy);
+        float ny = (((dw * qy) - (dx * qz)) + (dy * qw)) + (dz * qx);
+        float nz = (((dw * qz) + (dx * qy)) - (dy * qx)) + (dz * qw);
+        float nl = sqrtf((((nw * nw) + (nx * nx)) + (ny * ny)) + (nz * nz));
+        qw = nw / nl;
+        qx = nx / nl;
+        qy = ny / nl;
+        qz = nz / nl;
+    }
+    b.body_q[bd] = make_float4(qw, qx, qy, qz);
+    float m[9];
+    quat_to_mat(qw, qx, qy, qz, m);
+    for (uint32_t q = a0; q < a1; q++) {
+        const float4 r = b.body_rest[q];
+        b.body_goal[q] = make_float4((((m[0] * r.x) + (m[1] * r.y)) + (m[2] * r.z)) + cx,
+                                     (((m[3] * r.x) + (m[4] * r.y)) + (m[5] * r.z)) + cy,
+                                     (((m[6] * r.x) + (m[7] * r.y)) + (m[8] * r.z)) + cz, 0.0f);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// a11: v = (x - x*)/dt (subtract_functor, integration_kernel.cuh:465-475)
+// plus XSPH over fluid neighbours from displacements u = x - x*:
+//   v_i = u_i/dt + ((c/rho0_i) sum_j (u_j - u_i) m_j W_ij)/dt
+
+template <bool UNIFORM>
+__global__ __launch_bounds__(kBlock) void finalize_k(DevBuffers b, KConst k, uint32_t n, float mu,
+                                                     float dt)
+{
+    const uint32_t i = xcd_block() * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const float4 p = b.P[i];
+    const float4 x = b.X[i];
+    const float ux = p.x - x.x, uy = p.y - x.y, uz = p.z - x.z;
+    float vx = ux / dt, vy = uy / dt, vz = uz / dt;
+    const Attr a = b.A[i];
+    if ((UNIFORM || a.phase == PS_PHASE_FLUID) && k.xsph_c != 0.0f) {
+        float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+        const uint32_t c = b.ncnt[i];
+        const uint32_t stride = b.stride;
+        for (uint32_t q = 0; q < c; q++) {
+            const uint32_t j = b.nbr[(size_t)q * stride + i];
+            float mj = mu;
+            if (!UNIFORM) {
+                const Attr aj = b.A[j];
+                if (aj.phase != PS_PHASE_FLUID) continue;
+                mj = 1.0f / aj.w;
+            }
+            const float4 qj = b.P[j];
+            float dx = p.x - qj.x, dy = p.y - qj.y, dz = p.z - qj.z;
+            float r2 = ((dx * dx) + (dy * dy)) + (dz * dz);
+            if (!(r2 < k.h2)) continue;
+            float t = k.h2 - r2;
+            float W = k.poly6 * ((t * t) * t);
+            float m = W * mj;
+            const float4 xj = b.X[j];
+            float ex = qj.x - xj.x, ey = qj.y - xj.y, ez = qj.z - xj.z;
+            sx = sx + ((ex - ux) * m);
+            sy = sy + ((ey - uy) * m);
+            sz = sz + ((ez - uz) * m);
+        }
+        const float cr = k.xsph_c / a.rho0;
+        vx = vx + ((cr * sx) / dt);
+        vy = vy + ((cr * sy) / dt);
+        vz = vz + ((cr * sz) / dt);
+    }
+    b.V[i] = make_float4(vx, vy, vz, 0.0f);
+}
+
+// Sleeping (Particle::confirmGuess, cpu/src/particle.h:60-65): a particle
+// that moved less than 1e-4 keeps its start position and gets v = 0.
+__global__ __launch_bounds__(kBlock) void sleep_k(DevBuffers b, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const float4 p = b.P[i];
+    const float4 x = b.X[i];
+    float ex = p.x - x.x, ey = p.y - x.y, ez = p.z - x.z;
+    float l = sqrtf(((ex * ex) + (ey * ey)) + (ez * ez));
+    if (l < 1e-4f) {
+        b.V[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        b.P[i] = make_float4(x.x, x.y, x.z, 0.0f);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+
+static inline uint32_t blocks_for(uint32_t n) { return (n + kBlock - 1) / kBlock; }
+
+uint32_t scan_tiles(uint32_t num_cells) { return (num_cells + kScanTile - 1) / kScanTile; }
+
+hipError_t launch_predict_hash(hipStream_t st, const DevBuffers &b, const KConst &k, uint32_t n,
+                               float dt, bool predict, bool hash)
+{
+    if (n == 0) return hipSuccess;
+    const dim3 g(blocks_for(n)), t(kBlock);
+    if (predict && hash)
+        hipLaunchKernelGGL((predict_hash_k<true, true>), g, t, 0, st, b, k, n, dt);
+    else if (predict)
+        hipLaunchKernelGGL((predict_hash_k<true, false>), g, t, 0, st, b, k, n, dt);
+    else if (hash)
+        hipLaunchKernelGGL((predict_hash_k<false, true>), g, t, 0, st, b, k, n, dt);
+    return hipGetLastError();
+}
+
+hipError_t launch_scan(hipStream_t st, const DevBuffers &b, uint32_t num_cells)
+{
+    const uint32_t tiles = scan_tiles(num_cells);
+    hipLaunchKernelGGL(scan_partials_k, dim3(tiles), dim3(kBlock), 0, st, b.cell_count,
+                       b.scan_part, num_cells);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(scan_final_k, dim3(tiles), dim3(kBlock), 0, st, b.cell_count, b.scan_part,
+                       b.cell_off, num_cells);
+    return hipGetLastError();
+}
+
+hipError_t launch_scatter(hipStream_t st, const DevBuffers &b, uint32_t n)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(scatter_k, dim3(blocks_for(n)), dim3(kBlock), 0, st, b, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_reorder(hipStream_t st, const DevBuffers &b, uint32_t n)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(reorder_k, dim3(blocks_for(n)), dim3(kBlock), 0, st, b, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_neighbors(hipStream_t st, const DevBuffers &b, const KConst &k, uint32_t n,
+                            uint32_t kmax)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(neighbors_k, dim3(blocks_for(n)), dim3(kBlock), 0, st, b, k, n, kmax);
+    return hipGetLastError();
+}
+
+hipError_t launch_lambda(hipStream_t st, const DevBuffers &b, const KConst &k, uint32_t n,
+                         bool uniform, float wu)
+{
+    if (n == 0) return hipSuccess;
+    const dim3 g(blocks_for(n)), t(kBlock);
+    const float mu = 1.0f / wu;
+    if (uniform)
+        hipLaunchKernelGGL((lambda_k<true>), g, t, 0, st, b, k, n, mu);
+    else
+        hipLaunchKernelGGL((lambda_k<false>), g, t, 0, st, b, k, n, mu);
+    return hipGetLastError();
+}
+
+hipError_t launch_shape(hipStream_t st, const DevBuffers &b, uint32_t nbody)
+{
+    if (nbody == 0) return hipSuccess;
+    hipLaunchKernelGGL(shape_k, dim3(blocks_for(nbody)), dim3(kBlock), 0, st, b, nbody);
+    return hipGetLastError();
+}
+
+template <uint32_t F>
+static void solve_variant(hipStream_t st, const DevBuffers &b, const KConst &k, uint32_t n,
+                          bool uniform, float wu, uint64_t iter)
+{
+    const dim3 g(blocks_for(n)), t(kBlock);
+    if (uniform)
+        hipLaunchKernelGGL((solve_k<F, true>), g, t, 0, st, b, k, n, wu, iter);
+    else
+        hipLaunchKernelGGL((solve_k<F, false>), g, t, 0, st, b, k, n, wu, iter);
+}
+
+hipError_t launch_solve(hipStream_t st, const DevBuffers &b, const KConst &k, uint32_t n,
+                        uint32_t features, bool uniform, float wu, uint64_t iter)
+{
+    if (n == 0) return hipSuccess;
+    // uniform-fluid scenes have no contacts, constraints or bodies
+    switch (features) {
+    case F_FLUID: solve_variant<F_FLUID>(st, b, k, n, uniform, wu, iter); break;
+    case F_FLUID | F_PIN: solve_variant<F_FLUID | F_PIN>(st, b, k, n, false, wu, iter); break;
+    default:
+        solve_variant<F_FLUID | F_CONTACT | F_DIST | F_SHAPE | F_PIN>(st, b, k, n, false, wu, iter);
+        break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_finalize(hipStream_t st, const DevBuffers &b, const KConst &k, uint32_t n,
+                           float dt, bool sleeping, bool uniform, float wu)
+{
+    if (n == 0) return hipSuccess;
+    const dim3 g(blocks_for(n)), t(kBlock);
+    const float mu = 1.0f / wu;
+    if (uniform)
+        hipLaunchKernelGGL((finalize_k<true>), g, t, 0, st, b, k, n, mu, dt);
+    else
+        hipLaunchKernelGGL((finalize_k<false>), g, t, 0, st, b, k, n, mu, dt);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || !sleeping) return e;
+    hipLaunchKernelGGL(sleep_k, g, t, 0, st, b, n);
+    return hipGetLastError();
+}
+
+}  // namespace ps

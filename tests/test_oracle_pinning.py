@@ -1,0 +1,427 @@
+"""Pins the CPU oracle (no GPU needed).
+
+Each test checks an oracle function against something it does not share
+code with: the reference's own constants, closed forms worked out for
+one- and two-particle scenes, numpy brute force, invariants (momentum,
+rest lattice, rigid motion), an FP64 SVD, and the FP64 build of the same
+solver.  A plausible mistake (dropped term, wrong sign, swapped operand,
+off-by-one cell) fails at least one of them.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from particlesolver_amd import scenes
+from oracle import Oracle, load_scene
+
+
+def textbook_poly6(r2, h):
+    # Mueller et al. 2003: W(r) = 315/(64 pi h^9) (h^2 - r^2)^3 for r < h
+    return np.where(r2 < h * h, 315.0 / (64.0 * math.pi * h ** 9) * (h * h - r2) ** 3, 0.0)
+
+
+def textbook_spiky_grad_mag(r, h):
+    # |grad W_spiky| = 45/(pi h^6) (h - r)^2 for 0 < r < h
+    return np.where(r < h, 45.0 / (math.pi * h ** 6) * (h - r) ** 2, 0.0)
+
+
+def iso_params(n, h=0.2, radius=0.05, box=8.0, iters=1, prec_relax=600.0, xsph=0.0, k=0.1):
+    p = scenes.pbf_params((0.0, 0.0, 0.0), (box, box, box), h, radius, iters, n, 128,
+                          relaxation=prec_relax, xsph_c=xsph)
+    p.gravity[:] = [0.0, 0.0, 0.0]
+    p.k_corr = k
+    return p
+
+
+# ---------------------------------------------------------------- constants
+
+def test_kernel_constants_match_reference_kats():
+    # integration_kernel.cuh:29-30 at H = 2
+    p = iso_params(4, h=2.0, radius=0.25)
+    p.cell = 2.0
+    # the reference truncates to 11 decimals: agree to one unit in the last
+    # one (FP64), or to FP32 rounding
+    d = Oracle(p, "f64").derived()
+    assert 0.0 <= d["poly6"] - 0.00305992474 < 1e-11
+    assert 0.0 <= d["spiky"] - 0.22381163872 < 1e-11
+    assert abs(d["w_zero"] - 0.00305992474 * 64.0) <= 64 * 1e-11
+    d32 = Oracle(p, "f32").derived()
+    assert d32["poly6"] == np.float32(d["poly6"]) and d32["spiky"] == np.float32(d["spiky"])
+
+
+def test_lattice_rest_density_closed_form():
+    # cubic lattice, spacing h/2: self + 6 at s + 12 at s*sqrt2 + 8 at s*sqrt3
+    h, s = 0.2, 0.1
+    S = (h * h) ** 3 + 6 * (h * h - s * s) ** 3 + 12 * (h * h - 2 * s * s) ** 3 + 8 * (h * h - 3 * s * s) ** 3
+    expect = 315.0 / (64.0 * math.pi * h ** 9) * S
+    assert abs(scenes.lattice_rest_density(h, s) - expect) < 1e-9 * expect
+    assert abs(expect - 1009.7751668946854) < 1e-9
+
+
+def test_derived_constants_library_equals_oracle():
+    from particlesolver_amd import derive
+    for sc in (scenes.dam_break(8, 4, 8), ):
+        d = derive(sc.params)
+        o = Oracle(sc.params, "f32").derived()
+        for k in ("h2", "poly6", "spiky", "w_zero", "w_dq", "inv_w_dq", "tensile_k", "search_r2",
+                  "collide_r2", "wall_s", "wall_k", "jitter"):
+            assert np.float32(getattr(d, k)) == np.float32(o[k]), k
+
+
+def test_wall_uniform_is_splitmix64():
+    o = Oracle(iso_params(4), "f64")
+    for seed, it, orig, wall in [(1, 0, 0, 0), (0x5EED0002, 17, 123456, 5), (2 ** 63 + 5, 2 ** 20, 2 ** 31, 3)]:
+        ctr = ((it << 35) | (orig << 3) | wall) & (2 ** 64 - 1)
+        z = (seed + ctr * 0x9E3779B97F4A7C15) & (2 ** 64 - 1)
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & (2 ** 64 - 1)
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & (2 ** 64 - 1)
+        z ^= z >> 31
+        assert o.wall_uniform(seed, it, orig, wall) == (z >> 40) / 16777216.0
+
+
+# ---------------------------------------------------------------- grid
+
+def grid_case(n, seed, box=(3.0, 2.0, 4.0), h=0.25):
+    p = scenes.pbf_params((0.0, 0.0, 0.0), box, h, h / 4, 1, n, 256)
+    rng = np.random.default_rng(seed)
+    pos = (rng.uniform(-0.2, 1.1, (n, 3)) * np.asarray(box)).astype(np.float32)
+    m = rng.random(n) < 0.3
+    pos[m] = (rng.integers(-1, 14, (n, 3)) * h).astype(np.float32)[m]
+    o = Oracle(p, "f32")
+    o.add_particles(pos, None, np.ones(n, np.float32), np.zeros(n, np.int32), np.ones(n, np.float32))
+    o.predict(0.0)
+    o.build_grid()
+    o.find_neighbors()
+    return p, pos, o
+
+
+def numpy_cells(p, pos):
+    g = np.array(p.grid, dtype=np.int64)
+    f = np.floor((pos - np.asarray(p.origin, np.float32)) / np.float32(p.cell)).astype(np.float64)
+    c = np.where(np.isnan(f) | (f < 0), 0, np.minimum(f, g - 1)).astype(np.int64)
+    return (c[:, 2] * g[1] + c[:, 1]) * g[0] + c[:, 0]
+
+
+@pytest.mark.parametrize("n,seed", [(1, 0), (50, 1), (3000, 2)])
+def test_grid_against_numpy(n, seed):
+    p, pos, o = grid_case(n, seed)
+    idx, cell, off = o.grid()
+    key = numpy_cells(p, pos)
+    order = np.lexsort((np.arange(n), key))          # stable: ties by insertion index
+    assert np.array_equal(idx, order)
+    assert np.array_equal(cell, key[order])
+    nc = p.grid[0] * p.grid[1] * p.grid[2]
+    counts = np.bincount(key, minlength=nc)
+    assert np.array_equal(off, np.concatenate([[0], np.cumsum(counts)]))
+    assert off[-1] == n
+
+
+@pytest.mark.parametrize("n,seed", [(2, 3), (400, 4), (2500, 5)])
+def test_neighbors_against_brute_force(n, seed):
+    p, pos, o = grid_case(n, seed)
+    idx, cell, off = o.grid()
+    cnt, tab = o.neighbors()
+    sp = pos[idx]                                     # positions in sorted order
+    d = sp[:, None, :] - sp[None, :, :]               # float32, same ops as the oracle
+    r2 = (d[..., 0] * d[..., 0] + d[..., 1] * d[..., 1]) + d[..., 2] * d[..., 2]
+    within = (r2 < np.float32(p.h) * np.float32(p.h)) & ~np.eye(n, dtype=bool)
+    for i in range(n):
+        expect = np.nonzero(within[i])[0]             # ascending slot order
+        assert cnt[i] == len(expect), i
+        assert np.array_equal(tab[i, :cnt[i]], expect), i
+
+
+def test_neighbor_overflow_is_reported():
+    p = scenes.pbf_params((0.0, 0.0, 0.0), (1.0, 1.0, 1.0), 0.25, 0.05, 1, 200, 4)
+    rng = np.random.default_rng(9)
+    pos = rng.uniform(0.4, 0.6, (200, 3)).astype(np.float32)
+    o = Oracle(p, "f32")
+    o.add_particles(pos, None, np.ones(200, np.float32), np.zeros(200, np.int32), np.ones(200, np.float32))
+    o.predict(0.0)
+    o.build_grid()
+    o.find_neighbors()
+    st = o.status()
+    assert st.flags & 1
+    assert st.max_neighbors == 199
+    cnt, _ = o.neighbors()
+    assert cnt.max() == 4
+
+
+# ---------------------------------------------------------------- physics
+
+def test_predict_closed_form():
+    p = iso_params(2)
+    p.gravity[:] = [0.5, -9.8, 1.25]
+    o = Oracle(p, "f64")
+    x = np.array([[1.0, 2.0, 3.0], [4.0, 4.0, 4.0]], np.float32)
+    v = np.array([[0.25, 0.5, -1.0], [1.0, 1.0, 1.0]], np.float32)
+    o.add_particles(x, v, np.array([1.0, 0.0], np.float32), np.array([3, 3], np.int32), np.ones(2, np.float32))
+    dt = 0.125
+    o.predict(dt)
+    it, _ = o.sorted_state()
+    idx = o.grid()[0] if False else np.arange(2)
+    expect0 = x[0].astype(np.float64) + (v[0] + np.array([0.5, -9.8, 1.25], np.float64) * dt) * dt
+    # the oracle computes in FP64 from FP32 inputs; gravity is rounded to FP32 first
+    g32 = np.array([0.5, -9.8, 1.25], np.float32).astype(np.float64)
+    expect0 = x[0].astype(np.float64) + (v[0].astype(np.float64) + g32 * dt) * dt
+    assert np.allclose(it[idx[0], :3], expect0, rtol=0, atol=1e-14)
+    assert np.array_equal(it[idx[1], :3], x[1].astype(np.float64))   # static: w = 0
+
+
+def two_particle(r, h=0.2, rho0=1009.775, k=0.1, eps=600.0):
+    p = iso_params(2, h=h, k=k, prec_relax=eps)
+    o = Oracle(p, "f64")
+    x = np.array([[4.0, 4.0, 4.0], [4.0 + r, 4.0, 4.0]], np.float32)
+    o.add_particles(x, None, np.ones(2, np.float32), np.zeros(2, np.int32), np.full(2, rho0, np.float32))
+    o.begin(1.0 / 60.0)
+    return p, o, x.astype(np.float64)
+
+
+@pytest.mark.parametrize("r", [0.05, 0.1, 0.15, 0.199])
+def test_lambda_two_particles_closed_form(r):
+    h, rho0, eps = 0.2, np.float64(np.float32(1009.775)), 600.0
+    p, o, x = two_particle(r, h, rho0)
+    lam = o.compute_lambda()
+    rr = np.float64(np.float32(4.0 + r)) - 4.0
+    rho = textbook_poly6(0.0, h) + textbook_poly6(rr * rr, h)
+    g = textbook_spiky_grad_mag(rr, h) / rho0             # |grad_j C_i| = |grad_i C_i|
+    expect = -(rho / rho0 - 1.0) / (2.0 * g * g + eps)
+    assert np.allclose(lam, expect, rtol=2e-6, atol=0)     # FP32-rounded constants
+    assert lam[0] == lam[1]                                # symmetric pair
+
+
+@pytest.mark.parametrize("r", [0.07, 0.12])
+def test_delta_p_two_particles_closed_form(r):
+    h, rho0, eps, k = 0.2, np.float64(np.float32(1009.775)), 600.0, 0.1
+    p, o, x = two_particle(r, h, rho0, k, eps)
+    lam = o.compute_lambda()
+    o.iterate()
+    it, _ = o.sorted_state()
+    rr = x[1, 0] - x[0, 0]
+    W = textbook_poly6(rr * rr, h)
+    Wdq = textbook_poly6((0.2 * h) ** 2, h)
+    s_corr = -k * h * h * (W / Wdq) ** 4
+    # grad_i W(p_i - p_j) points from i towards j (+x for particle 0)
+    gradW = textbook_spiky_grad_mag(rr, h)
+    dp0 = (lam[0] + lam[1] + s_corr) * gradW / rho0       # x-component on particle 0
+    assert np.allclose(it[0, 0] - x[0, 0], dp0, rtol=2e-6, atol=1e-15)
+    assert np.allclose(it[1, 0] - x[1, 0], -dp0, rtol=2e-6, atol=1e-15)
+    assert np.allclose(it[:, 1:3], x[:, 1:3], atol=0)
+
+
+def test_rest_lattice_interior_lambda_vanishes():
+    h, s = 0.2, 0.1
+    rho0 = scenes.lattice_rest_density(h, s)
+    n = 9
+    p = iso_params(n ** 3, h=h, box=4.0)
+    pos = scenes.fluid_block(n, n, n, s, (1.0, 1.0, 1.0), 0.0, 1)
+    o = Oracle(p, "f64")
+    o.add_particles(pos, None, np.ones(n ** 3, np.float32), np.zeros(n ** 3, np.int32),
+                    np.full(n ** 3, rho0, np.float32))
+    o.begin(1e-3)
+    lam = o.compute_lambda()
+    idx = o.grid()[0]
+    ijk = np.stack(np.unravel_index(idx, (n, n, n), order="F"), 1)  # x fastest
+    interior = np.all((ijk >= 2) & (ijk <= n - 3), axis=1)
+    assert interior.sum() > 100
+    # only the FP32 rounding of the lattice positions and rho0 remains
+    assert np.abs(lam[interior]).max() < 1e-8
+    assert np.all(lam[~interior] >= -1e-8)   # boundary particles are under-dense: lambda >= 0
+
+
+def test_pbf_pair_terms_conserve_momentum():
+    rng = np.random.default_rng(4)
+    n = 300
+    p = iso_params(n, box=4.0)
+    pos = (2.0 + rng.uniform(-0.3, 0.3, (n, 3))).astype(np.float32)
+    o = Oracle(p, "f64")
+    o.add_particles(pos, None, np.ones(n, np.float32), np.zeros(n, np.int32), np.full(n, 1009.775, np.float32))
+    o.begin(1e-3)
+    before = o.sorted_state()[0][:, :3].copy()
+    o.iterate()
+    after = o.sorted_state()[0][:, :3]
+    disp = after - before
+    assert np.abs(disp).max() > 1e-4                 # something happened
+    assert np.abs(disp.sum(0)).max() < 1e-12 * n
+
+
+def test_walls_push_out_with_jitter_and_static_friction():
+    p = iso_params(3, box=2.0, k=0.0)
+    o = Oracle(p, "f64")
+    x = np.array([[1.0, -0.3, 1.0], [2.4, 1.0, 1.0], [1.0, 1.0, 1.0]], np.float32)
+    ph = np.array([0, 3, 0], np.int32)
+    o.add_particles(x, None, np.ones(3, np.float32), ph, np.ones(3, np.float32))
+    o.begin(1e-3)
+    o.iterate()
+    it, _ = o.sorted_state()
+    order = o.grid()[0]
+    got = {int(order[t]): it[t, :3] for t in range(3)}
+    r = float(np.float32(p.radius))
+    jit = float(np.float32(p.wall_jitter)) * r
+    # fluid below the floor: y = ymin + r + U5 * jitter
+    u = o.wall_uniform(p.seed, 0, 0, 5)
+    assert got[0][1] == (0.0 + r) + u * jit
+    assert got[0][0] == 1.0 and got[0][2] == 1.0
+    # solid beyond +x: x = xmax - (r + U0 * jitter); tangential motion 0 -> no friction change
+    u0 = o.wall_uniform(p.seed, 0, 1, 0)
+    assert got[1][0] == 2.0 - (r + u0 * jit)
+    assert np.array_equal(got[2], x[2].astype(np.float64))
+
+
+def test_distance_constraint_two_particles_exact():
+    p = iso_params(2, k=0.0)
+    o = Oracle(p, "f64")
+    x = np.array([[3.0, 3.0, 3.0], [3.3, 3.4, 3.0]], np.float32)   # cloth: no fluid terms
+    o.add_particles(x, None, np.array([1.0, 3.0], np.float32), np.full(2, 2, np.int32), np.ones(2, np.float32))
+    o.add_distance_constraints([[0, 1]], [0.25], [1.0])
+    o.begin(1e-3)
+    o.iterate()
+    it, _ = o.sorted_state()
+    idx = o.grid()[0]
+    a, b = it[np.argsort(idx)][:, :3]
+    assert abs(np.linalg.norm(a - b) - 0.25) < 1e-12
+    # mass-weighted: the lighter particle (w=3) moves 3x as far
+    x64 = x.astype(np.float64)
+    assert abs(np.linalg.norm(b - x64[1]) / np.linalg.norm(a - x64[0]) - 3.0) < 1e-9
+    # centre of mass kept
+    assert np.allclose(a * 1.0 + b / 3.0, x64[0] + x64[1] / 3.0, atol=1e-12)
+
+
+def test_contact_two_cloth_particles_resolves_penetration():
+    p = iso_params(2, k=0.0)
+    p.collide_dist = 0.1
+    o = Oracle(p, "f64")
+    x = np.array([[3.0, 3.0, 3.0], [3.06, 3.0, 3.0]], np.float32)
+    o.add_particles(x, None, np.ones(2, np.float32), np.array([4, 5], np.int32), np.ones(2, np.float32))
+    o.begin(1e-3)
+    o.iterate()
+    it, _ = o.sorted_state()
+    d = np.linalg.norm(it[0, :3] - it[1, :3])
+    assert abs(d - np.float64(np.float32(0.1))) < 1e-12
+    assert np.allclose(it[:, :3].sum(0), x.astype(np.float64).sum(0), atol=1e-12)
+
+
+def test_same_body_particles_do_not_collide():
+    p = iso_params(2, k=0.0)
+    p.collide_dist = 0.1
+    o = Oracle(p, "f64")
+    x = np.array([[3.0, 3.0, 3.0], [3.06, 3.0, 3.0]], np.float32)
+    o.add_particles(x, None, np.ones(2, np.float32), np.array([5, 5], np.int32), np.ones(2, np.float32))
+    o.begin(1e-3)
+    o.iterate()
+    it, _ = o.sorted_state()
+    assert np.allclose(np.sort(it[:, 0]), [3.0, np.float32(3.06)], atol=0)
+
+
+def test_xsph_closed_form_and_momentum():
+    # two fluid particles, no pressure terms (eps huge, k = 0), given velocities
+    h, rho0, c = 0.2, 1009.775, 0.01
+    p = iso_params(2, h=h, prec_relax=1e30, xsph=c, k=0.0)
+    o = Oracle(p, "f64")
+    x = np.array([[3.0, 3.0, 3.0], [3.1, 3.0, 3.0]], np.float32)
+    v = np.array([[1.0, 0.0, 0.0], [0.0, 2.0, 0.0]], np.float32)
+    o.add_particles(x, v, np.ones(2, np.float32), np.zeros(2, np.int32), np.full(2, rho0, np.float32))
+    dt = 1e-3
+    o.step(dt, 1)
+    vv = o.velocities()
+    xx = o.positions()
+    u = xx - x.astype(np.float64)
+    r2 = np.sum((xx[0] - xx[1]) ** 2)
+    W = textbook_poly6(r2, h)
+    rho0_32 = np.float64(np.float32(rho0))
+    expect0 = u[0] / dt + c / rho0_32 * (u[1] - u[0]) * W / dt
+    expect1 = u[1] / dt + c / rho0_32 * (u[0] - u[1]) * W / dt
+    assert np.allclose(vv[0], expect0, rtol=1e-5, atol=1e-9)
+    assert np.allclose(vv[1], expect1, rtol=1e-5, atol=1e-9)
+    assert np.allclose(vv.sum(0), (u[0] + u[1]) / dt, rtol=0, atol=1e-9)   # momentum
+
+
+def test_xsph_keeps_uniform_velocity():
+    n = 6
+    p = iso_params(n ** 3, xsph=0.01, k=0.0, prec_relax=1e30)
+    pos = scenes.fluid_block(n, n, n, 0.1, (2.0, 2.0, 2.0), 0.0, 1)
+    v = np.tile(np.array([[0.3, -0.2, 0.1]], np.float32), (n ** 3, 1))
+    o = Oracle(p, "f64")
+    o.add_particles(pos, v, np.ones(n ** 3, np.float32), np.zeros(n ** 3, np.int32), np.full(n ** 3, 1009.775, np.float32))
+    o.step(1e-3, 1)
+    assert np.allclose(o.velocities(), v.astype(np.float64), atol=1e-9)
+
+
+def rotation(axis, ang):
+    axis = np.asarray(axis, float) / np.linalg.norm(axis)
+    K = np.array([[0, -axis[2], axis[1]], [axis[2], 0, -axis[0]], [-axis[1], axis[0], 0]])
+    return np.eye(3) + math.sin(ang) * K + (1 - math.cos(ang)) * K @ K
+
+
+def quat_matrix(q):
+    w, x, y, z = q
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],
+                     [2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
+                     [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)]])
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_extract_rotation_matches_svd_polar(seed):
+    rng = np.random.default_rng(seed)
+    R = rotation(rng.normal(size=3), rng.uniform(-2.5, 2.5))
+    S = np.eye(3) + 0.3 * (lambda M: M + M.T)(rng.normal(size=(3, 3)))  # symmetric, near I
+    S = S @ S.T * 0.5 + np.eye(3) * 0.5                                    # keep it SPD
+    A = R @ S
+    U, _, Vt = np.linalg.svd(A)
+    Rp = U @ Vt
+    o = Oracle(iso_params(2), "f64")
+    q = o.extract_rotation(A, sweeps=60)
+    assert np.allclose(quat_matrix(q), Rp, atol=1e-7)
+
+
+def test_shape_matching_reproduces_rigid_motion():
+    # a 3x3x3 cube moved rigidly: every goal equals the moved position
+    rng = np.random.default_rng(1)
+    g = np.stack(np.meshgrid(*[np.arange(3.0)] * 3, indexing="ij"), -1).reshape(-1, 3) * 0.1
+    p = iso_params(27, k=0.0)
+    o = Oracle(p, "f64")
+    rest = (g + 2.0).astype(np.float32)
+    o.add_particles(rest, None, rng.uniform(0.5, 2.0, 27).astype(np.float32), np.full(27, 4, np.int32),
+                    np.ones(27, np.float32))
+    o.add_rigid_body(np.arange(27))
+    R = rotation([1.0, 2.0, 0.5], 0.7)
+    x64 = rest.astype(np.float64)
+    w = None
+    # rotate the current positions about the (mass-weighted) centre and shift
+    o2 = Oracle(p, "f64")
+    wts = rng.uniform(0.5, 2.0, 27).astype(np.float32)
+    o2.add_particles(rest, None, wts, np.full(27, 4, np.int32), np.ones(27, np.float32))
+    o2.add_rigid_body(np.arange(27))
+    m = 1.0 / wts.astype(np.float64)
+    c = (m[:, None] * x64).sum(0) / m.sum()
+    moved = ((x64 - c) @ R.T + c + np.array([0.1, -0.05, 0.02])).astype(np.float32)
+    o3 = Oracle(p, "f64")
+    o3.add_particles(rest, None, wts, np.full(27, 4, np.int32), np.ones(27, np.float32))
+    o3.add_rigid_body(np.arange(27))
+    # replace positions by the moved ones: a fresh oracle with the body defined at rest,
+    # then velocities that carry rest -> moved in one predict step
+    dt = 1.0
+    o3.__class__  # noqa
+    o4 = Oracle(p, "f64")
+    o4.add_particles(rest, (moved.astype(np.float64) - x64).astype(np.float32), wts,
+                     np.full(27, 4, np.int32), np.ones(27, np.float32))
+    o4.add_rigid_body(np.arange(27))
+    o4.begin(dt)
+    pred = o4.sorted_state()[0][:, :3].copy()
+    o4.iterate()
+    it, _ = o4.sorted_state()
+    # stiffness 1 and a rigid transform: the shape term leaves every particle where it is
+    assert np.allclose(it[:, :3], pred, atol=1e-6)
+    del w, o, o2
+
+
+def test_fp32_tracks_fp64_short_horizon():
+    sc = scenes.dam_break(12, 8, 12, max_neighbors=96)
+    a = load_scene(sc, "f32")
+    b = load_scene(sc, "f64")
+    for _ in range(5):
+        a.step(sc.dt, 1)
+        b.step(sc.dt, 1)
+    err = np.abs(a.positions().astype(np.float64) - b.positions()).max()
+    assert err < 1e-4, err
