@@ -90,7 +90,7 @@ def main():
     import numpy as np
 
     nx, ny, nz = (max(2, int(round(d * args.scale))) for d in (128, 64, 128))
-    sc = scenes.dam_break(nx, ny, nz, max_neighbors=128)
+    sc = scenes.dam_break(nx, ny, nz, max_neighbors=192)
     N, I = sc.n, sc.params.iterations
 
     g = System(sc.params)
@@ -139,7 +139,7 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        csc = scenes.dam_break(nx, ny, nz, max_neighbors=128)
+        csc = scenes.dam_break(nx, ny, nz, max_neighbors=192)
         cpu = cpu_baseline(csc, args.cpu_seconds)
 
     line = {

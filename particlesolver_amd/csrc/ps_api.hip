@@ -201,8 +201,8 @@ int validate(const ps_params *p, std::string &why)
         why = "max_particles must be in [1, 2^30]";
         return PS_ERR_INVALID_ARGUMENT;
     }
-    if (p->max_neighbors == 0 || p->max_neighbors > 4096) {
-        why = "max_neighbors must be in [1, 4096]";
+    if (p->max_neighbors == 0 || p->max_neighbors > 4096 || (p->max_neighbors & 3u)) {
+        why = "max_neighbors must be a multiple of 4 in [4, 4096]";
         return PS_ERR_INVALID_ARGUMENT;
     }
     const float nonneg[] = {p->relaxation, p->k_corr, p->dq_corr, p->xsph_c, p->s_solid,
@@ -561,7 +561,8 @@ int ps_create(const ps_params *params, ps_system **out)
     s->member_of.assign(s->cap, 0);
     const size_t c = s->cap;
     DevBuffers &b = s->b;
-    b.stride = (uint32_t)((c + 63) / 64 * 64);
+    b.kq = s->kmax / 4;
+    const size_t groups = (c + 63) / 64;
 #define ALLOC(ptr, cnt)                                 \
     do {                                                \
         int r_ = dev_alloc(s, &(ptr), (cnt));           \
@@ -576,7 +577,7 @@ int ps_create(const ps_params *params, ps_system **out)
     ALLOC(b.tmp_orig, c); ALLOC(b.slot_of, c);
     ALLOC(b.cell_count, s->ncell); ALLOC(b.cell_off, (size_t)s->ncell + 1);
     ALLOC(b.scan_part, s->tiles);
-    ALLOC(b.nbr, (size_t)s->kmax * b.stride); ALLOC(b.ncnt, c);
+    ALLOC(b.nbr, groups * 64 * s->kmax); ALLOC(b.ncnt, c);
     ALLOC(b.dist_off, c + 1); ALLOC(b.occ, c); ALLOC(b.pin_of, c);
     ALLOC(b.body_of, c); ALLOC(b.member_of, c); ALLOC(b.status, 1);
 #undef ALLOC
@@ -1026,12 +1027,15 @@ int ps_read_neighbors(ps_system *s, uint32_t *counts, uint32_t *table)
     HIP_TRY(s, hipMemcpy(cnt.data(), s->b.ncnt, sizeof(uint32_t) * s->n, hipMemcpyDeviceToHost));
     if (counts) memcpy(counts, cnt.data(), sizeof(uint32_t) * s->n);
     if (table) {
-        const uint32_t stride = s->b.stride;
-        std::vector<uint32_t> col((size_t)s->kmax * stride);
-        HIP_TRY(s, hipMemcpy(col.data(), s->b.nbr, sizeof(uint32_t) * col.size(), hipMemcpyDeviceToHost));
+        const size_t groups = (s->n + 63) / 64;
+        const uint32_t kq = s->b.kq;
+        std::vector<uint32_t> raw(groups * 64 * s->kmax);
+        HIP_TRY(s, hipMemcpy(raw.data(), s->b.nbr, sizeof(uint32_t) * raw.size(), hipMemcpyDeviceToHost));
         for (uint32_t i = 0; i < s->n; i++)
-            for (uint32_t k = 0; k < s->kmax; k++)
-                table[(size_t)i * s->kmax + k] = k < cnt[i] ? col[(size_t)k * stride + i] : 0u;
+            for (uint32_t k = 0; k < s->kmax; k++) {
+                const size_t at = (((size_t)(i >> 6) * kq + (k >> 2)) * 64 + (i & 63)) * 4 + (k & 3);
+                table[(size_t)i * s->kmax + k] = k < cnt[i] ? raw[at] : 0u;
+            }
     }
     return PS_OK;
 }

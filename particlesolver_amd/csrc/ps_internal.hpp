@@ -62,10 +62,10 @@ struct DevBuffers {
     uint32_t *cell_count;  // num_cells, zero between builds
     uint32_t *cell_off;    // num_cells + 1
     uint32_t *scan_part;   // per scan tile
-    // neighbours, column-major: nbr[k * stride + i]
+    // neighbours: sliced ELL in chunks of four (see ps_kernels.hip)
     uint32_t *nbr;
     uint32_t *ncnt;
-    uint32_t stride;
+    uint32_t kq;  // max_neighbors / 4
     // distance constraints (insertion-index CSR)
     uint32_t *dist_off;    // max_particles + 1
     uint32_t *dist_adj;    // 2*m: constraint ids

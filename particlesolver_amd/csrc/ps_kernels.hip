@@ -4,9 +4,11 @@
 // particle's insertion index travels in Attr::orig.  Positions and the PBF
 // multiplier share one float4 (x, y, z, lambda), so each neighbour visit in
 // the lambda and delta-p kernels is a single 16-byte gather.  The neighbour
-// table is column-major (entry k of particle i at nbr[k*stride + i]), so a
-// wavefront reading entry k of 64 consecutive particles issues one coalesced
-// 256-byte load.
+// table is a sliced ELL in chunks of four: for each group of 64 consecutive
+// particles, entries 4m..4m+3 of every lane form one uint4, and the 64 lanes'
+// uint4s are contiguous.  A wavefront therefore reads four neighbour indices
+// per lane with one coalesced 1-KiB load and issues the four position
+// gathers back to back; the builder writes 16 bytes per lane at a time.
 //
 // Arithmetic: built with -ffp-contract=off and IEEE division/sqrt, and every
 // expression is parenthesised as in the CPU oracle, so FP32 results are
@@ -61,6 +63,18 @@ __device__ __forceinline__ uint32_t wave_max(uint32_t v)
     return v;
 }
 
+// chunk m (entries 4m..4m+3) of particle i is nbr_lane(b, i)[m * 64]
+__device__ __forceinline__ const uint4 *nbr_lane(const DevBuffers &b, uint32_t i)
+{
+    return reinterpret_cast<const uint4 *>(b.nbr) + (size_t)(i >> 6) * b.kq * 64 + (i & 63);
+}
+
+// entry q of particle i (single-entry access for the less travelled loops)
+__device__ __forceinline__ uint32_t nbr_at(const DevBuffers &b, uint32_t i, uint32_t q)
+{
+    return reinterpret_cast<const uint32_t *>(nbr_lane(b, i) + (size_t)(q >> 2) * 64)[q & 3u];
+}
+
 // ---------------------------------------------------------------------------
 // a1 + a2: predict (integrate_functor, integration_kernel.cuh:159-184) and
 // cell hash (calcGridPos/calcGridHash/calcHashD :187-225) with a dense,
@@ -88,39 +102,61 @@ __global__ __launch_bounds__(kBlock) void predict_hash_k(DevBuffers b, KConst k,
                                                          float dt)
 {
     const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
-    if (s >= n) return;
-    float px, py, pz;
-    if (PRED) {
-        const float4 x = b.X[s];
-        const Attr a = b.A[s];
-        px = x.x;
-        py = x.y;
-        pz = x.z;
-        if (a.w != 0.0f) {
-            const float4 v = b.V[s];
-            float vx = v.x + (k.gravity[0] * dt);
-            float vy = v.y + (k.gravity[1] * dt);
-            float vz = v.z + (k.gravity[2] * dt);
-            px = x.x + (vx * dt);
-            py = x.y + (vy * dt);
-            pz = k.dims == 2u ? x.z : (x.z + (vz * dt));
+    const bool valid = s < n;
+    float px = 0.0f, py = 0.0f, pz = 0.0f;
+    if (valid) {
+        if (PRED) {
+            const float4 x = b.X[s];
+            const Attr a = b.A[s];
+            px = x.x;
+            py = x.y;
+            pz = x.z;
+            if (a.w != 0.0f) {
+                const float4 v = b.V[s];
+                float vx = v.x + (k.gravity[0] * dt);
+                float vy = v.y + (k.gravity[1] * dt);
+                float vz = v.z + (k.gravity[2] * dt);
+                px = x.x + (vx * dt);
+                py = x.y + (vy * dt);
+                pz = k.dims == 2u ? x.z : (x.z + (vz * dt));
+            }
+            b.P[s] = make_float4(px, py, pz, 0.0f);
+        } else {
+            const float4 p = b.P[s];
+            px = p.x;
+            py = p.y;
+            pz = p.z;
         }
-        b.P[s] = make_float4(px, py, pz, 0.0f);
-    } else {
-        const float4 p = b.P[s];
-        px = p.x;
-        py = p.y;
-        pz = p.z;
     }
     if (HASH) {
-        uint32_t flags = 0;
-        uint32_t cx = cell_axis(px, k.origin[0], k.cell, k.grid[0], flags);
-        uint32_t cy = cell_axis(py, k.origin[1], k.cell, k.grid[1], flags);
-        uint32_t cz = cell_axis(pz, k.origin[2], k.cell, k.grid[2], flags);
-        uint32_t key = (cz * k.grid[1] + cy) * k.grid[0] + cx;
-        b.key[s] = key;
-        b.rank[s] = atomicAdd(&b.cell_count[key], 1u);
-        if (flags) atomicOr(&b.status->flags, flags);
+        uint32_t flags = 0, key = 0xffffffffu;
+        if (valid) {
+            uint32_t cx = cell_axis(px, k.origin[0], k.cell, k.grid[0], flags);
+            uint32_t cy = cell_axis(py, k.origin[1], k.cell, k.grid[1], flags);
+            uint32_t cz = cell_axis(pz, k.origin[2], k.cell, k.grid[2], flags);
+            key = (cz * k.grid[1] + cy) * k.grid[0] + cx;
+        }
+        // Storage is in last substep's cell order, so equal keys sit in runs
+        // of adjacent lanes: one atomic per run hands out the run's ranks.
+        const uint32_t lane = threadIdx.x & 63u;
+        const uint32_t prev = __shfl_up(key, 1, 64);
+        const bool head = valid && (lane == 0 || key != prev);
+        const uint64_t heads = __ballot(head);
+        const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+        const uint64_t mine = heads & upto;
+        const uint32_t seg = mine ? 63u - (uint32_t)__clzll((unsigned long long)mine) : lane;
+        const uint64_t later = heads & ~upto;
+        const uint64_t vmask = __ballot(valid);
+        const uint32_t vend = vmask == ~0ull ? 64u : (uint32_t)__ffsll((unsigned long long)~vmask) - 1u;
+        const uint32_t next = later ? (uint32_t)__ffsll((unsigned long long)later) - 1u : vend;
+        uint32_t base = 0;
+        if (head) base = atomicAdd(&b.cell_count[key], next - seg);
+        base = __shfl(base, seg, 64);
+        if (valid) {
+            b.key[s] = key;
+            b.rank[s] = base + (lane - seg);
+            if (flags) atomicOr(&b.status->flags, flags);
+        }
     }
 }
 
@@ -269,7 +305,8 @@ __global__ __launch_bounds__(kBlock) void neighbors_k(DevBuffers b, KConst k, ui
         const uint32_t x0 = cx > 0 ? cx - 1 : 0, x1 = cx + 1 < gx ? cx + 1 : gx - 1;
         const float4 p = b.P[i];
         const float r2max = k.search_r2;
-        const uint32_t stride = b.stride;
+        uint4 *out = const_cast<uint4 *>(nbr_lane(b, i));
+        uint4 buf = make_uint4(i, i, i, i);
         for (int dz = -1; dz <= 1; dz++) {
             const int z = (int)cz + dz;
             if (z < 0 || z >= (int)gz) continue;
@@ -279,19 +316,38 @@ __global__ __launch_bounds__(kBlock) void neighbors_k(DevBuffers b, KConst k, ui
                 const uint32_t row = ((uint32_t)z * gy + (uint32_t)y) * gx;
                 const uint32_t j0 = b.cell_off[row + x0];
                 const uint32_t j1 = min(b.cell_off[row + x1 + 1], n);
-                for (uint32_t j = j0; j < j1; j++) {
-                    if (j == i) continue;
-                    const float4 q = b.P[j];
-                    float dx = p.x - q.x, dy_ = p.y - q.y, dz_ = p.z - q.z;
-                    float r2 = ((dx * dx) + (dy_ * dy_)) + (dz_ * dz_);
-                    if (r2 < r2max) {
-                        if (cnt < kmax) b.nbr[(size_t)cnt * stride + i] = j;
-                        cnt++;
+                for (uint32_t jb = j0; jb < j1; jb += 4) {
+                    float4 qv[4];
+#pragma unroll
+                    for (uint32_t u = 0; u < 4; u++)
+                        if (jb + u < j1) qv[u] = b.P[jb + u];
+#pragma unroll
+                    for (uint32_t u = 0; u < 4; u++) {
+                        const uint32_t j = jb + u;
+                        if (j >= j1 || j == i) continue;
+                        float dx = p.x - qv[u].x, dy_ = p.y - qv[u].y, dz_ = p.z - qv[u].z;
+                        float r2 = ((dx * dx) + (dy_ * dy_)) + (dz_ * dz_);
+                        if (r2 < r2max) {
+                            if (cnt < kmax) {
+                                const uint32_t sl = cnt & 3u;
+                                buf.x = sl == 0 ? j : buf.x;
+                                buf.y = sl == 1 ? j : buf.y;
+                                buf.z = sl == 2 ? j : buf.z;
+                                buf.w = sl == 3 ? j : buf.w;
+                                if (sl == 3) {
+                                    out[(size_t)(cnt >> 2) * 64] = buf;
+                                    buf = make_uint4(i, i, i, i);
+                                }
+                            }
+                            cnt++;
+                        }
                     }
                 }
             }
         }
-        b.ncnt[i] = cnt < kmax ? cnt : kmax;
+        const uint32_t kept = cnt < kmax ? cnt : kmax;
+        if (kept & 3u) out[(size_t)(kept >> 2) * 64] = buf;
+        b.ncnt[i] = kept;
     }
     // one status update per block
     uint32_t m = wave_max(cnt);
@@ -329,35 +385,47 @@ __global__ __launch_bounds__(kBlock) void lambda_k(DevBuffers b, KConst k, uint3
     float rho = k.w_zero * (UNIFORM ? mu : (1.0f / a.w));
     float gix = 0.0f, giy = 0.0f, giz = 0.0f, den = 0.0f;
     const uint32_t c = b.ncnt[i];
-    const uint32_t stride = b.stride;
-    for (uint32_t q = 0; q < c; q++) {
-        const uint32_t j = b.nbr[(size_t)q * stride + i];
-        float wj = 1.0f;
-        int32_t phj = PS_PHASE_FLUID;
-        if (!UNIFORM) {
-            const Attr aj = b.A[j];
-            wj = aj.w;
-            phj = aj.phase;
-            if (wj == 0.0f) continue;
+    const uint4 *nb = nbr_lane(b, i);
+    for (uint32_t q = 0; q < c; q += 4) {
+        const uint4 jj = nb[(size_t)(q >> 2) * 64];
+        const uint32_t jv[4] = {jj.x, jj.y, jj.z, jj.w};
+        float4 pv[4];
+        Attr av[4];
+#pragma unroll
+        for (uint32_t u = 0; u < 4; u++)
+            if (q + u < c) {
+                pv[u] = b.P[jv[u]];
+                if (!UNIFORM) av[u] = b.A[jv[u]];
+            }
+#pragma unroll
+        for (uint32_t u = 0; u < 4; u++) {
+            if (q + u >= c) break;
+            float wj = 1.0f;
+            int32_t phj = PS_PHASE_FLUID;
+            if (!UNIFORM) {
+                wj = av[u].w;
+                phj = av[u].phase;
+                if (wj == 0.0f) continue;
+            }
+            const float4 pj = pv[u];
+            float dx = p.x - pj.x, dy = p.y - pj.y, dz = p.z - pj.z;
+            float r2 = ((dx * dx) + (dy * dy)) + (dz * dz);
+            if (!(r2 < k.h2)) continue;
+            float t = k.h2 - r2;
+            float W = k.poly6 * ((t * t) * t);
+            float cc = W * (UNIFORM ? mu : (1.0f / wj));
+            if (phj != PS_PHASE_FLUID) cc = cc * k.s_solid;
+            rho = rho + cc;
+            float r = sqrtf(r2);
+            if (r < 1e-4f) continue;
+            float qq = k.h - r;
+            float aa = ((k.spiky * (qq * qq)) / r) * ir0;
+            float gx = aa * dx, gy = aa * dy, gz = aa * dz;
+            den = den + (((gx * gx) + (gy * gy)) + (gz * gz));
+            gix = gix + gx;
+            giy = giy + gy;
+            giz = giz + gz;
         }
-        const float4 pj = b.P[j];
-        float dx = p.x - pj.x, dy = p.y - pj.y, dz = p.z - pj.z;
-        float r2 = ((dx * dx) + (dy * dy)) + (dz * dz);
-        if (!(r2 < k.h2)) continue;
-        float t = k.h2 - r2;
-        float W = k.poly6 * ((t * t) * t);
-        float cc = W * (UNIFORM ? mu : (1.0f / wj));
-        if (phj != PS_PHASE_FLUID) cc = cc * k.s_solid;
-        rho = rho + cc;
-        float r = sqrtf(r2);
-        if (r < 1e-4f) continue;
-        float qq = k.h - r;
-        float aa = ((k.spiky * (qq * qq)) / r) * ir0;
-        float gx = aa * dx, gy = aa * dy, gz = aa * dz;
-        den = den + (((gx * gx) + (gy * gy)) + (gz * gz));
-        gix = gix + gx;
-        giy = giy + gy;
-        giz = giz + gz;
     }
     den = den + (((gix * gix) + (giy * giy)) + (giz * giz));
     float C = (rho / r0) - 1.0f;
@@ -488,7 +556,6 @@ __global__ __launch_bounds__(kBlock) void solve_k(DevBuffers b, KConst k, uint32
     const int32_t ph = UNIFORM ? PS_PHASE_FLUID : a.phase;
     const float wi = UNIFORM ? wu : a.w;
     const uint32_t c = b.ncnt[i];
-    const uint32_t stride = b.stride;
     float ax = 0.0f, ay = 0.0f, az = 0.0f;
 
     // a7: particle contacts + friction (collideD, integration_kernel.cuh:358-462)
@@ -496,7 +563,7 @@ __global__ __launch_bounds__(kBlock) void solve_k(DevBuffers b, KConst k, uint32
         if (ph >= PS_PHASE_CLOTH && wi != 0.0f) {
             uint32_t nc = 0;
             for (uint32_t q = 0; q < c; q++) {
-                const uint32_t j = b.nbr[(size_t)q * stride + i];
+                const uint32_t j = nbr_at(b, i, q);
                 const int32_t pj = b.A[j].phase;
                 if (ph > PS_PHASE_SOLID && ph == pj) continue;
                 const float4 qj = b.P[j];
@@ -510,7 +577,7 @@ __global__ __launch_bounds__(kBlock) void solve_k(DevBuffers b, KConst k, uint32
                 const float4 xi = b.X[i];
                 float cx = 0.0f, cy = 0.0f, cz = 0.0f;
                 for (uint32_t q = 0; q < c; q++) {
-                    const uint32_t j = b.nbr[(size_t)q * stride + i];
+                    const uint32_t j = nbr_at(b, i, q);
                     const Attr aj = b.A[j];
                     const int32_t pj = aj.phase;
                     if (ph > PS_PHASE_SOLID && ph == pj) continue;
@@ -573,27 +640,41 @@ __global__ __launch_bounds__(kBlock) void solve_k(DevBuffers b, KConst k, uint32
         if (ph == PS_PHASE_FLUID) {
             const float li = p.w, ir0 = 1.0f / a.rho0;
             float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-            for (uint32_t q = 0; q < c; q++) {
-                const uint32_t j = b.nbr[(size_t)q * stride + i];
-                if (!UNIFORM && b.A[j].w == 0.0f) continue;
-                const float4 pj = b.P[j];
-                float dx = px - pj.x, dy = py - pj.y, dz = pz - pj.z;
-                float r2 = ((dx * dx) + (dy * dy)) + (dz * dz);
-                if (!(r2 < k.h2)) continue;
-                float t = k.h2 - r2;
-                float W = k.poly6 * ((t * t) * t);
-                float corr = -(k.k_corr * ipow(W * k.inv_w_dq, k.n_corr));
-                float coef = (li + pj.w) + corr;
-                float r = sqrtf(r2);
-                float qq = k.h - r;
-                float g = k.spiky * (qq * qq);
-                if (r < 1e-4f) {
-                    fy = fy - (coef * (g * 0.001f));
-                } else {
-                    float aa = g / r;
-                    fx = fx - (coef * (aa * dx));
-                    fy = fy - (coef * (aa * dy));
-                    fz = fz - (coef * (aa * dz));
+            const uint4 *nb = nbr_lane(b, i);
+            for (uint32_t q = 0; q < c; q += 4) {
+                const uint4 jj = nb[(size_t)(q >> 2) * 64];
+                const uint32_t jv[4] = {jj.x, jj.y, jj.z, jj.w};
+                float4 pv[4];
+                float wv[4];
+#pragma unroll
+                for (uint32_t u = 0; u < 4; u++)
+                    if (q + u < c) {
+                        pv[u] = b.P[jv[u]];
+                        if (!UNIFORM) wv[u] = b.A[jv[u]].w;
+                    }
+#pragma unroll
+                for (uint32_t u = 0; u < 4; u++) {
+                    if (q + u >= c) break;
+                    if (!UNIFORM && wv[u] == 0.0f) continue;
+                    const float4 pj = pv[u];
+                    float dx = px - pj.x, dy = py - pj.y, dz = pz - pj.z;
+                    float r2 = ((dx * dx) + (dy * dy)) + (dz * dz);
+                    if (!(r2 < k.h2)) continue;
+                    float t = k.h2 - r2;
+                    float W = k.poly6 * ((t * t) * t);
+                    float corr = -(k.k_corr * ipow(W * k.inv_w_dq, k.n_corr));
+                    float coef = (li + pj.w) + corr;
+                    float r = sqrtf(r2);
+                    float qq = k.h - r;
+                    float g = k.spiky * (qq * qq);
+                    if (r < 1e-4f) {
+                        fy = fy - (coef * (g * 0.001f));
+                    } else {
+                        float aa = g / r;
+                        fx = fx - (coef * (aa * dx));
+                        fy = fy - (coef * (aa * dy));
+                        fz = fz - (coef * (aa * dz));
+                    }
                 }
             }
             ax = ax + (fx * ir0);
@@ -768,27 +849,40 @@ __global__ __launch_bounds__(kBlock) void finalize_k(DevBuffers b, KConst k, uin
     if ((UNIFORM || a.phase == PS_PHASE_FLUID) && k.xsph_c != 0.0f) {
         float sx = 0.0f, sy = 0.0f, sz = 0.0f;
         const uint32_t c = b.ncnt[i];
-        const uint32_t stride = b.stride;
-        for (uint32_t q = 0; q < c; q++) {
-            const uint32_t j = b.nbr[(size_t)q * stride + i];
-            float mj = mu;
-            if (!UNIFORM) {
-                const Attr aj = b.A[j];
-                if (aj.phase != PS_PHASE_FLUID) continue;
-                mj = 1.0f / aj.w;
+        const uint4 *nb = nbr_lane(b, i);
+        for (uint32_t q = 0; q < c; q += 4) {
+            const uint4 jj = nb[(size_t)(q >> 2) * 64];
+            const uint32_t jv[4] = {jj.x, jj.y, jj.z, jj.w};
+            float4 pv[4], xv[4];
+            Attr av[4];
+#pragma unroll
+            for (uint32_t u = 0; u < 4; u++)
+                if (q + u < c) {
+                    pv[u] = b.P[jv[u]];
+                    xv[u] = b.X[jv[u]];
+                    if (!UNIFORM) av[u] = b.A[jv[u]];
+                }
+#pragma unroll
+            for (uint32_t u = 0; u < 4; u++) {
+                if (q + u >= c) break;
+                float mj = mu;
+                if (!UNIFORM) {
+                    if (av[u].phase != PS_PHASE_FLUID) continue;
+                    mj = 1.0f / av[u].w;
+                }
+                const float4 qj = pv[u];
+                float dx = p.x - qj.x, dy = p.y - qj.y, dz = p.z - qj.z;
+                float r2 = ((dx * dx) + (dy * dy)) + (dz * dz);
+                if (!(r2 < k.h2)) continue;
+                float t = k.h2 - r2;
+                float W = k.poly6 * ((t * t) * t);
+                float m = W * mj;
+                const float4 xj = xv[u];
+                float ex = qj.x - xj.x, ey = qj.y - xj.y, ez = qj.z - xj.z;
+                sx = sx + ((ex - ux) * m);
+                sy = sy + ((ey - uy) * m);
+                sz = sz + ((ez - uz) * m);
             }
-            const float4 qj = b.P[j];
-            float dx = p.x - qj.x, dy = p.y - qj.y, dz = p.z - qj.z;
-            float r2 = ((dx * dx) + (dy * dy)) + (dz * dz);
-            if (!(r2 < k.h2)) continue;
-            float t = k.h2 - r2;
-            float W = k.poly6 * ((t * t) * t);
-            float m = W * mj;
-            const float4 xj = b.X[j];
-            float ex = qj.x - xj.x, ey = qj.y - xj.y, ez = qj.z - xj.z;
-            sx = sx + ((ex - ux) * m);
-            sy = sy + ((ey - uy) * m);
-            sz = sz + ((ez - uz) * m);
         }
         const float cr = k.xsph_c / a.rho0;
         vx = vx + ((cr * sx) / dt);

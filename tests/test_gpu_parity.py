@@ -78,6 +78,34 @@ def test_grid_and_neighbors_bitexact(gpu_available, seed, n):
         assert g.status().flags == ref.status().flags
 
 
+@pytest.mark.parametrize("kmax", [4, 8, 12])
+def test_neighbor_overflow_and_partial_chunks(gpu_available, kmax):
+    n = 3000
+    box = (2.0, 2.0, 2.0)
+    p = grid_params(n, box, kmax=kmax)
+    pos = random_positions(n, box, 11, edge_frac=0.1)
+    args = (pos, None, np.ones(n, np.float32), np.zeros(n, np.int32), np.full(n, 100.0, np.float32))
+    ref = Oracle(p, "f32")
+    ref.add_particles(*args)
+    with System(p) as g:
+        g.add_particles(*args)
+        for sysx in (g, ref):
+            sysx.predict(0.0)
+            sysx.build_grid()
+            sysx.find_neighbors()
+        gn, gt = g.neighbors()
+        rn, rt = ref.neighbors()
+        assert_bitexact(gn, rn, "neighbour counts")
+        assert_bitexact(gt, rt, "neighbour table")
+        gs, rs = g.status(), ref.status()
+        assert gs.max_neighbors == rs.max_neighbors > kmax
+        assert gs.flags == rs.flags and (gs.flags & 1)
+        # the truncated tables still drive identical iterations
+        for sysx in (g, ref):
+            sysx.iterate()
+        assert_bitexact(g.sorted_state()[0], ref.sorted_state()[0], "iterate after overflow")
+
+
 def test_empty_system(gpu_available):
     p = grid_params(16)
     with System(p) as g:
