@@ -269,7 +269,11 @@ typedef enum ps_kernel_id {
     PS_K_COUNT
 } ps_kernel_id;
 
-int ps_timing_enable(ps_system *sys, int enable);
+/* kernel_mask: bit k brackets every launch of kernel k with two events
+ * (PS_TIMING_ALL for every kernel, 0 to stop).  Each event pair costs a few
+ * microseconds of stream time, so time only the kernels you report. */
+#define PS_TIMING_ALL 0xffffffffu
+int ps_timing_enable(ps_system *sys, uint32_t kernel_mask);
 int ps_timing_reset(ps_system *sys);
 /* Total milliseconds and launch count for a kernel since the last reset. */
 int ps_timing_read(ps_system *sys, int kernel_id, double *total_ms, uint64_t *launches);

@@ -408,6 +408,7 @@ ORC_EXPORT int SFX(orc_predict)(orc_sys *s, real dt)
     const real gx = R(s->prm.gravity[0]), gy = R(s->prm.gravity[1]), gz = R(s->prm.gravity[2]);
     const int twod = s->prm.dims == 2;
     s->dt = dt;
+    s->stage = 1; /* the substep is open from the prediction on (as ps_predict) */
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < (int64_t)s->n; i++) {
         real x = s->X[3 * i], y = s->X[3 * i + 1], z = s->X[3 * i + 2];

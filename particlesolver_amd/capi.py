@@ -138,7 +138,7 @@ PROTOTYPES = {
     "ps_read_sorted_state": (_int, [_P, _P, _P]),
     "ps_get_status": (_int, [_P, ctypes.POINTER(PsStatus)]),
     "ps_clear_status": (_int, [_P]),
-    "ps_timing_enable": (_int, [_P, _int]),
+    "ps_timing_enable": (_int, [_P, _u32]),
     "ps_timing_reset": (_int, [_P]),
     "ps_timing_read": (_int, [_P, _int, ctypes.POINTER(ctypes.c_double),
                               ctypes.POINTER(ctypes.c_uint64)]),
@@ -342,8 +342,17 @@ class System:
         self._check(self.lib.ps_clear_status(self.h))
 
     # -- timing
-    def timing(self, enable=True):
-        self._check(self.lib.ps_timing_enable(self.h, 1 if enable else 0))
+    def timing(self, kernels=True):
+        """True: every kernel; False: none; or an iterable of kernel names."""
+        if kernels is True:
+            mask = 0xFFFFFFFF
+        elif not kernels:
+            mask = 0
+        else:
+            mask = 0
+            for name in kernels:
+                mask |= 1 << KERNELS.index(name)
+        self._check(self.lib.ps_timing_enable(self.h, mask))
 
     def timing_reset(self):
         self._check(self.lib.ps_timing_reset(self.h))

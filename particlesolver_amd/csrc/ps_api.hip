@@ -55,7 +55,7 @@ struct ps_system {
     float dt = 0.0f;
     uint64_t substeps = 0, iters = 0;
     // timing
-    bool timing = false;
+    uint32_t timing = 0;  // kernel mask
     std::vector<TimedLaunch> pending;
     std::vector<hipEvent_t> free_events;
     double total_ms[PS_K_COUNT] = {};
@@ -260,7 +260,7 @@ int timed_begin(ps_system *s, int id, TimedLaunch &tl)
 {
     tl.id = id;
     tl.start = tl.stop = nullptr;
-    if (!s->timing) return PS_OK;
+    if (!(s->timing & (1u << id))) return PS_OK;
     for (hipEvent_t *e : {&tl.start, &tl.stop}) {
         if (!s->free_events.empty()) {
             *e = s->free_events.back();
@@ -275,7 +275,7 @@ int timed_begin(ps_system *s, int id, TimedLaunch &tl)
 
 int timed_end(ps_system *s, TimedLaunch &tl)
 {
-    if (!s->timing) return PS_OK;
+    if (!tl.start) return PS_OK;
     HIP_TRY(s, hipEventRecord(tl.stop, s->stream));
     s->pending.push_back(tl);
     return PS_OK;
@@ -1086,11 +1086,11 @@ int ps_clear_status(ps_system *s)
 
 // ---- timing ---------------------------------------------------------------
 
-int ps_timing_enable(ps_system *s, int enable)
+int ps_timing_enable(ps_system *s, uint32_t kernel_mask)
 {
     if (int rc = check_sys(s)) return rc;
     if (int rc = timing_collect(s)) return rc;
-    s->timing = enable != 0;
+    s->timing = kernel_mask;
     return PS_OK;
 }
 

@@ -69,6 +69,50 @@ __device__ __forceinline__ const uint4 *nbr_lane(const DevBuffers &b, uint32_t i
     return reinterpret_cast<const uint4 *>(b.nbr) + (size_t)(i >> 6) * b.kq * 64 + (i & 63);
 }
 
+// Visit the first c neighbours of particle i in table order, calling
+// fn(j, gather(j)) for each.  Software-pipelined: while chunk m is being
+// processed, the four gathers of chunk m+1 and the index load of chunk m+2
+// are already in flight.  Every entry of a chunk at or below the last one is
+// a valid slot (the builder pads the last chunk with the particle's own
+// slot), so all loads are unconditional; only fn is guarded.
+template <typename G, typename Fn>
+__device__ __forceinline__ void for_neighbors(const DevBuffers &b, uint32_t i, uint32_t c, G gather,
+                                              Fn fn)
+{
+    if (c == 0) return;
+    const uint4 *nb = nbr_lane(b, i);
+    const uint32_t last = (c - 1) >> 2;
+    uint4 jj = nb[0];
+    uint4 jn = nb[(size_t)min(1u, last) * 64];
+    auto d0 = gather(jj.x);
+    auto d1 = gather(jj.y);
+    auto d2 = gather(jj.z);
+    auto d3 = gather(jj.w);
+    for (uint32_t m = 0; m <= last; m++) {
+        const uint4 jnn = nb[(size_t)min(m + 2, last) * 64];
+        auto e0 = gather(jn.x);
+        auto e1 = gather(jn.y);
+        auto e2 = gather(jn.z);
+        auto e3 = gather(jn.w);
+        const uint32_t q = m << 2;
+        fn(jj.x, d0);
+        if (q + 1 < c) fn(jj.y, d1);
+        if (q + 2 < c) fn(jj.z, d2);
+        if (q + 3 < c) fn(jj.w, d3);
+        jj = jn;
+        jn = jnn;
+        d0 = e0;
+        d1 = e1;
+        d2 = e2;
+        d3 = e3;
+    }
+}
+
+struct PA {
+    float4 p;
+    Attr a;
+};
+
 // entry q of particle i (single-entry access for the less travelled loops)
 __device__ __forceinline__ uint32_t nbr_at(const DevBuffers &b, uint32_t i, uint32_t q)
 {
@@ -319,8 +363,7 @@ __global__ __launch_bounds__(kBlock) void neighbors_k(DevBuffers b, KConst k, ui
                 for (uint32_t jb = j0; jb < j1; jb += 4) {
                     float4 qv[4];
 #pragma unroll
-                    for (uint32_t u = 0; u < 4; u++)
-                        if (jb + u < j1) qv[u] = b.P[jb + u];
+                    for (uint32_t u = 0; u < 4; u++) qv[u] = b.P[min(jb + u, j1 - 1)];
 #pragma unroll
                     for (uint32_t u = 0; u < 4; u++) {
                         const uint32_t j = jb + u;
@@ -385,47 +428,36 @@ __global__ __launch_bounds__(kBlock) void lambda_k(DevBuffers b, KConst k, uint3
     float rho = k.w_zero * (UNIFORM ? mu : (1.0f / a.w));
     float gix = 0.0f, giy = 0.0f, giz = 0.0f, den = 0.0f;
     const uint32_t c = b.ncnt[i];
-    const uint4 *nb = nbr_lane(b, i);
-    for (uint32_t q = 0; q < c; q += 4) {
-        const uint4 jj = nb[(size_t)(q >> 2) * 64];
-        const uint32_t jv[4] = {jj.x, jj.y, jj.z, jj.w};
-        float4 pv[4];
-        Attr av[4];
-#pragma unroll
-        for (uint32_t u = 0; u < 4; u++)
-            if (q + u < c) {
-                pv[u] = b.P[jv[u]];
-                if (!UNIFORM) av[u] = b.A[jv[u]];
-            }
-#pragma unroll
-        for (uint32_t u = 0; u < 4; u++) {
-            if (q + u >= c) break;
-            float wj = 1.0f;
-            int32_t phj = PS_PHASE_FLUID;
-            if (!UNIFORM) {
-                wj = av[u].w;
-                phj = av[u].phase;
-                if (wj == 0.0f) continue;
-            }
-            const float4 pj = pv[u];
-            float dx = p.x - pj.x, dy = p.y - pj.y, dz = p.z - pj.z;
-            float r2 = ((dx * dx) + (dy * dy)) + (dz * dz);
-            if (!(r2 < k.h2)) continue;
-            float t = k.h2 - r2;
-            float W = k.poly6 * ((t * t) * t);
-            float cc = W * (UNIFORM ? mu : (1.0f / wj));
-            if (phj != PS_PHASE_FLUID) cc = cc * k.s_solid;
-            rho = rho + cc;
-            float r = sqrtf(r2);
-            if (r < 1e-4f) continue;
-            float qq = k.h - r;
-            float aa = ((k.spiky * (qq * qq)) / r) * ir0;
-            float gx = aa * dx, gy = aa * dy, gz = aa * dz;
-            den = den + (((gx * gx) + (gy * gy)) + (gz * gz));
-            gix = gix + gx;
-            giy = giy + gy;
-            giz = giz + gz;
-        }
+    auto body = [&](uint32_t, const float4 &pj, float wj, int32_t phj) {
+        float dx = p.x - pj.x, dy = p.y - pj.y, dz = p.z - pj.z;
+        float r2 = ((dx * dx) + (dy * dy)) + (dz * dz);
+        if (!(r2 < k.h2)) return;
+        float t = k.h2 - r2;
+        float W = k.poly6 * ((t * t) * t);
+        float cc = W * (UNIFORM ? mu : (1.0f / wj));
+        if (phj != PS_PHASE_FLUID) cc = cc * k.s_solid;
+        rho = rho + cc;
+        float r = sqrtf(r2);
+        if (r < 1e-4f) return;
+        float qq = k.h - r;
+        float aa = ((k.spiky * (qq * qq)) / r) * ir0;
+        float gx = aa * dx, gy = aa * dy, gz = aa * dz;
+        den = den + (((gx * gx) + (gy * gy)) + (gz * gz));
+        gix = gix + gx;
+        giy = giy + gy;
+        giz = giz + gz;
+    };
+    if (UNIFORM) {
+        for_neighbors(
+            b, i, c, [&](uint32_t j) { return b.P[j]; },
+            [&](uint32_t j, const float4 &pj) { body(j, pj, 1.0f, PS_PHASE_FLUID); });
+    } else {
+        for_neighbors(
+            b, i, c, [&](uint32_t j) { return PA{b.P[j], b.A[j]}; },
+            [&](uint32_t j, const PA &d) {
+                if (d.a.w == 0.0f) return;
+                body(j, d.p, d.a.w, d.a.phase);
+            });
     }
     den = den + (((gix * gix) + (giy * giy)) + (giz * giz));
     float C = (rho / r0) - 1.0f;
@@ -544,7 +576,7 @@ __device__ __forceinline__ void walls(const KConst &k, int32_t ph, uint32_t o, f
 // (a6), distance (a9), shape goal (a12) summed in that order from the same
 // snapshot P, then walls (a8) and pins (a10); result to Q.
 
-template <uint32_t F, bool UNIFORM>
+template <uint32_t F, bool UNIFORM, uint32_t NC>
 __global__ __launch_bounds__(kBlock) void solve_k(DevBuffers b, KConst k, uint32_t n, float wu,
                                                   uint64_t iter)
 {
@@ -640,42 +672,37 @@ __global__ __launch_bounds__(kBlock) void solve_k(DevBuffers b, KConst k, uint32
         if (ph == PS_PHASE_FLUID) {
             const float li = p.w, ir0 = 1.0f / a.rho0;
             float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-            const uint4 *nb = nbr_lane(b, i);
-            for (uint32_t q = 0; q < c; q += 4) {
-                const uint4 jj = nb[(size_t)(q >> 2) * 64];
-                const uint32_t jv[4] = {jj.x, jj.y, jj.z, jj.w};
-                float4 pv[4];
-                float wv[4];
-#pragma unroll
-                for (uint32_t u = 0; u < 4; u++)
-                    if (q + u < c) {
-                        pv[u] = b.P[jv[u]];
-                        if (!UNIFORM) wv[u] = b.A[jv[u]].w;
-                    }
-#pragma unroll
-                for (uint32_t u = 0; u < 4; u++) {
-                    if (q + u >= c) break;
-                    if (!UNIFORM && wv[u] == 0.0f) continue;
-                    const float4 pj = pv[u];
-                    float dx = px - pj.x, dy = py - pj.y, dz = pz - pj.z;
-                    float r2 = ((dx * dx) + (dy * dy)) + (dz * dz);
-                    if (!(r2 < k.h2)) continue;
-                    float t = k.h2 - r2;
-                    float W = k.poly6 * ((t * t) * t);
-                    float corr = -(k.k_corr * ipow(W * k.inv_w_dq, k.n_corr));
-                    float coef = (li + pj.w) + corr;
-                    float r = sqrtf(r2);
-                    float qq = k.h - r;
-                    float g = k.spiky * (qq * qq);
-                    if (r < 1e-4f) {
-                        fy = fy - (coef * (g * 0.001f));
-                    } else {
-                        float aa = g / r;
-                        fx = fx - (coef * (aa * dx));
-                        fy = fy - (coef * (aa * dy));
-                        fz = fz - (coef * (aa * dz));
-                    }
-                }
+            auto body = [&](const float4 &pj) {
+                float dx = px - pj.x, dy = py - pj.y, dz = pz - pj.z;
+                float r2 = ((dx * dx) + (dy * dy)) + (dz * dz);
+                if (!(r2 < k.h2)) return;
+                float t = k.h2 - r2;
+                float W = k.poly6 * ((t * t) * t);
+                float corr = -(k.k_corr * ipow(W * k.inv_w_dq, NC ? NC : k.n_corr));
+                float coef = (li + pj.w) + corr;
+                float r = sqrtf(r2);
+                float qq = k.h - r;
+                float g = k.spiky * (qq * qq);
+                // r < 1e-4: gradient (0, -0.001 g, 0) (integration_kernel.cuh:625-626);
+                // selects rather than branches keep fx/fy/fz in registers
+                const bool deg = r < 1e-4f;
+                float aa = g / (deg ? 1.0f : r);
+                float tx = coef * (aa * dx), ty = coef * (aa * dy), tz = coef * (aa * dz);
+                fx = deg ? fx : (fx - tx);
+                fy = deg ? (fy - (coef * (g * 0.001f))) : (fy - ty);
+                fz = deg ? fz : (fz - tz);
+            };
+            if (UNIFORM) {
+                for_neighbors(
+                    b, i, c, [&](uint32_t j) { return b.P[j]; },
+                    [&](uint32_t, const float4 &pj) { body(pj); });
+            } else {
+                for_neighbors(
+                    b, i, c, [&](uint32_t j) { return PA{b.P[j], b.A[j]}; },
+                    [&](uint32_t, const PA &d) {
+                        if (d.a.w == 0.0f) return;
+                        body(d.p);
+                    });
             }
             ax = ax + (fx * ir0);
             ay = ay + (fy * ir0);
@@ -849,40 +876,36 @@ __global__ __launch_bounds__(kBlock) void finalize_k(DevBuffers b, KConst k, uin
     if ((UNIFORM || a.phase == PS_PHASE_FLUID) && k.xsph_c != 0.0f) {
         float sx = 0.0f, sy = 0.0f, sz = 0.0f;
         const uint32_t c = b.ncnt[i];
-        const uint4 *nb = nbr_lane(b, i);
-        for (uint32_t q = 0; q < c; q += 4) {
-            const uint4 jj = nb[(size_t)(q >> 2) * 64];
-            const uint32_t jv[4] = {jj.x, jj.y, jj.z, jj.w};
-            float4 pv[4], xv[4];
-            Attr av[4];
-#pragma unroll
-            for (uint32_t u = 0; u < 4; u++)
-                if (q + u < c) {
-                    pv[u] = b.P[jv[u]];
-                    xv[u] = b.X[jv[u]];
-                    if (!UNIFORM) av[u] = b.A[jv[u]];
-                }
-#pragma unroll
-            for (uint32_t u = 0; u < 4; u++) {
-                if (q + u >= c) break;
-                float mj = mu;
-                if (!UNIFORM) {
-                    if (av[u].phase != PS_PHASE_FLUID) continue;
-                    mj = 1.0f / av[u].w;
-                }
-                const float4 qj = pv[u];
-                float dx = p.x - qj.x, dy = p.y - qj.y, dz = p.z - qj.z;
-                float r2 = ((dx * dx) + (dy * dy)) + (dz * dz);
-                if (!(r2 < k.h2)) continue;
-                float t = k.h2 - r2;
-                float W = k.poly6 * ((t * t) * t);
-                float m = W * mj;
-                const float4 xj = xv[u];
-                float ex = qj.x - xj.x, ey = qj.y - xj.y, ez = qj.z - xj.z;
-                sx = sx + ((ex - ux) * m);
-                sy = sy + ((ey - uy) * m);
-                sz = sz + ((ez - uz) * m);
-            }
+        auto body = [&](const float4 &qj, const float4 &xj, float mj) {
+            float dx = p.x - qj.x, dy = p.y - qj.y, dz = p.z - qj.z;
+            float r2 = ((dx * dx) + (dy * dy)) + (dz * dz);
+            if (!(r2 < k.h2)) return;
+            float t = k.h2 - r2;
+            float W = k.poly6 * ((t * t) * t);
+            float m = W * mj;
+            float ex = qj.x - xj.x, ey = qj.y - xj.y, ez = qj.z - xj.z;
+            sx = sx + ((ex - ux) * m);
+            sy = sy + ((ey - uy) * m);
+            sz = sz + ((ez - uz) * m);
+        };
+        struct PX {
+            float4 p, x;
+        };
+        struct PXA {
+            float4 p, x;
+            Attr a;
+        };
+        if (UNIFORM) {
+            for_neighbors(
+                b, i, c, [&](uint32_t j) { return PX{b.P[j], b.X[j]}; },
+                [&](uint32_t, const PX &d) { body(d.p, d.x, mu); });
+        } else {
+            for_neighbors(
+                b, i, c, [&](uint32_t j) { return PXA{b.P[j], b.X[j], b.A[j]}; },
+                [&](uint32_t, const PXA &d) {
+                    if (d.a.phase != PS_PHASE_FLUID) return;
+                    body(d.p, d.x, 1.0f / d.a.w);
+                });
         }
         const float cr = k.xsph_c / a.rho0;
         vx = vx + ((cr * sx) / dt);
@@ -983,15 +1006,22 @@ hipError_t launch_shape(hipStream_t st, const DevBuffers &b, uint32_t nbody)
     return hipGetLastError();
 }
 
+// NC: tensile exponent known at compile time (4, as in the reference's
+// E_P, integration_kernel.cuh:34) or 0 = read from the parameters
 template <uint32_t F>
 static void solve_variant(hipStream_t st, const DevBuffers &b, const KConst &k, uint32_t n,
                           bool uniform, float wu, uint64_t iter)
 {
     const dim3 g(blocks_for(n)), t(kBlock);
-    if (uniform)
-        hipLaunchKernelGGL((solve_k<F, true>), g, t, 0, st, b, k, n, wu, iter);
+    const bool n4 = k.n_corr == 4u;
+    if (uniform && n4)
+        hipLaunchKernelGGL((solve_k<F, true, 4>), g, t, 0, st, b, k, n, wu, iter);
+    else if (uniform)
+        hipLaunchKernelGGL((solve_k<F, true, 0>), g, t, 0, st, b, k, n, wu, iter);
+    else if (n4)
+        hipLaunchKernelGGL((solve_k<F, false, 4>), g, t, 0, st, b, k, n, wu, iter);
     else
-        hipLaunchKernelGGL((solve_k<F, false>), g, t, 0, st, b, k, n, wu, iter);
+        hipLaunchKernelGGL((solve_k<F, false, 0>), g, t, 0, st, b, k, n, wu, iter);
 }
 
 hipError_t launch_solve(hipStream_t st, const DevBuffers &b, const KConst &k, uint32_t n,
