@@ -196,6 +196,11 @@ int ps_add_point_constraints(ps_system *sys, uint32_t m, const uint32_t *idx,
  * stiffness <= 0 uses params.shape_stiffness. */
 int ps_add_rigid_body(ps_system *sys, uint32_t n, const uint32_t *idx, float stiffness,
                       uint32_t *body_out);
+/* The same for nbody clusters at once: members of body b are
+ * idx[offsets[b] .. offsets[b+1]); stiffness: nbody values or NULL (default).
+ * *first_body_out (may be NULL) receives the id of the first new body. */
+int ps_add_rigid_bodies(ps_system *sys, uint32_t nbody, const uint32_t *offsets,
+                        const uint32_t *idx, const float *stiffness, uint32_t *first_body_out);
 
 /* Overwrite velocities of particles [first, first+n) (n*3 floats). */
 int ps_set_velocities(ps_system *sys, uint32_t first, uint32_t n, const float *vel);

@@ -102,6 +102,16 @@ class Oracle:
                                                   ctypes.c_float(stiffness), ctypes.byref(b)))
         return b.value
 
+    def add_rigid_bodies(self, offsets, idx, stiffness=None):
+        offsets = np.asarray(offsets).reshape(-1)
+        idx = np.asarray(idx).reshape(-1)
+        first = None
+        for b in range(len(offsets) - 1):
+            st = 0.0 if stiffness is None else float(stiffness[b])
+            bid = self.add_rigid_body(idx[offsets[b]:offsets[b + 1]], st)
+            first = bid if first is None else first
+        return first
+
     def set_params(self, params):
         self.params = params.copy()
         self._check(self._f("orc_set_params")(self.h, ctypes.byref(self.params)))

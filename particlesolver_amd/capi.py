@@ -121,6 +121,7 @@ PROTOTYPES = {
     "ps_add_distance_constraints": (_int, [_P, _u32, _P, _P, _P]),
     "ps_add_point_constraints": (_int, [_P, _u32, _P, _P]),
     "ps_add_rigid_body": (_int, [_P, _u32, _P, _f32, ctypes.POINTER(_u32)]),
+    "ps_add_rigid_bodies": (_int, [_P, _u32, _P, _P, _P, ctypes.POINTER(_u32)]),
     "ps_set_velocities": (_int, [_P, _u32, _u32, _P]),
     "ps_step": (_int, [_P, _f32, _u32]),
     "ps_substep_begin": (_int, [_P, _f32]),
@@ -260,6 +261,15 @@ class System:
         b = ctypes.c_uint32()
         self._check(self.lib.ps_add_rigid_body(self.h, idx.shape[0], _ptr(idx), stiffness,
                                                ctypes.byref(b)))
+        return b.value
+
+    def add_rigid_bodies(self, offsets, idx, stiffness=None):
+        offsets = np.ascontiguousarray(offsets, dtype=np.uint32).reshape(-1)
+        idx = np.ascontiguousarray(idx, dtype=np.uint32).reshape(-1)
+        st = None if stiffness is None else np.ascontiguousarray(stiffness, dtype=np.float32)
+        b = ctypes.c_uint32()
+        self._check(self.lib.ps_add_rigid_bodies(self.h, offsets.shape[0] - 1, _ptr(offsets),
+                                                 _ptr(idx), _ptr(st), ctypes.byref(b)))
         return b.value
 
     def set_velocities(self, vel, first=0):

@@ -759,67 +759,79 @@ int ps_add_point_constraints(ps_system *s, uint32_t m, const uint32_t *idx, cons
     return upload_pins(s);
 }
 
-int ps_add_rigid_body(ps_system *s, uint32_t n, const uint32_t *idx, float stiffness,
-                      uint32_t *body_out)
+int ps_add_rigid_bodies(ps_system *s, uint32_t nbody, const uint32_t *offsets,
+                        const uint32_t *idx, const float *stiffness, uint32_t *first_body_out)
 {
     if (int rc = check_sys(s)) return rc;
     if (s->stage) return fail(s, PS_ERR_STATE, "cannot add bodies inside a substep");
-    if (n < 2 || !idx)
-        // cpu/src/simulation.cpp:373-376: rigid bodies need at least 2 points
-        return fail(s, PS_ERR_INVALID_ARGUMENT, "a rigid body needs at least 2 particles");
-    if (stiffness > 1.0f || !is_finite(stiffness))
-        return fail(s, PS_ERR_INVALID_ARGUMENT, "stiffness must be <= 1");
+    if (nbody == 0) {
+        if (first_body_out) *first_body_out = (uint32_t)s->body_off.size() - 1;
+        return PS_OK;
+    }
+    if (!offsets || !idx) return fail(s, PS_ERR_INVALID_ARGUMENT, "NULL input array");
+    if (offsets[0] != 0) return fail(s, PS_ERR_INVALID_ARGUMENT, "offsets[0] must be 0");
+    for (uint32_t bb = 0; bb < nbody; bb++) {
+        if (offsets[bb + 1] < offsets[bb] + 2)
+            // cpu/src/simulation.cpp:373-376: rigid bodies need at least 2 points
+            return fail(s, PS_ERR_INVALID_ARGUMENT, "a rigid body needs at least 2 particles");
+        if (stiffness && (!(stiffness[bb] <= 1.0f) || !is_finite(stiffness[bb])))
+            return fail(s, PS_ERR_INVALID_ARGUMENT, "stiffness must be <= 1");
+    }
+    const uint32_t total = offsets[nbody];
     std::vector<float4> X;
     std::vector<Attr> A;
     if (int rc = read_sorted_positions(s, X, A)) return rc;
     std::vector<uint32_t> slot_of(s->n);
     for (uint32_t t = 0; t < s->n; t++) slot_of[A[t].orig] = t;
-    for (uint32_t k = 0; k < n; k++) {
-        if (idx[k] >= s->n || s->body_of[idx[k]])
-            return fail(s, PS_ERR_INVALID_ARGUMENT, "body member out of range or already in a body");
+    std::vector<uint8_t> seen(s->n, 0);
+    for (uint32_t k = 0; k < total; k++) {
+        if (idx[k] >= s->n || s->body_of[idx[k]] || seen[idx[k]])
+            return fail(s, PS_ERR_INVALID_ARGUMENT,
+                        "body member out of range, repeated, or already in a body");
+        seen[idx[k]] = 1;
         if (A[slot_of[idx[k]]].w == 0.0f)
             // cpu/src/simulation.cpp:387-390
             return fail(s, PS_ERR_INVALID_ARGUMENT, "a rigid body cannot have a static particle");
     }
-    for (uint32_t k = 0; k < n; k++)
-        for (uint32_t q = k + 1; q < n; q++)
-            if (idx[k] == idx[q]) return fail(s, PS_ERR_INVALID_ARGUMENT, "duplicate body member");
-    const uint32_t bnew = (uint32_t)s->body_off.size() - 1;
-    const uint32_t base = s->body_off.back();
-    // centre of mass and rest offsets (same float operations as the kernels)
-    float cx = 0.0f, cy = 0.0f, cz = 0.0f, M = 0.0f;
-    for (uint32_t k = 0; k < n; k++) {
-        const uint32_t t = slot_of[idx[k]];
-        const float m = 1.0f / A[t].w;
-        cx = cx + (m * X[t].x);
-        cy = cy + (m * X[t].y);
-        cz = cz + (m * X[t].z);
-        M = M + m;
+    const uint32_t bfirst = (uint32_t)s->body_off.size() - 1;
+    for (uint32_t bb = 0; bb < nbody; bb++) {
+        const uint32_t base = s->body_off.back();
+        const uint32_t k0 = offsets[bb], k1 = offsets[bb + 1];
+        // centre of mass and rest offsets, same float operations as the kernels
+        float cx = 0.0f, cy = 0.0f, cz = 0.0f, M = 0.0f;
+        for (uint32_t k = k0; k < k1; k++) {
+            const uint32_t t = slot_of[idx[k]];
+            const float m = 1.0f / A[t].w;
+            cx = cx + (m * X[t].x);
+            cy = cy + (m * X[t].y);
+            cz = cz + (m * X[t].z);
+            M = M + m;
+        }
+        cx = cx / M;
+        cy = cy / M;
+        cz = cz / M;
+        for (uint32_t k = k0; k < k1; k++) {
+            const uint32_t t = slot_of[idx[k]];
+            s->body_idx.push_back(idx[k]);
+            s->body_rest.push_back(X[t].x - cx);
+            s->body_rest.push_back(X[t].y - cy);
+            s->body_rest.push_back(X[t].z - cz);
+            s->body_of[idx[k]] = bfirst + bb + 1;
+            s->member_of[idx[k]] = base + (k - k0);
+        }
+        s->body_off.push_back(base + (k1 - k0));
+        const float st = stiffness ? stiffness[bb] : 0.0f;
+        s->body_stiff.push_back(st > 0.0f ? st : s->prm.shape_stiffness);
+        s->body_mass.push_back(M);
     }
-    cx = cx / M;
-    cy = cy / M;
-    cz = cz / M;
-    for (uint32_t k = 0; k < n; k++) {
-        const uint32_t t = slot_of[idx[k]];
-        s->body_idx.push_back(idx[k]);
-        s->body_rest.push_back(X[t].x - cx);
-        s->body_rest.push_back(X[t].y - cy);
-        s->body_rest.push_back(X[t].z - cz);
-        s->body_of[idx[k]] = bnew + 1;
-        s->member_of[idx[k]] = base + k;
-    }
-    s->body_off.push_back(base + n);
-    s->body_stiff.push_back(stiffness > 0.0f ? stiffness : s->prm.shape_stiffness);
-    s->body_mass.push_back(M);
-    // upload everything (bodies are added at scene-build time)
+    // upload every body table (bodies are added at scene-build time)
     DevBuffers &b = s->b;
     const size_t nb = s->body_stiff.size(), nm = s->body_idx.size();
+    std::vector<float4> q(nb, make_float4(1.f, 0.f, 0.f, 0.f));
+    if (bfirst > 0)
+        HIP_TRY(s, hipMemcpy(q.data(), b.body_q, sizeof(float4) * bfirst, hipMemcpyDeviceToHost));
     if (nb > s->body_cap || nm > s->member_cap) {
-        // keep the quaternions of existing bodies
-        std::vector<float4> q(nb, make_float4(1.f, 0.f, 0.f, 0.f));
-        if (s->body_cap && nb > 1)
-            HIP_TRY(s, hipMemcpy(q.data(), b.body_q, sizeof(float4) * (nb - 1), hipMemcpyDeviceToHost));
-        size_t bc = nb * 2, mc = nm * 2;
+        const size_t bc = nb * 2, mc = nm * 2;
         dev_free(b.body_off); dev_free(b.body_idx); dev_free(b.body_rest); dev_free(b.body_stiff);
         dev_free(b.body_mass); dev_free(b.body_q); dev_free(b.body_goal);
         int rc;
@@ -830,16 +842,13 @@ int ps_add_rigid_body(ps_system *s, uint32_t n, const uint32_t *idx, float stiff
         if ((rc = dev_alloc(s, &b.body_mass, bc))) return rc;
         if ((rc = dev_alloc(s, &b.body_q, bc))) return rc;
         if ((rc = dev_alloc(s, &b.body_goal, mc))) return rc;
-        HIP_TRY(s, hipMemcpy(b.body_q, q.data(), sizeof(float4) * nb, hipMemcpyHostToDevice));
         s->body_cap = bc;
         s->member_cap = mc;
-    } else {
-        float4 q1 = make_float4(1.f, 0.f, 0.f, 0.f);
-        HIP_TRY(s, hipMemcpy(b.body_q + bnew, &q1, sizeof(float4), hipMemcpyHostToDevice));
     }
     std::vector<float4> rest(nm);
     for (size_t k = 0; k < nm; k++)
         rest[k] = make_float4(s->body_rest[3 * k], s->body_rest[3 * k + 1], s->body_rest[3 * k + 2], 0.f);
+    HIP_TRY(s, hipMemcpy(b.body_q, q.data(), sizeof(float4) * nb, hipMemcpyHostToDevice));
     HIP_TRY(s, hipMemcpy(b.body_off, s->body_off.data(), sizeof(uint32_t) * s->body_off.size(), hipMemcpyHostToDevice));
     HIP_TRY(s, hipMemcpy(b.body_idx, s->body_idx.data(), sizeof(uint32_t) * nm, hipMemcpyHostToDevice));
     HIP_TRY(s, hipMemcpy(b.body_rest, rest.data(), sizeof(float4) * nm, hipMemcpyHostToDevice));
@@ -847,8 +856,15 @@ int ps_add_rigid_body(ps_system *s, uint32_t n, const uint32_t *idx, float stiff
     HIP_TRY(s, hipMemcpy(b.body_mass, s->body_mass.data(), sizeof(float) * nb, hipMemcpyHostToDevice));
     HIP_TRY(s, hipMemcpy(b.body_of, s->body_of.data(), sizeof(uint32_t) * s->cap, hipMemcpyHostToDevice));
     HIP_TRY(s, hipMemcpy(b.member_of, s->member_of.data(), sizeof(uint32_t) * s->cap, hipMemcpyHostToDevice));
-    if (body_out) *body_out = bnew;
+    if (first_body_out) *first_body_out = bfirst;
     return PS_OK;
+}
+
+int ps_add_rigid_body(ps_system *s, uint32_t n, const uint32_t *idx, float stiffness,
+                      uint32_t *body_out)
+{
+    const uint32_t off[2] = {0, n};
+    return ps_add_rigid_bodies(s, 1, off, idx, &stiffness, body_out);
 }
 
 int ps_set_velocities(ps_system *s, uint32_t first, uint32_t n, const float *vel)

@@ -12,6 +12,8 @@ import numpy as np
 
 from . import capi
 
+PHASE_RIGID0 = capi.PHASE_RIGID
+
 GOLDEN = np.uint64(0x9E3779B97F4A7C15)
 M1 = np.uint64(0xBF58476D1CE4E5B9)
 M2 = np.uint64(0x94D049BB133111EB)
@@ -133,8 +135,12 @@ class Scene:
             sys_.add_distance_constraints(*self.distance)
         if self.pins is not None:
             sys_.add_point_constraints(*self.pins)
-        for idx, st in self.bodies:
-            sys_.add_rigid_body(idx, st)
+        if self.bodies:
+            off = np.zeros(len(self.bodies) + 1, np.uint32)
+            off[1:] = np.cumsum([len(i) for i, _ in self.bodies])
+            idx = np.concatenate([i for i, _ in self.bodies])
+            st = np.array([s for _, s in self.bodies], np.float32)
+            sys_.add_rigid_bodies(off, idx, st)
 
 
 def dam_break(nx=128, ny=64, nz=128, spacing=0.1, box_scale=(2.0, 2.0, 2.0), jitter=1e-3,
@@ -155,3 +161,106 @@ def dam_break(nx=128, ny=64, nz=128, spacing=0.1, box_scale=(2.0, 2.0, 2.0), jit
               np.full(n, np.float32(rho0), np.float32), dt, name="dam_break_%dx%dx%d" % (nx, ny, nz))
     s.rho0 = rho0
     return s
+
+
+def cloth_constraints(nx, nz, first, spacing, stiffness=(1.0, 0.5, 0.1)):
+    """Structural (x and z neighbours), shear (both diagonals) and bend
+    (two apart) distance constraints of an nx*nz cloth, rest length = the
+    initial distance, with the given per-family stiffness."""
+    idx = np.arange(nx * nz, dtype=np.int64).reshape(nz, nx) + first
+    fams = []
+    a, b = idx[:, :-1].ravel(), idx[:, 1:].ravel()
+    c, d = idx[:-1, :].ravel(), idx[1:, :].ravel()
+    fams.append((np.concatenate([a, c]), np.concatenate([b, d]), spacing, stiffness[0]))
+    e, f = idx[:-1, :-1].ravel(), idx[1:, 1:].ravel()
+    g, hh = idx[1:, :-1].ravel(), idx[:-1, 1:].ravel()
+    fams.append((np.concatenate([e, g]), np.concatenate([f, hh]), spacing * math.sqrt(2.0), stiffness[1]))
+    i2, j2 = idx[:, :-2].ravel(), idx[:, 2:].ravel()
+    k2, l2 = idx[:-2, :].ravel(), idx[2:, :].ravel()
+    fams.append((np.concatenate([i2, k2]), np.concatenate([j2, l2]), 2.0 * spacing, stiffness[2]))
+    pairs = np.concatenate([np.stack([p, q], 1) for p, q, _, _ in fams]).astype(np.uint32)
+    rest = np.concatenate([np.full(len(p), r, np.float32) for p, _, r, _ in fams])
+    stiff = np.concatenate([np.full(len(p), s, np.float32) for p, _, _, s in fams])
+    return pairs, rest, stiff
+
+
+def mixed(cloth_n=256, cubes=(25, 5, 25), fluid=(100, 50, 50), spacing=0.1, box=(30.0, 20.0, 30.0),
+          iterations=4, max_neighbors=96, seed=0x5EED0003, dt=1.0 / 60.0):
+    """3D mixed-phase scene: a cloth_n^2 cloth (structural/shear/bend links,
+    stiffness 1.0/0.5/0.1, corners pinned) above prod(cubes) rigid 4x4x4
+    cubes (shape matching, mu_s 0.6 / mu_k 0.4) dropped from a seeded
+    lattice, above two fluid blocks of rest density 1.0 and 2.0 (in units of
+    the lattice kernel sum: particle masses 1 and 2).  Defaults: 65,536 +
+    200,000 + 500,000 = 765,536 particles."""
+    rng_off = 0
+    h = 2.0 * spacing
+    radius = 0.5 * spacing
+    S = lattice_rest_density(h, spacing)
+    parts = []
+    # two fluid blocks side by side on the floor
+    fx, fy, fz = fluid
+    f1 = fluid_block(fx, fy, fz, spacing, (1.0, 0.0, 1.0), 1e-3, seed)
+    f2 = fluid_block(fx, fy, fz, spacing, (1.0, 0.0, 1.0 + fz * spacing), 1e-3, seed + 1)
+    nf = f1.shape[0]
+    parts.append((f1, np.ones(nf, np.float32), np.zeros(nf, np.int32), np.full(nf, S, np.float32)))
+    parts.append((f2, np.full(nf, 0.5, np.float32), np.zeros(nf, np.int32), np.full(nf, 2.0 * S, np.float32)))
+    # rigid cubes on a jittered lattice above the fluid
+    cx, cy, cz = cubes
+    ncube = cx * cy * cz
+    u = uniform24(seed + 2, 3 * ncube).reshape(ncube, 3)
+    k = np.arange(4, dtype=np.float64)
+    cz_, cy_, cx_ = np.meshgrid(k, k, k, indexing="ij")
+    local = np.stack([cx_.ravel(), cy_.ravel(), cz_.ravel()], 1) * spacing
+    pitch = 1.0
+    cube_pos = []
+    for q in range(ncube):
+        i, rem = q % cx, q // cx
+        j, l = rem % cy, rem // cy
+        base = np.array([3.0 + i * pitch, fy * spacing + 3.0 + j * pitch, 3.0 + l * pitch])
+        base = base + (u[q] - 0.5) * 0.3
+        cube_pos.append(local + base)
+    cube_pos = np.concatenate(cube_pos).astype(np.float32) if ncube else np.zeros((0, 3), np.float32)
+    nc = cube_pos.shape[0]
+    cube_phase = (PHASE_RIGID0 + 1 + np.repeat(np.arange(ncube), 64)).astype(np.int32)
+    parts.append((cube_pos, np.ones(nc, np.float32), cube_phase, np.ones(nc, np.float32)))
+    # cloth above everything
+    ny_top = fy * spacing + 3.0 + cy * pitch + 2.0
+    gx = np.arange(cloth_n, dtype=np.float64) * spacing + 1.0
+    zz, xx = np.meshgrid(gx, gx, indexing="ij")
+    cloth = np.stack([xx.ravel(), np.full(cloth_n * cloth_n, ny_top), zz.ravel()], 1).astype(np.float32)
+    ncl = cloth.shape[0]
+    parts.append((cloth, np.ones(ncl, np.float32), np.full(ncl, PHASE_RIGID0, np.int32),
+                  np.ones(ncl, np.float32)))
+    pos = np.concatenate([p[0] for p in parts])
+    w = np.concatenate([p[1] for p in parts])
+    ph = np.concatenate([p[2] for p in parts])
+    r0 = np.concatenate([p[3] for p in parts])
+    n = pos.shape[0]
+    p = pbf_params((0.0, 0.0, 0.0), box, h, radius, iterations, n, max_neighbors, seed=seed)
+    p.mu_s, p.mu_k = 0.6, 0.4
+    s = Scene(p, pos, w, ph, r0, dt, name="mixed_%d" % n)
+    first_cloth = n - ncl
+    s.distance = cloth_constraints(cloth_n, cloth_n, first_cloth, spacing)
+    corners = np.array([0, cloth_n - 1, cloth_n * (cloth_n - 1), cloth_n * cloth_n - 1]) + first_cloth
+    s.pins = (corners.astype(np.uint32), pos[corners])
+    first_cube = 2 * nf
+    s.bodies = [(np.arange(first_cube + 64 * q, first_cube + 64 * (q + 1), dtype=np.uint32), 1.0)
+                for q in range(ncube)]
+    s.rho0 = S
+    del rng_off
+    return s
+
+
+def two_d_fluid(nx=40, ny=20, spacing=1.0, h=2.0, iterations=3, seed=0x5EED0001, dt=1.0 / 60.0):
+    """2D PBF block (dims = 2, z fixed) with rest density from the square
+    lattice kernel sum."""
+    radius = 0.5 * spacing
+    ext = (nx * spacing * 1.5, ny * spacing * 3.0, h)
+    p = pbf_params((0.0, 0.0, 0.0), ext, h, radius, iterations, nx * ny, 32, seed=seed, dims=2,
+                   relaxation=600.0)
+    pos = fluid_block(nx, ny, 1, spacing, (0.0, 0.0, 0.0), 0.01, seed)
+    pos[:, 2] = 0.5 * h
+    n = nx * ny
+    S = lattice_rest_density(h, spacing, dims=2)
+    return Scene(p, pos, np.ones(n, np.float32), np.zeros(n, np.int32),
+                 np.full(n, S, np.float32), dt, name="fluid2d")

@@ -1,0 +1,130 @@
+"""Host-side checks of the C-ABI that need no GPU: the library loads, exports
+every function include/ps.h declares, its structs have the layout the
+Python binding assumes, and parameter validation / error reporting work."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+from particlesolver_amd import capi, scenes
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "ps.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(ps_[a-z_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_function():
+    lib = capi.load()
+    names = declared_functions()
+    assert len(names) >= 30
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    # and the Python binding prototypes every one of them
+    assert sorted(capi.PROTOTYPES) == names
+
+
+def test_struct_layout_matches_binding(tmp_path):
+    src = tmp_path / "probe.c"
+    src.write_text("""
+#include <stdio.h>
+#include <stddef.h>
+#include "ps.h"
+int main(void) {
+  printf("%zu %zu %zu\\n", sizeof(ps_params), sizeof(ps_derived), sizeof(ps_status));
+  printf("%zu %zu %zu %zu\\n", offsetof(ps_params, grid), offsetof(ps_params, seed),
+         offsetof(ps_params, flags), offsetof(ps_derived, num_cells));
+  return 0;
+}
+""")
+    exe = tmp_path / "probe"
+    subprocess.run(["gcc", "-I" + os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()
+    sizes = [int(v) for v in out]
+    assert sizes[0] == ctypes.sizeof(capi.PsParams)
+    assert sizes[1] == ctypes.sizeof(capi.PsDerived)
+    assert sizes[2] == ctypes.sizeof(capi.PsStatus)
+    assert sizes[3] == capi.PsParams.grid.offset
+    assert sizes[4] == capi.PsParams.seed.offset
+    assert sizes[5] == capi.PsParams.flags.offset
+    assert sizes[6] == capi.PsDerived.num_cells.offset
+
+
+def test_default_params_are_the_reference_demo_constants():
+    p = capi.default_params()
+    assert p.abi_version == capi.PS_ABI_VERSION
+    assert p.radius == pytest.approx(0.25) and p.h == 2.0 and p.iterations == 5
+    assert list(p.box_min) == [-50, 0, -50] and list(p.box_max) == [50, 200, 50]
+    assert p.max_particles == 15000
+    assert p.relaxation == pytest.approx(0.01) and p.k_corr == pytest.approx(0.1)
+    assert p.n_corr == 4 and p.dq_corr == pytest.approx(0.2)
+    lib = capi.load()
+    assert lib.ps_params_validate(ctypes.byref(p)) == capi.PS_OK
+
+
+@pytest.mark.parametrize("field,value", [
+    ("abi_version", 99), ("dims", 4), ("radius", 0.0), ("h", -1.0), ("cell", 0.1),
+    ("iterations", 0), ("max_particles", 0), ("max_neighbors", 6), ("dq_corr", 1.5),
+    ("relaxation", float("nan")), ("k_corr", -0.1),
+])
+def test_validation_rejects_bad_values(field, value):
+    lib = capi.load()
+    p = capi.default_params()
+    setattr(p, field, value)
+    assert lib.ps_params_validate(ctypes.byref(p)) == capi.PS_ERR_INVALID_ARGUMENT
+    assert len(lib.ps_last_error(None)) > 0
+    h = ctypes.c_void_p()
+    assert lib.ps_create(ctypes.byref(p), ctypes.byref(h)) == capi.PS_ERR_INVALID_ARGUMENT
+    assert not h.value
+
+
+def test_validation_rejects_bad_geometry():
+    lib = capi.load()
+    p = capi.default_params()
+    p.box_max[1] = p.box_min[1]
+    assert lib.ps_params_validate(ctypes.byref(p)) == capi.PS_ERR_INVALID_ARGUMENT
+    p = capi.default_params()
+    p.dims = 2                      # 2D needs a single cell layer
+    assert lib.ps_params_validate(ctypes.byref(p)) == capi.PS_ERR_INVALID_ARGUMENT
+    p.grid[2] = 1
+    assert lib.ps_params_validate(ctypes.byref(p)) == capi.PS_OK
+    assert lib.ps_params_validate(None) == capi.PS_ERR_INVALID_ARGUMENT
+
+
+def test_derive_matches_closed_forms():
+    import math
+    sc = scenes.dam_break(8, 4, 8)
+    d = capi.derive(sc.params)
+    h = sc.params.h
+    assert d.poly6 == pytest.approx(315.0 / (64.0 * math.pi * h ** 9), rel=1e-6)
+    assert d.spiky == pytest.approx(45.0 / (math.pi * h ** 6), rel=1e-6)
+    assert d.tensile_k == pytest.approx(0.1 * h * h, rel=1e-6)
+    assert d.num_cells == sc.params.grid[0] * sc.params.grid[1] * sc.params.grid[2] == 8 * 4 * 8
+
+
+def test_null_handles_and_kernel_names():
+    lib = capi.load()
+    assert lib.ps_destroy(None) == capi.PS_OK
+    assert lib.ps_step(None, ctypes.c_float(0.01), 1) == capi.PS_ERR_INVALID_ARGUMENT
+    assert lib.ps_num_particles(None) == 0
+    assert [lib.ps_kernel_name(k).decode() for k in range(len(capi.KERNELS))] == capi.KERNELS
+    assert lib.ps_kernel_name(99) == b"unknown"
+
+
+def test_create_without_gpu_reports_no_device_or_succeeds():
+    # here (no GPU) this must be an error code, never a crash; on a GPU box it succeeds
+    lib = capi.load()
+    p = capi.default_params()
+    h = ctypes.c_void_p()
+    rc = lib.ps_create(ctypes.byref(p), ctypes.byref(h))
+    assert rc in (capi.PS_OK, capi.PS_ERR_NO_DEVICE, capi.PS_ERR_DEVICE)
+    if rc == capi.PS_OK:
+        assert lib.ps_destroy(h) == capi.PS_OK
+    else:
+        assert b"device" in lib.ps_last_error(None).lower()
