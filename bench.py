@@ -53,6 +53,9 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-timing", action="store_true",
                     help="no per-kernel events in the timed region")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="also run 3 device-to-device copies of MIB MiB (the runtime's copy "
+                         "kernel), to scale HBM counters against a known byte count")
     ap.add_argument("--breakdown", type=int, default=20,
                     help="substeps of an extra, separately timed run with events on every kernel")
     return ap.parse_args()
@@ -81,6 +84,25 @@ def cpu_baseline(sc, budget_s):
     return {"value": rate, "unit": "particle-iterations/s", "cores": threads, "kind": "port",
             "sample": "CPU oracle (C, FP32, OpenMP %d threads) on the same %d-particle dam break: "
                       "%d substeps after 1 warm-up, %.1f s" % (threads, sc.n, done, el)}
+
+
+def copy_calibration(mib):
+    """Three hipMemcpy device-to-device copies of `mib` MiB: each is one
+    dispatch of the runtime's copy kernel reading and writing exactly that
+    many bytes."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = mib << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(n)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(n)) == 0
+    assert hip.hipMemset(a, 1, ctypes.c_size_t(n)) == 0
+    for _ in range(3):
+        assert hip.hipMemcpy(b, a, ctypes.c_size_t(n), 3) == 0   # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print("copy calibration: 3 x %d bytes" % n, file=sys.stderr)
 
 
 def main():
@@ -184,6 +206,8 @@ def main():
         "breakdown_us_per_launch_all_events": breakdown,
     }
     print(json.dumps(line))
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
 
 
 if __name__ == "__main__":
