@@ -134,6 +134,19 @@ typedef struct ps_status {
     uint64_t iterations;       /* solver iterations completed                     */
 } ps_status;
 
+/* One scene of a batch (ps_create_batch): box (also its grid origin; the
+ * grid is dense with params.cell), gravity = params.gravity * gravity_scale,
+ * and the seed of its wall jitter.  A scene evolves exactly as a
+ * single-scene system with these values would. */
+typedef struct ps_scene_desc {
+    float box_min[3];
+    float box_max[3];
+    float gravity_scale;
+    uint32_t reserved0;
+    uint64_t seed;
+    uint32_t reserved[4];
+} ps_scene_desc;
+
 typedef struct ps_system ps_system;
 
 /* ---- parameters (host only; no GPU needed) ------------------------------ */
@@ -154,6 +167,15 @@ int ps_derive(const ps_params *params, ps_derived *out);
  * (gpu/src/cuda/util.cu:12-15) and setParameters (integration.cu:96-100).
  * Allocates every buffer for max_particles up front. */
 int ps_create(const ps_params *params, ps_system **out);
+/* Many independent scenes in one handle (no reference counterpart: the
+ * reference runs one scene per process).  Every kernel launch covers all
+ * scenes; each scene's cells are a disjoint range of one global grid.
+ * params supplies everything except the boxes, grids, gravity scales and
+ * seeds, which come from scenes[0..nscenes). */
+int ps_create_batch(const ps_params *params, uint32_t nscenes, const ps_scene_desc *scenes,
+                    ps_system **out);
+uint32_t ps_num_scenes(const ps_system *sys);
+
 /* Replaces _finalize (gpu/src/particlesystem.cpp:115-135) and the
  * free*Vectors calls (integration.cu:74-94, solver.cu:158-178,
  * shared_variables.cu:12-23). */
@@ -176,6 +198,14 @@ const char *ps_last_error(const ps_system *sys);
 int ps_add_particles(ps_system *sys, uint32_t n, const float *pos, const float *vel,
                      const float *inv_mass, const int32_t *phase,
                      const float *rest_density, uint32_t *first_out);
+
+/* Append particles to scene `scene` of a batch (ps_add_particles adds to
+ * scene 0).  A scene's particles must form one contiguous insertion range:
+ * fill scenes in increasing order.  Constraint and body calls take the
+ * global insertion indices. */
+int ps_add_scene_particles(ps_system *sys, uint32_t scene, uint32_t n, const float *pos,
+                           const float *vel, const float *inv_mass, const int32_t *phase,
+                           const float *rest_density, uint32_t *first_out);
 
 /* Replaces addDistanceConstraint + updateOccurences (solver.cu:72-106,125-156).
  * pairs: 2*m particle indices; rest: m lengths; stiffness: m values in (0,1]
@@ -229,6 +259,13 @@ int ps_substep_end(ps_system *sys);
 int ps_predict(ps_system *sys, float dt);
 int ps_build_grid(ps_system *sys);
 int ps_find_neighbors(ps_system *sys);
+
+/* ps_step through a captured HIP graph: the first call (or the first after
+ * any scene, parameter or dt change) records `substeps` substeps into a
+ * graph; later calls replay it.  substeps must be a multiple of the buffer
+ * rotation period (4 for an even iteration count, 6 for odd; 12 works for
+ * both). */
+int ps_step_graph(ps_system *sys, float dt, uint32_t substeps);
 
 int ps_synchronize(ps_system *sys);
 

@@ -96,6 +96,12 @@ class PsDerived(ctypes.Structure):
         ("num_cells", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 3)]
 
 
+class PsSceneDesc(ctypes.Structure):
+    _fields_ = [("box_min", ctypes.c_float * 3), ("box_max", ctypes.c_float * 3),
+                ("gravity_scale", ctypes.c_float), ("reserved0", ctypes.c_uint32),
+                ("seed", ctypes.c_uint64), ("reserved", ctypes.c_uint32 * 4)]
+
+
 class PsStatus(ctypes.Structure):
     _fields_ = [("flags", ctypes.c_uint32), ("max_neighbors", ctypes.c_uint32),
                 ("num_particles", ctypes.c_uint32), ("num_cells", ctypes.c_uint32),
@@ -113,6 +119,10 @@ PROTOTYPES = {
     "ps_params_validate": (_int, [ctypes.POINTER(PsParams)]),
     "ps_derive": (_int, [ctypes.POINTER(PsParams), ctypes.POINTER(PsDerived)]),
     "ps_create": (_int, [ctypes.POINTER(PsParams), ctypes.POINTER(_P)]),
+    "ps_create_batch": (_int, [ctypes.POINTER(PsParams), _u32, _P, ctypes.POINTER(_P)]),
+    "ps_num_scenes": (_u32, [_P]),
+    "ps_add_scene_particles": (_int, [_P, _u32, _u32, _P, _P, _P, _P, _P, ctypes.POINTER(_u32)]),
+    "ps_step_graph": (_int, [_P, _f32, _u32]),
     "ps_destroy": (_int, [_P]),
     "ps_set_params": (_int, [_P, ctypes.POINTER(PsParams)]),
     "ps_get_params": (_int, [_P, ctypes.POINTER(PsParams)]),
@@ -197,11 +207,17 @@ def derive(params):
 class System:
     """One particle system on the GPU (wraps a ps_system handle)."""
 
-    def __init__(self, params):
+    def __init__(self, params, scenes=None):
+        """scenes: None for one scene, or a list of PsSceneDesc (a batch)."""
         self.lib = load()
         self.params = params.copy()
         h = ctypes.c_void_p()
-        rc = self.lib.ps_create(ctypes.byref(self.params), ctypes.byref(h))
+        if scenes is None:
+            rc = self.lib.ps_create(ctypes.byref(self.params), ctypes.byref(h))
+        else:
+            arr = (PsSceneDesc * len(scenes))(*scenes)
+            rc = self.lib.ps_create_batch(ctypes.byref(self.params), len(scenes),
+                                          ctypes.cast(arr, ctypes.c_void_p), ctypes.byref(h))
         if rc:
             raise PsError(rc, self.lib.ps_last_error(None).decode())
         self.h = h
@@ -229,7 +245,8 @@ class System:
             pass
 
     # -- scene
-    def add_particles(self, pos, vel=None, inv_mass=None, phase=None, rest_density=None):
+    def add_particles(self, pos, vel=None, inv_mass=None, phase=None, rest_density=None,
+                      scene=0):
         pos = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1, 3)
         n = pos.shape[0]
         vel = None if vel is None else np.ascontiguousarray(vel, dtype=np.float32).reshape(n, 3)
@@ -240,8 +257,8 @@ class System:
         r0 = np.ascontiguousarray(np.broadcast_to(1.0 if rest_density is None else rest_density,
                                                   (n,)), dtype=np.float32)
         first = ctypes.c_uint32()
-        self._check(self.lib.ps_add_particles(self.h, n, _ptr(pos), _ptr(vel), _ptr(w), _ptr(ph),
-                                              _ptr(r0), ctypes.byref(first)))
+        self._check(self.lib.ps_add_scene_particles(self.h, scene, n, _ptr(pos), _ptr(vel), _ptr(w),
+                                                    _ptr(ph), _ptr(r0), ctypes.byref(first)))
         return first.value
 
     def add_distance_constraints(self, pairs, rest, stiffness=None):
@@ -283,6 +300,9 @@ class System:
     # -- stepping
     def step(self, dt, substeps=1):
         self._check(self.lib.ps_step(self.h, dt, substeps))
+
+    def step_graph(self, dt, substeps):
+        self._check(self.lib.ps_step_graph(self.h, dt, substeps))
 
     def begin(self, dt):
         self._check(self.lib.ps_substep_begin(self.h, dt))

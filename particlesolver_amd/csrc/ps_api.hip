@@ -48,7 +48,16 @@ struct ps_system {
     std::vector<uint32_t> body_off{0}, body_idx, body_of, member_of;
     std::vector<float> body_rest, body_stiff, body_mass;
     size_t dist_cap = 0, pin_cap = 0, body_cap = 0, member_cap = 0;
+    // scenes (one unless created with ps_create_batch)
+    std::vector<ps::SceneDev> scenes;
+    std::vector<uint32_t> scene_count;
+    uint32_t last_scene = 0;
+    // captured substep graph (ps_step_graph)
+    hipGraphExec_t gexec = nullptr;
+    float g_dt = 0.0f;
+    uint32_t g_sub = 0;
     // substep state
+    uint32_t it_sub = 0;    // iterations done in the open substep
     int stage = 0;          // 1 between predict and end
     bool hashed = false;    // keys/ranks of the current storage are valid
     bool nbr_valid = false; // neighbour table matches the current storage
@@ -252,6 +261,13 @@ void free_all(ps_system *s)
     dev_free(b.pin_of); dev_free(b.pin_xyz); dev_free(b.body_off); dev_free(b.body_idx);
     dev_free(b.body_rest); dev_free(b.body_stiff); dev_free(b.body_mass); dev_free(b.body_q);
     dev_free(b.body_goal); dev_free(b.body_of); dev_free(b.member_of); dev_free(b.status);
+    dev_free(b.sid); dev_free(b.sid2); dev_free(b.scenes);
+}
+
+void drop_graph(ps_system *s)
+{
+    if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
+    s->gexec = nullptr;
 }
 
 // ---- timing -------------------------------------------------------------
@@ -369,9 +385,10 @@ int stage_iterate(ps_system *s)
     const uint32_t nbody = (uint32_t)s->body_off.size() - 1;
     if (nbody) LAUNCH(s, PS_K_SHAPE, ps::launch_shape(s->stream, s->b, nbody));
     LAUNCH(s, PS_K_SOLVE,
-           ps::launch_solve(s->stream, s->b, s->kc, s->n, features(s), uni, s->wu, s->iters));
+           ps::launch_solve(s->stream, s->b, s->kc, s->n, features(s), uni, s->wu, s->it_sub));
     std::swap(s->b.P, s->b.Q);
     s->iters++;
+    s->it_sub++;
     return PS_OK;
 }
 
@@ -380,7 +397,8 @@ int stage_end(ps_system *s)
     const bool uni = uniform_fluid(s);
     LAUNCH(s, PS_K_FINALIZE,
            ps::launch_finalize(s->stream, s->b, s->kc, s->n, s->dt,
-                               (s->prm.flags & PS_FLAG_SLEEPING) != 0, uni, s->wu));
+                               (s->prm.flags & PS_FLAG_SLEEPING) != 0, uni, s->wu, s->it_sub));
+    s->it_sub = 0;
     std::swap(s->b.X, s->b.P);
     s->substeps++;
     s->stage = 0;
@@ -537,22 +555,67 @@ int ps_derive(const ps_params *p, ps_derived *out)
     return PS_OK;
 }
 
-int ps_create(const ps_params *params, ps_system **out)
+static int create_impl(const ps_params *params, uint32_t nscenes, const ps_scene_desc *descs,
+                       ps_system **out)
 {
     std::string why;
     int rc = validate(params, why);
     if (rc) return fail(nullptr, rc, why);
     if (!out) return fail(nullptr, PS_ERR_INVALID_ARGUMENT, "out is NULL");
     *out = nullptr;
+    if (nscenes == 0 || (nscenes > 1 && !descs))
+        return fail(nullptr, PS_ERR_INVALID_ARGUMENT, "a batch needs scene descriptions");
+    // scene table: one scene from the parameters, or one per description
+    std::vector<ps::SceneDev> scenes(nscenes);
+    uint64_t cells = 0;
+    for (uint32_t q = 0; q < nscenes; q++) {
+        ps::SceneDev &sc = scenes[q];
+        memset(&sc, 0, sizeof(sc));
+        sc.cell = params->cell;
+        sc.cell_base = (uint32_t)cells;
+        if (nscenes == 1 && !descs) {
+            for (int a = 0; a < 3; a++) {
+                sc.box_min[a] = params->box_min[a];
+                sc.box_max[a] = params->box_max[a];
+                sc.origin[a] = params->origin[a];
+                sc.gravity[a] = params->gravity[a];
+                sc.grid[a] = params->grid[a];
+            }
+            sc.seed = params->seed;
+        } else {
+            const ps_scene_desc &d = descs[q];
+            if (!is_finite(d.gravity_scale))
+                return fail(nullptr, PS_ERR_INVALID_ARGUMENT, "gravity_scale must be finite");
+            for (int a = 0; a < 3; a++) {
+                if (!is_finite(d.box_min[a]) || !is_finite(d.box_max[a]) || !(d.box_max[a] > d.box_min[a]))
+                    return fail(nullptr, PS_ERR_INVALID_ARGUMENT, "scene box must be finite and non-empty");
+                sc.box_min[a] = d.box_min[a];
+                sc.box_max[a] = d.box_max[a];
+                sc.origin[a] = d.box_min[a];
+                sc.gravity[a] = params->gravity[a] * d.gravity_scale;
+                const double g = ceil(((double)d.box_max[a] - (double)d.box_min[a]) / (double)params->cell - 1e-9);
+                sc.grid[a] = (params->dims == 2 && a == 2) ? 1u : (uint32_t)(g < 1.0 ? 1.0 : g);
+                if (sc.grid[a] > (1u << 20))
+                    return fail(nullptr, PS_ERR_INVALID_ARGUMENT, "scene grid too large");
+            }
+            sc.seed = d.seed;
+        }
+        cells += (uint64_t)sc.grid[0] * sc.grid[1] * sc.grid[2];
+        if (cells >= (1ull << 31)) return fail(nullptr, PS_ERR_INVALID_ARGUMENT, "too many grid cells");
+    }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
         return fail(nullptr, PS_ERR_NO_DEVICE, "no HIP device available");
     ps_system *s = new ps_system();
     s->prm = *params;
     derive_constants(s->prm, s->der);
+    s->der.num_cells = (uint32_t)cells;
     fill_kconst(s->prm, s->der, s->kc);
+    s->kc.nscenes = nscenes;
+    s->scenes = scenes;
+    s->scene_count.assign(nscenes, 0);
     s->cap = params->max_particles;
-    s->ncell = s->der.num_cells;
+    s->ncell = (uint32_t)cells;
     s->kmax = params->max_neighbors;
     s->tiles = ps::scan_tiles(s->ncell);
     s->occ.assign(s->cap, 0);
@@ -580,6 +643,11 @@ int ps_create(const ps_params *params, ps_system **out)
     ALLOC(b.nbr, groups * 64 * s->kmax); ALLOC(b.ncnt, c);
     ALLOC(b.dist_off, c + 1); ALLOC(b.occ, c); ALLOC(b.pin_of, c);
     ALLOC(b.body_of, c); ALLOC(b.member_of, c); ALLOC(b.status, 1);
+    ALLOC(b.scenes, nscenes);
+    if (nscenes > 1) {
+        ALLOC(b.sid, c);
+        ALLOC(b.sid2, c);
+    }
 #undef ALLOC
     hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMemset(b.cell_count, 0, sizeof(uint32_t) * s->ncell);
@@ -591,6 +659,8 @@ int ps_create(const ps_params *params, ps_system **out)
     if (e == hipSuccess) e = hipMemset(b.body_of, 0, sizeof(uint32_t) * c);
     if (e == hipSuccess) e = hipMemset(b.member_of, 0, sizeof(uint32_t) * c);
     if (e == hipSuccess) e = hipMemset(b.status, 0, sizeof(ps::DevStatus));
+    if (e == hipSuccess)
+        e = hipMemcpy(b.scenes, scenes.data(), sizeof(ps::SceneDev) * nscenes, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) {
         std::string msg = std::string("device setup: ") + hipGetErrorString(e);
@@ -603,6 +673,21 @@ int ps_create(const ps_params *params, ps_system **out)
     return PS_OK;
 }
 
+int ps_create(const ps_params *params, ps_system **out)
+{
+    return create_impl(params, 1, nullptr, out);
+}
+
+int ps_create_batch(const ps_params *params, uint32_t nscenes, const ps_scene_desc *scenes,
+                    ps_system **out)
+{
+    if (nscenes == 0 || !scenes)
+        return fail(nullptr, PS_ERR_INVALID_ARGUMENT, "a batch needs at least one scene");
+    return create_impl(params, nscenes, scenes, out);
+}
+
+uint32_t ps_num_scenes(const ps_system *s) { return s ? (uint32_t)s->scenes.size() : 0; }
+
 int ps_destroy(ps_system *s)
 {
     if (!s) return PS_OK;
@@ -612,6 +697,7 @@ int ps_destroy(ps_system *s)
         (void)hipEventDestroy(tl.stop);
     }
     for (hipEvent_t e : s->free_events) (void)hipEventDestroy(e);
+    drop_graph(s);
     free_all(s);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
@@ -628,9 +714,26 @@ int ps_set_params(ps_system *s, const ps_params *p)
         p->dims != s->prm.dims || memcmp(p->grid, s->prm.grid, sizeof(p->grid)) != 0)
         return fail(s, PS_ERR_INVALID_ARGUMENT,
                     "capacities, grid and dims are fixed at ps_create");
+    if (s->scenes.size() > 1 && p->cell != s->prm.cell)
+        return fail(s, PS_ERR_INVALID_ARGUMENT, "the cell size of a batch is fixed at creation");
+    const uint32_t ncells = s->der.num_cells, nsc = s->kc.nscenes;
     s->prm = *p;
     derive_constants(s->prm, s->der);
+    s->der.num_cells = ncells;
     fill_kconst(s->prm, s->der, s->kc);
+    s->kc.nscenes = nsc;
+    if (nsc == 1) {
+        // the single scene follows the parameters' box, gravity and seed
+        ps::SceneDev &sc = s->scenes[0];
+        for (int a = 0; a < 3; a++) {
+            sc.box_min[a] = p->box_min[a];
+            sc.box_max[a] = p->box_max[a];
+            sc.origin[a] = p->origin[a];
+            sc.gravity[a] = p->gravity[a];
+        }
+        sc.seed = p->seed;
+    }
+    drop_graph(s);
     return PS_OK;
 }
 
@@ -651,8 +754,19 @@ int ps_add_particles(ps_system *s, uint32_t n, const float *pos, const float *ve
                      const float *inv_mass, const int32_t *phase, const float *rest_density,
                      uint32_t *first_out)
 {
+    return ps_add_scene_particles(s, 0, n, pos, vel, inv_mass, phase, rest_density, first_out);
+}
+
+int ps_add_scene_particles(ps_system *s, uint32_t scene, uint32_t n, const float *pos,
+                           const float *vel, const float *inv_mass, const int32_t *phase,
+                           const float *rest_density, uint32_t *first_out)
+{
     if (int rc = check_sys(s)) return rc;
     if (s->stage) return fail(s, PS_ERR_STATE, "cannot add particles inside a substep");
+    if (scene >= s->scenes.size()) return fail(s, PS_ERR_INVALID_ARGUMENT, "scene out of range");
+    if (n && s->n && scene < s->last_scene)
+        return fail(s, PS_ERR_INVALID_ARGUMENT,
+                    "fill scenes in increasing order (a scene is one contiguous index range)");
     if (n == 0) {
         if (first_out) *first_out = s->n;
         return PS_OK;
@@ -700,6 +814,18 @@ int ps_add_particles(ps_system *s, uint32_t n, const float *pos, const float *ve
     HIP_TRY(s, hipMemcpy(b.V + o, V.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
     HIP_TRY(s, hipMemcpy(b.A + o, A.data(), sizeof(Attr) * n, hipMemcpyHostToDevice));
     HIP_TRY(s, hipMemcpy(b.slot_of + o, slot.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice));
+    if (s->scene_count[scene] == 0) {
+        s->scenes[scene].first = s->n;
+        HIP_TRY(s, hipMemcpy(b.scenes + scene, &s->scenes[scene], sizeof(ps::SceneDev),
+                             hipMemcpyHostToDevice));
+    }
+    if (b.sid) {
+        std::vector<uint32_t> sid(n, scene);
+        HIP_TRY(s, hipMemcpy(b.sid + o, sid.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice));
+    }
+    s->scene_count[scene] += n;
+    s->last_scene = scene;
+    drop_graph(s);
     if (first_out) *first_out = s->n;
     s->n += n;
     s->uniform = uni;
@@ -733,6 +859,7 @@ int ps_add_distance_constraints(ps_system *s, uint32_t m, const uint32_t *pairs,
         s->occ[pairs[2 * k + 1]]++;
     }
     HIP_TRY(s, hipStreamSynchronize(s->stream));
+    drop_graph(s);
     return upload_constraints(s);
 }
 
@@ -756,6 +883,7 @@ int ps_add_point_constraints(ps_system *s, uint32_t m, const uint32_t *idx, cons
         s->occ[idx[k]]++;
     }
     HIP_TRY(s, hipStreamSynchronize(s->stream));
+    drop_graph(s);
     return upload_pins(s);
 }
 
@@ -857,6 +985,7 @@ int ps_add_rigid_bodies(ps_system *s, uint32_t nbody, const uint32_t *offsets,
     HIP_TRY(s, hipMemcpy(b.body_of, s->body_of.data(), sizeof(uint32_t) * s->cap, hipMemcpyHostToDevice));
     HIP_TRY(s, hipMemcpy(b.member_of, s->member_of.data(), sizeof(uint32_t) * s->cap, hipMemcpyHostToDevice));
     if (first_body_out) *first_body_out = bfirst;
+    drop_graph(s);
     return PS_OK;
 }
 
@@ -900,6 +1029,7 @@ int ps_predict(ps_system *s, float dt)
     LAUNCH(s, PS_K_PREDICT_HASH, ps::launch_predict_hash(s->stream, s->b, s->kc, s->n, dt, true, false));
     s->dt = dt;
     s->stage = 1;
+    s->it_sub = 0;
     s->hashed = false;
     s->nbr_valid = false;
     return PS_OK;
@@ -928,6 +1058,7 @@ int ps_substep_begin(ps_system *s, float dt)
     if (!(dt > 0.0f) || !is_finite(dt)) return fail(s, PS_ERR_INVALID_ARGUMENT, "dt must be > 0");
     int rc;
     s->dt = dt;
+    s->it_sub = 0;
     if ((rc = stage_hash(s, true, dt))) return rc;  // fused predict + hash
     s->stage = 1;
     if ((rc = stage_sort(s))) return rc;
@@ -962,6 +1093,59 @@ int ps_step(ps_system *s, float dt, uint32_t substeps)
             if ((rc = stage_iterate(s))) return rc;
         if ((rc = stage_end(s))) return rc;
     }
+    return PS_OK;
+}
+
+int ps_step_graph(ps_system *s, float dt, uint32_t substeps)
+{
+    if (int rc = check_sys(s)) return rc;
+    if (s->stage) return fail(s, PS_ERR_STATE, "ps_step_graph inside an open substep");
+    if (!(dt > 0.0f) || !is_finite(dt)) return fail(s, PS_ERR_INVALID_ARGUMENT, "dt must be > 0");
+    if (s->n == 0 || substeps == 0) return PS_OK;
+    if (!s->gexec || s->g_dt != dt || s->g_sub != substeps) {
+        drop_graph(s);
+        const DevBuffers start = s->b;
+        const uint64_t sub0 = s->substeps, it0 = s->iters;
+        const uint32_t timing = s->timing;
+        s->timing = 0;  // no event records inside the graph
+        HIP_TRY(s, hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+        int rc = PS_OK;
+        for (uint32_t k = 0; k < substeps && rc == PS_OK; k++) {
+            rc = ps_substep_begin(s, dt);
+            for (uint32_t it = 0; it < s->prm.iterations && rc == PS_OK; it++) rc = stage_iterate(s);
+            if (rc == PS_OK) rc = stage_end(s);
+        }
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(s->stream, &g);
+        s->timing = timing;
+        s->substeps = sub0;
+        s->iters = it0;
+        s->stage = 0;
+        s->it_sub = 0;
+        const bool rotated_back = memcmp(&start, &s->b, sizeof(DevBuffers)) == 0;
+        s->b = start;
+        if (rc != PS_OK || e != hipSuccess || !rotated_back) {
+            if (g) (void)hipGraphDestroy(g);
+            if (rc != PS_OK) return rc;
+            if (e != hipSuccess)
+                return fail(s, PS_ERR_DEVICE, std::string("graph capture: ") + hipGetErrorString(e));
+            return fail(s, PS_ERR_INVALID_ARGUMENT,
+                        "substeps must be a multiple of the buffer rotation period "
+                        "(4 for an even iteration count, 6 for odd)");
+        }
+        e = hipGraphInstantiate(&s->gexec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            s->gexec = nullptr;
+            return fail(s, PS_ERR_DEVICE, std::string("graph instantiate: ") + hipGetErrorString(e));
+        }
+        s->g_dt = dt;
+        s->g_sub = substeps;
+    }
+    HIP_TRY(s, hipGraphLaunch(s->gexec, s->stream));
+    s->substeps += substeps;
+    s->iters += (uint64_t)substeps * s->prm.iterations;
+    s->nbr_valid = true;
     return PS_OK;
 }
 
@@ -1088,7 +1272,7 @@ int ps_get_status(ps_system *s, ps_status *out)
     out->num_particles = s->n;
     out->num_cells = s->ncell;
     out->substeps = s->substeps;
-    out->iterations = s->iters;
+    out->iterations = ds.iter + s->it_sub;
     return PS_OK;
 }
 

@@ -21,6 +21,25 @@ struct __align__(16) Attr {
 struct DevStatus {
     uint32_t flags;
     uint32_t max_neighbors;
+    uint64_t iter;  // solver iterations completed (keys the wall-jitter RNG)
+};
+
+// One scene of a batch: its own box, dense grid (cells cell_base ..
+// cell_base + gx*gy*gz - 1 of the global cell index), gravity and RNG seed.
+// Particles of a scene are one contiguous insertion range starting at first.
+struct __align__(16) SceneDev {
+    float box_min[3];
+    float cell;
+    float box_max[3];
+    uint32_t cell_base;
+    float origin[3];
+    uint32_t first;
+    float gravity[3];
+    uint32_t pad;
+    uint32_t grid[3];
+    uint32_t pad2;
+    uint64_t seed;
+    uint64_t pad3;
 };
 
 // Constants handed to every kernel by value (kernarg segment).
@@ -35,6 +54,7 @@ struct KConst {
     uint32_t n_corr;
     uint32_t dims;
     uint64_t seed;
+    uint32_t nscenes;  // > 1: per-particle scene table lookups
 };
 
 // Scene-composition switches chosen on the host per launch; each selects a
@@ -58,6 +78,8 @@ struct DevBuffers {
     uint32_t *rank;        // arrival rank inside the cell (from the hash atomics)
     uint32_t *tmp_orig;    // insertion index by provisional sorted slot
     uint32_t *slot_of;     // insertion index -> current slot
+    uint32_t *sid, *sid2;  // scene of each slot (batches only)
+    SceneDev *scenes;
     // grid
     uint32_t *cell_count;  // num_cells, zero between builds
     uint32_t *cell_off;    // num_cells + 1
@@ -100,10 +122,12 @@ hipError_t launch_neighbors(hipStream_t st, const DevBuffers &b, const KConst &k
 hipError_t launch_lambda(hipStream_t st, const DevBuffers &b, const KConst &k, uint32_t n,
                          bool uniform, float wu);
 hipError_t launch_shape(hipStream_t st, const DevBuffers &b, uint32_t nbody);
+// it: iteration index inside the current substep (the RNG key is the device
+// iteration counter + it); finalize advances the counter by `its`.
 hipError_t launch_solve(hipStream_t st, const DevBuffers &b, const KConst &k, uint32_t n,
-                        uint32_t features, bool uniform, float wu, uint64_t iter);
+                        uint32_t features, bool uniform, float wu, uint32_t it);
 hipError_t launch_finalize(hipStream_t st, const DevBuffers &b, const KConst &k, uint32_t n,
-                           float dt, bool sleeping, bool uniform, float wu);
+                           float dt, bool sleeping, bool uniform, float wu, uint32_t its);
 
 uint32_t scan_tiles(uint32_t num_cells);
 

@@ -147,6 +147,8 @@ __global__ __launch_bounds__(kBlock) void predict_hash_k(DevBuffers b, KConst k,
 {
     const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
     const bool valid = s < n;
+    const bool multi = k.nscenes > 1u;
+    const SceneDev *sc = (multi && valid) ? &b.scenes[b.sid[s]] : nullptr;
     float px = 0.0f, py = 0.0f, pz = 0.0f;
     if (valid) {
         if (PRED) {
@@ -157,9 +159,12 @@ __global__ __launch_bounds__(kBlock) void predict_hash_k(DevBuffers b, KConst k,
             pz = x.z;
             if (a.w != 0.0f) {
                 const float4 v = b.V[s];
-                float vx = v.x + (k.gravity[0] * dt);
-                float vy = v.y + (k.gravity[1] * dt);
-                float vz = v.z + (k.gravity[2] * dt);
+                const float gx = multi ? sc->gravity[0] : k.gravity[0];
+                const float gy = multi ? sc->gravity[1] : k.gravity[1];
+                const float gz = multi ? sc->gravity[2] : k.gravity[2];
+                float vx = v.x + (gx * dt);
+                float vy = v.y + (gy * dt);
+                float vz = v.z + (gz * dt);
                 px = x.x + (vx * dt);
                 py = x.y + (vy * dt);
                 pz = k.dims == 2u ? x.z : (x.z + (vz * dt));
@@ -175,10 +180,17 @@ __global__ __launch_bounds__(kBlock) void predict_hash_k(DevBuffers b, KConst k,
     if (HASH) {
         uint32_t flags = 0, key = 0xffffffffu;
         if (valid) {
-            uint32_t cx = cell_axis(px, k.origin[0], k.cell, k.grid[0], flags);
-            uint32_t cy = cell_axis(py, k.origin[1], k.cell, k.grid[1], flags);
-            uint32_t cz = cell_axis(pz, k.origin[2], k.cell, k.grid[2], flags);
-            key = (cz * k.grid[1] + cy) * k.grid[0] + cx;
+            if (multi) {
+                uint32_t cx = cell_axis(px, sc->origin[0], k.cell, sc->grid[0], flags);
+                uint32_t cy = cell_axis(py, sc->origin[1], k.cell, sc->grid[1], flags);
+                uint32_t cz = cell_axis(pz, sc->origin[2], k.cell, sc->grid[2], flags);
+                key = sc->cell_base + (cz * sc->grid[1] + cy) * sc->grid[0] + cx;
+            } else {
+                uint32_t cx = cell_axis(px, k.origin[0], k.cell, k.grid[0], flags);
+                uint32_t cy = cell_axis(py, k.origin[1], k.cell, k.grid[1], flags);
+                uint32_t cz = cell_axis(pz, k.origin[2], k.cell, k.grid[2], flags);
+                key = (cz * k.grid[1] + cy) * k.grid[0] + cx;
+            }
         }
         // Storage is in last substep's cell order, so equal keys sit in runs
         // of adjacent lanes: one atomic per run hands out the run's ranks.
@@ -328,6 +340,7 @@ __global__ __launch_bounds__(kBlock) void reorder_k(DevBuffers b, uint32_t n)
     b.V2[d] = b.V[s];
     b.A2[d] = a;
     b.key2[d] = c;
+    if (b.sid) b.sid2[d] = b.sid[s];
     b.slot_of[a.orig] = d;
 }
 
@@ -343,8 +356,15 @@ __global__ __launch_bounds__(kBlock) void neighbors_k(DevBuffers b, KConst k, ui
     const uint32_t i = xcd_block() * kBlock + threadIdx.x;
     uint32_t cnt = 0;
     if (i < n) {
-        const uint32_t gx = k.grid[0], gy = k.grid[1], gz = k.grid[2];
-        const uint32_t key = b.key[i];
+        uint32_t gx = k.grid[0], gy = k.grid[1], gz = k.grid[2], base = 0;
+        if (k.nscenes > 1u) {
+            const SceneDev &sc = b.scenes[b.sid[i]];
+            gx = sc.grid[0];
+            gy = sc.grid[1];
+            gz = sc.grid[2];
+            base = sc.cell_base;
+        }
+        const uint32_t key = b.key[i] - base;
         const uint32_t cx = key % gx, cy = (key / gx) % gy, cz = key / (gx * gy);
         const uint32_t x0 = cx > 0 ? cx - 1 : 0, x1 = cx + 1 < gx ? cx + 1 : gx - 1;
         const float4 p = b.P[i];
@@ -357,7 +377,7 @@ __global__ __launch_bounds__(kBlock) void neighbors_k(DevBuffers b, KConst k, ui
             for (int dy = -1; dy <= 1; dy++) {
                 const int y = (int)cy + dy;
                 if (y < 0 || y >= (int)gy) continue;
-                const uint32_t row = ((uint32_t)z * gy + (uint32_t)y) * gx;
+                const uint32_t row = base + ((uint32_t)z * gy + (uint32_t)y) * gx;
                 const uint32_t j0 = b.cell_off[row + x0];
                 const uint32_t j1 = min(b.cell_off[row + x1 + 1], n);
                 for (uint32_t jb = j0; jb < j1; jb += 4) {
@@ -519,35 +539,65 @@ __device__ __forceinline__ float ipow(float x, uint32_t n)
 }
 
 // a8: world box (collide_world_functor, integration_kernel.cuh:57-157).
-__device__ __forceinline__ void walls(const KConst &k, int32_t ph, uint32_t o, float4 xs,
+struct WallBox {
+    float lo[3], hi[3];
+    uint64_t seed;
+    uint32_t o;  // index of the particle inside its scene
+};
+
+__device__ __forceinline__ WallBox wall_box(const KConst &k, const DevBuffers &b, uint32_t i,
+                                            uint32_t orig)
+{
+    WallBox w;
+    if (k.nscenes > 1u) {
+        const SceneDev &sc = b.scenes[b.sid[i]];
+        for (int a = 0; a < 3; a++) {
+            w.lo[a] = sc.box_min[a];
+            w.hi[a] = sc.box_max[a];
+        }
+        w.seed = sc.seed;
+        w.o = orig - sc.first;
+    } else {
+        for (int a = 0; a < 3; a++) {
+            w.lo[a] = k.box_min[a];
+            w.hi[a] = k.box_max[a];
+        }
+        w.seed = k.seed;
+        w.o = orig;
+    }
+    return w;
+}
+
+__device__ __forceinline__ void walls(const KConst &k, const WallBox &wb, int32_t ph, float4 xs,
                                       float &px, float &py, float &pz, uint64_t iter)
 {
     const float r = k.radius;
+    const uint32_t o = wb.o;
     float nx = 0.0f, ny = 0.0f, nz = 0.0f;
     const float efl = ph < PS_PHASE_SOLID ? k.jitter : 0.0f;
-    if (py < k.box_min[1] + r) {
-        py = (k.box_min[1] + r) + (wall_uniform(k.seed, iter, o, 5) * efl);
+    if (py < wb.lo[1] + r) {
+        py = (wb.lo[1] + r) + (wall_uniform(wb.seed, iter, o, 5) * efl);
         ny = ny + 1.0f;
     }
-    if (px > k.box_max[0] - r) {
-        px = k.box_max[0] - (r + (wall_uniform(k.seed, iter, o, 0) * k.jitter));
+    if (px > wb.hi[0] - r) {
+        px = wb.hi[0] - (r + (wall_uniform(wb.seed, iter, o, 0) * k.jitter));
         nx = nx - 1.0f;
     }
-    if (px < k.box_min[0] + r) {
-        px = k.box_min[0] + (r + (wall_uniform(k.seed, iter, o, 1) * k.jitter));
+    if (px < wb.lo[0] + r) {
+        px = wb.lo[0] + (r + (wall_uniform(wb.seed, iter, o, 1) * k.jitter));
         nx = nx + 1.0f;
     }
-    if (py > k.box_max[1] - r) {
-        py = k.box_max[1] - (r + (wall_uniform(k.seed, iter, o, 2) * k.jitter));
+    if (py > wb.hi[1] - r) {
+        py = wb.hi[1] - (r + (wall_uniform(wb.seed, iter, o, 2) * k.jitter));
         ny = ny - 1.0f;
     }
     if (k.dims != 2u) {
-        if (pz > k.box_max[2] - r) {
-            pz = k.box_max[2] - (r + (wall_uniform(k.seed, iter, o, 3) * k.jitter));
+        if (pz > wb.hi[2] - r) {
+            pz = wb.hi[2] - (r + (wall_uniform(wb.seed, iter, o, 3) * k.jitter));
             nz = nz - 1.0f;
         }
-        if (pz < k.box_min[2] + r) {
-            pz = k.box_min[2] + (r + (wall_uniform(k.seed, iter, o, 4) * k.jitter));
+        if (pz < wb.lo[2] + r) {
+            pz = wb.lo[2] + (r + (wall_uniform(wb.seed, iter, o, 4) * k.jitter));
             nz = nz + 1.0f;
         }
     }
@@ -578,7 +628,7 @@ __device__ __forceinline__ void walls(const KConst &k, int32_t ph, uint32_t o, f
 
 template <uint32_t F, bool UNIFORM, uint32_t NC>
 __global__ __launch_bounds__(kBlock) void solve_k(DevBuffers b, KConst k, uint32_t n, float wu,
-                                                  uint64_t iter)
+                                                  uint32_t it)
 {
     const uint32_t i = xcd_block() * kBlock + threadIdx.x;
     if (i >= n) return;
@@ -755,7 +805,8 @@ __global__ __launch_bounds__(kBlock) void solve_k(DevBuffers b, KConst k, uint32
     float nxp = px + ax, nyp = py + ay, nzp = pz + az;
     if (k.dims == 2u) nzp = pz;
     const float4 xs = (ph >= PS_PHASE_CLOTH) ? b.X[i] : make_float4(0.f, 0.f, 0.f, 0.f);
-    walls(k, ph, a.orig, xs, nxp, nyp, nzp, iter);
+    const uint64_t iter = b.status->iter + it;
+    walls(k, wall_box(k, b, i, a.orig), ph, xs, nxp, nyp, nzp, iter);
     if (F & F_PIN) {
         const uint32_t pin = b.pin_of[a.orig];
         if (pin) {
@@ -864,9 +915,10 @@ __global__ __launch_bounds__(kBlock) void shape_k(DevBuffers b, uint32_t nbody)
 
 template <bool UNIFORM>
 __global__ __launch_bounds__(kBlock) void finalize_k(DevBuffers b, KConst k, uint32_t n, float mu,
-                                                     float dt)
+                                                     float dt, uint32_t its)
 {
     const uint32_t i = xcd_block() * kBlock + threadIdx.x;
+    if (i == 0) b.status->iter += its;  // every solve of this substep has completed
     if (i >= n) return;
     const float4 p = b.P[i];
     const float4 x = b.X[i];
@@ -1010,7 +1062,7 @@ hipError_t launch_shape(hipStream_t st, const DevBuffers &b, uint32_t nbody)
 // E_P, integration_kernel.cuh:34) or 0 = read from the parameters
 template <uint32_t F>
 static void solve_variant(hipStream_t st, const DevBuffers &b, const KConst &k, uint32_t n,
-                          bool uniform, float wu, uint64_t iter)
+                          bool uniform, float wu, uint32_t iter)
 {
     const dim3 g(blocks_for(n)), t(kBlock);
     const bool n4 = k.n_corr == 4u;
@@ -1025,7 +1077,7 @@ static void solve_variant(hipStream_t st, const DevBuffers &b, const KConst &k, 
 }
 
 hipError_t launch_solve(hipStream_t st, const DevBuffers &b, const KConst &k, uint32_t n,
-                        uint32_t features, bool uniform, float wu, uint64_t iter)
+                        uint32_t features, bool uniform, float wu, uint32_t iter)
 {
     if (n == 0) return hipSuccess;
     // uniform-fluid scenes have no contacts, constraints or bodies
@@ -1040,15 +1092,15 @@ hipError_t launch_solve(hipStream_t st, const DevBuffers &b, const KConst &k, ui
 }
 
 hipError_t launch_finalize(hipStream_t st, const DevBuffers &b, const KConst &k, uint32_t n,
-                           float dt, bool sleeping, bool uniform, float wu)
+                           float dt, bool sleeping, bool uniform, float wu, uint32_t its)
 {
     if (n == 0) return hipSuccess;
     const dim3 g(blocks_for(n)), t(kBlock);
     const float mu = 1.0f / wu;
     if (uniform)
-        hipLaunchKernelGGL((finalize_k<true>), g, t, 0, st, b, k, n, mu, dt);
+        hipLaunchKernelGGL((finalize_k<true>), g, t, 0, st, b, k, n, mu, dt, its);
     else
-        hipLaunchKernelGGL((finalize_k<false>), g, t, 0, st, b, k, n, mu, dt);
+        hipLaunchKernelGGL((finalize_k<false>), g, t, 0, st, b, k, n, mu, dt, its);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || !sleeping) return e;
     hipLaunchKernelGGL(sleep_k, g, t, 0, st, b, n);

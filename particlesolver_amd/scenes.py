@@ -264,3 +264,103 @@ def two_d_fluid(nx=40, ny=20, spacing=1.0, h=2.0, iterations=3, seed=0x5EED0001,
     S = lattice_rest_density(h, spacing, dims=2)
     return Scene(p, pos, np.ones(n, np.float32), np.zeros(n, np.int32),
                  np.full(n, S, np.float32), dt, name="fluid2d")
+
+
+class Batch:
+    """Scenes for ps_create_batch: a shared parameter set, one PsSceneDesc per
+    scene, and each scene's particles/constraints (global indices are
+    assigned when the batch is loaded, scenes filled in order)."""
+
+    def __init__(self, params, descs, scenes, dt):
+        self.params = params
+        self.descs = descs
+        self.scenes = scenes       # list of Scene (local indices)
+        self.dt = dt
+
+    @property
+    def n(self):
+        return sum(s.n for s in self.scenes)
+
+    def load_into(self, sys_):
+        first = 0
+        for q, s in enumerate(self.scenes):
+            sys_.add_particles(s.pos, s.vel, s.inv_mass, s.phase, s.rest_density, scene=q)
+            if s.distance is not None:
+                pairs, rest, st = s.distance
+                sys_.add_distance_constraints(pairs + np.uint32(first), rest, st)
+            if s.pins is not None:
+                idx, xyz = s.pins
+                sys_.add_point_constraints(idx + np.uint32(first), xyz)
+            first += s.n
+
+    def plain(self, q):
+        """Scene q as a single-scene system (its own params and box)."""
+        s = self.scenes[q]
+        d = self.descs[q]
+        p = self.params.copy()
+        p.max_particles = s.n
+        p.box_min[:] = list(d.box_min)
+        p.box_max[:] = list(d.box_max)
+        p.origin[:] = list(d.box_min)
+        for a in range(3):
+            ext = float(np.float32(d.box_max[a])) - float(np.float32(d.box_min[a]))
+            p.grid[a] = max(1, int(math.ceil(ext / float(np.float32(p.cell)) - 1e-9)))
+        g = np.float32(self.params.gravity[:]) * np.float32(d.gravity_scale)
+        p.gravity[:] = [float(v) for v in g]
+        p.seed = d.seed
+        return Scene(p, s.pos, s.inv_mass, s.phase, s.rest_density, self.dt, vel=s.vel,
+                     name="%s_plain_%d" % (s.name, q)) if s.distance is None else \
+            _with_constraints(Scene(p, s.pos, s.inv_mass, s.phase, s.rest_density, self.dt,
+                                    vel=s.vel), s)
+
+
+def _with_constraints(dst, src):
+    dst.distance = src.distance
+    dst.pins = src.pins
+    return dst
+
+
+def ensemble(nscenes=512, nmin=2048, nmax=65536, links=64, spacing=0.1, iterations=4,
+             max_neighbors=96, seed=0x5EED0004, dt=1.0 / 60.0):
+    """Batched ensemble: nscenes independent scenes, particle counts
+    log-uniform in [nmin, nmax] (rounded down), each a fluid block (rest
+    density U[0.8, 1.2] times the lattice kernel sum) plus one `links`-link
+    rope pinned at one end (stiffness U[0.3, 1.0]); gravity scale U[0.5, 1.5]."""
+    h = 2.0 * spacing
+    radius = 0.5 * spacing
+    S = lattice_rest_density(h, spacing)
+    u = uniform24(seed, 4 * nscenes).reshape(nscenes, 4)
+    counts = np.floor(np.exp(np.log(nmin) + u[:, 0] * (np.log(nmax) - np.log(nmin)))).astype(np.int64)
+    scenes, descs = [], []
+    nrope = links + 1
+    for q in range(nscenes):
+        nf = int(counts[q]) - nrope
+        nx = max(2, int(round((2.0 * nf) ** (1.0 / 3.0))))
+        ny = max(1, int(math.ceil(nf / float(nx * nx))))
+        blk = fluid_block(nx, ny, nx, spacing, (0.0, 0.0, 0.0), 1e-3, seed + 17 * (q + 1))[:nf]
+        bx, by = nx * spacing, ny * spacing
+        ext = (max(2.0 * bx, links * spacing + 1.0), max(2.0 * by, 2.0) + 1.0, 2.0 * bx)
+        top = ext[1] - 0.5
+        rope = np.stack([0.5 + np.arange(nrope) * spacing, np.full(nrope, top),
+                         np.full(nrope, 0.5 * ext[2])], 1).astype(np.float32)
+        pos = np.concatenate([blk, rope]).astype(np.float32)
+        n = pos.shape[0]
+        rho = np.float32((0.8 + 0.4 * u[q, 1]) * S)
+        r0 = np.concatenate([np.full(nf, rho, np.float32), np.ones(nrope, np.float32)])
+        ph = np.concatenate([np.zeros(nf, np.int32), np.full(nrope, PHASE_RIGID0, np.int32)])
+        sc = Scene(None, pos, np.ones(n, np.float32), ph, r0, dt, name="scene%d" % q)
+        pairs = np.stack([np.arange(nf, nf + links), np.arange(nf + 1, nf + nrope)], 1).astype(np.uint32)
+        sc.distance = (pairs, np.full(links, spacing, np.float32),
+                       np.full(links, 0.3 + 0.7 * u[q, 2], np.float32))
+        sc.pins = (np.array([nf], np.uint32), rope[:1].copy())
+        scenes.append(sc)
+        d = capi.PsSceneDesc()
+        d.box_min[:] = [0.0, 0.0, 0.0]
+        d.box_max[:] = list(ext)
+        d.gravity_scale = 0.5 + u[q, 3]
+        d.seed = seed + q
+        descs.append(d)
+    total = int(sum(s.n for s in scenes))
+    p = pbf_params((0.0, 0.0, 0.0), (1.0, 1.0, 1.0), h, radius, iterations, total, max_neighbors,
+                   seed=seed)
+    return Batch(p, descs, scenes, dt)
