@@ -321,6 +321,19 @@ int ps_timing_reset(ps_system *sys);
 int ps_timing_read(ps_system *sys, int kernel_id, double *total_ms, uint64_t *launches);
 const char *ps_kernel_name(int kernel_id);
 
+/* ---- autotuning -------------------------------------------------------- */
+
+/* Choose the workgroup size (64, 128, 256 or 512) of each per-particle
+ * kernel (predict_hash, reorder, neighbors, lambda, solve, finalize) by
+ * timing `substeps` substeps of this system per candidate with events.  The
+ * simulation state is restored afterwards.  Choices are cached per
+ * (kernel, log2 particle count, scene composition) for the process and, if
+ * the environment names a file in PS_AUTOTUNE_CACHE, across processes.
+ * blocks_out (PS_K_COUNT entries, may be NULL) receives the sizes in use. */
+int ps_autotune(ps_system *sys, float dt, uint32_t substeps, uint32_t *blocks_out);
+int ps_get_tuning(const ps_system *sys, uint32_t *blocks_out);
+int ps_set_tuning(ps_system *sys, const uint32_t *blocks);
+
 #ifdef __cplusplus
 }
 #endif

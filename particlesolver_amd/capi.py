@@ -154,6 +154,9 @@ PROTOTYPES = {
     "ps_timing_read": (_int, [_P, _int, ctypes.POINTER(ctypes.c_double),
                               ctypes.POINTER(ctypes.c_uint64)]),
     "ps_kernel_name": (ctypes.c_char_p, [_int]),
+    "ps_autotune": (_int, [_P, _f32, _u32, _P]),
+    "ps_get_tuning": (_int, [_P, _P]),
+    "ps_set_tuning": (_int, [_P, _P]),
 }
 
 _lib = None
@@ -370,6 +373,21 @@ class System:
 
     def clear_status(self):
         self._check(self.lib.ps_clear_status(self.h))
+
+    # -- tuning
+    def autotune(self, dt, substeps=2):
+        out = np.zeros(len(KERNELS), np.uint32)
+        self._check(self.lib.ps_autotune(self.h, dt, substeps, _ptr(out)))
+        return dict(zip(KERNELS, out.tolist()))
+
+    def tuning(self):
+        out = np.zeros(len(KERNELS), np.uint32)
+        self._check(self.lib.ps_get_tuning(self.h, _ptr(out)))
+        return dict(zip(KERNELS, out.tolist()))
+
+    def set_tuning(self, blocks):
+        arr = np.array([blocks.get(k, 256) for k in KERNELS], np.uint32)
+        self._check(self.lib.ps_set_tuning(self.h, _ptr(arr)))
 
     # -- timing
     def timing(self, kernels=True):

@@ -10,6 +10,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <map>
 #include <string>
 #include <vector>
 
@@ -63,6 +64,8 @@ struct ps_system {
     bool nbr_valid = false; // neighbour table matches the current storage
     float dt = 0.0f;
     uint64_t substeps = 0, iters = 0;
+    // workgroup size of each kernel (ps_autotune)
+    uint32_t block[PS_K_COUNT] = {256, 256, 256, 256, 256, 256, 256, 256, 256};
     // timing
     uint32_t timing = 0;  // kernel mask
     std::vector<TimedLaunch> pending;
@@ -342,7 +345,8 @@ bool uniform_fluid(const ps_system *s) { return s->uniform && features(s) == ps:
 
 int stage_hash(ps_system *s, bool predict, float dt)
 {
-    LAUNCH(s, PS_K_PREDICT_HASH, ps::launch_predict_hash(s->stream, s->b, s->kc, s->n, dt, predict, true));
+    LAUNCH(s, PS_K_PREDICT_HASH, ps::launch_predict_hash(s->stream, s->b, s->kc, s->n, dt, predict, true,
+                                                         s->block[PS_K_PREDICT_HASH]));
     s->hashed = true;
     return PS_OK;
 }
@@ -352,7 +356,7 @@ int stage_sort(ps_system *s)
 {
     LAUNCH(s, PS_K_SCAN, ps::launch_scan(s->stream, s->b, s->ncell));
     LAUNCH(s, PS_K_SCATTER, ps::launch_scatter(s->stream, s->b, s->n));
-    LAUNCH(s, PS_K_REORDER, ps::launch_reorder(s->stream, s->b, s->n));
+    LAUNCH(s, PS_K_REORDER, ps::launch_reorder(s->stream, s->b, s->n, s->block[PS_K_REORDER]));
     DevBuffers &b = s->b;
     std::swap(b.X, b.X2);
     std::swap(b.P, b.Q);
@@ -366,7 +370,8 @@ int stage_sort(ps_system *s)
 
 int stage_neighbors(ps_system *s)
 {
-    LAUNCH(s, PS_K_NEIGHBORS, ps::launch_neighbors(s->stream, s->b, s->kc, s->n, s->kmax));
+    LAUNCH(s, PS_K_NEIGHBORS,
+           ps::launch_neighbors(s->stream, s->b, s->kc, s->n, s->kmax, s->block[PS_K_NEIGHBORS]));
     s->nbr_valid = true;
     return PS_OK;
 }
@@ -381,11 +386,13 @@ int stage_iterate(ps_system *s)
     }
     if (!s->nbr_valid) return fail(s, PS_ERR_STATE, "neighbour table is stale: call ps_find_neighbors");
     const bool uni = uniform_fluid(s);
-    LAUNCH(s, PS_K_LAMBDA, ps::launch_lambda(s->stream, s->b, s->kc, s->n, uni, s->wu));
+    LAUNCH(s, PS_K_LAMBDA,
+           ps::launch_lambda(s->stream, s->b, s->kc, s->n, uni, s->wu, s->block[PS_K_LAMBDA]));
     const uint32_t nbody = (uint32_t)s->body_off.size() - 1;
     if (nbody) LAUNCH(s, PS_K_SHAPE, ps::launch_shape(s->stream, s->b, nbody));
     LAUNCH(s, PS_K_SOLVE,
-           ps::launch_solve(s->stream, s->b, s->kc, s->n, features(s), uni, s->wu, s->it_sub));
+           ps::launch_solve(s->stream, s->b, s->kc, s->n, features(s), uni, s->wu, s->it_sub,
+                            s->block[PS_K_SOLVE]));
     std::swap(s->b.P, s->b.Q);
     s->iters++;
     s->it_sub++;
@@ -397,7 +404,8 @@ int stage_end(ps_system *s)
     const bool uni = uniform_fluid(s);
     LAUNCH(s, PS_K_FINALIZE,
            ps::launch_finalize(s->stream, s->b, s->kc, s->n, s->dt,
-                               (s->prm.flags & PS_FLAG_SLEEPING) != 0, uni, s->wu, s->it_sub));
+                               (s->prm.flags & PS_FLAG_SLEEPING) != 0, uni, s->wu, s->it_sub,
+                               s->block[PS_K_FINALIZE]));
     s->it_sub = 0;
     std::swap(s->b.X, s->b.P);
     s->substeps++;
@@ -1026,7 +1034,8 @@ int ps_predict(ps_system *s, float dt)
 {
     if (int rc = check_sys(s)) return rc;
     if (!is_finite(dt) || dt < 0.0f) return fail(s, PS_ERR_INVALID_ARGUMENT, "dt must be finite and >= 0");
-    LAUNCH(s, PS_K_PREDICT_HASH, ps::launch_predict_hash(s->stream, s->b, s->kc, s->n, dt, true, false));
+    LAUNCH(s, PS_K_PREDICT_HASH, ps::launch_predict_hash(s->stream, s->b, s->kc, s->n, dt, true,
+                                                         false, s->block[PS_K_PREDICT_HASH]));
     s->dt = dt;
     s->stage = 1;
     s->it_sub = 0;
@@ -1313,6 +1322,192 @@ int ps_timing_read(ps_system *s, int kernel_id, double *total_ms, uint64_t *laun
     if (int rc = timing_collect(s)) return rc;
     if (total_ms) *total_ms = s->total_ms[kernel_id];
     if (launches) *launches = s->launches[kernel_id];
+    return PS_OK;
+}
+
+// ---- autotuning ------------------------------------------------------------
+
+namespace {
+
+const int kTunable[] = {PS_K_PREDICT_HASH, PS_K_REORDER, PS_K_NEIGHBORS, PS_K_LAMBDA, PS_K_SOLVE,
+                        PS_K_FINALIZE};
+const uint32_t kCandidates[] = {64, 128, 256, 512};
+
+// (kernel, log2 N bucket, scene features) -> block size, shared by the
+// handles of this process and, with PS_AUTOTUNE_CACHE=<file>, across runs
+struct TuneKey {
+    int kernel;
+    uint32_t bucket, features;
+    bool operator<(const TuneKey &o) const
+    {
+        return kernel != o.kernel ? kernel < o.kernel
+                                  : (bucket != o.bucket ? bucket < o.bucket : features < o.features);
+    }
+};
+std::map<TuneKey, uint32_t> g_tune;
+bool g_tune_loaded = false;
+
+void tune_cache_load()
+{
+    if (g_tune_loaded) return;
+    g_tune_loaded = true;
+    const char *path = getenv("PS_AUTOTUNE_CACHE");
+    if (!path) return;
+    FILE *f = fopen(path, "r");
+    if (!f) return;
+    int k;
+    unsigned b, ft, bs;
+    while (fscanf(f, "%d %u %u %u", &k, &b, &ft, &bs) == 4)
+        if (bs >= 64 && bs <= 512 && bs % 64 == 0 && k >= 0 && k < PS_K_COUNT) g_tune[{k, b, ft}] = bs;
+    fclose(f);
+}
+
+void tune_cache_store()
+{
+    const char *path = getenv("PS_AUTOTUNE_CACHE");
+    if (!path) return;
+    FILE *f = fopen(path, "w");
+    if (!f) return;
+    for (const auto &kv : g_tune)
+        fprintf(f, "%d %u %u %u\n", kv.first.kernel, kv.first.bucket, kv.first.features, kv.second);
+    fclose(f);
+}
+
+uint32_t log2_bucket(uint32_t n)
+{
+    uint32_t b = 0;
+    while ((1u << (b + 1)) <= n && b < 31) b++;
+    return b;
+}
+
+}  // namespace
+
+int ps_autotune(ps_system *s, float dt, uint32_t substeps, uint32_t *blocks_out)
+{
+    if (int rc = check_sys(s)) return rc;
+    if (s->stage) return fail(s, PS_ERR_STATE, "ps_autotune inside an open substep");
+    if (!(dt > 0.0f) || !is_finite(dt)) return fail(s, PS_ERR_INVALID_ARGUMENT, "dt must be > 0");
+    if (substeps == 0) substeps = 2;
+    if (s->n == 0) return fail(s, PS_ERR_STATE, "autotuning needs particles");
+    tune_cache_load();
+    const uint32_t bucket = log2_bucket(s->n), feat = features(s) | (s->uniform ? 0x100u : 0u);
+    bool need = false;
+    for (int k : kTunable) {
+        auto it = g_tune.find({k, bucket, feat});
+        if (it != g_tune.end())
+            s->block[k] = it->second;
+        else
+            need = true;
+    }
+    if (need) {
+        // snapshot the state the trials will advance
+        const uint32_t n = s->n, nbody = (uint32_t)s->body_off.size() - 1;
+        float4 *sx = nullptr, *sv = nullptr, *sq = nullptr;
+        Attr *sa = nullptr;
+        uint32_t *ss = nullptr, *ssid = nullptr;
+        ps::DevStatus *sst = nullptr;
+        int rc = PS_OK;
+        auto cleanup = [&]() {
+            dev_free(sx); dev_free(sv); dev_free(sq); dev_free(sa); dev_free(ss); dev_free(ssid);
+            dev_free(sst);
+        };
+        if ((rc = dev_alloc(s, &sx, n)) || (rc = dev_alloc(s, &sv, n)) || (rc = dev_alloc(s, &sa, n)) ||
+            (rc = dev_alloc(s, &ss, n)) || (rc = dev_alloc(s, &sst, 1)) ||
+            (nbody && (rc = dev_alloc(s, &sq, nbody))) || (s->b.sid && (rc = dev_alloc(s, &ssid, n)))) {
+            cleanup();
+            return rc;
+        }
+        DevBuffers &b = s->b;
+        hipError_t e = hipStreamSynchronize(s->stream);
+        auto cp = [&](void *dst, const void *src, size_t bytes) {
+            if (e == hipSuccess && bytes) e = hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice);
+        };
+        cp(sx, b.X, sizeof(float4) * n);
+        cp(sv, b.V, sizeof(float4) * n);
+        cp(sa, b.A, sizeof(Attr) * n);
+        cp(ss, b.slot_of, sizeof(uint32_t) * n);
+        cp(sst, b.status, sizeof(ps::DevStatus));
+        if (nbody) cp(sq, b.body_q, sizeof(float4) * nbody);
+        if (b.sid) cp(ssid, b.sid, sizeof(uint32_t) * n);
+        if (e != hipSuccess) {
+            cleanup();
+            return fail(s, PS_ERR_DEVICE, std::string("autotune snapshot: ") + hipGetErrorString(e));
+        }
+        const uint64_t sub0 = s->substeps, it0 = s->iters;
+        const uint32_t timing0 = s->timing;
+        double ms0[PS_K_COUNT];
+        uint64_t l0[PS_K_COUNT];
+        if ((rc = timing_collect(s))) { cleanup(); return rc; }
+        memcpy(ms0, s->total_ms, sizeof(ms0));
+        memcpy(l0, s->launches, sizeof(l0));
+        for (int k : kTunable) {
+            if (g_tune.count({k, bucket, feat})) continue;
+            uint32_t best = s->block[k];
+            double best_ms = 1e30;
+            for (uint32_t bs : kCandidates) {
+                s->block[k] = bs;
+                s->timing = 0;
+                if ((rc = ps_step(s, dt, 1))) break;  // warm-up of this shape
+                s->timing = 1u << k;
+                s->total_ms[k] = 0.0;
+                s->launches[k] = 0;
+                if ((rc = ps_step(s, dt, substeps))) break;
+                if ((rc = timing_collect(s))) break;
+                const double ms = s->launches[k] ? s->total_ms[k] / (double)s->launches[k] : 1e30;
+                if (ms < best_ms) {
+                    best_ms = ms;
+                    best = bs;
+                }
+            }
+            s->block[k] = best;
+            if (rc) break;
+            g_tune[{k, bucket, feat}] = best;
+        }
+        s->timing = timing0;
+        memcpy(s->total_ms, ms0, sizeof(ms0));
+        memcpy(s->launches, l0, sizeof(l0));
+        // restore the snapshot
+        e = hipStreamSynchronize(s->stream);
+        cp(b.X, sx, sizeof(float4) * n);
+        cp(b.V, sv, sizeof(float4) * n);
+        cp(b.A, sa, sizeof(Attr) * n);
+        cp(b.slot_of, ss, sizeof(uint32_t) * n);
+        cp(b.status, sst, sizeof(ps::DevStatus));
+        if (nbody) cp(b.body_q, sq, sizeof(float4) * nbody);
+        if (b.sid) cp(b.sid, ssid, sizeof(uint32_t) * n);
+        cleanup();
+        s->substeps = sub0;
+        s->iters = it0;
+        s->stage = 0;
+        s->it_sub = 0;
+        s->hashed = false;
+        s->nbr_valid = false;
+        if (rc) return rc;
+        if (e != hipSuccess)
+            return fail(s, PS_ERR_DEVICE, std::string("autotune restore: ") + hipGetErrorString(e));
+        tune_cache_store();
+    }
+    drop_graph(s);
+    if (blocks_out) memcpy(blocks_out, s->block, sizeof(s->block));
+    return PS_OK;
+}
+
+int ps_get_tuning(const ps_system *s, uint32_t *blocks_out)
+{
+    if (!s || !blocks_out) return fail(nullptr, PS_ERR_INVALID_ARGUMENT, "NULL argument");
+    memcpy(blocks_out, s->block, sizeof(s->block));
+    return PS_OK;
+}
+
+int ps_set_tuning(ps_system *s, const uint32_t *blocks)
+{
+    if (int rc = check_sys(s)) return rc;
+    if (!blocks) return fail(s, PS_ERR_INVALID_ARGUMENT, "NULL argument");
+    for (int k : kTunable)
+        if (blocks[k] < 64 || blocks[k] > 512 || blocks[k] % 64)
+            return fail(s, PS_ERR_INVALID_ARGUMENT, "block sizes must be 64..512, multiples of 64");
+    for (int k : kTunable) s->block[k] = blocks[k];
+    drop_graph(s);
     return PS_OK;
 }
 

@@ -112,22 +112,24 @@ struct DevBuffers {
 
 // Kernel launchers (ps_kernels.hip).  All enqueue on `st` and return the
 // launch error.
+// bs: workgroup size (64..512, a multiple of 64) of the tunable kernels
 hipError_t launch_predict_hash(hipStream_t st, const DevBuffers &b, const KConst &k, uint32_t n,
-                               float dt, bool predict, bool hash);
+                               float dt, bool predict, bool hash, uint32_t bs);
 hipError_t launch_scan(hipStream_t st, const DevBuffers &b, uint32_t num_cells);
 hipError_t launch_scatter(hipStream_t st, const DevBuffers &b, uint32_t n);
-hipError_t launch_reorder(hipStream_t st, const DevBuffers &b, uint32_t n);
+hipError_t launch_reorder(hipStream_t st, const DevBuffers &b, uint32_t n, uint32_t bs);
 hipError_t launch_neighbors(hipStream_t st, const DevBuffers &b, const KConst &k, uint32_t n,
-                            uint32_t kmax);
+                            uint32_t kmax, uint32_t bs);
 hipError_t launch_lambda(hipStream_t st, const DevBuffers &b, const KConst &k, uint32_t n,
-                         bool uniform, float wu);
+                         bool uniform, float wu, uint32_t bs);
 hipError_t launch_shape(hipStream_t st, const DevBuffers &b, uint32_t nbody);
 // it: iteration index inside the current substep (the RNG key is the device
 // iteration counter + it); finalize advances the counter by `its`.
 hipError_t launch_solve(hipStream_t st, const DevBuffers &b, const KConst &k, uint32_t n,
-                        uint32_t features, bool uniform, float wu, uint32_t it);
+                        uint32_t features, bool uniform, float wu, uint32_t it, uint32_t bs);
 hipError_t launch_finalize(hipStream_t st, const DevBuffers &b, const KConst &k, uint32_t n,
-                           float dt, bool sleeping, bool uniform, float wu, uint32_t its);
+                           float dt, bool sleeping, bool uniform, float wu, uint32_t its,
+                           uint32_t bs);
 
 uint32_t scan_tiles(uint32_t num_cells);
 

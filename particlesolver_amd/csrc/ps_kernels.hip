@@ -24,7 +24,8 @@
 
 namespace ps {
 
-static constexpr uint32_t kBlock = 256;
+static constexpr uint32_t kBlock = 256;     // fixed-shape kernels (scan, scatter, shape)
+static constexpr uint32_t kMaxBlock = 512;  // tunable kernels run with 64..512 threads
 static constexpr uint32_t kScanPerThread = 16;
 static constexpr uint32_t kScanTile = kBlock * kScanPerThread;
 
@@ -142,10 +143,10 @@ __device__ __forceinline__ uint32_t cell_axis(float p, float o, float cell, uint
 }
 
 template <bool PRED, bool HASH>
-__global__ __launch_bounds__(kBlock) void predict_hash_k(DevBuffers b, KConst k, uint32_t n,
+__global__ __launch_bounds__(kMaxBlock) void predict_hash_k(DevBuffers b, KConst k, uint32_t n,
                                                          float dt)
 {
-    const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     const bool valid = s < n;
     const bool multi = k.nscenes > 1u;
     const SceneDev *sc = (multi && valid) ? &b.scenes[b.sid[s]] : nullptr;
@@ -320,9 +321,9 @@ __global__ __launch_bounds__(kBlock) void scatter_k(DevBuffers b, uint32_t n)
 // index): a stable sort of insertion-ordered input, as thrust::sort_by_key
 // of (hash, index) in sortParticles (integration.cu:270-275) is meant to
 // give.  Every per-particle array moves with the particle.
-__global__ __launch_bounds__(kBlock) void reorder_k(DevBuffers b, uint32_t n)
+__global__ __launch_bounds__(kMaxBlock) void reorder_k(DevBuffers b, uint32_t n)
 {
-    const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     const uint32_t c = b.key[s];
     const Attr a = b.A[s];
@@ -349,11 +350,11 @@ __global__ __launch_bounds__(kBlock) void reorder_k(DevBuffers b, uint32_t n)
 // 303-356,481-518): every j != i with r^2 < search_r2, scanning the 3x3 rows
 // of x-adjacent cells; a row's three cells are one contiguous slot range.
 
-__global__ __launch_bounds__(kBlock) void neighbors_k(DevBuffers b, KConst k, uint32_t n,
+__global__ __launch_bounds__(kMaxBlock) void neighbors_k(DevBuffers b, KConst k, uint32_t n,
                                                       uint32_t kmax)
 {
-    __shared__ uint32_t lds4[4];
-    const uint32_t i = xcd_block() * kBlock + threadIdx.x;
+    __shared__ uint32_t ldsw[kMaxBlock / 64];
+    const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
     uint32_t cnt = 0;
     if (i < n) {
         uint32_t gx = k.grid[0], gy = k.grid[1], gz = k.grid[2], base = 0;
@@ -415,11 +416,11 @@ __global__ __launch_bounds__(kBlock) void neighbors_k(DevBuffers b, KConst k, ui
     // one status update per block
     uint32_t m = wave_max(cnt);
     const uint32_t w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) lds4[w] = m;
+    if ((threadIdx.x & 63) == 0) ldsw[w] = m;
     __syncthreads();
     if (threadIdx.x == 0) {
-        uint32_t mm = lds4[0];
-        for (int q = 1; q < 4; q++) mm = lds4[q] > mm ? lds4[q] : mm;
+        uint32_t mm = ldsw[0];
+        for (uint32_t q = 1; q < (blockDim.x >> 6); q++) mm = ldsw[q] > mm ? ldsw[q] : mm;
         if (mm > 0) atomicMax(&b.status->max_neighbors, mm);
         if (mm > kmax) atomicOr(&b.status->flags, (uint32_t)PS_STATUS_NEIGHBOR_OVERFLOW);
     }
@@ -432,9 +433,9 @@ __global__ __launch_bounds__(kBlock) void neighbors_k(DevBuffers b, KConst k, ui
 // UNIFORM: every particle is fluid with inverse mass wu (no Attr gathers).
 
 template <bool UNIFORM>
-__global__ __launch_bounds__(kBlock) void lambda_k(DevBuffers b, KConst k, uint32_t n, float mu)
+__global__ __launch_bounds__(kMaxBlock) void lambda_k(DevBuffers b, KConst k, uint32_t n, float mu)
 {
-    const uint32_t i = xcd_block() * kBlock + threadIdx.x;
+    const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= n) return;
     float *Pw = reinterpret_cast<float *>(b.P);
     const Attr a = b.A[i];
@@ -627,10 +628,10 @@ __device__ __forceinline__ void walls(const KConst &k, const WallBox &wb, int32_
 // snapshot P, then walls (a8) and pins (a10); result to Q.
 
 template <uint32_t F, bool UNIFORM, uint32_t NC>
-__global__ __launch_bounds__(kBlock) void solve_k(DevBuffers b, KConst k, uint32_t n, float wu,
+__global__ __launch_bounds__(kMaxBlock) void solve_k(DevBuffers b, KConst k, uint32_t n, float wu,
                                                   uint32_t it)
 {
-    const uint32_t i = xcd_block() * kBlock + threadIdx.x;
+    const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float4 p = b.P[i];
     const Attr a = b.A[i];
@@ -914,10 +915,10 @@ __global__ __launch_bounds__(kBlock) void shape_k(DevBuffers b, uint32_t nbody)
 //   v_i = u_i/dt + ((c/rho0_i) sum_j (u_j - u_i) m_j W_ij)/dt
 
 template <bool UNIFORM>
-__global__ __launch_bounds__(kBlock) void finalize_k(DevBuffers b, KConst k, uint32_t n, float mu,
+__global__ __launch_bounds__(kMaxBlock) void finalize_k(DevBuffers b, KConst k, uint32_t n, float mu,
                                                      float dt, uint32_t its)
 {
-    const uint32_t i = xcd_block() * kBlock + threadIdx.x;
+    const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
     if (i == 0) b.status->iter += its;  // every solve of this substep has completed
     if (i >= n) return;
     const float4 p = b.P[i];
@@ -986,15 +987,15 @@ __global__ __launch_bounds__(kBlock) void sleep_k(DevBuffers b, uint32_t n)
 // ---------------------------------------------------------------------------
 // launchers
 
-static inline uint32_t blocks_for(uint32_t n) { return (n + kBlock - 1) / kBlock; }
+static inline uint32_t blocks_for(uint32_t n, uint32_t bs = kBlock) { return (n + bs - 1) / bs; }
 
 uint32_t scan_tiles(uint32_t num_cells) { return (num_cells + kScanTile - 1) / kScanTile; }
 
 hipError_t launch_predict_hash(hipStream_t st, const DevBuffers &b, const KConst &k, uint32_t n,
-                               float dt, bool predict, bool hash)
+                               float dt, bool predict, bool hash, uint32_t bs)
 {
     if (n == 0) return hipSuccess;
-    const dim3 g(blocks_for(n)), t(kBlock);
+    const dim3 g(blocks_for(n, bs)), t(bs);
     if (predict && hash)
         hipLaunchKernelGGL((predict_hash_k<true, true>), g, t, 0, st, b, k, n, dt);
     else if (predict)
@@ -1023,26 +1024,26 @@ hipError_t launch_scatter(hipStream_t st, const DevBuffers &b, uint32_t n)
     return hipGetLastError();
 }
 
-hipError_t launch_reorder(hipStream_t st, const DevBuffers &b, uint32_t n)
+hipError_t launch_reorder(hipStream_t st, const DevBuffers &b, uint32_t n, uint32_t bs)
 {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(reorder_k, dim3(blocks_for(n)), dim3(kBlock), 0, st, b, n);
+    hipLaunchKernelGGL(reorder_k, dim3(blocks_for(n, bs)), dim3(bs), 0, st, b, n);
     return hipGetLastError();
 }
 
 hipError_t launch_neighbors(hipStream_t st, const DevBuffers &b, const KConst &k, uint32_t n,
-                            uint32_t kmax)
+                            uint32_t kmax, uint32_t bs)
 {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(neighbors_k, dim3(blocks_for(n)), dim3(kBlock), 0, st, b, k, n, kmax);
+    hipLaunchKernelGGL(neighbors_k, dim3(blocks_for(n, bs)), dim3(bs), 0, st, b, k, n, kmax);
     return hipGetLastError();
 }
 
 hipError_t launch_lambda(hipStream_t st, const DevBuffers &b, const KConst &k, uint32_t n,
-                         bool uniform, float wu)
+                         bool uniform, float wu, uint32_t bs)
 {
     if (n == 0) return hipSuccess;
-    const dim3 g(blocks_for(n)), t(kBlock);
+    const dim3 g(blocks_for(n, bs)), t(bs);
     const float mu = 1.0f / wu;
     if (uniform)
         hipLaunchKernelGGL((lambda_k<true>), g, t, 0, st, b, k, n, mu);
@@ -1062,9 +1063,9 @@ hipError_t launch_shape(hipStream_t st, const DevBuffers &b, uint32_t nbody)
 // E_P, integration_kernel.cuh:34) or 0 = read from the parameters
 template <uint32_t F>
 static void solve_variant(hipStream_t st, const DevBuffers &b, const KConst &k, uint32_t n,
-                          bool uniform, float wu, uint32_t iter)
+                          bool uniform, float wu, uint32_t iter, uint32_t bs)
 {
-    const dim3 g(blocks_for(n)), t(kBlock);
+    const dim3 g(blocks_for(n, bs)), t(bs);
     const bool n4 = k.n_corr == 4u;
     if (uniform && n4)
         hipLaunchKernelGGL((solve_k<F, true, 4>), g, t, 0, st, b, k, n, wu, iter);
@@ -1077,25 +1078,26 @@ static void solve_variant(hipStream_t st, const DevBuffers &b, const KConst &k, 
 }
 
 hipError_t launch_solve(hipStream_t st, const DevBuffers &b, const KConst &k, uint32_t n,
-                        uint32_t features, bool uniform, float wu, uint32_t iter)
+                        uint32_t features, bool uniform, float wu, uint32_t iter, uint32_t bs)
 {
     if (n == 0) return hipSuccess;
     // uniform-fluid scenes have no contacts, constraints or bodies
     switch (features) {
-    case F_FLUID: solve_variant<F_FLUID>(st, b, k, n, uniform, wu, iter); break;
-    case F_FLUID | F_PIN: solve_variant<F_FLUID | F_PIN>(st, b, k, n, false, wu, iter); break;
+    case F_FLUID: solve_variant<F_FLUID>(st, b, k, n, uniform, wu, iter, bs); break;
+    case F_FLUID | F_PIN: solve_variant<F_FLUID | F_PIN>(st, b, k, n, false, wu, iter, bs); break;
     default:
-        solve_variant<F_FLUID | F_CONTACT | F_DIST | F_SHAPE | F_PIN>(st, b, k, n, false, wu, iter);
+        solve_variant<F_FLUID | F_CONTACT | F_DIST | F_SHAPE | F_PIN>(st, b, k, n, false, wu, iter, bs);
         break;
     }
     return hipGetLastError();
 }
 
 hipError_t launch_finalize(hipStream_t st, const DevBuffers &b, const KConst &k, uint32_t n,
-                           float dt, bool sleeping, bool uniform, float wu, uint32_t its)
+                           float dt, bool sleeping, bool uniform, float wu, uint32_t its,
+                           uint32_t bs)
 {
     if (n == 0) return hipSuccess;
-    const dim3 g(blocks_for(n)), t(kBlock);
+    const dim3 g(blocks_for(n, bs)), t(bs);
     const float mu = 1.0f / wu;
     if (uniform)
         hipLaunchKernelGGL((finalize_k<true>), g, t, 0, st, b, k, n, mu, dt, its);
@@ -1103,7 +1105,7 @@ hipError_t launch_finalize(hipStream_t st, const DevBuffers &b, const KConst &k,
         hipLaunchKernelGGL((finalize_k<false>), g, t, 0, st, b, k, n, mu, dt, its);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || !sleeping) return e;
-    hipLaunchKernelGGL(sleep_k, g, t, 0, st, b, n);
+    hipLaunchKernelGGL(sleep_k, dim3(blocks_for(n)), dim3(kBlock), 0, st, b, n);
     return hipGetLastError();
 }
 

@@ -97,3 +97,29 @@ def test_batch_rejects_out_of_order_scenes(gpu_available):
             g.add_particles(s0.pos, None, s0.inv_mass, s0.phase, s0.rest_density, scene=0)
         with pytest.raises(capi.PsError):
             g.add_particles(s0.pos, None, s0.inv_mass, s0.phase, s0.rest_density, scene=7)
+
+
+TUNABLE = ("predict_hash", "reorder", "neighbors", "lambda", "solve", "finalize")
+
+
+def test_autotune_restores_state_and_block_size_never_changes_results(gpu_available):
+    sc = scenes.dam_break(16, 8, 16, max_neighbors=64)
+    ref = load_scene(sc, "f32")
+    with System(sc.params) as g:
+        sc.load_into(g)
+        g.step(sc.dt, 2)
+        ref.step(sc.dt, 2)
+        x0, v0, it0 = g.positions(), g.velocities(), g.status().iterations
+        blocks = g.autotune(sc.dt, 1)
+        assert all(blocks[k] in (64, 128, 256, 512) for k in TUNABLE)
+        assert_bitexact(g.positions(), x0, "positions restored after tuning")
+        assert_bitexact(g.velocities(), v0, "velocities restored after tuning")
+        assert g.status().iterations == it0
+        for bs in (blocks, {k: 64 for k in TUNABLE}, {k: 512 for k in TUNABLE}):
+            g.set_tuning(bs)
+            g.step(sc.dt, 2)
+            ref.step(sc.dt, 2)
+            assert_bitexact(g.positions(), ref.positions(), "positions with blocks %s" % bs)
+            assert_bitexact(g.velocities(), ref.velocities(), "velocities with blocks %s" % bs)
+        with pytest.raises(capi.PsError):
+            g.set_tuning({k: 96 for k in TUNABLE})
