@@ -49,7 +49,8 @@ typedef enum ps_error {
     PS_ERR_NO_DEVICE = -4,        /* no usable GPU                              */
     PS_ERR_STATE = -5,            /* call out of order (e.g. iterate w/o begin) */
     PS_ERR_OUT_OF_MEMORY = -6,    /* device allocation failed                   */
-    PS_ERR_NOT_FINITE = -7        /* non-finite input values                    */
+    PS_ERR_NOT_FINITE = -7,       /* non-finite input values                    */
+    PS_ERR_UNSUPPORTED = -8       /* call not available in this solver mode     */
 } ps_error;
 
 /* Flags for ps_params.flags. */
@@ -57,6 +58,18 @@ typedef enum ps_error {
                                                 (gpu/src/particlesystem.cpp:171-197) */
 #define PS_FLAG_SLEEPING 0x2u /* zero v and keep x when |x_new - x| < 1e-4
                                  (cpu/src/particle.h:60-65)                     */
+/* The reference's CPU solver instead of the 3D path: Simulation::tick
+ * (cpu/src/simulation.cpp:115-369, ITERATIVE).  Box boundaries are
+ * constraints made at prediction and counted in the Jacobi divisor; each
+ * iteration projects the boundaries, then every fluid object in creation
+ * order (Gauss-Seidel across objects, Jacobi inside one,
+ * totalfluidconstraint.cpp:41-116): lambda_j = 0 for neighbours outside the
+ * object, dp = sum / rho0 / (neighbours + self + boundary count), tensile
+ * term -k_corr (W/W(dq h))^n with k_corr unscaled.  Neighbours come from an
+ * all-pairs scan summed in index order; no XSPH.  Particles enter only
+ * through ps_add_fluid_object; grid and max_neighbors are unused. Set
+ * PS_FLAG_SLEEPING too for the reference's behaviour (it always sleeps). */
+#define PS_FLAG_CPU_REFERENCE 0x4u
 
 /* Status bits raised by the kernels, read with ps_get_status. */
 #define PS_STATUS_NEIGHBOR_OVERFLOW 0x1u /* a particle had more than max_neighbors
@@ -206,6 +219,14 @@ int ps_add_particles(ps_system *sys, uint32_t n, const float *pos, const float *
 int ps_add_scene_particles(ps_system *sys, uint32_t scene, uint32_t n, const float *pos,
                            const float *vel, const float *inv_mass, const int32_t *phase,
                            const float *rest_density, uint32_t *first_out);
+
+/* Add one fluid object: n particles (pos, vel and inverse mass as in
+ * ps_add_particles, vel may be NULL) of one rest density.  Replaces
+ * Simulation::createFluid (cpu/src/simulation.cpp:431-449).  In
+ * PS_FLAG_CPU_REFERENCE mode the particles form one TotalFluidConstraint;
+ * otherwise this is ps_add_particles with phase PS_PHASE_FLUID. */
+int ps_add_fluid_object(ps_system *sys, uint32_t n, const float *pos, const float *vel,
+                        const float *inv_mass, float rest_density, uint32_t *first_out);
 
 /* Replaces addDistanceConstraint + updateOccurences (solver.cu:72-106,125-156).
  * pairs: 2*m particle indices; rest: m lengths; stiffness: m values in (0,1]

@@ -1,7 +1,8 @@
 """Build recipe of the CPU oracle (test infrastructure only).
 
-ps_oracle.c is compiled twice with gcc: FP32 (liboracle_f32.so) and FP64
-(liboracle_f64.so).  -ffp-contract=off and no -march/-ffast-math keep the
+ps_oracle.c (the substep of the 3D path) and ref2d.c (the reference's own
+CPU solver, Simulation::tick) are compiled together twice with gcc: FP32
+(liboracle_f32.so) and FP64 (liboracle_f64.so).  -ffp-contract=off and no -march/-ffast-math keep the
 FP32 build to exactly the IEEE operations written in the source.  OpenMP
 parallelises the per-particle loops (each particle is a pure gather, so the
 results do not depend on the thread count).
@@ -10,7 +11,7 @@ import os
 import subprocess
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SRC = os.path.join(HERE, "ps_oracle.c")
+SRCS = [os.path.join(HERE, "ps_oracle.c"), os.path.join(HERE, "ref2d.c")]
 
 CFLAGS = ["-std=c99", "-O2", "-fPIC", "-shared", "-ffp-contract=off", "-fno-fast-math",
           "-fopenmp", "-Wall", "-Wno-unknown-pragmas"]
@@ -21,7 +22,7 @@ def build_oracle(verbose=True):
     for dbl, name in ((0, "liboracle_f32.so"), (1, "liboracle_f64.so")):
         out = os.path.join(HERE, name)
         tmp = out + ".tmp"
-        cmd = ["gcc"] + CFLAGS + ["-DORC_DOUBLE=%d" % dbl, SRC, "-o", tmp, "-lm"]
+        cmd = ["gcc"] + CFLAGS + ["-DORC_DOUBLE=%d" % dbl] + SRCS + ["-o", tmp, "-lm"]
         if verbose:
             print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True)

@@ -1,0 +1,397 @@
+/*
+ * ref2d.c — CPU oracle of the reference's own CPU solver (TEST
+ * INFRASTRUCTURE ONLY; loaded by tests/ and bench.py's CPU baseline, never
+ * by the product library).
+ *
+ * Restates Simulation::tick (cpu/src/simulation.cpp:115-369) for scenes of
+ * fluid objects inside a box: prediction (:139-161), boundary constraints
+ * created from the predicted positions (:201-224) and counted (:232-245),
+ * SOLVER_ITERATIONS passes over the CONTACT group (boundaries,
+ * BoundaryConstraint::project, cpu/src/constraint/boundaryconstraint.cpp:
+ * 14-93) then the STANDARD group (one TotalFluidConstraint::project per
+ * fluid object in creation order, cpu/src/constraint/totalfluidconstraint.cpp:
+ * 41-116), velocity v = (ep - p)/dt and sleeping (Particle::confirmGuess,
+ * cpu/src/particle.h:60-65).  The reference's compile-time constants
+ * (PARTICLE_RAD, H, RELAXATION, K_P, E_P, DQ_P, S_SOLID) are runtime
+ * parameters; frand() jitter becomes the counter RNG of ps_oracle.c; the
+ * vector operations follow glm's order (normalize(r) = r * (1/sqrt(r.r))).
+ * Works in 2D (dims = 2) and 3D.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../include/ps.h"
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#if ORC_DOUBLE
+typedef double real;
+#define SQRT sqrt
+#define SFX(name) name##_f64
+#else
+typedef float real;
+#define SQRT sqrtf
+#define SFX(name) name##_f32
+#endif
+#define R(x) ((real)(x))
+#define EXPORT __attribute__((visibility("default")))
+
+#define BX_LOW 1u
+#define BX_HIGH 2u
+#define BY_LOW 4u
+#define BY_HIGH 8u
+#define BZ_LOW 16u
+#define BZ_HIGH 32u
+
+static uint64_t mix64(uint64_t z)
+{
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+/* boundary jitter in [0,1): SplitMix64 finaliser of seed + golden * counter
+ * + 1, counter = (iteration, insertion index, axis), top 24 bits (replaces
+ * frand(), boundaryconstraint.cpp:19) */
+static real uniform(uint64_t seed, uint64_t iter, uint32_t i, uint32_t axis)
+{
+    uint64_t ctr = (iter << 35) | ((uint64_t)i << 3) | (uint64_t)axis;
+    uint64_t z = mix64(seed + ctr * 0x9E3779B97F4A7C15ull + 1u);
+    return R((double)(uint32_t)(z >> 40) * (1.0 / 16777216.0));
+}
+
+typedef struct r2d_sys r2d_sys;
+static real poly6(const r2d_sys *s, real r2);
+
+struct r2d_sys {
+    ps_params prm;
+    uint32_t n, cap, ngroup;
+    real *p, *ep, *v; /* 3 per particle */
+    real *w;
+    int32_t *phase;
+    int32_t *group; /* fluid object, -1 none */
+    real *group_rho0;
+    uint32_t *bflags, *counts;
+    real *lambda, *delta;
+    uint32_t *nnb;
+    real poly6, spiky, h, h2, w_dq, radius, relax, k, jit;
+    uint64_t iters;
+};
+
+EXPORT int SFX(r2d_create)(const ps_params *prm, r2d_sys **out)
+{
+    r2d_sys *s = (r2d_sys *)calloc(1, sizeof(r2d_sys));
+    s->prm = *prm;
+    s->cap = prm->max_particles;
+    size_t c = s->cap ? s->cap : 1;
+    s->p = (real *)calloc(3 * c, sizeof(real));
+    s->ep = (real *)calloc(3 * c, sizeof(real));
+    s->v = (real *)calloc(3 * c, sizeof(real));
+    s->w = (real *)calloc(c, sizeof(real));
+    s->phase = (int32_t *)calloc(c, sizeof(int32_t));
+    s->group = (int32_t *)calloc(c, sizeof(int32_t));
+    s->group_rho0 = (real *)calloc(c, sizeof(real));
+    s->bflags = (uint32_t *)calloc(c, sizeof(uint32_t));
+    s->counts = (uint32_t *)calloc(c, sizeof(uint32_t));
+    s->lambda = (real *)calloc(c, sizeof(real));
+    s->delta = (real *)calloc(3 * c, sizeof(real));
+    s->nnb = (uint32_t *)calloc(c, sizeof(uint32_t));
+    /* constants as totalfluidconstraint.cpp:123-137 with a runtime H */
+    const double pi = 3.14159265358979323846, h = prm->h;
+    const double h2 = h * h, h6 = h2 * h2 * h2, h9 = h6 * h2 * h;
+    const double poly6 = 315.0 / (64.0 * pi * h9);
+    /* poly6(DQ_P * DQ_P * H * H), totalfluidconstraint.cpp:107 */
+    const double dq = prm->dq_corr, t = h2 - (((dq * dq) * h) * h);
+    s->poly6 = R(poly6);
+    s->spiky = R(45.0 / (pi * h6));
+    s->h = R(prm->h);
+    s->h2 = R(h2);
+    s->w_dq = R(poly6 * ((t * t) * t));
+    s->radius = R(prm->radius);
+    s->relax = R(prm->relaxation);
+    s->k = R(prm->k_corr);
+    s->jit = R((double)prm->wall_jitter * (double)prm->radius);
+    *out = s;
+    return PS_OK;
+}
+
+/* derived constants: poly6 and spiky coefficients, W(0), W(dq h), jitter */
+EXPORT void SFX(r2d_constants)(const r2d_sys *s, double *out)
+{
+    out[0] = s->poly6;
+    out[1] = s->spiky;
+    out[2] = poly6(s, R(0));
+    out[3] = s->w_dq;
+    out[4] = s->jit;
+}
+
+/* OpenMP threads of the per-particle loops (1 = the reference's single thread) */
+EXPORT void SFX(r2d_set_threads)(int n)
+{
+#ifdef _OPENMP
+    omp_set_num_threads(n > 0 ? n : 1);
+#else
+    (void)n;
+#endif
+}
+
+EXPORT void SFX(r2d_destroy)(r2d_sys *s)
+{
+    if (!s) return;
+    void *ptrs[] = {s->p, s->ep, s->v, s->w, s->phase, s->group, s->group_rho0, s->bflags,
+                    s->counts, s->lambda, s->delta, s->nnb};
+    for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) free(ptrs[i]);
+    free(s);
+}
+
+EXPORT int SFX(r2d_add_particles)(r2d_sys *s, uint32_t n, const float *pos, const float *vel,
+                                  const float *w, const int32_t *phase)
+{
+    if ((uint64_t)s->n + n > s->cap) return PS_ERR_CAPACITY;
+    for (uint32_t k = 0; k < n; k++) {
+        uint32_t i = s->n + k;
+        for (int a = 0; a < 3; a++) {
+            s->p[3 * i + a] = pos[3 * k + a];
+            s->v[3 * i + a] = vel ? R(vel[3 * k + a]) : R(0);
+        }
+        s->w[i] = w[k];
+        s->phase[i] = phase[k];
+        s->group[i] = -1;
+    }
+    s->n += n;
+    return PS_OK;
+}
+
+/* createFluid (cpu/src/simulation.cpp:431-449): particles [first, first+n)
+ * form one TotalFluidConstraint of rest density rho0 */
+EXPORT int SFX(r2d_add_fluid_group)(r2d_sys *s, uint32_t first, uint32_t n, float rho0)
+{
+    if ((uint64_t)first + n > s->n) return PS_ERR_INVALID_ARGUMENT;
+    for (uint32_t i = first; i < first + n; i++) s->group[i] = (int32_t)s->ngroup;
+    s->group_rho0[s->ngroup] = rho0;
+    s->ngroup++;
+    return PS_OK;
+}
+
+/* spikyGrad (totalfluidconstraint.cpp:131-137): -normalize(r) * c * (H-l) * (H-l) */
+static void spiky_grad(const r2d_sys *s, const real *r, real l, real *g)
+{
+    if (l >= s->h || l == R(0)) {
+        g[0] = g[1] = g[2] = R(0);
+        return;
+    }
+    real dot = ((r[0] * r[0]) + (r[1] * r[1])) + (r[2] * r[2]);
+    real inv = R(1) / SQRT(dot);
+    real q = s->h - l;
+    for (int a = 0; a < 3; a++) g[a] = (((-(r[a] * inv)) * s->spiky) * q) * q;
+}
+
+/* pow(x, E_P) (totalfluidconstraint.cpp:107): the FP64 build calls pow as the
+ * reference does; the FP32 build squares (x^4 = (x^2)^2), which the kernels
+ * reproduce bit for bit */
+static real ipow_corr(real x, uint32_t n)
+{
+#if ORC_DOUBLE
+    return pow(x, (double)n);
+#else
+    real r = R(1), b = x;
+    int first = 1;
+    while (n) {
+        if (n & 1u) {
+            r = first ? b : (r * b);
+            first = 0;
+        }
+        n >>= 1;
+        if (n) b = b * b;
+    }
+    return r;
+#endif
+}
+
+static real poly6(const r2d_sys *s, real r2)
+{
+    if (r2 >= s->h2) return R(0);
+    real t = s->h2 - r2;
+    return s->poly6 * ((t * t) * t);
+}
+
+static real length3(const real *r)
+{
+    return SQRT(((r[0] * r[0]) + (r[1] * r[1])) + (r[2] * r[2]));
+}
+
+/* TotalFluidConstraint::project for fluid object g: lambdas, then deltas,
+ * then ep += delta / (neighbours incl. self + counts) (Jacobi inside the
+ * object; objects run one after another) */
+static void fluid_group(r2d_sys *s, int32_t g)
+{
+    const uint32_t n = s->n;
+    const real p0 = s->group_rho0[g];
+#pragma omp parallel for schedule(static)
+    for (uint32_t i = 0; i < n; i++) {
+        if (s->group[i] != g) continue;
+        real pi = R(0), den = R(0);
+        real gi[3] = {R(0), R(0), R(0)};
+        for (uint32_t j = 0; j < n; j++) {
+            if (j != i) {
+                if (s->w[j] == R(0)) continue;
+                real r[3] = {s->ep[3 * i] - s->ep[3 * j], s->ep[3 * i + 1] - s->ep[3 * j + 1],
+                             s->ep[3 * i + 2] - s->ep[3 * j + 2]};
+                real r2 = ((r[0] * r[0]) + (r[1] * r[1])) + (r[2] * r[2]);
+                if (r2 < s->h2) {
+                    real incr = poly6(s, r2) / s->w[j];
+                    if (s->phase[j] == PS_PHASE_SOLID) incr = incr * R(s->prm.s_solid);
+                    pi = pi + incr;
+                    real l = length3(r), sg[3];
+                    spiky_grad(s, r, l, sg);
+                    real gr[3];
+                    for (int a = 0; a < 3; a++) gr[a] = (-sg[a]) / p0;
+                    den = den + (((gr[0] * gr[0]) + (gr[1] * gr[1])) + (gr[2] * gr[2]));
+                    real f = s->phase[j] == PS_PHASE_SOLID ? R(s->prm.s_solid) : R(1);
+                    for (int a = 0; a < 3; a++) gi[a] = gi[a] + (f * sg[a]);
+                }
+            } else {
+                pi = pi + (poly6(s, R(0)) / s->w[i]);
+            }
+        }
+        real gr[3] = {gi[0] / p0, gi[1] / p0, gi[2] / p0};
+        den = den + (((gr[0] * gr[0]) + (gr[1] * gr[1])) + (gr[2] * gr[2]));
+        s->lambda[i] = -((pi / p0) - R(1)) / (den + s->relax);
+    }
+#pragma omp parallel for schedule(static)
+    for (uint32_t i = 0; i < n; i++) {
+        if (s->group[i] != g) continue;
+        real d[3] = {R(0), R(0), R(0)};
+        uint32_t cnt = 0;
+        for (uint32_t j = 0; j < n; j++) {
+            if (j == i) {
+                cnt++;
+                continue;
+            }
+            if (s->w[j] == R(0)) continue;
+            real r[3] = {s->ep[3 * i] - s->ep[3 * j], s->ep[3 * i + 1] - s->ep[3 * j + 1],
+                         s->ep[3 * i + 2] - s->ep[3 * j + 2]};
+            real r2 = ((r[0] * r[0]) + (r[1] * r[1])) + (r[2] * r[2]);
+            if (!(r2 < s->h2)) continue;
+            cnt++;
+            real l = length3(r), sg[3];
+            spiky_grad(s, r, l, sg);
+            real x = poly6(s, l * l) / s->w_dq;
+            real corr = -(s->k * ipow_corr(x, s->prm.n_corr));
+            real lj = s->group[j] == g ? s->lambda[j] : R(0);
+            real coef = (s->lambda[i] + lj) + corr;
+            for (int a = 0; a < 3; a++) d[a] = d[a] + (coef * sg[a]);
+        }
+        for (int a = 0; a < 3; a++) s->delta[3 * i + a] = d[a] / p0;
+        s->nnb[i] = cnt;
+    }
+    for (uint32_t i = 0; i < n; i++) {
+        if (s->group[i] != g) continue;
+        real c = (real)(s->nnb[i] + s->counts[i]);
+        for (int a = 0; a < 3; a++) s->ep[3 * i + a] = s->ep[3 * i + a] + (s->delta[3 * i + a] / c);
+    }
+}
+
+/* BoundaryConstraint::project (boundaryconstraint.cpp:14-93), one axis */
+static void boundary(r2d_sys *s, uint32_t i, int axis, int greater, uint64_t iter)
+{
+    const real r = s->radius;
+    const real value = greater ? R(s->prm.box_min[axis]) : R(s->prm.box_max[axis]);
+    const int fl = s->phase[i] == PS_PHASE_FLUID || s->phase[i] == PS_PHASE_GAS;
+    const real extra = fl ? uniform(s->prm.seed, iter, i, (uint32_t)axis) * s->jit : R(0);
+    const real d = r + extra;
+    real *e = &s->ep[3 * i];
+    real nv[3] = {R(0), R(0), R(0)};
+    if (greater) {
+        if (e[axis] >= value + r) return;
+        e[axis] = value + d;
+        nv[axis] = R(1);
+    } else {
+        if (e[axis] <= value - r) return;
+        e[axis] = value - d;
+        nv[axis] = R(-1);
+    }
+    real c = (real)s->counts[i];
+    real dp[3];
+    for (int a = 0; a < 3; a++) dp[a] = (e[a] - s->p[3 * i + a]) / c;
+    real dn = ((dp[0] * nv[0]) + (dp[1] * nv[1])) + (dp[2] * nv[2]);
+    real t[3];
+    for (int a = 0; a < 3; a++) t[a] = dp[a] - (dn * nv[a]);
+    real l = length3(t);
+    if (l < R(1e-4)) return;
+    const real sf = SQRT(R(s->prm.wall_mu_s)), kf = SQRT(R(s->prm.wall_mu_k));
+    if (l < sf * d) {
+        for (int a = 0; a < 3; a++) e[a] = e[a] - t[a];
+    } else {
+        real f = (kf * d) / l;
+        if (f > R(1)) f = R(1);
+        for (int a = 0; a < 3; a++) e[a] = e[a] - (t[a] * f);
+    }
+}
+
+EXPORT int SFX(r2d_step)(r2d_sys *s, real dt, uint32_t steps)
+{
+    const real gx = R(s->prm.gravity[0]), gy = R(s->prm.gravity[1]), gz = R(s->prm.gravity[2]);
+    const int twod = s->prm.dims == 2;
+    for (uint32_t st = 0; st < steps; st++) {
+        /* (1)-(5) predict; (8) boundary contacts from the predicted positions */
+        for (uint32_t i = 0; i < s->n; i++) {
+            real *v = &s->v[3 * i], *p = &s->p[3 * i], *e = &s->ep[3 * i];
+            v[0] = v[0] + (dt * gx);
+            v[1] = v[1] + (dt * gy);
+            if (!twod) v[2] = v[2] + (dt * gz);
+            for (int a = 0; a < 3; a++)
+                e[a] = s->w[i] == R(0) ? p[a] : ((twod && a == 2) ? p[a] : (p[a] + (dt * v[a])));
+            uint32_t f = 0;
+            const real r = s->radius;
+            if (e[0] < R(s->prm.box_min[0]) + r) f |= BX_LOW;
+            else if (e[0] > R(s->prm.box_max[0]) - r) f |= BX_HIGH;
+            if (e[1] < R(s->prm.box_min[1]) + r) f |= BY_LOW;
+            else if (e[1] > R(s->prm.box_max[1]) - r) f |= BY_HIGH;
+            if (!twod) {
+                if (e[2] < R(s->prm.box_min[2]) + r) f |= BZ_LOW;
+                else if (e[2] > R(s->prm.box_max[2]) - r) f |= BZ_HIGH;
+            }
+            s->bflags[i] = f;
+            s->counts[i] = (uint32_t)__builtin_popcount(f);
+        }
+        /* (16)-(22) iterations: CONTACT group (boundaries, per particle x
+         * then y then z), STANDARD group (fluid objects in order) */
+        for (uint32_t it = 0; it < s->prm.iterations; it++) {
+            for (uint32_t i = 0; i < s->n; i++) {
+                uint32_t f = s->bflags[i];
+                if (f & BX_LOW) boundary(s, i, 0, 1, s->iters);
+                if (f & BX_HIGH) boundary(s, i, 0, 0, s->iters);
+                if (f & BY_LOW) boundary(s, i, 1, 1, s->iters);
+                if (f & BY_HIGH) boundary(s, i, 1, 0, s->iters);
+                if (f & BZ_LOW) boundary(s, i, 2, 1, s->iters);
+                if (f & BZ_HIGH) boundary(s, i, 2, 0, s->iters);
+            }
+            for (int32_t g = 0; g < (int32_t)s->ngroup; g++) fluid_group(s, g);
+            s->iters++;
+        }
+        /* (23)-(28) velocities, sleeping */
+        for (uint32_t i = 0; i < s->n; i++) {
+            real *v = &s->v[3 * i], *p = &s->p[3 * i], *e = &s->ep[3 * i];
+            real d[3] = {e[0] - p[0], e[1] - p[1], e[2] - p[2]};
+            for (int a = 0; a < 3; a++) v[a] = d[a] / dt;
+            if ((s->prm.flags & PS_FLAG_SLEEPING) && length3(d) < R(1e-4)) {
+                v[0] = v[1] = v[2] = R(0);
+                continue;
+            }
+            for (int a = 0; a < 3; a++) p[a] = e[a];
+        }
+    }
+    return PS_OK;
+}
+
+EXPORT int SFX(r2d_read)(const r2d_sys *s, real *pos, real *vel)
+{
+    if (pos) memcpy(pos, s->p, sizeof(real) * 3 * s->n);
+    if (vel) memcpy(vel, s->v, sizeof(real) * 3 * s->n);
+    return PS_OK;
+}

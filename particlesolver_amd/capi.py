@@ -28,9 +28,11 @@ PS_ERR_NO_DEVICE = -4
 PS_ERR_STATE = -5
 PS_ERR_OUT_OF_MEMORY = -6
 PS_ERR_NOT_FINITE = -7
+PS_ERR_UNSUPPORTED = -8
 
 PS_FLAG_REBUILD_EVERY_ITERATION = 0x1
 PS_FLAG_SLEEPING = 0x2
+PS_FLAG_CPU_REFERENCE = 0x4
 
 PS_STATUS_NEIGHBOR_OVERFLOW = 0x1
 PS_STATUS_NONFINITE = 0x2
@@ -128,6 +130,7 @@ PROTOTYPES = {
     "ps_get_params": (_int, [_P, ctypes.POINTER(PsParams)]),
     "ps_last_error": (ctypes.c_char_p, [_P]),
     "ps_add_particles": (_int, [_P, _u32, _P, _P, _P, _P, _P, ctypes.POINTER(_u32)]),
+    "ps_add_fluid_object": (_int, [_P, _u32, _P, _P, _P, _f32, ctypes.POINTER(_u32)]),
     "ps_add_distance_constraints": (_int, [_P, _u32, _P, _P, _P]),
     "ps_add_point_constraints": (_int, [_P, _u32, _P, _P]),
     "ps_add_rigid_body": (_int, [_P, _u32, _P, _f32, ctypes.POINTER(_u32)]),
@@ -262,6 +265,19 @@ class System:
         first = ctypes.c_uint32()
         self._check(self.lib.ps_add_scene_particles(self.h, scene, n, _ptr(pos), _ptr(vel), _ptr(w),
                                                     _ptr(ph), _ptr(r0), ctypes.byref(first)))
+        return first.value
+
+    def add_fluid_object(self, pos, vel=None, inv_mass=None, rest_density=1.0):
+        """One fluid object (Simulation::createFluid): in CPU-reference mode its
+        particles form one density constraint solved as a unit."""
+        pos = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1, 3)
+        n = pos.shape[0]
+        vel = None if vel is None else np.ascontiguousarray(vel, dtype=np.float32).reshape(n, 3)
+        w = np.ascontiguousarray(np.broadcast_to(1.0 if inv_mass is None else inv_mass, (n,)),
+                                 dtype=np.float32)
+        first = ctypes.c_uint32()
+        self._check(self.lib.ps_add_fluid_object(self.h, n, _ptr(pos), _ptr(vel), _ptr(w),
+                                                 float(rest_density), ctypes.byref(first)))
         return first.value
 
     def add_distance_constraints(self, pairs, rest, stiffness=None):

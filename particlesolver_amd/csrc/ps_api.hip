@@ -64,6 +64,16 @@ struct ps_system {
     bool nbr_valid = false; // neighbour table matches the current storage
     float dt = 0.0f;
     uint64_t substeps = 0, iters = 0;
+    // CPU-reference mode (PS_FLAG_CPU_REFERENCE)
+    bool cpuref = false;
+    ps::CrConst cr;
+    struct FluidObject {
+        uint32_t first, count;
+        float rho0;
+    };
+    std::vector<FluidObject> objects;
+    bool cr_bnd_done = false;  // boundaries of iteration it_sub already projected
+    bool cr_swapped = false;   // iterate lives in b.Q (odd number of object passes)
     // workgroup size of each kernel (ps_autotune)
     uint32_t block[PS_K_COUNT] = {256, 256, 256, 256, 256, 256, 256, 256, 256};
     // timing
@@ -159,6 +169,36 @@ void fill_kconst(const ps_params &p, const ps_derived &d, KConst &k)
     k.n_corr = p.n_corr;
     k.dims = p.dims;
     k.seed = p.seed;
+}
+
+// Constants of the CPU-reference mode, in the order the reference evaluates
+// them (totalfluidconstraint.cpp:107,123-137) in FP64, rounded once.
+void fill_crconst(const ps_params &p, const ps_derived &d, ps::CrConst &c)
+{
+    memset(&c, 0, sizeof(c));
+    const double pi = 3.14159265358979323846;
+    const double h = p.h, h2 = h * h, h6 = h2 * h2 * h2, h9 = h6 * h2 * h;
+    const double poly6 = 315.0 / (64.0 * pi * h9);
+    const double dq = p.dq_corr, t = h2 - (((dq * dq) * h) * h);
+    c.h = p.h;
+    c.h2 = (float)h2;
+    c.poly6 = (float)poly6;
+    c.spiky = (float)(45.0 / (pi * h6));
+    c.w_dq = (float)(poly6 * ((t * t) * t));
+    c.relax = p.relaxation;
+    c.k_corr = p.k_corr;
+    c.jitter = d.jitter;
+    c.radius = p.radius;
+    c.sf = sqrtf(p.wall_mu_s);
+    c.kf = sqrtf(p.wall_mu_k);
+    for (int a = 0; a < 3; a++) {
+        c.gravity[a] = p.gravity[a];
+        c.box_min[a] = p.box_min[a];
+        c.box_max[a] = p.box_max[a];
+    }
+    c.n_corr = p.n_corr;
+    c.dims = p.dims;
+    c.seed = p.seed;
 }
 
 int validate(const ps_params *p, std::string &why)
@@ -264,7 +304,7 @@ void free_all(ps_system *s)
     dev_free(b.pin_of); dev_free(b.pin_xyz); dev_free(b.body_off); dev_free(b.body_idx);
     dev_free(b.body_rest); dev_free(b.body_stiff); dev_free(b.body_mass); dev_free(b.body_q);
     dev_free(b.body_goal); dev_free(b.body_of); dev_free(b.member_of); dev_free(b.status);
-    dev_free(b.sid); dev_free(b.sid2); dev_free(b.scenes);
+    dev_free(b.sid); dev_free(b.sid2); dev_free(b.scenes); dev_free(b.lam); dev_free(b.bc);
 }
 
 void drop_graph(ps_system *s)
@@ -411,6 +451,72 @@ int stage_end(ps_system *s)
     s->substeps++;
     s->stage = 0;
     return PS_OK;
+}
+
+// ---- CPU-reference mode stages --------------------------------------------
+
+int cr_begin(ps_system *s, float dt)
+{
+    s->dt = dt;
+    s->it_sub = 0;
+    if (s->cr_swapped) {
+        std::swap(s->b.P, s->b.Q);
+        s->cr_swapped = false;
+    }
+    LAUNCH(s, PS_K_PREDICT_HASH, ps::launch_cr_predict(s->stream, s->b, s->cr, s->n, dt));
+    s->cr_bnd_done = true;  // prediction projects the boundaries of iteration 0
+    s->stage = 1;
+    return PS_OK;
+}
+
+int cr_iterate(ps_system *s)
+{
+    const uint32_t it = s->it_sub;
+    if (!s->cr_bnd_done)
+        LAUNCH(s, PS_K_SOLVE, ps::launch_cr_boundary(s->stream, s->b, s->cr, s->n, it));
+    // the last object pass also projects the next iteration's boundaries
+    const int next = it + 1 < s->prm.iterations ? (int)(it + 1) : -1;
+    const size_t nobj = s->objects.size();
+    for (size_t k = 0; k < nobj; k++) {
+        const ps_system::FluidObject &o = s->objects[k];
+        LAUNCH(s, PS_K_LAMBDA,
+               ps::launch_cr_lambda(s->stream, s->b, s->cr, s->n, o.first, o.count, o.rho0));
+        LAUNCH(s, PS_K_SOLVE,
+               ps::launch_cr_delta(s->stream, s->b, s->cr, s->n, o.first, o.count, o.rho0,
+                                   k + 1 == nobj ? next : -1));
+        std::swap(s->b.P, s->b.Q);
+        s->cr_swapped = !s->cr_swapped;
+    }
+    s->cr_bnd_done = nobj > 0 && next >= 0;
+    s->iters++;
+    s->it_sub++;
+    return PS_OK;
+}
+
+int cr_end(ps_system *s)
+{
+    LAUNCH(s, PS_K_FINALIZE,
+           ps::launch_cr_end(s->stream, s->b, s->n, s->dt, (s->prm.flags & PS_FLAG_SLEEPING) != 0,
+                             s->it_sub));
+    // the iterate is rebuilt by the next prediction: restore the buffer order
+    if (s->cr_swapped) {
+        std::swap(s->b.P, s->b.Q);
+        s->cr_swapped = false;
+    }
+    s->it_sub = 0;
+    s->substeps++;
+    s->stage = 0;
+    return PS_OK;
+}
+
+// mode dispatch of the substep stages
+int any_iterate(ps_system *s) { return s->cpuref ? cr_iterate(s) : stage_iterate(s); }
+int any_end(ps_system *s) { return s->cpuref ? cr_end(s) : stage_end(s); }
+
+int unsupported_in_cpuref(ps_system *s, const char *what)
+{
+    return fail(s, PS_ERR_UNSUPPORTED,
+                std::string(what) + " is not available with PS_FLAG_CPU_REFERENCE");
 }
 
 int check_sys(ps_system *s)
@@ -573,6 +679,9 @@ static int create_impl(const ps_params *params, uint32_t nscenes, const ps_scene
     *out = nullptr;
     if (nscenes == 0 || (nscenes > 1 && !descs))
         return fail(nullptr, PS_ERR_INVALID_ARGUMENT, "a batch needs scene descriptions");
+    const bool cpuref = (params->flags & PS_FLAG_CPU_REFERENCE) != 0;
+    if (cpuref && (nscenes > 1 || descs))
+        return fail(nullptr, PS_ERR_UNSUPPORTED, "the CPU-reference mode runs one scene per handle");
     // scene table: one scene from the parameters, or one per description
     std::vector<ps::SceneDev> scenes(nscenes);
     uint64_t cells = 0;
@@ -620,6 +729,8 @@ static int create_impl(const ps_params *params, uint32_t nscenes, const ps_scene
     s->der.num_cells = (uint32_t)cells;
     fill_kconst(s->prm, s->der, s->kc);
     s->kc.nscenes = nscenes;
+    s->cpuref = cpuref;
+    fill_crconst(s->prm, s->der, s->cr);
     s->scenes = scenes;
     s->scene_count.assign(nscenes, 0);
     s->cap = params->max_particles;
@@ -655,6 +766,10 @@ static int create_impl(const ps_params *params, uint32_t nscenes, const ps_scene
     if (nscenes > 1) {
         ALLOC(b.sid, c);
         ALLOC(b.sid2, c);
+    }
+    if (cpuref) {
+        ALLOC(b.lam, c);
+        ALLOC(b.bc, c);
     }
 #undef ALLOC
     hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
@@ -724,12 +839,16 @@ int ps_set_params(ps_system *s, const ps_params *p)
                     "capacities, grid and dims are fixed at ps_create");
     if (s->scenes.size() > 1 && p->cell != s->prm.cell)
         return fail(s, PS_ERR_INVALID_ARGUMENT, "the cell size of a batch is fixed at creation");
+    if (((p->flags ^ s->prm.flags) & PS_FLAG_CPU_REFERENCE) != 0)
+        return fail(s, PS_ERR_INVALID_ARGUMENT, "PS_FLAG_CPU_REFERENCE is fixed at ps_create");
+    if (s->stage) return fail(s, PS_ERR_STATE, "cannot change parameters inside a substep");
     const uint32_t ncells = s->der.num_cells, nsc = s->kc.nscenes;
     s->prm = *p;
     derive_constants(s->prm, s->der);
     s->der.num_cells = ncells;
     fill_kconst(s->prm, s->der, s->kc);
     s->kc.nscenes = nsc;
+    fill_crconst(s->prm, s->der, s->cr);
     if (nsc == 1) {
         // the single scene follows the parameters' box, gravity and seed
         ps::SceneDev &sc = s->scenes[0];
@@ -758,6 +877,10 @@ const char *ps_last_error(const ps_system *s)
     return g_err.c_str();
 }
 
+static int add_particles_impl(ps_system *s, uint32_t scene, uint32_t n, const float *pos,
+                              const float *vel, const float *inv_mass, const int32_t *phase,
+                              const float *rest_density, uint32_t *first_out);
+
 int ps_add_particles(ps_system *s, uint32_t n, const float *pos, const float *vel,
                      const float *inv_mass, const int32_t *phase, const float *rest_density,
                      uint32_t *first_out)
@@ -770,6 +893,30 @@ int ps_add_scene_particles(ps_system *s, uint32_t scene, uint32_t n, const float
                            const float *rest_density, uint32_t *first_out)
 {
     if (int rc = check_sys(s)) return rc;
+    if (s->cpuref) return unsupported_in_cpuref(s, "ps_add_particles (use ps_add_fluid_object)");
+    return add_particles_impl(s, scene, n, pos, vel, inv_mass, phase, rest_density, first_out);
+}
+
+int ps_add_fluid_object(ps_system *s, uint32_t n, const float *pos, const float *vel,
+                        const float *inv_mass, float rest_density, uint32_t *first_out)
+{
+    if (int rc = check_sys(s)) return rc;
+    if (!(rest_density > 0.0f) || !is_finite(rest_density))
+        return fail(s, PS_ERR_INVALID_ARGUMENT, "fluid rest density must be > 0");
+    std::vector<int32_t> phase(n, PS_PHASE_FLUID);
+    std::vector<float> rho(n, rest_density);
+    uint32_t first = 0;
+    int rc = add_particles_impl(s, 0, n, pos, vel, inv_mass, phase.data(), rho.data(), &first);
+    if (rc) return rc;
+    if (s->cpuref && n) s->objects.push_back({first, n, rest_density});
+    if (first_out) *first_out = first;
+    return PS_OK;
+}
+
+static int add_particles_impl(ps_system *s, uint32_t scene, uint32_t n, const float *pos,
+                              const float *vel, const float *inv_mass, const int32_t *phase,
+                              const float *rest_density, uint32_t *first_out)
+{
     if (s->stage) return fail(s, PS_ERR_STATE, "cannot add particles inside a substep");
     if (scene >= s->scenes.size()) return fail(s, PS_ERR_INVALID_ARGUMENT, "scene out of range");
     if (n && s->n && scene < s->last_scene)
@@ -847,6 +994,7 @@ int ps_add_distance_constraints(ps_system *s, uint32_t m, const uint32_t *pairs,
                                 const float *rest, const float *stiffness)
 {
     if (int rc = check_sys(s)) return rc;
+    if (s->cpuref) return unsupported_in_cpuref(s, "ps_add_distance_constraints");
     if (s->stage) return fail(s, PS_ERR_STATE, "cannot add constraints inside a substep");
     if (m == 0) return PS_OK;
     if (!pairs || !rest) return fail(s, PS_ERR_INVALID_ARGUMENT, "NULL input array");
@@ -874,6 +1022,7 @@ int ps_add_distance_constraints(ps_system *s, uint32_t m, const uint32_t *pairs,
 int ps_add_point_constraints(ps_system *s, uint32_t m, const uint32_t *idx, const float *xyz)
 {
     if (int rc = check_sys(s)) return rc;
+    if (s->cpuref) return unsupported_in_cpuref(s, "ps_add_point_constraints");
     if (s->stage) return fail(s, PS_ERR_STATE, "cannot add constraints inside a substep");
     if (m == 0) return PS_OK;
     if (!idx || !xyz) return fail(s, PS_ERR_INVALID_ARGUMENT, "NULL input array");
@@ -899,6 +1048,7 @@ int ps_add_rigid_bodies(ps_system *s, uint32_t nbody, const uint32_t *offsets,
                         const uint32_t *idx, const float *stiffness, uint32_t *first_body_out)
 {
     if (int rc = check_sys(s)) return rc;
+    if (s->cpuref) return unsupported_in_cpuref(s, "ps_add_rigid_bodies");
     if (s->stage) return fail(s, PS_ERR_STATE, "cannot add bodies inside a substep");
     if (nbody == 0) {
         if (first_body_out) *first_body_out = (uint32_t)s->body_off.size() - 1;
@@ -1033,6 +1183,7 @@ int ps_set_velocities(ps_system *s, uint32_t first, uint32_t n, const float *vel
 int ps_predict(ps_system *s, float dt)
 {
     if (int rc = check_sys(s)) return rc;
+    if (s->cpuref) return unsupported_in_cpuref(s, "ps_predict");
     if (!is_finite(dt) || dt < 0.0f) return fail(s, PS_ERR_INVALID_ARGUMENT, "dt must be finite and >= 0");
     LAUNCH(s, PS_K_PREDICT_HASH, ps::launch_predict_hash(s->stream, s->b, s->kc, s->n, dt, true,
                                                          false, s->block[PS_K_PREDICT_HASH]));
@@ -1047,6 +1198,7 @@ int ps_predict(ps_system *s, float dt)
 int ps_build_grid(ps_system *s)
 {
     if (int rc = check_sys(s)) return rc;
+    if (s->cpuref) return unsupported_in_cpuref(s, "ps_build_grid");
     if (s->stage != 1) return fail(s, PS_ERR_STATE, "ps_build_grid before ps_predict");
     int rc;
     if (!s->hashed && (rc = stage_hash(s, false, 0.0f))) return rc;
@@ -1056,6 +1208,7 @@ int ps_build_grid(ps_system *s)
 int ps_find_neighbors(ps_system *s)
 {
     if (int rc = check_sys(s)) return rc;
+    if (s->cpuref) return unsupported_in_cpuref(s, "ps_find_neighbors");
     if (s->stage != 1) return fail(s, PS_ERR_STATE, "ps_find_neighbors before ps_predict");
     if (s->hashed) return fail(s, PS_ERR_STATE, "ps_find_neighbors before ps_build_grid");
     return stage_neighbors(s);
@@ -1065,6 +1218,7 @@ int ps_substep_begin(ps_system *s, float dt)
 {
     if (int rc = check_sys(s)) return rc;
     if (!(dt > 0.0f) || !is_finite(dt)) return fail(s, PS_ERR_INVALID_ARGUMENT, "dt must be > 0");
+    if (s->cpuref) return cr_begin(s, dt);
     int rc;
     s->dt = dt;
     s->it_sub = 0;
@@ -1078,7 +1232,7 @@ int ps_substep_iterate(ps_system *s)
 {
     if (int rc = check_sys(s)) return rc;
     if (s->stage != 1) return fail(s, PS_ERR_STATE, "ps_substep_iterate before ps_substep_begin");
-    return stage_iterate(s);
+    return any_iterate(s);
 }
 
 int ps_substep_end(ps_system *s)
@@ -1086,7 +1240,7 @@ int ps_substep_end(ps_system *s)
     if (int rc = check_sys(s)) return rc;
     if (s->stage != 1) return fail(s, PS_ERR_STATE, "ps_substep_end before ps_substep_begin");
     if (!(s->dt > 0.0f)) return fail(s, PS_ERR_STATE, "substep has dt = 0");
-    return stage_end(s);
+    return any_end(s);
 }
 
 int ps_step(ps_system *s, float dt, uint32_t substeps)
@@ -1099,8 +1253,8 @@ int ps_step(ps_system *s, float dt, uint32_t substeps)
         int rc;
         if ((rc = ps_substep_begin(s, dt))) return rc;
         for (uint32_t it = 0; it < s->prm.iterations; it++)
-            if ((rc = stage_iterate(s))) return rc;
-        if ((rc = stage_end(s))) return rc;
+            if ((rc = any_iterate(s))) return rc;
+        if ((rc = any_end(s))) return rc;
     }
     return PS_OK;
 }
@@ -1121,8 +1275,8 @@ int ps_step_graph(ps_system *s, float dt, uint32_t substeps)
         int rc = PS_OK;
         for (uint32_t k = 0; k < substeps && rc == PS_OK; k++) {
             rc = ps_substep_begin(s, dt);
-            for (uint32_t it = 0; it < s->prm.iterations && rc == PS_OK; it++) rc = stage_iterate(s);
-            if (rc == PS_OK) rc = stage_end(s);
+            for (uint32_t it = 0; it < s->prm.iterations && rc == PS_OK; it++) rc = any_iterate(s);
+            if (rc == PS_OK) rc = any_end(s);
         }
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(s->stream, &g);
@@ -1131,6 +1285,7 @@ int ps_step_graph(ps_system *s, float dt, uint32_t substeps)
         s->iters = it0;
         s->stage = 0;
         s->it_sub = 0;
+        s->cr_swapped = false;
         const bool rotated_back = memcmp(&start, &s->b, sizeof(DevBuffers)) == 0;
         s->b = start;
         if (rc != PS_OK || e != hipSuccess || !rotated_back) {
@@ -1213,6 +1368,7 @@ int ps_read_velocities(ps_system *s, uint32_t first, uint32_t n, float *xyz)
 int ps_read_grid(ps_system *s, uint32_t *sorted_index, uint32_t *cell, uint32_t *cell_offsets)
 {
     if (int rc = check_sys(s)) return rc;
+    if (s->cpuref) return unsupported_in_cpuref(s, "ps_read_grid");
     HIP_TRY(s, hipStreamSynchronize(s->stream));
     if (sorted_index && s->n) {
         std::vector<Attr> A(s->n);
@@ -1230,6 +1386,7 @@ int ps_read_grid(ps_system *s, uint32_t *sorted_index, uint32_t *cell, uint32_t 
 int ps_read_neighbors(ps_system *s, uint32_t *counts, uint32_t *table)
 {
     if (int rc = check_sys(s)) return rc;
+    if (s->cpuref) return unsupported_in_cpuref(s, "ps_read_neighbors");
     if (s->n == 0) return PS_OK;
     HIP_TRY(s, hipStreamSynchronize(s->stream));
     std::vector<uint32_t> cnt(s->n);
@@ -1252,6 +1409,7 @@ int ps_read_neighbors(ps_system *s, uint32_t *counts, uint32_t *table)
 int ps_read_sorted_state(ps_system *s, float *iterate_xyzl, float *start_xyz)
 {
     if (int rc = check_sys(s)) return rc;
+    if (s->cpuref) return unsupported_in_cpuref(s, "ps_read_sorted_state");
     if (s->n == 0) return PS_OK;
     HIP_TRY(s, hipStreamSynchronize(s->stream));
     if (iterate_xyzl)
@@ -1385,6 +1543,7 @@ uint32_t log2_bucket(uint32_t n)
 int ps_autotune(ps_system *s, float dt, uint32_t substeps, uint32_t *blocks_out)
 {
     if (int rc = check_sys(s)) return rc;
+    if (s->cpuref) return unsupported_in_cpuref(s, "ps_autotune");
     if (s->stage) return fail(s, PS_ERR_STATE, "ps_autotune inside an open substep");
     if (!(dt > 0.0f) || !is_finite(dt)) return fail(s, PS_ERR_INVALID_ARGUMENT, "dt must be > 0");
     if (substeps == 0) substeps = 2;

@@ -1,0 +1,405 @@
+// ps_cpuref.hip — kernels of the CPU-reference mode (PS_FLAG_CPU_REFERENCE).
+//
+// This mode evolves a scene with the reference's own CPU solver semantics,
+// Simulation::tick (cpu/src/simulation.cpp:115-369) with its ITERATIVE path:
+//   - prediction v += dt g, ep = p + dt v (static particles keep p)
+//     (:139-161, Particle::guess cpu/src/particle.h:56-58);
+//   - box boundaries become constraints at prediction time (:201-224) and
+//     count towards the particle's Jacobi divisor (:232-245);
+//   - each iteration projects the CONTACT group (BoundaryConstraint::project,
+//     cpu/src/constraint/boundaryconstraint.cpp:14-93) then the STANDARD
+//     group: one TotalFluidConstraint per fluid object, in creation order
+//     (cpu/src/constraint/totalfluidconstraint.cpp:41-116);
+//   - v = (ep - p)/dt, sleeping below |ep - p| < 1e-4 (:322-333,
+//     particle.h:60-65).
+// The fluid object scans every particle for neighbours (O(N^2), as the
+// reference does) and accumulates in ascending particle index, so its sums
+// are the reference's sums term for term.
+//
+// Storage is insertion order.  X: positions (.w unused), E/E2: iterate
+// ping-pong (x, y, z, inverse mass), V: velocities, lam: lambda per
+// particle, bc: boundary flags (bits 0-5) | boundary count << 8.
+//
+// Work shape: one wavefront per particle for the two fluid passes.  The 64
+// lanes test 64 candidates at once; the neighbours found are ballot-masked
+// and their terms are added one after another in lane (= index) order with
+// readlane, which keeps the sequential summation order of the reference.
+// Boundaries and the Jacobi update are particle-local, so the boundary pass
+// of iteration k+1 is fused into the last fluid pass of iteration k (and the
+// one of iteration 0 into prediction).
+
+#include "ps_internal.hpp"
+
+namespace ps {
+
+namespace {
+
+constexpr uint32_t kBlock = 256;
+constexpr uint32_t kWaves = kBlock / 64;
+
+enum : uint32_t { BX_LOW = 1u, BX_HIGH = 2u, BY_LOW = 4u, BY_HIGH = 8u, BZ_LOW = 16u, BZ_HIGH = 32u };
+
+__device__ __forceinline__ uint64_t cr_mix64(uint64_t z)
+{
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+// Boundary jitter in [0,1) (replaces frand(), boundaryconstraint.cpp:19):
+// SplitMix64 finaliser of seed + golden * counter + 1, counter = (iteration,
+// insertion index, axis), top 24 bits.
+__device__ __forceinline__ float cr_uniform(uint64_t seed, uint64_t iter, uint32_t i, uint32_t axis)
+{
+    uint64_t ctr = (iter << 35) | ((uint64_t)i << 3) | (uint64_t)axis;
+    uint64_t z = cr_mix64(seed + ctr * 0x9E3779B97F4A7C15ull + 1ull);
+    return (float)(uint32_t)(z >> 40) * (1.0f / 16777216.0f);
+}
+
+// poly6 (totalfluidconstraint.cpp:123-129): 0 at r2 >= h2
+__device__ __forceinline__ float cr_poly6(const CrConst &c, float r2)
+{
+    if (r2 >= c.h2) return 0.0f;
+    const float t = c.h2 - r2;
+    return c.poly6 * ((t * t) * t);
+}
+
+// spikyGrad (totalfluidconstraint.cpp:131-137): -normalize(r) c (h-l)^2,
+// zero at l >= h or l == 0; normalize(r) = r * (1 / sqrt(r.r))
+__device__ __forceinline__ void cr_spiky(const CrConst &c, float rx, float ry, float rz, float r2,
+                                         float l, float &gx, float &gy, float &gz)
+{
+    const float inv = 1.0f / sqrtf(r2);
+    const float q = c.h - l;
+    const bool on = !(l >= c.h || l == 0.0f);
+    gx = on ? (((-(rx * inv)) * c.spiky) * q) * q : 0.0f;
+    gy = on ? (((-(ry * inv)) * c.spiky) * q) * q : 0.0f;
+    gz = on ? (((-(rz * inv)) * c.spiky) * q) * q : 0.0f;
+}
+
+// x^n by binary powering (x^4 = (x^2)^2)
+__device__ __forceinline__ float cr_ipow(float x, uint32_t n)
+{
+    float r = 1.0f, b = x;
+    bool first = true;
+    while (n) {
+        if (n & 1u) {
+            r = first ? b : (r * b);
+            first = false;
+        }
+        n >>= 1;
+        if (n) b = b * b;
+    }
+    return r;
+}
+
+__device__ __forceinline__ float lane_value(float v, uint32_t lane)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), (int)lane));
+}
+
+// BoundaryConstraint::project for one axis (boundaryconstraint.cpp:14-93):
+// push to value +- (r + jitter), then friction on the tangential part of
+// the displacement since the substep start divided by the boundary count.
+__device__ __forceinline__ void cr_boundary_axis(const CrConst &c, float e[3], const float p[3],
+                                                 uint32_t axis, bool greater, float cnt,
+                                                 float extra)
+{
+    const float r = c.radius;
+    const float value = greater ? c.box_min[axis] : c.box_max[axis];
+    const float d = r + extra;
+    float nv[3] = {0.0f, 0.0f, 0.0f};
+    if (greater) {
+        if (e[axis] >= value + r) return;
+        e[axis] = value + d;
+        nv[axis] = 1.0f;
+    } else {
+        if (e[axis] <= value - r) return;
+        e[axis] = value - d;
+        nv[axis] = -1.0f;
+    }
+    float dp[3], t[3];
+    for (int a = 0; a < 3; a++) dp[a] = (e[a] - p[a]) / cnt;
+    const float dn = ((dp[0] * nv[0]) + (dp[1] * nv[1])) + (dp[2] * nv[2]);
+    for (int a = 0; a < 3; a++) t[a] = dp[a] - (dn * nv[a]);
+    const float l = sqrtf(((t[0] * t[0]) + (t[1] * t[1])) + (t[2] * t[2]));
+    if (l < 1e-4f) return;
+    if (l < c.sf * d) {
+        for (int a = 0; a < 3; a++) e[a] = e[a] - t[a];
+    } else {
+        float f = (c.kf * d) / l;
+        f = f > 1.0f ? 1.0f : f;
+        for (int a = 0; a < 3; a++) e[a] = e[a] - (t[a] * f);
+    }
+}
+
+// All boundary constraints of particle i, in the order the reference creates
+// them (x then y then z, simulation.cpp:201-224).  Fluids get the jitter.
+__device__ __forceinline__ void cr_boundaries(const CrConst &c, float e[3], const float p[3],
+                                              uint32_t bc, uint64_t iter, uint32_t i)
+{
+    if (!(bc & 0x3Fu)) return;
+    const float cnt = (float)(bc >> 8);
+    for (uint32_t axis = 0; axis < 3; axis++) {
+        const uint32_t lo = 1u << (2 * axis), hi = 2u << (2 * axis);
+        if (!(bc & (lo | hi))) continue;
+        const float extra = cr_uniform(c.seed, iter, i, axis) * c.jitter;
+        cr_boundary_axis(c, e, p, axis, (bc & lo) != 0, cnt, extra);
+    }
+}
+
+// (1)-(9): velocity and predicted position, boundary flags and counts, then
+// the boundary projections of iteration 0.
+__global__ __launch_bounds__(kBlock) void cr_predict_k(const float4 *__restrict__ X,
+                                                       float4 *__restrict__ V,
+                                                       const Attr *__restrict__ A,
+                                                       float4 *__restrict__ E,
+                                                       uint32_t *__restrict__ bcs,
+                                                       const DevStatus *__restrict__ st, CrConst c,
+                                                       uint32_t n, float dt)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const float4 x = X[i];
+    float4 v = V[i];
+    const float w = A[i].w;
+    const bool two = c.dims == 2;
+    v.x = v.x + (dt * c.gravity[0]);
+    v.y = v.y + (dt * c.gravity[1]);
+    if (!two) v.z = v.z + (dt * c.gravity[2]);
+    const float p[3] = {x.x, x.y, x.z};
+    float e[3];
+    if (w == 0.0f) {
+        e[0] = x.x;
+        e[1] = x.y;
+        e[2] = x.z;
+    } else {
+        e[0] = x.x + (dt * v.x);
+        e[1] = x.y + (dt * v.y);
+        e[2] = two ? x.z : (x.z + (dt * v.z));
+    }
+    const float r = c.radius;
+    uint32_t f = 0;
+    if (e[0] < c.box_min[0] + r) f |= BX_LOW;
+    else if (e[0] > c.box_max[0] - r) f |= BX_HIGH;
+    if (e[1] < c.box_min[1] + r) f |= BY_LOW;
+    else if (e[1] > c.box_max[1] - r) f |= BY_HIGH;
+    if (!two) {
+        if (e[2] < c.box_min[2] + r) f |= BZ_LOW;
+        else if (e[2] > c.box_max[2] - r) f |= BZ_HIGH;
+    }
+    const uint32_t bc = f | ((uint32_t)__popc(f) << 8);
+    cr_boundaries(c, e, p, bc, st->iter, i);
+    bcs[i] = bc;
+    V[i] = v;
+    E[i] = make_float4(e[0], e[1], e[2], w);
+}
+
+// TotalFluidConstraint::project, first loop (totalfluidconstraint.cpp:44-90):
+// lambda of every particle of the object [first, first + count).
+__global__ __launch_bounds__(kBlock) void cr_lambda_k(const float4 *__restrict__ E,
+                                                      float *__restrict__ lam, CrConst c,
+                                                      uint32_t n, uint32_t first, uint32_t count,
+                                                      float rho0)
+{
+    const uint32_t wv = blockIdx.x * kWaves + (threadIdx.x >> 6);
+    const uint32_t lane = threadIdx.x & 63u;
+    if (wv >= count) return;  // whole wavefront
+    const uint32_t i = first + wv;
+    const float4 ei = E[i];
+    float acc = 0.0f, den = 0.0f, gx = 0.0f, gy = 0.0f, gz = 0.0f;
+    for (uint32_t base = 0; base < n; base += 64) {
+        const uint32_t j = base + lane;
+        const bool valid = j < n;
+        const float4 ej = E[valid ? j : i];
+        const float rx = ei.x - ej.x, ry = ei.y - ej.y, rz = ei.z - ej.z;
+        const float r2 = ((rx * rx) + (ry * ry)) + (rz * rz);
+        const bool self = valid && j == i;
+        const bool nb = valid && !self && ej.w != 0.0f && r2 < c.h2;
+        uint64_t mask = __ballot(nb || self);
+        if (!mask) continue;
+        float incr = 0.0f, dd = 0.0f, sx = 0.0f, sy = 0.0f, sz = 0.0f;
+        if (nb) {
+            incr = cr_poly6(c, r2) / ej.w;
+            const float l = sqrtf(r2);
+            cr_spiky(c, rx, ry, rz, r2, l, sx, sy, sz);
+            const float grx = (-sx) / rho0, gry = (-sy) / rho0, grz = (-sz) / rho0;
+            dd = ((grx * grx) + (gry * gry)) + (grz * grz);
+        } else if (self) {
+            incr = cr_poly6(c, 0.0f) / ei.w;
+        }
+        // in index order: the self lane adds +0 to den and the gradient sum,
+        // which leaves them unchanged (neither can be -0)
+        while (mask) {
+            const uint32_t b = (uint32_t)__builtin_ctzll(mask);
+            mask &= mask - 1;
+            acc = acc + lane_value(incr, b);
+            den = den + lane_value(dd, b);
+            gx = gx + lane_value(sx, b);
+            gy = gy + lane_value(sy, b);
+            gz = gz + lane_value(sz, b);
+        }
+    }
+    const float grx = gx / rho0, gry = gy / rho0, grz = gz / rho0;
+    den = den + (((grx * grx) + (gry * gry)) + (grz * grz));
+    if (lane == 0) lam[i] = -((acc / rho0) - 1.0f) / (den + c.relax);
+}
+
+// TotalFluidConstraint::project, delta and update loops
+// (totalfluidconstraint.cpp:92-115) for the object [first, first + count),
+// ping-pong E -> E2 over every particle (others are copied).  With
+// next_iter >= 0 the boundary projections of the next iteration follow.
+__global__ __launch_bounds__(kBlock) void cr_delta_k(const float4 *__restrict__ E,
+                                                     float4 *__restrict__ E2,
+                                                     const float *__restrict__ lam,
+                                                     const float4 *__restrict__ X,
+                                                     const uint32_t *__restrict__ bcs,
+                                                     const DevStatus *__restrict__ st, CrConst c,
+                                                     uint32_t n, uint32_t first, uint32_t count,
+                                                     float rho0, int next_iter)
+{
+    const uint32_t i = blockIdx.x * kWaves + (threadIdx.x >> 6);
+    const uint32_t lane = threadIdx.x & 63u;
+    if (i >= n) return;  // whole wavefront
+    float4 ei = E[i];
+    const uint32_t bc = bcs[i];
+    if (i - first < count) {
+        const float li = lam[i];
+        float dx = 0.0f, dy = 0.0f, dz = 0.0f;
+        uint32_t cnt = 1;  // the particle itself is in its neighbour list
+        for (uint32_t base = 0; base < n; base += 64) {
+            const uint32_t j = base + lane;
+            const bool valid = j < n;
+            const float4 ej = E[valid ? j : i];
+            const float rx = ei.x - ej.x, ry = ei.y - ej.y, rz = ei.z - ej.z;
+            const float r2 = ((rx * rx) + (ry * ry)) + (rz * rz);
+            const bool nb = valid && j != i && ej.w != 0.0f && r2 < c.h2;
+            uint64_t mask = __ballot(nb);
+            if (!mask) continue;
+            cnt += (uint32_t)__popcll(mask);
+            float tx = 0.0f, ty = 0.0f, tz = 0.0f;
+            if (nb) {
+                const float l = sqrtf(r2);
+                float sx, sy, sz;
+                cr_spiky(c, rx, ry, rz, r2, l, sx, sy, sz);
+                const float xr = cr_poly6(c, l * l) / c.w_dq;
+                const float corr = -(c.k_corr * cr_ipow(xr, c.n_corr));
+                // lambdas of particles outside this object read as 0 (QHash default)
+                const float lj = (j - first < count) ? lam[j] : 0.0f;
+                const float coef = (li + lj) + corr;
+                tx = coef * sx;
+                ty = coef * sy;
+                tz = coef * sz;
+            }
+            while (mask) {
+                const uint32_t b = (uint32_t)__builtin_ctzll(mask);
+                mask &= mask - 1;
+                dx = dx + lane_value(tx, b);
+                dy = dy + lane_value(ty, b);
+                dz = dz + lane_value(tz, b);
+            }
+        }
+        const float div = (float)(cnt + (bc >> 8));
+        ei.x = ei.x + ((dx / rho0) / div);
+        ei.y = ei.y + ((dy / rho0) / div);
+        ei.z = ei.z + ((dz / rho0) / div);
+    }
+    if (next_iter >= 0) {
+        const float4 x = X[i];
+        const float p[3] = {x.x, x.y, x.z};
+        float e[3] = {ei.x, ei.y, ei.z};
+        cr_boundaries(c, e, p, bc, st->iter + (uint64_t)next_iter, i);
+        ei.x = e[0];
+        ei.y = e[1];
+        ei.z = e[2];
+    }
+    if (lane == 0) E2[i] = ei;
+}
+
+// Boundary projections alone (a scene with no fluid object).
+__global__ __launch_bounds__(kBlock) void cr_boundary_k(float4 *__restrict__ E,
+                                                        const float4 *__restrict__ X,
+                                                        const uint32_t *__restrict__ bcs,
+                                                        const DevStatus *__restrict__ st,
+                                                        CrConst c, uint32_t n, uint32_t it)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const float4 x = X[i];
+    float4 ei = E[i];
+    const float p[3] = {x.x, x.y, x.z};
+    float e[3] = {ei.x, ei.y, ei.z};
+    cr_boundaries(c, e, p, bcs[i], st->iter + it, i);
+    E[i] = make_float4(e[0], e[1], e[2], ei.w);
+}
+
+// (23)-(28): v = (ep - p)/dt; sleeping keeps p and zeroes v.
+__global__ __launch_bounds__(kBlock) void cr_end_k(const float4 *__restrict__ E,
+                                                   float4 *__restrict__ X, float4 *__restrict__ V,
+                                                   DevStatus *__restrict__ st, uint32_t n,
+                                                   float dt, int sleeping, uint32_t its)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i == 0) st->iter += its;
+    if (i >= n) return;
+    const float4 e = E[i], x = X[i];
+    const float dx = e.x - x.x, dy = e.y - x.y, dz = e.z - x.z;
+    float4 v = make_float4(dx / dt, dy / dt, dz / dt, 0.0f);
+    if (!(isfinite(e.x) && isfinite(e.y) && isfinite(e.z))) atomicOr(&st->flags, PS_STATUS_NONFINITE);
+    if (sleeping && sqrtf(((dx * dx) + (dy * dy)) + (dz * dz)) < 1e-4f) {
+        V[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        return;
+    }
+    V[i] = v;
+    X[i] = make_float4(e.x, e.y, e.z, 0.0f);
+}
+
+inline uint32_t blocks_for(uint32_t threads) { return (threads + kBlock - 1) / kBlock; }
+
+}  // namespace
+
+hipError_t launch_cr_predict(hipStream_t st, const DevBuffers &b, const CrConst &c, uint32_t n,
+                             float dt)
+{
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(cr_predict_k, dim3(blocks_for(n)), dim3(kBlock), 0, st, b.X, b.V, b.A, b.P,
+                       b.bc, b.status, c, n, dt);
+    return hipGetLastError();
+}
+
+hipError_t launch_cr_lambda(hipStream_t st, const DevBuffers &b, const CrConst &c, uint32_t n,
+                            uint32_t first, uint32_t count, float rho0)
+{
+    if (!n || !count) return hipSuccess;
+    hipLaunchKernelGGL(cr_lambda_k, dim3((count + kWaves - 1) / kWaves), dim3(kBlock), 0, st, b.P,
+                       b.lam, c, n, first, count, rho0);
+    return hipGetLastError();
+}
+
+hipError_t launch_cr_delta(hipStream_t st, const DevBuffers &b, const CrConst &c, uint32_t n,
+                           uint32_t first, uint32_t count, float rho0, int next_iter)
+{
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(cr_delta_k, dim3((n + kWaves - 1) / kWaves), dim3(kBlock), 0, st, b.P, b.Q,
+                       b.lam, b.X, b.bc, b.status, c, n, first, count, rho0, next_iter);
+    return hipGetLastError();
+}
+
+hipError_t launch_cr_boundary(hipStream_t st, const DevBuffers &b, const CrConst &c, uint32_t n,
+                              uint32_t it)
+{
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(cr_boundary_k, dim3(blocks_for(n)), dim3(kBlock), 0, st, b.P, b.X, b.bc,
+                       b.status, c, n, it);
+    return hipGetLastError();
+}
+
+hipError_t launch_cr_end(hipStream_t st, const DevBuffers &b, uint32_t n, float dt, bool sleeping,
+                         uint32_t its)
+{
+    hipLaunchKernelGGL(cr_end_k, dim3(blocks_for(n ? n : 1)), dim3(kBlock), 0, st, b.P, b.X, b.V,
+                       b.status, n, dt, sleeping ? 1 : 0, its);
+    return hipGetLastError();
+}
+
+}  // namespace ps

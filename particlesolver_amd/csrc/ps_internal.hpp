@@ -57,6 +57,17 @@ struct KConst {
     uint32_t nscenes;  // > 1: per-particle scene table lookups
 };
 
+// Constants of the CPU-reference mode (ps_cpuref.hip), rounded once to
+// float on the host in the order the reference evaluates them.
+struct CrConst {
+    float h, h2, poly6, spiky, w_dq, relax, k_corr, jitter, radius, sf, kf;
+    float gravity[3];
+    float box_min[3], box_max[3];
+    uint32_t n_corr;
+    uint32_t dims;
+    uint64_t seed;
+};
+
 // Scene-composition switches chosen on the host per launch; each selects a
 // separately compiled variant of the solve kernel, so a pure-fluid scene runs
 // without the contact/distance/shape code paths.
@@ -108,6 +119,9 @@ struct DevBuffers {
     uint32_t *body_of;     // per particle: body + 1
     uint32_t *member_of;   // per particle: member index
     DevStatus *status;
+    // CPU-reference mode only
+    float *lam;            // lambda per particle
+    uint32_t *bc;          // boundary flags | boundary count << 8
 };
 
 // Kernel launchers (ps_kernels.hip).  All enqueue on `st` and return the
@@ -132,5 +146,21 @@ hipError_t launch_finalize(hipStream_t st, const DevBuffers &b, const KConst &k,
                            uint32_t bs);
 
 uint32_t scan_tiles(uint32_t num_cells);
+
+// CPU-reference mode (ps_cpuref.hip); storage in insertion order, iterate
+// ping-pong b.P -> b.Q per fluid object.
+hipError_t launch_cr_predict(hipStream_t st, const DevBuffers &b, const CrConst &c, uint32_t n,
+                             float dt);
+// lambda, then delta + Jacobi update (b.P -> b.Q) of one fluid object
+// [first, first + count); next_iter >= 0 also projects the boundaries of
+// that iteration of the substep
+hipError_t launch_cr_lambda(hipStream_t st, const DevBuffers &b, const CrConst &c, uint32_t n,
+                            uint32_t first, uint32_t count, float rho0);
+hipError_t launch_cr_delta(hipStream_t st, const DevBuffers &b, const CrConst &c, uint32_t n,
+                           uint32_t first, uint32_t count, float rho0, int next_iter);
+hipError_t launch_cr_boundary(hipStream_t st, const DevBuffers &b, const CrConst &c, uint32_t n,
+                              uint32_t it);
+hipError_t launch_cr_end(hipStream_t st, const DevBuffers &b, uint32_t n, float dt, bool sleeping,
+                         uint32_t its);
 
 }  // namespace ps

@@ -266,6 +266,132 @@ def two_d_fluid(nx=40, ny=20, spacing=1.0, h=2.0, iterations=3, seed=0x5EED0001,
                  np.full(n, S, np.float32), dt, name="fluid2d")
 
 
+class FluidObjectsScene:
+    """Parameters plus fluid objects (positions, rest density), added with
+    ps_add_fluid_object in creation order (Simulation::createFluid,
+    cpu/src/simulation.cpp:431-449)."""
+
+    def __init__(self, params, objects, dt, name=""):
+        self.params = params
+        self.objects = objects
+        self.dt = dt
+        self.name = name
+
+    @property
+    def n(self):
+        return sum(pos.shape[0] for pos, _ in self.objects)
+
+    def load_into(self, sys_):
+        for pos, rho0 in self.objects:
+            sys_.add_fluid_object(pos, None, 1.0, rho0)
+
+
+def cpu_reference_params(box_min, box_max, radius, h, iterations, max_particles, dims=2,
+                         seed=0x5EED0003):
+    """PS_FLAG_CPU_REFERENCE parameters with the reference CPU solver's
+    constants (cpu/src/constraint/totalfluidconstraint.h:22-30:
+    RELAXATION .01, K_P .1, E_P 4, DQ_P .2, S_SOLID 0; boundary jitter
+    frand() * .003, boundaryconstraint.cpp:19; fluids have no wall friction,
+    particle.h:49-50).  The grid is unused in this mode."""
+    p = capi.PsParams()
+    p.abi_version = capi.PS_ABI_VERSION
+    p.dims = dims
+    p.radius = radius
+    p.h = h
+    p.cell = h
+    p.collide_dist = h
+    p.gravity[:] = [0.0, -9.8, 0.0]
+    p.box_min[:] = list(box_min)
+    p.box_max[:] = list(box_max)
+    p.origin[:] = list(box_min)
+    p.grid[:] = [1, 1, 1]
+    p.iterations = iterations
+    p.relaxation = 0.01
+    p.k_corr = 0.1
+    p.n_corr = 4
+    p.dq_corr = 0.2
+    p.xsph_c = 0.0
+    p.s_solid = 0.0
+    p.mu_s = p.mu_k = 0.0
+    p.wall_mu_s = p.wall_mu_k = 0.0
+    p.wall_jitter = 0.003 / radius
+    p.shape_stiffness = 1.0
+    p.max_particles = max_particles
+    p.max_neighbors = 4
+    p.seed = seed
+    p.flags = capi.PS_FLAG_CPU_REFERENCE | capi.PS_FLAG_SLEEPING
+    return p
+
+
+def rayleigh_taylor_2d(nx=40, ny=20, spacing=1.0, jitter=0.01, iterations=3, seed=0x5EED0004,
+                       dt=1.0 / 60.0):
+    """Two-fluid Rayleigh-Taylor scene in 2D: a light fluid (rest density 1)
+    block of nx x ny particles on the floor and a heavy one (rest density 3)
+    on top of it, spacing 1.0 with seeded jitter, radius spacing/2, h = 2r,
+    box x in [-25, 25], y in [0, 1e6) (open top, as the reference's
+    m_yBoundaries, simulation.cpp:661-662).  The light object is created
+    first, as initFluid creates its objects in increasing density
+    (simulation.cpp:894-913)."""
+    radius = 0.5 * spacing
+    h = 2.0 * radius
+    half = 0.5 * nx * spacing
+    light = fluid_block(nx, ny, 1, spacing, (-half, 0.0, 0.0), jitter, seed)
+    heavy = fluid_block(nx, ny, 1, spacing, (-half, ny * spacing, 0.0), jitter, seed + 1)
+    light[:, 2] = 0.0
+    heavy[:, 2] = 0.0
+    p = cpu_reference_params((-25.0, 0.0, -1.0), (25.0, 1.0e6, 1.0), radius, h, iterations,
+                             2 * nx * ny)
+    return FluidObjectsScene(p, [(light, 1.0), (heavy, 3.0)], dt, name="rayleigh_taylor_2d")
+
+
+def reference_two_fluids(seed=0x5EED0005, dt=0.01):
+    """The reference CPU app's fluid scene, Simulation::initFluid
+    (cpu/src/simulation.cpp:894-913): two fluid objects side by side, rest
+    densities 1 and 1.75, lattice step .7 from x = -8 (and 0) and y = -8,
+    jitter .2 (u - .5) per axis (seeded SplitMix64 in place of frand()),
+    H = 2, radius .25, box x in [-8, 8], y in [-8, 40], 3 iterations, dt =
+    .01 (the app's fixed tick, cpu/src/view.cpp:197)."""
+    scale, delta = 4.0, 0.7
+    objects = []
+    for d in range(2):
+        start = -2.0 * scale + 4.0 * scale * (d / 2.0)
+        pts = []
+        x = start
+        while x < start + 4.0 * scale / 2.0:
+            y = -2.0 * scale
+            while y < scale:
+                pts.append((x, y))
+                y += delta
+            x += delta
+        pts = np.array(pts, np.float64)
+        u = uniform24(seed + d, 2 * len(pts)).reshape(-1, 2)
+        pos = np.zeros((len(pts), 3), np.float64)
+        pos[:, :2] = pts + 0.2 * (u - 0.5)
+        objects.append((pos.astype(np.float32), 1.0 + 0.75 * d))
+    n = sum(o[0].shape[0] for o in objects)
+    p = cpu_reference_params((-2.0 * scale, -2.0 * scale, -1.0), (2.0 * scale, 10.0 * scale, 1.0),
+                             0.25, 2.0, 3, n)
+    return FluidObjectsScene(p, objects, dt, name="reference_two_fluids")
+
+
+def fluid_object_metrics(params, objects, pos):
+    """FP64 densities of every particle at `pos` (all-pairs poly6 sum with the
+    3D normalisation the reference's CPU solver uses in 2D,
+    totalfluidconstraint.cpp:123-129) and the density constraint value
+    C_i = rho_i / rho0_i - 1 of each particle's object."""
+    x = np.asarray(pos, np.float64)
+    h = float(params.h)
+    c = poly6_coef(h)
+    rho0 = np.concatenate([np.full(o[0].shape[0], r) for o, r in
+                           zip(objects, [r for _, r in objects])])
+    rho = np.zeros(x.shape[0])
+    for a in range(0, x.shape[0], 512):
+        d2 = ((x[a:a + 512, None, :] - x[None, :, :]) ** 2).sum(-1)
+        w = np.where(d2 < h * h, c * (h * h - d2) ** 3, 0.0)
+        rho[a:a + 512] = w.sum(1)
+    return rho, rho / rho0 - 1.0
+
+
 class Batch:
     """Scenes for ps_create_batch: a shared parameter set, one PsSceneDesc per
     scene, and each scene's particles/constraints (global indices are
