@@ -14,6 +14,12 @@ MI355X HBM peak of 8 TB/s.
 
 The CPU baseline is the FP32 CPU oracle (OpenMP over the host cores given
 to this process) on the same scene for a bounded number of substeps.
+
+HBM traffic per launch comes from two rocprofv3 counter runs of this script
+(FETCH_SIZE, then WRITE_SIZE, with --copy-calibration so the same runs hold a
+copy of known size).  `--traffic-report` turns the two counter CSVs into the
+summary kept under profiles/; the bench line then quotes the traffic of its
+dominant kernel from that summary when the workload matches.
 """
 import argparse
 import json
@@ -25,6 +31,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "traffic_dam_break_1M.md")
 ROOF_KERNELS = ("lambda", "solve")
 
 # Algorithmic bytes per particle per launch, K = neighbour-table length.
@@ -53,16 +60,112 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-timing", action="store_true",
                     help="no per-kernel events in the timed region")
-    ap.add_argument("--workload", default="dam_break", choices=["dam_break", "ensemble"],
+    ap.add_argument("--workload", default="dam_break",
+                    choices=["dam_break", "ensemble", "rayleigh_taylor_2d"],
                     help="dam_break: the 1M-particle PBF metric; ensemble: 512 batched "
-                         "scenes (one graph launch per 8 substeps) vs 512 per-scene systems")
+                         "scenes (one graph launch per 8 substeps) vs 512 per-scene systems; "
+                         "rayleigh_taylor_2d: the 1,600-particle 2D scene in the reference's "
+                         "CPU-solver semantics")
     ap.add_argument("--scenes", type=int, default=512)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also run 3 device-to-device copies of MIB MiB (the runtime's copy "
                          "kernel), to scale HBM counters against a known byte count")
     ap.add_argument("--breakdown", type=int, default=20,
                     help="substeps of an extra, separately timed run with events on every kernel")
+    ap.add_argument("--traffic-report", nargs=3, metavar=("FETCH_CSV", "WRITE_CSV", "BENCH_LOG"),
+                    help="summarise two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) of this "
+                         "script with --copy-calibration; BENCH_LOG holds that run's JSON line")
     return ap.parse_args()
+
+
+KERNEL_OF = {"predict_hash_k": "predict_hash", "scan_partials_k": "scan", "scan_final_k": "scan",
+             "scatter_k": "scatter", "reorder_k": "reorder", "neighbors_k": "neighbors",
+             "lambda_k": "lambda", "solve_k": "solve", "shape_k": "shape",
+             "finalize_k": "finalize"}
+
+
+def _counter_means(path):
+    """Mean counter value (KiB) per dispatch for each kernel of a rocprofv3
+    counter_collection.csv, and the values of the largest runtime copy."""
+    import csv
+    per, copies = {}, []
+    for row in csv.DictReader(open(path)):
+        name = row["Kernel_Name"]
+        v = float(row["Counter_Value"])
+        if "copyBuffer" in name:
+            copies.append((int(row["Grid_Size"]), v))
+            continue
+        for key in KERNEL_OF:
+            if "ps::" + key + "(" in name or "ps::" + key + "<" in name:
+                per.setdefault(key, []).append(v)
+                break
+    big = max(g for g, _ in copies)
+    out = {}
+    for key, v in per.items():          # kernels of one stage add up per launch
+        out[KERNEL_OF[key]] = out.get(KERNEL_OF[key], 0.0) + sum(v) / len(v)
+    return out, [v for g, v in copies if g == big]
+
+
+def traffic_report(fetch_csv, write_csv, log):
+    """Markdown summary of the HBM traffic per launch (printed to stdout)."""
+    line = None
+    for l in open(log):
+        l = l.strip()
+        if l.startswith("{") and '"metric"' in l:
+            line = json.loads(l)
+    N, K = line["config"]["particles"], line["config"]["mean_neighbors"]
+    cal = None
+    for l in open(log):
+        if l.startswith("copy calibration:"):
+            cal = int(l.split("x")[-1].split()[0])
+    fetch, fcopy = _counter_means(fetch_csv)
+    write, wcopy = _counter_means(write_csv)
+    ref_kib = cal / 1024.0
+    f_scale = ref_kib / (sum(fcopy) / len(fcopy))
+    w_scale = ref_kib / (sum(wcopy) / len(wcopy))
+    out = ["# HBM traffic per launch, 1M-particle dam break", "",
+           "Commands (repository root on the GPU host, R = its absolute path):", "",
+           "    cd /tmp && export TMPDIR=/tmp",
+           "    rocprofv3 --pmc FETCH_SIZE --output-format csv -d $R/gpurun_out/pmc_fetch -o fetch "
+           "-- python3 $R/bench.py --steps 5 --warmup 2 --no-cpu --breakdown 0 "
+           "--copy-calibration 512 > $R/gpurun_out/pmc_fetch.log 2>&1",
+           "    rocprofv3 --pmc WRITE_SIZE --output-format csv -d $R/gpurun_out/pmc_write -o write "
+           "-- python3 $R/bench.py --steps 5 --warmup 2 --no-cpu --breakdown 0 "
+           "--copy-calibration 512 > $R/gpurun_out/pmc_write.log 2>&1",
+           "    python bench.py --traffic-report gpurun_out/pmc_fetch/fetch_counter_collection.csv "
+           "gpurun_out/pmc_write/write_counter_collection.csv gpurun_out/pmc_fetch.log", "",
+           "Counters: rocprofv3 FETCH_SIZE and WRITE_SIZE (KiB per dispatch), one run each.",
+           "Calibration: the runtime copy kernel of a %d-byte hipMemcpy (device to device) in the "
+           "same runs reads FETCH_SIZE %.1f KiB and WRITE_SIZE %.1f KiB for %.1f KiB moved each "
+           "way, so FETCH_SIZE is scaled by %.4f and WRITE_SIZE by %.4f."
+           % (cal, sum(fcopy) / len(fcopy), sum(wcopy) / len(wcopy), ref_kib, f_scale, w_scale),
+           "N = %d particles, mean neighbour-table length K = %.2f." % (N, K), "",
+           "| kernel | FETCH KiB | WRITE KiB | traffic MB/launch | algorithmic MB/launch | traffic / algorithmic |",
+           "|---|---:|---:|---:|---:|---:|"]
+    table = {}
+    for k in sorted(set(fetch) | set(write), key=lambda k: -(fetch.get(k, 0) * f_scale + write.get(k, 0))):
+        t = (fetch.get(k, 0.0) * f_scale + write.get(k, 0.0) * w_scale) * 1024.0
+        table[k] = int(t)
+        algo = ALGO_BYTES[k](K) * N if k in ALGO_BYTES else None
+        out.append("| %s | %.1f | %.1f | %.1f | %s | %s |" % (
+            k, fetch.get(k, 0.0), write.get(k, 0.0), t / 1e6,
+            "%.1f" % (algo / 1e6) if algo else "-", "%.2f" % (t / algo) if algo else "-"))
+    out += ["", "traffic_bytes_per_launch: " + json.dumps({"particles": N, "kernels": table})]
+    print("\n".join(out))
+
+
+def quoted_traffic(kernel, n):
+    """Traffic per launch of `kernel` from the committed summary, if it was
+    measured on the same particle count."""
+    try:
+        for l in open(TRAFFIC_SUMMARY):
+            if l.startswith("traffic_bytes_per_launch:"):
+                d = json.loads(l.split(":", 1)[1])
+                if d["particles"] == n and kernel in d["kernels"]:
+                    return d["kernels"][kernel]
+    except OSError:
+        pass
+    return None
 
 
 def cpu_baseline(sc, budget_s):
@@ -165,14 +268,102 @@ def ensemble_main(args):
     print(json.dumps(line))
 
 
+def rt2d_main(args):
+    """1,600-particle two-fluid 2D scene with the reference CPU solver's
+    semantics: GPU FP32 (graph-captured steps) against ref2d in FP64 on one
+    core (the reference's CPU solver is single-threaded); the FP32 error
+    against FP64 is stated per result after 1, 10 and 200 steps."""
+    import numpy as np
+    from particlesolver_amd import System, scenes
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    sc = scenes.rayleigh_taylor_2d()
+    N, I = sc.n, sc.params.iterations
+    steps = args.steps if args.steps % 10 == 0 else 10 * (args.steps // 10 + 1)
+    g = System(sc.params)
+    sc.load_into(g)
+    g.step_graph(sc.dt, 10)              # capture + warm-up
+    g.close()
+    g = System(sc.params)
+    sc.load_into(g)
+    g.step_graph(sc.dt, 10)
+    g.synchronize()
+    g.close()
+    # timed: a fresh system, `steps` steps from the initial state
+    g = System(sc.params)
+    sc.load_into(g)
+    g.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps // 10):
+        g.step_graph(sc.dt, 10)
+    g.synchronize()
+    el = time.perf_counter() - t0
+    x_gpu = g.positions()
+    st = g.status()
+    g.close()
+    errors = {}
+    cpu = None
+    if not args.no_cpu:
+        from oracle import load_ref2d
+        ref = load_ref2d(sc, "f64")
+        ref.set_threads(1)
+        gpu_at = {}
+        with System(sc.params) as h:
+            sc.load_into(h)
+            done = 0
+            for k in (1, 10):
+                h.step(sc.dt, k - done)
+                done = k
+                gpu_at[k] = h.positions()
+        gpu_at[steps] = x_gpu
+        done, t_cpu = 0, 0.0
+        for k in sorted(gpu_at):
+            t1 = time.perf_counter()
+            ref.step(sc.dt, k - done)
+            t_cpu += time.perf_counter() - t1
+            done = k
+            x64 = ref.positions()
+            x32 = gpu_at[k].astype(np.float64)
+            e = np.abs(x32 - x64).max(1)
+            rho32, c32 = scenes.fluid_object_metrics(sc.params, sc.objects, x32)
+            rho64, c64 = scenes.fluid_object_metrics(sc.params, sc.objects, x64)
+            errors[str(k)] = {"pos_max": float("%.3e" % e.max()),
+                              "pos_rms": float("%.3e" % np.sqrt((e ** 2).mean())),
+                              "density_max_abs": float("%.3e" % np.abs(rho32 - rho64).max()),
+                              "residual_mean_abs_fp32": float("%.4e" % np.abs(c32).mean()),
+                              "residual_mean_abs_fp64": float("%.4e" % np.abs(c64).mean())}
+        ref.set_threads(0)
+        cpu = {"value": round(N * I * steps / t_cpu, 1), "unit": "particle-iterations/s",
+               "cores": 1, "kind": "port",
+               "sample": "ref2d (the reference CPU solver restated in C, FP64, 1 thread) on the "
+                         "same scene, all %d steps, %.2f s" % (steps, t_cpu)}
+    line = {
+        "metric": "particle-iterations/s, 2D two-fluid scene in the reference CPU solver's semantics",
+        "value": round(N * I * steps / el, 1), "unit": "particle-iterations/s", "n_gpus": 1,
+        "steps": steps, "warmup": 10, "ms_per_step": round(1e3 * el / steps, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded SplitMix64 lattice jitter)",
+        "config": {"workload": "2D Rayleigh-Taylor, light (rho0 1) and heavy (rho0 3) 40x20 "
+                               "blocks = %d particles, r=0.5, h=1, 3 iterations, dt=1/60, "
+                               "all-pairs neighbour scan" % N,
+                   "status_flags": st.flags, "fp32_vs_fp64_by_step": errors},
+        "roofline": None,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
 def main():
     args = parse()
+    if args.traffic_report:
+        return traffic_report(*args.traffic_report)
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
               "(--gpus %d requested)" % args.gpus, file=sys.stderr)
         sys.exit(2)
     if args.workload == "ensemble":
         return ensemble_main(args)
+    if args.workload == "rayleigh_taylor_2d":
+        return rt2d_main(args)
     from particlesolver_amd import System, scenes
     import numpy as np
 
@@ -228,7 +419,8 @@ def main():
         bytes_per_launch = ALGO_BYTES[dom](K) * N
         achieved = bytes_per_launch / per_launch_s / 1e9
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": quoted_traffic(dom, N),
                 "kernel": dom, "algo_bytes_per_launch": int(bytes_per_launch),
                 "K": round(K, 2)}
         for k in kernels:

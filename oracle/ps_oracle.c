@@ -46,6 +46,45 @@ typedef float real;
 #endif
 #define R(x) ((real)(x))
 
+/* Pair arithmetic of the neighbour search, lambda, delta-p and XSPH, shared
+ * with the kernels: squared distances and accumulations use fused
+ * multiply-add (one rounding, as fmaf and the GPU's v_fma_f32 both give),
+ * and 1/r comes from RSQ below instead of a square root and a division. */
+#if ORC_DOUBLE
+#define FMA fma
+static double RSQ(double x) { return 1.0 / sqrt(x); }
+#else
+#define FMA fmaf
+/* 1/sqrt(x), x > 0 normal: the integer seed 0x5f375a86 - (bits(x) >> 1)
+ * of the "fast inverse square root" (constant from C. Lomont, Fast Inverse
+ * Square Root, 2003) and three Newton steps y <- y (3/2 - (x/2) y^2), each
+ * as t = (x/2) y; u = fma(-t, y, 3/2); y = y u.  Relative error about 1e-7
+ * (the seed's 3.4e-2 squares at every step).  Only IEEE operations, so the
+ * kernels' copy of these steps gives the same float. */
+static float RSQ(float x)
+{
+    uint32_t i;
+    memcpy(&i, &x, sizeof(i));
+    i = 0x5f375a86u - (i >> 1);
+    float y;
+    memcpy(&y, &i, sizeof(y));
+    const float hx = 0.5f * x;
+    for (int k = 0; k < 3; k++) {
+        const float t = hx * y;
+        const float u = fmaf(-t, y, 1.5f);
+        y = y * u;
+    }
+    return y;
+}
+#endif
+/* RSQ exported for its accuracy test */
+__attribute__((visibility("default"))) real SFX(orc_rsq)(real x) { return RSQ(x); }
+
+/* |d|^2 = fma(dz, dz, fma(dy, dy, dx dx)) */
+static real DIST2(real dx, real dy, real dz) { return FMA(dz, dz, FMA(dy, dy, dx * dx)); }
+/* below this squared distance a pair is degenerate (no direction) */
+#define DEGENERATE_R2 R(1e-8)
+
 #if ORC_DOUBLE
 #define EXP exp
 #else
@@ -515,7 +554,7 @@ ORC_EXPORT int SFX(orc_build_grid)(orc_sys *s)
  * integration_kernel.cuh:303-356,481-518): every j != i within the search
  * radius, scanned over the 3x3 rows of x-adjacent cells (each row is one
  * contiguous range of sorted slots), so the list is in ascending slot order.
- * r^2 = (dx*dx + dy*dy) + dz*dz.  Overflow keeps the first kmax entries and
+ * r^2 = DIST2.  Overflow keeps the first kmax entries and
  * raises PS_STATUS_NEIGHBOR_OVERFLOW (the reference truncates silently at
  * 500, :344,508). */
 ORC_EXPORT int SFX(orc_find_neighbors)(orc_sys *s)
@@ -544,7 +583,7 @@ ORC_EXPORT int SFX(orc_find_neighbors)(orc_sys *s)
                     if (j == i) continue;
                     real dx_ = px - s->P[4 * j], dy_ = py - s->P[4 * j + 1],
                          dz_ = pz - s->P[4 * j + 2];
-                    real r2 = ((dx_ * dx_) + (dy_ * dy_)) + (dz_ * dz_);
+                    real r2 = DIST2(dx_, dy_, dz_);
                     if (r2 < r2max) {
                         if (cnt < kmax) s->nbr[(size_t)i * kmax + cnt] = j;
                         cnt++;
@@ -574,9 +613,11 @@ ORC_EXPORT int SFX(orc_begin)(orc_sys *s, real dt)
 /* a5: lambda (findLambdasD, integration_kernel.cuh:521-593; CPU
  * TotalFluidConstraint::project first loop, totalfluidconstraint.cpp:41-90).
  *   rho_i = W(0)/w_i + sum_j s_j W(r_ij)/w_j   (s_j = 1 fluid, s_solid else)
- *   grad_j C_i = (spiky (h-r)^2 / r) / rho0_i * d,   d = p_i - p_j
- *   lambda_i = -(rho_i/rho0_i - 1) / (sum_j |grad_j|^2 + |sum_j grad_j|^2 + eps)
- * Static neighbours (w_j = 0) are skipped (cpu totalfluidconstraint.cpp:122). */
+ *   grad_j C_i = a d,  a = (spiky / rho0_i) (h-r)^2 / r,   d = p_i - p_j
+ *   lambda_i = -(rho_i/rho0_i - 1) / (sum_j a^2 r^2 + |sum_j grad_j|^2 + eps)
+ * with 1/r = RSQ(r^2), r = r^2 (1/r); degenerate pairs (r^2 < 1e-8) add
+ * density only.  Static neighbours (w_j = 0) are skipped (cpu
+ * totalfluidconstraint.cpp:122). */
 static void lambda_pass(orc_sys *s)
 {
     const orc_derived *d = &s->d;
@@ -590,7 +631,7 @@ static void lambda_pass(orc_sys *s)
         }
         real px = s->P[4 * i], py = s->P[4 * i + 1], pz = s->P[4 * i + 2];
         real r0 = s->rho0[i];
-        real ir0 = R(1) / r0;
+        real sk = d->spiky * (R(1) / r0);
         real rho = d->w_zero * (R(1) / s->w[i]);
         real gix = R(0), giy = R(0), giz = R(0), den = R(0);
         for (uint32_t k = 0; k < s->ncnt[i]; k++) {
@@ -598,24 +639,23 @@ static void lambda_pass(orc_sys *s)
             real wj = s->w[j];
             if (wj == R(0)) continue;
             real dx = px - s->P[4 * j], dy = py - s->P[4 * j + 1], dz = pz - s->P[4 * j + 2];
-            real r2 = ((dx * dx) + (dy * dy)) + (dz * dz);
+            real r2 = DIST2(dx, dy, dz);
             if (!(r2 < d->h2)) continue;
             real t = d->h2 - r2;
             real W = d->poly6 * ((t * t) * t);
-            real c = W * (R(1) / wj);
-            if (s->phase[j] != PS_PHASE_FLUID) c = c * d->s_solid;
-            rho = rho + c;
-            real r = SQRT(r2);
-            if (r < R(1e-4)) continue;
-            real q = d->h - r;
-            real a = ((d->spiky * (q * q)) / r) * ir0;
-            real gx = a * dx, gy = a * dy, gz = a * dz;
-            den = den + (((gx * gx) + (gy * gy)) + (gz * gz));
-            gix = gix + gx;
-            giy = giy + gy;
-            giz = giz + gz;
+            real m = R(1) / wj;
+            if (s->phase[j] != PS_PHASE_FLUID) m = m * d->s_solid;
+            rho = FMA(W, m, rho);
+            if (r2 < DEGENERATE_R2) continue;
+            real ri = RSQ(r2);
+            real q = d->h - (r2 * ri);
+            real a = (sk * (q * q)) * ri;
+            den = FMA(a * a, r2, den);
+            gix = FMA(a, dx, gix);
+            giy = FMA(a, dy, giy);
+            giz = FMA(a, dz, giz);
         }
-        den = den + (((gix * gix) + (giy * giy)) + (giz * giz));
+        den = den + DIST2(gix, giy, giz);
         real C = (rho / r0) - R(1);
         s->P[4 * i + 3] = -C / (den + d->relax);
     }
@@ -929,7 +969,8 @@ static void solve_pass(orc_sys *s)
          * integration_kernel.cuh:596-642; cpu totalfluidconstraint.cpp:92-115)
          *   dp_i = (1/rho0_i) sum_j (l_i + l_j + s_corr) gradW_ij,
          *   gradW_ij = -spiky (h-r)^2 d/r,  s_corr = -k h^2 (W(r)/W(dq h))^n,
-         *   r < 1e-4: gradW = (0, -0.001 spiky (h-r)^2, 0)   (:625-626) */
+         *   1/r = RSQ(r^2), r = r^2 (1/r);
+         *   r^2 < 1e-8: gradW = (0, -0.001 spiky h^2, 0)   (:625-626) */
         if (ph == PS_PHASE_FLUID) {
             real li = s->P[4 * i + 3], ir0 = R(1) / s->rho0[i];
             real fx = R(0), fy = R(0), fz = R(0);
@@ -937,22 +978,21 @@ static void solve_pass(orc_sys *s)
                 uint32_t j = s->nbr[(size_t)i * kmax + k];
                 if (s->w[j] == R(0)) continue;
                 real dx = px - s->P[4 * j], dy = py - s->P[4 * j + 1], dz = pz - s->P[4 * j + 2];
-                real r2 = ((dx * dx) + (dy * dy)) + (dz * dz);
+                real r2 = DIST2(dx, dy, dz);
                 if (!(r2 < d->h2)) continue;
                 real t = d->h2 - r2;
                 real W = d->poly6 * ((t * t) * t);
                 real corr = -(d->k_corr * ipow(W * d->inv_w_dq, d->n_corr));
                 real coef = (li + s->P[4 * j + 3]) + corr;
-                real r = SQRT(r2);
-                real q = d->h - r;
-                real g = d->spiky * (q * q);
-                if (r < R(1e-4)) {
-                    fy = fy - (coef * (g * R(0.001)));
+                if (r2 < DEGENERATE_R2) {
+                    fy = fy - (coef * ((d->spiky * (d->h * d->h)) * R(0.001)));
                 } else {
-                    real a = g / r;
-                    fx = fx - (coef * (a * dx));
-                    fy = fy - (coef * (a * dy));
-                    fz = fz - (coef * (a * dz));
+                    real ri = RSQ(r2);
+                    real q = d->h - (r2 * ri);
+                    real sc = coef * ((d->spiky * (q * q)) * ri);
+                    fx = FMA(-sc, dx, fx);
+                    fy = FMA(-sc, dy, fy);
+                    fz = FMA(-sc, dz, fz);
                 }
             }
             ax = ax + (fx * ir0);
@@ -1075,15 +1115,15 @@ ORC_EXPORT int SFX(orc_end)(orc_sys *s)
                 if (s->phase[j] != PS_PHASE_FLUID) continue;
                 real qx = s->P[4 * j], qy = s->P[4 * j + 1], qz = s->P[4 * j + 2];
                 real dx = px - qx, dy = py - qy, dz = pz - qz;
-                real r2 = ((dx * dx) + (dy * dy)) + (dz * dz);
+                real r2 = DIST2(dx, dy, dz);
                 if (!(r2 < d->h2)) continue;
                 real t = d->h2 - r2;
                 real W = d->poly6 * ((t * t) * t);
                 real m = W * (R(1) / s->w[j]);
                 real ex = qx - s->X[3 * j], ey = qy - s->X[3 * j + 1], ez = qz - s->X[3 * j + 2];
-                sx = sx + ((ex - ux) * m);
-                sy = sy + ((ey - uy) * m);
-                sz = sz + ((ez - uz) * m);
+                sx = FMA(ex - ux, m, sx);
+                sy = FMA(ey - uy, m, sy);
+                sz = FMA(ez - uz, m, sz);
             }
             real cr = d->xsph_c / s->rho0[i];
             vx = vx + ((cr * sx) / dt);

@@ -346,6 +346,37 @@ __global__ __launch_bounds__(kMaxBlock) void reorder_k(DevBuffers b, uint32_t n)
 }
 
 // ---------------------------------------------------------------------------
+// Pair arithmetic shared with the CPU oracle: squared distances and sums use
+// fused multiply-add, and 1/r comes from rsq() instead of sqrt + division
+// (an IEEE square root and division cost ~30 VALU per pair, rsq 12).
+
+// |d|^2 = fma(dz, dz, fma(dy, dy, dx dx))
+__device__ __forceinline__ float dist2(float dx, float dy, float dz)
+{
+    return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+}
+
+// 1/sqrt(x), x > 0 normal: integer seed 0x5f375a86 - (bits(x) >> 1) (the
+// "fast inverse square root" constant of C. Lomont, 2003) and three Newton
+// steps y <- y (3/2 - (x/2) y^2) as t = (x/2) y; u = fma(-t, y, 3/2);
+// y = y u.  Relative error about 1e-7.
+__device__ __forceinline__ float rsq(float x)
+{
+    float y = __uint_as_float(0x5f375a86u - (__float_as_uint(x) >> 1));
+    const float hx = 0.5f * x;
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        const float t = hx * y;
+        const float u = __builtin_fmaf(-t, y, 1.5f);
+        y = y * u;
+    }
+    return y;
+}
+
+// below this squared distance a pair has no direction
+constexpr float kDegenerateR2 = 1e-8f;
+
+// ---------------------------------------------------------------------------
 // Neighbour table (collideCell/collideCellRadius, integration_kernel.cuh:
 // 303-356,481-518): every j != i with r^2 < search_r2, scanning the 3x3 rows
 // of x-adjacent cells; a row's three cells are one contiguous slot range.
@@ -390,7 +421,7 @@ __global__ __launch_bounds__(kMaxBlock) void neighbors_k(DevBuffers b, KConst k,
                         const uint32_t j = jb + u;
                         if (j >= j1 || j == i) continue;
                         float dx = p.x - qv[u].x, dy_ = p.y - qv[u].y, dz_ = p.z - qv[u].z;
-                        float r2 = ((dx * dx) + (dy_ * dy_)) + (dz_ * dz_);
+                        float r2 = dist2(dx, dy_, dz_);
                         if (r2 < r2max) {
                             if (cnt < kmax) {
                                 const uint32_t sl = cnt & 3u;
@@ -428,8 +459,10 @@ __global__ __launch_bounds__(kMaxBlock) void neighbors_k(DevBuffers b, KConst k,
 
 // ---------------------------------------------------------------------------
 // a5: PBF multiplier (findLambdasD, integration_kernel.cuh:521-593)
-//   rho_i = W(0)/w_i + sum_j s_j W(r)/w_j,  grad_j = (spiky (h-r)^2 / r)/rho0_i d
-//   lambda_i = -(rho_i/rho0_i - 1) / (sum |grad_j|^2 + |sum grad_j|^2 + eps)
+//   rho_i = W(0)/w_i + sum_j s_j W(r)/w_j,  grad_j = a d,
+//   a = (spiky/rho0_i) (h-r)^2 / r,  1/r = rsq(r^2),  r = r^2 (1/r)
+//   lambda_i = -(rho_i/rho0_i - 1) / (sum a^2 r^2 + |sum grad_j|^2 + eps)
+// degenerate pairs (r^2 < 1e-8) add density only
 // UNIFORM: every particle is fluid with inverse mass wu (no Attr gathers).
 
 template <bool UNIFORM>
@@ -445,42 +478,43 @@ __global__ __launch_bounds__(kMaxBlock) void lambda_k(DevBuffers b, KConst k, ui
     }
     const float4 p = b.P[i];
     const float r0 = a.rho0;
-    const float ir0 = 1.0f / r0;
+    const float sk = k.spiky * (1.0f / r0);
     float rho = k.w_zero * (UNIFORM ? mu : (1.0f / a.w));
     float gix = 0.0f, giy = 0.0f, giz = 0.0f, den = 0.0f;
     const uint32_t c = b.ncnt[i];
-    auto body = [&](uint32_t, const float4 &pj, float wj, int32_t phj) {
+    // mj: 1/w_j, times s_solid for a non-fluid neighbour
+    auto body = [&](const float4 &pj, float mj) {
         float dx = p.x - pj.x, dy = p.y - pj.y, dz = p.z - pj.z;
-        float r2 = ((dx * dx) + (dy * dy)) + (dz * dz);
+        float r2 = dist2(dx, dy, dz);
         if (!(r2 < k.h2)) return;
         float t = k.h2 - r2;
         float W = k.poly6 * ((t * t) * t);
-        float cc = W * (UNIFORM ? mu : (1.0f / wj));
-        if (phj != PS_PHASE_FLUID) cc = cc * k.s_solid;
-        rho = rho + cc;
-        float r = sqrtf(r2);
-        if (r < 1e-4f) return;
-        float qq = k.h - r;
-        float aa = ((k.spiky * (qq * qq)) / r) * ir0;
-        float gx = aa * dx, gy = aa * dy, gz = aa * dz;
-        den = den + (((gx * gx) + (gy * gy)) + (gz * gz));
-        gix = gix + gx;
-        giy = giy + gy;
-        giz = giz + gz;
+        rho = __builtin_fmaf(W, mj, rho);
+        const float ri = rsq(r2);
+        const float qq = k.h - (r2 * ri);
+        // a degenerate pair adds exact zeros: its d components are tiny and
+        // finite, and sums that start at +0 never become -0, so x + 0 = x
+        const float aa = r2 < kDegenerateR2 ? 0.0f : (sk * (qq * qq)) * ri;
+        den = __builtin_fmaf(aa * aa, r2, den);
+        gix = __builtin_fmaf(aa, dx, gix);
+        giy = __builtin_fmaf(aa, dy, giy);
+        giz = __builtin_fmaf(aa, dz, giz);
     };
     if (UNIFORM) {
         for_neighbors(
             b, i, c, [&](uint32_t j) { return b.P[j]; },
-            [&](uint32_t j, const float4 &pj) { body(j, pj, 1.0f, PS_PHASE_FLUID); });
+            [&](uint32_t, const float4 &pj) { body(pj, mu); });
     } else {
         for_neighbors(
             b, i, c, [&](uint32_t j) { return PA{b.P[j], b.A[j]}; },
-            [&](uint32_t j, const PA &d) {
+            [&](uint32_t, const PA &d) {
                 if (d.a.w == 0.0f) return;
-                body(j, d.p, d.a.w, d.a.phase);
+                float mj = 1.0f / d.a.w;
+                if (d.a.phase != PS_PHASE_FLUID) mj = mj * k.s_solid;
+                body(d.p, mj);
             });
     }
-    den = den + (((gix * gix) + (giy * giy)) + (giz * giz));
+    den = den + dist2(gix, giy, giz);
     float C = (rho / r0) - 1.0f;
     Pw[4 * (size_t)i + 3] = -C / (den + k.relax);
 }
@@ -722,26 +756,28 @@ __global__ __launch_bounds__(kMaxBlock) void solve_k(DevBuffers b, KConst k, uin
     if (F & F_FLUID) {
         if (ph == PS_PHASE_FLUID) {
             const float li = p.w, ir0 = 1.0f / a.rho0;
+            // r^2 < 1e-8: gradient (0, -0.001 spiky h^2, 0) (integration_kernel.cuh:625-626)
+            const float gdeg = (k.spiky * (k.h * k.h)) * 0.001f;
             float fx = 0.0f, fy = 0.0f, fz = 0.0f;
             auto body = [&](const float4 &pj) {
                 float dx = px - pj.x, dy = py - pj.y, dz = pz - pj.z;
-                float r2 = ((dx * dx) + (dy * dy)) + (dz * dz);
+                float r2 = dist2(dx, dy, dz);
                 if (!(r2 < k.h2)) return;
                 float t = k.h2 - r2;
                 float W = k.poly6 * ((t * t) * t);
                 float corr = -(k.k_corr * ipow(W * k.inv_w_dq, NC ? NC : k.n_corr));
                 float coef = (li + pj.w) + corr;
-                float r = sqrtf(r2);
-                float qq = k.h - r;
-                float g = k.spiky * (qq * qq);
-                // r < 1e-4: gradient (0, -0.001 g, 0) (integration_kernel.cuh:625-626);
-                // selects rather than branches keep fx/fy/fz in registers
-                const bool deg = r < 1e-4f;
-                float aa = g / (deg ? 1.0f : r);
-                float tx = coef * (aa * dx), ty = coef * (aa * dy), tz = coef * (aa * dz);
-                fx = deg ? fx : (fx - tx);
-                fy = deg ? (fy - (coef * (g * 0.001f))) : (fy - ty);
-                fz = deg ? fz : (fz - tz);
+                // selects rather than branches keep fx/fy/fz in registers; a
+                // degenerate pair adds exact zeros (tiny finite d, sums that
+                // start at +0 never become -0) before its own y term
+                const bool deg = r2 < kDegenerateR2;
+                const float ri = rsq(r2);
+                const float qq = k.h - (r2 * ri);
+                const float sc = deg ? 0.0f : coef * ((k.spiky * (qq * qq)) * ri);
+                fx = __builtin_fmaf(-sc, dx, fx);
+                fy = __builtin_fmaf(-sc, dy, fy);
+                fz = __builtin_fmaf(-sc, dz, fz);
+                fy = deg ? (fy - (coef * gdeg)) : fy;
             };
             if (UNIFORM) {
                 for_neighbors(
@@ -931,15 +967,15 @@ __global__ __launch_bounds__(kMaxBlock) void finalize_k(DevBuffers b, KConst k, 
         const uint32_t c = b.ncnt[i];
         auto body = [&](const float4 &qj, const float4 &xj, float mj) {
             float dx = p.x - qj.x, dy = p.y - qj.y, dz = p.z - qj.z;
-            float r2 = ((dx * dx) + (dy * dy)) + (dz * dz);
+            float r2 = dist2(dx, dy, dz);
             if (!(r2 < k.h2)) return;
             float t = k.h2 - r2;
             float W = k.poly6 * ((t * t) * t);
             float m = W * mj;
             float ex = qj.x - xj.x, ey = qj.y - xj.y, ez = qj.z - xj.z;
-            sx = sx + ((ex - ux) * m);
-            sy = sy + ((ey - uy) * m);
-            sz = sz + ((ez - uz) * m);
+            sx = __builtin_fmaf(ex - ux, m, sx);
+            sy = __builtin_fmaf(ey - uy, m, sy);
+            sz = __builtin_fmaf(ez - uz, m, sz);
         };
         struct PX {
             float4 p, x;

@@ -425,3 +425,22 @@ def test_fp32_tracks_fp64_short_horizon():
         b.step(sc.dt, 1)
     err = np.abs(a.positions().astype(np.float64) - b.positions()).max()
     assert err < 1e-4, err
+
+
+def test_rsq_accuracy_and_fp64_form():
+    """1/r of the pair terms: the FP32 oracle's seeded Newton form is within
+    2.5e-7 relative of 1/sqrt(x) over the squared distances the solver
+    meets (1e-8 .. 16); the FP64 build uses 1/sqrt exactly."""
+    import ctypes
+    from oracle import _lib
+    f32, f64 = _lib("f32").orc_rsq_f32, _lib("f64").orc_rsq_f64
+    f32.restype, f32.argtypes = ctypes.c_float, [ctypes.c_float]
+    f64.restype, f64.argtypes = ctypes.c_double, [ctypes.c_double]
+    xs = np.float32(np.logspace(-8, np.log10(16.0), 20001))
+    xs = np.concatenate([xs, np.nextafter(xs, np.float32(np.inf)), np.float32([1.0, 0.25, 4.0, 2.0])])
+    got = np.array([f32(float(x)) for x in xs], np.float64)
+    ref = 1.0 / np.sqrt(xs.astype(np.float64))
+    rel = np.abs(got - ref) / ref
+    assert rel.max() < 2.5e-7, rel.max()
+    for x in (1e-8, 0.01, 0.04, 3.0):
+        assert f64(x) == 1.0 / np.sqrt(x)
