@@ -72,6 +72,8 @@ def parse():
                          "kernel), to scale HBM counters against a known byte count")
     ap.add_argument("--breakdown", type=int, default=20,
                     help="substeps of an extra, separately timed run with events on every kernel")
+    ap.add_argument("--no-staging", action="store_true",
+                    help="neighbour reads from global memory instead of LDS-staged cells")
     ap.add_argument("--traffic-report", nargs=3, metavar=("FETCH_CSV", "WRITE_CSV", "BENCH_LOG"),
                     help="summarise two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) of this "
                          "script with --copy-calibration; BENCH_LOG holds that run's JSON line")
@@ -369,6 +371,9 @@ def main():
 
     nx, ny, nz = (max(2, int(round(d * args.scale))) for d in (128, 64, 128))
     sc = scenes.dam_break(nx, ny, nz, max_neighbors=192)
+    if args.no_staging:
+        from particlesolver_amd import capi
+        sc.params.flags |= capi.PS_FLAG_NO_LDS_STAGING
     N, I = sc.n, sc.params.iterations
 
     g = System(sc.params)
@@ -391,6 +396,9 @@ def main():
     k_end = float(cnt.mean())
     K = 0.5 * (k_start + k_end)
     st = g.status()
+    occ = np.diff(g.grid()[2].astype(np.int64))
+    occupancy = {"max": int(occ.max()), "mean_nonempty": round(float(occ[occ > 0].mean()), 2),
+                 "particles_in_cells_over_16": int(occ[occ > 16].sum())}
     # diagnostic breakdown of every kernel (not part of the timed region)
     breakdown = {}
     if not args.no_timing and args.breakdown:
@@ -453,7 +461,7 @@ def main():
                                   sc.params.grid[2], I),
                    "particles": N, "iterations": I, "substeps_per_s": round(args.steps / el, 2),
                    "mean_neighbors": round(K, 2), "max_neighbors_seen": st.max_neighbors,
-                   "status_flags": st.flags},
+                   "cell_occupancy_end": occupancy, "status_flags": st.flags},
         "roofline": roof,
         "cpu_baseline": cpu,
         "kernels": kernels,

@@ -70,6 +70,10 @@ typedef enum ps_error {
  * through ps_add_fluid_object; grid and max_neighbors are unused. Set
  * PS_FLAG_SLEEPING too for the reference's behaviour (it always sleeps). */
 #define PS_FLAG_CPU_REFERENCE 0x4u
+/* Read neighbours from global memory even where the kernels could stage
+ * each workgroup's neighbour cells in LDS (uniform single-scene fluids).
+ * Results are identical either way. */
+#define PS_FLAG_NO_LDS_STAGING 0x8u
 
 /* Status bits raised by the kernels, read with ps_get_status. */
 #define PS_STATUS_NEIGHBOR_OVERFLOW 0x1u /* a particle had more than max_neighbors
@@ -145,6 +149,8 @@ typedef struct ps_status {
     uint32_t num_cells;
     uint64_t substeps;         /* substeps completed                              */
     uint64_t iterations;       /* solver iterations completed                     */
+    uint32_t units;            /* workgroups of the last neighbour build          */
+    uint32_t staged_units;     /* of those, read neighbours from LDS-staged cells */
 } ps_status;
 
 /* One scene of a batch (ps_create_batch): box (also its grid origin; the

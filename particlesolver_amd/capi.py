@@ -33,6 +33,7 @@ PS_ERR_UNSUPPORTED = -8
 PS_FLAG_REBUILD_EVERY_ITERATION = 0x1
 PS_FLAG_SLEEPING = 0x2
 PS_FLAG_CPU_REFERENCE = 0x4
+PS_FLAG_NO_LDS_STAGING = 0x8
 
 PS_STATUS_NEIGHBOR_OVERFLOW = 0x1
 PS_STATUS_NONFINITE = 0x2
@@ -107,7 +108,8 @@ class PsSceneDesc(ctypes.Structure):
 class PsStatus(ctypes.Structure):
     _fields_ = [("flags", ctypes.c_uint32), ("max_neighbors", ctypes.c_uint32),
                 ("num_particles", ctypes.c_uint32), ("num_cells", ctypes.c_uint32),
-                ("substeps", ctypes.c_uint64), ("iterations", ctypes.c_uint64)]
+                ("substeps", ctypes.c_uint64), ("iterations", ctypes.c_uint64),
+                ("units", ctypes.c_uint32), ("staged_units", ctypes.c_uint32)]
 
 
 _P = ctypes.c_void_p

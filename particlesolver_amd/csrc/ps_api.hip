@@ -305,6 +305,7 @@ void free_all(ps_system *s)
     dev_free(b.body_rest); dev_free(b.body_stiff); dev_free(b.body_mass); dev_free(b.body_q);
     dev_free(b.body_goal); dev_free(b.body_of); dev_free(b.member_of); dev_free(b.status);
     dev_free(b.sid); dev_free(b.sid2); dev_free(b.scenes); dev_free(b.lam); dev_free(b.bc);
+    dev_free(b.stage);
 }
 
 void drop_graph(ps_system *s)
@@ -408,8 +409,17 @@ int stage_sort(ps_system *s)
     return PS_OK;
 }
 
+// Neighbour cells are staged in LDS for uniform single-scene fluids (the
+// composition whose kernels have staged variants).
+bool staging_ok(const ps_system *s)
+{
+    return !s->cpuref && !(s->prm.flags & PS_FLAG_NO_LDS_STAGING) && s->kc.nscenes == 1 &&
+           uniform_fluid(s);
+}
+
 int stage_neighbors(ps_system *s)
 {
+    s->b.staging = staging_ok(s) ? 1u : 0u;
     LAUNCH(s, PS_K_NEIGHBORS,
            ps::launch_neighbors(s->stream, s->b, s->kc, s->n, s->kmax, s->block[PS_K_NEIGHBORS]));
     s->nbr_valid = true;
@@ -592,6 +602,24 @@ int upload_pins(ps_system *s)
     return PS_OK;
 }
 
+// Start keeping slot_of (insertion index -> storage slot) current: rebuilt
+// from the attributes once, then maintained by every reorder.
+int enable_slot_tracking(ps_system *s)
+{
+    if (s->b.track_slots) return PS_OK;
+    std::vector<Attr> A(s->n);
+    std::vector<uint32_t> slot(s->cap, 0);
+    HIP_TRY(s, hipStreamSynchronize(s->stream));
+    if (s->n) {
+        HIP_TRY(s, hipMemcpy(A.data(), s->b.A, sizeof(Attr) * s->n, hipMemcpyDeviceToHost));
+        for (uint32_t t = 0; t < s->n; t++) slot[A[t].orig] = t;
+        HIP_TRY(s, hipMemcpy(s->b.slot_of, slot.data(), sizeof(uint32_t) * s->n, hipMemcpyHostToDevice));
+    }
+    s->b.track_slots = 1;
+    drop_graph(s);
+    return PS_OK;
+}
+
 // Copy the sorted positions and insertion indices to the host.
 int read_sorted_positions(ps_system *s, std::vector<float4> &X, std::vector<Attr> &A)
 {
@@ -762,6 +790,7 @@ static int create_impl(const ps_params *params, uint32_t nscenes, const ps_scene
     ALLOC(b.nbr, groups * 64 * s->kmax); ALLOC(b.ncnt, c);
     ALLOC(b.dist_off, c + 1); ALLOC(b.occ, c); ALLOC(b.pin_of, c);
     ALLOC(b.body_of, c); ALLOC(b.member_of, c); ALLOC(b.status, 1);
+    ALLOC(b.stage, (c + ps::kStageUnit - 1) / ps::kStageUnit);
     ALLOC(b.scenes, nscenes);
     if (nscenes > 1) {
         ALLOC(b.sid, c);
@@ -782,6 +811,8 @@ static int create_impl(const ps_params *params, uint32_t nscenes, const ps_scene
     if (e == hipSuccess) e = hipMemset(b.body_of, 0, sizeof(uint32_t) * c);
     if (e == hipSuccess) e = hipMemset(b.member_of, 0, sizeof(uint32_t) * c);
     if (e == hipSuccess) e = hipMemset(b.status, 0, sizeof(ps::DevStatus));
+    if (e == hipSuccess)
+        e = hipMemset(b.stage, 0, sizeof(ps::StageDesc) * ((c + ps::kStageUnit - 1) / ps::kStageUnit));
     if (e == hipSuccess)
         e = hipMemcpy(b.scenes, scenes.data(), sizeof(ps::SceneDev) * nscenes, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipDeviceSynchronize();
@@ -1016,6 +1047,7 @@ int ps_add_distance_constraints(ps_system *s, uint32_t m, const uint32_t *pairs,
     }
     HIP_TRY(s, hipStreamSynchronize(s->stream));
     drop_graph(s);
+    if (int rc = enable_slot_tracking(s)) return rc;
     return upload_constraints(s);
 }
 
@@ -1134,6 +1166,7 @@ int ps_add_rigid_bodies(ps_system *s, uint32_t nbody, const uint32_t *offsets,
     std::vector<float4> rest(nm);
     for (size_t k = 0; k < nm; k++)
         rest[k] = make_float4(s->body_rest[3 * k], s->body_rest[3 * k + 1], s->body_rest[3 * k + 2], 0.f);
+    if (int rc = enable_slot_tracking(s)) return rc;
     HIP_TRY(s, hipMemcpy(b.body_q, q.data(), sizeof(float4) * nb, hipMemcpyHostToDevice));
     HIP_TRY(s, hipMemcpy(b.body_off, s->body_off.data(), sizeof(uint32_t) * s->body_off.size(), hipMemcpyHostToDevice));
     HIP_TRY(s, hipMemcpy(b.body_idx, s->body_idx.data(), sizeof(uint32_t) * nm, hipMemcpyHostToDevice));
@@ -1286,8 +1319,14 @@ int ps_step_graph(ps_system *s, float dt, uint32_t substeps)
         s->stage = 0;
         s->it_sub = 0;
         s->cr_swapped = false;
-        const bool rotated_back = memcmp(&start, &s->b, sizeof(DevBuffers)) == 0;
+        // the buffer pointers must come back to where the capture began; the
+        // staging mode is whatever the captured neighbour builds chose
+        DevBuffers end = s->b, begin = start;
+        end.staging = begin.staging = 0;
+        const bool rotated_back = memcmp(&begin, &end, sizeof(DevBuffers)) == 0;
+        const uint32_t staging = s->b.staging;
         s->b = start;
+        s->b.staging = staging;
         if (rc != PS_OK || e != hipSuccess || !rotated_back) {
             if (g) (void)hipGraphDestroy(g);
             if (rc != PS_OK) return rc;
@@ -1397,11 +1436,27 @@ int ps_read_neighbors(ps_system *s, uint32_t *counts, uint32_t *table)
         const uint32_t kq = s->b.kq;
         std::vector<uint32_t> raw(groups * 64 * s->kmax);
         HIP_TRY(s, hipMemcpy(raw.data(), s->b.nbr, sizeof(uint32_t) * raw.size(), hipMemcpyDeviceToHost));
-        for (uint32_t i = 0; i < s->n; i++)
+        // staged units hold LDS indices: map them back through their intervals
+        std::vector<ps::StageDesc> desc;
+        if (s->b.staging) {
+            desc.resize((s->n + ps::kStageUnit - 1) / ps::kStageUnit);
+            HIP_TRY(s, hipMemcpy(desc.data(), s->b.stage, sizeof(ps::StageDesc) * desc.size(),
+                                 hipMemcpyDeviceToHost));
+        }
+        for (uint32_t i = 0; i < s->n; i++) {
+            const ps::StageDesc *d = desc.empty() ? nullptr : &desc[i / ps::kStageUnit];
+            if (d && !d->staged) d = nullptr;
             for (uint32_t k = 0; k < s->kmax; k++) {
                 const size_t at = (((size_t)(i >> 6) * kq + (k >> 2)) * 64 + (i & 63)) * 4 + (k & 3);
-                table[(size_t)i * s->kmax + k] = k < cnt[i] ? raw[at] : 0u;
+                uint32_t e = k < cnt[i] ? raw[at] : 0u;
+                if (d && k < cnt[i]) {
+                    const uint32_t c = e / ps::kStageChunk;
+                    e = c < d->nchunks && c < ps::kStageChunks ? d->src[c] + e % ps::kStageChunk
+                                                               : 0xffffffffu;
+                }
+                table[(size_t)i * s->kmax + k] = e;
             }
+        }
     }
     return PS_OK;
 }
@@ -1440,6 +1495,13 @@ int ps_get_status(ps_system *s, ps_status *out)
     out->num_cells = s->ncell;
     out->substeps = s->substeps;
     out->iterations = ds.iter + s->it_sub;
+    if (s->b.staging && s->n && !s->cpuref) {
+        std::vector<ps::StageDesc> desc((s->n + ps::kStageUnit - 1) / ps::kStageUnit);
+        HIP_TRY(s, hipMemcpy(desc.data(), s->b.stage, sizeof(ps::StageDesc) * desc.size(),
+                             hipMemcpyDeviceToHost));
+        out->units = (uint32_t)desc.size();
+        for (const ps::StageDesc &d : desc) out->staged_units += d.staged ? 1u : 0u;
+    }
     return PS_OK;
 }
 

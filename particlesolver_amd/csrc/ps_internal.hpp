@@ -79,6 +79,27 @@ enum SolveFeature : uint32_t {
     F_PIN = 16u,
 };
 
+// LDS staging of neighbour cells (ps_kernels.hip).  A unit is one
+// workgroup of kStageUnit consecutive sorted particles; its particles lie in
+// at most kStageRows grid rows, and the candidates of a particle in row q
+// and stencil row t = (dz+1)*3 + (dy+1) form a slot range inside interval
+// 9q + t: the slots of cells x in [xa_q - 1, xb_q + 1] of the target row.
+// The intervals are laid out in LDS back to back, each padded to a whole
+// number of kStageChunk-slot chunks; a staged unit's table holds LDS
+// indices, and src[c] is the global slot of the first element of chunk c.
+constexpr uint32_t kStageUnit = 256;
+constexpr uint32_t kStageRows = 3;
+constexpr uint32_t kStageIv = 9 * kStageRows;
+constexpr uint32_t kStageSlots = 3072;
+constexpr uint32_t kStageChunk = 16;
+constexpr uint32_t kStageChunks = kStageSlots / kStageChunk;
+
+struct StageDesc {
+    uint32_t staged;              // 1: the unit's table entries are LDS indices
+    uint32_t nchunks;             // chunks in use
+    uint32_t src[kStageChunks];   // global slot of each chunk's first element
+};
+
 struct DevBuffers {
     // particle state, sorted storage order
     float4 *X, *X2;  // positions at substep start (x*), .w unused
@@ -88,7 +109,9 @@ struct DevBuffers {
     uint32_t *key, *key2;  // cell of each slot
     uint32_t *rank;        // arrival rank inside the cell (from the hash atomics)
     uint32_t *tmp_orig;    // insertion index by provisional sorted slot
-    uint32_t *slot_of;     // insertion index -> current slot
+    uint32_t *slot_of;     // insertion index -> current slot (kept only while
+                           // track_slots: distance constraints or bodies exist)
+    uint32_t track_slots;
     uint32_t *sid, *sid2;  // scene of each slot (batches only)
     SceneDev *scenes;
     // grid
@@ -119,6 +142,9 @@ struct DevBuffers {
     uint32_t *body_of;     // per particle: body + 1
     uint32_t *member_of;   // per particle: member index
     DevStatus *status;
+    // LDS staging (uniform single-scene fluid): one descriptor per unit
+    StageDesc *stage;
+    uint32_t staging;      // 1: the last neighbour build staged its units
     // CPU-reference mode only
     float *lam;            // lambda per particle
     uint32_t *bc;          // boundary flags | boundary count << 8

@@ -114,6 +114,10 @@ struct PA {
     Attr a;
 };
 
+struct PX {
+    float4 p, x;
+};
+
 // entry q of particle i (single-entry access for the less travelled loops)
 __device__ __forceinline__ uint32_t nbr_at(const DevBuffers &b, uint32_t i, uint32_t q)
 {
@@ -146,7 +150,7 @@ template <bool PRED, bool HASH>
 __global__ __launch_bounds__(kMaxBlock) void predict_hash_k(DevBuffers b, KConst k, uint32_t n,
                                                          float dt)
 {
-    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t s = xcd_block() * blockDim.x + threadIdx.x;
     const bool valid = s < n;
     const bool multi = k.nscenes > 1u;
     const SceneDev *sc = (multi && valid) ? &b.scenes[b.sid[s]] : nullptr;
@@ -212,8 +216,13 @@ __global__ __launch_bounds__(kMaxBlock) void predict_hash_k(DevBuffers b, KConst
         if (valid) {
             b.key[s] = key;
             b.rank[s] = base + (lane - seg);
-            if (flags) atomicOr(&b.status->flags, flags);
         }
+        // status bits: one atomic per wavefront, not per particle (a splash
+        // can put thousands of predictions outside the grid at once)
+        const uint64_t nf = __ballot((flags & PS_STATUS_NONFINITE) != 0u);
+        const uint64_t og = __ballot((flags & PS_STATUS_OUTSIDE_GRID) != 0u);
+        if (lane == 0 && (nf | og))
+            atomicOr(&b.status->flags, (nf ? PS_STATUS_NONFINITE : 0u) | (og ? PS_STATUS_OUTSIDE_GRID : 0u));
     }
 }
 
@@ -342,7 +351,8 @@ __global__ __launch_bounds__(kMaxBlock) void reorder_k(DevBuffers b, uint32_t n)
     b.A2[d] = a;
     b.key2[d] = c;
     if (b.sid) b.sid2[d] = b.sid[s];
-    b.slot_of[a.orig] = d;
+    // a scatter by insertion index: skipped when no kernel reads it
+    if (b.track_slots) b.slot_of[a.orig] = d;
 }
 
 // ---------------------------------------------------------------------------
@@ -381,6 +391,76 @@ constexpr float kDegenerateR2 = 1e-8f;
 // 303-356,481-518): every j != i with r^2 < search_r2, scanning the 3x3 rows
 // of x-adjacent cells; a row's three cells are one contiguous slot range.
 
+// Scan the nine stencil rows of particle i in order (t = (dz+1)*3 + dy+1)
+// and write its table: every slot j != i of range(t) = [j0, j1) with
+// |p - P[j]|^2 < search_r2, in ascending j, stored as emit(t, j) (the slot
+// itself, or its LDS index in a staged unit); pad is i's own entry, which
+// fills the last chunk.  Returns the number found (the table keeps the
+// first kmax).
+template <typename Range, typename Emit>
+__device__ __forceinline__ uint32_t scan_stencil(const DevBuffers &b, const KConst &k, uint32_t i,
+                                                 uint32_t pad, const float4 &p, uint32_t kmax,
+                                                 Range range, Emit emit)
+{
+    uint4 *out = const_cast<uint4 *>(nbr_lane(b, i));
+    uint4 buf = make_uint4(pad, pad, pad, pad);
+    uint32_t cnt = 0;
+    const float r2max = k.search_r2;
+    for (uint32_t t = 0; t < 9; t++) {
+        uint32_t j0, j1;
+        if (!range(t, j0, j1)) continue;
+        for (uint32_t jb = j0; jb < j1; jb += 4) {
+            float4 qv[4];
+#pragma unroll
+            for (uint32_t u = 0; u < 4; u++) qv[u] = b.P[min(jb + u, j1 - 1)];
+            // the distances first, for all four: loads stay ahead of the branches
+            float r2[4];
+#pragma unroll
+            for (uint32_t u = 0; u < 4; u++)
+                r2[u] = dist2(p.x - qv[u].x, p.y - qv[u].y, p.z - qv[u].z);
+#pragma unroll
+            for (uint32_t u = 0; u < 4; u++) {
+                const uint32_t j = jb + u;
+                if (j < j1 && j != i && r2[u] < r2max) {
+                    if (cnt < kmax) {
+                        const uint32_t e = emit(t, j);
+                        const uint32_t sl = cnt & 3u;
+                        buf.x = sl == 0 ? e : buf.x;
+                        buf.y = sl == 1 ? e : buf.y;
+                        buf.z = sl == 2 ? e : buf.z;
+                        buf.w = sl == 3 ? e : buf.w;
+                        if (sl == 3) {
+                            out[(size_t)(cnt >> 2) * 64] = buf;
+                            buf = make_uint4(pad, pad, pad, pad);
+                        }
+                    }
+                    cnt++;
+                }
+            }
+        }
+    }
+    const uint32_t kept = cnt < kmax ? cnt : kmax;
+    if (kept & 3u) out[(size_t)(kept >> 2) * 64] = buf;
+    b.ncnt[i] = kept;
+    return cnt;
+}
+
+// one status update per block: max count and overflow
+__device__ __forceinline__ void neighbor_status(const DevBuffers &b, uint32_t cnt, uint32_t kmax,
+                                                uint32_t *ldsw)
+{
+    uint32_t m = wave_max(cnt);
+    const uint32_t w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) ldsw[w] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t mm = ldsw[0];
+        for (uint32_t q = 1; q < (blockDim.x >> 6); q++) mm = ldsw[q] > mm ? ldsw[q] : mm;
+        if (mm > 0) atomicMax(&b.status->max_neighbors, mm);
+        if (mm > kmax) atomicOr(&b.status->flags, (uint32_t)PS_STATUS_NEIGHBOR_OVERFLOW);
+    }
+}
+
 __global__ __launch_bounds__(kMaxBlock) void neighbors_k(DevBuffers b, KConst k, uint32_t n,
                                                       uint32_t kmax)
 {
@@ -399,62 +479,182 @@ __global__ __launch_bounds__(kMaxBlock) void neighbors_k(DevBuffers b, KConst k,
         const uint32_t key = b.key[i] - base;
         const uint32_t cx = key % gx, cy = (key / gx) % gy, cz = key / (gx * gy);
         const uint32_t x0 = cx > 0 ? cx - 1 : 0, x1 = cx + 1 < gx ? cx + 1 : gx - 1;
-        const float4 p = b.P[i];
-        const float r2max = k.search_r2;
-        uint4 *out = const_cast<uint4 *>(nbr_lane(b, i));
-        uint4 buf = make_uint4(i, i, i, i);
-        for (int dz = -1; dz <= 1; dz++) {
-            const int z = (int)cz + dz;
-            if (z < 0 || z >= (int)gz) continue;
-            for (int dy = -1; dy <= 1; dy++) {
-                const int y = (int)cy + dy;
-                if (y < 0 || y >= (int)gy) continue;
-                const uint32_t row = base + ((uint32_t)z * gy + (uint32_t)y) * gx;
-                const uint32_t j0 = b.cell_off[row + x0];
-                const uint32_t j1 = min(b.cell_off[row + x1 + 1], n);
-                for (uint32_t jb = j0; jb < j1; jb += 4) {
-                    float4 qv[4];
+        auto range = [&](uint32_t t, uint32_t &j0, uint32_t &j1) {
+            const int z = (int)cz + (int)(t / 3) - 1, y = (int)cy + (int)(t % 3) - 1;
+            if (z < 0 || z >= (int)gz || y < 0 || y >= (int)gy) return false;
+            const uint32_t row = base + ((uint32_t)z * gy + (uint32_t)y) * gx;
+            j0 = b.cell_off[row + x0];
+            j1 = min(b.cell_off[row + x1 + 1], n);
+            return j0 < j1;
+        };
+        cnt = scan_stencil(b, k, i, i, b.P[i], kmax, range, [](uint32_t, uint32_t j) { return j; });
+    }
+    neighbor_status(b, cnt, kmax, ldsw);
+}
+
+// Copy a staged unit's chunks of the current data into LDS.  Thread t
+// handles element t % 16 of chunks c = 16 k + t / 16, BATCH chunks' loads in
+// flight at a time; a chunk's elements past its interval read a clamped slot
+// and fill padding that no table entry refers to.  load(j) reads slot j,
+// store(l, v) writes LDS index l.  Every thread of the block calls it.
+template <uint32_t BATCH, typename Load, typename Store>
+__device__ __forceinline__ void stage_copy(const StageDesc *d, uint32_t n, Load load, Store store)
+{
+    constexpr uint32_t kPerStep = kStageUnit / kStageChunk;  // chunks per step
+    constexpr uint32_t kSteps = kStageChunks / kPerStep;
+    const uint32_t nch = d->nchunks, sub = threadIdx.x % kStageChunk;
+    const uint32_t c0 = threadIdx.x / kStageChunk;
 #pragma unroll
-                    for (uint32_t u = 0; u < 4; u++) qv[u] = b.P[min(jb + u, j1 - 1)];
+    for (uint32_t k0 = 0; k0 < kSteps; k0 += BATCH) {
+        if (k0 * kPerStep >= nch) break;
+        uint32_t src[BATCH];
 #pragma unroll
-                    for (uint32_t u = 0; u < 4; u++) {
-                        const uint32_t j = jb + u;
-                        if (j >= j1 || j == i) continue;
-                        float dx = p.x - qv[u].x, dy_ = p.y - qv[u].y, dz_ = p.z - qv[u].z;
-                        float r2 = dist2(dx, dy_, dz_);
-                        if (r2 < r2max) {
-                            if (cnt < kmax) {
-                                const uint32_t sl = cnt & 3u;
-                                buf.x = sl == 0 ? j : buf.x;
-                                buf.y = sl == 1 ? j : buf.y;
-                                buf.z = sl == 2 ? j : buf.z;
-                                buf.w = sl == 3 ? j : buf.w;
-                                if (sl == 3) {
-                                    out[(size_t)(cnt >> 2) * 64] = buf;
-                                    buf = make_uint4(i, i, i, i);
-                                }
-                            }
-                            cnt++;
-                        }
-                    }
+        for (uint32_t k = 0; k < BATCH; k++) {
+            const uint32_t c = (k0 + k) * kPerStep + c0;
+            src[k] = c < nch ? d->src[c] : 0u;
+        }
+        decltype(load(0u)) v[BATCH];
+#pragma unroll
+        for (uint32_t k = 0; k < BATCH; k++) v[k] = load(min(src[k] + sub, n - 1u));
+#pragma unroll
+        for (uint32_t k = 0; k < BATCH; k++) {
+            const uint32_t c = (k0 + k) * kPerStep + c0;
+            if (c < nch) store(c * kStageChunk + sub, v[k]);
+        }
+    }
+    __syncthreads();
+}
+
+// Staged build (one single-scene grid): plan the unit's intervals and its
+// chunk table for the kernels that copy them into LDS, then scan the
+// stencil in global memory (full occupancy; this kernel reads each
+// candidate once per particle, so LDS would not save it traffic) and store
+// each neighbour as its LDS index j + base - start of its interval.  A unit
+// spanning more than kStageRows rows or kStageSlots slots keeps global slots
+// (staged = 0).
+__global__ __launch_bounds__(kStageUnit) void neighbors_staged_k(DevBuffers b, KConst k, uint32_t n,
+                                                              uint32_t kmax)
+{
+    __shared__ uint32_t s_start[kStageIv], s_base[kStageIv + 1], s_len[kStageIv];
+    __shared__ uint32_t s_row[kStageRows], s_xa[kStageRows], s_xb[kStageRows];
+    __shared__ uint32_t s_wheads[kStageUnit / 64], s_wlast[kStageUnit / 64];
+    __shared__ uint32_t s_staged;
+    __shared__ StageDesc s_desc;
+    __shared__ uint32_t ldsw[kStageUnit / 64];
+    const uint32_t unit = xcd_block();
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
+    const uint32_t i = unit * kStageUnit + tid;
+    const bool valid = i < n;  // the valid threads are a prefix of the unit
+    const uint32_t gx = k.grid[0], gy = k.grid[1], gz = k.grid[2];
+    const uint32_t key = valid ? b.key[i] : 0u;
+    const uint32_t row = key / gx, x = key % gx;  // row = z gy + y
+    const uint32_t ry = row % gy, rz = row / gy;
+    const uint32_t x0 = x > 0 ? x - 1 : 0, x1 = x + 1 < gx ? x + 1 : gx - 1;
+
+    // rows of the unit: runs of equal row in the (sorted) keys
+    uint32_t prev = __shfl_up(row, 1, 64);
+    if (lane == 63) s_wlast[w] = row;
+    if (tid < kStageRows) {
+        s_xa[tid] = 0xffffffffu;
+        s_xb[tid] = 0u;
+    }
+    __syncthreads();
+    if (lane == 0 && w > 0) prev = s_wlast[w - 1];
+    const bool head = valid && (tid == 0 || row != prev);
+    const uint64_t hm = __ballot(head);
+    if (lane == 0) s_wheads[w] = (uint32_t)__popcll(hm);
+    __syncthreads();
+    uint32_t before = 0, nrows = 0;
+    for (uint32_t q = 0; q < kStageUnit / 64; q++) {
+        before += q < w ? s_wheads[q] : 0u;
+        nrows += s_wheads[q];
+    }
+    const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+    const uint32_t q = before + (uint32_t)__popcll(hm & upto) - 1u;  // this particle's row
+    if (valid && nrows <= kStageRows) {
+        if (head) s_row[q] = row;
+        atomicMin(&s_xa[q], x);
+        atomicMax(&s_xb[q], x);
+    }
+    __syncthreads();
+
+    // intervals: thread 9q + t plans target row t of unit row q
+    const uint32_t niv = nrows <= kStageRows ? 9u * nrows : 0u;
+    if (tid < niv) {
+        const uint32_t qq = tid / 9u, t = tid % 9u;
+        const uint32_t r = s_row[qq], rry = r % gy, rrz = r / gy;
+        const int z = (int)rrz + (int)(t / 3) - 1, y = (int)rry + (int)(t % 3) - 1;
+        uint32_t st = 0, len = 0;
+        if (z >= 0 && z < (int)gz && y >= 0 && y < (int)gy) {
+            const uint32_t r2 = ((uint32_t)z * gy + (uint32_t)y) * gx;
+            const uint32_t xa = s_xa[qq], xb = s_xb[qq];
+            st = b.cell_off[r2 + (xa > 0 ? xa - 1 : 0)];
+            const uint32_t en = min(b.cell_off[r2 + (xb + 1 < gx ? xb + 1 : gx - 1) + 1], n);
+            len = en > st ? en - st : 0u;
+        }
+        s_start[tid] = st;
+        s_len[tid] = len;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        // LDS bases: intervals padded to whole chunks
+        uint32_t acc = 0;
+        for (uint32_t iv = 0; iv < niv; iv++) {
+            s_base[iv] = acc;
+            acc += (s_len[iv] + kStageChunk - 1) / kStageChunk * kStageChunk;
+        }
+        s_base[niv] = acc;
+        s_staged = (niv > 0 && acc <= kStageSlots) ? 1u : 0u;
+    }
+    __syncthreads();
+    const bool staged = s_staged != 0u;
+    const uint32_t nch = staged ? s_base[niv] / kStageChunk : 0u;
+    if (tid == 0) {
+        s_desc.staged = staged ? 1u : 0u;
+        s_desc.nchunks = nch;
+    }
+    // chunk table: chunk c lies in the interval whose padded region holds it
+    for (uint32_t c = tid; c < nch; c += kStageUnit) {
+        uint32_t iv = 0;
+        while (iv + 1 < niv && s_base[iv + 1] <= c * kStageChunk) iv++;
+        s_desc.src[c] = s_start[iv] + (c * kStageChunk - s_base[iv]);
+    }
+    __syncthreads();
+    // the descriptor the other staged kernels of this substep read
+    StageDesc *desc = b.stage + unit;
+    if (tid == 0) {
+        desc->staged = s_desc.staged;
+        desc->nchunks = nch;
+    }
+    for (uint32_t c = tid; c < nch; c += kStageUnit) desc->src[c] = s_desc.src[c];
+
+    uint32_t cnt = 0;
+    if (valid) {
+        auto range = [&](uint32_t t, uint32_t &j0, uint32_t &j1) {
+            const int z = (int)rz + (int)(t / 3) - 1, y = (int)ry + (int)(t % 3) - 1;
+            if (z < 0 || z >= (int)gz || y < 0 || y >= (int)gy) return false;
+            const uint32_t r2 = ((uint32_t)z * gy + (uint32_t)y) * gx;
+            j0 = b.cell_off[r2 + x0];
+            j1 = min(b.cell_off[r2 + x1 + 1], n);
+            if (j0 >= j1) return false;
+            if (staged) {
+                // the row range lies inside its interval (by construction)
+                const uint32_t iv = 9u * q + t;
+                if (j0 < s_start[iv] || j1 - s_start[iv] > s_len[iv]) {
+                    atomicOr(&b.status->flags, PS_STATUS_INTERNAL);
+                    return false;
                 }
             }
-        }
-        const uint32_t kept = cnt < kmax ? cnt : kmax;
-        if (kept & 3u) out[(size_t)(kept >> 2) * 64] = buf;
-        b.ncnt[i] = kept;
+            return true;
+        };
+        // LDS index = slot + (base - start) of the stencil row's interval
+        auto emit = [&](uint32_t t, uint32_t j) {
+            const uint32_t iv = 9u * q + t;
+            return staged ? j + (s_base[iv] - s_start[iv]) : j;
+        };
+        cnt = scan_stencil(b, k, i, emit(4, i), b.P[i], kmax, range, emit);
     }
-    // one status update per block
-    uint32_t m = wave_max(cnt);
-    const uint32_t w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) ldsw[w] = m;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t mm = ldsw[0];
-        for (uint32_t q = 1; q < (blockDim.x >> 6); q++) mm = ldsw[q] > mm ? ldsw[q] : mm;
-        if (mm > 0) atomicMax(&b.status->max_neighbors, mm);
-        if (mm > kmax) atomicOr(&b.status->flags, (uint32_t)PS_STATUS_NEIGHBOR_OVERFLOW);
-    }
+    neighbor_status(b, cnt, kmax, ldsw);
 }
 
 // ---------------------------------------------------------------------------
@@ -465,10 +665,20 @@ __global__ __launch_bounds__(kMaxBlock) void neighbors_k(DevBuffers b, KConst k,
 // degenerate pairs (r^2 < 1e-8) add density only
 // UNIFORM: every particle is fluid with inverse mass wu (no Attr gathers).
 
-template <bool UNIFORM>
+template <bool UNIFORM, bool STAGED>
 __global__ __launch_bounds__(kMaxBlock) void lambda_k(DevBuffers b, KConst k, uint32_t n, float mu)
 {
-    const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
+    __shared__ float4 lds[STAGED ? kStageSlots : 1];
+    const uint32_t unit = xcd_block();
+    const uint32_t i = unit * blockDim.x + threadIdx.x;
+    bool staged = false;
+    if (STAGED) {
+        const StageDesc *d = b.stage + unit;
+        staged = d->staged != 0u;
+        if (staged)
+            stage_copy<6>(d, n, [&](uint32_t j) { return b.P[j]; },
+                          [&](uint32_t l, const float4 &v) { lds[l] = v; });
+    }
     if (i >= n) return;
     float *Pw = reinterpret_cast<float *>(b.P);
     const Attr a = b.A[i];
@@ -500,7 +710,11 @@ __global__ __launch_bounds__(kMaxBlock) void lambda_k(DevBuffers b, KConst k, ui
         giy = __builtin_fmaf(aa, dy, giy);
         giz = __builtin_fmaf(aa, dz, giz);
     };
-    if (UNIFORM) {
+    if (STAGED && staged) {
+        for_neighbors(
+            b, i, c, [&](uint32_t l) { return lds[l]; },
+            [&](uint32_t, const float4 &pj) { body(pj, mu); });
+    } else if (UNIFORM) {
         for_neighbors(
             b, i, c, [&](uint32_t j) { return b.P[j]; },
             [&](uint32_t, const float4 &pj) { body(pj, mu); });
@@ -661,11 +875,21 @@ __device__ __forceinline__ void walls(const KConst &k, const WallBox &wb, int32_
 // (a6), distance (a9), shape goal (a12) summed in that order from the same
 // snapshot P, then walls (a8) and pins (a10); result to Q.
 
-template <uint32_t F, bool UNIFORM, uint32_t NC>
+template <uint32_t F, bool UNIFORM, uint32_t NC, bool STAGED>
 __global__ __launch_bounds__(kMaxBlock) void solve_k(DevBuffers b, KConst k, uint32_t n, float wu,
                                                   uint32_t it)
 {
-    const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
+    __shared__ float4 lds[STAGED ? kStageSlots : 1];
+    const uint32_t unit = xcd_block();
+    const uint32_t i = unit * blockDim.x + threadIdx.x;
+    bool staged = false;
+    if (STAGED) {
+        const StageDesc *d = b.stage + unit;
+        staged = d->staged != 0u;
+        if (staged)
+            stage_copy<6>(d, n, [&](uint32_t j) { return b.P[j]; },
+                          [&](uint32_t l, const float4 &v) { lds[l] = v; });
+    }
     if (i >= n) return;
     const float4 p = b.P[i];
     const Attr a = b.A[i];
@@ -779,7 +1003,11 @@ __global__ __launch_bounds__(kMaxBlock) void solve_k(DevBuffers b, KConst k, uin
                 fz = __builtin_fmaf(-sc, dz, fz);
                 fy = deg ? (fy - (coef * gdeg)) : fy;
             };
-            if (UNIFORM) {
+            if (STAGED && staged) {
+                for_neighbors(
+                    b, i, c, [&](uint32_t l) { return lds[l]; },
+                    [&](uint32_t, const float4 &pj) { body(pj); });
+            } else if (UNIFORM) {
                 for_neighbors(
                     b, i, c, [&](uint32_t j) { return b.P[j]; },
                     [&](uint32_t, const float4 &pj) { body(pj); });
@@ -950,11 +1178,27 @@ __global__ __launch_bounds__(kBlock) void shape_k(DevBuffers b, uint32_t nbody)
 // plus XSPH over fluid neighbours from displacements u = x - x*:
 //   v_i = u_i/dt + ((c/rho0_i) sum_j (u_j - u_i) m_j W_ij)/dt
 
-template <bool UNIFORM>
+template <bool UNIFORM, bool STAGED>
 __global__ __launch_bounds__(kMaxBlock) void finalize_k(DevBuffers b, KConst k, uint32_t n, float mu,
                                                      float dt, uint32_t its)
 {
-    const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
+    // staged: neighbour position and displacement u = p - x* (x, y, z, u.x)
+    // and (u.y, u.z), formed with the same subtractions as the global path
+    __shared__ float4 lds_pu[STAGED ? kStageSlots : 1];
+    __shared__ float2 lds_u[STAGED ? kStageSlots : 1];
+    const uint32_t unit = xcd_block();
+    const uint32_t i = unit * blockDim.x + threadIdx.x;
+    bool staged = false;
+    if (STAGED) {
+        const StageDesc *d = b.stage + unit;
+        staged = d->staged != 0u;
+        if (staged)
+            stage_copy<4>(d, n, [&](uint32_t j) { return PX{b.P[j], b.X[j]}; },
+                          [&](uint32_t l, const PX &v) {
+                              lds_pu[l] = make_float4(v.p.x, v.p.y, v.p.z, v.p.x - v.x.x);
+                              lds_u[l] = make_float2(v.p.y - v.x.y, v.p.z - v.x.z);
+                          });
+    }
     if (i == 0) b.status->iter += its;  // every solve of this substep has completed
     if (i >= n) return;
     const float4 p = b.P[i];
@@ -965,35 +1209,42 @@ __global__ __launch_bounds__(kMaxBlock) void finalize_k(DevBuffers b, KConst k, 
     if ((UNIFORM || a.phase == PS_PHASE_FLUID) && k.xsph_c != 0.0f) {
         float sx = 0.0f, sy = 0.0f, sz = 0.0f;
         const uint32_t c = b.ncnt[i];
-        auto body = [&](const float4 &qj, const float4 &xj, float mj) {
+        // qj: neighbour position, e: its displacement over the substep
+        auto body = [&](const float4 &qj, float ex, float ey, float ez, float mj) {
             float dx = p.x - qj.x, dy = p.y - qj.y, dz = p.z - qj.z;
             float r2 = dist2(dx, dy, dz);
             if (!(r2 < k.h2)) return;
             float t = k.h2 - r2;
             float W = k.poly6 * ((t * t) * t);
             float m = W * mj;
-            float ex = qj.x - xj.x, ey = qj.y - xj.y, ez = qj.z - xj.z;
             sx = __builtin_fmaf(ex - ux, m, sx);
             sy = __builtin_fmaf(ey - uy, m, sy);
             sz = __builtin_fmaf(ez - uz, m, sz);
         };
-        struct PX {
-            float4 p, x;
+        struct PU {
+            float4 pu;
+            float2 u;
         };
         struct PXA {
             float4 p, x;
             Attr a;
         };
-        if (UNIFORM) {
+        if (STAGED && staged) {
+            for_neighbors(
+                b, i, c, [&](uint32_t l) { return PU{lds_pu[l], lds_u[l]}; },
+                [&](uint32_t, const PU &d) { body(d.pu, d.pu.w, d.u.x, d.u.y, mu); });
+        } else if (UNIFORM) {
             for_neighbors(
                 b, i, c, [&](uint32_t j) { return PX{b.P[j], b.X[j]}; },
-                [&](uint32_t, const PX &d) { body(d.p, d.x, mu); });
+                [&](uint32_t, const PX &d) {
+                    body(d.p, d.p.x - d.x.x, d.p.y - d.x.y, d.p.z - d.x.z, mu);
+                });
         } else {
             for_neighbors(
                 b, i, c, [&](uint32_t j) { return PXA{b.P[j], b.X[j], b.A[j]}; },
                 [&](uint32_t, const PXA &d) {
                     if (d.a.phase != PS_PHASE_FLUID) return;
-                    body(d.p, d.x, 1.0f / d.a.w);
+                    body(d.p, d.p.x - d.x.x, d.p.y - d.x.y, d.p.z - d.x.z, 1.0f / d.a.w);
                 });
         }
         const float cr = k.xsph_c / a.rho0;
@@ -1071,7 +1322,11 @@ hipError_t launch_neighbors(hipStream_t st, const DevBuffers &b, const KConst &k
                             uint32_t kmax, uint32_t bs)
 {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(neighbors_k, dim3(blocks_for(n, bs)), dim3(bs), 0, st, b, k, n, kmax);
+    if (b.staging)
+        hipLaunchKernelGGL(neighbors_staged_k, dim3(blocks_for(n, kStageUnit)), dim3(kStageUnit), 0,
+                           st, b, k, n, kmax);
+    else
+        hipLaunchKernelGGL(neighbors_k, dim3(blocks_for(n, bs)), dim3(bs), 0, st, b, k, n, kmax);
     return hipGetLastError();
 }
 
@@ -1081,10 +1336,13 @@ hipError_t launch_lambda(hipStream_t st, const DevBuffers &b, const KConst &k, u
     if (n == 0) return hipSuccess;
     const dim3 g(blocks_for(n, bs)), t(bs);
     const float mu = 1.0f / wu;
-    if (uniform)
-        hipLaunchKernelGGL((lambda_k<true>), g, t, 0, st, b, k, n, mu);
+    if (uniform && b.staging)
+        hipLaunchKernelGGL((lambda_k<true, true>), dim3(blocks_for(n, kStageUnit)), dim3(kStageUnit),
+                           0, st, b, k, n, mu);
+    else if (uniform)
+        hipLaunchKernelGGL((lambda_k<true, false>), g, t, 0, st, b, k, n, mu);
     else
-        hipLaunchKernelGGL((lambda_k<false>), g, t, 0, st, b, k, n, mu);
+        hipLaunchKernelGGL((lambda_k<false, false>), g, t, 0, st, b, k, n, mu);
     return hipGetLastError();
 }
 
@@ -1102,15 +1360,22 @@ static void solve_variant(hipStream_t st, const DevBuffers &b, const KConst &k, 
                           bool uniform, float wu, uint32_t iter, uint32_t bs)
 {
     const dim3 g(blocks_for(n, bs)), t(bs);
+    const dim3 gs(blocks_for(n, kStageUnit)), ts(kStageUnit);
     const bool n4 = k.n_corr == 4u;
-    if (uniform && n4)
-        hipLaunchKernelGGL((solve_k<F, true, 4>), g, t, 0, st, b, k, n, wu, iter);
+    // the staged variant exists for the uniform pure-fluid composition only
+    constexpr bool CAN_STAGE = F == F_FLUID;
+    if (CAN_STAGE && uniform && b.staging && n4)
+        hipLaunchKernelGGL((solve_k<F, true, 4, CAN_STAGE>), gs, ts, 0, st, b, k, n, wu, iter);
+    else if (CAN_STAGE && uniform && b.staging)
+        hipLaunchKernelGGL((solve_k<F, true, 0, CAN_STAGE>), gs, ts, 0, st, b, k, n, wu, iter);
+    else if (uniform && n4)
+        hipLaunchKernelGGL((solve_k<F, true, 4, false>), g, t, 0, st, b, k, n, wu, iter);
     else if (uniform)
-        hipLaunchKernelGGL((solve_k<F, true, 0>), g, t, 0, st, b, k, n, wu, iter);
+        hipLaunchKernelGGL((solve_k<F, true, 0, false>), g, t, 0, st, b, k, n, wu, iter);
     else if (n4)
-        hipLaunchKernelGGL((solve_k<F, false, 4>), g, t, 0, st, b, k, n, wu, iter);
+        hipLaunchKernelGGL((solve_k<F, false, 4, false>), g, t, 0, st, b, k, n, wu, iter);
     else
-        hipLaunchKernelGGL((solve_k<F, false, 0>), g, t, 0, st, b, k, n, wu, iter);
+        hipLaunchKernelGGL((solve_k<F, false, 0, false>), g, t, 0, st, b, k, n, wu, iter);
 }
 
 hipError_t launch_solve(hipStream_t st, const DevBuffers &b, const KConst &k, uint32_t n,
@@ -1135,10 +1400,13 @@ hipError_t launch_finalize(hipStream_t st, const DevBuffers &b, const KConst &k,
     if (n == 0) return hipSuccess;
     const dim3 g(blocks_for(n, bs)), t(bs);
     const float mu = 1.0f / wu;
-    if (uniform)
-        hipLaunchKernelGGL((finalize_k<true>), g, t, 0, st, b, k, n, mu, dt, its);
+    if (uniform && b.staging)
+        hipLaunchKernelGGL((finalize_k<true, true>), dim3(blocks_for(n, kStageUnit)),
+                           dim3(kStageUnit), 0, st, b, k, n, mu, dt, its);
+    else if (uniform)
+        hipLaunchKernelGGL((finalize_k<true, false>), g, t, 0, st, b, k, n, mu, dt, its);
     else
-        hipLaunchKernelGGL((finalize_k<false>), g, t, 0, st, b, k, n, mu, dt, its);
+        hipLaunchKernelGGL((finalize_k<false, false>), g, t, 0, st, b, k, n, mu, dt, its);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || !sleeping) return e;
     hipLaunchKernelGGL(sleep_k, dim3(blocks_for(n)), dim3(kBlock), 0, st, b, n);
