@@ -129,11 +129,12 @@ EXPORT void SFX(r2d_constants)(const r2d_sys *s, double *out)
     out[4] = s->jit;
 }
 
-/* OpenMP threads of the per-particle loops (1 = the reference's single thread) */
+/* OpenMP threads of the per-particle loops (1 = the reference's single
+ * thread, 0 = every processor) */
 EXPORT void SFX(r2d_set_threads)(int n)
 {
 #ifdef _OPENMP
-    omp_set_num_threads(n > 0 ? n : 1);
+    omp_set_num_threads(n > 0 ? n : omp_get_num_procs());
 #else
     (void)n;
 #endif
