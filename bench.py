@@ -82,6 +82,7 @@ def parse():
 
 KERNEL_OF = {"predict_hash_k": "predict_hash", "scan_partials_k": "scan", "scan_final_k": "scan",
              "scatter_k": "scatter", "reorder_k": "reorder", "neighbors_k": "neighbors",
+             "neighbors_staged_k": "neighbors",
              "lambda_k": "lambda", "solve_k": "solve", "shape_k": "shape",
              "finalize_k": "finalize"}
 

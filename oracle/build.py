@@ -13,8 +13,11 @@ import subprocess
 HERE = os.path.dirname(os.path.abspath(__file__))
 SRCS = [os.path.join(HERE, "ps_oracle.c"), os.path.join(HERE, "ref2d.c")]
 
+# -mfma turns the explicit fmaf/fma calls into the FMA instruction (the same
+# correctly rounded result as the libm call); -ffp-contract=off still keeps
+# every other product and sum separately rounded.
 CFLAGS = ["-std=c99", "-O2", "-fPIC", "-shared", "-ffp-contract=off", "-fno-fast-math",
-          "-fopenmp", "-Wall", "-Wno-unknown-pragmas"]
+          "-mfma", "-fopenmp", "-Wall", "-Wno-unknown-pragmas"]
 
 
 def build_oracle(verbose=True):
