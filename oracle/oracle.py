@@ -189,6 +189,48 @@ class Oracle:
         return dict(zip(["h2", "poly6", "spiky", "w_zero", "w_dq", "inv_w_dq", "tensile_k",
                          "search_r2", "collide_r2", "wall_s", "wall_k", "jitter"], out))
 
+    def set_state(self, x=None, v=None):
+        """positions / velocities in insertion order at the oracle's precision"""
+        x = None if x is None else np.ascontiguousarray(x, dtype=self.dtype).reshape(-1, 3)
+        v = None if v is None else np.ascontiguousarray(v, dtype=self.dtype).reshape(-1, 3)
+        self._check(self._f("orc_set_state")(self.h, _ptr(x), _ptr(v)))
+
+    def set_coefficients(self, rest_density=None, stiffness=None):
+        """per-particle rest densities (insertion order) and per-constraint
+        distance stiffnesses at the oracle's precision"""
+        r0 = None if rest_density is None else np.ascontiguousarray(rest_density, dtype=self.dtype)
+        st = None if stiffness is None else np.ascontiguousarray(stiffness, dtype=self.dtype)
+        self._check(self._f("orc_set_coefficients")(self.h, _ptr(r0), _ptr(st)))
+
+    def set_wall_k(self, wall_k):
+        """kinetic wall-friction distance sqrt(mu_k) radius, full precision"""
+        fn = self._f("orc_set_wall_k")
+        fn.argtypes = [ctypes.c_void_p, self.real]
+        self._check(fn(self.h, wall_k))
+
+    def step_backward(self, dt, substeps, gx=None, gv=None, n_dist=0, with_mu=False):
+        """Reverse-mode gradient of L(x_T, v_T) after `substeps` substeps from
+        the current state, given gx = dL/dx_T and gv = dL/dv_T.  Returns
+        (dL/dx_0, dL/dv_0, dL/drho0, dL/dstiffness) in insertion order, plus
+        dL/dwall_mu_k when with_mu, and leaves the system in the final
+        state."""
+        n = self.n
+        gx = None if gx is None else np.ascontiguousarray(gx, dtype=self.dtype).reshape(n, 3)
+        gv = None if gv is None else np.ascontiguousarray(gv, dtype=self.dtype).reshape(n, 3)
+        dx = np.empty((n, 3), self.dtype)
+        dv = np.empty((n, 3), self.dtype)
+        dr = np.empty(n, self.dtype)
+        ds = np.empty(max(n_dist, 1), self.dtype)
+        dm = np.zeros(1, self.dtype)
+        fn = self._f("orc_step_backward")
+        fn.argtypes = [ctypes.c_void_p, self.real, ctypes.c_uint32] + [ctypes.c_void_p] * 7
+        rc = fn(self.h, dt, substeps, _ptr(gx), _ptr(gv), _ptr(dx), _ptr(dv), _ptr(dr), _ptr(ds),
+                _ptr(dm))
+        if rc:
+            raise RuntimeError("orc_step_backward failed: %d" % rc)
+        out = (dx, dv, dr, ds[:n_dist])
+        return out + (float(dm[0]),) if with_mu else out
+
     def wall_uniform(self, seed, it, orig, wall):
         return self._f("orc_wall_uniform")(seed, it, orig, wall)
 

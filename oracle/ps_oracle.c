@@ -244,6 +244,10 @@ typedef struct orc_sys {
     uint64_t substeps, iters;
     uint32_t status_flags, max_seen;
     int stage; /* 0 idle, 1 begun */
+    /* adjoint recompute: when set, solve_pass records per particle which
+     * coordinates the walls or a pin overwrote (bits 0-2 x, y, z) */
+    uint8_t *clamp_rec;
+    real *wall_rec; /* 4 per particle, with clamp_rec */
 } orc_sys;
 
 static void *xcalloc(size_t n, size_t sz)
@@ -680,7 +684,10 @@ static real ipow(real x, uint32_t n)
 /* wall push-out + wall friction for one particle (collide_world_functor,
  * integration_kernel.cuh:57-157).  Order of the walls as there: floor,
  * +x, -x, ceiling, +z, -z. */
-static void walls(const orc_sys *s, uint32_t i, real *p, uint64_t iter)
+/* returns the coordinates it set (bits 0-2), bit 3 if friction acted, bit 4
+ * for the kinetic branch with f < 1; when friction acts and rec is given,
+ * rec = (position before friction, (n_x+1)*9 + (n_y+1)*3 + (n_z+1)) */
+static uint32_t walls(const orc_sys *s, uint32_t i, real *p, uint64_t iter, real *rec)
 {
     const orc_derived *d = &s->d;
     const real r = d->radius;
@@ -691,50 +698,68 @@ static void walls(const orc_sys *s, uint32_t i, real *p, uint64_t iter)
     const uint64_t seed = s->prm.seed;
     real nx = R(0), ny = R(0), nz = R(0);
     real efl = ph < PS_PHASE_SOLID ? d->jitter : R(0);
+    uint32_t set = 0;
     if (p[1] < by0 + r) {
         p[1] = (by0 + r) + (wall_uniform(seed, iter, o, 5) * efl);
         ny = ny + R(1);
+        set |= 2u;
     }
     if (p[0] > bx1 - r) {
         p[0] = bx1 - (r + (wall_uniform(seed, iter, o, 0) * d->jitter));
         nx = nx - R(1);
+        set |= 1u;
     }
     if (p[0] < bx0 + r) {
         p[0] = bx0 + (r + (wall_uniform(seed, iter, o, 1) * d->jitter));
         nx = nx + R(1);
+        set |= 1u;
     }
     if (p[1] > by1 - r) {
         p[1] = by1 - (r + (wall_uniform(seed, iter, o, 2) * d->jitter));
         ny = ny - R(1);
+        set |= 2u;
     }
     if (s->prm.dims != 2) {
         if (p[2] > bz1 - r) {
             p[2] = bz1 - (r + (wall_uniform(seed, iter, o, 3) * d->jitter));
             nz = nz - R(1);
+            set |= 4u;
         }
         if (p[2] < bz0 + r) {
             p[2] = bz0 + (r + (wall_uniform(seed, iter, o, 4) * d->jitter));
             nz = nz + R(1);
+            set |= 4u;
         }
     }
-    if (ph < PS_PHASE_CLOTH) return;
-    if (nx == R(0) && ny == R(0) && nz == R(0)) return;
+    if (ph < PS_PHASE_CLOTH) return set;
+    if (nx == R(0) && ny == R(0) && nz == R(0)) return set;
     real ex = p[0] - s->X[3 * i], ey = p[1] - s->X[3 * i + 1], ez = p[2] - s->X[3 * i + 2];
     real dn = ((ex * nx) + (ey * ny)) + (ez * nz);
     real tx = ex - (dn * nx), ty = ey - (dn * ny), tz = ez - (dn * nz);
     real l = SQRT(((tx * tx) + (ty * ty)) + (tz * tz));
-    if (l < R(0.001)) return;
+    if (l < R(0.001)) return set;
+    set |= 8u;
+    if (rec) {
+        rec[0] = p[0];
+        rec[1] = p[1];
+        rec[2] = p[2];
+        rec[3] = (real)((((int)nx + 1) * 9 + ((int)ny + 1) * 3) + ((int)nz + 1));
+    }
     if (l < d->wall_s) {
         p[0] = p[0] - tx;
         p[1] = p[1] - ty;
         p[2] = p[2] - tz;
     } else {
         real f = d->wall_k / l;
-        if (f > R(1)) f = R(1);
+        if (f > R(1))
+            f = R(1);
+        else
+            set |= 16u;
         p[0] = p[0] - (tx * f);
         p[1] = p[1] - (ty * f);
         p[2] = p[2] - (tz * f);
     }
+    return set;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1047,7 +1072,7 @@ static void solve_pass(orc_sys *s)
         real p[3] = {px + ax, py + ay, pz + az};
         if (s->prm.dims == 2) p[2] = pz;
         /* a8: walls */
-        walls(s, i, p, iter);
+        uint32_t set = walls(s, i, p, iter, s->clamp_rec ? s->wall_rec + 4 * (size_t)i : NULL);
         /* a10: pins (point_constraint_functor, solver_kernel.cuh:10-25) */
         {
             uint32_t pin = s->pin_of[s->orig[i]];
@@ -1055,8 +1080,10 @@ static void solve_pass(orc_sys *s)
                 p[0] = s->pin_xyz[3 * (pin - 1)];
                 p[1] = s->pin_xyz[3 * (pin - 1) + 1];
                 p[2] = s->pin_xyz[3 * (pin - 1) + 2];
+                set |= 39u; /* all three coordinates, and bit 5: pinned */
             }
         }
+        if (s->clamp_rec) s->clamp_rec[i] = (uint8_t)set;
         s->Q[4 * i] = p[0];
         s->Q[4 * i + 1] = p[1];
         s->Q[4 * i + 2] = p[2];
@@ -1237,3 +1264,36 @@ ORC_EXPORT void SFX(orc_extract_rotation)(const real *A, real *q, int sweeps)
 {
     extract_rotation(A, q, sweeps);
 }
+
+/* ------------------------------------------------------------------------ */
+/* Full-precision setters for finite-difference checks (insertion order; a
+ * NULL array is left as it is). */
+ORC_EXPORT int SFX(orc_set_state)(orc_sys *s, const real *x, const real *v)
+{
+    for (uint32_t t = 0; t < s->n; t++) {
+        uint32_t o = s->orig[t];
+        for (int a = 0; a < 3; a++) {
+            if (x) s->X[3 * t + a] = x[3 * o + a];
+            if (v) s->V[3 * t + a] = v[3 * o + a];
+        }
+    }
+    return PS_OK;
+}
+
+/* wall friction coefficients in full precision: wall_k = sqrt(mu_k) radius */
+ORC_EXPORT int SFX(orc_set_wall_k)(orc_sys *s, real wall_k)
+{
+    s->d.wall_k = wall_k;
+    return PS_OK;
+}
+
+ORC_EXPORT int SFX(orc_set_coefficients)(orc_sys *s, const real *rho0, const real *stiffness)
+{
+    if (rho0)
+        for (uint32_t t = 0; t < s->n; t++) s->rho0[t] = rho0[s->orig[t]];
+    if (stiffness)
+        for (uint32_t c = 0; c < s->ndist; c++) s->dist_stiff[c] = stiffness[c];
+    return PS_OK;
+}
+
+#include "ps_adjoint.inc"

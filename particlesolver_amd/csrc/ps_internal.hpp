@@ -145,6 +145,9 @@ struct DevBuffers {
     // LDS staging (uniform single-scene fluid): one descriptor per unit
     StageDesc *stage;
     uint32_t staging;      // 1: the last neighbour build staged its units
+    // adjoint recompute only (else null): per particle, the coordinates the
+    // walls and pins set and the wall-friction branch (walls() bits)
+    uint8_t *clamp_rec;
     // CPU-reference mode only
     float *lam;            // lambda per particle
     uint32_t *bc;          // boundary flags | boundary count << 8

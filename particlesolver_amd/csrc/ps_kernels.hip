@@ -20,7 +20,7 @@
 // contiguous range of sorted particles, keeping a spatial region's
 // neighbour data inside one XCD's L2.
 
-#include "ps_internal.hpp"
+#include "ps_device.hpp"
 
 namespace ps {
 
@@ -29,12 +29,6 @@ static constexpr uint32_t kMaxBlock = 512;  // tunable kernels run with 64..512 
 static constexpr uint32_t kScanPerThread = 16;
 static constexpr uint32_t kScanTile = kBlock * kScanPerThread;
 
-__device__ __forceinline__ uint32_t xcd_block()
-{
-    const uint32_t nb = gridDim.x, b = blockIdx.x;
-    const uint32_t x = b & 7u, q = nb >> 3, r = nb & 7u;
-    return x * q + (x < r ? x : r) + (b >> 3);
-}
 
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v)
 {
@@ -64,65 +58,6 @@ __device__ __forceinline__ uint32_t wave_max(uint32_t v)
     return v;
 }
 
-// chunk m (entries 4m..4m+3) of particle i is nbr_lane(b, i)[m * 64]
-__device__ __forceinline__ const uint4 *nbr_lane(const DevBuffers &b, uint32_t i)
-{
-    return reinterpret_cast<const uint4 *>(b.nbr) + (size_t)(i >> 6) * b.kq * 64 + (i & 63);
-}
-
-// Visit the first c neighbours of particle i in table order, calling
-// fn(j, gather(j)) for each.  Software-pipelined: while chunk m is being
-// processed, the four gathers of chunk m+1 and the index load of chunk m+2
-// are already in flight.  Every entry of a chunk at or below the last one is
-// a valid slot (the builder pads the last chunk with the particle's own
-// slot), so all loads are unconditional; only fn is guarded.
-template <typename G, typename Fn>
-__device__ __forceinline__ void for_neighbors(const DevBuffers &b, uint32_t i, uint32_t c, G gather,
-                                              Fn fn)
-{
-    if (c == 0) return;
-    const uint4 *nb = nbr_lane(b, i);
-    const uint32_t last = (c - 1) >> 2;
-    uint4 jj = nb[0];
-    uint4 jn = nb[(size_t)min(1u, last) * 64];
-    auto d0 = gather(jj.x);
-    auto d1 = gather(jj.y);
-    auto d2 = gather(jj.z);
-    auto d3 = gather(jj.w);
-    for (uint32_t m = 0; m <= last; m++) {
-        const uint4 jnn = nb[(size_t)min(m + 2, last) * 64];
-        auto e0 = gather(jn.x);
-        auto e1 = gather(jn.y);
-        auto e2 = gather(jn.z);
-        auto e3 = gather(jn.w);
-        const uint32_t q = m << 2;
-        fn(jj.x, d0);
-        if (q + 1 < c) fn(jj.y, d1);
-        if (q + 2 < c) fn(jj.z, d2);
-        if (q + 3 < c) fn(jj.w, d3);
-        jj = jn;
-        jn = jnn;
-        d0 = e0;
-        d1 = e1;
-        d2 = e2;
-        d3 = e3;
-    }
-}
-
-struct PA {
-    float4 p;
-    Attr a;
-};
-
-struct PX {
-    float4 p, x;
-};
-
-// entry q of particle i (single-entry access for the less travelled loops)
-__device__ __forceinline__ uint32_t nbr_at(const DevBuffers &b, uint32_t i, uint32_t q)
-{
-    return reinterpret_cast<const uint32_t *>(nbr_lane(b, i) + (size_t)(q >> 2) * 64)[q & 3u];
-}
 
 // ---------------------------------------------------------------------------
 // a1 + a2: predict (integrate_functor, integration_kernel.cuh:159-184) and
@@ -355,36 +290,6 @@ __global__ __launch_bounds__(kMaxBlock) void reorder_k(DevBuffers b, uint32_t n)
     if (b.track_slots) b.slot_of[a.orig] = d;
 }
 
-// ---------------------------------------------------------------------------
-// Pair arithmetic shared with the CPU oracle: squared distances and sums use
-// fused multiply-add, and 1/r comes from rsq() instead of sqrt + division
-// (an IEEE square root and division cost ~30 VALU per pair, rsq 12).
-
-// |d|^2 = fma(dz, dz, fma(dy, dy, dx dx))
-__device__ __forceinline__ float dist2(float dx, float dy, float dz)
-{
-    return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-}
-
-// 1/sqrt(x), x > 0 normal: integer seed 0x5f375a86 - (bits(x) >> 1) (the
-// "fast inverse square root" constant of C. Lomont, 2003) and three Newton
-// steps y <- y (3/2 - (x/2) y^2) as t = (x/2) y; u = fma(-t, y, 3/2);
-// y = y u.  Relative error about 1e-7.
-__device__ __forceinline__ float rsq(float x)
-{
-    float y = __uint_as_float(0x5f375a86u - (__float_as_uint(x) >> 1));
-    const float hx = 0.5f * x;
-#pragma unroll
-    for (int q = 0; q < 3; q++) {
-        const float t = hx * y;
-        const float u = __builtin_fmaf(-t, y, 1.5f);
-        y = y * u;
-    }
-    return y;
-}
-
-// below this squared distance a pair has no direction
-constexpr float kDegenerateR2 = 1e-8f;
 
 // ---------------------------------------------------------------------------
 // Neighbour table (collideCell/collideCellRadius, integration_kernel.cuh:
@@ -733,142 +638,6 @@ __global__ __launch_bounds__(kMaxBlock) void lambda_k(DevBuffers b, KConst k, ui
     Pw[4 * (size_t)i + 3] = -C / (den + k.relax);
 }
 
-// ---------------------------------------------------------------------------
-// helpers of the solve kernel
-
-__device__ __forceinline__ uint64_t mix64(uint64_t z)
-{
-    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-    return z ^ (z >> 31);
-}
-
-// Wall-jitter uniform in [0,1): SplitMix64 finaliser of seed + golden*counter,
-// counter = (iteration, insertion index, wall), top 24 bits.
-__device__ __forceinline__ float wall_uniform(uint64_t seed, uint64_t iter, uint32_t orig,
-                                              uint32_t wall)
-{
-    uint64_t ctr = (iter << 35) | ((uint64_t)orig << 3) | (uint64_t)wall;
-    uint64_t z = mix64(seed + ctr * 0x9E3779B97F4A7C15ull);
-    return (float)(uint32_t)(z >> 40) * (1.0f / 16777216.0f);
-}
-
-// e^x: Cody-Waite reduction x = k ln2 + r, degree-7 Taylor polynomial of
-// e^r (Horner), exact scaling by 2^k.
-__device__ __forceinline__ float ps_exp(float x)
-{
-    if (x > 80.0f) x = 80.0f;
-    if (x < -80.0f) x = -80.0f;
-    float kk = floorf((x * 1.44269504088896341f) + 0.5f);
-    float r = (x - (kk * 0.693145751953125f)) - (kk * 1.42860682030941723e-6f);
-    float p = 1.98412698412698413e-4f;
-    p = (p * r) + 1.38888888888888889e-3f;
-    p = (p * r) + 8.33333333333333333e-3f;
-    p = (p * r) + 4.16666666666666667e-2f;
-    p = (p * r) + 1.66666666666666667e-1f;
-    p = (p * r) + 0.5f;
-    p = (p * r) + 1.0f;
-    p = (p * r) + 1.0f;
-    return ldexpf(p, (int)kk);
-}
-
-__device__ __forceinline__ float ipow(float x, uint32_t n)
-{
-    float r = 1.0f, bb = x;
-    bool first = true;
-    while (n) {
-        if (n & 1u) {
-            r = first ? bb : (r * bb);
-            first = false;
-        }
-        n >>= 1;
-        if (n) bb = bb * bb;
-    }
-    return r;
-}
-
-// a8: world box (collide_world_functor, integration_kernel.cuh:57-157).
-struct WallBox {
-    float lo[3], hi[3];
-    uint64_t seed;
-    uint32_t o;  // index of the particle inside its scene
-};
-
-__device__ __forceinline__ WallBox wall_box(const KConst &k, const DevBuffers &b, uint32_t i,
-                                            uint32_t orig)
-{
-    WallBox w;
-    if (k.nscenes > 1u) {
-        const SceneDev &sc = b.scenes[b.sid[i]];
-        for (int a = 0; a < 3; a++) {
-            w.lo[a] = sc.box_min[a];
-            w.hi[a] = sc.box_max[a];
-        }
-        w.seed = sc.seed;
-        w.o = orig - sc.first;
-    } else {
-        for (int a = 0; a < 3; a++) {
-            w.lo[a] = k.box_min[a];
-            w.hi[a] = k.box_max[a];
-        }
-        w.seed = k.seed;
-        w.o = orig;
-    }
-    return w;
-}
-
-__device__ __forceinline__ void walls(const KConst &k, const WallBox &wb, int32_t ph, float4 xs,
-                                      float &px, float &py, float &pz, uint64_t iter)
-{
-    const float r = k.radius;
-    const uint32_t o = wb.o;
-    float nx = 0.0f, ny = 0.0f, nz = 0.0f;
-    const float efl = ph < PS_PHASE_SOLID ? k.jitter : 0.0f;
-    if (py < wb.lo[1] + r) {
-        py = (wb.lo[1] + r) + (wall_uniform(wb.seed, iter, o, 5) * efl);
-        ny = ny + 1.0f;
-    }
-    if (px > wb.hi[0] - r) {
-        px = wb.hi[0] - (r + (wall_uniform(wb.seed, iter, o, 0) * k.jitter));
-        nx = nx - 1.0f;
-    }
-    if (px < wb.lo[0] + r) {
-        px = wb.lo[0] + (r + (wall_uniform(wb.seed, iter, o, 1) * k.jitter));
-        nx = nx + 1.0f;
-    }
-    if (py > wb.hi[1] - r) {
-        py = wb.hi[1] - (r + (wall_uniform(wb.seed, iter, o, 2) * k.jitter));
-        ny = ny - 1.0f;
-    }
-    if (k.dims != 2u) {
-        if (pz > wb.hi[2] - r) {
-            pz = wb.hi[2] - (r + (wall_uniform(wb.seed, iter, o, 3) * k.jitter));
-            nz = nz - 1.0f;
-        }
-        if (pz < wb.lo[2] + r) {
-            pz = wb.lo[2] + (r + (wall_uniform(wb.seed, iter, o, 4) * k.jitter));
-            nz = nz + 1.0f;
-        }
-    }
-    if (ph < PS_PHASE_CLOTH) return;
-    if (nx == 0.0f && ny == 0.0f && nz == 0.0f) return;
-    float ex = px - xs.x, ey = py - xs.y, ez = pz - xs.z;
-    float dn = ((ex * nx) + (ey * ny)) + (ez * nz);
-    float tx = ex - (dn * nx), ty = ey - (dn * ny), tz = ez - (dn * nz);
-    float l = sqrtf(((tx * tx) + (ty * ty)) + (tz * tz));
-    if (l < 0.001f) return;
-    if (l < k.wall_s) {
-        px = px - tx;
-        py = py - ty;
-        pz = pz - tz;
-    } else {
-        float f = k.wall_k / l;
-        if (f > 1.0f) f = 1.0f;
-        px = px - (tx * f);
-        py = py - (ty * f);
-        pz = pz - (tz * f);
-    }
-}
 
 // ---------------------------------------------------------------------------
 // One Jacobi iteration for every particle: contacts (a7), fluid delta-p
@@ -1071,7 +840,7 @@ __global__ __launch_bounds__(kMaxBlock) void solve_k(DevBuffers b, KConst k, uin
     if (k.dims == 2u) nzp = pz;
     const float4 xs = (ph >= PS_PHASE_CLOTH) ? b.X[i] : make_float4(0.f, 0.f, 0.f, 0.f);
     const uint64_t iter = b.status->iter + it;
-    walls(k, wall_box(k, b, i, a.orig), ph, xs, nxp, nyp, nzp, iter);
+    uint32_t set = walls(k, wall_box(k, b, i, a.orig), ph, xs, nxp, nyp, nzp, iter);
     if (F & F_PIN) {
         const uint32_t pin = b.pin_of[a.orig];
         if (pin) {
@@ -1079,8 +848,10 @@ __global__ __launch_bounds__(kMaxBlock) void solve_k(DevBuffers b, KConst k, uin
             nxp = t.x;
             nyp = t.y;
             nzp = t.z;
+            set |= 7u;
         }
     }
+    if (b.clamp_rec) b.clamp_rec[i] = (uint8_t)set;
     b.Q[i] = make_float4(nxp, nyp, nzp, 0.0f);
 }
 

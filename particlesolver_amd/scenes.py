@@ -251,6 +251,75 @@ def mixed(cloth_n=256, cubes=(25, 5, 25), fluid=(100, 50, 50), spacing=0.1, box=
     return s
 
 
+def rope_over_fluid(fluid=(5, 3, 5), rope=10, spacing=0.1, iterations=3, seed=0x5EED0006,
+                    dt=1.0 / 60.0, stiffness=(0.35, 0.9)):
+    """A fluid block on the floor with a GAS-phase rope (phase 1: no
+    particle contacts, counted in the fluid's density with s_solid) hanging
+    just above it: distance constraints between rope neighbours, rest =
+    spacing, stiffness ramped over the given range, first rope particle
+    pinned where it starts.  The scene the adjoint is checked on (fluid,
+    distance constraints, pins, walls and XSPH together)."""
+    nx, ny, nz = fluid
+    h = 2.0 * spacing
+    ext = [max(nx * 2.0, rope + 3.0) * spacing, ny * spacing * 4.0, nz * spacing * 2.0]
+    pf = fluid_block(nx, ny, nz, spacing, (0.0, 0.0, 0.0), 1e-3, seed)
+    x = (np.arange(rope, dtype=np.float64) + 0.5) * spacing
+    pr = np.stack([x, np.full(rope, (ny + 0.3) * spacing), np.full(rope, 0.5 * nz * spacing)], 1)
+    pos = np.concatenate([pf, pr.astype(np.float32)])
+    n, nf = pos.shape[0], pf.shape[0]
+    p = pbf_params((0.0, 0.0, 0.0), ext, h, 0.5 * spacing, iterations, n, 64, seed=seed)
+    rho0 = lattice_rest_density(h, spacing)
+    phase = np.zeros(n, np.int32)
+    phase[nf:] = 1
+    s = Scene(p, pos, np.ones(n, np.float32), phase, np.full(n, np.float32(rho0), np.float32), dt,
+              name="rope_over_fluid")
+    pairs = np.stack([np.arange(nf, n - 1), np.arange(nf + 1, n)], 1).astype(np.uint32)
+    s.distance = (pairs, np.full(rope - 1, spacing, np.float32),
+                  np.linspace(stiffness[0], stiffness[1], rope - 1).astype(np.float32))
+    s.pins = (np.array([nf], np.uint32), pos[nf:nf + 1].copy())
+    s.rho0 = rho0
+    return s
+
+
+def cloth_and_fluid(cloth=(6, 6), fluid=(4, 3, 4), spacing=0.1, cloth_spacing=0.08, iterations=3,
+                    seed=0x5EED0007, dt=1.0 / 60.0, velocity=(1.2, 0.0, 0.3),
+                    stiffness=(0.9, 0.5, 0.2)):
+    """A fluid block in the box corner and a cloth (structural, shear and
+    bend distance constraints, first particle pinned) lying on the floor
+    beside it, sliding away with the given initial velocity: cloth-cloth and
+    cloth-fluid particle contacts, the floor's push-out and kinetic wall
+    friction every iteration, fluid, XSPH and a pin in one small scene.  The
+    cloth spacing is below the contact distance, so adjacent cloth particles
+    stay strictly inside their contact range (a contact projects its pair
+    onto the range's edge, where the contact set switches)."""
+    fx, fy, fz = fluid
+    cx, cz = cloth
+    h = 2.0 * spacing
+    radius = 0.5 * spacing
+    pf = fluid_block(fx, fy, fz, spacing, (0.0, 0.0, 0.0), 1e-3, seed)
+    i = np.arange(cx, dtype=np.float64)
+    k = np.arange(cz, dtype=np.float64)
+    zz, xx = np.meshgrid(k, i, indexing="ij")
+    pc = np.stack([(fx + 0.2) * spacing + xx.ravel() * cloth_spacing,
+                   np.full(cx * cz, 0.9 * radius),
+                   spacing + zz.ravel() * cloth_spacing], 1).astype(np.float32)
+    pos = np.concatenate([pf, pc])
+    n, nf = pos.shape[0], pf.shape[0]
+    ext = (max(2.0, (fx + cx + 8) * spacing), 1.0, max(1.0, (cz + 4) * spacing))
+    p = pbf_params((0.0, 0.0, 0.0), ext, h, radius, iterations, n, 64, seed=seed)
+    rho0 = lattice_rest_density(h, spacing)
+    phase = np.zeros(n, np.int32)
+    phase[nf:] = 2
+    vel = np.zeros((n, 3), np.float32)
+    vel[nf:] = velocity
+    s = Scene(p, pos, np.ones(n, np.float32), phase, np.full(n, np.float32(rho0), np.float32), dt,
+              vel=vel, name="cloth_and_fluid")
+    s.distance = cloth_constraints(cx, cz, nf, cloth_spacing, stiffness)
+    s.pins = (np.array([nf], np.uint32), pos[nf:nf + 1].copy())
+    s.rho0 = rho0
+    return s
+
+
 def two_d_fluid(nx=40, ny=20, spacing=1.0, h=2.0, iterations=3, seed=0x5EED0001, dt=1.0 / 60.0):
     """2D PBF block (dims = 2, z fixed) with rest density from the square
     lattice kernel sum."""

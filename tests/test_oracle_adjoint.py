@@ -1,0 +1,176 @@
+"""The oracle's reverse-mode adjoint of the substep (orc_step_backward) against
+central finite differences of its own FP64 forward pass.
+
+The finite differences only run the forward solver, so they pin the
+adjoint independently of how it is written: a dropped term, a wrong sign, a
+transposed Jacobian or a wrong owner of a pair term changes a directional
+derivative well beyond the stated tolerance.  The step is 1e-6 in FP64
+(truncation O(eps^2), rounding about 1e-16 |L| / eps); measured agreement
+is 1e-10 .. 4e-7 relative.  The scenes keep every pair and wall strictly
+away from the thresholds where the frozen sets switch (there the map is not
+differentiable and a difference quotient straddles the jump).
+
+Scenes: a 3D dam break (fluid, XSPH, walls, tensile term), a GAS-phase rope
+over fluid (distance constraints with per-constraint stiffness, a pin,
+fluid-rope density coupling), cloth on the floor beside fluid (cloth-cloth
+and cloth-fluid contacts, kinetic wall friction, constraints, a pin) and a
+2D fluid (z held)."""
+import math
+
+import numpy as np
+import pytest
+
+from particlesolver_amd import capi, scenes
+from oracle import load_scene
+
+EPS = 1e-6
+RTOL = 1e-5
+
+SCENES = {
+    "dam_break": lambda: scenes.dam_break(5, 4, 5),
+    "rope_over_fluid": lambda: scenes.rope_over_fluid(),
+    "cloth_and_fluid": lambda: scenes.cloth_and_fluid(),
+    "fluid2d": lambda: scenes.two_d_fluid(8, 5),
+}
+
+
+class Problem:
+    """L = <cx, x_T> + <cv, v_T> after T substeps from (x0, v0) with rest
+    densities r0, stiffnesses st0 and wall_k wk0, all at full precision."""
+
+    def __init__(self, sc, T=3, seed=3):
+        self.sc, self.T = sc, T
+        rng = np.random.default_rng(seed)
+        n = sc.n
+        self.rng = rng
+        self.nd = 0 if sc.distance is None else sc.distance[0].shape[0]
+        self.cx = rng.standard_normal((n, 3))
+        self.cv = rng.standard_normal((n, 3))
+        o = load_scene(sc, "f64")
+        self.x0 = o.positions().copy()
+        self.v0 = o.velocities().copy() + 0.3 * rng.standard_normal((n, 3))
+        if sc.params.dims == 2:
+            self.v0[:, 2] = 0.0
+        self.r0 = sc.rest_density.astype(np.float64)
+        self.st0 = sc.distance[2].astype(np.float64) if self.nd else None
+        self.wk0 = float(o.derived()["wall_k"])
+
+    def system(self, x=None, v=None, r=None, st=None, wk=None):
+        o = load_scene(self.sc, "f64")
+        o.set_state(self.x0 if x is None else x, self.v0 if v is None else v)
+        o.set_coefficients(self.r0 if r is None else r, self.st0 if st is None else st)
+        o.set_wall_k(self.wk0 if wk is None else wk)
+        return o
+
+    def loss(self, **kw):
+        o = self.system(**kw)
+        o.step(self.sc.dt, self.T)
+        return float((self.cx * o.positions()).sum() + (self.cv * o.velocities()).sum())
+
+    def adjoint(self):
+        o = self.system()
+        return o.step_backward(self.sc.dt, self.T, self.cx, self.cv, self.nd, with_mu=True)
+
+    def fd(self, name, d):
+        base = {"x": self.x0, "v": self.v0, "r": self.r0, "st": self.st0, "wk": self.wk0}[name]
+        lp = self.loss(**{name: base + EPS * d})
+        lm = self.loss(**{name: base - EPS * d})
+        return (lp - lm) / (2.0 * EPS)
+
+
+@pytest.fixture(scope="module", params=sorted(SCENES))
+def problem(request):
+    return Problem(SCENES[request.param]())
+
+
+def assert_close(fd, an, what):
+    assert abs(fd - an) <= RTOL * max(abs(fd), 1e-3), "%s: fd %r adjoint %r" % (what, fd, an)
+
+
+def test_directional_derivatives(problem):
+    dx, dv, dr, ds, dmu = problem.adjoint()
+    rng, n = problem.rng, problem.sc.n
+    for name, g, shape in (("x", dx, (n, 3)), ("v", dv, (n, 3)), ("r", dr, (n,))):
+        d = rng.standard_normal(shape) * (10.0 if name == "r" else 1.0)
+        if name == "v" and problem.sc.params.dims == 2:
+            d[:, 2] = 0.0
+        assert_close(problem.fd(name, d), float((g * d).sum()), name)
+    if problem.nd:
+        d = rng.standard_normal(problem.nd)
+        assert_close(problem.fd("st", d), float((ds * d).sum()), "stiffness")
+    # wall_k = sqrt(mu_k) radius: the adjoint reports dL/dmu_k
+    p = problem.sc.params
+    chain = float(np.float32(p.radius)) / (2.0 * math.sqrt(float(np.float32(p.wall_mu_k))))
+    fdk = problem.fd("wk", 1.0)
+    assert_close(fdk, dmu / chain, "wall_mu_k")
+    if problem.sc.name == "cloth_and_fluid":
+        assert abs(fdk) > 1.0            # kinetic wall friction acts in this scene
+
+
+def test_coordinate_derivatives(problem):
+    dx, dv = problem.adjoint()[:2]
+    rng, n = problem.rng, problem.sc.n
+    for _ in range(4):
+        i, a = int(rng.integers(n)), int(rng.integers(3))
+        for name, g in (("x", dx), ("v", dv)):
+            d = np.zeros((n, 3))
+            d[i, a] = 1.0
+            if name == "v" and problem.sc.params.dims == 2 and a == 2:
+                assert g[i, a] == 0.0    # v_z never enters a 2D substep
+                continue
+            assert_close(problem.fd(name, d), float(g[i, a]), "%s[%d,%d]" % (name, i, a))
+
+
+def test_linear_in_the_cotangent(problem):
+    o = problem.system()
+    sc, T, nd = problem.sc, problem.T, problem.nd
+    a = o.step_backward(sc.dt, T, problem.cx, None, nd)
+    o = problem.system()
+    b = o.step_backward(sc.dt, T, None, problem.cv, nd)
+    o = problem.system()
+    ab = o.step_backward(sc.dt, T, 2.0 * problem.cx, -0.5 * problem.cv, nd)
+    for ga, gb, gab in zip(a, b, ab):
+        if gab.size:
+            np.testing.assert_allclose(gab, 2.0 * ga - 0.5 * gb, rtol=1e-9,
+                                       atol=1e-9 * np.abs(gab).max())
+
+
+def test_leaves_the_final_state(problem):
+    sc = problem.sc
+    o = problem.system()
+    o.step_backward(sc.dt, problem.T, problem.cx, problem.cv, problem.nd)
+    f = problem.system()
+    f.step(sc.dt, problem.T)
+    assert np.array_equal(o.positions(), f.positions())
+    assert np.array_equal(o.velocities(), f.velocities())
+    # and stepping on from there matches stepping on without the adjoint
+    o.step(sc.dt, 1)
+    f.step(sc.dt, 1)
+    assert np.array_equal(o.positions(), f.positions())
+
+
+def test_fp32_adjoint_tracks_fp64():
+    sc = scenes.cloth_and_fluid()
+    pr = Problem(sc)
+    g64 = pr.adjoint()
+    o = load_scene(sc, "f32")
+    o.set_state(pr.x0, pr.v0)
+    g32 = o.step_backward(sc.dt, pr.T, pr.cx, pr.cv, pr.nd, with_mu=True)
+    for a, b, what in zip(g32[:4], g64[:4], ("x", "v", "rho0", "stiffness")):
+        err = np.abs(a.astype(np.float64) - b).max() / np.abs(b).max()
+        assert err < 1e-3, (what, err)
+    assert abs(g32[4] - g64[4]) < 1e-3 * abs(g64[4])
+
+
+def test_rejects_what_it_does_not_cover():
+    sc = scenes.mixed(cloth_n=4, cubes=(1, 1, 1), fluid=(4, 4, 4), box=(3.0, 3.0, 3.0))
+    o = load_scene(sc, "f64")
+    with pytest.raises(RuntimeError):
+        o.step_backward(sc.dt, 1, None, None)           # rigid bodies
+    sc = scenes.dam_break(4, 4, 4)
+    p = sc.params.copy()
+    p.flags = capi.PS_FLAG_SLEEPING
+    sc.params = p
+    o = load_scene(sc, "f64")
+    with pytest.raises(RuntimeError):
+        o.step_backward(sc.dt, 1, None, None)           # sleeping
