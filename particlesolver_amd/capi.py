@@ -127,6 +127,7 @@ PROTOTYPES = {
     "ps_num_scenes": (_u32, [_P]),
     "ps_add_scene_particles": (_int, [_P, _u32, _u32, _P, _P, _P, _P, _P, ctypes.POINTER(_u32)]),
     "ps_step_graph": (_int, [_P, _f32, _u32]),
+    "ps_step_backward": (_int, [_P, _f32, _u32, _u32, _P, _P, _P, _P, _P, _P, _P]),
     "ps_destroy": (_int, [_P]),
     "ps_set_params": (_int, [_P, ctypes.POINTER(PsParams)]),
     "ps_get_params": (_int, [_P, ctypes.POINTER(PsParams)]),
@@ -288,6 +289,7 @@ class System:
         st = None if stiffness is None else np.ascontiguousarray(stiffness, dtype=np.float32)
         self._check(self.lib.ps_add_distance_constraints(self.h, pairs.shape[0], _ptr(pairs),
                                                          _ptr(rest), _ptr(st)))
+        self.n_dist = getattr(self, "n_dist", 0) + pairs.shape[0]
 
     def add_point_constraints(self, idx, xyz):
         idx = np.ascontiguousarray(idx, dtype=np.uint32).reshape(-1)
@@ -324,6 +326,24 @@ class System:
 
     def step_graph(self, dt, substeps):
         self._check(self.lib.ps_step_graph(self.h, dt, substeps))
+
+    def step_backward(self, dt, substeps, dl_dx=None, dl_dv=None, checkpoint_every=0):
+        """ps_step_backward: runs `substeps` substeps and returns
+        (dL/dx0, dL/dv0, dL/drho0, dL/dstiffness, dL/dwall_mu_k) for the
+        loss gradients dl_dx, dl_dv of the final state (insertion order)."""
+        n = self.n
+        gx = None if dl_dx is None else np.ascontiguousarray(dl_dx, dtype=np.float32).reshape(n, 3)
+        gv = None if dl_dv is None else np.ascontiguousarray(dl_dv, dtype=np.float32).reshape(n, 3)
+        dx = np.empty((n, 3), np.float32)
+        dv = np.empty((n, 3), np.float32)
+        dr = np.empty(n, np.float32)
+        nd = getattr(self, "n_dist", 0)
+        ds = np.empty(max(nd, 1), np.float32)
+        dm = np.zeros(1, np.float32)
+        self._check(self.lib.ps_step_backward(self.h, dt, substeps, checkpoint_every, _ptr(gx),
+                                              _ptr(gv), _ptr(dx), _ptr(dv), _ptr(dr), _ptr(ds),
+                                              _ptr(dm)))
+        return dx, dv, dr, ds[:nd], float(dm[0])
 
     def begin(self, dt):
         self._check(self.lib.ps_substep_begin(self.h, dt))

@@ -1,0 +1,599 @@
+// ps_adjoint.hip — reverse-mode adjoint of one substep (ps_step_backward).
+//
+// The forward map of a substep is predict -> I Jacobi iterations -> velocity
+// + XSPH.  With the neighbour table, the contact sets and counts, the
+// coordinates walls and pins set and the wall-friction branches frozen (as
+// the recomputed forward pass recorded them), each piece is smooth and its
+// vector-Jacobian product is written out by hand below.  Every transposed
+// sum is a gather over the particle's own neighbour table, which holds the
+// same pairs as its neighbours' tables (r^2 < R^2 is symmetric and no table
+// overflowed), so there are no scatters and no atomics: one thread per
+// particle (or per distance constraint), sums in table order.  The
+// arithmetic is the CPU oracle's, operation by operation (built with
+// -ffp-contract=off), so FP32 gradients match it bit for bit.
+//
+// Per iteration, with y = p + Delta before walls and pins:
+//   Delta_i = f_i / rho0_i (fluid) + contacts + distance terms,
+//   f_i = sum_j (l_i + l_j + s_ij) G_ij, G_ij = -b(r) d_ij,
+//   b = spiky (h - r)^2 / r (G = (0, -gdeg, 0) for a degenerate pair),
+//   l_i = -C_i / (D_i + eps), C_i = rho_i / rho0_i - 1,
+//   D_i = sum_j a_ij^2 r^2 + |S_i|^2, S_i = sum_j a_ij d_ij, a = b / rho0_i.
+// Kernels per reversed iteration: adj_pre_k (walls, pins and wall friction
+// backwards, contact counts, rho/S/D, f), adj_lambda_k (dL/dl -> dL/dC,
+// dL/dD), adj_position_k (dL/dp: pair, density, D, contact and distance
+// terms), adj_stiffness_k (per constraint).  Per substep: adj_xsph_k and
+// adj_finalize_k (velocity + XSPH backwards) before the iterations,
+// adj_predict_k after them.
+
+#include "ps_device.hpp"
+
+namespace ps {
+
+static constexpr uint32_t kAdjBlock = 256;
+
+__device__ __forceinline__ float dot3(const float *a, const float *b)
+{
+    return ((a[0] * b[0]) + (a[1] * b[1])) + (a[2] * b[2]);
+}
+
+// G_ij for d = p_i - p_j (r2 = |d|^2 < h2)
+__device__ __forceinline__ void adj_gamma(const KConst &k, const float *dv, float r2, float *g)
+{
+    if (r2 < kDegenerateR2) {
+        g[0] = 0.0f;
+        g[1] = -((k.spiky * (k.h * k.h)) * 0.001f);
+        g[2] = 0.0f;
+        return;
+    }
+    const float ri = rsq(r2);
+    const float q = k.h - (r2 * ri);
+    const float bb = (k.spiky * (q * q)) * ri;
+    for (int a = 0; a < 3; a++) g[a] = -(bb * dv[a]);
+}
+
+// gradient of W_poly6(|d|) in d: -6 poly6 (h^2 - r^2)^2 d
+__device__ __forceinline__ void adj_grad_w6(const KConst &k, const float *dv, float r2, float *g)
+{
+    const float t = k.h2 - r2;
+    const float c = -((6.0f * k.poly6) * (t * t));
+    for (int a = 0; a < 3; a++) g[a] = c * dv[a];
+}
+
+// tensile term s(d) = -k (W / W(dq h))^n and its gradient in d
+__device__ __forceinline__ float adj_corr(const KConst &k, const float *dv, float r2, float *grad)
+{
+    const float t = k.h2 - r2;
+    const float W = k.poly6 * ((t * t) * t);
+    const float x = W * k.inv_w_dq;
+    const uint32_t n = k.n_corr;
+    const float s = -(k.k_corr * ipow(x, n));
+    const float xn1 = n > 1u ? ipow(x, n - 1u) : 1.0f;
+    const float c = (((k.k_corr * (float)n) * xn1) * k.inv_w_dq) * ((6.0f * k.poly6) * (t * t));
+    for (int a = 0; a < 3; a++) grad[a] = c * dv[a];
+    return s;
+}
+
+// E = d/dd [mu . (l_i + l_j + s(d)) G(d)] for owner i, d = p_i - p_j;
+// J_G = -(b I + (b'/r) d d^T), b' = -spiky (h - r)(h + r) / r^2
+__device__ __forceinline__ void adj_pair(const KConst &k, const float *dv, float r2, float li, float lj,
+                                         const float *mu, float *E)
+{
+    float gc[3], g[3];
+    const float s = adj_corr(k, dv, r2, gc);
+    const float coef = (li + lj) + s;
+    adj_gamma(k, dv, r2, g);
+    const float mg = dot3(mu, g);
+    for (int a = 0; a < 3; a++) E[a] = mg * gc[a];
+    if (r2 < kDegenerateR2) return;
+    const float ri = rsq(r2);
+    const float r = r2 * ri;
+    const float q = k.h - r;
+    const float bb = (k.spiky * (q * q)) * ri;
+    const float bp = -(((k.spiky * q) * (k.h + r)) * (ri * ri));
+    const float dm = dot3(dv, mu);
+    const float c2 = (bp * ri) * dm;
+    for (int a = 0; a < 3; a++) E[a] = E[a] - (coef * ((bb * mu[a]) + (c2 * dv[a])));
+}
+
+// dD_i/dd for owner i (non-degenerate pair), sk = spiky / rho0_i
+__device__ __forceinline__ void adj_dD(const KConst &k, const float *dv, float r2, float sk, const float *S,
+                                       float *out)
+{
+    const float ri = rsq(r2);
+    const float r = r2 * ri;
+    const float q = k.h - r;
+    const float aa = (sk * (q * q)) * ri;
+    const float ap = -(((sk * q) * (k.h + r)) * (ri * ri));
+    const float ds = dot3(dv, S);
+    // d(a^2 r^2)/dd = -4 sk^2 (h - r)^3 d / r
+    const float c1 = -((((4.0f * sk) * sk) * ((q * q) * q)) * ri);
+    for (int a = 0; a < 3; a++) out[a] = (c1 * dv[a]) + (2.0f * ((aa * S[a]) + (((ap * ri) * ds) * dv[a])));
+}
+
+// density multiplier of neighbour j in rho_i
+__device__ __forceinline__ float adj_mass(const KConst &k, const Attr &aj)
+{
+    float m = 1.0f / aj.w;
+    if (aj.phase != PS_PHASE_FLUID) m = m * k.s_solid;
+    return m;
+}
+
+// Cloth contact of owner i with j (collideD without friction: a distance
+// constraint of rest collide_dist, stiffness 1, count nc): M^T y for
+// M = -(w_i / (wsum nc)) [(1 - cd/r) I + (cd/r^3) d d^T]; false when the
+// pair is not in i's contact set.
+__device__ __forceinline__ bool adj_contact(const KConst &k, const Attr &ai, const Attr &aj, uint32_t nci,
+                                            const float *dv, const float *y, float *out)
+{
+    const int32_t ph = ai.phase;
+    if (ph < PS_PHASE_CLOTH || ai.w == 0.0f || nci == 0u) return false;
+    if (ph > PS_PHASE_SOLID && ph == aj.phase) return false;
+    const float r2 = ((dv[0] * dv[0]) + (dv[1] * dv[1])) + (dv[2] * dv[2]);
+    if (!(r2 < k.collide_r2)) return false;
+    const float dist = sqrtf(r2);
+    if (dist < 1e-6f) return false;
+    const float wsum = ai.w + aj.w;
+    if (wsum == 0.0f) return false;
+    const float cf = -((ai.w / wsum) / (float)nci);
+    const float k1 = 1.0f - (k.collide_dist / dist), k2 = k.collide_dist / ((dist * dist) * dist);
+    const float dy = dot3(dv, y);
+    for (int a = 0; a < 3; a++) out[a] = cf * ((k1 * y[a]) + ((k2 * dy) * dv[a]));
+    return true;
+}
+
+__device__ __forceinline__ void load3(const float4 &v, float *o)
+{
+    o[0] = v.x;
+    o[1] = v.y;
+    o[2] = v.z;
+}
+
+// masked dL/dy of particle i as the Jacobian uses it (z dropped in 2D)
+__device__ __forceinline__ void yd_of(const KConst &k, const float4 &Y, float *yd)
+{
+    yd[0] = Y.x;
+    yd[1] = Y.y;
+    yd[2] = k.dims == 2u ? 0.0f : Y.z;
+}
+
+// ---------------------------------------------------------------------------
+// Pins, walls and wall friction backwards, contact counts, and the forward
+// quantities of the lambda pass (rho, S, D) and of the fluid update (f).
+// Wall friction (walls(): p'' = p' - t f, t = e - (e.n) n, e = p' - x*):
+// static, or kinetic clamped to f = 1: dp''/dp' = n n^T, dp''/dx* = I - n n^T;
+// kinetic with f = wall_k / l < 1: dp''/dp' = I - wall_k K (I - n n^T),
+// K = (I - t^ t^T) / l, the x* part its negative second term, and
+// dp''/dwall_k = -t / l.
+__global__ __launch_bounds__(kAdjBlock) void adj_pre_k(DevBuffers b, KConst k, AdjBuffers a, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * kAdjBlock + threadIdx.x;
+    if (i >= n) return;
+    const float4 pi = a.Pit[i];
+    const Attr ai = b.A[i];
+    float g[3];
+    load3(a.pbn[i], g);
+    const uint32_t cl = a.clamp[i];
+    if (cl & kPinned) {
+        g[0] = g[1] = g[2] = 0.0f;
+    } else if (cl & kWallFriction) {
+        const float4 rec = a.wrec[i];
+        const float4 xs = b.X[i];
+        const int code = (int)rec.w;
+        const float nn[3] = {(float)(code / 9 - 1), (float)((code / 3) % 3 - 1), (float)(code % 3 - 1)};
+        const float e[3] = {rec.x - xs.x, rec.y - xs.y, rec.z - xs.z};
+        const float dn = dot3(e, nn);
+        const float t[3] = {e[0] - (dn * nn[0]), e[1] - (dn * nn[1]), e[2] - (dn * nn[2])};
+        float4 xb = a.xsb[i];
+        float xv[3] = {xb.x, xb.y, xb.z};
+        if (!(cl & kWallKinetic)) {
+            const float ng = dot3(nn, g);
+            for (int q = 0; q < 3; q++) {
+                const float pa = ng * nn[q];
+                xv[q] = xv[q] + (g[q] - pa);
+                g[q] = pa;
+            }
+        } else {
+            const float l = sqrtf(dot3(t, t));
+            const float tg = dot3(t, g);
+            float kg[3];
+            for (int q = 0; q < 3; q++) kg[q] = (g[q] - ((t[q] * tg) / (l * l))) / l;
+            const float nk = dot3(nn, kg);
+            for (int q = 0; q < 3; q++) {
+                const float qq = k.wall_k * (kg[q] - (nk * nn[q]));
+                xv[q] = xv[q] + qq;
+                g[q] = g[q] - qq;
+            }
+            a.wkb[i] = a.wkb[i] - (tg / l);
+        }
+        a.xsb[i] = make_float4(xv[0], xv[1], xv[2], 0.0f);
+    }
+    for (int q = 0; q < 3; q++)
+        if (cl & (1u << q)) g[q] = 0.0f;
+    const float yd[3] = {g[0], g[1], k.dims == 2u ? 0.0f : g[2]};
+    const uint32_t c = b.ncnt[i];
+    // contact count (collideD's first loop)
+    uint32_t nc = 0;
+    if (ai.phase >= PS_PHASE_CLOTH && ai.w != 0.0f) {
+        for (uint32_t q = 0; q < c; q++) {
+            const uint32_t j = nbr_at(b, i, q);
+            if (ai.phase > PS_PHASE_SOLID && ai.phase == b.A[j].phase) continue;
+            const float4 pj = a.Pit[j];
+            const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+            const float r2 = ((dx * dx) + (dy * dy)) + (dz * dz);
+            if (r2 < k.collide_r2) nc++;
+        }
+    }
+    a.nc[i] = nc;
+    float rho = 0.0f, D = 0.0f, S[3] = {0.0f, 0.0f, 0.0f};
+    if (ai.phase == PS_PHASE_FLUID) {
+        const float r0 = ai.rho0;
+        const float sk = k.spiky * (1.0f / r0);
+        float rr = k.w_zero * (1.0f / ai.w);
+        float gix = 0.0f, giy = 0.0f, giz = 0.0f, den = 0.0f;
+        float f[3] = {0.0f, 0.0f, 0.0f};
+        for (uint32_t q = 0; q < c; q++) {
+            const uint32_t j = nbr_at(b, i, q);
+            const Attr aj = b.A[j];
+            if (aj.w == 0.0f) continue;
+            const float4 pj = a.Pit[j];
+            const float dv[3] = {pi.x - pj.x, pi.y - pj.y, pi.z - pj.z};
+            const float r2 = dist2(dv[0], dv[1], dv[2]);
+            if (!(r2 < k.h2)) continue;
+            const float t = k.h2 - r2;
+            const float W = k.poly6 * ((t * t) * t);
+            rr = __builtin_fmaf(W, adj_mass(k, aj), rr);
+            float gc[3], gg[3];
+            const float coef = (pi.w + pj.w) + adj_corr(k, dv, r2, gc);
+            adj_gamma(k, dv, r2, gg);
+            for (int q2 = 0; q2 < 3; q2++) f[q2] = f[q2] + (coef * gg[q2]);
+            if (r2 < kDegenerateR2) continue;
+            const float ri = rsq(r2);
+            const float qq = k.h - (r2 * ri);
+            const float aa = (sk * (qq * qq)) * ri;
+            den = __builtin_fmaf(aa * aa, r2, den);
+            gix = __builtin_fmaf(aa, dv[0], gix);
+            giy = __builtin_fmaf(aa, dv[1], giy);
+            giz = __builtin_fmaf(aa, dv[2], giz);
+        }
+        den = den + dist2(gix, giy, giz);
+        rho = rr;
+        D = den;
+        S[0] = gix;
+        S[1] = giy;
+        S[2] = giz;
+        const float ir0 = 1.0f / r0;
+        a.r0b[i] = a.r0b[i] - (dot3(yd, f) * (ir0 * ir0));
+    }
+    a.Y[i] = make_float4(g[0], g[1], g[2], rho);
+    a.SD[i] = make_float4(S[0], S[1], S[2], D);
+}
+
+// dL/dl -> dL/dC = -lb / (D + eps), dL/dD = -lb l / (D + eps); rho0 terms
+__global__ __launch_bounds__(kAdjBlock) void adj_lambda_k(DevBuffers b, KConst k, AdjBuffers a, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * kAdjBlock + threadIdx.x;
+    if (i >= n) return;
+    const Attr ai = b.A[i];
+    if (ai.phase != PS_PHASE_FLUID) {
+        a.CD[i] = make_float2(0.0f, 0.0f);
+        return;
+    }
+    const float4 pi = a.Pit[i];
+    const float4 Yi = a.Y[i];
+    const float ir0 = 1.0f / ai.rho0;
+    float yd[3], mu[3];
+    yd_of(k, Yi, yd);
+    for (int q = 0; q < 3; q++) mu[q] = yd[q] * ir0;
+    float lb = 0.0f;
+    const uint32_t c = b.ncnt[i];
+    for (uint32_t q = 0; q < c; q++) {
+        const uint32_t j = nbr_at(b, i, q);
+        const Attr aj = b.A[j];
+        if (aj.w == 0.0f) continue;
+        const float4 pj = a.Pit[j];
+        const float dv[3] = {pi.x - pj.x, pi.y - pj.y, pi.z - pj.z};
+        const float r2 = dist2(dv[0], dv[1], dv[2]);
+        if (!(r2 < k.h2)) continue;
+        float g[3];
+        adj_gamma(k, dv, r2, g);
+        lb = lb + dot3(mu, g);
+        if (aj.phase == PS_PHASE_FLUID) {
+            const float dj[3] = {-dv[0], -dv[1], -dv[2]};
+            adj_gamma(k, dj, r2, g);
+            float ydj[3], muj[3];
+            yd_of(k, a.Y[j], ydj);
+            const float irj = 1.0f / aj.rho0;
+            for (int q2 = 0; q2 < 3; q2++) muj[q2] = ydj[q2] * irj;
+            lb = lb + dot3(muj, g);
+        }
+    }
+    const float4 sd = a.SD[i];
+    const float den = sd.w + k.relax;
+    const float lam = pi.w;
+    const float Cb = -(lb / den);
+    const float Db = -((lb * lam) / den);
+    a.CD[i] = make_float2(Cb, Db);
+    float r = a.r0b[i];
+    r = r - (Cb * (Yi.w * (ir0 * ir0)));
+    r = r - (Db * ((2.0f * sd.w) * ir0));
+    a.r0b[i] = r;
+}
+
+// dL/dp before the iteration
+__global__ __launch_bounds__(kAdjBlock) void adj_position_k(DevBuffers b, KConst k, AdjBuffers a, uint32_t n)
+{
+    const uint32_t p = blockIdx.x * kAdjBlock + threadIdx.x;
+    if (p >= n) return;
+    const float4 pp = a.Pit[p];
+    const Attr ap = b.A[p];
+    const float4 Yp = a.Y[p];
+    float acc[3] = {Yp.x, Yp.y, Yp.z};
+    float ydp[3], mup[3];
+    yd_of(k, Yp, ydp);
+    const bool pf = ap.phase == PS_PHASE_FLUID;
+    const float irp = 1.0f / ap.rho0;
+    for (int q = 0; q < 3; q++) mup[q] = pf ? ydp[q] * irp : 0.0f;
+    const float2 cdp = a.CD[p];
+    const float4 sdp = a.SD[p];
+    const float Sp[3] = {sdp.x, sdp.y, sdp.z};
+    const uint32_t c = b.ncnt[p];
+    for (uint32_t q = 0; q < c; q++) {
+        const uint32_t j = nbr_at(b, p, q);
+        const float4 pj = a.Pit[j];
+        const float dv[3] = {pp.x - pj.x, pp.y - pj.y, pp.z - pj.z};
+        const float r2 = dist2(dv[0], dv[1], dv[2]);
+        if (!(r2 < k.h2)) continue;
+        const Attr aj = b.A[j];
+        const float dj[3] = {-dv[0], -dv[1], -dv[2]};
+        float E[3], g6[3], dd[3];
+        if (pf && aj.w != 0.0f) {
+            // p owns the pair
+            adj_pair(k, dv, r2, pp.w, pj.w, mup, E);
+            adj_grad_w6(k, dv, r2, g6);
+            const float cm = (cdp.x * adj_mass(k, aj)) * (1.0f / ap.rho0);
+            for (int q2 = 0; q2 < 3; q2++) acc[q2] = (acc[q2] + E[q2]) + (cm * g6[q2]);
+            if (!(r2 < kDegenerateR2)) {
+                adj_dD(k, dv, r2, k.spiky * (1.0f / ap.rho0), Sp, dd);
+                for (int q2 = 0; q2 < 3; q2++) acc[q2] = acc[q2] + (cdp.y * dd[q2]);
+            }
+        }
+        if (aj.phase == PS_PHASE_FLUID && ap.w != 0.0f) {
+            // j owns the pair (d_jp = -d)
+            float ydj[3], muj[3];
+            yd_of(k, a.Y[j], ydj);
+            const float irj = 1.0f / aj.rho0;
+            for (int q2 = 0; q2 < 3; q2++) muj[q2] = ydj[q2] * irj;
+            adj_pair(k, dj, r2, pj.w, pp.w, muj, E);
+            adj_grad_w6(k, dj, r2, g6);
+            const float2 cdj = a.CD[j];
+            const float cm = (cdj.x * adj_mass(k, ap)) * (1.0f / aj.rho0);
+            for (int q2 = 0; q2 < 3; q2++) acc[q2] = (acc[q2] - E[q2]) - (cm * g6[q2]);
+            if (!(r2 < kDegenerateR2)) {
+                const float4 sdj = a.SD[j];
+                const float Sj[3] = {sdj.x, sdj.y, sdj.z};
+                adj_dD(k, dj, r2, k.spiky * (1.0f / aj.rho0), Sj, dd);
+                for (int q2 = 0; q2 < 3; q2++) acc[q2] = acc[q2] - (cdj.y * dd[q2]);
+            }
+        }
+    }
+    // cloth contacts: p's own, and p as the partner of j's
+    if (a.contacts) {
+        const uint32_t ncp = a.nc[p];
+        for (uint32_t q = 0; q < c; q++) {
+            const uint32_t j = nbr_at(b, p, q);
+            const float4 pj = a.Pit[j];
+            const Attr aj = b.A[j];
+            const float dv[3] = {pp.x - pj.x, pp.y - pj.y, pp.z - pj.z};
+            const float dj[3] = {-dv[0], -dv[1], -dv[2]};
+            float m[3];
+            if (adj_contact(k, ap, aj, ncp, dv, ydp, m))
+                for (int q2 = 0; q2 < 3; q2++) acc[q2] = acc[q2] + m[q2];
+            float ydj[3];
+            yd_of(k, a.Y[j], ydj);
+            if (adj_contact(k, aj, ap, a.nc[j], dj, ydj, m))
+                for (int q2 = 0; q2 < 3; q2++) acc[q2] = acc[q2] - m[q2];
+        }
+    }
+    // distance constraints incident to p
+    if (a.dist) {
+        const uint32_t o = ap.orig;
+        for (uint32_t e = b.dist_off[o]; e < b.dist_off[o + 1]; e++) {
+            const uint32_t cid = b.dist_adj[e];
+            const uint2 pr = b.dist_pairs[cid];
+            const uint32_t j = b.slot_of[pr.x == o ? pr.y : pr.x];
+            const float4 pj = a.Pit[j];
+            const float dv[3] = {pp.x - pj.x, pp.y - pj.y, pp.z - pj.z};
+            const float dist = sqrtf(((dv[0] * dv[0]) + (dv[1] * dv[1])) + (dv[2] * dv[2]));
+            const Attr aj = b.A[j];
+            const float wsum = ap.w + aj.w;
+            if (dist < 1e-6f || wsum == 0.0f) continue;
+            const float2 rs = b.dist_rs[cid];
+            const float L = rs.x, st = rs.y;
+            const float k1 = 1.0f - (L / dist), k2 = L / ((dist * dist) * dist);
+            // M(d) = -(w st / (wsum occ)) [(1 - L/r) I + (L/r^3) d d^T]
+            if (ap.w != 0.0f) {
+                const float cf = -(((ap.w * st) / wsum) / (float)b.occ[o]);
+                const float dy = dot3(dv, ydp);
+                for (int q2 = 0; q2 < 3; q2++) acc[q2] = acc[q2] + (cf * ((k1 * ydp[q2]) + ((k2 * dy) * dv[q2])));
+            }
+            if (aj.w != 0.0f) {
+                float ydj[3];
+                yd_of(k, a.Y[j], ydj);
+                const float cf = -(((aj.w * st) / wsum) / (float)b.occ[aj.orig]);
+                const float dy = dot3(dv, ydj);
+                for (int q2 = 0; q2 < 3; q2++) acc[q2] = acc[q2] - (cf * ((k1 * ydj[q2]) + ((k2 * dy) * dv[q2])));
+            }
+        }
+    }
+    a.pb[p] = make_float4(acc[0], acc[1], acc[2], 0.0f);
+}
+
+// dL/dstiffness: Delta_a / st = -(w_a / (wsum occ_a)) (1 - L/r) d_ab
+__global__ __launch_bounds__(kAdjBlock) void adj_stiffness_k(DevBuffers b, KConst k, AdjBuffers a, uint32_t m)
+{
+    const uint32_t c = blockIdx.x * kAdjBlock + threadIdx.x;
+    if (c >= m) return;
+    const uint2 pr = b.dist_pairs[c];
+    const uint32_t ia = b.slot_of[pr.x], ib = b.slot_of[pr.y];
+    const float4 pa = a.Pit[ia], pb = a.Pit[ib];
+    const float dv[3] = {pa.x - pb.x, pa.y - pb.y, pa.z - pb.z};
+    const float dist = sqrtf(((dv[0] * dv[0]) + (dv[1] * dv[1])) + (dv[2] * dv[2]));
+    const float wa = b.A[ia].w, wb = b.A[ib].w, wsum = wa + wb;
+    if (dist < 1e-6f || wsum == 0.0f) return;
+    const float k1 = 1.0f - (b.dist_rs[c].x / dist);
+    float yda[3], ydb[3];
+    yd_of(k, a.Y[ia], yda);
+    yd_of(k, a.Y[ib], ydb);
+    const float ya = dot3(dv, yda);
+    const float ybv = dot3(dv, ydb);
+    float g = 0.0f;
+    if (wa != 0.0f) g = g - (((wa / wsum) / (float)b.occ[pr.x]) * (k1 * ya));
+    if (wb != 0.0f) g = g + (((wb / wsum) / (float)b.occ[pr.y]) * (k1 * ybv));
+    a.sb[c] = a.sb[c] + g;
+}
+
+// ---------------------------------------------------------------------------
+// velocity + XSPH backwards: v_i = u_i/dt + ((c/rho0_i) S_i)/dt,
+// S_i = sum_fluid j (u_j - u_i) m_j W_ij, u = p - x*.  adj_xsph_k forms
+// dL/dS_i = (c/rho0_i) vbar_i / dt (in Y) and the rho0 term; adj_finalize_k
+// gathers dL/dp (the final iterate) and dL/dx*.
+
+__global__ __launch_bounds__(kAdjBlock) void adj_xsph_k(DevBuffers b, KConst k, AdjBuffers a, uint32_t n,
+                                                     float dt)
+{
+    const uint32_t i = blockIdx.x * kAdjBlock + threadIdx.x;
+    if (i >= n) return;
+    const Attr ai = b.A[i];
+    float r0 = 0.0f;
+    float Sh[3] = {0.0f, 0.0f, 0.0f};
+    if (k.xsph_c != 0.0f && ai.phase == PS_PHASE_FLUID) {
+        const float4 pi = b.P[i], xi = b.X[i];
+        const float ui[3] = {pi.x - xi.x, pi.y - xi.y, pi.z - xi.z};
+        const float4 vb = a.gv[ai.orig];
+        const float vbn[3] = {vb.x, vb.y, vb.z};
+        const float cr = k.xsph_c / ai.rho0;
+        float Sx = 0.0f, Sy = 0.0f, Sz = 0.0f;
+        const uint32_t c = b.ncnt[i];
+        for (uint32_t q = 0; q < c; q++) {
+            const uint32_t j = nbr_at(b, i, q);
+            const Attr aj = b.A[j];
+            if (aj.phase != PS_PHASE_FLUID) continue;
+            const float4 pj = b.P[j], xj = b.X[j];
+            const float r2 = dist2(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z);
+            if (!(r2 < k.h2)) continue;
+            const float t = k.h2 - r2;
+            const float m = (k.poly6 * ((t * t) * t)) * (1.0f / aj.w);
+            Sx = __builtin_fmaf((pj.x - xj.x) - ui[0], m, Sx);
+            Sy = __builtin_fmaf((pj.y - xj.y) - ui[1], m, Sy);
+            Sz = __builtin_fmaf((pj.z - xj.z) - ui[2], m, Sz);
+        }
+        const float S[3] = {Sx, Sy, Sz};
+        const float vS = dot3(vbn, S);
+        r0 = r0 - (((k.xsph_c / (ai.rho0 * ai.rho0)) * vS) / dt);
+        for (int q = 0; q < 3; q++) Sh[q] = (vbn[q] * cr) / dt;
+    }
+    a.r0b[i] = r0;
+    a.wkb[i] = 0.0f;
+    a.Y[i] = make_float4(Sh[0], Sh[1], Sh[2], 0.0f);
+}
+
+__global__ __launch_bounds__(kAdjBlock) void adj_finalize_k(DevBuffers b, KConst k, AdjBuffers a, uint32_t n,
+                                                         float dt)
+{
+    const uint32_t p = blockIdx.x * kAdjBlock + threadIdx.x;
+    if (p >= n) return;
+    const Attr ap = b.A[p];
+    const float4 vb = a.gv[ap.orig], xb = a.gx[ap.orig];
+    float ub[3] = {vb.x / dt, vb.y / dt, vb.z / dt};
+    float acc[3] = {xb.x, xb.y, xb.z};
+    if (k.xsph_c != 0.0f && ap.phase == PS_PHASE_FLUID) {
+        const float4 pp = b.P[p], xp = b.X[p];
+        const float up[3] = {pp.x - xp.x, pp.y - xp.y, pp.z - xp.z};
+        const float4 shp4 = a.Y[p];
+        const float Shp[3] = {shp4.x, shp4.y, shp4.z};
+        const float mp = 1.0f / ap.w;
+        const uint32_t c = b.ncnt[p];
+        for (uint32_t q = 0; q < c; q++) {
+            const uint32_t j = nbr_at(b, p, q);
+            const Attr aj = b.A[j];
+            if (aj.phase != PS_PHASE_FLUID) continue;
+            const float4 pj = b.P[j], xj = b.X[j];
+            const float dv[3] = {pp.x - pj.x, pp.y - pj.y, pp.z - pj.z};
+            const float r2 = dist2(dv[0], dv[1], dv[2]);
+            if (!(r2 < k.h2)) continue;
+            const float t = k.h2 - r2;
+            const float W = k.poly6 * ((t * t) * t);
+            float g6[3];
+            adj_grad_w6(k, dv, r2, g6);
+            const float uj[3] = {pj.x - xj.x, pj.y - xj.y, pj.z - xj.z};
+            const float4 shj4 = a.Y[j];
+            const float Shj[3] = {shj4.x, shj4.y, shj4.z};
+            // p as owner: S_p += (u_j - u_p) m_j W; p as neighbour of j
+            const float mj = 1.0f / aj.w;
+            const float wb = mj * ((((uj[0] - up[0]) * Shp[0]) + ((uj[1] - up[1]) * Shp[1])) +
+                                   ((uj[2] - up[2]) * Shp[2]));
+            const float wb2 = mp * ((((up[0] - uj[0]) * Shj[0]) + ((up[1] - uj[1]) * Shj[1])) +
+                                    ((up[2] - uj[2]) * Shj[2]));
+            for (int q2 = 0; q2 < 3; q2++) {
+                ub[q2] = (ub[q2] - ((mj * W) * Shp[q2])) + ((mp * W) * Shj[q2]);
+                acc[q2] = (acc[q2] + (wb * g6[q2])) + (wb2 * g6[q2]);
+            }
+        }
+    }
+    a.pb[p] = make_float4(acc[0] + ub[0], acc[1] + ub[1], acc[2] + ub[2], 0.0f);
+    a.xsb[p] = make_float4(-ub[0], -ub[1], -ub[2], 0.0f);
+}
+
+// predict backwards: p0 = x* + dt (v + dt g) for w != 0 (z held in 2D);
+// results to insertion order, rho0 and wall_k terms accumulated there
+__global__ __launch_bounds__(kAdjBlock) void adj_predict_k(DevBuffers b, KConst k, AdjBuffers a, uint32_t n,
+                                                        float dt)
+{
+    const uint32_t t = blockIdx.x * kAdjBlock + threadIdx.x;
+    if (t >= n) return;
+    const Attr at = b.A[t];
+    const uint32_t o = at.orig;
+    const float4 xs = a.xsb[t], pb = a.pb[t];
+    const bool mv = at.w != 0.0f;
+    const bool twod = k.dims == 2u;
+    a.gx[o] = make_float4(xs.x + pb.x, xs.y + pb.y, xs.z + pb.z, 0.0f);
+    a.gv[o] = make_float4(mv ? dt * pb.x : 0.0f, mv ? dt * pb.y : 0.0f, (mv && !twod) ? dt * pb.z : 0.0f,
+                          0.0f);
+    a.drho[o] = a.drho[o] + a.r0b[t];
+    a.dwk[o] = a.dwk[o] + a.wkb[t];
+}
+
+// ---------------------------------------------------------------------------
+
+static inline uint32_t adj_blocks(uint32_t n) { return (n + kAdjBlock - 1) / kAdjBlock; }
+
+hipError_t launch_adj_finalize(hipStream_t st, const DevBuffers &b, const KConst &k, const AdjBuffers &a,
+                               uint32_t n, float dt)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(adj_xsph_k, dim3(adj_blocks(n)), dim3(kAdjBlock), 0, st, b, k, a, n, dt);
+    hipLaunchKernelGGL(adj_finalize_k, dim3(adj_blocks(n)), dim3(kAdjBlock), 0, st, b, k, a, n, dt);
+    return hipGetLastError();
+}
+
+hipError_t launch_adj_iteration(hipStream_t st, const DevBuffers &b, const KConst &k, const AdjBuffers &a,
+                                uint32_t n, uint32_t ndist)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(adj_pre_k, dim3(adj_blocks(n)), dim3(kAdjBlock), 0, st, b, k, a, n);
+    hipLaunchKernelGGL(adj_lambda_k, dim3(adj_blocks(n)), dim3(kAdjBlock), 0, st, b, k, a, n);
+    hipLaunchKernelGGL(adj_position_k, dim3(adj_blocks(n)), dim3(kAdjBlock), 0, st, b, k, a, n);
+    if (ndist)
+        hipLaunchKernelGGL(adj_stiffness_k, dim3(adj_blocks(ndist)), dim3(kAdjBlock), 0, st, b, k, a, ndist);
+    return hipGetLastError();
+}
+
+hipError_t launch_adj_predict(hipStream_t st, const DevBuffers &b, const KConst &k, const AdjBuffers &a,
+                              uint32_t n, float dt)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(adj_predict_k, dim3(adj_blocks(n)), dim3(kAdjBlock), 0, st, b, k, a, n, dt);
+    return hipGetLastError();
+}
+
+}  // namespace ps

@@ -40,6 +40,7 @@ struct ps_system {
     bool uniform = true;     // every particle fluid with one inverse mass
     float wu = 1.0f;
     bool any_contact_phase = false;
+    bool any_solid_phase = false;  // SOLID or RIGID particles present
     // host copies of the constraint sets (insertion indices)
     std::vector<uint32_t> dist_pairs;
     std::vector<float> dist_rest, dist_stiff;
@@ -83,6 +84,22 @@ struct ps_system {
     double total_ms[PS_K_COUNT] = {};
     uint64_t launches[PS_K_COUNT] = {};
     std::string err;
+    // reverse pass (ps_step_backward): workspace grown on demand and kept
+    struct AdjWork {
+        size_t cap = 0, slots = 0, iters = 0, ndist = 0;
+        float4 *ckX = nullptr, *ckV = nullptr;  // checkpoints, slots x cap
+        Attr *ckA = nullptr;
+        uint32_t *ckS = nullptr;
+        ps::DevStatus *ckSt = nullptr;
+        float4 *Pit = nullptr, *wrec = nullptr;  // iters x cap
+        uint8_t *clamp = nullptr;
+        float4 *pb = nullptr, *pb2 = nullptr, *Y = nullptr, *SD = nullptr, *xsb = nullptr;
+        float4 *gx = nullptr, *gv = nullptr;
+        float2 *CD = nullptr;
+        uint32_t *nc = nullptr;
+        float *r0b = nullptr, *wkb = nullptr, *drho = nullptr, *dwk = nullptr, *sb = nullptr;
+    } adj;
+    bool adj_active = false;  // inside ps_step_backward: neighbours stay global
 };
 
 namespace {
@@ -293,6 +310,17 @@ void dev_free(T *&p)
     p = nullptr;
 }
 
+void adj_free(ps_system *s)
+{
+    ps_system::AdjWork &w = s->adj;
+    dev_free(w.ckX); dev_free(w.ckV); dev_free(w.ckA); dev_free(w.ckS); dev_free(w.ckSt);
+    dev_free(w.Pit); dev_free(w.wrec); dev_free(w.clamp); dev_free(w.pb); dev_free(w.pb2);
+    dev_free(w.Y); dev_free(w.SD); dev_free(w.xsb); dev_free(w.gx); dev_free(w.gv); dev_free(w.CD);
+    dev_free(w.nc); dev_free(w.r0b); dev_free(w.wkb); dev_free(w.drho); dev_free(w.dwk);
+    dev_free(w.sb);
+    w.cap = w.slots = w.iters = w.ndist = 0;
+}
+
 void free_all(ps_system *s)
 {
     DevBuffers &b = s->b;
@@ -306,6 +334,7 @@ void free_all(ps_system *s)
     dev_free(b.body_goal); dev_free(b.body_of); dev_free(b.member_of); dev_free(b.status);
     dev_free(b.sid); dev_free(b.sid2); dev_free(b.scenes); dev_free(b.lam); dev_free(b.bc);
     dev_free(b.stage);
+    adj_free(s);
 }
 
 void drop_graph(ps_system *s)
@@ -413,8 +442,8 @@ int stage_sort(ps_system *s)
 // composition whose kernels have staged variants).
 bool staging_ok(const ps_system *s)
 {
-    return !s->cpuref && !(s->prm.flags & PS_FLAG_NO_LDS_STAGING) && s->kc.nscenes == 1 &&
-           uniform_fluid(s);
+    return !s->cpuref && !s->adj_active && !(s->prm.flags & PS_FLAG_NO_LDS_STAGING) &&
+           s->kc.nscenes == 1 && uniform_fluid(s);
 }
 
 int stage_neighbors(ps_system *s)
@@ -966,7 +995,7 @@ static int add_particles_impl(ps_system *s, uint32_t scene, uint32_t n, const fl
     std::vector<uint32_t> slot(n);
     bool uni = s->uniform;
     float wu = s->n ? s->wu : inv_mass[0];
-    bool contact = s->any_contact_phase;
+    bool contact = s->any_contact_phase, solid = s->any_solid_phase;
     for (uint32_t k = 0; k < n; k++) {
         for (int a = 0; a < 3; a++) {
             if (!is_finite(pos[3 * k + a]) || (vel && !is_finite(vel[3 * k + a])))
@@ -990,6 +1019,7 @@ static int add_particles_impl(ps_system *s, uint32_t scene, uint32_t n, const fl
         slot[k] = s->n + k;
         if (phase[k] != PS_PHASE_FLUID || inv_mass[k] != wu) uni = false;
         if (phase[k] >= PS_PHASE_CLOTH) contact = true;
+        if (phase[k] >= PS_PHASE_SOLID) solid = true;
     }
     const size_t o = s->n;
     DevBuffers &b = s->b;
@@ -1017,6 +1047,7 @@ static int add_particles_impl(ps_system *s, uint32_t scene, uint32_t n, const fl
     s->uniform = uni;
     s->wu = wu;
     s->any_contact_phase = contact;
+    s->any_solid_phase = solid;
     s->nbr_valid = false;
     return PS_OK;
 }
@@ -1350,6 +1381,276 @@ int ps_step_graph(ps_system *s, float dt, uint32_t substeps)
     s->iters += (uint64_t)substeps * s->prm.iterations;
     s->nbr_valid = true;
     return PS_OK;
+}
+
+// ---- reverse mode ---------------------------------------------------------
+
+namespace {
+
+int adj_reserve(ps_system *s, size_t slots, size_t iters)
+{
+    ps_system::AdjWork &w = s->adj;
+    const size_t ndist = s->dist_rest.size();
+    if (w.cap >= s->cap && w.slots >= slots && w.iters >= iters && w.ndist >= ndist && w.ckX) return PS_OK;
+    adj_free(s);
+    const size_t c = s->cap;
+    int rc;
+    if ((rc = dev_alloc(s, &w.ckX, slots * c)) || (rc = dev_alloc(s, &w.ckV, slots * c)) ||
+        (rc = dev_alloc(s, &w.ckA, slots * c)) || (rc = dev_alloc(s, &w.ckS, slots * c)) ||
+        (rc = dev_alloc(s, &w.ckSt, slots)) || (rc = dev_alloc(s, &w.Pit, iters * c)) ||
+        (rc = dev_alloc(s, &w.wrec, iters * c)) || (rc = dev_alloc(s, &w.clamp, iters * c)) ||
+        (rc = dev_alloc(s, &w.pb, c)) || (rc = dev_alloc(s, &w.pb2, c)) || (rc = dev_alloc(s, &w.Y, c)) ||
+        (rc = dev_alloc(s, &w.SD, c)) || (rc = dev_alloc(s, &w.xsb, c)) || (rc = dev_alloc(s, &w.gx, c)) ||
+        (rc = dev_alloc(s, &w.gv, c)) || (rc = dev_alloc(s, &w.CD, c)) || (rc = dev_alloc(s, &w.nc, c)) ||
+        (rc = dev_alloc(s, &w.r0b, c)) || (rc = dev_alloc(s, &w.wkb, c)) || (rc = dev_alloc(s, &w.drho, c)) ||
+        (rc = dev_alloc(s, &w.dwk, c)) || (rc = dev_alloc(s, &w.sb, ndist))) {
+        adj_free(s);
+        return rc == PS_ERR_DEVICE ? fail(s, PS_ERR_OUT_OF_MEMORY, "reverse-pass workspace") : rc;
+    }
+    w.cap = c;
+    w.slots = slots;
+    w.iters = iters;
+    w.ndist = ndist;
+    return PS_OK;
+}
+
+// copy the per-particle state (and the device status word) to / from slot k
+int adj_copy(ps_system *s, size_t k, bool save)
+{
+    ps_system::AdjWork &w = s->adj;
+    DevBuffers &b = s->b;
+    const size_t n = s->n, off = k * w.cap;
+    hipError_t e = hipSuccess;
+    auto cp = [&](void *keep, void *live, size_t bytes) {
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(save ? keep : live, save ? live : keep, bytes, hipMemcpyDeviceToDevice,
+                               s->stream);
+    };
+    cp(w.ckX + off, b.X, sizeof(float4) * n);
+    cp(w.ckV + off, b.V, sizeof(float4) * n);
+    cp(w.ckA + off, b.A, sizeof(Attr) * n);
+    cp(w.ckS + off, b.slot_of, sizeof(uint32_t) * n);
+    cp(w.ckSt + k, b.status, sizeof(ps::DevStatus));
+    if (e != hipSuccess) return fail(s, PS_ERR_DEVICE, std::string("checkpoint copy: ") + hipGetErrorString(e));
+    return PS_OK;
+}
+
+int adj_substep(ps_system *s, float dt)
+{
+    int rc;
+    if ((rc = ps_substep_begin(s, dt))) return rc;
+    for (uint32_t it = 0; it < s->prm.iterations; it++)
+        if ((rc = stage_iterate(s))) return rc;
+    return stage_end(s);
+}
+
+// The substep once more, keeping each iteration's iterate (with lambda),
+// clamp bits and wall-friction records; the final iterate stays in b.P.
+int adj_record(ps_system *s, float dt)
+{
+    int rc;
+    if ((rc = ps_substep_begin(s, dt))) return rc;
+    ps_system::AdjWork &w = s->adj;
+    DevBuffers &b = s->b;
+    const bool uni = uniform_fluid(s);
+    for (uint32_t it = 0; it < s->prm.iterations; it++) {
+        LAUNCH(s, PS_K_LAMBDA, ps::launch_lambda(s->stream, b, s->kc, s->n, uni, s->wu, s->block[PS_K_LAMBDA]));
+        HIP_TRY(s, hipMemcpyAsync(w.Pit + it * w.cap, b.P, sizeof(float4) * s->n, hipMemcpyDeviceToDevice,
+                                  s->stream));
+        b.clamp_rec = w.clamp + it * w.cap;
+        b.wall_rec = w.wrec + it * w.cap;
+        const hipError_t e = ps::launch_solve(s->stream, b, s->kc, s->n, features(s), uni, s->wu, s->it_sub,
+                                              s->block[PS_K_SOLVE]);
+        b.clamp_rec = nullptr;
+        b.wall_rec = nullptr;
+        if (e != hipSuccess) return fail(s, PS_ERR_DEVICE, std::string("launch solve: ") + hipGetErrorString(e));
+        std::swap(b.P, b.Q);
+        s->iters++;
+        s->it_sub++;
+    }
+    return PS_OK;
+}
+
+// reverse of the recorded substep: insertion-order gx/gv of its end state
+// in, of its start state out; rho0, stiffness and wall_k terms accumulate
+int adj_reverse(ps_system *s, float dt)
+{
+    ps_system::AdjWork &w = s->adj;
+    ps::AdjBuffers a{};
+    a.Y = w.Y;
+    a.SD = w.SD;
+    a.CD = w.CD;
+    a.nc = w.nc;
+    a.r0b = w.r0b;
+    a.wkb = w.wkb;
+    a.xsb = w.xsb;
+    a.sb = w.sb;
+    a.gx = w.gx;
+    a.gv = w.gv;
+    a.drho = w.drho;
+    a.dwk = w.dwk;
+    a.contacts = s->any_contact_phase ? 1u : 0u;
+    a.dist = s->dist_rest.empty() ? 0u : 1u;
+    float4 *cur = w.pb, *nxt = w.pb2;
+    a.pb = cur;
+    hipError_t e = ps::launch_adj_finalize(s->stream, s->b, s->kc, a, s->n, dt);
+    const uint32_t ndist = (uint32_t)s->dist_rest.size();
+    for (uint32_t it = s->prm.iterations; e == hipSuccess && it-- > 0;) {
+        a.Pit = w.Pit + it * w.cap;
+        a.clamp = w.clamp + it * w.cap;
+        a.wrec = w.wrec + it * w.cap;
+        a.pbn = cur;
+        a.pb = nxt;
+        e = ps::launch_adj_iteration(s->stream, s->b, s->kc, a, s->n, ndist);
+        std::swap(cur, nxt);
+    }
+    a.pb = cur;
+    if (e == hipSuccess) e = ps::launch_adj_predict(s->stream, s->b, s->kc, a, s->n, dt);
+    if (e != hipSuccess) return fail(s, PS_ERR_DEVICE, std::string("launch adjoint: ") + hipGetErrorString(e));
+    s->stage = 0;
+    s->it_sub = 0;
+    s->hashed = false;
+    s->nbr_valid = false;
+    return PS_OK;
+}
+
+int step_backward_impl(ps_system *s, float dt, uint32_t S, uint32_t K, const float *dl_dx,
+                       const float *dl_dv, float *dl_dx0, float *dl_dv0, float *dl_drho0,
+                       float *dl_dstiffness, float *dl_dmu_k)
+{
+    const uint32_t n = s->n;
+    const uint32_t nseg = (S + K - 1) / K;
+    const size_t fin = nseg, seg0 = nseg + 1;
+    int rc;
+    if ((rc = adj_reserve(s, seg0 + K, s->prm.iterations ? s->prm.iterations : 1))) return rc;
+    ps_system::AdjWork &w = s->adj;
+    std::vector<uint64_t> sub(nseg + 1 + K), its(nseg + 1 + K);
+    auto save = [&](size_t k) {
+        sub[k] = s->substeps;
+        its[k] = s->iters;
+        return adj_copy(s, k, true);
+    };
+    auto load = [&](size_t k) {
+        s->substeps = sub[k];
+        s->iters = its[k];
+        return adj_copy(s, k, false);
+    };
+    // forward, checkpointing every K substeps
+    for (uint32_t t = 0; t < S; t++) {
+        if (t % K == 0 && (rc = save(t / K))) return rc;
+        if ((rc = adj_substep(s, dt))) return rc;
+    }
+    if ((rc = save(fin))) return rc;
+    ps::DevStatus ds;
+    HIP_TRY(s, hipStreamSynchronize(s->stream));
+    HIP_TRY(s, hipMemcpy(&ds, s->b.status, sizeof(ds), hipMemcpyDeviceToHost));
+    if (ds.flags & PS_STATUS_NEIGHBOR_OVERFLOW)
+        return fail(s, PS_ERR_CAPACITY,
+                    "a neighbour table overflowed (raise max_neighbors, or clear the status of an earlier "
+                    "overflow): the reverse pass needs every pair");
+    // seed: dL/dx_T, dL/dv_T in insertion order
+    std::vector<float4> g(n);
+    for (int which = 0; which < 2; which++) {
+        const float *src = which ? dl_dv : dl_dx;
+        for (uint32_t o = 0; o < n; o++)
+            g[o] = src ? make_float4(src[3 * o], src[3 * o + 1], src[3 * o + 2], 0.0f)
+                       : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        HIP_TRY(s, hipMemcpy(which ? w.gv : w.gx, g.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
+    }
+    HIP_TRY(s, hipMemsetAsync(w.drho, 0, sizeof(float) * n, s->stream));
+    HIP_TRY(s, hipMemsetAsync(w.dwk, 0, sizeof(float) * n, s->stream));
+    if (!s->dist_rest.empty())
+        HIP_TRY(s, hipMemsetAsync(w.sb, 0, sizeof(float) * s->dist_rest.size(), s->stream));
+    // segments backwards: recompute each substep start of the segment from
+    // its checkpoint, then reverse the substeps one by one
+    for (uint32_t gi = nseg; gi-- > 0;) {
+        const uint32_t s0 = gi * K, s1 = std::min(S, s0 + K);
+        if ((rc = load(gi))) return rc;
+        for (uint32_t t = s0; t < s1; t++) {
+            if ((rc = save(seg0 + (t - s0)))) return rc;
+            if (t + 1 < s1 && (rc = adj_substep(s, dt))) return rc;
+        }
+        for (uint32_t t = s1; t-- > s0;) {
+            if ((rc = load(seg0 + (t - s0)))) return rc;
+            if ((rc = adj_record(s, dt))) return rc;
+            if ((rc = adj_reverse(s, dt))) return rc;
+        }
+    }
+    if ((rc = load(fin))) return rc;
+    HIP_TRY(s, hipStreamSynchronize(s->stream));
+    // results
+    for (int which = 0; which < 2; which++) {
+        float *dst = which ? dl_dv0 : dl_dx0;
+        if (!dst) continue;
+        HIP_TRY(s, hipMemcpy(g.data(), which ? w.gv : w.gx, sizeof(float4) * n, hipMemcpyDeviceToHost));
+        for (uint32_t o = 0; o < n; o++) {
+            dst[3 * o] = g[o].x;
+            dst[3 * o + 1] = g[o].y;
+            dst[3 * o + 2] = g[o].z;
+        }
+    }
+    if (dl_drho0) HIP_TRY(s, hipMemcpy(dl_drho0, w.drho, sizeof(float) * n, hipMemcpyDeviceToHost));
+    if (dl_dstiffness && !s->dist_rest.empty())
+        HIP_TRY(s, hipMemcpy(dl_dstiffness, w.sb, sizeof(float) * s->dist_rest.size(), hipMemcpyDeviceToHost));
+    if (dl_dmu_k) {
+        // wall_k = sqrt(mu_k) radius: dwall_k/dmu_k = radius / (2 sqrt(mu_k)); the
+        // per-particle terms are summed in insertion order
+        std::vector<float> wk(n);
+        HIP_TRY(s, hipMemcpy(wk.data(), w.dwk, sizeof(float) * n, hipMemcpyDeviceToHost));
+        float sum = 0.0f;
+        for (uint32_t o = 0; o < n; o++) sum = sum + wk[o];
+        *dl_dmu_k = sum * (s->prm.radius / (2.0f * sqrtf(s->prm.wall_mu_k)));
+    }
+    return PS_OK;
+}
+
+}  // namespace
+
+int ps_step_backward(ps_system *s, float dt, uint32_t substeps, uint32_t checkpoint_every,
+                     const float *dl_dx, const float *dl_dv, float *dl_dx0, float *dl_dv0,
+                     float *dl_drho0, float *dl_dstiffness, float *dl_dmu_k)
+{
+    if (int rc = check_sys(s)) return rc;
+    if (s->cpuref) return unsupported_in_cpuref(s, "ps_step_backward");
+    if (s->stage) return fail(s, PS_ERR_STATE, "ps_step_backward inside an open substep");
+    if (!(dt > 0.0f) || !is_finite(dt)) return fail(s, PS_ERR_INVALID_ARGUMENT, "dt must be > 0");
+    if (s->kc.nscenes > 1) return fail(s, PS_ERR_UNSUPPORTED, "ps_step_backward runs on single-scene handles");
+    if (s->body_off.size() > 1 || s->any_solid_phase)
+        return fail(s, PS_ERR_UNSUPPORTED, "ps_step_backward covers no solid or rigid particles");
+    if (s->prm.flags & (PS_FLAG_SLEEPING | PS_FLAG_REBUILD_EVERY_ITERATION))
+        return fail(s, PS_ERR_UNSUPPORTED,
+                    "ps_step_backward needs one neighbour table per substep and no sleeping");
+    const uint32_t n = s->n;
+    const float *in[2] = {dl_dx, dl_dv};
+    for (const float *p : in)
+        if (p)
+            for (size_t q = 0; q < 3 * (size_t)n; q++)
+                if (!is_finite(p[q])) return fail(s, PS_ERR_NOT_FINITE, "non-finite loss gradient");
+    if (substeps == 0 || n == 0) {
+        for (size_t q = 0; q < 3 * (size_t)n; q++) {
+            if (dl_dx0) dl_dx0[q] = dl_dx ? dl_dx[q] : 0.0f;
+            if (dl_dv0) dl_dv0[q] = dl_dv ? dl_dv[q] : 0.0f;
+        }
+        if (dl_drho0) std::fill(dl_drho0, dl_drho0 + n, 0.0f);
+        if (dl_dstiffness) std::fill(dl_dstiffness, dl_dstiffness + s->dist_rest.size(), 0.0f);
+        if (dl_dmu_k) *dl_dmu_k = 0.0f;
+        return PS_OK;
+    }
+    const uint32_t K = checkpoint_every ? std::min(checkpoint_every, substeps) : 1u;
+    drop_graph(s);
+    s->adj_active = true;
+    const int rc = step_backward_impl(s, dt, substeps, K, dl_dx, dl_dv, dl_dx0, dl_dv0, dl_drho0,
+                                      dl_dstiffness, dl_dmu_k);
+    s->adj_active = false;
+    s->b.clamp_rec = nullptr;
+    s->b.wall_rec = nullptr;
+    if (rc) {
+        s->stage = 0;
+        s->it_sub = 0;
+        s->hashed = false;
+        s->nbr_valid = false;
+    }
+    return rc;
 }
 
 int ps_synchronize(ps_system *s)

@@ -191,11 +191,13 @@ __device__ __forceinline__ WallBox wall_box(const KConst &k, const DevBuffers &b
 }
 
 // Returns what it did, for the adjoint: bits 0-2 the coordinates a wall
-// set, bit 3 wall friction acted, bit 4 the kinetic branch with f < 1.
-constexpr uint32_t kWallFriction = 8u, kWallKinetic = 16u;
+// set, bit 3 wall friction acted, bit 4 the kinetic branch with f < 1 (the
+// solve kernel adds bit 5 for a pin, with bits 0-2).  When friction acts,
+// rec = (position before friction, (n_x+1)*9 + (n_y+1)*3 + (n_z+1)).
+constexpr uint32_t kWallFriction = 8u, kWallKinetic = 16u, kPinned = 32u;
 
 __device__ __forceinline__ uint32_t walls(const KConst &k, const WallBox &wb, int32_t ph, float4 xs,
-                                          float &px, float &py, float &pz, uint64_t iter)
+                                          float &px, float &py, float &pz, uint64_t iter, float4 &rec)
 {
     uint32_t set = 0u;
     const float r = k.radius;
@@ -242,6 +244,8 @@ __device__ __forceinline__ uint32_t walls(const KConst &k, const WallBox &wb, in
     float l = sqrtf(((tx * tx) + (ty * ty)) + (tz * tz));
     if (l < 0.001f) return set;
     set |= kWallFriction;
+    rec = make_float4(px, py, pz,
+                      (float)((((int)nx + 1) * 9 + ((int)ny + 1) * 3) + ((int)nz + 1)));
     if (l < k.wall_s) {
         px = px - tx;
         py = py - ty;

@@ -148,9 +148,32 @@ struct DevBuffers {
     // adjoint recompute only (else null): per particle, the coordinates the
     // walls and pins set and the wall-friction branch (walls() bits)
     uint8_t *clamp_rec;
+    float4 *wall_rec;      // with clamp_rec: position before wall friction,
+                           // normal code (n_x+1)*9 + (n_y+1)*3 + (n_z+1) in .w
     // CPU-reference mode only
     float *lam;            // lambda per particle
     uint32_t *bc;          // boundary flags | boundary count << 8
+};
+
+// Work buffers of the reverse pass (ps_adjoint.hip), storage order unless
+// noted.
+struct AdjBuffers {
+    const float4 *Pit;     // iterate of the iteration being reversed (x, y, z, lambda)
+    const uint8_t *clamp;  // its walls()/pin bits
+    const float4 *wrec;    // its wall-friction records
+    const float4 *pbn;     // dL/dp after the iteration
+    float4 *pb;            // dL/dp before it (finalize: of the final iterate)
+    float4 *Y;             // masked dL/dy (xyz) and density (w); finalize: dL/dS
+    float4 *SD;            // S = sum_j a d (xyz), D (w)
+    float2 *CD;            // dL/dC, dL/dD
+    uint32_t *nc;          // contact count
+    float *r0b, *wkb;      // dL/drho0 and dL/dwall_k of this substep
+    float4 *xsb;           // dL/dx* of this substep
+    float *sb;             // dL/dstiffness, per distance constraint
+    float4 *gx, *gv;       // insertion order: dL/dx, dL/dv of the state
+    float *drho, *dwk;     // insertion order: accumulated dL/drho0, dL/dwall_k
+    uint32_t contacts;     // the scene has contact phases
+    uint32_t dist;         // the scene has distance constraints
 };
 
 // Kernel launchers (ps_kernels.hip).  All enqueue on `st` and return the
@@ -175,6 +198,16 @@ hipError_t launch_finalize(hipStream_t st, const DevBuffers &b, const KConst &k,
                            uint32_t bs);
 
 uint32_t scan_tiles(uint32_t num_cells);
+
+// Reverse pass (ps_adjoint.hip): velocity + XSPH backwards (b.P the final
+// iterate, b.X the substep start), one iteration backwards (a.Pit, a.clamp,
+// a.wrec of that iteration; a.pbn -> a.pb), predict backwards.
+hipError_t launch_adj_finalize(hipStream_t st, const DevBuffers &b, const KConst &k, const AdjBuffers &a,
+                               uint32_t n, float dt);
+hipError_t launch_adj_iteration(hipStream_t st, const DevBuffers &b, const KConst &k, const AdjBuffers &a,
+                                uint32_t n, uint32_t ndist);
+hipError_t launch_adj_predict(hipStream_t st, const DevBuffers &b, const KConst &k, const AdjBuffers &a,
+                              uint32_t n, float dt);
 
 // CPU-reference mode (ps_cpuref.hip); storage in insertion order, iterate
 // ping-pong b.P -> b.Q per fluid object.

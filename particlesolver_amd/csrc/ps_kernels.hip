@@ -840,7 +840,8 @@ __global__ __launch_bounds__(kMaxBlock) void solve_k(DevBuffers b, KConst k, uin
     if (k.dims == 2u) nzp = pz;
     const float4 xs = (ph >= PS_PHASE_CLOTH) ? b.X[i] : make_float4(0.f, 0.f, 0.f, 0.f);
     const uint64_t iter = b.status->iter + it;
-    uint32_t set = walls(k, wall_box(k, b, i, a.orig), ph, xs, nxp, nyp, nzp, iter);
+    float4 wrec;
+    uint32_t set = walls(k, wall_box(k, b, i, a.orig), ph, xs, nxp, nyp, nzp, iter, wrec);
     if (F & F_PIN) {
         const uint32_t pin = b.pin_of[a.orig];
         if (pin) {
@@ -848,10 +849,13 @@ __global__ __launch_bounds__(kMaxBlock) void solve_k(DevBuffers b, KConst k, uin
             nxp = t.x;
             nyp = t.y;
             nzp = t.z;
-            set |= 7u;
+            set |= 7u | kPinned;
         }
     }
-    if (b.clamp_rec) b.clamp_rec[i] = (uint8_t)set;
+    if (b.clamp_rec) {
+        b.clamp_rec[i] = (uint8_t)set;
+        if (set & kWallFriction) b.wall_rec[i] = wrec;
+    }
     b.Q[i] = make_float4(nxp, nyp, nzp, 0.0f);
 }
 

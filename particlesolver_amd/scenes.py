@@ -309,13 +309,52 @@ def cloth_and_fluid(cloth=(6, 6), fluid=(4, 3, 4), spacing=0.1, cloth_spacing=0.
     p = pbf_params((0.0, 0.0, 0.0), ext, h, radius, iterations, n, 64, seed=seed)
     rho0 = lattice_rest_density(h, spacing)
     phase = np.zeros(n, np.int32)
-    phase[nf:] = 2
+    phase[nf:] = capi.PHASE_CLOTH
     vel = np.zeros((n, 3), np.float32)
     vel[nf:] = velocity
     s = Scene(p, pos, np.ones(n, np.float32), phase, np.full(n, np.float32(rho0), np.float32), dt,
               vel=vel, name="cloth_and_fluid")
     s.distance = cloth_constraints(cx, cz, nf, cloth_spacing, stiffness)
     s.pins = (np.array([nf], np.uint32), pos[nf:nf + 1].copy())
+    s.rho0 = rho0
+    return s
+
+
+def cloth_over_pool(cloth=(128, 128), fluid=(64, 16, 64), spacing=0.1, cloth_spacing=0.08,
+                    height=2.5, iterations=4, seed=0x5EED0005, dt=1.0 / 60.0,
+                    stiffness=(0.9, 0.5, 0.2)):
+    """The adjoint workload: a cloth (structural, shear and bend distance
+    constraints, the four corners pinned) held `height` above a fluid pool
+    resting on the floor of the box, falling onto it.  With the defaults,
+    16,384 cloth + 65,536 fluid particles.  The cloth spacing is below the
+    contact distance (adjacent cloth particles are in contact throughout)."""
+    fx, fy, fz = fluid
+    cx, cz = cloth
+    h = 2.0 * spacing
+    radius = 0.5 * spacing
+    wx = max(fx * spacing, cx * cloth_spacing) + 4.0 * spacing
+    wz = max(fz * spacing, cz * cloth_spacing) + 4.0 * spacing
+    ext = (wx, height + 1.5, wz)
+    pf = fluid_block(fx, fy, fz, spacing, (0.5 * (wx - fx * spacing), 0.0, 0.5 * (wz - fz * spacing)), 1e-3,
+                     seed)
+    i = np.arange(cx, dtype=np.float64)
+    k = np.arange(cz, dtype=np.float64)
+    zz, xx = np.meshgrid(k, i, indexing="ij")
+    x0 = 0.5 * (wx - (cx - 1) * cloth_spacing)
+    z0 = 0.5 * (wz - (cz - 1) * cloth_spacing)
+    pc = np.stack([x0 + xx.ravel() * cloth_spacing, np.full(cx * cz, height),
+                   z0 + zz.ravel() * cloth_spacing], 1).astype(np.float32)
+    pos = np.concatenate([pf, pc])
+    n, nf = pos.shape[0], pf.shape[0]
+    p = pbf_params((0.0, 0.0, 0.0), ext, h, radius, iterations, n, 96, seed=seed)
+    rho0 = lattice_rest_density(h, spacing)
+    phase = np.zeros(n, np.int32)
+    phase[nf:] = capi.PHASE_CLOTH
+    s = Scene(p, pos, np.ones(n, np.float32), phase, np.full(n, np.float32(rho0), np.float32), dt,
+              name="cloth_over_pool_%dx%d_%d" % (cx, cz, nf))
+    s.distance = cloth_constraints(cx, cz, nf, cloth_spacing, stiffness)
+    corners = np.array([nf, nf + cx - 1, nf + (cz - 1) * cx, nf + cz * cx - 1], np.uint32)
+    s.pins = (corners, pos[corners].copy())
     s.rho0 = rho0
     return s
 

@@ -1,0 +1,138 @@
+"""ps_step_backward (the reverse pass on the GPU) against the oracle's adjoint.
+
+The GPU kernels follow the oracle's adjoint operation by operation, so
+FP32 gradients are compared bit for bit with the FP32 oracle; the oracle's
+FP64 adjoint is pinned by central differences (test_oracle_adjoint.py), and
+the FP32 GPU gradients are compared with it under a stated tolerance.
+Checkpoint spacing only changes what is recomputed, never a bit of the
+result, and the handle is left in the state ps_step would leave."""
+import numpy as np
+import pytest
+
+from particlesolver_amd import System, capi, scenes
+from oracle import load_scene
+from test_gpu_parity import assert_bitexact
+
+pytestmark = pytest.mark.gpu
+
+SCENES = {
+    "dam_break": (lambda: scenes.dam_break(5, 4, 5), 3),
+    "rope_over_fluid": (lambda: scenes.rope_over_fluid(), 3),
+    "cloth_and_fluid": (lambda: scenes.cloth_and_fluid(), 4),
+    "fluid2d": (lambda: scenes.two_d_fluid(8, 5), 3),
+    "dam_break_4096": (lambda: scenes.dam_break(16, 16, 16), 5),
+    "cloth_over_pool_32x32_4096": (lambda: scenes.cloth_over_pool((32, 32), (16, 16, 16)), 6),
+}
+
+
+def cotangents(n, seed=5):
+    rng = np.random.default_rng(seed)
+    return (rng.standard_normal((n, 3)).astype(np.float32),
+            rng.standard_normal((n, 3)).astype(np.float32))
+
+
+def n_dist(sc):
+    return 0 if sc.distance is None else sc.distance[0].shape[0]
+
+
+@pytest.mark.parametrize("name", sorted(SCENES))
+def test_matches_fp32_oracle_bitexact(gpu_available, name):
+    make, T = SCENES[name]
+    sc = make()
+    gx, gv = cotangents(sc.n)
+    ref = load_scene(sc, "f32")
+    r = ref.step_backward(sc.dt, T, gx, gv, n_dist(sc), with_mu=True)
+    with System(sc.params) as g:
+        sc.load_into(g)
+        out = g.step_backward(sc.dt, T, gx, gv)
+        for a, b, what in zip(out[:4], r[:4], ("dL/dx0", "dL/dv0", "dL/drho0", "dL/dstiffness")):
+            assert_bitexact(a, b, "%s %s" % (name, what))
+        assert np.float32(out[4]).view(np.uint32) == np.float32(r[4]).view(np.uint32), (out[4], r[4])
+        # the handle is left where the forward pass ended
+        assert_bitexact(g.positions(), ref.positions(), "%s final positions" % name)
+        assert_bitexact(g.velocities(), ref.velocities(), "%s final velocities" % name)
+
+
+def test_checkpoint_spacing_changes_no_bit(gpu_available):
+    sc = scenes.cloth_and_fluid()
+    gx, gv = cotangents(sc.n)
+    outs = []
+    for every in (0, 2, 3, 7):
+        with System(sc.params) as g:
+            sc.load_into(g)
+            outs.append(g.step_backward(sc.dt, 7, gx, gv, checkpoint_every=every))
+    for o in outs[1:]:
+        for a, b in zip(o[:4], outs[0][:4]):
+            assert_bitexact(a, b, "checkpoint spacing")
+        assert o[4] == outs[0][4]
+
+
+def test_leaves_the_state_of_ps_step(gpu_available):
+    sc = scenes.rope_over_fluid()
+    gx, gv = cotangents(sc.n)
+    with System(sc.params) as a, System(sc.params) as b:
+        sc.load_into(a)
+        sc.load_into(b)
+        a.step_backward(sc.dt, 5, gx, gv, checkpoint_every=2)
+        b.step(sc.dt, 5)
+        assert_bitexact(a.positions(), b.positions(), "positions")
+        assert_bitexact(a.velocities(), b.velocities(), "velocities")
+        assert a.status().substeps == b.status().substeps
+        assert a.status().iterations == b.status().iterations
+        a.step(sc.dt, 3)
+        b.step(sc.dt, 3)
+        assert_bitexact(a.positions(), b.positions(), "positions after stepping on")
+        # a second reverse pass on the same handle (workspace reused)
+        ra = a.step_backward(sc.dt, 2, gx, gv)
+        rb = b.step_backward(sc.dt, 2, gx, gv, checkpoint_every=2)
+        for u, v in zip(ra[:4], rb[:4]):
+            assert_bitexact(u, v, "second reverse pass")
+
+
+def test_fp32_gradients_track_the_fp64_adjoint(gpu_available):
+    sc = scenes.cloth_over_pool((32, 32), (16, 16, 16))
+    T = 6
+    gx, gv = cotangents(sc.n)
+    r64 = load_scene(sc, "f64").step_backward(sc.dt, T, gx.astype(np.float64), gv.astype(np.float64),
+                                              n_dist(sc), with_mu=True)
+    with System(sc.params) as g:
+        sc.load_into(g)
+        out = g.step_backward(sc.dt, T, gx, gv, checkpoint_every=3)
+    # stated tolerance: max error relative to the largest entry, 6 substeps x 4 iterations
+    for a, b, what in zip(out[:4], r64[:4], ("x", "v", "rho0", "stiffness")):
+        err = np.abs(a.astype(np.float64) - b).max() / np.abs(b).max()
+        assert err < 1e-3, (what, err)
+
+
+def test_zero_substeps_and_null_cotangents(gpu_available):
+    sc = scenes.dam_break(5, 4, 5)
+    gx, gv = cotangents(sc.n)
+    with System(sc.params) as g:
+        sc.load_into(g)
+        dx, dv, dr, ds, dm = g.step_backward(sc.dt, 0, gx, gv)
+        assert np.array_equal(dx, gx) and np.array_equal(dv, gv)
+        assert not dr.any() and dm == 0.0
+        dx, dv, dr, ds, dm = g.step_backward(sc.dt, 2, None, None)
+        assert not dx.any() and not dv.any() and not dr.any()
+
+
+def test_rejected_scopes(gpu_available):
+    sc = scenes.mixed(cloth_n=4, cubes=(1, 1, 1), fluid=(4, 4, 4), box=(3.0, 3.0, 3.0))
+    with System(sc.params) as g:
+        sc.load_into(g)
+        with pytest.raises(capi.PsError) as e:
+            g.step_backward(sc.dt, 1)
+        assert e.value.code == capi.PS_ERR_UNSUPPORTED
+        g.step(sc.dt, 1)                              # the handle still works
+    sc = scenes.reference_two_fluids()
+    with System(sc.params) as g:
+        sc.load_into(g)
+        with pytest.raises(capi.PsError) as e:
+            g.step_backward(sc.dt, 1)
+        assert e.value.code == capi.PS_ERR_UNSUPPORTED
+    sc = scenes.dam_break(6, 6, 6, max_neighbors=8)   # the table overflows
+    with System(sc.params) as g:
+        sc.load_into(g)
+        with pytest.raises(capi.PsError) as e:
+            g.step_backward(sc.dt, 1)
+        assert e.value.code == capi.PS_ERR_CAPACITY
