@@ -61,11 +61,14 @@ def parse():
     ap.add_argument("--no-timing", action="store_true",
                     help="no per-kernel events in the timed region")
     ap.add_argument("--workload", default="dam_break",
-                    choices=["dam_break", "ensemble", "rayleigh_taylor_2d"],
+                    choices=["dam_break", "ensemble", "rayleigh_taylor_2d", "adjoint_cloth_pool"],
                     help="dam_break: the 1M-particle PBF metric; ensemble: 512 batched "
                          "scenes (one graph launch per 8 substeps) vs 512 per-scene systems; "
                          "rayleigh_taylor_2d: the 1,600-particle 2D scene in the reference's "
-                         "CPU-solver semantics")
+                         "CPU-solver semantics; adjoint_cloth_pool: the reverse pass "
+                         "(ps_step_backward) on 16,384 cloth + 65,536 fluid particles")
+    ap.add_argument("--no-adjoint", action="store_true",
+                    help="dam_break: skip the reverse-pass summary")
     ap.add_argument("--scenes", type=int, default=512)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also run 3 device-to-device copies of MIB MiB (the runtime's copy "
@@ -355,6 +358,133 @@ def rt2d_main(args):
     print(json.dumps(line))
 
 
+# reverse-pass kernels: algorithmic bytes per particle per launch, each
+# array read or written once (neighbour data counted inside the own reads)
+ADJ_ALGO_BYTES = {
+    # dL/dp in 16 + clamp 1 + iterate 16 + attributes 16 + count 4 + K indices
+    # + dL/dy,rho out 16 + S,D out 16 + rho0 term r/w 8 + contact count out 4
+    "adj_pre": lambda K: 97.0 + 4.0 * K,
+    # attributes 16 + iterate 16 + dL/dy,rho 16 + S,D 16 + count 4 + K indices
+    # + rho0 term r/w 8 + dL/dC,dL/dD out 8
+    "adj_lambda": lambda K: 84.0 + 4.0 * K,
+    # iterate 16 + attributes 16 + dL/dy 16 + dL/dC,dD 8 + S,D 16 + count 4
+    # + contact count 4 + K indices + dL/dp out 16
+    "adj_position": lambda K: 96.0 + 4.0 * K,
+}
+
+
+def adjoint_measure(args, cloth=(128, 128), fluid=(64, 16, 64), T=60, every=10, cpu=True):
+    """The adjoint workload: cloth (128x128, corners pinned) over a fluid pool
+    (64x16x64), 60 substeps x 4 iterations, loss L = |x_T - x_T^target|^2
+    with the target from seeded perturbed parameters (stiffness x U[0.9,1.1]
+    per constraint, rho0 x U[0.95,1.05] per particle).  Times the forward
+    pass (ps_step) and the reverse pass (ps_step_backward, checkpoints every
+    `every` substeps, its own forward pass included)."""
+    import numpy as np
+    from particlesolver_amd import System, scenes
+    sc = scenes.cloth_over_pool(cloth, fluid)
+    N, I = sc.n, sc.params.iterations
+    rng = np.random.default_rng(0x5EED0005)
+    tgt = scenes.cloth_over_pool(cloth, fluid)
+    pairs, rest, stiff = tgt.distance
+    tgt.distance = (pairs, rest, (stiff * rng.uniform(0.9, 1.1, stiff.shape)).astype(np.float32))
+    tgt.rest_density = (tgt.rest_density * rng.uniform(0.95, 1.05, N)).astype(np.float32)
+    with System(tgt.params) as g:
+        tgt.load_into(g)
+        g.step(sc.dt, T)
+        x_target = g.positions().astype(np.float64)
+    # warm-up (allocations, code objects) on a throwaway handle
+    with System(sc.params) as g:
+        sc.load_into(g)
+        g.step(sc.dt, 2)
+        g.step_backward(sc.dt, 2, np.zeros((N, 3), np.float32), None, checkpoint_every=every)
+    with System(sc.params) as g:
+        sc.load_into(g)
+        g.synchronize()
+        t0 = time.perf_counter()
+        g.step(sc.dt, T)
+        g.synchronize()
+        t_fwd = time.perf_counter() - t0
+        x_T = g.positions().astype(np.float64)
+        K = float(g.neighbors()[0].mean())
+        flags = g.status().flags
+    loss = float(((x_T - x_target) ** 2).sum())
+    gx = (2.0 * (x_T - x_target)).astype(np.float32)
+    with System(sc.params) as g:
+        sc.load_into(g)
+        g.synchronize()
+        t0 = time.perf_counter()
+        dx, dv, dr, ds, dm = g.step_backward(sc.dt, T, gx, None, checkpoint_every=every)
+        t_bwd = time.perf_counter() - t0
+    # per-kernel breakdown, a separate run with events on every reverse kernel
+    kernels = {}
+    with System(sc.params) as g:
+        sc.load_into(g)
+        g.timing([k for k in ("lambda", "solve") + tuple(ADJ_ALGO_BYTES) +
+                  ("adj_xsph", "adj_finalize", "adj_stiffness", "adj_predict")])
+        g.timing_reset()
+        g.step_backward(sc.dt, T, gx, None, checkpoint_every=every)
+        for name, (ms, launches) in g.timing_read().items():
+            if launches:
+                kernels[name] = {"ms_total": round(ms, 3), "launches": launches,
+                                 "us_per_launch": round(1000.0 * ms / launches, 2)}
+    roof = None
+    timed = {k: v for k, v in kernels.items() if k in ADJ_ALGO_BYTES}
+    if timed:
+        dom = max(timed, key=lambda k: timed[k]["ms_total"])
+        b = ADJ_ALGO_BYTES[dom](K) * N
+        ach = b / (timed[dom]["us_per_launch"] * 1e-6) / 1e9
+        roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                "algo_bytes_per_launch": int(b), "K": round(K, 2)}
+        for k in timed:
+            kernels[k]["algo_GBs"] = round(ADJ_ALGO_BYTES[k](K) * N / (kernels[k]["us_per_launch"] * 1e-6)
+                                           / 1e9, 1)
+    cpu_line = None
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        from oracle import load_scene
+        o = load_scene(sc, "f32")
+        Tc = 2
+        t0 = time.perf_counter()
+        o.step_backward(sc.dt, Tc, gx, None, pairs.shape[0])
+        tc = time.perf_counter() - t0
+        cpu_line = {"value": round(N * I * Tc / tc, 1), "unit": "particle-iterations/s (reverse pass)",
+                    "cores": os.cpu_count(), "kind": "port",
+                    "sample": "FP32 oracle orc_step_backward, the first %d substeps of the same scene "
+                              "(its forward passes use OpenMP, the adjoint loops one thread), %.2f s"
+                              % (Tc, tc)}
+    return {
+        "workload": "cloth %dx%d (corners pinned) over a %dx%dx%d fluid pool = %d particles, "
+                    "%d substeps x %d iterations, checkpoints every %d substeps"
+                    % (cloth[0], cloth[1], fluid[0], fluid[1], fluid[2], N, T, I, every),
+        "particles": N, "substeps": T, "iterations": I, "checkpoint_every": every,
+        "forward_ms": round(1e3 * t_fwd, 2), "backward_ms": round(1e3 * t_bwd, 2),
+        "backward_over_forward": round(t_bwd / t_fwd, 3),
+        "value": round(N * I * T / t_bwd, 1), "mean_neighbors": round(K, 2), "status_flags": flags,
+        "loss": loss,
+        "grad_norms": {"x0": float(np.linalg.norm(dx)), "v0": float(np.linalg.norm(dv)),
+                       "rho0": float(np.linalg.norm(dr)), "stiffness": float(np.linalg.norm(ds)),
+                       "wall_mu_k": dm},
+        "roofline": roof, "kernels": kernels, "cpu_baseline": cpu_line,
+    }
+
+
+def adjoint_main(args):
+    r = adjoint_measure(args, cpu=not args.no_cpu)
+    line = {
+        "metric": "reverse-pass particle-iterations/s (ps_step_backward incl. its forward pass)",
+        "value": r.pop("value"), "unit": "particle-iterations/s", "n_gpus": 1,
+        "steps": r["substeps"], "warmup": 2,
+        "ms_per_step": round(r["backward_ms"] / r["substeps"], 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded lattice jitter, seeded parameter perturbation for the target)",
+        "config": {k: v for k, v in r.items() if k not in ("roofline", "cpu_baseline", "kernels")},
+        "roofline": r["roofline"], "cpu_baseline": r["cpu_baseline"], "kernels": r["kernels"],
+    }
+    print(json.dumps(line))
+
+
 def main():
     args = parse()
     if args.traffic_report:
@@ -367,6 +497,8 @@ def main():
         return ensemble_main(args)
     if args.workload == "rayleigh_taylor_2d":
         return rt2d_main(args)
+    if args.workload == "adjoint_cloth_pool":
+        return adjoint_main(args)
     from particlesolver_amd import System, scenes
     import numpy as np
 
@@ -468,6 +600,10 @@ def main():
         "kernels": kernels,
         "breakdown_us_per_launch_all_events": breakdown,
     }
+    if not args.no_adjoint:
+        r = adjoint_measure(args, cpu=False)
+        line["adjoint"] = {k: r[k] for k in ("workload", "forward_ms", "backward_ms",
+                                             "backward_over_forward", "value", "roofline")}
     print(json.dumps(line))
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)

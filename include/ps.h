@@ -361,6 +361,14 @@ typedef enum ps_kernel_id {
     PS_K_SOLVE,
     PS_K_SHAPE,
     PS_K_FINALIZE,
+    /* reverse pass (ps_step_backward) */
+    PS_K_ADJ_XSPH,
+    PS_K_ADJ_FINALIZE,
+    PS_K_ADJ_PRE,
+    PS_K_ADJ_LAMBDA,
+    PS_K_ADJ_POSITION,
+    PS_K_ADJ_STIFFNESS,
+    PS_K_ADJ_PREDICT,
     PS_K_COUNT
 } ps_kernel_id;
 

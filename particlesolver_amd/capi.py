@@ -41,7 +41,8 @@ PS_STATUS_OUTSIDE_GRID = 0x4
 PS_STATUS_INTERNAL = 0x80000000
 
 KERNELS = ["predict_hash", "scan", "scatter", "reorder", "neighbors", "lambda", "solve",
-           "shape", "finalize"]
+           "shape", "finalize", "adj_xsph", "adj_finalize", "adj_pre", "adj_lambda",
+           "adj_position", "adj_stiffness", "adj_predict"]
 K_PREDICT_HASH, K_SCAN, K_SCATTER, K_REORDER, K_NEIGHBORS, K_LAMBDA, K_SOLVE, K_SHAPE, \
     K_FINALIZE = range(9)
 

@@ -567,32 +567,21 @@ __global__ __launch_bounds__(kAdjBlock) void adj_predict_k(DevBuffers b, KConst 
 
 static inline uint32_t adj_blocks(uint32_t n) { return (n + kAdjBlock - 1) / kAdjBlock; }
 
-hipError_t launch_adj_finalize(hipStream_t st, const DevBuffers &b, const KConst &k, const AdjBuffers &a,
-                               uint32_t n, float dt)
+hipError_t launch_adj(hipStream_t st, AdjKernel which, const DevBuffers &b, const KConst &k,
+                      const AdjBuffers &a, uint32_t count, float dt)
 {
-    if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(adj_xsph_k, dim3(adj_blocks(n)), dim3(kAdjBlock), 0, st, b, k, a, n, dt);
-    hipLaunchKernelGGL(adj_finalize_k, dim3(adj_blocks(n)), dim3(kAdjBlock), 0, st, b, k, a, n, dt);
-    return hipGetLastError();
-}
-
-hipError_t launch_adj_iteration(hipStream_t st, const DevBuffers &b, const KConst &k, const AdjBuffers &a,
-                                uint32_t n, uint32_t ndist)
-{
-    if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(adj_pre_k, dim3(adj_blocks(n)), dim3(kAdjBlock), 0, st, b, k, a, n);
-    hipLaunchKernelGGL(adj_lambda_k, dim3(adj_blocks(n)), dim3(kAdjBlock), 0, st, b, k, a, n);
-    hipLaunchKernelGGL(adj_position_k, dim3(adj_blocks(n)), dim3(kAdjBlock), 0, st, b, k, a, n);
-    if (ndist)
-        hipLaunchKernelGGL(adj_stiffness_k, dim3(adj_blocks(ndist)), dim3(kAdjBlock), 0, st, b, k, a, ndist);
-    return hipGetLastError();
-}
-
-hipError_t launch_adj_predict(hipStream_t st, const DevBuffers &b, const KConst &k, const AdjBuffers &a,
-                              uint32_t n, float dt)
-{
-    if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(adj_predict_k, dim3(adj_blocks(n)), dim3(kAdjBlock), 0, st, b, k, a, n, dt);
+    if (count == 0) return hipSuccess;
+    const dim3 grid(adj_blocks(count)), block(kAdjBlock);
+    switch (which) {
+    case ADJ_XSPH: hipLaunchKernelGGL(adj_xsph_k, grid, block, 0, st, b, k, a, count, dt); break;
+    case ADJ_FINALIZE: hipLaunchKernelGGL(adj_finalize_k, grid, block, 0, st, b, k, a, count, dt); break;
+    case ADJ_PRE: hipLaunchKernelGGL(adj_pre_k, grid, block, 0, st, b, k, a, count); break;
+    case ADJ_LAMBDA: hipLaunchKernelGGL(adj_lambda_k, grid, block, 0, st, b, k, a, count); break;
+    case ADJ_POSITION: hipLaunchKernelGGL(adj_position_k, grid, block, 0, st, b, k, a, count); break;
+    case ADJ_STIFFNESS: hipLaunchKernelGGL(adj_stiffness_k, grid, block, 0, st, b, k, a, count); break;
+    case ADJ_PREDICT: hipLaunchKernelGGL(adj_predict_k, grid, block, 0, st, b, k, a, count, dt); break;
+    default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

@@ -76,7 +76,8 @@ struct ps_system {
     bool cr_bnd_done = false;  // boundaries of iteration it_sub already projected
     bool cr_swapped = false;   // iterate lives in b.Q (odd number of object passes)
     // workgroup size of each kernel (ps_autotune)
-    uint32_t block[PS_K_COUNT] = {256, 256, 256, 256, 256, 256, 256, 256, 256};
+    uint32_t block[PS_K_COUNT] = {256, 256, 256, 256, 256, 256, 256, 256, 256,
+                                  256, 256, 256, 256, 256, 256, 256};  // reverse pass: fixed 256
     // timing
     uint32_t timing = 0;  // kernel mask
     std::vector<TimedLaunch> pending;
@@ -1493,20 +1494,25 @@ int adj_reverse(ps_system *s, float dt)
     a.dist = s->dist_rest.empty() ? 0u : 1u;
     float4 *cur = w.pb, *nxt = w.pb2;
     a.pb = cur;
-    hipError_t e = ps::launch_adj_finalize(s->stream, s->b, s->kc, a, s->n, dt);
-    const uint32_t ndist = (uint32_t)s->dist_rest.size();
-    for (uint32_t it = s->prm.iterations; e == hipSuccess && it-- > 0;) {
+    const uint32_t n = s->n, ndist = (uint32_t)s->dist_rest.size();
+    LAUNCH(s, PS_K_ADJ_XSPH, ps::launch_adj(s->stream, ps::ADJ_XSPH, s->b, s->kc, a, n, dt));
+    LAUNCH(s, PS_K_ADJ_FINALIZE, ps::launch_adj(s->stream, ps::ADJ_FINALIZE, s->b, s->kc, a, n, dt));
+    for (uint32_t it = s->prm.iterations; it-- > 0;) {
         a.Pit = w.Pit + it * w.cap;
         a.clamp = w.clamp + it * w.cap;
         a.wrec = w.wrec + it * w.cap;
         a.pbn = cur;
         a.pb = nxt;
-        e = ps::launch_adj_iteration(s->stream, s->b, s->kc, a, s->n, ndist);
+        LAUNCH(s, PS_K_ADJ_PRE, ps::launch_adj(s->stream, ps::ADJ_PRE, s->b, s->kc, a, n, dt));
+        LAUNCH(s, PS_K_ADJ_LAMBDA, ps::launch_adj(s->stream, ps::ADJ_LAMBDA, s->b, s->kc, a, n, dt));
+        LAUNCH(s, PS_K_ADJ_POSITION, ps::launch_adj(s->stream, ps::ADJ_POSITION, s->b, s->kc, a, n, dt));
+        if (ndist)
+            LAUNCH(s, PS_K_ADJ_STIFFNESS,
+                   ps::launch_adj(s->stream, ps::ADJ_STIFFNESS, s->b, s->kc, a, ndist, dt));
         std::swap(cur, nxt);
     }
     a.pb = cur;
-    if (e == hipSuccess) e = ps::launch_adj_predict(s->stream, s->b, s->kc, a, s->n, dt);
-    if (e != hipSuccess) return fail(s, PS_ERR_DEVICE, std::string("launch adjoint: ") + hipGetErrorString(e));
+    LAUNCH(s, PS_K_ADJ_PREDICT, ps::launch_adj(s->stream, ps::ADJ_PREDICT, s->b, s->kc, a, n, dt));
     s->stage = 0;
     s->it_sub = 0;
     s->hashed = false;
@@ -2035,8 +2041,10 @@ int ps_set_tuning(ps_system *s, const uint32_t *blocks)
 
 const char *ps_kernel_name(int kernel_id)
 {
-    static const char *names[PS_K_COUNT] = {"predict_hash", "scan", "scatter", "reorder",
-                                            "neighbors", "lambda", "solve", "shape", "finalize"};
+    static const char *names[PS_K_COUNT] = {"predict_hash", "scan", "scatter", "reorder", "neighbors",
+                                            "lambda", "solve", "shape", "finalize", "adj_xsph",
+                                            "adj_finalize", "adj_pre", "adj_lambda", "adj_position",
+                                            "adj_stiffness", "adj_predict"};
     if (kernel_id < 0 || kernel_id >= PS_K_COUNT) return "unknown";
     return names[kernel_id];
 }

@@ -199,15 +199,23 @@ hipError_t launch_finalize(hipStream_t st, const DevBuffers &b, const KConst &k,
 
 uint32_t scan_tiles(uint32_t num_cells);
 
-// Reverse pass (ps_adjoint.hip): velocity + XSPH backwards (b.P the final
-// iterate, b.X the substep start), one iteration backwards (a.Pit, a.clamp,
-// a.wrec of that iteration; a.pbn -> a.pb), predict backwards.
-hipError_t launch_adj_finalize(hipStream_t st, const DevBuffers &b, const KConst &k, const AdjBuffers &a,
-                               uint32_t n, float dt);
-hipError_t launch_adj_iteration(hipStream_t st, const DevBuffers &b, const KConst &k, const AdjBuffers &a,
-                                uint32_t n, uint32_t ndist);
-hipError_t launch_adj_predict(hipStream_t st, const DevBuffers &b, const KConst &k, const AdjBuffers &a,
-                              uint32_t n, float dt);
+// Reverse pass (ps_adjoint.hip), one kernel per call: velocity + XSPH
+// backwards (ADJ_XSPH then ADJ_FINALIZE; b.P the final iterate, b.X the
+// substep start), one iteration backwards (ADJ_PRE, ADJ_LAMBDA,
+// ADJ_POSITION, then ADJ_STIFFNESS over count = the distance constraints;
+// a.Pit, a.clamp, a.wrec of that iteration, a.pbn -> a.pb), predict
+// backwards (ADJ_PREDICT).  count: particles, or constraints.
+enum AdjKernel : uint32_t {
+    ADJ_XSPH,
+    ADJ_FINALIZE,
+    ADJ_PRE,
+    ADJ_LAMBDA,
+    ADJ_POSITION,
+    ADJ_STIFFNESS,
+    ADJ_PREDICT,
+};
+hipError_t launch_adj(hipStream_t st, AdjKernel which, const DevBuffers &b, const KConst &k,
+                      const AdjBuffers &a, uint32_t count, float dt);
 
 // CPU-reference mode (ps_cpuref.hip); storage in insertion order, iterate
 // ping-pong b.P -> b.Q per fluid object.
