@@ -298,24 +298,28 @@ int ps_step_graph(ps_system *sys, float dt, uint32_t substeps);
 
 /* Gradient of a loss L(x_T, v_T) of the state T = `substeps` substeps ahead
  * (the reference has no gradient path: this is the reverse of ps_step,
- * ParticleSystem::update, gpu/src/particlesystem.cpp:144-246).  Runs the substeps from
- * the current state, keeping (x, v) every `checkpoint_every` substeps (0 or
- * 1: every substep), then walks them backwards: each segment is recomputed
- * from its checkpoint and each substep once more, recording its iterates,
- * the coordinates walls and pins set and the wall-friction branches; the
- * neighbour tables, contact sets and branches are held fixed, so the result
- * is the exact gradient of the smooth map they select.  Inputs, host
- * arrays in insertion order, NULL meaning zero: dl_dx, dl_dv (3n).  Outputs
- * (any may be NULL): dl_dx0, dl_dv0 (3n), dl_drho0 (n: per-particle rest
- * density), dl_dstiffness (one per distance constraint, in the order they
- * were added), dl_dmu_k (1: wall_mu_k).  Leaves the system in the state
- * after the substeps, as ps_step would.  Covers fluid, gas and cloth
- * particles, distance constraints, pins, walls with friction and XSPH;
- * PS_ERR_UNSUPPORTED for solid or rigid particles, rigid bodies, sleeping,
- * PS_FLAG_REBUILD_EVERY_ITERATION, batches and the CPU-reference mode;
- * PS_ERR_CAPACITY if a neighbour table overflowed (the transposed sums
- * need every pair).  Device memory: 52 bytes per particle per checkpoint
- * plus 33 per particle per iteration, kept by the handle.  Synchronous. */
+ * ParticleSystem::update, gpu/src/particlesystem.cpp:144-246).  Runs the
+ * substeps from the current state, then walks them backwards.  Each substep
+ * is reversed from a tape of its forward pass: its neighbour table, sorted
+ * start state, every iterate (with lambda), the coordinates walls and pins
+ * set and the wall-friction branches; the tables, contact sets and branches
+ * are held fixed, so the result is the exact gradient of the smooth map they
+ * select.  checkpoint_every = 0 (or >= substeps) tapes the whole horizon
+ * during the forward pass; k > 0 keeps (x, v) every k substeps and one
+ * segment of k substeps on tape: the last segment is taped by the forward
+ * pass, each earlier one is recomputed from its checkpoint when the reverse
+ * pass reaches it.  Inputs, host arrays in insertion order, NULL meaning
+ * zero: dl_dx, dl_dv (3n).  Outputs (any may be NULL): dl_dx0, dl_dv0 (3n),
+ * dl_drho0 (n: per-particle rest density), dl_dstiffness (one per distance
+ * constraint, in the order they were added), dl_dmu_k (1: wall_mu_k).
+ * Leaves the system in the state after the substeps, as ps_step would.
+ * Covers fluid, gas and cloth particles, distance constraints, pins, walls
+ * with friction and XSPH; PS_ERR_UNSUPPORTED for solid or rigid particles,
+ * rigid bodies, sleeping, PS_FLAG_REBUILD_EVERY_ITERATION, batches and the
+ * CPU-reference mode; PS_ERR_CAPACITY if a neighbour table overflowed (the
+ * transposed sums need every pair).  Device memory, kept by the handle:
+ * (4 max_neighbors + 56 + 33 iterations) bytes per particle per taped
+ * substep, plus 52 per particle per checkpoint.  Synchronous. */
 int ps_step_backward(ps_system *sys, float dt, uint32_t substeps, uint32_t checkpoint_every,
                      const float *dl_dx, const float *dl_dv, float *dl_dx0, float *dl_dv0,
                      float *dl_drho0, float *dl_dstiffness, float *dl_dmu_k);

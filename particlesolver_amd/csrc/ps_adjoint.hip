@@ -110,31 +110,23 @@ __device__ __forceinline__ void adj_dD(const KConst &k, const float *dv, float r
     for (int a = 0; a < 3; a++) out[a] = (c1 * dv[a]) + (2.0f * ((aa * S[a]) + (((ap * ri) * ds) * dv[a])));
 }
 
-// density multiplier of neighbour j in rho_i
-__device__ __forceinline__ float adj_mass(const KConst &k, const Attr &aj)
-{
-    float m = 1.0f / aj.w;
-    if (aj.phase != PS_PHASE_FLUID) m = m * k.s_solid;
-    return m;
-}
 
 // Cloth contact of owner i with j (collideD without friction: a distance
 // constraint of rest collide_dist, stiffness 1, count nc): M^T y for
 // M = -(w_i / (wsum nc)) [(1 - cd/r) I + (cd/r^3) d d^T]; false when the
 // pair is not in i's contact set.
-__device__ __forceinline__ bool adj_contact(const KConst &k, const Attr &ai, const Attr &aj, uint32_t nci,
-                                            const float *dv, const float *y, float *out)
+__device__ __forceinline__ bool adj_contact(const KConst &k, float wi, int32_t ph, float wj, int32_t phj,
+                                            uint32_t nci, const float *dv, const float *y, float *out)
 {
-    const int32_t ph = ai.phase;
-    if (ph < PS_PHASE_CLOTH || ai.w == 0.0f || nci == 0u) return false;
-    if (ph > PS_PHASE_SOLID && ph == aj.phase) return false;
+    if (ph < PS_PHASE_CLOTH || wi == 0.0f || nci == 0u) return false;
+    if (ph > PS_PHASE_SOLID && ph == phj) return false;
     const float r2 = ((dv[0] * dv[0]) + (dv[1] * dv[1])) + (dv[2] * dv[2]);
     if (!(r2 < k.collide_r2)) return false;
     const float dist = sqrtf(r2);
     if (dist < 1e-6f) return false;
-    const float wsum = ai.w + aj.w;
+    const float wsum = wi + wj;
     if (wsum == 0.0f) return false;
-    const float cf = -((ai.w / wsum) / (float)nci);
+    const float cf = -((wi / wsum) / (float)nci);
     const float k1 = 1.0f - (k.collide_dist / dist), k2 = k.collide_dist / ((dist * dist) * dist);
     const float dy = dot3(dv, y);
     for (int a = 0; a < 3; a++) out[a] = cf * ((k1 * y[a]) + ((k2 * dy) * dv[a]));
@@ -156,6 +148,111 @@ __device__ __forceinline__ void yd_of(const KConst &k, const float4 &Y, float *y
     yd[2] = k.dims == 2u ? 0.0f : Y.z;
 }
 
+// Quantities of one pair shared by every term of both owners, evaluated
+// once (each is the value the term-by-term helpers above compute).
+struct PairQ {
+    float t, tt, s, cc, c6;  // h^2 - r^2, its square, tensile term, d s/dd and dW/dd per unit d
+    bool deg;
+    float ri, r, q, qq, bb, bp;  // 1/r, r, h - r, (h - r)^2, b, b' (non-degenerate pairs)
+};
+
+__device__ __forceinline__ PairQ pair_q(const KConst &k, float r2)
+{
+    PairQ Q;
+    Q.t = k.h2 - r2;
+    Q.tt = Q.t * Q.t;
+    const float W = k.poly6 * (Q.tt * Q.t);
+    const float x = W * k.inv_w_dq;
+    const uint32_t n = k.n_corr;
+    Q.s = -(k.k_corr * ipow(x, n));
+    const float xn1 = n > 1u ? ipow(x, n - 1u) : 1.0f;
+    Q.cc = (((k.k_corr * (float)n) * xn1) * k.inv_w_dq) * ((6.0f * k.poly6) * Q.tt);
+    Q.c6 = -((6.0f * k.poly6) * Q.tt);
+    Q.deg = r2 < kDegenerateR2;
+    Q.ri = rsq(r2);
+    Q.r = r2 * Q.ri;
+    Q.q = k.h - Q.r;
+    Q.qq = Q.q * Q.q;
+    Q.bb = (k.spiky * Q.qq) * Q.ri;
+    Q.bp = -(((k.spiky * Q.q) * (k.h + Q.r)) * (Q.ri * Q.ri));
+    return Q;
+}
+
+// adj_pair from shared quantities (coef = (l_i + l_j) + s)
+__device__ __forceinline__ void pair_E(const KConst &k, const PairQ &Q, const float *dv, float coef,
+                                       const float *mu, float *E)
+{
+    float g[3];
+    if (Q.deg) {
+        g[0] = 0.0f;
+        g[1] = -((k.spiky * (k.h * k.h)) * 0.001f);
+        g[2] = 0.0f;
+    } else {
+        for (int a = 0; a < 3; a++) g[a] = -(Q.bb * dv[a]);
+    }
+    const float mg = dot3(mu, g);
+    for (int a = 0; a < 3; a++) E[a] = mg * (Q.cc * dv[a]);
+    if (Q.deg) return;
+    const float c2 = (Q.bp * Q.ri) * dot3(dv, mu);
+    for (int a = 0; a < 3; a++) E[a] = E[a] - (coef * ((Q.bb * mu[a]) + (c2 * dv[a])));
+}
+
+// adj_dD from shared quantities (non-degenerate pair)
+__device__ __forceinline__ void pair_dD(const KConst &k, const PairQ &Q, const float *dv, float sk, const float *S,
+                                        float *out)
+{
+    const float aa = (sk * Q.qq) * Q.ri;
+    const float ap = -(((sk * Q.q) * (k.h + Q.r)) * (Q.ri * Q.ri));
+    const float ds = dot3(dv, S);
+    const float c1 = -((((4.0f * sk) * sk) * (Q.qq * Q.q)) * Q.ri);
+    for (int a = 0; a < 3; a++) out[a] = (c1 * dv[a]) + (2.0f * ((aa * S[a]) + (((ap * Q.ri) * ds) * dv[a])));
+}
+
+// density multiplier of a neighbour in rho_i
+__device__ __forceinline__ float adj_mass(const KConst &k, float wj, int32_t phj)
+{
+    float m = 1.0f / wj;
+    if (phj != PS_PHASE_FLUID) m = m * k.s_solid;
+    return m;
+}
+
+// Neighbour records of the gathers.  UNIFORM (every particle fluid, one
+// inverse mass) variants leave out the attribute loads.
+struct GPit {
+    float4 p;
+};
+struct GPitAttr {
+    float4 p;
+    AdjAttr a;
+};
+struct GLam {
+    float4 p, m;
+};
+struct GLamAttr {
+    float4 p, m;
+    AdjAttr a;
+};
+struct GPos {
+    float4 p, m, sd;
+    float2 cd;
+};
+struct GPosAttr {
+    float4 p, m, sd;
+    float2 cd;
+    AdjAttr a;
+};
+struct GXA {
+    float4 p, x;
+    Attr a;
+};
+struct GXY {
+    float4 p, x, y;
+};
+struct GXYA {
+    float4 p, x, y;
+    Attr a;
+};
+
 // ---------------------------------------------------------------------------
 // Pins, walls and wall friction backwards, contact counts, and the forward
 // quantities of the lambda pass (rho, S, D) and of the fluid update (f).
@@ -163,7 +260,9 @@ __device__ __forceinline__ void yd_of(const KConst &k, const float4 &Y, float *y
 // static, or kinetic clamped to f = 1: dp''/dp' = n n^T, dp''/dx* = I - n n^T;
 // kinetic with f = wall_k / l < 1: dp''/dp' = I - wall_k K (I - n n^T),
 // K = (I - t^ t^T) / l, the x* part its negative second term, and
-// dp''/dwall_k = -t / l.
+// dp''/dwall_k = -t / l.  Writes M = (mu = yd / rho0 for fluid, yd
+// otherwise; 1 / rho0) and the attribute record the later passes gather.
+template <bool UNIFORM>
 __global__ __launch_bounds__(kAdjBlock) void adj_pre_k(DevBuffers b, KConst k, AdjBuffers a, uint32_t n)
 {
     const uint32_t i = blockIdx.x * kAdjBlock + threadIdx.x;
@@ -210,102 +309,115 @@ __global__ __launch_bounds__(kAdjBlock) void adj_pre_k(DevBuffers b, KConst k, A
     for (int q = 0; q < 3; q++)
         if (cl & (1u << q)) g[q] = 0.0f;
     const float yd[3] = {g[0], g[1], k.dims == 2u ? 0.0f : g[2]};
+    const bool fl = UNIFORM || ai.phase == PS_PHASE_FLUID;
+    const bool cloth = !UNIFORM && ai.phase >= PS_PHASE_CLOTH && ai.w != 0.0f;
     const uint32_t c = b.ncnt[i];
-    // contact count (collideD's first loop)
     uint32_t nc = 0;
-    if (ai.phase >= PS_PHASE_CLOTH && ai.w != 0.0f) {
-        for (uint32_t q = 0; q < c; q++) {
-            const uint32_t j = nbr_at(b, i, q);
-            if (ai.phase > PS_PHASE_SOLID && ai.phase == b.A[j].phase) continue;
-            const float4 pj = a.Pit[j];
-            const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-            const float r2 = ((dx * dx) + (dy * dy)) + (dz * dz);
-            if (r2 < k.collide_r2) nc++;
+    const float r0 = ai.rho0;
+    const float ir0 = 1.0f / r0;
+    const float sk = k.spiky * (1.0f / r0);
+    float rr = k.w_zero * (1.0f / ai.w);
+    float gix = 0.0f, giy = 0.0f, giz = 0.0f, den = 0.0f;
+    float f[3] = {0.0f, 0.0f, 0.0f};
+    auto body = [&](const float4 &pj, float wj, int32_t phj) {
+        const float dv[3] = {pi.x - pj.x, pi.y - pj.y, pi.z - pj.z};
+        if (cloth && !(ai.phase > PS_PHASE_SOLID && ai.phase == phj)) {
+            // contact count (collideD's first loop)
+            const float r2c = ((dv[0] * dv[0]) + (dv[1] * dv[1])) + (dv[2] * dv[2]);
+            if (r2c < k.collide_r2) nc++;
         }
+        if (!fl || wj == 0.0f) return;
+        const float r2 = dist2(dv[0], dv[1], dv[2]);
+        if (!(r2 < k.h2)) return;
+        const float t = k.h2 - r2;
+        const float W = k.poly6 * ((t * t) * t);
+        rr = __builtin_fmaf(W, adj_mass(k, wj, phj), rr);
+        const PairQ Q = pair_q(k, r2);
+        const float coef = (pi.w + pj.w) + Q.s;
+        if (Q.deg) {
+            f[1] = f[1] + (coef * -((k.spiky * (k.h * k.h)) * 0.001f));
+            f[0] = f[0] + (coef * 0.0f);
+            f[2] = f[2] + (coef * 0.0f);
+            return;
+        }
+        for (int q2 = 0; q2 < 3; q2++) f[q2] = f[q2] + (coef * -(Q.bb * dv[q2]));
+        const float ri = Q.ri;
+        const float qq = Q.q;
+        const float aa = (sk * (qq * qq)) * ri;
+        den = __builtin_fmaf(aa * aa, r2, den);
+        gix = __builtin_fmaf(aa, dv[0], gix);
+        giy = __builtin_fmaf(aa, dv[1], giy);
+        giz = __builtin_fmaf(aa, dv[2], giz);
+    };
+    if (UNIFORM) {
+        for_neighbors(
+            b, i, c, [&](uint32_t j) { return GPit{a.Pit[j]}; },
+            [&](uint32_t, const GPit &d) { body(d.p, ai.w, PS_PHASE_FLUID); });
+    } else {
+        for_neighbors(
+            b, i, c, [&](uint32_t j) { return GPitAttr{a.Pit[j], AdjAttr{b.A[j].w, b.A[j].phase, 0u, 0.0f}}; },
+            [&](uint32_t, const GPitAttr &d) { body(d.p, d.a.w, d.a.phase); });
     }
-    a.nc[i] = nc;
     float rho = 0.0f, D = 0.0f, S[3] = {0.0f, 0.0f, 0.0f};
-    if (ai.phase == PS_PHASE_FLUID) {
-        const float r0 = ai.rho0;
-        const float sk = k.spiky * (1.0f / r0);
-        float rr = k.w_zero * (1.0f / ai.w);
-        float gix = 0.0f, giy = 0.0f, giz = 0.0f, den = 0.0f;
-        float f[3] = {0.0f, 0.0f, 0.0f};
-        for (uint32_t q = 0; q < c; q++) {
-            const uint32_t j = nbr_at(b, i, q);
-            const Attr aj = b.A[j];
-            if (aj.w == 0.0f) continue;
-            const float4 pj = a.Pit[j];
-            const float dv[3] = {pi.x - pj.x, pi.y - pj.y, pi.z - pj.z};
-            const float r2 = dist2(dv[0], dv[1], dv[2]);
-            if (!(r2 < k.h2)) continue;
-            const float t = k.h2 - r2;
-            const float W = k.poly6 * ((t * t) * t);
-            rr = __builtin_fmaf(W, adj_mass(k, aj), rr);
-            float gc[3], gg[3];
-            const float coef = (pi.w + pj.w) + adj_corr(k, dv, r2, gc);
-            adj_gamma(k, dv, r2, gg);
-            for (int q2 = 0; q2 < 3; q2++) f[q2] = f[q2] + (coef * gg[q2]);
-            if (r2 < kDegenerateR2) continue;
-            const float ri = rsq(r2);
-            const float qq = k.h - (r2 * ri);
-            const float aa = (sk * (qq * qq)) * ri;
-            den = __builtin_fmaf(aa * aa, r2, den);
-            gix = __builtin_fmaf(aa, dv[0], gix);
-            giy = __builtin_fmaf(aa, dv[1], giy);
-            giz = __builtin_fmaf(aa, dv[2], giz);
-        }
+    if (fl) {
         den = den + dist2(gix, giy, giz);
         rho = rr;
         D = den;
         S[0] = gix;
         S[1] = giy;
         S[2] = giz;
-        const float ir0 = 1.0f / r0;
         a.r0b[i] = a.r0b[i] - (dot3(yd, f) * (ir0 * ir0));
     }
     a.Y[i] = make_float4(g[0], g[1], g[2], rho);
     a.SD[i] = make_float4(S[0], S[1], S[2], D);
+    a.M[i] = fl ? make_float4(yd[0] * ir0, yd[1] * ir0, yd[2] * ir0, ir0) : make_float4(yd[0], yd[1], yd[2], ir0);
+    if (!UNIFORM) {
+        a.nc[i] = nc;
+        a.AB[i] = AdjAttr{ai.w, ai.phase, nc, 0.0f};
+    }
 }
 
 // dL/dl -> dL/dC = -lb / (D + eps), dL/dD = -lb l / (D + eps); rho0 terms
+template <bool UNIFORM>
 __global__ __launch_bounds__(kAdjBlock) void adj_lambda_k(DevBuffers b, KConst k, AdjBuffers a, uint32_t n)
 {
     const uint32_t i = blockIdx.x * kAdjBlock + threadIdx.x;
     if (i >= n) return;
-    const Attr ai = b.A[i];
-    if (ai.phase != PS_PHASE_FLUID) {
+    if (!UNIFORM && b.A[i].phase != PS_PHASE_FLUID) {
         a.CD[i] = make_float2(0.0f, 0.0f);
         return;
     }
     const float4 pi = a.Pit[i];
-    const float4 Yi = a.Y[i];
-    const float ir0 = 1.0f / ai.rho0;
-    float yd[3], mu[3];
-    yd_of(k, Yi, yd);
-    for (int q = 0; q < 3; q++) mu[q] = yd[q] * ir0;
+    const float4 mi = a.M[i];
+    const float mu[3] = {mi.x, mi.y, mi.z};
+    const float ir0 = mi.w;
     float lb = 0.0f;
-    const uint32_t c = b.ncnt[i];
-    for (uint32_t q = 0; q < c; q++) {
-        const uint32_t j = nbr_at(b, i, q);
-        const Attr aj = b.A[j];
-        if (aj.w == 0.0f) continue;
-        const float4 pj = a.Pit[j];
+    auto body = [&](const float4 &pj, const float4 &mj, float wj, int32_t phj) {
+        if (wj == 0.0f) return;
         const float dv[3] = {pi.x - pj.x, pi.y - pj.y, pi.z - pj.z};
         const float r2 = dist2(dv[0], dv[1], dv[2]);
-        if (!(r2 < k.h2)) continue;
+        if (!(r2 < k.h2)) return;
         float g[3];
         adj_gamma(k, dv, r2, g);
         lb = lb + dot3(mu, g);
-        if (aj.phase == PS_PHASE_FLUID) {
-            const float dj[3] = {-dv[0], -dv[1], -dv[2]};
-            adj_gamma(k, dj, r2, g);
-            float ydj[3], muj[3];
-            yd_of(k, a.Y[j], ydj);
-            const float irj = 1.0f / aj.rho0;
-            for (int q2 = 0; q2 < 3; q2++) muj[q2] = ydj[q2] * irj;
-            lb = lb + dot3(muj, g);
+        if (phj == PS_PHASE_FLUID) {
+            // G(-d) = -G(d) exactly (rounding is symmetric); the degenerate
+            // direction is the same for both
+            const bool deg = r2 < kDegenerateR2;
+            const float gj[3] = {deg ? g[0] : -g[0], deg ? g[1] : -g[1], deg ? g[2] : -g[2]};
+            const float muj[3] = {mj.x, mj.y, mj.z};
+            lb = lb + dot3(muj, gj);
         }
+    };
+    const uint32_t c = b.ncnt[i];
+    if (UNIFORM) {
+        for_neighbors(
+            b, i, c, [&](uint32_t j) { return GLam{a.Pit[j], a.M[j]}; },
+            [&](uint32_t, const GLam &d) { body(d.p, d.m, 1.0f, PS_PHASE_FLUID); });
+    } else {
+        for_neighbors(
+            b, i, c, [&](uint32_t j) { return GLamAttr{a.Pit[j], a.M[j], a.AB[j]}; },
+            [&](uint32_t, const GLamAttr &d) { body(d.p, d.m, d.a.w, d.a.phase); });
     }
     const float4 sd = a.SD[i];
     const float den = sd.w + k.relax;
@@ -314,12 +426,13 @@ __global__ __launch_bounds__(kAdjBlock) void adj_lambda_k(DevBuffers b, KConst k
     const float Db = -((lb * lam) / den);
     a.CD[i] = make_float2(Cb, Db);
     float r = a.r0b[i];
-    r = r - (Cb * (Yi.w * (ir0 * ir0)));
+    r = r - (Cb * (a.Y[i].w * (ir0 * ir0)));
     r = r - (Db * ((2.0f * sd.w) * ir0));
     a.r0b[i] = r;
 }
 
 // dL/dp before the iteration
+template <bool UNIFORM>
 __global__ __launch_bounds__(kAdjBlock) void adj_position_k(DevBuffers b, KConst k, AdjBuffers a, uint32_t n)
 {
     const uint32_t p = blockIdx.x * kAdjBlock + threadIdx.x;
@@ -328,71 +441,70 @@ __global__ __launch_bounds__(kAdjBlock) void adj_position_k(DevBuffers b, KConst
     const Attr ap = b.A[p];
     const float4 Yp = a.Y[p];
     float acc[3] = {Yp.x, Yp.y, Yp.z};
-    float ydp[3], mup[3];
+    float ydp[3];
     yd_of(k, Yp, ydp);
-    const bool pf = ap.phase == PS_PHASE_FLUID;
-    const float irp = 1.0f / ap.rho0;
-    for (int q = 0; q < 3; q++) mup[q] = pf ? ydp[q] * irp : 0.0f;
+    const bool pf = UNIFORM || ap.phase == PS_PHASE_FLUID;
+    const float4 mp4 = a.M[p];
+    const float mup[3] = {mp4.x, mp4.y, mp4.z};
+    const float irp = mp4.w;
     const float2 cdp = a.CD[p];
     const float4 sdp = a.SD[p];
     const float Sp[3] = {sdp.x, sdp.y, sdp.z};
-    const uint32_t c = b.ncnt[p];
-    for (uint32_t q = 0; q < c; q++) {
-        const uint32_t j = nbr_at(b, p, q);
-        const float4 pj = a.Pit[j];
+    const uint32_t ncp = UNIFORM ? 0u : a.nc[p];
+    const bool contacts = !UNIFORM && a.contacts;
+    auto body = [&](const float4 &pj, const float4 &mj, const float4 &sdj, const float2 &cdj, float wj,
+                    int32_t phj, uint32_t ncj) {
         const float dv[3] = {pp.x - pj.x, pp.y - pj.y, pp.z - pj.z};
-        const float r2 = dist2(dv[0], dv[1], dv[2]);
-        if (!(r2 < k.h2)) continue;
-        const Attr aj = b.A[j];
         const float dj[3] = {-dv[0], -dv[1], -dv[2]};
-        float E[3], g6[3], dd[3];
-        if (pf && aj.w != 0.0f) {
-            // p owns the pair
-            adj_pair(k, dv, r2, pp.w, pj.w, mup, E);
-            adj_grad_w6(k, dv, r2, g6);
-            const float cm = (cdp.x * adj_mass(k, aj)) * (1.0f / ap.rho0);
-            for (int q2 = 0; q2 < 3; q2++) acc[q2] = (acc[q2] + E[q2]) + (cm * g6[q2]);
-            if (!(r2 < kDegenerateR2)) {
-                adj_dD(k, dv, r2, k.spiky * (1.0f / ap.rho0), Sp, dd);
-                for (int q2 = 0; q2 < 3; q2++) acc[q2] = acc[q2] + (cdp.y * dd[q2]);
+        const float r2 = dist2(dv[0], dv[1], dv[2]);
+        if (r2 < k.h2) {
+            const PairQ Q = pair_q(k, r2);
+            const float coef = (pp.w + pj.w) + Q.s;
+            float E[3], dd[3];
+            if (pf && wj != 0.0f) {
+                // p owns the pair
+                pair_E(k, Q, dv, coef, mup, E);
+                const float cm = (cdp.x * adj_mass(k, wj, phj)) * irp;
+                for (int q2 = 0; q2 < 3; q2++) acc[q2] = (acc[q2] + E[q2]) + (cm * (Q.c6 * dv[q2]));
+                if (!Q.deg) {
+                    pair_dD(k, Q, dv, k.spiky * irp, Sp, dd);
+                    for (int q2 = 0; q2 < 3; q2++) acc[q2] = acc[q2] + (cdp.y * dd[q2]);
+                }
+            }
+            if (phj == PS_PHASE_FLUID && ap.w != 0.0f) {
+                // j owns the pair (d_jp = -d)
+                const float muj[3] = {mj.x, mj.y, mj.z};
+                const float irj = mj.w;
+                pair_E(k, Q, dj, coef, muj, E);
+                const float cm = (cdj.x * adj_mass(k, ap.w, ap.phase)) * irj;
+                for (int q2 = 0; q2 < 3; q2++) acc[q2] = (acc[q2] - E[q2]) - (cm * (Q.c6 * dj[q2]));
+                if (!Q.deg) {
+                    const float Sj[3] = {sdj.x, sdj.y, sdj.z};
+                    pair_dD(k, Q, dj, k.spiky * irj, Sj, dd);
+                    for (int q2 = 0; q2 < 3; q2++) acc[q2] = acc[q2] - (cdj.y * dd[q2]);
+                }
             }
         }
-        if (aj.phase == PS_PHASE_FLUID && ap.w != 0.0f) {
-            // j owns the pair (d_jp = -d)
-            float ydj[3], muj[3];
-            yd_of(k, a.Y[j], ydj);
-            const float irj = 1.0f / aj.rho0;
-            for (int q2 = 0; q2 < 3; q2++) muj[q2] = ydj[q2] * irj;
-            adj_pair(k, dj, r2, pj.w, pp.w, muj, E);
-            adj_grad_w6(k, dj, r2, g6);
-            const float2 cdj = a.CD[j];
-            const float cm = (cdj.x * adj_mass(k, ap)) * (1.0f / aj.rho0);
-            for (int q2 = 0; q2 < 3; q2++) acc[q2] = (acc[q2] - E[q2]) - (cm * g6[q2]);
-            if (!(r2 < kDegenerateR2)) {
-                const float4 sdj = a.SD[j];
-                const float Sj[3] = {sdj.x, sdj.y, sdj.z};
-                adj_dD(k, dj, r2, k.spiky * (1.0f / aj.rho0), Sj, dd);
-                for (int q2 = 0; q2 < 3; q2++) acc[q2] = acc[q2] - (cdj.y * dd[q2]);
-            }
-        }
-    }
-    // cloth contacts: p's own, and p as the partner of j's
-    if (a.contacts) {
-        const uint32_t ncp = a.nc[p];
-        for (uint32_t q = 0; q < c; q++) {
-            const uint32_t j = nbr_at(b, p, q);
-            const float4 pj = a.Pit[j];
-            const Attr aj = b.A[j];
-            const float dv[3] = {pp.x - pj.x, pp.y - pj.y, pp.z - pj.z};
-            const float dj[3] = {-dv[0], -dv[1], -dv[2]};
+        if (contacts) {
+            // cloth contacts: p's own, then p as the partner of j's (for a
+            // non-fluid j, M holds its masked dL/dy)
             float m[3];
-            if (adj_contact(k, ap, aj, ncp, dv, ydp, m))
+            if (adj_contact(k, ap.w, ap.phase, wj, phj, ncp, dv, ydp, m))
                 for (int q2 = 0; q2 < 3; q2++) acc[q2] = acc[q2] + m[q2];
-            float ydj[3];
-            yd_of(k, a.Y[j], ydj);
-            if (adj_contact(k, aj, ap, a.nc[j], dj, ydj, m))
+            const float ydj[3] = {mj.x, mj.y, mj.z};
+            if (adj_contact(k, wj, phj, ap.w, ap.phase, ncj, dj, ydj, m))
                 for (int q2 = 0; q2 < 3; q2++) acc[q2] = acc[q2] - m[q2];
         }
+    };
+    const uint32_t c = b.ncnt[p];
+    if (UNIFORM) {
+        for_neighbors_chunked(
+            b, p, c, [&](uint32_t j) { return GPos{a.Pit[j], a.M[j], a.SD[j], a.CD[j]}; },
+            [&](uint32_t, const GPos &d) { body(d.p, d.m, d.sd, d.cd, ap.w, PS_PHASE_FLUID, 0u); });
+    } else {
+        for_neighbors_chunked(
+            b, p, c, [&](uint32_t j) { return GPosAttr{a.Pit[j], a.M[j], a.SD[j], a.CD[j], a.AB[j]}; },
+            [&](uint32_t, const GPosAttr &d) { body(d.p, d.m, d.sd, d.cd, d.a.w, d.a.phase, d.a.nc); });
     }
     // distance constraints incident to p
     if (a.dist) {
@@ -458,6 +570,7 @@ __global__ __launch_bounds__(kAdjBlock) void adj_stiffness_k(DevBuffers b, KCons
 // dL/dS_i = (c/rho0_i) vbar_i / dt (in Y) and the rho0 term; adj_finalize_k
 // gathers dL/dp (the final iterate) and dL/dx*.
 
+template <bool UNIFORM>
 __global__ __launch_bounds__(kAdjBlock) void adj_xsph_k(DevBuffers b, KConst k, AdjBuffers a, uint32_t n,
                                                      float dt)
 {
@@ -466,26 +579,33 @@ __global__ __launch_bounds__(kAdjBlock) void adj_xsph_k(DevBuffers b, KConst k, 
     const Attr ai = b.A[i];
     float r0 = 0.0f;
     float Sh[3] = {0.0f, 0.0f, 0.0f};
-    if (k.xsph_c != 0.0f && ai.phase == PS_PHASE_FLUID) {
+    if (k.xsph_c != 0.0f && (UNIFORM || ai.phase == PS_PHASE_FLUID)) {
         const float4 pi = b.P[i], xi = b.X[i];
         const float ui[3] = {pi.x - xi.x, pi.y - xi.y, pi.z - xi.z};
         const float4 vb = a.gv[ai.orig];
         const float vbn[3] = {vb.x, vb.y, vb.z};
         const float cr = k.xsph_c / ai.rho0;
         float Sx = 0.0f, Sy = 0.0f, Sz = 0.0f;
-        const uint32_t c = b.ncnt[i];
-        for (uint32_t q = 0; q < c; q++) {
-            const uint32_t j = nbr_at(b, i, q);
-            const Attr aj = b.A[j];
-            if (aj.phase != PS_PHASE_FLUID) continue;
-            const float4 pj = b.P[j], xj = b.X[j];
+        auto body = [&](const float4 &pj, const float4 &xj, float wj) {
             const float r2 = dist2(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z);
-            if (!(r2 < k.h2)) continue;
+            if (!(r2 < k.h2)) return;
             const float t = k.h2 - r2;
-            const float m = (k.poly6 * ((t * t) * t)) * (1.0f / aj.w);
+            const float m = (k.poly6 * ((t * t) * t)) * (1.0f / wj);
             Sx = __builtin_fmaf((pj.x - xj.x) - ui[0], m, Sx);
             Sy = __builtin_fmaf((pj.y - xj.y) - ui[1], m, Sy);
             Sz = __builtin_fmaf((pj.z - xj.z) - ui[2], m, Sz);
+        };
+        const uint32_t c = b.ncnt[i];
+        if (UNIFORM) {
+            for_neighbors(
+                b, i, c, [&](uint32_t j) { return PX{b.P[j], b.X[j]}; },
+                [&](uint32_t, const PX &d) { body(d.p, d.x, ai.w); });
+        } else {
+            for_neighbors(
+                b, i, c, [&](uint32_t j) { return GXA{b.P[j], b.X[j], b.A[j]}; },
+                [&](uint32_t, const GXA &d) {
+                    if (d.a.phase == PS_PHASE_FLUID) body(d.p, d.x, d.a.w);
+                });
         }
         const float S[3] = {Sx, Sy, Sz};
         const float vS = dot3(vbn, S);
@@ -497,6 +617,7 @@ __global__ __launch_bounds__(kAdjBlock) void adj_xsph_k(DevBuffers b, KConst k, 
     a.Y[i] = make_float4(Sh[0], Sh[1], Sh[2], 0.0f);
 }
 
+template <bool UNIFORM>
 __global__ __launch_bounds__(kAdjBlock) void adj_finalize_k(DevBuffers b, KConst k, AdjBuffers a, uint32_t n,
                                                          float dt)
 {
@@ -506,30 +627,24 @@ __global__ __launch_bounds__(kAdjBlock) void adj_finalize_k(DevBuffers b, KConst
     const float4 vb = a.gv[ap.orig], xb = a.gx[ap.orig];
     float ub[3] = {vb.x / dt, vb.y / dt, vb.z / dt};
     float acc[3] = {xb.x, xb.y, xb.z};
-    if (k.xsph_c != 0.0f && ap.phase == PS_PHASE_FLUID) {
+    if (k.xsph_c != 0.0f && (UNIFORM || ap.phase == PS_PHASE_FLUID)) {
         const float4 pp = b.P[p], xp = b.X[p];
         const float up[3] = {pp.x - xp.x, pp.y - xp.y, pp.z - xp.z};
         const float4 shp4 = a.Y[p];
         const float Shp[3] = {shp4.x, shp4.y, shp4.z};
         const float mp = 1.0f / ap.w;
-        const uint32_t c = b.ncnt[p];
-        for (uint32_t q = 0; q < c; q++) {
-            const uint32_t j = nbr_at(b, p, q);
-            const Attr aj = b.A[j];
-            if (aj.phase != PS_PHASE_FLUID) continue;
-            const float4 pj = b.P[j], xj = b.X[j];
+        auto body = [&](const float4 &pj, const float4 &xj, const float4 &shj4, float wj) {
             const float dv[3] = {pp.x - pj.x, pp.y - pj.y, pp.z - pj.z};
             const float r2 = dist2(dv[0], dv[1], dv[2]);
-            if (!(r2 < k.h2)) continue;
+            if (!(r2 < k.h2)) return;
             const float t = k.h2 - r2;
             const float W = k.poly6 * ((t * t) * t);
             float g6[3];
             adj_grad_w6(k, dv, r2, g6);
             const float uj[3] = {pj.x - xj.x, pj.y - xj.y, pj.z - xj.z};
-            const float4 shj4 = a.Y[j];
             const float Shj[3] = {shj4.x, shj4.y, shj4.z};
             // p as owner: S_p += (u_j - u_p) m_j W; p as neighbour of j
-            const float mj = 1.0f / aj.w;
+            const float mj = 1.0f / wj;
             const float wb = mj * ((((uj[0] - up[0]) * Shp[0]) + ((uj[1] - up[1]) * Shp[1])) +
                                    ((uj[2] - up[2]) * Shp[2]));
             const float wb2 = mp * ((((up[0] - uj[0]) * Shj[0]) + ((up[1] - uj[1]) * Shj[1])) +
@@ -538,6 +653,18 @@ __global__ __launch_bounds__(kAdjBlock) void adj_finalize_k(DevBuffers b, KConst
                 ub[q2] = (ub[q2] - ((mj * W) * Shp[q2])) + ((mp * W) * Shj[q2]);
                 acc[q2] = (acc[q2] + (wb * g6[q2])) + (wb2 * g6[q2]);
             }
+        };
+        const uint32_t c = b.ncnt[p];
+        if (UNIFORM) {
+            for_neighbors(
+                b, p, c, [&](uint32_t j) { return GXY{b.P[j], b.X[j], a.Y[j]}; },
+                [&](uint32_t, const GXY &d) { body(d.p, d.x, d.y, ap.w); });
+        } else {
+            for_neighbors(
+                b, p, c, [&](uint32_t j) { return GXYA{b.P[j], b.X[j], a.Y[j], b.A[j]}; },
+                [&](uint32_t, const GXYA &d) {
+                    if (d.a.phase == PS_PHASE_FLUID) body(d.p, d.x, d.y, d.a.w);
+                });
         }
     }
     a.pb[p] = make_float4(acc[0] + ub[0], acc[1] + ub[1], acc[2] + ub[2], 0.0f);
@@ -568,20 +695,28 @@ __global__ __launch_bounds__(kAdjBlock) void adj_predict_k(DevBuffers b, KConst 
 static inline uint32_t adj_blocks(uint32_t n) { return (n + kAdjBlock - 1) / kAdjBlock; }
 
 hipError_t launch_adj(hipStream_t st, AdjKernel which, const DevBuffers &b, const KConst &k,
-                      const AdjBuffers &a, uint32_t count, float dt)
+                      const AdjBuffers &a, uint32_t count, float dt, bool uniform)
 {
     if (count == 0) return hipSuccess;
     const dim3 grid(adj_blocks(count)), block(kAdjBlock);
+#define PS_ADJ_LAUNCH(kern, ...)                                                                  \
+    do {                                                                                           \
+        if (uniform)                                                                               \
+            hipLaunchKernelGGL(kern<true>, grid, block, 0, st, __VA_ARGS__);                       \
+        else                                                                                       \
+            hipLaunchKernelGGL(kern<false>, grid, block, 0, st, __VA_ARGS__);                      \
+    } while (0)
     switch (which) {
-    case ADJ_XSPH: hipLaunchKernelGGL(adj_xsph_k, grid, block, 0, st, b, k, a, count, dt); break;
-    case ADJ_FINALIZE: hipLaunchKernelGGL(adj_finalize_k, grid, block, 0, st, b, k, a, count, dt); break;
-    case ADJ_PRE: hipLaunchKernelGGL(adj_pre_k, grid, block, 0, st, b, k, a, count); break;
-    case ADJ_LAMBDA: hipLaunchKernelGGL(adj_lambda_k, grid, block, 0, st, b, k, a, count); break;
-    case ADJ_POSITION: hipLaunchKernelGGL(adj_position_k, grid, block, 0, st, b, k, a, count); break;
+    case ADJ_XSPH: PS_ADJ_LAUNCH(adj_xsph_k, b, k, a, count, dt); break;
+    case ADJ_FINALIZE: PS_ADJ_LAUNCH(adj_finalize_k, b, k, a, count, dt); break;
+    case ADJ_PRE: PS_ADJ_LAUNCH(adj_pre_k, b, k, a, count); break;
+    case ADJ_LAMBDA: PS_ADJ_LAUNCH(adj_lambda_k, b, k, a, count); break;
+    case ADJ_POSITION: PS_ADJ_LAUNCH(adj_position_k, b, k, a, count); break;
     case ADJ_STIFFNESS: hipLaunchKernelGGL(adj_stiffness_k, grid, block, 0, st, b, k, a, count); break;
     case ADJ_PREDICT: hipLaunchKernelGGL(adj_predict_k, grid, block, 0, st, b, k, a, count, dt); break;
     default: return hipErrorInvalidValue;
     }
+#undef PS_ADJ_LAUNCH
     return hipGetLastError();
 }
 

@@ -87,14 +87,23 @@ struct ps_system {
     std::string err;
     // reverse pass (ps_step_backward): workspace grown on demand and kept
     struct AdjWork {
-        size_t cap = 0, slots = 0, iters = 0, ndist = 0;
-        float4 *ckX = nullptr, *ckV = nullptr;  // checkpoints, slots x cap
+        size_t cap = 0, cks = 0, tslots = 0, iters = 0, ndist = 0, tab = 0;
+        // checkpoints of the per-particle state, cks slots x cap
+        float4 *ckX = nullptr, *ckV = nullptr;
         Attr *ckA = nullptr;
         uint32_t *ckS = nullptr;
         ps::DevStatus *ckSt = nullptr;
-        float4 *Pit = nullptr, *wrec = nullptr;  // iters x cap
-        uint8_t *clamp = nullptr;
-        float4 *pb = nullptr, *pb2 = nullptr, *Y = nullptr, *SD = nullptr, *xsb = nullptr;
+        // tape of one segment, tslots substeps: neighbour table (tab words
+        // per slot) and counts, storage-order start state, attributes, slot
+        // map and final iterate; per iteration the iterate, clamp bits and
+        // wall-friction records
+        uint32_t *tN = nullptr, *tC = nullptr, *tS = nullptr;
+        float4 *tX = nullptr, *tP = nullptr, *tPit = nullptr, *tW = nullptr;
+        Attr *tA = nullptr;
+        uint8_t *tClamp = nullptr;
+        // reverse-pass work arrays
+        float4 *pb = nullptr, *pb2 = nullptr, *Y = nullptr, *SD = nullptr, *M = nullptr, *xsb = nullptr;
+        ps::AdjAttr *AB = nullptr;
         float4 *gx = nullptr, *gv = nullptr;
         float2 *CD = nullptr;
         uint32_t *nc = nullptr;
@@ -315,11 +324,12 @@ void adj_free(ps_system *s)
 {
     ps_system::AdjWork &w = s->adj;
     dev_free(w.ckX); dev_free(w.ckV); dev_free(w.ckA); dev_free(w.ckS); dev_free(w.ckSt);
-    dev_free(w.Pit); dev_free(w.wrec); dev_free(w.clamp); dev_free(w.pb); dev_free(w.pb2);
-    dev_free(w.Y); dev_free(w.SD); dev_free(w.xsb); dev_free(w.gx); dev_free(w.gv); dev_free(w.CD);
-    dev_free(w.nc); dev_free(w.r0b); dev_free(w.wkb); dev_free(w.drho); dev_free(w.dwk);
-    dev_free(w.sb);
-    w.cap = w.slots = w.iters = w.ndist = 0;
+    dev_free(w.tN); dev_free(w.tC); dev_free(w.tS); dev_free(w.tX); dev_free(w.tP); dev_free(w.tPit);
+    dev_free(w.tW); dev_free(w.tA); dev_free(w.tClamp);
+    dev_free(w.pb); dev_free(w.pb2); dev_free(w.Y); dev_free(w.SD); dev_free(w.M); dev_free(w.AB);
+    dev_free(w.xsb); dev_free(w.gx); dev_free(w.gv); dev_free(w.CD); dev_free(w.nc); dev_free(w.r0b);
+    dev_free(w.wkb); dev_free(w.drho); dev_free(w.dwk); dev_free(w.sb);
+    w.cap = w.cks = w.tslots = w.iters = w.ndist = w.tab = 0;
 }
 
 void free_all(ps_system *s)
@@ -1388,34 +1398,44 @@ int ps_step_graph(ps_system *s, float dt, uint32_t substeps)
 
 namespace {
 
-int adj_reserve(ps_system *s, size_t slots, size_t iters)
+int adj_reserve(ps_system *s, size_t cks, size_t tslots, size_t iters)
 {
     ps_system::AdjWork &w = s->adj;
     const size_t ndist = s->dist_rest.size();
-    if (w.cap >= s->cap && w.slots >= slots && w.iters >= iters && w.ndist >= ndist && w.ckX) return PS_OK;
+    const size_t c = s->cap, tab = ((c + 63) / 64) * 64 * (size_t)s->kmax;
+    if (w.ckX && w.cap >= c && w.cks >= cks && w.tslots >= tslots && w.iters >= iters && w.ndist >= ndist &&
+        w.tab >= tab)
+        return PS_OK;
     adj_free(s);
-    const size_t c = s->cap;
+    const size_t ti = tslots * iters;
     int rc;
-    if ((rc = dev_alloc(s, &w.ckX, slots * c)) || (rc = dev_alloc(s, &w.ckV, slots * c)) ||
-        (rc = dev_alloc(s, &w.ckA, slots * c)) || (rc = dev_alloc(s, &w.ckS, slots * c)) ||
-        (rc = dev_alloc(s, &w.ckSt, slots)) || (rc = dev_alloc(s, &w.Pit, iters * c)) ||
-        (rc = dev_alloc(s, &w.wrec, iters * c)) || (rc = dev_alloc(s, &w.clamp, iters * c)) ||
+    if ((rc = dev_alloc(s, &w.ckX, cks * c)) || (rc = dev_alloc(s, &w.ckV, cks * c)) ||
+        (rc = dev_alloc(s, &w.ckA, cks * c)) || (rc = dev_alloc(s, &w.ckS, cks * c)) ||
+        (rc = dev_alloc(s, &w.ckSt, cks)) || (rc = dev_alloc(s, &w.tN, tslots * tab)) ||
+        (rc = dev_alloc(s, &w.tC, tslots * c)) || (rc = dev_alloc(s, &w.tS, tslots * c)) ||
+        (rc = dev_alloc(s, &w.tX, tslots * c)) || (rc = dev_alloc(s, &w.tP, tslots * c)) ||
+        (rc = dev_alloc(s, &w.tA, tslots * c)) || (rc = dev_alloc(s, &w.tPit, ti * c)) ||
+        (rc = dev_alloc(s, &w.tW, ti * c)) || (rc = dev_alloc(s, &w.tClamp, ti * c)) ||
         (rc = dev_alloc(s, &w.pb, c)) || (rc = dev_alloc(s, &w.pb2, c)) || (rc = dev_alloc(s, &w.Y, c)) ||
-        (rc = dev_alloc(s, &w.SD, c)) || (rc = dev_alloc(s, &w.xsb, c)) || (rc = dev_alloc(s, &w.gx, c)) ||
-        (rc = dev_alloc(s, &w.gv, c)) || (rc = dev_alloc(s, &w.CD, c)) || (rc = dev_alloc(s, &w.nc, c)) ||
-        (rc = dev_alloc(s, &w.r0b, c)) || (rc = dev_alloc(s, &w.wkb, c)) || (rc = dev_alloc(s, &w.drho, c)) ||
-        (rc = dev_alloc(s, &w.dwk, c)) || (rc = dev_alloc(s, &w.sb, ndist))) {
+        (rc = dev_alloc(s, &w.SD, c)) || (rc = dev_alloc(s, &w.M, c)) || (rc = dev_alloc(s, &w.AB, c)) ||
+        (rc = dev_alloc(s, &w.xsb, c)) || (rc = dev_alloc(s, &w.gx, c)) || (rc = dev_alloc(s, &w.gv, c)) ||
+        (rc = dev_alloc(s, &w.CD, c)) || (rc = dev_alloc(s, &w.nc, c)) || (rc = dev_alloc(s, &w.r0b, c)) ||
+        (rc = dev_alloc(s, &w.wkb, c)) || (rc = dev_alloc(s, &w.drho, c)) || (rc = dev_alloc(s, &w.dwk, c)) ||
+        (rc = dev_alloc(s, &w.sb, ndist))) {
         adj_free(s);
         return rc == PS_ERR_DEVICE ? fail(s, PS_ERR_OUT_OF_MEMORY, "reverse-pass workspace") : rc;
     }
     w.cap = c;
-    w.slots = slots;
+    w.cks = cks;
+    w.tslots = tslots;
     w.iters = iters;
     w.ndist = ndist;
+    w.tab = tab;
     return PS_OK;
 }
 
-// copy the per-particle state (and the device status word) to / from slot k
+// copy the per-particle state (and the device status word) to / from
+// checkpoint k
 int adj_copy(ps_system *s, size_t k, bool save)
 {
     ps_system::AdjWork &w = s->adj;
@@ -1445,41 +1465,72 @@ int adj_substep(ps_system *s, float dt)
     return stage_end(s);
 }
 
-// The substep once more, keeping each iteration's iterate (with lambda),
-// clamp bits and wall-friction records; the final iterate stays in b.P.
-int adj_record(ps_system *s, float dt)
+// One substep that also writes tape slot t: its neighbour table goes
+// straight into the slot; the sorted start state, attributes, slot map,
+// every iterate (with lambda), the clamp bits, the wall-friction records and
+// the final iterate are copied there.
+int adj_taped_substep(ps_system *s, float dt, size_t t)
 {
-    int rc;
-    if ((rc = ps_substep_begin(s, dt))) return rc;
     ps_system::AdjWork &w = s->adj;
     DevBuffers &b = s->b;
+    uint32_t *own_nbr = b.nbr, *own_cnt = b.ncnt;
+    b.nbr = w.tN + t * w.tab;
+    b.ncnt = w.tC + t * w.cap;
+    int rc = ps_substep_begin(s, dt);
+    const size_t n = s->n, c = w.cap, I = s->prm.iterations;
+    hipError_t e = hipSuccess;
+    auto cp = [&](void *dst, const void *src, size_t bytes) {
+        if (e == hipSuccess) e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, s->stream);
+    };
+    if (!rc) {
+        cp(w.tX + t * c, b.X, sizeof(float4) * n);
+        cp(w.tA + t * c, b.A, sizeof(Attr) * n);
+        cp(w.tS + t * c, b.slot_of, sizeof(uint32_t) * n);
+    }
     const bool uni = uniform_fluid(s);
-    for (uint32_t it = 0; it < s->prm.iterations; it++) {
-        LAUNCH(s, PS_K_LAMBDA, ps::launch_lambda(s->stream, b, s->kc, s->n, uni, s->wu, s->block[PS_K_LAMBDA]));
-        HIP_TRY(s, hipMemcpyAsync(w.Pit + it * w.cap, b.P, sizeof(float4) * s->n, hipMemcpyDeviceToDevice,
-                                  s->stream));
-        b.clamp_rec = w.clamp + it * w.cap;
-        b.wall_rec = w.wrec + it * w.cap;
-        const hipError_t e = ps::launch_solve(s->stream, b, s->kc, s->n, features(s), uni, s->wu, s->it_sub,
-                                              s->block[PS_K_SOLVE]);
+    for (uint32_t it = 0; !rc && e == hipSuccess && it < I; it++) {
+        const size_t q = (t * I + it) * c;
+        e = ps::launch_lambda(s->stream, b, s->kc, s->n, uni, s->wu, s->block[PS_K_LAMBDA]);
+        cp(w.tPit + q, b.P, sizeof(float4) * n);
+        b.clamp_rec = w.tClamp + q;
+        b.wall_rec = w.tW + q;
+        if (e == hipSuccess)
+            e = ps::launch_solve(s->stream, b, s->kc, s->n, features(s), uni, s->wu, s->it_sub,
+                                 s->block[PS_K_SOLVE]);
         b.clamp_rec = nullptr;
         b.wall_rec = nullptr;
-        if (e != hipSuccess) return fail(s, PS_ERR_DEVICE, std::string("launch solve: ") + hipGetErrorString(e));
         std::swap(b.P, b.Q);
         s->iters++;
         s->it_sub++;
     }
-    return PS_OK;
+    if (!rc) cp(w.tP + t * c, b.P, sizeof(float4) * n);
+    if (!rc && e == hipSuccess) rc = stage_end(s);
+    b.nbr = own_nbr;
+    b.ncnt = own_cnt;
+    s->nbr_valid = false;
+    if (!rc && e != hipSuccess) rc = fail(s, PS_ERR_DEVICE, std::string("taped substep: ") + hipGetErrorString(e));
+    return rc;
 }
 
-// reverse of the recorded substep: insertion-order gx/gv of its end state
-// in, of its start state out; rho0, stiffness and wall_k terms accumulate
-int adj_reverse(ps_system *s, float dt)
+// Reverse of the substep on tape slot t: insertion-order gx/gv of its end
+// state in, of its start state out; rho0, stiffness and wall_k terms
+// accumulate.
+int adj_reverse(ps_system *s, float dt, size_t t)
 {
     ps_system::AdjWork &w = s->adj;
+    const size_t c = w.cap, I = s->prm.iterations;
+    DevBuffers bt = s->b;  // the substep as taped
+    bt.nbr = w.tN + t * w.tab;
+    bt.ncnt = w.tC + t * c;
+    bt.X = w.tX + t * c;
+    bt.A = w.tA + t * c;
+    bt.slot_of = w.tS + t * c;
+    bt.P = w.tP + t * c;
     ps::AdjBuffers a{};
     a.Y = w.Y;
     a.SD = w.SD;
+    a.M = w.M;
+    a.AB = w.AB;
     a.CD = w.CD;
     a.nc = w.nc;
     a.r0b = w.r0b;
@@ -1495,28 +1546,26 @@ int adj_reverse(ps_system *s, float dt)
     float4 *cur = w.pb, *nxt = w.pb2;
     a.pb = cur;
     const uint32_t n = s->n, ndist = (uint32_t)s->dist_rest.size();
-    LAUNCH(s, PS_K_ADJ_XSPH, ps::launch_adj(s->stream, ps::ADJ_XSPH, s->b, s->kc, a, n, dt));
-    LAUNCH(s, PS_K_ADJ_FINALIZE, ps::launch_adj(s->stream, ps::ADJ_FINALIZE, s->b, s->kc, a, n, dt));
-    for (uint32_t it = s->prm.iterations; it-- > 0;) {
-        a.Pit = w.Pit + it * w.cap;
-        a.clamp = w.clamp + it * w.cap;
-        a.wrec = w.wrec + it * w.cap;
+    const bool uni = s->uniform;
+    LAUNCH(s, PS_K_ADJ_XSPH, ps::launch_adj(s->stream, ps::ADJ_XSPH, bt, s->kc, a, n, dt, uni));
+    LAUNCH(s, PS_K_ADJ_FINALIZE, ps::launch_adj(s->stream, ps::ADJ_FINALIZE, bt, s->kc, a, n, dt, uni));
+    for (size_t it = I; it-- > 0;) {
+        const size_t q = (t * I + it) * c;
+        a.Pit = w.tPit + q;
+        a.clamp = w.tClamp + q;
+        a.wrec = w.tW + q;
         a.pbn = cur;
         a.pb = nxt;
-        LAUNCH(s, PS_K_ADJ_PRE, ps::launch_adj(s->stream, ps::ADJ_PRE, s->b, s->kc, a, n, dt));
-        LAUNCH(s, PS_K_ADJ_LAMBDA, ps::launch_adj(s->stream, ps::ADJ_LAMBDA, s->b, s->kc, a, n, dt));
-        LAUNCH(s, PS_K_ADJ_POSITION, ps::launch_adj(s->stream, ps::ADJ_POSITION, s->b, s->kc, a, n, dt));
+        LAUNCH(s, PS_K_ADJ_PRE, ps::launch_adj(s->stream, ps::ADJ_PRE, bt, s->kc, a, n, dt, uni));
+        LAUNCH(s, PS_K_ADJ_LAMBDA, ps::launch_adj(s->stream, ps::ADJ_LAMBDA, bt, s->kc, a, n, dt, uni));
+        LAUNCH(s, PS_K_ADJ_POSITION, ps::launch_adj(s->stream, ps::ADJ_POSITION, bt, s->kc, a, n, dt, uni));
         if (ndist)
             LAUNCH(s, PS_K_ADJ_STIFFNESS,
-                   ps::launch_adj(s->stream, ps::ADJ_STIFFNESS, s->b, s->kc, a, ndist, dt));
+                   ps::launch_adj(s->stream, ps::ADJ_STIFFNESS, bt, s->kc, a, ndist, dt, uni));
         std::swap(cur, nxt);
     }
     a.pb = cur;
-    LAUNCH(s, PS_K_ADJ_PREDICT, ps::launch_adj(s->stream, ps::ADJ_PREDICT, s->b, s->kc, a, n, dt));
-    s->stage = 0;
-    s->it_sub = 0;
-    s->hashed = false;
-    s->nbr_valid = false;
+    LAUNCH(s, PS_K_ADJ_PREDICT, ps::launch_adj(s->stream, ps::ADJ_PREDICT, bt, s->kc, a, n, dt, uni));
     return PS_OK;
 }
 
@@ -1525,12 +1574,16 @@ int step_backward_impl(ps_system *s, float dt, uint32_t S, uint32_t K, const flo
                        float *dl_dstiffness, float *dl_dmu_k)
 {
     const uint32_t n = s->n;
-    const uint32_t nseg = (S + K - 1) / K;
-    const size_t fin = nseg, seg0 = nseg + 1;
+    // segments of K substeps (K = 0: one segment, the whole horizon); the
+    // last is taped by the forward pass, the others recomputed from their
+    // checkpoints and taped when the reverse pass reaches them
+    const uint32_t seg = K ? K : S;
+    const uint32_t nseg = (S + seg - 1) / seg;
+    const size_t fin = nseg;  // checkpoint of the final state
     int rc;
-    if ((rc = adj_reserve(s, seg0 + K, s->prm.iterations ? s->prm.iterations : 1))) return rc;
+    if ((rc = adj_reserve(s, nseg + 1, seg, s->prm.iterations ? s->prm.iterations : 1))) return rc;
     ps_system::AdjWork &w = s->adj;
-    std::vector<uint64_t> sub(nseg + 1 + K), its(nseg + 1 + K);
+    std::vector<uint64_t> sub(nseg + 1), its(nseg + 1);
     auto save = [&](size_t k) {
         sub[k] = s->substeps;
         its[k] = s->iters;
@@ -1541,10 +1594,20 @@ int step_backward_impl(ps_system *s, float dt, uint32_t S, uint32_t K, const flo
         s->iters = its[k];
         return adj_copy(s, k, false);
     };
-    // forward, checkpointing every K substeps
-    for (uint32_t t = 0; t < S; t++) {
-        if (t % K == 0 && (rc = save(t / K))) return rc;
-        if ((rc = adj_substep(s, dt))) return rc;
+    auto tape_segment = [&](uint32_t g) {
+        const uint32_t s0 = g * seg, s1 = std::min(S, s0 + seg);
+        for (uint32_t t = s0; t < s1; t++)
+            if (int r = adj_taped_substep(s, dt, t - s0)) return r;
+        return (int)PS_OK;
+    };
+    for (uint32_t g = 0; g < nseg; g++) {
+        if ((rc = save(g))) return rc;
+        if (g + 1 < nseg) {
+            for (uint32_t t = g * seg; t < std::min(S, (g + 1) * seg); t++)
+                if ((rc = adj_substep(s, dt))) return rc;
+        } else if ((rc = tape_segment(g))) {
+            return rc;
+        }
     }
     if ((rc = save(fin))) return rc;
     ps::DevStatus ds;
@@ -1555,44 +1618,42 @@ int step_backward_impl(ps_system *s, float dt, uint32_t S, uint32_t K, const flo
                     "a neighbour table overflowed (raise max_neighbors, or clear the status of an earlier "
                     "overflow): the reverse pass needs every pair");
     // seed: dL/dx_T, dL/dv_T in insertion order
-    std::vector<float4> g(n);
+    std::vector<float4> g4(n);
     for (int which = 0; which < 2; which++) {
         const float *src = which ? dl_dv : dl_dx;
         for (uint32_t o = 0; o < n; o++)
-            g[o] = src ? make_float4(src[3 * o], src[3 * o + 1], src[3 * o + 2], 0.0f)
-                       : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        HIP_TRY(s, hipMemcpy(which ? w.gv : w.gx, g.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
+            g4[o] = src ? make_float4(src[3 * o], src[3 * o + 1], src[3 * o + 2], 0.0f)
+                        : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        HIP_TRY(s, hipMemcpy(which ? w.gv : w.gx, g4.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
     }
     HIP_TRY(s, hipMemsetAsync(w.drho, 0, sizeof(float) * n, s->stream));
     HIP_TRY(s, hipMemsetAsync(w.dwk, 0, sizeof(float) * n, s->stream));
     if (!s->dist_rest.empty())
         HIP_TRY(s, hipMemsetAsync(w.sb, 0, sizeof(float) * s->dist_rest.size(), s->stream));
-    // segments backwards: recompute each substep start of the segment from
-    // its checkpoint, then reverse the substeps one by one
-    for (uint32_t gi = nseg; gi-- > 0;) {
-        const uint32_t s0 = gi * K, s1 = std::min(S, s0 + K);
-        if ((rc = load(gi))) return rc;
-        for (uint32_t t = s0; t < s1; t++) {
-            if ((rc = save(seg0 + (t - s0)))) return rc;
-            if (t + 1 < s1 && (rc = adj_substep(s, dt))) return rc;
+    for (uint32_t g = nseg; g-- > 0;) {
+        const uint32_t s0 = g * seg, s1 = std::min(S, s0 + seg);
+        if (g + 1 < nseg) {
+            if ((rc = load(g))) return rc;
+            if ((rc = tape_segment(g))) return rc;
         }
-        for (uint32_t t = s1; t-- > s0;) {
-            if ((rc = load(seg0 + (t - s0)))) return rc;
-            if ((rc = adj_record(s, dt))) return rc;
-            if ((rc = adj_reverse(s, dt))) return rc;
-        }
+        for (uint32_t t = s1; t-- > s0;)
+            if ((rc = adj_reverse(s, dt, t - s0))) return rc;
     }
     if ((rc = load(fin))) return rc;
+    s->stage = 0;
+    s->it_sub = 0;
+    s->hashed = false;
+    s->nbr_valid = false;
     HIP_TRY(s, hipStreamSynchronize(s->stream));
     // results
     for (int which = 0; which < 2; which++) {
         float *dst = which ? dl_dv0 : dl_dx0;
         if (!dst) continue;
-        HIP_TRY(s, hipMemcpy(g.data(), which ? w.gv : w.gx, sizeof(float4) * n, hipMemcpyDeviceToHost));
+        HIP_TRY(s, hipMemcpy(g4.data(), which ? w.gv : w.gx, sizeof(float4) * n, hipMemcpyDeviceToHost));
         for (uint32_t o = 0; o < n; o++) {
-            dst[3 * o] = g[o].x;
-            dst[3 * o + 1] = g[o].y;
-            dst[3 * o + 2] = g[o].z;
+            dst[3 * o] = g4[o].x;
+            dst[3 * o + 1] = g4[o].y;
+            dst[3 * o + 2] = g4[o].z;
         }
     }
     if (dl_drho0) HIP_TRY(s, hipMemcpy(dl_drho0, w.drho, sizeof(float) * n, hipMemcpyDeviceToHost));
@@ -1642,7 +1703,7 @@ int ps_step_backward(ps_system *s, float dt, uint32_t substeps, uint32_t checkpo
         if (dl_dmu_k) *dl_dmu_k = 0.0f;
         return PS_OK;
     }
-    const uint32_t K = checkpoint_every ? std::min(checkpoint_every, substeps) : 1u;
+    const uint32_t K = checkpoint_every >= substeps ? 0u : checkpoint_every;
     drop_graph(s);
     s->adj_active = true;
     const int rc = step_backward_impl(s, dt, substeps, K, dl_dx, dl_dv, dl_dx0, dl_dv0, dl_drho0,

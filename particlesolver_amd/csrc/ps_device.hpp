@@ -60,6 +60,32 @@ __device__ __forceinline__ void for_neighbors(const DevBuffers &b, uint32_t i, u
     }
 }
 
+// As for_neighbors, one chunk at a time: the four gathers of chunk m are
+// issued together (and the index load of chunk m+1), then processed.  Half
+// the records in flight, for kernels whose records or arithmetic are large.
+template <typename G, typename Fn>
+__device__ __forceinline__ void for_neighbors_chunked(const DevBuffers &b, uint32_t i, uint32_t c, G gather,
+                                                      Fn fn)
+{
+    if (c == 0) return;
+    const uint4 *nb = nbr_lane(b, i);
+    const uint32_t last = (c - 1) >> 2;
+    uint4 jj = nb[0];
+    for (uint32_t m = 0; m <= last; m++) {
+        const uint4 jn = nb[(size_t)min(m + 1, last) * 64];
+        auto d0 = gather(jj.x);
+        auto d1 = gather(jj.y);
+        auto d2 = gather(jj.z);
+        auto d3 = gather(jj.w);
+        const uint32_t q = m << 2;
+        fn(jj.x, d0);
+        if (q + 1 < c) fn(jj.y, d1);
+        if (q + 2 < c) fn(jj.z, d2);
+        if (q + 3 < c) fn(jj.w, d3);
+        jj = jn;
+    }
+}
+
 struct PA {
     float4 p;
     Attr a;

@@ -155,6 +155,14 @@ struct DevBuffers {
     uint32_t *bc;          // boundary flags | boundary count << 8
 };
 
+// Per-particle attributes the reverse pass gathers from neighbours.
+struct __align__(16) AdjAttr {
+    float w;
+    int32_t phase;
+    uint32_t nc;  // contact count
+    float pad;
+};
+
 // Work buffers of the reverse pass (ps_adjoint.hip), storage order unless
 // noted.
 struct AdjBuffers {
@@ -165,6 +173,8 @@ struct AdjBuffers {
     float4 *pb;            // dL/dp before it (finalize: of the final iterate)
     float4 *Y;             // masked dL/dy (xyz) and density (w); finalize: dL/dS
     float4 *SD;            // S = sum_j a d (xyz), D (w)
+    float4 *M;             // mu = dL/dy / rho0 (fluid) or dL/dy (xyz), 1/rho0 (w)
+    AdjAttr *AB;           // attributes with contact counts (non-uniform scenes)
     float2 *CD;            // dL/dC, dL/dD
     uint32_t *nc;          // contact count
     float *r0b, *wkb;      // dL/drho0 and dL/dwall_k of this substep
@@ -214,8 +224,9 @@ enum AdjKernel : uint32_t {
     ADJ_STIFFNESS,
     ADJ_PREDICT,
 };
+// uniform: every particle fluid with one inverse mass (no attribute gathers)
 hipError_t launch_adj(hipStream_t st, AdjKernel which, const DevBuffers &b, const KConst &k,
-                      const AdjBuffers &a, uint32_t count, float dt);
+                      const AdjBuffers &a, uint32_t count, float dt, bool uniform);
 
 // CPU-reference mode (ps_cpuref.hip); storage in insertion order, iterate
 // ping-pong b.P -> b.Q per fluid object.
