@@ -361,15 +361,16 @@ def rt2d_main(args):
 # reverse-pass kernels: algorithmic bytes per particle per launch, each
 # array read or written once (neighbour data counted inside the own reads)
 ADJ_ALGO_BYTES = {
-    # dL/dp in 16 + clamp 1 + iterate 16 + attributes 16 + count 4 + K indices
-    # + dL/dy,rho out 16 + S,D out 16 + rho0 term r/w 8 + contact count out 4
-    "adj_pre": lambda K: 97.0 + 4.0 * K,
-    # attributes 16 + iterate 16 + dL/dy,rho 16 + S,D 16 + count 4 + K indices
-    # + rho0 term r/w 8 + dL/dC,dL/dD out 8
-    "adj_lambda": lambda K: 84.0 + 4.0 * K,
-    # iterate 16 + attributes 16 + dL/dy 16 + dL/dC,dD 8 + S,D 16 + count 4
-    # + contact count 4 + K indices + dL/dp out 16
-    "adj_position": lambda K: 96.0 + 4.0 * K,
+    # elementwise: dL/dp in 16 + clamp 1 + attributes 16 + recorded f and
+    # contact count 16 + rho0 term r/w 8 + dL/dy out 16 + mu,1/rho0 out 16
+    # + attribute record out 16
+    "adj_pre": lambda K: 105.0,
+    # attributes 16 + iterate 16 + mu 16 + recorded S,D 16 + recorded rho 4
+    # + count 4 + K indices + rho0 term r/w 8 + dL/dC,dL/dD out 8
+    "adj_lambda": lambda K: 88.0 + 4.0 * K,
+    # iterate 16 + attributes 16 + dL/dy 16 + mu 16 + dL/dC,dD 8 + S,D 16
+    # + count 4 + contact count 4 + K indices + dL/dp out 16
+    "adj_position": lambda K: 112.0 + 4.0 * K,
 }
 
 

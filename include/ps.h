@@ -318,7 +318,7 @@ int ps_step_graph(ps_system *sys, float dt, uint32_t substeps);
  * rigid bodies, sleeping, PS_FLAG_REBUILD_EVERY_ITERATION, batches and the
  * CPU-reference mode; PS_ERR_CAPACITY if a neighbour table overflowed (the
  * transposed sums need every pair).  Device memory, kept by the handle:
- * (4 max_neighbors + 56 + 33 iterations) bytes per particle per taped
+ * (4 max_neighbors + 56 + 69 iterations) bytes per particle per taped
  * substep, plus 52 per particle per checkpoint.  Synchronous. */
 int ps_step_backward(ps_system *sys, float dt, uint32_t substeps, uint32_t checkpoint_every,
                      const float *dl_dx, const float *dl_dv, float *dl_dx0, float *dl_dv0,

@@ -218,13 +218,6 @@ __device__ __forceinline__ float adj_mass(const KConst &k, float wj, int32_t phj
 
 // Neighbour records of the gathers.  UNIFORM (every particle fluid, one
 // inverse mass) variants leave out the attribute loads.
-struct GPit {
-    float4 p;
-};
-struct GPitAttr {
-    float4 p;
-    AdjAttr a;
-};
 struct GLam {
     float4 p, m;
 };
@@ -254,20 +247,20 @@ struct GXYA {
 };
 
 // ---------------------------------------------------------------------------
-// Pins, walls and wall friction backwards, contact counts, and the forward
-// quantities of the lambda pass (rho, S, D) and of the fluid update (f).
-// Wall friction (walls(): p'' = p' - t f, t = e - (e.n) n, e = p' - x*):
-// static, or kinetic clamped to f = 1: dp''/dp' = n n^T, dp''/dx* = I - n n^T;
-// kinetic with f = wall_k / l < 1: dp''/dp' = I - wall_k K (I - n n^T),
-// K = (I - t^ t^T) / l, the x* part its negative second term, and
-// dp''/dwall_k = -t / l.  Writes M = (mu = yd / rho0 for fluid, yd
-// otherwise; 1 / rho0) and the attribute record the later passes gather.
+// Pins, walls and wall friction backwards, and the fluid sum's rho0 term,
+// from the recorded forward quantities (rho, S, D of the lambda pass; f and
+// the contact count of the solve).  Wall friction (walls(): p'' = p' - t f,
+// t = e - (e.n) n, e = p' - x*): static, or kinetic clamped to f = 1:
+// dp''/dp' = n n^T, dp''/dx* = I - n n^T; kinetic with f = wall_k / l < 1:
+// dp''/dp' = I - wall_k K (I - n n^T), K = (I - t^ t^T) / l, the x* part its
+// negative second term, and dp''/dwall_k = -t / l.  Writes M = (mu =
+// yd / rho0 for fluid, yd otherwise; 1 / rho0) and the attribute record the
+// later passes gather.
 template <bool UNIFORM>
 __global__ __launch_bounds__(kAdjBlock) void adj_pre_k(DevBuffers b, KConst k, AdjBuffers a, uint32_t n)
 {
     const uint32_t i = blockIdx.x * kAdjBlock + threadIdx.x;
     if (i >= n) return;
-    const float4 pi = a.Pit[i];
     const Attr ai = b.A[i];
     float g[3];
     load3(a.pbn[i], g);
@@ -310,70 +303,18 @@ __global__ __launch_bounds__(kAdjBlock) void adj_pre_k(DevBuffers b, KConst k, A
         if (cl & (1u << q)) g[q] = 0.0f;
     const float yd[3] = {g[0], g[1], k.dims == 2u ? 0.0f : g[2]};
     const bool fl = UNIFORM || ai.phase == PS_PHASE_FLUID;
-    const bool cloth = !UNIFORM && ai.phase >= PS_PHASE_CLOTH && ai.w != 0.0f;
-    const uint32_t c = b.ncnt[i];
-    uint32_t nc = 0;
-    const float r0 = ai.rho0;
-    const float ir0 = 1.0f / r0;
-    const float sk = k.spiky * (1.0f / r0);
-    float rr = k.w_zero * (1.0f / ai.w);
-    float gix = 0.0f, giy = 0.0f, giz = 0.0f, den = 0.0f;
-    float f[3] = {0.0f, 0.0f, 0.0f};
-    auto body = [&](const float4 &pj, float wj, int32_t phj) {
-        const float dv[3] = {pi.x - pj.x, pi.y - pj.y, pi.z - pj.z};
-        if (cloth && !(ai.phase > PS_PHASE_SOLID && ai.phase == phj)) {
-            // contact count (collideD's first loop)
-            const float r2c = ((dv[0] * dv[0]) + (dv[1] * dv[1])) + (dv[2] * dv[2]);
-            if (r2c < k.collide_r2) nc++;
-        }
-        if (!fl || wj == 0.0f) return;
-        const float r2 = dist2(dv[0], dv[1], dv[2]);
-        if (!(r2 < k.h2)) return;
-        const float t = k.h2 - r2;
-        const float W = k.poly6 * ((t * t) * t);
-        rr = __builtin_fmaf(W, adj_mass(k, wj, phj), rr);
-        const PairQ Q = pair_q(k, r2);
-        const float coef = (pi.w + pj.w) + Q.s;
-        if (Q.deg) {
-            f[1] = f[1] + (coef * -((k.spiky * (k.h * k.h)) * 0.001f));
-            f[0] = f[0] + (coef * 0.0f);
-            f[2] = f[2] + (coef * 0.0f);
-            return;
-        }
-        for (int q2 = 0; q2 < 3; q2++) f[q2] = f[q2] + (coef * -(Q.bb * dv[q2]));
-        const float ri = Q.ri;
-        const float qq = Q.q;
-        const float aa = (sk * (qq * qq)) * ri;
-        den = __builtin_fmaf(aa * aa, r2, den);
-        gix = __builtin_fmaf(aa, dv[0], gix);
-        giy = __builtin_fmaf(aa, dv[1], giy);
-        giz = __builtin_fmaf(aa, dv[2], giz);
-    };
-    if (UNIFORM) {
-        for_neighbors(
-            b, i, c, [&](uint32_t j) { return GPit{a.Pit[j]}; },
-            [&](uint32_t, const GPit &d) { body(d.p, ai.w, PS_PHASE_FLUID); });
-    } else {
-        for_neighbors(
-            b, i, c, [&](uint32_t j) { return GPitAttr{a.Pit[j], AdjAttr{b.A[j].w, b.A[j].phase, 0u, 0.0f}}; },
-            [&](uint32_t, const GPitAttr &d) { body(d.p, d.a.w, d.a.phase); });
-    }
-    float rho = 0.0f, D = 0.0f, S[3] = {0.0f, 0.0f, 0.0f};
+    const float ir0 = 1.0f / ai.rho0;
+    const float4 tf = a.tf[i];
     if (fl) {
-        den = den + dist2(gix, giy, giz);
-        rho = rr;
-        D = den;
-        S[0] = gix;
-        S[1] = giy;
-        S[2] = giz;
+        const float f[3] = {tf.x, tf.y, tf.z};
         a.r0b[i] = a.r0b[i] - (dot3(yd, f) * (ir0 * ir0));
     }
-    a.Y[i] = make_float4(g[0], g[1], g[2], rho);
-    a.SD[i] = make_float4(S[0], S[1], S[2], D);
+    a.Y[i] = make_float4(g[0], g[1], g[2], 0.0f);
     a.M[i] = fl ? make_float4(yd[0] * ir0, yd[1] * ir0, yd[2] * ir0, ir0) : make_float4(yd[0], yd[1], yd[2], ir0);
     if (!UNIFORM) {
+        const uint32_t nc = __float_as_uint(tf.w);
         a.nc[i] = nc;
-        a.AB[i] = AdjAttr{ai.w, ai.phase, nc, 0.0f};
+        a.AB[i] = AdjAttr{ai.w, ai.phase, nc, adj_mass(k, ai.w, ai.phase)};
     }
 }
 
@@ -426,7 +367,7 @@ __global__ __launch_bounds__(kAdjBlock) void adj_lambda_k(DevBuffers b, KConst k
     const float Db = -((lb * lam) / den);
     a.CD[i] = make_float2(Cb, Db);
     float r = a.r0b[i];
-    r = r - (Cb * (a.Y[i].w * (ir0 * ir0)));
+    r = r - (Cb * (a.rho[i] * (ir0 * ir0)));
     r = r - (Db * ((2.0f * sd.w) * ir0));
     a.r0b[i] = r;
 }
@@ -452,8 +393,9 @@ __global__ __launch_bounds__(kAdjBlock) void adj_position_k(DevBuffers b, KConst
     const float Sp[3] = {sdp.x, sdp.y, sdp.z};
     const uint32_t ncp = UNIFORM ? 0u : a.nc[p];
     const bool contacts = !UNIFORM && a.contacts;
+    const float massp = adj_mass(k, ap.w, ap.phase);  // p as a neighbour of j
     auto body = [&](const float4 &pj, const float4 &mj, const float4 &sdj, const float2 &cdj, float wj,
-                    int32_t phj, uint32_t ncj) {
+                    int32_t phj, uint32_t ncj, float massj) {
         const float dv[3] = {pp.x - pj.x, pp.y - pj.y, pp.z - pj.z};
         const float dj[3] = {-dv[0], -dv[1], -dv[2]};
         const float r2 = dist2(dv[0], dv[1], dv[2]);
@@ -464,7 +406,7 @@ __global__ __launch_bounds__(kAdjBlock) void adj_position_k(DevBuffers b, KConst
             if (pf && wj != 0.0f) {
                 // p owns the pair
                 pair_E(k, Q, dv, coef, mup, E);
-                const float cm = (cdp.x * adj_mass(k, wj, phj)) * irp;
+                const float cm = (cdp.x * massj) * irp;
                 for (int q2 = 0; q2 < 3; q2++) acc[q2] = (acc[q2] + E[q2]) + (cm * (Q.c6 * dv[q2]));
                 if (!Q.deg) {
                     pair_dD(k, Q, dv, k.spiky * irp, Sp, dd);
@@ -476,7 +418,7 @@ __global__ __launch_bounds__(kAdjBlock) void adj_position_k(DevBuffers b, KConst
                 const float muj[3] = {mj.x, mj.y, mj.z};
                 const float irj = mj.w;
                 pair_E(k, Q, dj, coef, muj, E);
-                const float cm = (cdj.x * adj_mass(k, ap.w, ap.phase)) * irj;
+                const float cm = (cdj.x * massp) * irj;
                 for (int q2 = 0; q2 < 3; q2++) acc[q2] = (acc[q2] - E[q2]) - (cm * (Q.c6 * dj[q2]));
                 if (!Q.deg) {
                     const float Sj[3] = {sdj.x, sdj.y, sdj.z};
@@ -500,11 +442,13 @@ __global__ __launch_bounds__(kAdjBlock) void adj_position_k(DevBuffers b, KConst
     if (UNIFORM) {
         for_neighbors_chunked(
             b, p, c, [&](uint32_t j) { return GPos{a.Pit[j], a.M[j], a.SD[j], a.CD[j]}; },
-            [&](uint32_t, const GPos &d) { body(d.p, d.m, d.sd, d.cd, ap.w, PS_PHASE_FLUID, 0u); });
+            [&](uint32_t, const GPos &d) { body(d.p, d.m, d.sd, d.cd, ap.w, PS_PHASE_FLUID, 0u, massp); });
     } else {
         for_neighbors_chunked(
             b, p, c, [&](uint32_t j) { return GPosAttr{a.Pit[j], a.M[j], a.SD[j], a.CD[j], a.AB[j]}; },
-            [&](uint32_t, const GPosAttr &d) { body(d.p, d.m, d.sd, d.cd, d.a.w, d.a.phase, d.a.nc); });
+            [&](uint32_t, const GPosAttr &d) {
+                body(d.p, d.m, d.sd, d.cd, d.a.w, d.a.phase, d.a.nc, d.a.mass);
+            });
     }
     // distance constraints incident to p
     if (a.dist) {

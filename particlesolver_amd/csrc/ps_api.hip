@@ -98,11 +98,12 @@ struct ps_system {
         // map and final iterate; per iteration the iterate, clamp bits and
         // wall-friction records
         uint32_t *tN = nullptr, *tC = nullptr, *tS = nullptr;
-        float4 *tX = nullptr, *tP = nullptr, *tPit = nullptr, *tW = nullptr;
+        float4 *tX = nullptr, *tP = nullptr, *tPit = nullptr, *tW = nullptr, *tLQ = nullptr, *tF = nullptr;
+        float *tRho = nullptr;
         Attr *tA = nullptr;
         uint8_t *tClamp = nullptr;
         // reverse-pass work arrays
-        float4 *pb = nullptr, *pb2 = nullptr, *Y = nullptr, *SD = nullptr, *M = nullptr, *xsb = nullptr;
+        float4 *pb = nullptr, *pb2 = nullptr, *Y = nullptr, *M = nullptr, *xsb = nullptr;
         ps::AdjAttr *AB = nullptr;
         float4 *gx = nullptr, *gv = nullptr;
         float2 *CD = nullptr;
@@ -325,8 +326,8 @@ void adj_free(ps_system *s)
     ps_system::AdjWork &w = s->adj;
     dev_free(w.ckX); dev_free(w.ckV); dev_free(w.ckA); dev_free(w.ckS); dev_free(w.ckSt);
     dev_free(w.tN); dev_free(w.tC); dev_free(w.tS); dev_free(w.tX); dev_free(w.tP); dev_free(w.tPit);
-    dev_free(w.tW); dev_free(w.tA); dev_free(w.tClamp);
-    dev_free(w.pb); dev_free(w.pb2); dev_free(w.Y); dev_free(w.SD); dev_free(w.M); dev_free(w.AB);
+    dev_free(w.tW); dev_free(w.tA); dev_free(w.tClamp); dev_free(w.tLQ); dev_free(w.tF); dev_free(w.tRho);
+    dev_free(w.pb); dev_free(w.pb2); dev_free(w.Y); dev_free(w.M); dev_free(w.AB);
     dev_free(w.xsb); dev_free(w.gx); dev_free(w.gv); dev_free(w.CD); dev_free(w.nc); dev_free(w.r0b);
     dev_free(w.wkb); dev_free(w.drho); dev_free(w.dwk); dev_free(w.sb);
     w.cap = w.cks = w.tslots = w.iters = w.ndist = w.tab = 0;
@@ -1416,8 +1417,10 @@ int adj_reserve(ps_system *s, size_t cks, size_t tslots, size_t iters)
         (rc = dev_alloc(s, &w.tX, tslots * c)) || (rc = dev_alloc(s, &w.tP, tslots * c)) ||
         (rc = dev_alloc(s, &w.tA, tslots * c)) || (rc = dev_alloc(s, &w.tPit, ti * c)) ||
         (rc = dev_alloc(s, &w.tW, ti * c)) || (rc = dev_alloc(s, &w.tClamp, ti * c)) ||
+        (rc = dev_alloc(s, &w.tLQ, ti * c)) || (rc = dev_alloc(s, &w.tF, ti * c)) ||
+        (rc = dev_alloc(s, &w.tRho, ti * c)) ||
         (rc = dev_alloc(s, &w.pb, c)) || (rc = dev_alloc(s, &w.pb2, c)) || (rc = dev_alloc(s, &w.Y, c)) ||
-        (rc = dev_alloc(s, &w.SD, c)) || (rc = dev_alloc(s, &w.M, c)) || (rc = dev_alloc(s, &w.AB, c)) ||
+        (rc = dev_alloc(s, &w.M, c)) || (rc = dev_alloc(s, &w.AB, c)) ||
         (rc = dev_alloc(s, &w.xsb, c)) || (rc = dev_alloc(s, &w.gx, c)) || (rc = dev_alloc(s, &w.gv, c)) ||
         (rc = dev_alloc(s, &w.CD, c)) || (rc = dev_alloc(s, &w.nc, c)) || (rc = dev_alloc(s, &w.r0b, c)) ||
         (rc = dev_alloc(s, &w.wkb, c)) || (rc = dev_alloc(s, &w.drho, c)) || (rc = dev_alloc(s, &w.dwk, c)) ||
@@ -1490,15 +1493,21 @@ int adj_taped_substep(ps_system *s, float dt, size_t t)
     const bool uni = uniform_fluid(s);
     for (uint32_t it = 0; !rc && e == hipSuccess && it < I; it++) {
         const size_t q = (t * I + it) * c;
+        b.tape_lq = w.tLQ + q;
+        b.tape_rho = w.tRho + q;
         e = ps::launch_lambda(s->stream, b, s->kc, s->n, uni, s->wu, s->block[PS_K_LAMBDA]);
+        b.tape_lq = nullptr;
+        b.tape_rho = nullptr;
         cp(w.tPit + q, b.P, sizeof(float4) * n);
         b.clamp_rec = w.tClamp + q;
         b.wall_rec = w.tW + q;
+        b.tape_f = w.tF + q;
         if (e == hipSuccess)
             e = ps::launch_solve(s->stream, b, s->kc, s->n, features(s), uni, s->wu, s->it_sub,
                                  s->block[PS_K_SOLVE]);
         b.clamp_rec = nullptr;
         b.wall_rec = nullptr;
+        b.tape_f = nullptr;
         std::swap(b.P, b.Q);
         s->iters++;
         s->it_sub++;
@@ -1528,7 +1537,6 @@ int adj_reverse(ps_system *s, float dt, size_t t)
     bt.P = w.tP + t * c;
     ps::AdjBuffers a{};
     a.Y = w.Y;
-    a.SD = w.SD;
     a.M = w.M;
     a.AB = w.AB;
     a.CD = w.CD;
@@ -1554,6 +1562,9 @@ int adj_reverse(ps_system *s, float dt, size_t t)
         a.Pit = w.tPit + q;
         a.clamp = w.tClamp + q;
         a.wrec = w.tW + q;
+        a.SD = w.tLQ + q;
+        a.rho = w.tRho + q;
+        a.tf = w.tF + q;
         a.pbn = cur;
         a.pb = nxt;
         LAUNCH(s, PS_K_ADJ_PRE, ps::launch_adj(s->stream, ps::ADJ_PRE, bt, s->kc, a, n, dt, uni));
@@ -1711,6 +1722,9 @@ int ps_step_backward(ps_system *s, float dt, uint32_t substeps, uint32_t checkpo
     s->adj_active = false;
     s->b.clamp_rec = nullptr;
     s->b.wall_rec = nullptr;
+    s->b.tape_lq = nullptr;
+    s->b.tape_rho = nullptr;
+    s->b.tape_f = nullptr;
     if (rc) {
         s->stage = 0;
         s->it_sub = 0;

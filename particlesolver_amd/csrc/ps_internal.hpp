@@ -150,6 +150,9 @@ struct DevBuffers {
     uint8_t *clamp_rec;
     float4 *wall_rec;      // with clamp_rec: position before wall friction,
                            // normal code (n_x+1)*9 + (n_y+1)*3 + (n_z+1) in .w
+    float4 *tape_lq;       // lambda pass: S = sum_j a d (xyz), D (w)
+    float *tape_rho;       // lambda pass: density
+    float4 *tape_f;        // solve: fluid sum f (xyz), contact count bits (w)
     // CPU-reference mode only
     float *lam;            // lambda per particle
     uint32_t *bc;          // boundary flags | boundary count << 8
@@ -160,7 +163,7 @@ struct __align__(16) AdjAttr {
     float w;
     int32_t phase;
     uint32_t nc;  // contact count
-    float pad;
+    float mass;   // density multiplier as a neighbour: 1/w, times s_solid if not fluid
 };
 
 // Work buffers of the reverse pass (ps_adjoint.hip), storage order unless
@@ -171,8 +174,10 @@ struct AdjBuffers {
     const float4 *wrec;    // its wall-friction records
     const float4 *pbn;     // dL/dp after the iteration
     float4 *pb;            // dL/dp before it (finalize: of the final iterate)
-    float4 *Y;             // masked dL/dy (xyz) and density (w); finalize: dL/dS
-    float4 *SD;            // S = sum_j a d (xyz), D (w)
+    const float4 *SD;      // recorded lambda pass: S = sum_j a d (xyz), D (w)
+    const float *rho;      // recorded lambda pass: density
+    const float4 *tf;      // recorded solve: fluid sum f (xyz), contact count bits (w)
+    float4 *Y;             // masked dL/dy (xyz); finalize: dL/dS
     float4 *M;             // mu = dL/dy / rho0 (fluid) or dL/dy (xyz), 1/rho0 (w)
     AdjAttr *AB;           // attributes with contact counts (non-uniform scenes)
     float2 *CD;            // dL/dC, dL/dD

@@ -570,7 +570,8 @@ __global__ __launch_bounds__(kStageUnit) void neighbors_staged_k(DevBuffers b, K
 // degenerate pairs (r^2 < 1e-8) add density only
 // UNIFORM: every particle is fluid with inverse mass wu (no Attr gathers).
 
-template <bool UNIFORM, bool STAGED>
+// TAPE (reverse-pass recording): also store S, D and rho per particle.
+template <bool UNIFORM, bool STAGED, bool TAPE = false>
 __global__ __launch_bounds__(kMaxBlock) void lambda_k(DevBuffers b, KConst k, uint32_t n, float mu)
 {
     __shared__ float4 lds[STAGED ? kStageSlots : 1];
@@ -589,6 +590,10 @@ __global__ __launch_bounds__(kMaxBlock) void lambda_k(DevBuffers b, KConst k, ui
     const Attr a = b.A[i];
     if (!UNIFORM && a.phase != PS_PHASE_FLUID) {
         Pw[4 * (size_t)i + 3] = 0.0f;
+        if (TAPE) {
+            b.tape_lq[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            b.tape_rho[i] = 0.0f;
+        }
         return;
     }
     const float4 p = b.P[i];
@@ -636,6 +641,10 @@ __global__ __launch_bounds__(kMaxBlock) void lambda_k(DevBuffers b, KConst k, ui
     den = den + dist2(gix, giy, giz);
     float C = (rho / r0) - 1.0f;
     Pw[4 * (size_t)i + 3] = -C / (den + k.relax);
+    if (TAPE) {
+        b.tape_lq[i] = make_float4(gix, giy, giz, den);
+        b.tape_rho[i] = rho;
+    }
 }
 
 
@@ -644,7 +653,9 @@ __global__ __launch_bounds__(kMaxBlock) void lambda_k(DevBuffers b, KConst k, ui
 // (a6), distance (a9), shape goal (a12) summed in that order from the same
 // snapshot P, then walls (a8) and pins (a10); result to Q.
 
-template <uint32_t F, bool UNIFORM, uint32_t NC, bool STAGED>
+// TAPE (reverse-pass recording): also store the fluid sum f (before the
+// division by rho0) and the contact count per particle.
+template <uint32_t F, bool UNIFORM, uint32_t NC, bool STAGED, bool TAPE = false>
 __global__ __launch_bounds__(kMaxBlock) void solve_k(DevBuffers b, KConst k, uint32_t n, float wu,
                                                   uint32_t it)
 {
@@ -667,6 +678,7 @@ __global__ __launch_bounds__(kMaxBlock) void solve_k(DevBuffers b, KConst k, uin
     const float wi = UNIFORM ? wu : a.w;
     const uint32_t c = b.ncnt[i];
     float ax = 0.0f, ay = 0.0f, az = 0.0f;
+    float4 tape = make_float4(0.0f, 0.0f, 0.0f, 0.0f);  // (f, contact count) when TAPE
 
     // a7: particle contacts + friction (collideD, integration_kernel.cuh:358-462)
     if (F & F_CONTACT) {
@@ -681,6 +693,7 @@ __global__ __launch_bounds__(kMaxBlock) void solve_k(DevBuffers b, KConst k, uin
                 float r2 = ((dx * dx) + (dy * dy)) + (dz * dz);
                 if (r2 < k.collide_r2) nc++;
             }
+            if (TAPE) tape.w = __uint_as_float(nc);
             if (nc > 0) {
                 const float fnc = (float)nc;
                 const float ewi = wi * ps_exp(py);
@@ -791,6 +804,11 @@ __global__ __launch_bounds__(kMaxBlock) void solve_k(DevBuffers b, KConst k, uin
             ax = ax + (fx * ir0);
             ay = ay + (fy * ir0);
             az = az + (fz * ir0);
+            if (TAPE) {
+                tape.x = fx;
+                tape.y = fy;
+                tape.z = fz;
+            }
         }
     }
 
@@ -852,9 +870,10 @@ __global__ __launch_bounds__(kMaxBlock) void solve_k(DevBuffers b, KConst k, uin
             set |= 7u | kPinned;
         }
     }
-    if (b.clamp_rec) {
+    if (TAPE) {
         b.clamp_rec[i] = (uint8_t)set;
         if (set & kWallFriction) b.wall_rec[i] = wrec;
+        b.tape_f[i] = tape;
     }
     b.Q[i] = make_float4(nxp, nyp, nzp, 0.0f);
 }
@@ -1111,7 +1130,11 @@ hipError_t launch_lambda(hipStream_t st, const DevBuffers &b, const KConst &k, u
     if (n == 0) return hipSuccess;
     const dim3 g(blocks_for(n, bs)), t(bs);
     const float mu = 1.0f / wu;
-    if (uniform && b.staging)
+    if (b.tape_lq && uniform)  // recording for the reverse pass (never staged)
+        hipLaunchKernelGGL((lambda_k<true, false, true>), g, t, 0, st, b, k, n, mu);
+    else if (b.tape_lq)
+        hipLaunchKernelGGL((lambda_k<false, false, true>), g, t, 0, st, b, k, n, mu);
+    else if (uniform && b.staging)
         hipLaunchKernelGGL((lambda_k<true, true>), dim3(blocks_for(n, kStageUnit)), dim3(kStageUnit),
                            0, st, b, k, n, mu);
     else if (uniform)
@@ -1139,6 +1162,17 @@ static void solve_variant(hipStream_t st, const DevBuffers &b, const KConst &k, 
     const bool n4 = k.n_corr == 4u;
     // the staged variant exists for the uniform pure-fluid composition only
     constexpr bool CAN_STAGE = F == F_FLUID;
+    if (b.tape_f) {  // recording for the reverse pass (never staged)
+        if (uniform && n4)
+            hipLaunchKernelGGL((solve_k<F, true, 4, false, true>), g, t, 0, st, b, k, n, wu, iter);
+        else if (uniform)
+            hipLaunchKernelGGL((solve_k<F, true, 0, false, true>), g, t, 0, st, b, k, n, wu, iter);
+        else if (n4)
+            hipLaunchKernelGGL((solve_k<F, false, 4, false, true>), g, t, 0, st, b, k, n, wu, iter);
+        else
+            hipLaunchKernelGGL((solve_k<F, false, 0, false, true>), g, t, 0, st, b, k, n, wu, iter);
+        return;
+    }
     if (CAN_STAGE && uniform && b.staging && n4)
         hipLaunchKernelGGL((solve_k<F, true, 4, CAN_STAGE>), gs, ts, 0, st, b, k, n, wu, iter);
     else if (CAN_STAGE && uniform && b.staging)
