@@ -24,9 +24,11 @@ __device__ __forceinline__ const uint4 *nbr_lane(const DevBuffers &b, uint32_t i
 // Visit the first c neighbours of particle i in table order, calling
 // fn(j, gather(j)) for each.  Software-pipelined: while chunk m is being
 // processed, the four gathers of chunk m+1 and the index load of chunk m+2
-// are already in flight.  Every entry of a chunk at or below the last one is
-// a valid slot (the builder pads the last chunk with the particle's own
-// slot), so all loads are unconditional; only fn is guarded.
+// are already in flight.  The loop is unrolled over two chunks whose record
+// sets swap roles, so no records are copied between registers.  Every entry
+// of a chunk at or below the last one is a valid slot (the builder pads the
+// last chunk with the particle's own slot), so all loads are unconditional;
+// only fn is guarded.
 template <typename G, typename Fn>
 __device__ __forceinline__ void for_neighbors(const DevBuffers &b, uint32_t i, uint32_t c, G gather,
                                               Fn fn)
@@ -34,29 +36,39 @@ __device__ __forceinline__ void for_neighbors(const DevBuffers &b, uint32_t i, u
     if (c == 0) return;
     const uint4 *nb = nbr_lane(b, i);
     const uint32_t last = (c - 1) >> 2;
-    uint4 jj = nb[0];
-    uint4 jn = nb[(size_t)min(1u, last) * 64];
-    auto d0 = gather(jj.x);
-    auto d1 = gather(jj.y);
-    auto d2 = gather(jj.z);
-    auto d3 = gather(jj.w);
-    for (uint32_t m = 0; m <= last; m++) {
-        const uint4 jnn = nb[(size_t)min(m + 2, last) * 64];
-        auto e0 = gather(jn.x);
-        auto e1 = gather(jn.y);
-        auto e2 = gather(jn.z);
-        auto e3 = gather(jn.w);
-        const uint32_t q = m << 2;
+    auto chunk = [&](uint32_t m) { return nb[(size_t)min(m, last) * 64]; };
+    auto visit = [&](const uint4 &jj, uint32_t q, const auto &d0, const auto &d1, const auto &d2,
+                     const auto &d3) {
         fn(jj.x, d0);
         if (q + 1 < c) fn(jj.y, d1);
         if (q + 2 < c) fn(jj.z, d2);
         if (q + 3 < c) fn(jj.w, d3);
-        jj = jn;
-        jn = jnn;
-        d0 = e0;
-        d1 = e1;
-        d2 = e2;
-        d3 = e3;
+    };
+    uint4 ja = nb[0];
+    uint4 jb = chunk(1);
+    auto a0 = gather(ja.x);
+    auto a1 = gather(ja.y);
+    auto a2 = gather(ja.z);
+    auto a3 = gather(ja.w);
+    for (uint32_t m = 0;; m += 2) {
+        // records of chunk m are in a*; chunk m+1 goes in flight
+        const uint4 jc = chunk(m + 2);
+        auto b0 = gather(jb.x);
+        auto b1 = gather(jb.y);
+        auto b2 = gather(jb.z);
+        auto b3 = gather(jb.w);
+        visit(ja, m << 2, a0, a1, a2, a3);
+        if (m + 1 > last) break;
+        // chunk m+2 goes in flight into a*; chunk m+1 is processed
+        const uint4 jd = chunk(m + 3);
+        a0 = gather(jc.x);
+        a1 = gather(jc.y);
+        a2 = gather(jc.z);
+        a3 = gather(jc.w);
+        visit(jb, (m + 1) << 2, b0, b1, b2, b3);
+        if (m + 2 > last) break;
+        ja = jc;
+        jb = jd;
     }
 }
 

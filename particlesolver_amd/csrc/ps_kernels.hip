@@ -773,17 +773,18 @@ __global__ __launch_bounds__(kMaxBlock) void solve_k(DevBuffers b, KConst k, uin
                 float W = k.poly6 * ((t * t) * t);
                 float corr = -(k.k_corr * ipow(W * k.inv_w_dq, NC ? NC : k.n_corr));
                 float coef = (li + pj.w) + corr;
-                // selects rather than branches keep fx/fy/fz in registers; a
-                // degenerate pair adds exact zeros (tiny finite d, sums that
-                // start at +0 never become -0) before its own y term
-                const bool deg = r2 < kDegenerateR2;
                 const float ri = rsq(r2);
                 const float qq = k.h - (r2 * ri);
-                const float sc = deg ? 0.0f : coef * ((k.spiky * (qq * qq)) * ri);
-                fx = __builtin_fmaf(-sc, dx, fx);
-                fy = __builtin_fmaf(-sc, dy, fy);
-                fz = __builtin_fmaf(-sc, dz, fz);
-                fy = deg ? (fy - (coef * gdeg)) : fy;
+                const float sc = coef * ((k.spiky * (qq * qq)) * ri);
+                // a degenerate pair (essentially never taken: one compare per
+                // pair) has only its y term
+                if (r2 < kDegenerateR2) {
+                    fy = fy - (coef * gdeg);
+                } else {
+                    fx = __builtin_fmaf(-sc, dx, fx);
+                    fy = __builtin_fmaf(-sc, dy, fy);
+                    fz = __builtin_fmaf(-sc, dz, fz);
+                }
             };
             if (STAGED && staged) {
                 for_neighbors(
