@@ -234,6 +234,18 @@ int ps_add_scene_particles(ps_system *sys, uint32_t scene, uint32_t n, const flo
 int ps_add_fluid_object(ps_system *sys, uint32_t n, const float *pos, const float *vel,
                         const float *inv_mass, float rest_density, uint32_t *first_out);
 
+/* Add one gas object (PS_FLAG_CPU_REFERENCE mode, dims = 2): n particles of
+ * phase PS_PHASE_GAS forming one GasConstraint of the given rest density,
+ * open (drag towards the rest density) or closed.  Replaces
+ * Simulation::createGas (cpu/src/simulation.cpp:408-428) and the
+ * GasConstraint it creates (cpu/src/constraint/gasconstraint.cpp:3-114):
+ * the fluid constraint with the gas constants (gasconstraint.h:10-19), a
+ * per-particle force from the open-gas drag and the vorticity term applied
+ * at the next prediction, and gas gravity scaled by ALPHA = -0.2
+ * (simulation.cpp:143-154).  PS_ERR_UNSUPPORTED outside that mode or in 3D. */
+int ps_add_gas_object(ps_system *sys, uint32_t n, const float *pos, const float *vel,
+                      const float *inv_mass, float rest_density, int open, uint32_t *first_out);
+
 /* Replaces addDistanceConstraint + updateOccurences (solver.cu:72-106,125-156).
  * pairs: 2*m particle indices; rest: m lengths; stiffness: m values in (0,1]
  * (NULL means 1). */

@@ -267,6 +267,8 @@ class Ref2D:
         self._f("r2d_step").argtypes = [ctypes.c_void_p, self.real, ctypes.c_uint32]
         self._f("r2d_add_fluid_group").argtypes = [ctypes.c_void_p, ctypes.c_uint32,
                                                    ctypes.c_uint32, ctypes.c_float]
+        self._f("r2d_add_gas_group").argtypes = [ctypes.c_void_p, ctypes.c_uint32,
+                                                 ctypes.c_uint32, ctypes.c_float, ctypes.c_int]
         self.n = 0
 
     def close(self):
@@ -296,6 +298,30 @@ class Ref2D:
             raise RuntimeError("r2d_add_fluid_group failed: %d" % rc)
         self.n += n
         return first
+
+    def add_gas_object(self, pos, vel=None, inv_mass=None, rest_density=1.0, open=False):
+        """createGas (simulation.cpp:408-428): a GasConstraint object, 2D"""
+        pos = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1, 3)
+        n = pos.shape[0]
+        vel = None if vel is None else np.ascontiguousarray(vel, dtype=np.float32).reshape(n, 3)
+        w = np.ascontiguousarray(np.broadcast_to(1.0 if inv_mass is None else inv_mass, (n,)),
+                                 dtype=np.float32)
+        ph = np.full(n, 1, np.int32)
+        first = self.n
+        rc = self._f("r2d_add_particles")(self.h, n, _ptr(pos), _ptr(vel), _ptr(w), _ptr(ph))
+        if rc:
+            raise RuntimeError("r2d_add_particles failed: %d" % rc)
+        rc = self._f("r2d_add_gas_group")(self.h, first, n, float(rest_density), int(bool(open)))
+        if rc:
+            raise RuntimeError("r2d_add_gas_group failed: %d" % rc)
+        self.n += n
+        return first
+
+    def forces(self):
+        """the force each particle carries into the next prediction"""
+        f = np.zeros((self.n, 3), self.dtype)
+        self._f("r2d_read_force")(self.h, _ptr(f))
+        return f
 
     def step(self, dt, steps=1):
         self._f("r2d_step")(self.h, dt, steps)

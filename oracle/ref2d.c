@@ -10,8 +10,12 @@
  * BoundaryConstraint::project, cpu/src/constraint/boundaryconstraint.cpp:
  * 14-93) then the STANDARD group (one TotalFluidConstraint::project per
  * fluid object in creation order, cpu/src/constraint/totalfluidconstraint.cpp:
- * 41-116), velocity v = (ep - p)/dt and sleeping (Particle::confirmGuess,
- * cpu/src/particle.h:60-65).  The reference's compile-time constants
+ * 41-116, and one GasConstraint::project per gas object,
+ * cpu/src/constraint/gasconstraint.cpp:31-114), velocity v = (ep - p)/dt and
+ * sleeping (Particle::confirmGuess, cpu/src/particle.h:60-65).  Gas objects
+ * (2D, as the reference's dvec2 forms) also accumulate a per-particle force
+ * (open-gas drag and the vorticity term) that the next prediction applies,
+ * with gas gravity scaled by ALPHA (simulation.cpp:143-154, simulation.h:21).  The reference's compile-time constants
  * (PARTICLE_RAD, H, RELAXATION, K_P, E_P, DQ_P, S_SOLID) are runtime
  * parameters; frand() jitter becomes the counter RNG of ps_oracle.c; the
  * vector operations follow glm's order (normalize(r) = r * (1/sqrt(r.r))).
@@ -67,14 +71,27 @@ static real uniform(uint64_t seed, uint64_t iter, uint32_t i, uint32_t axis)
 typedef struct r2d_sys r2d_sys;
 static real poly6(const r2d_sys *s, real r2);
 
+/* GasConstraint's compile-time constants (gasconstraint.h:10-19) and the
+ * gas gravity scale (simulation.h:21) */
+#define GAS_RELAXATION 0.01
+#define GAS_K_P 0.2
+#define GAS_E_P 4u
+#define GAS_DQ_P 0.25
+#define GAS_S_SOLID 0.5
+#define GAS_ALPHA (-0.2)
+#define GAS_OPEN_DRAG (-50.0)
+
 struct r2d_sys {
     ps_params prm;
     uint32_t n, cap, ngroup;
     real *p, *ep, *v; /* 3 per particle */
+    real *force;      /* 3 per particle: applied at the next prediction */
     real *w;
     int32_t *phase;
-    int32_t *group; /* fluid object, -1 none */
+    int32_t *group; /* fluid or gas object, -1 none */
     real *group_rho0;
+    int32_t *group_gas; /* 0 fluid, 1 closed gas, 2 open gas */
+    real g_wdq;         /* poly6(DQ_P^2 H^2) of the gas constraint */
     uint32_t *bflags, *counts;
     real *lambda, *delta;
     uint32_t *nnb;
@@ -91,6 +108,8 @@ EXPORT int SFX(r2d_create)(const ps_params *prm, r2d_sys **out)
     s->p = (real *)calloc(3 * c, sizeof(real));
     s->ep = (real *)calloc(3 * c, sizeof(real));
     s->v = (real *)calloc(3 * c, sizeof(real));
+    s->force = (real *)calloc(3 * c, sizeof(real));
+    s->group_gas = (int32_t *)calloc(c, sizeof(int32_t));
     s->w = (real *)calloc(c, sizeof(real));
     s->phase = (int32_t *)calloc(c, sizeof(int32_t));
     s->group = (int32_t *)calloc(c, sizeof(int32_t));
@@ -111,6 +130,9 @@ EXPORT int SFX(r2d_create)(const ps_params *prm, r2d_sys **out)
     s->h = R(prm->h);
     s->h2 = R(h2);
     s->w_dq = R(poly6 * ((t * t) * t));
+    /* poly6(DQ_P * DQ_P * H * H), gasconstraint.cpp:94 */
+    const double tg = h2 - (((GAS_DQ_P * GAS_DQ_P) * h) * h);
+    s->g_wdq = R(poly6 * ((tg * tg) * tg));
     s->radius = R(prm->radius);
     s->relax = R(prm->relaxation);
     s->k = R(prm->k_corr);
@@ -143,8 +165,8 @@ EXPORT void SFX(r2d_set_threads)(int n)
 EXPORT void SFX(r2d_destroy)(r2d_sys *s)
 {
     if (!s) return;
-    void *ptrs[] = {s->p, s->ep, s->v, s->w, s->phase, s->group, s->group_rho0, s->bflags,
-                    s->counts, s->lambda, s->delta, s->nnb};
+    void *ptrs[] = {s->p, s->ep, s->v, s->force, s->w, s->phase, s->group, s->group_rho0,
+                    s->group_gas, s->bflags, s->counts, s->lambda, s->delta, s->nnb};
     for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) free(ptrs[i]);
     free(s);
 }
@@ -174,7 +196,32 @@ EXPORT int SFX(r2d_add_fluid_group)(r2d_sys *s, uint32_t first, uint32_t n, floa
     if ((uint64_t)first + n > s->n) return PS_ERR_INVALID_ARGUMENT;
     for (uint32_t i = first; i < first + n; i++) s->group[i] = (int32_t)s->ngroup;
     s->group_rho0[s->ngroup] = rho0;
+    s->group_gas[s->ngroup] = 0;
     s->ngroup++;
+    return PS_OK;
+}
+
+/* createGas (cpu/src/simulation.cpp:408-428): particles [first, first+n)
+ * form one GasConstraint of rest density rho0, open (drag towards rest
+ * density) or closed; 2D only (the constraint's forms are the reference's
+ * dvec2 ones) */
+EXPORT int SFX(r2d_add_gas_group)(r2d_sys *s, uint32_t first, uint32_t n, float rho0, int open)
+{
+    if ((uint64_t)first + n > s->n || s->prm.dims != 2) return PS_ERR_INVALID_ARGUMENT;
+    for (uint32_t i = first; i < first + n; i++) {
+        s->group[i] = (int32_t)s->ngroup;
+        s->phase[i] = PS_PHASE_GAS;
+    }
+    s->group_rho0[s->ngroup] = rho0;
+    s->group_gas[s->ngroup] = open ? 2 : 1;
+    s->ngroup++;
+    return PS_OK;
+}
+
+/* the force each particle carries into the next prediction */
+EXPORT int SFX(r2d_read_force)(const r2d_sys *s, real *force)
+{
+    memcpy(force, s->force, sizeof(real) * 3 * s->n);
     return PS_OK;
 }
 
@@ -297,6 +344,99 @@ static void fluid_group(r2d_sys *s, int32_t g)
     }
 }
 
+/* GasConstraint::project for gas object g (gasconstraint.cpp:31-114): the
+ * fluid constraint's lambda with the gas constants (self gradient summed
+ * over every neighbour without the solid weight, :220-232), the open-gas
+ * drag f += v (1 - rho/rho0) (-50) (:80), deltas with the gas tensile term
+ * and the vorticity force f += sum_j cross((0,0,|G_j * v_j|), r) W(r^2)
+ * (:96-103; G_j = spikyGrad(r, r.r), the squared length where the length
+ * is expected, and G_j * v_j component-wise, as written there), then
+ * ep += delta / (neighbours incl. self + counts). */
+static void gas_group(r2d_sys *s, int32_t g)
+{
+    const uint32_t n = s->n;
+    const real p0 = s->group_rho0[g];
+    const int open = s->group_gas[g] == 2;
+#pragma omp parallel for schedule(static)
+    for (uint32_t i = 0; i < n; i++) {
+        if (s->group[i] != g) continue;
+        real pi = R(0), den = R(0);
+        real gi[3] = {R(0), R(0), R(0)};
+        for (uint32_t j = 0; j < n; j++) {
+            if (j != i) {
+                if (s->w[j] == R(0)) continue;
+                real r[3] = {s->ep[3 * i] - s->ep[3 * j], s->ep[3 * i + 1] - s->ep[3 * j + 1],
+                             s->ep[3 * i + 2] - s->ep[3 * j + 2]};
+                real r2 = ((r[0] * r[0]) + (r[1] * r[1])) + (r[2] * r[2]);
+                if (r2 < s->h2) {
+                    real incr = poly6(s, r2) / s->w[j];
+                    if (s->phase[j] == PS_PHASE_SOLID) incr = incr * R(GAS_S_SOLID);
+                    pi = pi + incr;
+                    real l = length3(r), sg[3];
+                    spiky_grad(s, r, l, sg);
+                    real gr[3];
+                    for (int a = 0; a < 3; a++) gr[a] = (-sg[a]) / p0;
+                    den = den + (((gr[0] * gr[0]) + (gr[1] * gr[1])) + (gr[2] * gr[2]));
+                    for (int a = 0; a < 3; a++) gi[a] = gi[a] + sg[a];
+                }
+            } else {
+                pi = pi + (poly6(s, R(0)) / s->w[i]);
+            }
+        }
+        real gr[3] = {gi[0] / p0, gi[1] / p0, gi[2] / p0};
+        den = den + (((gr[0] * gr[0]) + (gr[1] * gr[1])) + (gr[2] * gr[2]));
+        real p_rat = pi / p0;
+        if (open)
+            for (int a = 0; a < 2; a++)
+                s->force[3 * i + a] = s->force[3 * i + a] + ((s->v[3 * i + a] * (R(1) - p_rat)) * R(GAS_OPEN_DRAG));
+        s->lambda[i] = -(p_rat - R(1)) / (den + R(GAS_RELAXATION));
+    }
+#pragma omp parallel for schedule(static)
+    for (uint32_t i = 0; i < n; i++) {
+        if (s->group[i] != g) continue;
+        real d[3] = {R(0), R(0), R(0)};
+        real fv[2] = {R(0), R(0)};
+        uint32_t cnt = 0;
+        for (uint32_t j = 0; j < n; j++) {
+            if (j == i) {
+                cnt++;
+                continue;
+            }
+            if (s->w[j] == R(0)) continue;
+            real r[3] = {s->ep[3 * i] - s->ep[3 * j], s->ep[3 * i + 1] - s->ep[3 * j + 1],
+                         s->ep[3 * i + 2] - s->ep[3 * j + 2]};
+            real r2 = ((r[0] * r[0]) + (r[1] * r[1])) + (r[2] * r[2]);
+            if (!(r2 < s->h2)) continue;
+            cnt++;
+            real l = length3(r), sg[3];
+            spiky_grad(s, r, l, sg);
+            real x = poly6(s, l * l) / s->g_wdq;
+            real corr = -(R(GAS_K_P) * ipow_corr(x, GAS_E_P));
+            real lj = s->group[j] == g ? s->lambda[j] : R(0);
+            real coef = (s->lambda[i] + lj) + corr;
+            for (int a = 0; a < 3; a++) d[a] = d[a] + (coef * sg[a]);
+            /* vorticity */
+            real vg[3];
+            spiky_grad(s, r, r2, vg);
+            real wv[2] = {vg[0] * s->v[3 * j], vg[1] * s->v[3 * j + 1]};
+            real W = SQRT((wv[0] * wv[0]) + (wv[1] * wv[1]));
+            real cx = (R(0) * R(0)) - (r[1] * W), cy = (W * r[0]) - (R(0) * R(0));
+            real p6 = poly6(s, r2);
+            fv[0] = fv[0] + (cx * p6);
+            fv[1] = fv[1] + (cy * p6);
+        }
+        for (int a = 0; a < 3; a++) s->delta[3 * i + a] = d[a] / p0;
+        s->nnb[i] = cnt;
+        s->force[3 * i] = s->force[3 * i] + fv[0];
+        s->force[3 * i + 1] = s->force[3 * i + 1] + fv[1];
+    }
+    for (uint32_t i = 0; i < n; i++) {
+        if (s->group[i] != g) continue;
+        real c = (real)(s->nnb[i] + s->counts[i]);
+        for (int a = 0; a < 3; a++) s->ep[3 * i + a] = s->ep[3 * i + a] + (s->delta[3 * i + a] / c);
+    }
+}
+
 /* BoundaryConstraint::project (boundaryconstraint.cpp:14-93), one axis */
 static void boundary(r2d_sys *s, uint32_t i, int axis, int greater, uint64_t iter)
 {
@@ -341,10 +481,15 @@ EXPORT int SFX(r2d_step)(r2d_sys *s, real dt, uint32_t steps)
     for (uint32_t st = 0; st < steps; st++) {
         /* (1)-(5) predict; (8) boundary contacts from the predicted positions */
         for (uint32_t i = 0; i < s->n; i++) {
-            real *v = &s->v[3 * i], *p = &s->p[3 * i], *e = &s->ep[3 * i];
-            v[0] = v[0] + (dt * gx);
-            v[1] = v[1] + (dt * gy);
-            if (!twod) v[2] = v[2] + (dt * gz);
+            real *v = &s->v[3 * i], *p = &s->p[3 * i], *e = &s->ep[3 * i], *fo = &s->force[3 * i];
+            /* v = v + dt g' + dt f, g' = g ALPHA for gas; f = 0 */
+            real g3[3] = {gx, gy, gz};
+            if (s->phase[i] == PS_PHASE_GAS)
+                for (int a = 0; a < 3; a++) g3[a] = g3[a] * R(GAS_ALPHA);
+            v[0] = (v[0] + (dt * g3[0])) + (dt * fo[0]);
+            v[1] = (v[1] + (dt * g3[1])) + (dt * fo[1]);
+            if (!twod) v[2] = (v[2] + (dt * g3[2])) + (dt * fo[2]);
+            fo[0] = fo[1] = fo[2] = R(0);
             for (int a = 0; a < 3; a++)
                 e[a] = s->w[i] == R(0) ? p[a] : ((twod && a == 2) ? p[a] : (p[a] + (dt * v[a])));
             uint32_t f = 0;
@@ -372,7 +517,12 @@ EXPORT int SFX(r2d_step)(r2d_sys *s, real dt, uint32_t steps)
                 if (f & BZ_LOW) boundary(s, i, 2, 1, s->iters);
                 if (f & BZ_HIGH) boundary(s, i, 2, 0, s->iters);
             }
-            for (int32_t g = 0; g < (int32_t)s->ngroup; g++) fluid_group(s, g);
+            for (int32_t g = 0; g < (int32_t)s->ngroup; g++) {
+                if (s->group_gas[g])
+                    gas_group(s, g);
+                else
+                    fluid_group(s, g);
+            }
             s->iters++;
         }
         /* (23)-(28) velocities, sleeping */

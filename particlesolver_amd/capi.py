@@ -135,6 +135,7 @@ PROTOTYPES = {
     "ps_last_error": (ctypes.c_char_p, [_P]),
     "ps_add_particles": (_int, [_P, _u32, _P, _P, _P, _P, _P, ctypes.POINTER(_u32)]),
     "ps_add_fluid_object": (_int, [_P, _u32, _P, _P, _P, _f32, ctypes.POINTER(_u32)]),
+    "ps_add_gas_object": (_int, [_P, _u32, _P, _P, _P, _f32, _int, ctypes.POINTER(_u32)]),
     "ps_add_distance_constraints": (_int, [_P, _u32, _P, _P, _P]),
     "ps_add_point_constraints": (_int, [_P, _u32, _P, _P]),
     "ps_add_rigid_body": (_int, [_P, _u32, _P, _f32, ctypes.POINTER(_u32)]),
@@ -282,6 +283,19 @@ class System:
         first = ctypes.c_uint32()
         self._check(self.lib.ps_add_fluid_object(self.h, n, _ptr(pos), _ptr(vel), _ptr(w),
                                                  float(rest_density), ctypes.byref(first)))
+        return first.value
+
+    def add_gas_object(self, pos, vel=None, inv_mass=None, rest_density=1.0, open=False):
+        """ps_add_gas_object: one GasConstraint object (CPU-reference mode, 2D)."""
+        pos = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1, 3)
+        n = pos.shape[0]
+        vel = None if vel is None else np.ascontiguousarray(vel, dtype=np.float32).reshape(n, 3)
+        w = np.ascontiguousarray(np.broadcast_to(1.0 if inv_mass is None else inv_mass, (n,)),
+                                 dtype=np.float32)
+        first = ctypes.c_uint32()
+        self._check(self.lib.ps_add_gas_object(self.h, n, _ptr(pos), _ptr(vel), _ptr(w),
+                                               float(rest_density), int(bool(open)),
+                                               ctypes.byref(first)))
         return first.value
 
     def add_distance_constraints(self, pairs, rest, stiffness=None):

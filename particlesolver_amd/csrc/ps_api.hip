@@ -71,6 +71,7 @@ struct ps_system {
     struct FluidObject {
         uint32_t first, count;
         float rho0;
+        int gas;  // 0 fluid, 1 closed gas, 2 open gas
     };
     std::vector<FluidObject> objects;
     bool cr_bnd_done = false;  // boundaries of iteration it_sub already projected
@@ -213,6 +214,9 @@ void fill_crconst(const ps_params &p, const ps_derived &d, ps::CrConst &c)
     c.poly6 = (float)poly6;
     c.spiky = (float)(45.0 / (pi * h6));
     c.w_dq = (float)(poly6 * ((t * t) * t));
+    // GasConstraint: poly6(DQ_P * DQ_P * H * H), DQ_P = .25 (gasconstraint.h:16, .cpp:94)
+    const double tg = h2 - (((0.25 * 0.25) * h) * h);
+    c.g_wdq = (float)(poly6 * ((tg * tg) * tg));
     c.relax = p.relaxation;
     c.k_corr = p.k_corr;
     c.jitter = d.jitter;
@@ -345,6 +349,7 @@ void free_all(ps_system *s)
     dev_free(b.body_rest); dev_free(b.body_stiff); dev_free(b.body_mass); dev_free(b.body_q);
     dev_free(b.body_goal); dev_free(b.body_of); dev_free(b.member_of); dev_free(b.status);
     dev_free(b.sid); dev_free(b.sid2); dev_free(b.scenes); dev_free(b.lam); dev_free(b.bc);
+    dev_free(b.force);
     dev_free(b.stage);
     adj_free(s);
 }
@@ -531,9 +536,9 @@ int cr_iterate(ps_system *s)
     for (size_t k = 0; k < nobj; k++) {
         const ps_system::FluidObject &o = s->objects[k];
         LAUNCH(s, PS_K_LAMBDA,
-               ps::launch_cr_lambda(s->stream, s->b, s->cr, s->n, o.first, o.count, o.rho0));
+               ps::launch_cr_lambda(s->stream, s->b, s->cr, s->n, o.first, o.count, o.rho0, o.gas));
         LAUNCH(s, PS_K_SOLVE,
-               ps::launch_cr_delta(s->stream, s->b, s->cr, s->n, o.first, o.count, o.rho0,
+               ps::launch_cr_delta(s->stream, s->b, s->cr, s->n, o.first, o.count, o.rho0, o.gas,
                                    k + 1 == nobj ? next : -1));
         std::swap(s->b.P, s->b.Q);
         s->cr_swapped = !s->cr_swapped;
@@ -840,6 +845,7 @@ static int create_impl(const ps_params *params, uint32_t nscenes, const ps_scene
     if (cpuref) {
         ALLOC(b.lam, c);
         ALLOC(b.bc, c);
+        ALLOC(b.force, c);
     }
 #undef ALLOC
     hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
@@ -980,7 +986,28 @@ int ps_add_fluid_object(ps_system *s, uint32_t n, const float *pos, const float 
     uint32_t first = 0;
     int rc = add_particles_impl(s, 0, n, pos, vel, inv_mass, phase.data(), rho.data(), &first);
     if (rc) return rc;
-    if (s->cpuref && n) s->objects.push_back({first, n, rest_density});
+    if (s->cpuref && n) s->objects.push_back({first, n, rest_density, 0});
+    if (first_out) *first_out = first;
+    return PS_OK;
+}
+
+int ps_add_gas_object(ps_system *s, uint32_t n, const float *pos, const float *vel,
+                      const float *inv_mass, float rest_density, int open, uint32_t *first_out)
+{
+    if (int rc = check_sys(s)) return rc;
+    if (!s->cpuref || s->prm.dims != 2)
+        return fail(s, PS_ERR_UNSUPPORTED, "gas objects belong to the 2D CPU-reference mode");
+    if (!(rest_density > 0.0f) || !is_finite(rest_density))
+        return fail(s, PS_ERR_INVALID_ARGUMENT, "gas rest density must be > 0");
+    for (uint32_t k = 0; k < n && inv_mass; k++)
+        if (inv_mass[k] == 0.0f)  // simulation.cpp:418-421
+            return fail(s, PS_ERR_INVALID_ARGUMENT, "a gas particle cannot have infinite mass");
+    std::vector<int32_t> phase(n, PS_PHASE_GAS);
+    std::vector<float> rho(n, rest_density);
+    uint32_t first = 0;
+    int rc = add_particles_impl(s, 0, n, pos, vel, inv_mass, phase.data(), rho.data(), &first);
+    if (rc) return rc;
+    if (n) s->objects.push_back({first, n, rest_density, open ? 2 : 1});
     if (first_out) *first_out = first;
     return PS_OK;
 }
@@ -1042,6 +1069,7 @@ static int add_particles_impl(ps_system *s, uint32_t scene, uint32_t n, const fl
     HIP_TRY(s, hipMemcpy(b.V + o, V.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
     HIP_TRY(s, hipMemcpy(b.A + o, A.data(), sizeof(Attr) * n, hipMemcpyHostToDevice));
     HIP_TRY(s, hipMemcpy(b.slot_of + o, slot.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice));
+    if (b.force) HIP_TRY(s, hipMemset(b.force + o, 0, sizeof(float4) * n));
     if (s->scene_count[scene] == 0) {
         s->scenes[scene].first = s->n;
         HIP_TRY(s, hipMemcpy(b.scenes + scene, &s->scenes[scene], sizeof(ps::SceneDev),

@@ -8,8 +8,13 @@
 //     count towards the particle's Jacobi divisor (:232-245);
 //   - each iteration projects the CONTACT group (BoundaryConstraint::project,
 //     cpu/src/constraint/boundaryconstraint.cpp:14-93) then the STANDARD
-//     group: one TotalFluidConstraint per fluid object, in creation order
-//     (cpu/src/constraint/totalfluidconstraint.cpp:41-116);
+//     group: one TotalFluidConstraint per fluid object and one GasConstraint
+//     per gas object, in creation order
+//     (cpu/src/constraint/totalfluidconstraint.cpp:41-116,
+//     cpu/src/constraint/gasconstraint.cpp:31-114);
+//   - gas objects add the open-gas drag and a vorticity term to a
+//     per-particle force, which the next prediction applies together with
+//     gravity scaled by ALPHA for gas particles (simulation.cpp:143-154);
 //   - v = (ep - p)/dt, sleeping below |ep - p| < 1e-4 (:322-333,
 //     particle.h:60-65).
 // The fluid object scans every particle for neighbours (O(N^2), as the
@@ -38,6 +43,11 @@ constexpr uint32_t kBlock = 256;
 constexpr uint32_t kWaves = kBlock / 64;
 
 enum : uint32_t { BX_LOW = 1u, BX_HIGH = 2u, BY_LOW = 4u, BY_HIGH = 8u, BZ_LOW = 16u, BZ_HIGH = 32u };
+
+// GasConstraint's compile-time constants (gasconstraint.h:10-19) and the gas
+// gravity scale ALPHA (simulation.h:21), rounded once to float
+constexpr float kGasRelax = 0.01f, kGasKP = 0.2f, kGasAlpha = -0.2f, kGasOpenDrag = -50.0f;
+constexpr uint32_t kGasEP = 4;
 
 __device__ __forceinline__ uint64_t cr_mix64(uint64_t z)
 {
@@ -155,6 +165,7 @@ __global__ __launch_bounds__(kBlock) void cr_predict_k(const float4 *__restrict_
                                                        const Attr *__restrict__ A,
                                                        float4 *__restrict__ E,
                                                        uint32_t *__restrict__ bcs,
+                                                       float4 *__restrict__ F,
                                                        const DevStatus *__restrict__ st, CrConst c,
                                                        uint32_t n, float dt)
 {
@@ -162,11 +173,18 @@ __global__ __launch_bounds__(kBlock) void cr_predict_k(const float4 *__restrict_
     if (i >= n) return;
     const float4 x = X[i];
     float4 v = V[i];
-    const float w = A[i].w;
+    const Attr at = A[i];
+    const float w = at.w;
     const bool two = c.dims == 2;
-    v.x = v.x + (dt * c.gravity[0]);
-    v.y = v.y + (dt * c.gravity[1]);
-    if (!two) v.z = v.z + (dt * c.gravity[2]);
+    // v = v + dt g' + dt f (g' = g ALPHA for gas), then f = 0
+    float g3[3] = {c.gravity[0], c.gravity[1], c.gravity[2]};
+    if (at.phase == PS_PHASE_GAS)
+        for (int a = 0; a < 3; a++) g3[a] = g3[a] * kGasAlpha;
+    const float4 fo = F[i];
+    v.x = (v.x + (dt * g3[0])) + (dt * fo.x);
+    v.y = (v.y + (dt * g3[1])) + (dt * fo.y);
+    if (!two) v.z = (v.z + (dt * g3[2])) + (dt * fo.z);
+    F[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     const float p[3] = {x.x, x.y, x.z};
     float e[3];
     if (w == 0.0f) {
@@ -197,10 +215,16 @@ __global__ __launch_bounds__(kBlock) void cr_predict_k(const float4 *__restrict_
 
 // TotalFluidConstraint::project, first loop (totalfluidconstraint.cpp:44-90):
 // lambda of every particle of the object [first, first + count).
+// gas != 0: GasConstraint::project's first loop (gasconstraint.cpp:34-84):
+// its relaxation, and for an open gas (gas == 2) the drag
+// f += v (1 - rho/rho0) (-50).  (Neither kind of object has solid
+// neighbours in this mode, so no solid weights appear.)
 __global__ __launch_bounds__(kBlock) void cr_lambda_k(const float4 *__restrict__ E,
-                                                      float *__restrict__ lam, CrConst c,
+                                                      float *__restrict__ lam,
+                                                      const float4 *__restrict__ V,
+                                                      float4 *__restrict__ F, CrConst c,
                                                       uint32_t n, uint32_t first, uint32_t count,
-                                                      float rho0)
+                                                      float rho0, int gas)
 {
     const uint32_t wv = blockIdx.x * kWaves + (threadIdx.x >> 6);
     const uint32_t lane = threadIdx.x & 63u;
@@ -242,21 +266,37 @@ __global__ __launch_bounds__(kBlock) void cr_lambda_k(const float4 *__restrict__
     }
     const float grx = gx / rho0, gry = gy / rho0, grz = gz / rho0;
     den = den + (((grx * grx) + (gry * gry)) + (grz * grz));
-    if (lane == 0) lam[i] = -((acc / rho0) - 1.0f) / (den + c.relax);
+    if (lane != 0) return;
+    const float p_rat = acc / rho0;
+    if (gas == 2) {
+        const float4 vi = V[i];
+        float4 f = F[i];
+        f.x = f.x + ((vi.x * (1.0f - p_rat)) * kGasOpenDrag);
+        f.y = f.y + ((vi.y * (1.0f - p_rat)) * kGasOpenDrag);
+        F[i] = f;
+    }
+    lam[i] = -(p_rat - 1.0f) / (den + (gas ? kGasRelax : c.relax));
 }
 
 // TotalFluidConstraint::project, delta and update loops
 // (totalfluidconstraint.cpp:92-115) for the object [first, first + count),
 // ping-pong E -> E2 over every particle (others are copied).  With
 // next_iter >= 0 the boundary projections of the next iteration follow.
+// gas != 0: GasConstraint::project's delta loop (gasconstraint.cpp:86-107):
+// the gas tensile term and the vorticity force
+// f += sum_j cross((0,0,|G_j * v_j|), r) W(r^2), G_j = spikyGrad(r, r.r)
+// (the squared length where the length is expected, and G_j * v_j
+// component-wise, as written there; 2D).
 __global__ __launch_bounds__(kBlock) void cr_delta_k(const float4 *__restrict__ E,
                                                      float4 *__restrict__ E2,
                                                      const float *__restrict__ lam,
                                                      const float4 *__restrict__ X,
                                                      const uint32_t *__restrict__ bcs,
+                                                     const float4 *__restrict__ V,
+                                                     float4 *__restrict__ F,
                                                      const DevStatus *__restrict__ st, CrConst c,
                                                      uint32_t n, uint32_t first, uint32_t count,
-                                                     float rho0, int next_iter)
+                                                     float rho0, int gas, int next_iter)
 {
     const uint32_t i = blockIdx.x * kWaves + (threadIdx.x >> 6);
     const uint32_t lane = threadIdx.x & 63u;
@@ -265,7 +305,9 @@ __global__ __launch_bounds__(kBlock) void cr_delta_k(const float4 *__restrict__ 
     const uint32_t bc = bcs[i];
     if (i - first < count) {
         const float li = lam[i];
-        float dx = 0.0f, dy = 0.0f, dz = 0.0f;
+        const float kc = gas ? kGasKP : c.k_corr, wdq = gas ? c.g_wdq : c.w_dq;
+        const uint32_t nc = gas ? kGasEP : c.n_corr;
+        float dx = 0.0f, dy = 0.0f, dz = 0.0f, fvx = 0.0f, fvy = 0.0f;
         uint32_t cnt = 1;  // the particle itself is in its neighbour list
         for (uint32_t base = 0; base < n; base += 64) {
             const uint32_t j = base + lane;
@@ -277,19 +319,29 @@ __global__ __launch_bounds__(kBlock) void cr_delta_k(const float4 *__restrict__ 
             uint64_t mask = __ballot(nb);
             if (!mask) continue;
             cnt += (uint32_t)__popcll(mask);
-            float tx = 0.0f, ty = 0.0f, tz = 0.0f;
+            float tx = 0.0f, ty = 0.0f, tz = 0.0f, ux = 0.0f, uy = 0.0f;
             if (nb) {
                 const float l = sqrtf(r2);
                 float sx, sy, sz;
                 cr_spiky(c, rx, ry, rz, r2, l, sx, sy, sz);
-                const float xr = cr_poly6(c, l * l) / c.w_dq;
-                const float corr = -(c.k_corr * cr_ipow(xr, c.n_corr));
+                const float xr = cr_poly6(c, l * l) / wdq;
+                const float corr = -(kc * cr_ipow(xr, nc));
                 // lambdas of particles outside this object read as 0 (QHash default)
                 const float lj = (j - first < count) ? lam[j] : 0.0f;
                 const float coef = (li + lj) + corr;
                 tx = coef * sx;
                 ty = coef * sy;
                 tz = coef * sz;
+                if (gas) {
+                    float vx, vy, vz;
+                    cr_spiky(c, rx, ry, rz, r2, r2, vx, vy, vz);
+                    const float4 vj = V[j];
+                    const float wx = vx * vj.x, wy = vy * vj.y;
+                    const float W = sqrtf((wx * wx) + (wy * wy));
+                    const float p6 = cr_poly6(c, r2);
+                    ux = ((0.0f * 0.0f) - (ry * W)) * p6;
+                    uy = ((W * rx) - (0.0f * 0.0f)) * p6;
+                }
             }
             while (mask) {
                 const uint32_t b = (uint32_t)__builtin_ctzll(mask);
@@ -297,12 +349,22 @@ __global__ __launch_bounds__(kBlock) void cr_delta_k(const float4 *__restrict__ 
                 dx = dx + lane_value(tx, b);
                 dy = dy + lane_value(ty, b);
                 dz = dz + lane_value(tz, b);
+                if (gas) {
+                    fvx = fvx + lane_value(ux, b);
+                    fvy = fvy + lane_value(uy, b);
+                }
             }
         }
         const float div = (float)(cnt + (bc >> 8));
         ei.x = ei.x + ((dx / rho0) / div);
         ei.y = ei.y + ((dy / rho0) / div);
         ei.z = ei.z + ((dz / rho0) / div);
+        if (gas && lane == 0) {
+            float4 f = F[i];
+            f.x = f.x + fvx;
+            f.y = f.y + fvy;
+            F[i] = f;
+        }
     }
     if (next_iter >= 0) {
         const float4 x = X[i];
@@ -363,25 +425,26 @@ hipError_t launch_cr_predict(hipStream_t st, const DevBuffers &b, const CrConst 
 {
     if (!n) return hipSuccess;
     hipLaunchKernelGGL(cr_predict_k, dim3(blocks_for(n)), dim3(kBlock), 0, st, b.X, b.V, b.A, b.P,
-                       b.bc, b.status, c, n, dt);
+                       b.bc, b.force, b.status, c, n, dt);
     return hipGetLastError();
 }
 
 hipError_t launch_cr_lambda(hipStream_t st, const DevBuffers &b, const CrConst &c, uint32_t n,
-                            uint32_t first, uint32_t count, float rho0)
+                            uint32_t first, uint32_t count, float rho0, int gas)
 {
     if (!n || !count) return hipSuccess;
     hipLaunchKernelGGL(cr_lambda_k, dim3((count + kWaves - 1) / kWaves), dim3(kBlock), 0, st, b.P,
-                       b.lam, c, n, first, count, rho0);
+                       b.lam, b.V, b.force, c, n, first, count, rho0, gas);
     return hipGetLastError();
 }
 
 hipError_t launch_cr_delta(hipStream_t st, const DevBuffers &b, const CrConst &c, uint32_t n,
-                           uint32_t first, uint32_t count, float rho0, int next_iter)
+                           uint32_t first, uint32_t count, float rho0, int gas, int next_iter)
 {
     if (!n) return hipSuccess;
     hipLaunchKernelGGL(cr_delta_k, dim3((n + kWaves - 1) / kWaves), dim3(kBlock), 0, st, b.P, b.Q,
-                       b.lam, b.X, b.bc, b.status, c, n, first, count, rho0, next_iter);
+                       b.lam, b.X, b.bc, b.V, b.force, b.status, c, n, first, count, rho0, gas,
+                       next_iter);
     return hipGetLastError();
 }
 

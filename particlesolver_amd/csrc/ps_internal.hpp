@@ -61,6 +61,7 @@ struct KConst {
 // float on the host in the order the reference evaluates them.
 struct CrConst {
     float h, h2, poly6, spiky, w_dq, relax, k_corr, jitter, radius, sf, kf;
+    float g_wdq;  // gas objects: poly6(DQ_P^2 H^2) of GasConstraint
     float gravity[3];
     float box_min[3], box_max[3];
     uint32_t n_corr;
@@ -154,6 +155,7 @@ struct DevBuffers {
     float *tape_rho;       // lambda pass: density
     float4 *tape_f;        // solve: fluid sum f (xyz), contact count bits (w)
     // CPU-reference mode only
+    float4 *force;         // per particle, applied at the next prediction (gas)
     float *lam;            // lambda per particle
     uint32_t *bc;          // boundary flags | boundary count << 8
 };
@@ -237,13 +239,14 @@ hipError_t launch_adj(hipStream_t st, AdjKernel which, const DevBuffers &b, cons
 // ping-pong b.P -> b.Q per fluid object.
 hipError_t launch_cr_predict(hipStream_t st, const DevBuffers &b, const CrConst &c, uint32_t n,
                              float dt);
-// lambda, then delta + Jacobi update (b.P -> b.Q) of one fluid object
-// [first, first + count); next_iter >= 0 also projects the boundaries of
-// that iteration of the substep
+// lambda, then delta + Jacobi update (b.P -> b.Q) of one fluid or gas
+// object [first, first + count); gas: 0 fluid, 1 closed gas, 2 open gas;
+// next_iter >= 0 also projects the boundaries of that iteration of the
+// substep
 hipError_t launch_cr_lambda(hipStream_t st, const DevBuffers &b, const CrConst &c, uint32_t n,
-                            uint32_t first, uint32_t count, float rho0);
+                            uint32_t first, uint32_t count, float rho0, int gas);
 hipError_t launch_cr_delta(hipStream_t st, const DevBuffers &b, const CrConst &c, uint32_t n,
-                           uint32_t first, uint32_t count, float rho0, int next_iter);
+                           uint32_t first, uint32_t count, float rho0, int gas, int next_iter);
 hipError_t launch_cr_boundary(hipStream_t st, const DevBuffers &b, const CrConst &c, uint32_t n,
                               uint32_t it);
 hipError_t launch_cr_end(hipStream_t st, const DevBuffers &b, uint32_t n, float dt, bool sleeping,

@@ -381,7 +381,10 @@ def two_d_fluid(nx=40, ny=20, spacing=1.0, h=2.0, iterations=3, seed=0x5EED0001,
 class FluidObjectsScene:
     """Parameters plus fluid objects (positions, rest density), added with
     ps_add_fluid_object in creation order (Simulation::createFluid,
-    cpu/src/simulation.cpp:431-449)."""
+    cpu/src/simulation.cpp:431-449), and gas objects (positions, rest
+    density, "gas" or "open gas"), added with ps_add_gas_object
+    (Simulation::createGas, :408-428).  An object is (pos, rho0) or
+    (pos, rho0, kind)."""
 
     def __init__(self, params, objects, dt, name=""):
         self.params = params
@@ -391,11 +394,16 @@ class FluidObjectsScene:
 
     @property
     def n(self):
-        return sum(pos.shape[0] for pos, _ in self.objects)
+        return sum(o[0].shape[0] for o in self.objects)
 
     def load_into(self, sys_):
-        for pos, rho0 in self.objects:
-            sys_.add_fluid_object(pos, None, 1.0, rho0)
+        for o in self.objects:
+            pos, rho0 = o[0], o[1]
+            kind = o[2] if len(o) > 2 else "fluid"
+            if kind == "fluid":
+                sys_.add_fluid_object(pos, None, 1.0, rho0)
+            else:
+                sys_.add_gas_object(pos, None, 1.0, rho0, open=(kind == "open gas"))
 
 
 def cpu_reference_params(box_min, box_max, radius, h, iterations, max_particles, dims=2,
@@ -486,6 +494,72 @@ def reference_two_fluids(seed=0x5EED0005, dt=0.01):
     return FluidObjectsScene(p, objects, dt, name="reference_two_fluids")
 
 
+def _reference_lattice(x0, x1, y0, y1, delta, seed):
+    """The CPU app's scene lattice: x from x0 while x < x1, y from y0 while
+    y < y1, step delta, plus .2 (u - .5) per axis (seeded SplitMix64 in
+    place of frand())."""
+    pts = []
+    x = x0
+    while x < x1:
+        y = y0
+        while y < y1:
+            pts.append((x, y))
+            y += delta
+        x += delta
+    pts = np.array(pts, np.float64)
+    u = uniform24(seed, 2 * len(pts)).reshape(-1, 2)
+    pos = np.zeros((len(pts), 3), np.float64)
+    pos[:, :2] = pts + 0.2 * (u - 0.5)
+    return pos.astype(np.float32)
+
+
+def reference_gas(seed=0x5EED0008, dt=0.01):
+    """The reference CPU app's gas scene, Simulation::initGas
+    (cpu/src/simulation.cpp:985-1016): two closed gas objects side by side
+    (rest densities .75 and 3.75, lattice step .7 over x in [-4, 4), y in
+    [-4, 4)) and above them two fluid objects (rest densities 4.75 and 5.5,
+    x in [-6, 6), y in [4, 16)), created in that order; box x in [-4, 4],
+    y in [-4, 20] (set before the fluid lattice's scale changes, so the
+    fluid starts partly outside the box, as there); H = 2, radius .25,
+    3 iterations, dt .01."""
+    delta = 0.7
+    objects = []
+    scale = 2.0
+    for d in range(2):
+        start = -2.0 * scale + 4.0 * scale * (d / 2.0)
+        pos = _reference_lattice(start, start + 4.0 * scale / 2.0, -2.0 * scale, 2.0 * scale, delta,
+                                 seed + d)
+        objects.append((pos, 0.75 + 3.0 * d, "gas"))
+    scale = 3.0
+    for d in range(2):
+        start = -2.0 * scale + 4.0 * scale * (d / 2.0)
+        pos = _reference_lattice(start, start + 4.0 * scale / 2.0, -2.0 * scale + 10.0,
+                                 2.0 * scale + 10.0, delta, seed + 2 + d)
+        objects.append((pos, 4.0 + 0.75 * (d + 1)))
+    n = sum(o[0].shape[0] for o in objects)
+    p = cpu_reference_params((-4.0, -4.0, -1.0), (4.0, 20.0, 1.0), 0.25, 2.0, 3, n)
+    return FluidObjectsScene(p, objects, dt, name="reference_gas")
+
+
+def reference_smoke(seed=0x5EED0009, dt=0.01, open_gas=True):
+    """The gas body of the CPU app's smoke scenes, Simulation::initSmokeOpen
+    and initSmokeClosed (cpu/src/simulation.cpp:1096-1134): one gas object
+    of rest density 1.5, lattice step .63 over x in [-4, 4), y in [-4, 4);
+    open: box x in [-6, 6], y in [-4, 200] and the open-gas drag; closed:
+    box [-4, 4] x [-4, 4].  (Their emitters are not part of this scene.)"""
+    scale, delta = 2.0, 0.63
+    pos = _reference_lattice(-2.0 * scale, 2.0 * scale, -2.0 * scale, 2.0 * scale, delta, seed)
+    if open_gas:
+        p = cpu_reference_params((-3.0 * scale, -2.0 * scale, -1.0), (3.0 * scale, 100.0 * scale, 1.0),
+                                 0.25, 2.0, 3, pos.shape[0])
+    else:
+        p = cpu_reference_params((-2.0 * scale, -2.0 * scale, -1.0), (2.0 * scale, 2.0 * scale, 1.0),
+                                 0.25, 2.0, 3, pos.shape[0])
+    kind = "open gas" if open_gas else "gas"
+    return FluidObjectsScene(p, [(pos, 1.5, kind)], dt,
+                             name="reference_smoke_%s" % ("open" if open_gas else "closed"))
+
+
 def fluid_object_metrics(params, objects, pos):
     """FP64 densities of every particle at `pos` (all-pairs poly6 sum with the
     3D normalisation the reference's CPU solver uses in 2D,
@@ -494,8 +568,7 @@ def fluid_object_metrics(params, objects, pos):
     x = np.asarray(pos, np.float64)
     h = float(params.h)
     c = poly6_coef(h)
-    rho0 = np.concatenate([np.full(o[0].shape[0], r) for o, r in
-                           zip(objects, [r for _, r in objects])])
+    rho0 = np.concatenate([np.full(o[0].shape[0], o[1]) for o in objects])
     rho = np.zeros(x.shape[0])
     for a in range(0, x.shape[0], 512):
         d2 = ((x[a:a + 512, None, :] - x[None, :, :]) ** 2).sum(-1)

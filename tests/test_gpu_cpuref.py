@@ -119,3 +119,32 @@ def test_unsupported_calls_in_cpu_reference_mode(gpu_available):
         assert e.value.code == capi.PS_ERR_INVALID_ARGUMENT
         g.step(sc.dt, 1)                     # the handle still works
         assert np.isfinite(g.positions()).all()
+
+
+def test_reference_gas_bitexact(gpu_available):
+    # Simulation::initGas: two closed gas objects and two fluid objects
+    sc = scenes.reference_gas()
+    assert [o[0].shape[0] for o in sc.objects] == [72, 72, 162, 162]
+    run_parity(sc, [1, 5, 50, 200])
+
+
+@pytest.mark.parametrize("open_gas", [True, False])
+def test_smoke_gas_bitexact(gpu_available, open_gas):
+    x = run_parity(scenes.reference_smoke(open_gas=open_gas), [1, 10, 100])
+    assert np.all(np.isfinite(x))
+
+
+def test_gas_objects_need_the_2d_cpu_reference_mode(gpu_available):
+    pos = np.zeros((1, 3), np.float32)
+    sc = scenes.dam_break(4, 4, 4)
+    with System(sc.params) as g:
+        with pytest.raises(capi.PsError) as e:
+            g.add_gas_object(pos)
+        assert e.value.code == capi.PS_ERR_UNSUPPORTED
+    p = scenes.reference_smoke().params.copy()
+    p.dims = 3
+    p.box_min[2], p.box_max[2] = -4.0, 4.0
+    with System(p) as g:
+        with pytest.raises(capi.PsError) as e:
+            g.add_gas_object(pos)
+        assert e.value.code == capi.PS_ERR_UNSUPPORTED
