@@ -246,6 +246,29 @@ int ps_add_fluid_object(ps_system *sys, uint32_t n, const float *pos, const floa
 int ps_add_gas_object(ps_system *sys, uint32_t n, const float *pos, const float *vel,
                       const float *inv_mass, float rest_density, int open, uint32_t *first_out);
 
+/* Add a smoke emitter (PS_FLAG_CPU_REFERENCE mode, dims = 2) at pos[0..3),
+ * emitting particles_per_sec markers and, when object >= 0, as many gas
+ * particles into gas object `object` (the last object added; no object may
+ * follow it).  Replaces Simulation::createSmokeEmitter
+ * (cpu/src/simulation.cpp:454-457) and OpenSmokeEmitter
+ * (cpu/src/opensmokeemitter.cpp:3-46): at the end of every substep, in
+ * creation order, the emitter's double-precision timer gains dt and pays
+ * 1/particles_per_sec per emission; each emission adds a marker and a gas
+ * particle of inverse mass 1 at rest at pos (at the next free index); then
+ * every marker moves by dt times the poly6-weighted mean velocity of all
+ * particles and is clamped to the box in x and y (simulation.cpp:348-362).
+ * Markers are not simulated; at most max_markers per emitter.  A ps_step
+ * whose emissions would exceed max_markers or max_particles fails with
+ * PS_ERR_CAPACITY before its first substep.  ps_step_graph is
+ * PS_ERR_UNSUPPORTED on a system with emitters. */
+int ps_add_smoke_emitter(ps_system *sys, const float *pos, double particles_per_sec, int32_t object,
+                         uint32_t max_markers, uint32_t *emitter_out);
+
+/* Markers of an emitter in emission order (OpenSmokeEmitter::getParticles,
+ * opensmokeemitter.cpp:48-51): *count receives their number and, unless
+ * xyz is NULL, xyz[3k..3k+2] their positions (room for *count). */
+int ps_read_markers(ps_system *sys, uint32_t emitter, uint32_t *count, float *xyz);
+
 /* Replaces addDistanceConstraint + updateOccurences (solver.cu:72-106,125-156).
  * pairs: 2*m particle indices; rest: m lengths; stiffness: m values in (0,1]
  * (NULL means 1). */

@@ -317,6 +317,29 @@ class Ref2D:
         self.n += n
         return first
 
+    def add_smoke_emitter(self, pos, particles_per_sec, object=-1, max_markers=4096):
+        """createSmokeEmitter (simulation.cpp:454-457) feeding gas object
+        `object` (an index in creation order, -1: markers only)"""
+        p = np.ascontiguousarray(pos, dtype=np.float32).reshape(3)
+        f = self._f("r2d_add_smoke_emitter")
+        f.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_int32,
+                      ctypes.c_uint32]
+        rc = f(self.h, _ptr(p), float(particles_per_sec), int(object), int(max_markers))
+        if rc:
+            raise RuntimeError("r2d_add_smoke_emitter failed: %d" % rc)
+        self.nem = getattr(self, "nem", 0) + 1
+        return self.nem - 1
+
+    def markers(self, emitter):
+        f = self._f("r2d_read_markers")
+        f.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p]
+        f.restype = ctypes.c_uint32
+        m = f(self.h, emitter, None)
+        out = np.zeros((m, 3), self.dtype)
+        if m:
+            f(self.h, emitter, _ptr(out))
+        return out
+
     def forces(self):
         """the force each particle carries into the next prediction"""
         f = np.zeros((self.n, 3), self.dtype)
@@ -324,7 +347,12 @@ class Ref2D:
         return f
 
     def step(self, dt, steps=1):
-        self._f("r2d_step")(self.h, dt, steps)
+        rc = self._f("r2d_step")(self.h, dt, steps)
+        if rc:
+            raise RuntimeError("r2d_step failed: %d" % rc)
+        f = self._f("r2d_num_particles")
+        f.restype = ctypes.c_uint32
+        self.n = f(self.h)
 
     def _read(self):
         x = np.zeros((self.n, 3), self.dtype)

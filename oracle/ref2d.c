@@ -81,7 +81,20 @@ static real poly6(const r2d_sys *s, real r2);
 #define GAS_ALPHA (-0.2)
 #define GAS_OPEN_DRAG (-50.0)
 
+/* OpenSmokeEmitter (cpu/src/opensmokeemitter.cpp:3-51): a double timer,
+ * markers (diffuse particles, not simulated) and an optional gas object
+ * that receives one particle per period */
+typedef struct {
+    double timer, rate;
+    real pos[3];
+    int32_t group; /* gas object, -1 none */
+    uint32_t nmark, cap;
+    real *mark; /* 3 per marker */
+} r2d_emitter;
+
 struct r2d_sys {
+    r2d_emitter em[16];
+    uint32_t nem;
     ps_params prm;
     uint32_t n, cap, ngroup;
     real *p, *ep, *v; /* 3 per particle */
@@ -168,6 +181,7 @@ EXPORT void SFX(r2d_destroy)(r2d_sys *s)
     void *ptrs[] = {s->p, s->ep, s->v, s->force, s->w, s->phase, s->group, s->group_rho0,
                     s->group_gas, s->bflags, s->counts, s->lambda, s->delta, s->nnb};
     for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) free(ptrs[i]);
+    for (uint32_t e = 0; e < s->nem; e++) free(s->em[e].mark);
     free(s);
 }
 
@@ -215,6 +229,79 @@ EXPORT int SFX(r2d_add_gas_group)(r2d_sys *s, uint32_t first, uint32_t n, float 
     s->group_rho0[s->ngroup] = rho0;
     s->group_gas[s->ngroup] = open ? 2 : 1;
     s->ngroup++;
+    return PS_OK;
+}
+
+/* createSmokeEmitter (simulation.cpp:454-457): emits particles_per_sec
+ * markers, and as many gas particles into gas object `group` (-1: none) */
+EXPORT int SFX(r2d_add_smoke_emitter)(r2d_sys *s, const float *pos, double rate, int32_t group,
+                                      uint32_t max_markers)
+{
+    if (s->nem >= 16 || !(rate > 0.0)) return PS_ERR_INVALID_ARGUMENT;
+    if (group >= (int32_t)s->ngroup || (group >= 0 && !s->group_gas[group])) return PS_ERR_INVALID_ARGUMENT;
+    r2d_emitter *e = &s->em[s->nem];
+    memset(e, 0, sizeof(*e));
+    e->rate = rate;
+    for (int a = 0; a < 3; a++) e->pos[a] = pos[a];
+    e->group = group;
+    e->cap = max_markers;
+    e->mark = (real *)calloc(3 * (size_t)(max_markers ? max_markers : 1), sizeof(real));
+    s->nem++;
+    return PS_OK;
+}
+
+EXPORT uint32_t SFX(r2d_read_markers)(const r2d_sys *s, uint32_t emitter, real *xyz)
+{
+    const r2d_emitter *e = &s->em[emitter];
+    if (xyz) memcpy(xyz, e->mark, sizeof(real) * 3 * e->nmark);
+    return e->nmark;
+}
+
+/* OpenSmokeEmitter::tick (opensmokeemitter.cpp:17-46) at the end of the
+ * tick, then the markers' box clamp (simulation.cpp:348-362): emit on the
+ * timer (a marker, and a gas particle at rest into the object), then move
+ * every marker by the poly6-weighted mean velocity of all particles,
+ * p += (v dt) / sum */
+static int emitter_tick(r2d_sys *s, r2d_emitter *e, real dt)
+{
+    e->timer += (double)dt;
+    while (e->timer >= 1.0 / e->rate) {
+        e->timer -= 1.0 / e->rate;
+        if (e->nmark >= e->cap) return PS_ERR_CAPACITY;
+        for (int a = 0; a < 3; a++) e->mark[3 * e->nmark + a] = e->pos[a];
+        e->nmark++;
+        if (e->group >= 0) {
+            if (s->n >= s->cap) return PS_ERR_CAPACITY;
+            uint32_t i = s->n++;
+            for (int a = 0; a < 3; a++) {
+                s->p[3 * i + a] = e->pos[a];
+                s->ep[3 * i + a] = e->pos[a];
+                s->v[3 * i + a] = R(0);
+                s->force[3 * i + a] = R(0);
+            }
+            s->w[i] = R(1);
+            s->phase[i] = PS_PHASE_GAS;
+            s->group[i] = e->group;
+        }
+    }
+    for (uint32_t m = 0; m < e->nmark; m++) {
+        real *q = &e->mark[3 * m];
+        real v[3] = {R(0), R(0), R(0)}, sum = R(0);
+        for (uint32_t j = 0; j < s->n; j++) {
+            real r[3] = {q[0] - s->p[3 * j], q[1] - s->p[3 * j + 1], q[2] - s->p[3 * j + 2]};
+            real p6 = poly6(s, ((r[0] * r[0]) + (r[1] * r[1])) + (r[2] * r[2]));
+            for (int a = 0; a < 3; a++) v[a] = v[a] + (s->v[3 * j + a] * p6);
+            sum = sum + p6;
+        }
+        if (sum > R(0))
+            for (int a = 0; a < 3; a++) q[a] = q[a] + ((v[a] * dt) / sum);
+        for (int a = 0; a < 2; a++) {
+            if (q[a] < R(s->prm.box_min[a]))
+                q[a] = R(s->prm.box_min[a]);
+            else if (q[a] > R(s->prm.box_max[a]))
+                q[a] = R(s->prm.box_max[a]);
+        }
+    }
     return PS_OK;
 }
 
@@ -536,9 +623,15 @@ EXPORT int SFX(r2d_step)(r2d_sys *s, real dt, uint32_t steps)
             }
             for (int a = 0; a < 3; a++) p[a] = e[a];
         }
+        for (uint32_t k = 0; k < s->nem; k++) {
+            int rc = emitter_tick(s, &s->em[k], dt);
+            if (rc) return rc;
+        }
     }
     return PS_OK;
 }
+
+EXPORT uint32_t SFX(r2d_num_particles)(const r2d_sys *s) { return s->n; }
 
 EXPORT int SFX(r2d_read)(const r2d_sys *s, real *pos, real *vel)
 {

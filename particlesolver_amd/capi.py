@@ -136,6 +136,9 @@ PROTOTYPES = {
     "ps_add_particles": (_int, [_P, _u32, _P, _P, _P, _P, _P, ctypes.POINTER(_u32)]),
     "ps_add_fluid_object": (_int, [_P, _u32, _P, _P, _P, _f32, ctypes.POINTER(_u32)]),
     "ps_add_gas_object": (_int, [_P, _u32, _P, _P, _P, _f32, _int, ctypes.POINTER(_u32)]),
+    "ps_add_smoke_emitter": (_int, [_P, _P, ctypes.c_double, ctypes.c_int32, _u32,
+                                    ctypes.POINTER(_u32)]),
+    "ps_read_markers": (_int, [_P, _u32, ctypes.POINTER(_u32), _P]),
     "ps_add_distance_constraints": (_int, [_P, _u32, _P, _P, _P]),
     "ps_add_point_constraints": (_int, [_P, _u32, _P, _P]),
     "ps_add_rigid_body": (_int, [_P, _u32, _P, _f32, ctypes.POINTER(_u32)]),
@@ -297,6 +300,24 @@ class System:
                                                float(rest_density), int(bool(open)),
                                                ctypes.byref(first)))
         return first.value
+
+    def add_smoke_emitter(self, pos, particles_per_sec, object=-1, max_markers=4096):
+        """ps_add_smoke_emitter: an OpenSmokeEmitter at pos feeding gas object
+        `object` (-1: markers only); returns the emitter index."""
+        p = np.ascontiguousarray(pos, dtype=np.float32).reshape(3)
+        e = ctypes.c_uint32()
+        self._check(self.lib.ps_add_smoke_emitter(self.h, _ptr(p), float(particles_per_sec),
+                                                  int(object), int(max_markers), ctypes.byref(e)))
+        return e.value
+
+    def markers(self, emitter):
+        """ps_read_markers: the emitter's marker positions, (count, 3)"""
+        c = ctypes.c_uint32()
+        self._check(self.lib.ps_read_markers(self.h, emitter, ctypes.byref(c), None))
+        out = np.empty((c.value, 3), dtype=np.float32)
+        if c.value:
+            self._check(self.lib.ps_read_markers(self.h, emitter, ctypes.byref(c), _ptr(out)))
+        return out
 
     def add_distance_constraints(self, pairs, rest, stiffness=None):
         pairs = np.ascontiguousarray(pairs, dtype=np.uint32).reshape(-1, 2)

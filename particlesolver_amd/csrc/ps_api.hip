@@ -10,6 +10,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <map>
 #include <string>
 #include <vector>
@@ -74,6 +75,17 @@ struct ps_system {
         int gas;  // 0 fluid, 1 closed gas, 2 open gas
     };
     std::vector<FluidObject> objects;
+    // smoke emitters (OpenSmokeEmitter), ticked in creation order at the end
+    // of every substep; `object` indexes `objects` (-1: markers only) and is
+    // the last object, so its particles stay one contiguous range
+    struct Emitter {
+        double timer, rate;
+        float pos[3];
+        int32_t object;
+        uint32_t nmark, cap;
+        float4 *mark;
+    };
+    std::vector<Emitter> emitters;
     bool cr_bnd_done = false;  // boundaries of iteration it_sub already projected
     bool cr_swapped = false;   // iterate lives in b.Q (odd number of object passes)
     // workgroup size of each kernel (ps_autotune)
@@ -351,6 +363,7 @@ void free_all(ps_system *s)
     dev_free(b.sid); dev_free(b.sid2); dev_free(b.scenes); dev_free(b.lam); dev_free(b.bc);
     dev_free(b.force);
     dev_free(b.stage);
+    for (ps_system::Emitter &e : s->emitters) dev_free(e.mark);
     adj_free(s);
 }
 
@@ -549,8 +562,71 @@ int cr_iterate(ps_system *s)
     return PS_OK;
 }
 
+// Emissions of every emitter over `substeps` substeps of dt from the current
+// timers (OpenSmokeEmitter::tick, opensmokeemitter.cpp:18-29: the timer
+// gains dt in double and pays 1/rate per emission); adds the particles and
+// the markers of each emitter to what is already there and reports the
+// first emitter to overflow.
+int emission_check(ps_system *s, float dt, uint32_t substeps)
+{
+    uint64_t n = s->n;
+    for (size_t k = 0; k < s->emitters.size(); k++) {
+        const ps_system::Emitter &e = s->emitters[k];
+        double timer = e.timer;
+        uint64_t count = 0;
+        for (uint32_t t = 0; t < substeps; t++) {
+            timer += (double)dt;
+            while (timer >= 1.0 / e.rate) {
+                timer -= 1.0 / e.rate;
+                count++;
+            }
+            if (e.nmark + count > e.cap)
+                return fail(s, PS_ERR_CAPACITY,
+                            "smoke emitter " + std::to_string(k) + " exceeds its marker capacity");
+        }
+        if (e.object >= 0) n += count;
+    }
+    if (n > s->cap) return fail(s, PS_ERR_CAPACITY, "smoke emitters exceed max_particles");
+    return PS_OK;
+}
+
+// OpenSmokeEmitter::tick and the markers' box clamp (simulation.cpp:348-362)
+// of every emitter, in creation order, after the velocity update: the
+// emissions of this substep, then the markers' advection through all
+// particles (those just emitted included).
+int cr_emitters(ps_system *s)
+{
+    for (ps_system::Emitter &e : s->emitters) {
+        uint32_t count = 0;
+        e.timer += (double)s->dt;
+        while (e.timer >= 1.0 / e.rate) {
+            e.timer -= 1.0 / e.rate;
+            count++;
+        }
+        ps_system::FluidObject *o = e.object >= 0 ? &s->objects[e.object] : nullptr;
+        for (uint32_t done = 0; done < count;) {
+            const uint32_t c = std::min(count - done, 64u);
+            LAUNCH(s, PS_K_FINALIZE,
+                   ps::launch_cr_emit(s->stream, s->b, e.mark, e.nmark, o ? s->n : ~0u, c, e.pos,
+                                      o ? o->rho0 : 0.0f));
+            e.nmark += c;
+            if (o) {
+                o->count += c;
+                s->n += c;
+                s->scene_count[0] += c;
+            }
+            done += c;
+        }
+        LAUNCH(s, PS_K_FINALIZE,
+               ps::launch_cr_markers(s->stream, s->b, e.mark, s->cr, e.nmark, s->n, s->dt));
+    }
+    return PS_OK;
+}
+
 int cr_end(ps_system *s)
 {
+    if (!s->emitters.empty())
+        if (int rc = emission_check(s, s->dt, 1)) return rc;
     LAUNCH(s, PS_K_FINALIZE,
            ps::launch_cr_end(s->stream, s->b, s->n, s->dt, (s->prm.flags & PS_FLAG_SLEEPING) != 0,
                              s->it_sub));
@@ -559,6 +635,7 @@ int cr_end(ps_system *s)
         std::swap(s->b.P, s->b.Q);
         s->cr_swapped = false;
     }
+    if (int rc = cr_emitters(s)) return rc;
     s->it_sub = 0;
     s->substeps++;
     s->stage = 0;
@@ -959,6 +1036,16 @@ static int add_particles_impl(ps_system *s, uint32_t scene, uint32_t n, const fl
                               const float *vel, const float *inv_mass, const int32_t *phase,
                               const float *rest_density, uint32_t *first_out);
 
+// An emitter feeding an object appends to the end of the particle range, so
+// that object stays the last one.
+static int objects_open(ps_system *s)
+{
+    for (const ps_system::Emitter &e : s->emitters)
+        if (e.object >= 0)
+            return fail(s, PS_ERR_STATE, "no objects can follow a smoke emitter that feeds an object");
+    return PS_OK;
+}
+
 int ps_add_particles(ps_system *s, uint32_t n, const float *pos, const float *vel,
                      const float *inv_mass, const int32_t *phase, const float *rest_density,
                      uint32_t *first_out)
@@ -979,6 +1066,7 @@ int ps_add_fluid_object(ps_system *s, uint32_t n, const float *pos, const float 
                         const float *inv_mass, float rest_density, uint32_t *first_out)
 {
     if (int rc = check_sys(s)) return rc;
+    if (int rc = objects_open(s)) return rc;
     if (!(rest_density > 0.0f) || !is_finite(rest_density))
         return fail(s, PS_ERR_INVALID_ARGUMENT, "fluid rest density must be > 0");
     std::vector<int32_t> phase(n, PS_PHASE_FLUID);
@@ -995,6 +1083,7 @@ int ps_add_gas_object(ps_system *s, uint32_t n, const float *pos, const float *v
                       const float *inv_mass, float rest_density, int open, uint32_t *first_out)
 {
     if (int rc = check_sys(s)) return rc;
+    if (int rc = objects_open(s)) return rc;
     if (!s->cpuref || s->prm.dims != 2)
         return fail(s, PS_ERR_UNSUPPORTED, "gas objects belong to the 2D CPU-reference mode");
     if (!(rest_density > 0.0f) || !is_finite(rest_density))
@@ -1009,6 +1098,56 @@ int ps_add_gas_object(ps_system *s, uint32_t n, const float *pos, const float *v
     if (rc) return rc;
     if (n) s->objects.push_back({first, n, rest_density, open ? 2 : 1});
     if (first_out) *first_out = first;
+    return PS_OK;
+}
+
+int ps_add_smoke_emitter(ps_system *s, const float *pos, double particles_per_sec, int32_t object,
+                         uint32_t max_markers, uint32_t *emitter_out)
+{
+    if (int rc = check_sys(s)) return rc;
+    if (!s->cpuref || s->prm.dims != 2)
+        return fail(s, PS_ERR_UNSUPPORTED, "smoke emitters belong to the 2D CPU-reference mode");
+    if (s->stage) return fail(s, PS_ERR_STATE, "cannot add an emitter inside a substep");
+    if (!pos || !is_finite(pos[0]) || !is_finite(pos[1]) || !is_finite(pos[2]))
+        return fail(s, PS_ERR_INVALID_ARGUMENT, "emitter position must be finite");
+    if (!(particles_per_sec > 0.0) || !isfinite(particles_per_sec))
+        return fail(s, PS_ERR_INVALID_ARGUMENT, "particles_per_sec must be > 0");
+    if (object >= (int32_t)s->objects.size() || object < -1)
+        return fail(s, PS_ERR_INVALID_ARGUMENT, "object out of range");
+    if (object >= 0) {
+        const ps_system::FluidObject &o = s->objects[object];
+        if (!o.gas) return fail(s, PS_ERR_INVALID_ARGUMENT, "a smoke emitter feeds a gas object");
+        if (object + 1 != (int32_t)s->objects.size() || o.first + o.count != s->n)
+            return fail(s, PS_ERR_UNSUPPORTED, "a smoke emitter feeds the last object only");
+    }
+    ps_system::Emitter e{};
+    e.timer = 0.0;
+    e.rate = particles_per_sec;
+    for (int a = 0; a < 3; a++) e.pos[a] = pos[a];
+    e.object = object;
+    e.cap = max_markers;
+    HIP_TRY(s, hipMalloc((void **)&e.mark, sizeof(float4) * (max_markers ? max_markers : 1)));
+    drop_graph(s);
+    s->emitters.push_back(e);
+    if (emitter_out) *emitter_out = (uint32_t)s->emitters.size() - 1;
+    return PS_OK;
+}
+
+int ps_read_markers(ps_system *s, uint32_t emitter, uint32_t *count, float *xyz)
+{
+    if (int rc = check_sys(s)) return rc;
+    if (emitter >= s->emitters.size()) return fail(s, PS_ERR_INVALID_ARGUMENT, "emitter out of range");
+    const ps_system::Emitter &e = s->emitters[emitter];
+    if (count) *count = e.nmark;
+    if (!xyz || !e.nmark) return PS_OK;
+    std::vector<float4> m(e.nmark);
+    HIP_TRY(s, hipStreamSynchronize(s->stream));
+    HIP_TRY(s, hipMemcpy(m.data(), e.mark, sizeof(float4) * e.nmark, hipMemcpyDeviceToHost));
+    for (uint32_t k = 0; k < e.nmark; k++) {
+        xyz[3 * k] = m[k].x;
+        xyz[3 * k + 1] = m[k].y;
+        xyz[3 * k + 2] = m[k].z;
+    }
     return PS_OK;
 }
 
@@ -1353,6 +1492,9 @@ int ps_step(ps_system *s, float dt, uint32_t substeps)
     if (s->stage) return fail(s, PS_ERR_STATE, "ps_step inside an open substep");
     if (!(dt > 0.0f) || !is_finite(dt)) return fail(s, PS_ERR_INVALID_ARGUMENT, "dt must be > 0");
     if (s->n == 0) return PS_OK;
+    // emitters overflowing within the call fail it before the first substep
+    if (!s->emitters.empty())
+        if (int rc = emission_check(s, dt, substeps)) return rc;
     for (uint32_t k = 0; k < substeps; k++) {
         int rc;
         if ((rc = ps_substep_begin(s, dt))) return rc;
@@ -1369,6 +1511,8 @@ int ps_step_graph(ps_system *s, float dt, uint32_t substeps)
     if (s->stage) return fail(s, PS_ERR_STATE, "ps_step_graph inside an open substep");
     if (!(dt > 0.0f) || !is_finite(dt)) return fail(s, PS_ERR_INVALID_ARGUMENT, "dt must be > 0");
     if (s->n == 0 || substeps == 0) return PS_OK;
+    if (!s->emitters.empty())
+        return fail(s, PS_ERR_UNSUPPORTED, "ps_step_graph with smoke emitters (the particle count grows)");
     if (!s->gexec || s->g_dt != dt || s->g_sub != substeps) {
         drop_graph(s);
         const DevBuffers start = s->b;

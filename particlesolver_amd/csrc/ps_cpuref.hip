@@ -416,9 +416,107 @@ __global__ __launch_bounds__(kBlock) void cr_end_k(const float4 *__restrict__ E,
     X[i] = make_float4(e.x, e.y, e.z, 0.0f);
 }
 
+// OpenSmokeEmitter (opensmokeemitter.cpp:17-46): `count` emissions at pos.
+// Each writes a marker (slot first_mark + k) and, with first_part != ~0, a
+// gas particle at rest of inverse mass 1 (slot first_part + k, Particle's
+// constructor, cpu/src/particle.h:31-52).
+__global__ __launch_bounds__(64) void cr_emit_k(float4 *__restrict__ X, float4 *__restrict__ V,
+                                                Attr *__restrict__ A, float4 *__restrict__ F,
+                                                uint32_t *__restrict__ slot_of, float4 *__restrict__ M,
+                                                uint32_t first_mark, uint32_t first_part, uint32_t count,
+                                                float px, float py, float pz, float rho0)
+{
+    const uint32_t k = threadIdx.x;
+    if (k >= count) return;
+    M[first_mark + k] = make_float4(px, py, pz, 0.0f);
+    if (first_part == ~0u) return;
+    const uint32_t i = first_part + k;
+    X[i] = make_float4(px, py, pz, 0.0f);
+    V[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    F[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    Attr a;
+    a.w = 1.0f;
+    a.phase = PS_PHASE_GAS;
+    a.rho0 = rho0;
+    a.orig = i;
+    A[i] = a;
+    slot_of[i] = i;
+}
+
+// Marker advection (opensmokeemitter.cpp:37-45) and box clamp
+// (simulation.cpp:348-362): v = sum_j v_j W(|m - x_j|^2), s = sum_j W,
+// m += (v dt) / s, then x and y clamped to the box.  One wavefront per
+// marker; the terms are added in particle order (zero terms, which leave
+// the sums unchanged, are skipped).
+__global__ __launch_bounds__(kBlock) void cr_markers_k(float4 *__restrict__ M,
+                                                       const float4 *__restrict__ X,
+                                                       const float4 *__restrict__ V, CrConst c,
+                                                       uint32_t nmark, uint32_t n, float dt)
+{
+    const uint32_t m = blockIdx.x * kWaves + (threadIdx.x >> 6);
+    const uint32_t lane = threadIdx.x & 63u;
+    if (m >= nmark) return;  // whole wavefront
+    float4 q = M[m];
+    float vx = 0.0f, vy = 0.0f, vz = 0.0f, sum = 0.0f;
+    for (uint32_t base = 0; base < n; base += 64) {
+        const uint32_t j = base + lane;
+        const bool valid = j < n;
+        const float4 xj = X[valid ? j : 0];
+        const float rx = q.x - xj.x, ry = q.y - xj.y, rz = q.z - xj.z;
+        const float p6 = valid ? cr_poly6(c, ((rx * rx) + (ry * ry)) + (rz * rz)) : 0.0f;
+        uint64_t mask = __ballot(p6 != 0.0f);
+        if (!mask) continue;
+        float tx = 0.0f, ty = 0.0f, tz = 0.0f;
+        if (p6 != 0.0f) {
+            const float4 vj = V[j];
+            tx = vj.x * p6;
+            ty = vj.y * p6;
+            tz = vj.z * p6;
+        }
+        while (mask) {
+            const uint32_t b = (uint32_t)__builtin_ctzll(mask);
+            mask &= mask - 1;
+            vx = vx + lane_value(tx, b);
+            vy = vy + lane_value(ty, b);
+            vz = vz + lane_value(tz, b);
+            sum = sum + lane_value(p6, b);
+        }
+    }
+    if (lane != 0) return;
+    if (sum > 0.0f) {
+        q.x = q.x + ((vx * dt) / sum);
+        q.y = q.y + ((vy * dt) / sum);
+        q.z = q.z + ((vz * dt) / sum);
+    }
+    if (q.x < c.box_min[0]) q.x = c.box_min[0];
+    else if (q.x > c.box_max[0]) q.x = c.box_max[0];
+    if (q.y < c.box_min[1]) q.y = c.box_min[1];
+    else if (q.y > c.box_max[1]) q.y = c.box_max[1];
+    M[m] = q;
+}
+
 inline uint32_t blocks_for(uint32_t threads) { return (threads + kBlock - 1) / kBlock; }
 
 }  // namespace
+
+hipError_t launch_cr_emit(hipStream_t st, const DevBuffers &b, float4 *markers, uint32_t first_mark,
+                          uint32_t first_part, uint32_t count, const float *pos, float rho0)
+{
+    if (!count) return hipSuccess;
+    if (count > 64) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(cr_emit_k, dim3(1), dim3(64), 0, st, b.X, b.V, b.A, b.force, b.slot_of, markers,
+                       first_mark, first_part, count, pos[0], pos[1], pos[2], rho0);
+    return hipGetLastError();
+}
+
+hipError_t launch_cr_markers(hipStream_t st, const DevBuffers &b, float4 *markers, const CrConst &c,
+                             uint32_t nmark, uint32_t n, float dt)
+{
+    if (!nmark) return hipSuccess;
+    hipLaunchKernelGGL(cr_markers_k, dim3((nmark + kWaves - 1) / kWaves), dim3(kBlock), 0, st, markers,
+                       b.X, b.V, c, nmark, n, dt);
+    return hipGetLastError();
+}
 
 hipError_t launch_cr_predict(hipStream_t st, const DevBuffers &b, const CrConst &c, uint32_t n,
                              float dt)

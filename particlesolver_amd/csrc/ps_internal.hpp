@@ -251,5 +251,13 @@ hipError_t launch_cr_boundary(hipStream_t st, const DevBuffers &b, const CrConst
                               uint32_t it);
 hipError_t launch_cr_end(hipStream_t st, const DevBuffers &b, uint32_t n, float dt, bool sleeping,
                          uint32_t its);
+// smoke emitters: `count` (<= 64) emissions at pos into markers[first_mark..]
+// and, unless first_part == ~0u, gas particles of rest density rho0 at
+// [first_part, +count);
+// then the advection of markers[0, nmark) through the n particles
+hipError_t launch_cr_emit(hipStream_t st, const DevBuffers &b, float4 *markers, uint32_t first_mark,
+                          uint32_t first_part, uint32_t count, const float *pos, float rho0);
+hipError_t launch_cr_markers(hipStream_t st, const DevBuffers &b, float4 *markers, const CrConst &c,
+                             uint32_t nmark, uint32_t n, float dt);
 
 }  // namespace ps

@@ -384,13 +384,16 @@ class FluidObjectsScene:
     cpu/src/simulation.cpp:431-449), and gas objects (positions, rest
     density, "gas" or "open gas"), added with ps_add_gas_object
     (Simulation::createGas, :408-428).  An object is (pos, rho0) or
-    (pos, rho0, kind)."""
+    (pos, rho0, kind).  Smoke emitters (Simulation::createSmokeEmitter,
+    :454-457) follow the objects, each (pos, particles_per_sec, object,
+    max_markers) with object the index of the gas object it feeds or -1."""
 
-    def __init__(self, params, objects, dt, name=""):
+    def __init__(self, params, objects, dt, name="", emitters=()):
         self.params = params
         self.objects = objects
         self.dt = dt
         self.name = name
+        self.emitters = list(emitters)
 
     @property
     def n(self):
@@ -404,6 +407,8 @@ class FluidObjectsScene:
                 sys_.add_fluid_object(pos, None, 1.0, rho0)
             else:
                 sys_.add_gas_object(pos, None, 1.0, rho0, open=(kind == "open gas"))
+        for e in self.emitters:
+            sys_.add_smoke_emitter(*e)
 
 
 def cpu_reference_params(box_min, box_max, radius, h, iterations, max_particles, dims=2,
@@ -541,23 +546,30 @@ def reference_gas(seed=0x5EED0008, dt=0.01):
     return FluidObjectsScene(p, objects, dt, name="reference_gas")
 
 
-def reference_smoke(seed=0x5EED0009, dt=0.01, open_gas=True):
-    """The gas body of the CPU app's smoke scenes, Simulation::initSmokeOpen
-    and initSmokeClosed (cpu/src/simulation.cpp:1096-1134): one gas object
-    of rest density 1.5, lattice step .63 over x in [-4, 4), y in [-4, 4);
-    open: box x in [-6, 6], y in [-4, 200] and the open-gas drag; closed:
-    box [-4, 4] x [-4, 4].  (Their emitters are not part of this scene.)"""
+def reference_smoke(seed=0x5EED0009, dt=0.01, open_gas=True, emitter=False, max_emitted=512):
+    """The CPU app's smoke scenes, Simulation::initSmokeOpen and
+    initSmokeClosed (cpu/src/simulation.cpp:1096-1134): one gas object of
+    rest density 1.5, lattice step .63 over x in [-4, 4), y in [-4, 4); open:
+    box x in [-6, 6], y in [-4, 200] and the open-gas drag; closed: box
+    [-4, 4] x [-4, 4].  With `emitter`, their smoke emitter at (0, -3), 15
+    per second: feeding the gas object (open) or markers only (closed), with
+    room for `max_emitted` emissions."""
     scale, delta = 2.0, 0.63
     pos = _reference_lattice(-2.0 * scale, 2.0 * scale, -2.0 * scale, 2.0 * scale, delta, seed)
+    cap = pos.shape[0] + (max_emitted if emitter and open_gas else 0)
     if open_gas:
         p = cpu_reference_params((-3.0 * scale, -2.0 * scale, -1.0), (3.0 * scale, 100.0 * scale, 1.0),
-                                 0.25, 2.0, 3, pos.shape[0])
+                                 0.25, 2.0, 3, cap)
     else:
         p = cpu_reference_params((-2.0 * scale, -2.0 * scale, -1.0), (2.0 * scale, 2.0 * scale, 1.0),
-                                 0.25, 2.0, 3, pos.shape[0])
+                                 0.25, 2.0, 3, cap)
     kind = "open gas" if open_gas else "gas"
-    return FluidObjectsScene(p, [(pos, 1.5, kind)], dt,
-                             name="reference_smoke_%s" % ("open" if open_gas else "closed"))
+    emitters = []
+    if emitter:
+        emitters.append(((0.0, -2.0 * scale + 1.0, 0.0), 15.0, 0 if open_gas else -1, max_emitted))
+    return FluidObjectsScene(p, [(pos, 1.5, kind)], dt, emitters=emitters,
+                             name="reference_smoke_%s%s" % ("open" if open_gas else "closed",
+                                                            "_emitter" if emitter else ""))
 
 
 def fluid_object_metrics(params, objects, pos):

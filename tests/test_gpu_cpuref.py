@@ -26,6 +26,8 @@ def run_parity(sc, checkpoints):
             done = k
             assert_bitexact(g.positions(), ref.positions(), "positions after %d steps" % k)
             assert_bitexact(g.velocities(), ref.velocities(), "velocities after %d steps" % k)
+            for e in range(len(getattr(sc, "emitters", ()))):
+                assert_bitexact(g.markers(e), ref.markers(e), "markers of %d after %d steps" % (e, k))
         st = g.status()
         assert st.flags == 0
         assert st.iterations == done * sc.params.iterations
@@ -132,6 +134,62 @@ def test_reference_gas_bitexact(gpu_available):
 def test_smoke_gas_bitexact(gpu_available, open_gas):
     x = run_parity(scenes.reference_smoke(open_gas=open_gas), [1, 10, 100])
     assert np.all(np.isfinite(x))
+
+
+@pytest.mark.parametrize("open_gas", [True, False])
+def test_smoke_emitter_bitexact(gpu_available, open_gas):
+    # initSmokeOpen: the emitter feeds the open gas; initSmokeClosed: markers only
+    sc = scenes.reference_smoke(open_gas=open_gas, emitter=True)
+    n0 = sc.n
+    run_parity(sc, [1, 7, 10, 100, 300])
+    with System(sc.params) as g:
+        sc.load_into(g)
+        g.step(sc.dt, 300)
+        assert g.markers(0).shape[0] == 44          # 300 x 0.01f s at 15/s
+        assert g.n == n0 + (44 if open_gas else 0)
+
+
+def test_smoke_emitter_rules(gpu_available):
+    sc = scenes.reference_smoke(open_gas=True, emitter=False)
+    p = sc.params.copy()
+    p.max_particles = sc.n + 2
+    with System(p) as g:
+        sc.load_into(g)
+        with pytest.raises(capi.PsError) as e:       # a fluid object cannot be fed
+            g.add_fluid_object(np.array([[5.0, 5.0, 0.0]], np.float32), None, 1.0, 1.0)
+            g.add_smoke_emitter((0.0, 0.0, 0.0), 15.0, 1, 8)
+        assert e.value.code == capi.PS_ERR_INVALID_ARGUMENT
+    with System(p) as g:
+        sc.load_into(g)
+        g.add_fluid_object(np.array([[5.0, 5.0, 0.0]], np.float32), None, 1.0, 1.0)
+        with pytest.raises(capi.PsError) as e:       # only the last object
+            g.add_smoke_emitter((0.0, 0.0, 0.0), 15.0, 0, 8)
+        assert e.value.code == capi.PS_ERR_UNSUPPORTED
+    with System(p) as g:
+        sc.load_into(g)
+        g.add_smoke_emitter((0.0, -3.0, 0.0), 15.0, 0, 8)
+        with pytest.raises(capi.PsError) as e:       # nothing may follow it
+            g.add_fluid_object(np.array([[5.0, 5.0, 0.0]], np.float32), None, 1.0, 1.0)
+        assert e.value.code == capi.PS_ERR_STATE
+        with pytest.raises(capi.PsError) as e:
+            g.step_graph(sc.dt, 12)
+        assert e.value.code == capi.PS_ERR_UNSUPPORTED
+        # room for 2 emissions: 14 substeps emit 2, 21 would emit 3 and fail
+        # before the first substep, leaving the handle as it was
+        x0 = g.positions()
+        with pytest.raises(capi.PsError) as e:
+            g.step(sc.dt, 21)
+        assert e.value.code == capi.PS_ERR_CAPACITY
+        assert np.array_equal(g.positions(), x0) and g.status().iterations == 0
+        g.step(sc.dt, 14)
+        assert g.n == sc.n + 2 and g.markers(0).shape[0] == 2
+    q = sc.params.copy()
+    q.dims = 3
+    q.box_min[2], q.box_max[2] = -4.0, 4.0
+    with System(q) as g:
+        with pytest.raises(capi.PsError) as e:
+            g.add_smoke_emitter((0.0, 0.0, 0.0), 15.0, -1, 8)
+        assert e.value.code == capi.PS_ERR_UNSUPPORTED
 
 
 def test_gas_objects_need_the_2d_cpu_reference_mode(gpu_available):
