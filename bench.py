@@ -555,6 +555,23 @@ def main():
             if launches:
                 breakdown[name] = round(1000.0 * ms / launches, 2)
         g.timing(False)
+    # insertion-order export of the positions (what a renderer copies into
+    # its vertex buffer each frame): reads X and A, writes one float4
+    export = None
+    if not args.no_timing:
+        from particlesolver_amd import capi
+        with capi.DeviceBuffer(16 * N) as buf:
+            g.timing({"export"})
+            g.timing_reset()
+            for _ in range(20):
+                g.export_device(capi.PS_FIELD_POSITIONS, buf.ptr)
+            g.synchronize()
+            ms, launches = g.timing_read()["export"]
+            g.timing(False)
+        us = 1000.0 * ms / launches
+        export = {"kernel": "export_vec_k", "us_per_launch": round(us, 2),
+                  "algo_bytes_per_particle": 48,
+                  "algo_GBs": round(48.0 * N / (us * 1e-6) / 1e9, 1)}
     g.close()
 
     value = N * I * args.steps / el
@@ -611,6 +628,7 @@ def main():
         "cpu_baseline": cpu,
         "kernels": kernels,
         "breakdown_us_per_launch_all_events": breakdown,
+        "export": export,
     }
     if not args.no_adjoint:
         r = adjoint_measure(args, cpu=False)

@@ -50,7 +50,8 @@ typedef enum ps_error {
     PS_ERR_STATE = -5,            /* call out of order (e.g. iterate w/o begin) */
     PS_ERR_OUT_OF_MEMORY = -6,    /* device allocation failed                   */
     PS_ERR_NOT_FINITE = -7,       /* non-finite input values                    */
-    PS_ERR_UNSUPPORTED = -8       /* call not available in this solver mode     */
+    PS_ERR_UNSUPPORTED = -8,      /* call not available in this solver mode     */
+    PS_ERR_IO = -9                /* a file could not be opened or written      */
 } ps_error;
 
 /* Flags for ps_params.flags. */
@@ -66,9 +67,10 @@ typedef enum ps_error {
  * totalfluidconstraint.cpp:41-116): lambda_j = 0 for neighbours outside the
  * object, dp = sum / rho0 / (neighbours + self + boundary count), tensile
  * term -k_corr (W/W(dq h))^n with k_corr unscaled.  Neighbours come from an
- * all-pairs scan summed in index order; no XSPH.  Particles enter only
- * through ps_add_fluid_object; grid and max_neighbors are unused. Set
- * PS_FLAG_SLEEPING too for the reference's behaviour (it always sleeps). */
+ * all-pairs scan summed in index order; no XSPH.  Particles enter through
+ * ps_add_fluid_object, ps_add_gas_object (2D) and smoke emitters; grid and
+ * max_neighbors are unused. Set PS_FLAG_SLEEPING too for the reference's
+ * behaviour (it always sleeps). */
 #define PS_FLAG_CPU_REFERENCE 0x4u
 /* Read neighbours from global memory even where the kernels could stage
  * each workgroup's neighbour cells in LDS (uniform single-scene fluids).
@@ -370,6 +372,35 @@ uint32_t ps_num_particles(const ps_system *sys); /* getNumParticles, particlesys
 int ps_read_positions(ps_system *sys, uint32_t first, uint32_t n, float *xyz);
 int ps_read_velocities(ps_system *sys, uint32_t first, uint32_t n, float *xyz);
 
+/* ---- export to device memory and trajectory files ------------------------ */
+
+enum ps_field { PS_FIELD_POSITIONS = 1, PS_FIELD_VELOCITIES = 2, PS_FIELD_PHASES = 4 };
+
+/* One field of particles [first, first+n) in insertion order, written to
+ * DEVICE memory dst without a host round trip: PS_FIELD_POSITIONS as float4
+ * (x, y, z, 1), the layout of the reference's position VBO
+ * (particlesystem.cpp:95-96, mapped for the solver by mapGLBufferObject,
+ * util.cu:39-62); PS_FIELD_VELOCITIES as float4 (vx, vy, vz, 0);
+ * PS_FIELD_PHASES as int32.  dst is any device-accessible pointer, e.g. a GL
+ * buffer mapped with hipGraphicsMapResources +
+ * hipGraphicsResourceGetMappedPointer.  Enqueued on the system's stream
+ * without waiting: ps_synchronize (or any readback) orders the caller after
+ * it. */
+int ps_export_device(ps_system *sys, uint32_t field, uint32_t first, uint32_t n, void *dst);
+
+/* Binary trajectory files (the reference has no on-disk format).
+ * Little-endian.  File header, 32 bytes: "PSTRAJ01" (8 bytes), uint32
+ * version = 1, uint32 fields (PS_FIELD_* bits), uint32 dims, 12 zero bytes.
+ * Each frame: uint64 substeps completed (ps_status.substeps), uint32 n,
+ * uint32 fields, then per field in bit order, n records in insertion order:
+ * positions and velocities as 3 float32, phases as int32.  The particle
+ * count may change between frames (emitters).  ps_trajectory_write appends
+ * the system's current state (it synchronises the system's stream). */
+typedef struct ps_trajectory ps_trajectory;
+int ps_trajectory_open(const char *path, uint32_t fields, uint32_t dims, ps_trajectory **out);
+int ps_trajectory_write(ps_trajectory *traj, ps_system *sys);
+int ps_trajectory_close(ps_trajectory *traj);
+
 /* Internal sorted-order state, for parity tests and tools.
  * sorted_index[t] = insertion index of the particle in sorted slot t;
  * cell[t] = its cell key; cell_offsets: num_cells+1 entries, particles of
@@ -408,6 +439,8 @@ typedef enum ps_kernel_id {
     PS_K_ADJ_POSITION,
     PS_K_ADJ_STIFFNESS,
     PS_K_ADJ_PREDICT,
+    /* insertion-order export (ps_export_device, trajectory files) */
+    PS_K_EXPORT,
     PS_K_COUNT
 } ps_kernel_id;
 

@@ -29,6 +29,11 @@ PS_ERR_STATE = -5
 PS_ERR_OUT_OF_MEMORY = -6
 PS_ERR_NOT_FINITE = -7
 PS_ERR_UNSUPPORTED = -8
+PS_ERR_IO = -9
+
+PS_FIELD_POSITIONS = 1
+PS_FIELD_VELOCITIES = 2
+PS_FIELD_PHASES = 4
 
 PS_FLAG_REBUILD_EVERY_ITERATION = 0x1
 PS_FLAG_SLEEPING = 0x2
@@ -42,7 +47,7 @@ PS_STATUS_INTERNAL = 0x80000000
 
 KERNELS = ["predict_hash", "scan", "scatter", "reorder", "neighbors", "lambda", "solve",
            "shape", "finalize", "adj_xsph", "adj_finalize", "adj_pre", "adj_lambda",
-           "adj_position", "adj_stiffness", "adj_predict"]
+           "adj_position", "adj_stiffness", "adj_predict", "export"]
 K_PREDICT_HASH, K_SCAN, K_SCATTER, K_REORDER, K_NEIGHBORS, K_LAMBDA, K_SOLVE, K_SHAPE, \
     K_FINALIZE = range(9)
 
@@ -139,6 +144,10 @@ PROTOTYPES = {
     "ps_add_smoke_emitter": (_int, [_P, _P, ctypes.c_double, ctypes.c_int32, _u32,
                                     ctypes.POINTER(_u32)]),
     "ps_read_markers": (_int, [_P, _u32, ctypes.POINTER(_u32), _P]),
+    "ps_export_device": (_int, [_P, _u32, _u32, _u32, _P]),
+    "ps_trajectory_open": (_int, [ctypes.c_char_p, _u32, _u32, ctypes.POINTER(_P)]),
+    "ps_trajectory_write": (_int, [_P, _P]),
+    "ps_trajectory_close": (_int, [_P]),
     "ps_add_distance_constraints": (_int, [_P, _u32, _P, _P, _P]),
     "ps_add_point_constraints": (_int, [_P, _u32, _P, _P]),
     "ps_add_rigid_body": (_int, [_P, _u32, _P, _f32, ctypes.POINTER(_u32)]),
@@ -189,6 +198,48 @@ def load(path=None):
     if path is None:
         _lib = lib
     return lib
+
+
+class DeviceBuffer:
+    """Device memory from the HIP runtime the library itself links
+    (libamdhip64.so.7), for ps_export_device in tests and tools: no second
+    runtime (such as the one bundled with torch) enters the process."""
+
+    def __init__(self, nbytes):
+        load()
+        self.hip = ctypes.CDLL("libamdhip64.so.7")
+        self.nbytes = int(nbytes)
+        p = ctypes.c_void_p()
+        rc = self.hip.hipMalloc(ctypes.byref(p), ctypes.c_size_t(max(self.nbytes, 1)))
+        if rc:
+            raise PsError(PS_ERR_OUT_OF_MEMORY, "hipMalloc(%d) failed: %d" % (self.nbytes, rc))
+        self.ptr = p.value
+
+    def fill(self, byte):
+        rc = self.hip.hipMemset(ctypes.c_void_p(self.ptr), int(byte), ctypes.c_size_t(self.nbytes))
+        if rc:
+            raise PsError(PS_ERR_DEVICE, "hipMemset failed: %d" % rc)
+
+    def numpy(self, dtype, shape):
+        """synchronous copy to the host (hipMemcpy waits for prior work)"""
+        out = np.empty(shape, dtype)
+        assert out.nbytes <= self.nbytes
+        rc = self.hip.hipMemcpy(out.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(self.ptr),
+                                ctypes.c_size_t(out.nbytes), 2)     # hipMemcpyDeviceToHost
+        if rc:
+            raise PsError(PS_ERR_DEVICE, "hipMemcpy failed: %d" % rc)
+        return out
+
+    def free(self):
+        if self.ptr:
+            self.hip.hipFree(ctypes.c_void_p(self.ptr))
+            self.ptr = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.free()
 
 
 class PsError(RuntimeError):
@@ -318,6 +369,12 @@ class System:
         if c.value:
             self._check(self.lib.ps_read_markers(self.h, emitter, ctypes.byref(c), _ptr(out)))
         return out
+
+    def export_device(self, field, dst_ptr, first=0, n=None):
+        """ps_export_device: field PS_FIELD_* of [first, first+n) into the
+        device pointer dst_ptr (an int), enqueued on the system's stream"""
+        n = self.n - first if n is None else n
+        self._check(self.lib.ps_export_device(self.h, field, first, n, ctypes.c_void_p(dst_ptr)))
 
     def add_distance_constraints(self, pairs, rest, stiffness=None):
         pairs = np.ascontiguousarray(pairs, dtype=np.uint32).reshape(-1, 2)

@@ -89,8 +89,8 @@ struct ps_system {
     bool cr_bnd_done = false;  // boundaries of iteration it_sub already projected
     bool cr_swapped = false;   // iterate lives in b.Q (odd number of object passes)
     // workgroup size of each kernel (ps_autotune)
-    uint32_t block[PS_K_COUNT] = {256, 256, 256, 256, 256, 256, 256, 256, 256,
-                                  256, 256, 256, 256, 256, 256, 256};  // reverse pass: fixed 256
+    uint32_t block[PS_K_COUNT] = {256, 256, 256, 256, 256, 256, 256, 256, 256, 256,
+                                  256, 256, 256, 256, 256, 256, 256};  // reverse pass, export: fixed 256
     // timing
     uint32_t timing = 0;  // kernel mask
     std::vector<TimedLaunch> pending;
@@ -1906,6 +1906,17 @@ int ps_step_backward(ps_system *s, float dt, uint32_t substeps, uint32_t checkpo
     return rc;
 }
 
+int ps_export_device(ps_system *s, uint32_t field, uint32_t first, uint32_t n, void *dst)
+{
+    if (int rc = check_sys(s)) return rc;
+    if (field != PS_FIELD_POSITIONS && field != PS_FIELD_VELOCITIES && field != PS_FIELD_PHASES)
+        return fail(s, PS_ERR_INVALID_ARGUMENT, "field must be one PS_FIELD_* value");
+    if ((uint64_t)first + n > s->n || (n && !dst))
+        return fail(s, PS_ERR_INVALID_ARGUMENT, "range out of bounds");
+    LAUNCH(s, PS_K_EXPORT, ps::launch_export(s->stream, s->b, s->n, field, first, n, dst));
+    return PS_OK;
+}
+
 int ps_synchronize(ps_system *s)
 {
     if (int rc = check_sys(s)) return rc;
@@ -2291,7 +2302,7 @@ const char *ps_kernel_name(int kernel_id)
     static const char *names[PS_K_COUNT] = {"predict_hash", "scan", "scatter", "reorder", "neighbors",
                                             "lambda", "solve", "shape", "finalize", "adj_xsph",
                                             "adj_finalize", "adj_pre", "adj_lambda", "adj_position",
-                                            "adj_stiffness", "adj_predict"};
+                                            "adj_stiffness", "adj_predict", "export"};
     if (kernel_id < 0 || kernel_id >= PS_K_COUNT) return "unknown";
     return names[kernel_id];
 }

@@ -257,6 +257,9 @@ hipError_t launch_cr_end(hipStream_t st, const DevBuffers &b, uint32_t n, float 
 // then the advection of markers[0, nmark) through the n particles
 hipError_t launch_cr_emit(hipStream_t st, const DevBuffers &b, float4 *markers, uint32_t first_mark,
                           uint32_t first_part, uint32_t count, const float *pos, float rho0);
+// insertion-order export of field PS_FIELD_* of records [first, first+count)
+hipError_t launch_export(hipStream_t st, const DevBuffers &b, uint32_t n, uint32_t field,
+                         uint32_t first, uint32_t count, void *dst);
 hipError_t launch_cr_markers(hipStream_t st, const DevBuffers &b, float4 *markers, const CrConst &c,
                              uint32_t nmark, uint32_t n, float dt);
 
