@@ -68,9 +68,10 @@ typedef enum ps_error {
  * object, dp = sum / rho0 / (neighbours + self + boundary count), tensile
  * term -k_corr (W/W(dq h))^n with k_corr unscaled.  Neighbours come from an
  * all-pairs scan summed in index order; no XSPH.  Particles enter through
- * ps_add_fluid_object, ps_add_gas_object (2D) and smoke emitters; grid and
- * max_neighbors are unused. Set PS_FLAG_SLEEPING too for the reference's
- * behaviour (it always sleeps). */
+ * ps_add_fluid_object, ps_add_gas_object (2D), emitters (2D) and, for
+ * immovable solids only (phase PS_PHASE_SOLID, inverse mass 0),
+ * ps_add_particles; grid and max_neighbors are unused. Set PS_FLAG_SLEEPING
+ * too for the reference's behaviour (it always sleeps). */
 #define PS_FLAG_CPU_REFERENCE 0x4u
 /* Read neighbours from global memory even where the kernels could stage
  * each workgroup's neighbour cells in LDS (uniform single-scene fluids).
@@ -265,6 +266,34 @@ int ps_add_gas_object(ps_system *sys, uint32_t n, const float *pos, const float 
  * PS_ERR_UNSUPPORTED on a system with emitters. */
 int ps_add_smoke_emitter(ps_system *sys, const float *pos, double particles_per_sec, int32_t object,
                          uint32_t max_markers, uint32_t *emitter_out);
+
+/* Add a fluid emitter (PS_FLAG_CPU_REFERENCE mode, dims = 2) at pos[0..3)
+ * feeding fluid object `object`, the last object added (nothing may be
+ * added after it).  Replaces Simulation::createFluidEmitter
+ * (cpu/src/simulation.cpp:459-461) and FluidEmitter::tick
+ * (cpu/src/fluidemitter.cpp:13-79), run at the end of every substep after
+ * the smoke emitters, in creation order:
+ *  - every particle of the object slower than .06 at height <= 5 counts
+ *    its timer t (4 at creation) down by 1 when lambdas[k] <= 0 and turns
+ *    into an immovable solid at t <= 0 (inverse mass 0, phase SOLID, v = 0,
+ *    out of the object); otherwise t gains dt, up to 3.  As in the
+ *    reference, k is the particle's position in the object's list and
+ *    `lambdas` is keyed by particle index: the lambda of particle k if that
+ *    one was in the object at its last projection, else 0;
+ *  - while the emitter's total time (double, this substep included) is
+ *    below 5 s, one fluid particle of inverse mass 1 per 1/particles_per_sec
+ *    of its timer, at pos with v = (u, 1, 0).  u replaces frand(): the top
+ *    24 bits of SplitMix64(seed + golden (2^62 | emitter << 40 | emission
+ *    index) + 1), divided by 2^24.
+ * Solids (made here or added with ps_add_particles) make the substep find
+ * contacts: ContactConstraint (contactconstraint.cpp:14-40) between a solid
+ * and any other particle closer than 2 radius - 1e-4 after prediction,
+ * projected with the height-scaled mass of Particle::scaleMass
+ * (particle.h:67-73) before the boundaries of each iteration.  ps_step
+ * fails with PS_ERR_CAPACITY before its first substep when emissions would
+ * exceed max_particles; ps_step_graph is PS_ERR_UNSUPPORTED. */
+int ps_add_fluid_emitter(ps_system *sys, const float *pos, double particles_per_sec, int32_t object,
+                         uint32_t *emitter_out);
 
 /* Markers of an emitter in emission order (OpenSmokeEmitter::getParticles,
  * opensmokeemitter.cpp:48-51): *count receives their number and, unless

@@ -317,6 +317,38 @@ class Ref2D:
         self.n += n
         return first
 
+    def add_static_particles(self, pos):
+        """immovable solids, new Particle(pos, 0) (particle.h:31-52): mass
+        0, phase SOLID; a contact partner of every other particle"""
+        pos = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1, 3)
+        n = pos.shape[0]
+        w = np.zeros(n, np.float32)
+        ph = np.full(n, 3, np.int32)
+        first = self.n
+        rc = self._f("r2d_add_particles")(self.h, n, _ptr(pos), None, _ptr(w), _ptr(ph))
+        if rc:
+            raise RuntimeError("r2d_add_particles failed: %d" % rc)
+        self.n += n
+        return first
+
+    def add_fluid_emitter(self, pos, particles_per_sec, object):
+        """createFluidEmitter (simulation.cpp:459-461) feeding fluid object
+        `object` (index in creation order)"""
+        p = np.ascontiguousarray(pos, dtype=np.float32).reshape(3)
+        f = self._f("r2d_add_fluid_emitter")
+        f.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_int32]
+        rc = f(self.h, _ptr(p), float(particles_per_sec), int(object))
+        if rc:
+            raise RuntimeError("r2d_add_fluid_emitter failed: %d" % rc)
+
+    def attrs(self):
+        """(phase, inverse mass, solidification timer t) per particle"""
+        ph = np.zeros(self.n, np.int32)
+        w = np.zeros(self.n, self.dtype)
+        t = np.zeros(self.n, self.dtype)
+        self._f("r2d_read_attrs")(self.h, _ptr(ph), _ptr(w), _ptr(t))
+        return ph, w, t
+
     def add_smoke_emitter(self, pos, particles_per_sec, object=-1, max_markers=4096):
         """createSmokeEmitter (simulation.cpp:454-457) feeding gas object
         `object` (an index in creation order, -1: markers only)"""

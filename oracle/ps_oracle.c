@@ -85,30 +85,8 @@ static real DIST2(real dx, real dy, real dz) { return FMA(dz, dz, FMA(dy, dy, dx
 /* below this squared distance a pair is degenerate (no direction) */
 #define DEGENERATE_R2 R(1e-8)
 
-#if ORC_DOUBLE
-#define EXP exp
-#else
-/* e^x for the contact mass scaling: Cody-Waite reduction x = k ln2 + r
- * (ln2 split into a 17-bit head and a tail), degree-7 Taylor polynomial of
- * e^r by Horner's rule, exact scaling by 2^k.  Only basic IEEE operations,
- * so the GPU's version of the same steps gives the same float. */
-static float EXP(float x)
-{
-    if (x > 80.0f) x = 80.0f;
-    if (x < -80.0f) x = -80.0f;
-    float k = floorf((x * 1.44269504088896341f) + 0.5f);
-    float r = (x - (k * 0.693145751953125f)) - (k * 1.42860682030941723e-6f);
-    float p = 1.98412698412698413e-4f;
-    p = (p * r) + 1.38888888888888889e-3f;
-    p = (p * r) + 8.33333333333333333e-3f;
-    p = (p * r) + 4.16666666666666667e-2f;
-    p = (p * r) + 1.66666666666666667e-1f;
-    p = (p * r) + 0.5f;
-    p = (p * r) + 1.0f;
-    p = (p * r) + 1.0f;
-    return ldexpf(p, (int)k);
-}
-#endif
+#include "orc_exp.h"
+#define EXP orc_exp
 
 #define ORC_EXPORT __attribute__((visibility("default")))
 

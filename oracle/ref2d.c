@@ -44,6 +44,8 @@ typedef float real;
 #define R(x) ((real)(x))
 #define EXPORT __attribute__((visibility("default")))
 
+#include "orc_exp.h"
+
 #define BX_LOW 1u
 #define BX_HIGH 2u
 #define BY_LOW 4u
@@ -70,6 +72,7 @@ static real uniform(uint64_t seed, uint64_t iter, uint32_t i, uint32_t axis)
 
 typedef struct r2d_sys r2d_sys;
 static real poly6(const r2d_sys *s, real r2);
+static real length3(const real *r);
 
 /* GasConstraint's compile-time constants (gasconstraint.h:10-19) and the
  * gas gravity scale (simulation.h:21) */
@@ -80,6 +83,16 @@ static real poly6(const r2d_sys *s, real r2);
 #define GAS_S_SOLID 0.5
 #define GAS_ALPHA (-0.2)
 #define GAS_OPEN_DRAG (-50.0)
+
+/* FluidEmitter (cpu/src/fluidemitter.cpp:3-79): double timers, the fluid
+ * object it feeds and solidifies, and the count of its emissions (the
+ * counter of the frand() replacement) */
+typedef struct {
+    double timer, total, rate;
+    real pos[3];
+    int32_t group;
+    uint64_t emitted;
+} r2d_femitter;
 
 /* OpenSmokeEmitter (cpu/src/opensmokeemitter.cpp:3-51): a double timer,
  * markers (diffuse particles, not simulated) and an optional gas object
@@ -95,6 +108,15 @@ typedef struct {
 struct r2d_sys {
     r2d_emitter em[16];
     uint32_t nem;
+    r2d_femitter fem[4];
+    uint32_t nfem;
+    real *tm;          /* tmass, Particle::scaleMass (particle.h:67-73) */
+    real *tt;          /* Particle::t, the FluidEmitter's solidification timer */
+    uint32_t *cstart;  /* contacts created while visiting particle i: [cstart[i], cstart[i+1]) */
+    uint32_t *cpair;   /* 2 per contact, in creation order */
+    uint32_t ncontact, ccap;
+    uint32_t *members; /* scratch: a fluid object's particles */
+    uint8_t *snap;     /* scratch: membership before a FluidEmitter tick */
     ps_params prm;
     uint32_t n, cap, ngroup;
     real *p, *ep, *v; /* 3 per particle */
@@ -132,6 +154,11 @@ EXPORT int SFX(r2d_create)(const ps_params *prm, r2d_sys **out)
     s->lambda = (real *)calloc(c, sizeof(real));
     s->delta = (real *)calloc(3 * c, sizeof(real));
     s->nnb = (uint32_t *)calloc(c, sizeof(uint32_t));
+    s->tm = (real *)calloc(c, sizeof(real));
+    s->tt = (real *)calloc(c, sizeof(real));
+    s->cstart = (uint32_t *)calloc(c + 1, sizeof(uint32_t));
+    s->members = (uint32_t *)calloc(c, sizeof(uint32_t));
+    s->snap = (uint8_t *)calloc(c, 1);
     /* constants as totalfluidconstraint.cpp:123-137 with a runtime H */
     const double pi = 3.14159265358979323846, h = prm->h;
     const double h2 = h * h, h6 = h2 * h2 * h2, h9 = h6 * h2 * h;
@@ -179,7 +206,8 @@ EXPORT void SFX(r2d_destroy)(r2d_sys *s)
 {
     if (!s) return;
     void *ptrs[] = {s->p, s->ep, s->v, s->force, s->w, s->phase, s->group, s->group_rho0,
-                    s->group_gas, s->bflags, s->counts, s->lambda, s->delta, s->nnb};
+                    s->group_gas, s->bflags, s->counts, s->lambda, s->delta, s->nnb,
+                    s->tm, s->tt, s->cstart, s->cpair, s->members, s->snap};
     for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) free(ptrs[i]);
     for (uint32_t e = 0; e < s->nem; e++) free(s->em[e].mark);
     free(s);
@@ -189,6 +217,11 @@ EXPORT int SFX(r2d_add_particles)(r2d_sys *s, uint32_t n, const float *pos, cons
                                   const float *w, const int32_t *phase)
 {
     if ((uint64_t)s->n + n > s->cap) return PS_ERR_CAPACITY;
+    /* solids only as immovable particles (new Particle(pos, 0), mass 0 =>
+     * imass 0): their contacts are ContactConstraints; movable solids would
+     * need RigidContactConstraint, which this restatement leaves out */
+    for (uint32_t k = 0; k < n; k++)
+        if (phase[k] == PS_PHASE_SOLID && w[k] != 0.0f) return PS_ERR_UNSUPPORTED;
     for (uint32_t k = 0; k < n; k++) {
         uint32_t i = s->n + k;
         for (int a = 0; a < 3; a++) {
@@ -198,6 +231,7 @@ EXPORT int SFX(r2d_add_particles)(r2d_sys *s, uint32_t n, const float *pos, cons
         s->w[i] = w[k];
         s->phase[i] = phase[k];
         s->group[i] = -1;
+        s->tt[i] = R(4); /* Particle::init, particle.h:37-38 */
     }
     s->n += n;
     return PS_OK;
@@ -301,6 +335,128 @@ static int emitter_tick(r2d_sys *s, r2d_emitter *e, real dt)
             else if (q[a] > R(s->prm.box_max[a]))
                 q[a] = R(s->prm.box_max[a]);
         }
+    }
+    return PS_OK;
+}
+
+/* createFluidEmitter (simulation.cpp:459-461): particles_per_sec fluid
+ * particles into fluid object `group` for its first 5 seconds */
+EXPORT int SFX(r2d_add_fluid_emitter)(r2d_sys *s, const float *pos, double rate, int32_t group)
+{
+    if (s->nfem >= 4 || !(rate > 0.0)) return PS_ERR_INVALID_ARGUMENT;
+    if (group < 0 || group >= (int32_t)s->ngroup || s->group_gas[group]) return PS_ERR_INVALID_ARGUMENT;
+    r2d_femitter *e = &s->fem[s->nfem];
+    memset(e, 0, sizeof(*e));
+    e->rate = rate;
+    for (int a = 0; a < 3; a++) e->pos[a] = pos[a];
+    e->group = group;
+    s->nfem++;
+    return PS_OK;
+}
+
+/* phases, inverse masses and solidification timers (t) of every particle */
+EXPORT int SFX(r2d_read_attrs)(const r2d_sys *s, int32_t *phase, real *w, real *t)
+{
+    if (phase) memcpy(phase, s->phase, sizeof(int32_t) * s->n);
+    if (w) memcpy(w, s->w, sizeof(real) * s->n);
+    if (t) memcpy(t, s->tt, sizeof(real) * s->n);
+    return PS_OK;
+}
+
+/* frand() of an emitted particle's velocity (fluidemitter.cpp:74), in
+ * [0, 1): SplitMix64 finaliser of seed + golden * counter + 1 with counter
+ * 2^62 | emitter << 40 | emission, top 24 bits (a stream apart from the
+ * boundary jitter's counters, which stay below 2^62) */
+static real emit_uniform(uint64_t seed, uint32_t emitter, uint64_t k)
+{
+    uint64_t ctr = (1ull << 62) | ((uint64_t)emitter << 40) | k;
+    uint64_t z = mix64(seed + ctr * 0x9E3779B97F4A7C15ull + 1u);
+    return R((double)(uint32_t)(z >> 40) * (1.0 / 16777216.0));
+}
+
+/* ContactConstraint::project (cpu/src/constraint/contactconstraint.cpp:
+ * 14-40) with the height-scaled masses tmass and the particles' counts:
+ * diff = ep1 - ep2, mag = |diff| - diameter; when the pair overlaps,
+ * dp = (mag / (t1 + t2) / |diff|) diff, ep1 -= t1 dp / n1, ep2 += t2 dp / n2 */
+static void contact(r2d_sys *s, uint32_t i1, uint32_t i2)
+{
+    real *e1 = &s->ep[3 * i1], *e2 = &s->ep[3 * i2];
+    const real t1 = s->tm[i1], t2 = s->tm[i2];
+    if (t1 == R(0) && t2 == R(0)) return;
+    real d[3] = {e1[0] - e2[0], e1[1] - e2[1], e1[2] - e2[2]};
+    const real wsum = t1 + t2, dist = length3(d), mag = dist - (R(2) * s->radius);
+    if (mag > R(0)) return;
+    const real scale = mag / wsum;
+    const real c1 = (real)s->counts[i1], c2 = (real)s->counts[i2];
+    real dp[3];
+    for (int a = 0; a < 3; a++) dp[a] = (scale / dist) * d[a];
+    for (int a = 0; a < 3; a++) {
+        e1[a] = e1[a] + (((-t1) * dp[a]) / c1);
+        e2[a] = e2[a] + ((t2 * dp[a]) / c2);
+    }
+}
+
+/* FluidEmitter::tick (cpu/src/fluidemitter.cpp:13-79) at the end of the
+ * tick, after the smoke emitters (simulation.cpp:364-366).  Over the fluid
+ * object's particles (its ps list, in index order; the reference walks it
+ * from the back, but no decision depends on another): a particle slower
+ * than .06 at height <= 5 loses 1 from t when lambdas[k] <= 0 and turns
+ * into an immovable solid when t reaches 0 (imass 0, SOLID, ep = p, v = f =
+ * 0, out of the object); otherwise it gains dt, up to 3.  `lambdas` is a
+ * QHash keyed by particle index and the lookup passes the position k in ps:
+ * it reads the lambda of particle k if particle k was in the object at its
+ * last projection, else 0 (the hash's default).  Then emissions: a fluid
+ * particle of mass 1 at pos with v = (frand(), 1) per 1/rate of the timer
+ * while the total time is below 5 s. */
+static int femitter_tick(r2d_sys *s, r2d_femitter *e, uint32_t eidx, real dt)
+{
+    const int32_t g = e->group;
+    uint32_t m = 0;
+    for (uint32_t i = 0; i < s->n; i++) {
+        s->snap[i] = s->group[i] == g;
+        if (s->snap[i]) s->members[m++] = i;
+    }
+    for (uint32_t k = m; k-- > 0;) {
+        const uint32_t i = s->members[k];
+        const real *v = &s->v[3 * i], *p = &s->p[3 * i];
+        if (!(length3(v) < R(.06) && p[1] <= R(5))) continue;
+        const real lam = (k < s->n && s->snap[k]) ? s->lambda[k] : R(0);
+        if (lam <= R(0)) {
+            s->tt[i] = s->tt[i] - R(1);
+            if (s->tt[i] <= R(0)) {
+                s->tt[i] = R(0);
+                s->w[i] = R(0);
+                s->phase[i] = PS_PHASE_SOLID;
+                s->group[i] = -1;
+                for (int a = 0; a < 3; a++) {
+                    s->ep[3 * i + a] = s->p[3 * i + a];
+                    s->v[3 * i + a] = R(0);
+                    s->force[3 * i + a] = R(0);
+                }
+            }
+        } else {
+            s->tt[i] = s->tt[i] + dt;
+            if (s->tt[i] > R(3)) s->tt[i] = R(3);
+        }
+    }
+    e->timer += (double)dt;
+    e->total += (double)dt;
+    while (e->total < 5.0 && e->timer >= 1.0 / e->rate) {
+        e->timer -= 1.0 / e->rate;
+        if (s->n >= s->cap) return PS_ERR_CAPACITY;
+        const uint32_t i = s->n++;
+        for (int a = 0; a < 3; a++) {
+            s->p[3 * i + a] = e->pos[a];
+            s->ep[3 * i + a] = e->pos[a];
+            s->force[3 * i + a] = R(0);
+        }
+        s->v[3 * i] = emit_uniform(s->prm.seed, eidx, e->emitted++);
+        s->v[3 * i + 1] = R(1);
+        s->v[3 * i + 2] = R(0);
+        s->w[i] = R(1);
+        s->phase[i] = PS_PHASE_FLUID;
+        s->group[i] = g;
+        s->tt[i] = R(4);
     }
     return PS_OK;
 }
@@ -591,11 +747,40 @@ EXPORT int SFX(r2d_step)(r2d_sys *s, real dt, uint32_t steps)
             }
             s->bflags[i] = f;
             s->counts[i] = (uint32_t)__builtin_popcount(f);
+            /* (4) scaleMass from the start height */
+            s->tm[i] = s->w[i] != R(0) ? R(1) / ((R(1) / s->w[i]) * orc_exp(-p[1])) : R(0);
         }
-        /* (16)-(22) iterations: CONTACT group (boundaries, per particle x
-         * then y then z), STANDARD group (fluid objects in order) */
+        /* (7) contacts of a solid with any other particle, not both
+         * immovable, closer than a diameter less EPSILON; counted for both */
+        s->ncontact = 0;
+        const real touch = R(2.0 * (double)s->prm.radius - 1e-4);
+        for (uint32_t i = 0; i < s->n; i++) {
+            s->cstart[i] = s->ncontact;
+            for (uint32_t j = i + 1; j < s->n; j++) {
+                if (s->w[i] == R(0) && s->w[j] == R(0)) continue;
+                if (s->phase[i] != PS_PHASE_SOLID && s->phase[j] != PS_PHASE_SOLID) continue;
+                const real *a = &s->ep[3 * i], *b = &s->ep[3 * j];
+                real d[3] = {a[0] - b[0], a[1] - b[1], a[2] - b[2]};
+                if (!(length3(d) < touch)) continue;
+                if (s->ncontact == s->ccap) {
+                    s->ccap = s->ccap ? 2 * s->ccap : 1024;
+                    s->cpair = (uint32_t *)realloc(s->cpair, 2 * sizeof(uint32_t) * s->ccap);
+                }
+                s->cpair[2 * s->ncontact] = i;
+                s->cpair[2 * s->ncontact + 1] = j;
+                s->ncontact++;
+                s->counts[i]++;
+                s->counts[j]++;
+            }
+        }
+        s->cstart[s->n] = s->ncontact;
+        /* (16)-(22) iterations: CONTACT group (per particle, the contacts
+         * made while visiting it, then its boundaries x, y, z), STANDARD
+         * group (fluid objects in order) */
         for (uint32_t it = 0; it < s->prm.iterations; it++) {
             for (uint32_t i = 0; i < s->n; i++) {
+                for (uint32_t c = s->cstart[i]; c < s->cstart[i + 1]; c++)
+                    contact(s, s->cpair[2 * c], s->cpair[2 * c + 1]);
                 uint32_t f = s->bflags[i];
                 if (f & BX_LOW) boundary(s, i, 0, 1, s->iters);
                 if (f & BX_HIGH) boundary(s, i, 0, 0, s->iters);
@@ -625,6 +810,10 @@ EXPORT int SFX(r2d_step)(r2d_sys *s, real dt, uint32_t steps)
         }
         for (uint32_t k = 0; k < s->nem; k++) {
             int rc = emitter_tick(s, &s->em[k], dt);
+            if (rc) return rc;
+        }
+        for (uint32_t k = 0; k < s->nfem; k++) {
+            int rc = femitter_tick(s, &s->fem[k], k, dt);
             if (rc) return rc;
         }
     }

@@ -144,6 +144,7 @@ PROTOTYPES = {
     "ps_add_smoke_emitter": (_int, [_P, _P, ctypes.c_double, ctypes.c_int32, _u32,
                                     ctypes.POINTER(_u32)]),
     "ps_read_markers": (_int, [_P, _u32, ctypes.POINTER(_u32), _P]),
+    "ps_add_fluid_emitter": (_int, [_P, _P, ctypes.c_double, ctypes.c_int32, ctypes.POINTER(_u32)]),
     "ps_export_device": (_int, [_P, _u32, _u32, _u32, _P]),
     "ps_trajectory_open": (_int, [ctypes.c_char_p, _u32, _u32, ctypes.POINTER(_P)]),
     "ps_trajectory_write": (_int, [_P, _P]),
@@ -360,6 +361,31 @@ class System:
         self._check(self.lib.ps_add_smoke_emitter(self.h, _ptr(p), float(particles_per_sec),
                                                   int(object), int(max_markers), ctypes.byref(e)))
         return e.value
+
+    def add_fluid_emitter(self, pos, particles_per_sec, object):
+        """ps_add_fluid_emitter: a FluidEmitter at pos feeding fluid object
+        `object`; returns the emitter index."""
+        p = np.ascontiguousarray(pos, dtype=np.float32).reshape(3)
+        e = ctypes.c_uint32()
+        self._check(self.lib.ps_add_fluid_emitter(self.h, _ptr(p), float(particles_per_sec),
+                                                  int(object), ctypes.byref(e)))
+        return e.value
+
+    def add_static_particles(self, pos):
+        """immovable solids (phase SOLID, inverse mass 0) through
+        ps_add_particles"""
+        pos = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1, 3)
+        n = pos.shape[0]
+        return self.add_particles(pos, None, np.zeros(n, np.float32),
+                                  np.full(n, PHASE_SOLID, np.int32), np.zeros(n, np.float32))
+
+    def phases(self):
+        """phase of every particle, insertion order (through ps_export_device)"""
+        with DeviceBuffer(4 * max(self.n, 1)) as buf:
+            if self.n:
+                self.export_device(PS_FIELD_PHASES, buf.ptr)
+                self.synchronize()
+            return buf.numpy(np.int32, (self.n,))
 
     def markers(self, emitter):
         """ps_read_markers: the emitter's marker positions, (count, 3)"""

@@ -86,6 +86,15 @@ struct ps_system {
         float4 *mark;
     };
     std::vector<Emitter> emitters;
+    // fluid emitters (FluidEmitter), ticked after the smoke emitters; each
+    // feeds and solidifies fluid object `object` (the last object)
+    struct FluidEmitter {
+        double timer, total, rate;
+        float pos[3];
+        int32_t object;
+        uint64_t emitted;
+    };
+    std::vector<FluidEmitter> femitters;
     bool cr_bnd_done = false;  // boundaries of iteration it_sub already projected
     bool cr_swapped = false;   // iterate lives in b.Q (odd number of object passes)
     // workgroup size of each kernel (ps_autotune)
@@ -235,6 +244,8 @@ void fill_crconst(const ps_params &p, const ps_derived &d, ps::CrConst &c)
     c.radius = p.radius;
     c.sf = sqrtf(p.wall_mu_s);
     c.kf = sqrtf(p.wall_mu_k);
+    // PARTICLE_DIAM - EPSILON (simulation.cpp:182) with the runtime radius
+    c.touch = (float)(2.0 * (double)p.radius - 1e-4);
     for (int a = 0; a < 3; a++) {
         c.gravity[a] = p.gravity[a];
         c.box_min[a] = p.box_min[a];
@@ -361,7 +372,8 @@ void free_all(ps_system *s)
     dev_free(b.body_rest); dev_free(b.body_stiff); dev_free(b.body_mass); dev_free(b.body_q);
     dev_free(b.body_goal); dev_free(b.body_of); dev_free(b.member_of); dev_free(b.status);
     dev_free(b.sid); dev_free(b.sid2); dev_free(b.scenes); dev_free(b.lam); dev_free(b.bc);
-    dev_free(b.force);
+    dev_free(b.force); dev_free(b.tmass); dev_free(b.ttimer); dev_free(b.cnum); dev_free(b.clist);
+    dev_free(b.sflag);
     dev_free(b.stage);
     for (ps_system::Emitter &e : s->emitters) dev_free(e.mark);
     adj_free(s);
@@ -524,6 +536,11 @@ int stage_end(ps_system *s)
 
 // ---- CPU-reference mode stages --------------------------------------------
 
+// Immovable solids (or a FluidEmitter that makes them) switch on the
+// contact pass: contacts are found after prediction and each iteration
+// starts with them and the boundaries.
+bool cr_contacts(const ps_system *s) { return s->any_solid_phase || !s->femitters.empty(); }
+
 int cr_begin(ps_system *s, float dt)
 {
     s->dt = dt;
@@ -532,8 +549,11 @@ int cr_begin(ps_system *s, float dt)
         std::swap(s->b.P, s->b.Q);
         s->cr_swapped = false;
     }
-    LAUNCH(s, PS_K_PREDICT_HASH, ps::launch_cr_predict(s->stream, s->b, s->cr, s->n, dt));
-    s->cr_bnd_done = true;  // prediction projects the boundaries of iteration 0
+    const bool contacts = cr_contacts(s);
+    LAUNCH(s, PS_K_PREDICT_HASH, ps::launch_cr_predict(s->stream, s->b, s->cr, s->n, dt, !contacts));
+    if (contacts) LAUNCH(s, PS_K_NEIGHBORS, ps::launch_cr_find_contacts(s->stream, s->b, s->cr, s->n));
+    // without contacts, prediction projects the boundaries of iteration 0
+    s->cr_bnd_done = !contacts;
     s->stage = 1;
     return PS_OK;
 }
@@ -541,10 +561,18 @@ int cr_begin(ps_system *s, float dt)
 int cr_iterate(ps_system *s)
 {
     const uint32_t it = s->it_sub;
-    if (!s->cr_bnd_done)
+    const bool contacts = cr_contacts(s);
+    if (contacts) {
+        // the CONTACT group: contacts, then boundaries, per particle
+        LAUNCH(s, PS_K_SOLVE, ps::launch_cr_contacts(s->stream, s->b, s->cr, s->n, it));
+        std::swap(s->b.P, s->b.Q);
+        s->cr_swapped = !s->cr_swapped;
+    } else if (!s->cr_bnd_done) {
         LAUNCH(s, PS_K_SOLVE, ps::launch_cr_boundary(s->stream, s->b, s->cr, s->n, it));
-    // the last object pass also projects the next iteration's boundaries
-    const int next = it + 1 < s->prm.iterations ? (int)(it + 1) : -1;
+    }
+    // without contacts, the last object pass also projects the next
+    // iteration's boundaries
+    const int next = !contacts && it + 1 < s->prm.iterations ? (int)(it + 1) : -1;
     const size_t nobj = s->objects.size();
     for (size_t k = 0; k < nobj; k++) {
         const ps_system::FluidObject &o = s->objects[k];
@@ -586,7 +614,18 @@ int emission_check(ps_system *s, float dt, uint32_t substeps)
         }
         if (e.object >= 0) n += count;
     }
-    if (n > s->cap) return fail(s, PS_ERR_CAPACITY, "smoke emitters exceed max_particles");
+    for (const ps_system::FluidEmitter &e : s->femitters) {
+        double timer = e.timer, total = e.total;
+        for (uint32_t t = 0; t < substeps; t++) {
+            timer += (double)dt;
+            total += (double)dt;
+            while (total < 5.0 && timer >= 1.0 / e.rate) {
+                timer -= 1.0 / e.rate;
+                n++;
+            }
+        }
+    }
+    if (n > s->cap) return fail(s, PS_ERR_CAPACITY, "emitters exceed max_particles");
     return PS_OK;
 }
 
@@ -623,9 +662,40 @@ int cr_emitters(ps_system *s)
     return PS_OK;
 }
 
+// FluidEmitter::tick (fluidemitter.cpp:13-79) of every fluid emitter in
+// creation order: solidification over its object, then its emissions (a
+// double timer and the 5 s total, counted on the host).
+int cr_fluid_emitters(ps_system *s)
+{
+    for (size_t k = 0; k < s->femitters.size(); k++) {
+        ps_system::FluidEmitter &e = s->femitters[k];
+        ps_system::FluidObject &o = s->objects[e.object];
+        LAUNCH(s, PS_K_FINALIZE, ps::launch_cr_solidify(s->stream, s->b, o.first, o.count, s->dt));
+        uint32_t count = 0;
+        e.timer += (double)s->dt;
+        e.total += (double)s->dt;
+        while (e.total < 5.0 && e.timer >= 1.0 / e.rate) {
+            e.timer -= 1.0 / e.rate;
+            count++;
+        }
+        for (uint32_t done = 0; done < count;) {
+            const uint32_t c = std::min(count - done, 64u);
+            LAUNCH(s, PS_K_FINALIZE,
+                   ps::launch_cr_emit_fluid(s->stream, s->b, s->cr, s->n, c, e.pos, o.rho0,
+                                            (uint32_t)k, e.emitted));
+            e.emitted += c;
+            o.count += c;
+            s->n += c;
+            s->scene_count[0] += c;
+            done += c;
+        }
+    }
+    return PS_OK;
+}
+
 int cr_end(ps_system *s)
 {
-    if (!s->emitters.empty())
+    if (!s->emitters.empty() || !s->femitters.empty())
         if (int rc = emission_check(s, s->dt, 1)) return rc;
     LAUNCH(s, PS_K_FINALIZE,
            ps::launch_cr_end(s->stream, s->b, s->n, s->dt, (s->prm.flags & PS_FLAG_SLEEPING) != 0,
@@ -636,6 +706,7 @@ int cr_end(ps_system *s)
         s->cr_swapped = false;
     }
     if (int rc = cr_emitters(s)) return rc;
+    if (int rc = cr_fluid_emitters(s)) return rc;
     s->it_sub = 0;
     s->substeps++;
     s->stage = 0;
@@ -923,6 +994,11 @@ static int create_impl(const ps_params *params, uint32_t nscenes, const ps_scene
         ALLOC(b.lam, c);
         ALLOC(b.bc, c);
         ALLOC(b.force, c);
+        ALLOC(b.tmass, c);
+        ALLOC(b.ttimer, c);
+        ALLOC(b.cnum, c);
+        ALLOC(b.clist, c * ps::kCrMaxContacts);
+        ALLOC(b.sflag, c);
     }
 #undef ALLOC
     hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
@@ -1042,7 +1118,9 @@ static int objects_open(ps_system *s)
 {
     for (const ps_system::Emitter &e : s->emitters)
         if (e.object >= 0)
-            return fail(s, PS_ERR_STATE, "no objects can follow a smoke emitter that feeds an object");
+            return fail(s, PS_ERR_STATE, "nothing can be added after a smoke emitter that feeds an object");
+    if (!s->femitters.empty())
+        return fail(s, PS_ERR_STATE, "nothing can be added after a fluid emitter");
     return PS_OK;
 }
 
@@ -1058,7 +1136,14 @@ int ps_add_scene_particles(ps_system *s, uint32_t scene, uint32_t n, const float
                            const float *rest_density, uint32_t *first_out)
 {
     if (int rc = check_sys(s)) return rc;
-    if (s->cpuref) return unsupported_in_cpuref(s, "ps_add_particles (use ps_add_fluid_object)");
+    if (s->cpuref) {
+        // immovable solids only (new Particle(pos, 0), simulation.cpp:1182-1185)
+        for (uint32_t k = 0; k < n; k++)
+            if (!phase || !inv_mass || phase[k] != PS_PHASE_SOLID || inv_mass[k] != 0.0f)
+                return unsupported_in_cpuref(
+                    s, "ps_add_particles other than immovable solids (use ps_add_fluid_object)");
+        if (int rc = objects_open(s)) return rc;
+    }
     return add_particles_impl(s, scene, n, pos, vel, inv_mass, phase, rest_density, first_out);
 }
 
@@ -1119,6 +1204,8 @@ int ps_add_smoke_emitter(ps_system *s, const float *pos, double particles_per_se
         if (!o.gas) return fail(s, PS_ERR_INVALID_ARGUMENT, "a smoke emitter feeds a gas object");
         if (object + 1 != (int32_t)s->objects.size() || o.first + o.count != s->n)
             return fail(s, PS_ERR_UNSUPPORTED, "a smoke emitter feeds the last object only");
+        if (!s->femitters.empty())
+            return fail(s, PS_ERR_UNSUPPORTED, "a fluid emitter feeds a different object");
     }
     ps_system::Emitter e{};
     e.timer = 0.0;
@@ -1130,6 +1217,38 @@ int ps_add_smoke_emitter(ps_system *s, const float *pos, double particles_per_se
     drop_graph(s);
     s->emitters.push_back(e);
     if (emitter_out) *emitter_out = (uint32_t)s->emitters.size() - 1;
+    return PS_OK;
+}
+
+int ps_add_fluid_emitter(ps_system *s, const float *pos, double particles_per_sec, int32_t object,
+                         uint32_t *emitter_out)
+{
+    if (int rc = check_sys(s)) return rc;
+    if (!s->cpuref || s->prm.dims != 2)
+        return fail(s, PS_ERR_UNSUPPORTED, "fluid emitters belong to the 2D CPU-reference mode");
+    if (s->stage) return fail(s, PS_ERR_STATE, "cannot add an emitter inside a substep");
+    if (!pos || !is_finite(pos[0]) || !is_finite(pos[1]) || !is_finite(pos[2]))
+        return fail(s, PS_ERR_INVALID_ARGUMENT, "emitter position must be finite");
+    if (!(particles_per_sec > 0.0) || !isfinite(particles_per_sec))
+        return fail(s, PS_ERR_INVALID_ARGUMENT, "particles_per_sec must be > 0");
+    if (object < 0 || object >= (int32_t)s->objects.size())
+        return fail(s, PS_ERR_INVALID_ARGUMENT, "object out of range");
+    const ps_system::FluidObject &o = s->objects[object];
+    if (o.gas) return fail(s, PS_ERR_INVALID_ARGUMENT, "a fluid emitter feeds a fluid object");
+    if (object + 1 != (int32_t)s->objects.size() || o.first + o.count != s->n)
+        return fail(s, PS_ERR_UNSUPPORTED, "a fluid emitter feeds the last object only");
+    for (const ps_system::Emitter &e : s->emitters)
+        if (e.object >= 0 && e.object != object)
+            return fail(s, PS_ERR_UNSUPPORTED, "another emitter feeds a different object");
+    ps_system::FluidEmitter e{};
+    e.timer = e.total = 0.0;
+    e.rate = particles_per_sec;
+    for (int a = 0; a < 3; a++) e.pos[a] = pos[a];
+    e.object = object;
+    e.emitted = 0;
+    drop_graph(s);
+    s->femitters.push_back(e);
+    if (emitter_out) *emitter_out = (uint32_t)s->femitters.size() - 1;
     return PS_OK;
 }
 
@@ -1209,6 +1328,10 @@ static int add_particles_impl(ps_system *s, uint32_t scene, uint32_t n, const fl
     HIP_TRY(s, hipMemcpy(b.A + o, A.data(), sizeof(Attr) * n, hipMemcpyHostToDevice));
     HIP_TRY(s, hipMemcpy(b.slot_of + o, slot.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice));
     if (b.force) HIP_TRY(s, hipMemset(b.force + o, 0, sizeof(float4) * n));
+    if (b.ttimer) {
+        const std::vector<float> t4(n, 4.0f);  // Particle::init, particle.h:37-38
+        HIP_TRY(s, hipMemcpy(b.ttimer + o, t4.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+    }
     if (s->scene_count[scene] == 0) {
         s->scenes[scene].first = s->n;
         HIP_TRY(s, hipMemcpy(b.scenes + scene, &s->scenes[scene], sizeof(ps::SceneDev),
@@ -1493,7 +1616,7 @@ int ps_step(ps_system *s, float dt, uint32_t substeps)
     if (!(dt > 0.0f) || !is_finite(dt)) return fail(s, PS_ERR_INVALID_ARGUMENT, "dt must be > 0");
     if (s->n == 0) return PS_OK;
     // emitters overflowing within the call fail it before the first substep
-    if (!s->emitters.empty())
+    if (!s->emitters.empty() || !s->femitters.empty())
         if (int rc = emission_check(s, dt, substeps)) return rc;
     for (uint32_t k = 0; k < substeps; k++) {
         int rc;
@@ -1511,8 +1634,8 @@ int ps_step_graph(ps_system *s, float dt, uint32_t substeps)
     if (s->stage) return fail(s, PS_ERR_STATE, "ps_step_graph inside an open substep");
     if (!(dt > 0.0f) || !is_finite(dt)) return fail(s, PS_ERR_INVALID_ARGUMENT, "dt must be > 0");
     if (s->n == 0 || substeps == 0) return PS_OK;
-    if (!s->emitters.empty())
-        return fail(s, PS_ERR_UNSUPPORTED, "ps_step_graph with smoke emitters (the particle count grows)");
+    if (!s->emitters.empty() || !s->femitters.empty())
+        return fail(s, PS_ERR_UNSUPPORTED, "ps_step_graph with emitters (the particle count grows)");
     if (!s->gexec || s->g_dt != dt || s->g_sub != substeps) {
         drop_graph(s);
         const DevBuffers start = s->b;

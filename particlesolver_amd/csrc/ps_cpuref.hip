@@ -33,6 +33,7 @@
 // of iteration k+1 is fused into the last fluid pass of iteration k (and the
 // one of iteration 0 into prediction).
 
+#include "ps_device.hpp"
 #include "ps_internal.hpp"
 
 namespace ps {
@@ -43,6 +44,9 @@ constexpr uint32_t kBlock = 256;
 constexpr uint32_t kWaves = kBlock / 64;
 
 enum : uint32_t { BX_LOW = 1u, BX_HIGH = 2u, BY_LOW = 4u, BY_HIGH = 8u, BZ_LOW = 16u, BZ_HIGH = 32u };
+// bc bit 6: the particle is neither fluid nor gas, so its boundary
+// projections add no jitter (boundaryconstraint.cpp:18-19)
+constexpr uint32_t kNoJitter = 64u;
 
 // GasConstraint's compile-time constants (gasconstraint.h:10-19) and the gas
 // gravity scale ALPHA (simulation.h:21), rounded once to float
@@ -153,7 +157,7 @@ __device__ __forceinline__ void cr_boundaries(const CrConst &c, float e[3], cons
     for (uint32_t axis = 0; axis < 3; axis++) {
         const uint32_t lo = 1u << (2 * axis), hi = 2u << (2 * axis);
         if (!(bc & (lo | hi))) continue;
-        const float extra = cr_uniform(c.seed, iter, i, axis) * c.jitter;
+        const float extra = (bc & kNoJitter) ? 0.0f : cr_uniform(c.seed, iter, i, axis) * c.jitter;
         cr_boundary_axis(c, e, p, axis, (bc & lo) != 0, cnt, extra);
     }
 }
@@ -166,8 +170,9 @@ __global__ __launch_bounds__(kBlock) void cr_predict_k(const float4 *__restrict_
                                                        float4 *__restrict__ E,
                                                        uint32_t *__restrict__ bcs,
                                                        float4 *__restrict__ F,
+                                                       float *__restrict__ tm,
                                                        const DevStatus *__restrict__ st, CrConst c,
-                                                       uint32_t n, float dt)
+                                                       uint32_t n, float dt, int boundaries)
 {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
@@ -206,11 +211,108 @@ __global__ __launch_bounds__(kBlock) void cr_predict_k(const float4 *__restrict_
         if (e[2] < c.box_min[2] + r) f |= BZ_LOW;
         else if (e[2] > c.box_max[2] - r) f |= BZ_HIGH;
     }
-    const uint32_t bc = f | ((uint32_t)__popc(f) << 8);
-    cr_boundaries(c, e, p, bc, st->iter, i);
+    const bool fl = at.phase == PS_PHASE_FLUID || at.phase == PS_PHASE_GAS;
+    const uint32_t bc = f | (fl ? 0u : kNoJitter) | ((uint32_t)__popc(f) << 8);
+    if (boundaries) cr_boundaries(c, e, p, bc, st->iter, i);
     bcs[i] = bc;
     V[i] = v;
     E[i] = make_float4(e[0], e[1], e[2], w);
+    // (4) Particle::scaleMass (particle.h:67-73) from the start height
+    if (tm) tm[i] = w != 0.0f ? 1.0f / ((1.0f / w) * ps_exp(-x.y)) : 0.0f;
+}
+
+// (7) contacts (simulation.cpp:164-199): a pair with exactly one immovable
+// particle (every solid of this mode is immovable, and every other particle
+// movable) whose predicted centres are closer than c.touch.  One wavefront
+// per particle: its partner count joins its constraint count (both ends of
+// a ContactConstraint count it, contactconstraint.cpp:94-98), and a movable
+// particle lists its partners in index order.
+__global__ __launch_bounds__(kBlock) void cr_find_contacts_k(const float4 *__restrict__ E,
+                                                             uint32_t *__restrict__ bcs,
+                                                             uint32_t *__restrict__ cnum,
+                                                             uint32_t *__restrict__ clist,
+                                                             DevStatus *__restrict__ st, CrConst c,
+                                                             uint32_t n)
+{
+    const uint32_t i = blockIdx.x * kWaves + (threadIdx.x >> 6);
+    const uint32_t lane = threadIdx.x & 63u;
+    if (i >= n) return;  // whole wavefront
+    const float4 ei = E[i];
+    const bool movable = ei.w != 0.0f;
+    uint32_t cnt = 0;
+    for (uint32_t base = 0; base < n; base += 64) {
+        const uint32_t j = base + lane;
+        const bool valid = j < n && j != i;
+        const float4 ej = E[valid ? j : i];
+        const float dx = ei.x - ej.x, dy = ei.y - ej.y, dz = ei.z - ej.z;
+        const float dist = sqrtf(((dx * dx) + (dy * dy)) + (dz * dz));
+        const bool hit = valid && ((ej.w != 0.0f) != movable) && dist < c.touch;
+        const uint64_t mask = __ballot(hit);
+        if (movable && hit) {
+            const uint32_t k = cnt + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+            if (k < kCrMaxContacts) clist[(size_t)i * kCrMaxContacts + k] = j;
+        }
+        cnt += (uint32_t)__popcll(mask);
+    }
+    if (lane != 0) return;
+    if (movable && cnt > kCrMaxContacts) atomicOr(&st->flags, PS_STATUS_NEIGHBOR_OVERFLOW);
+    cnum[i] = movable ? (cnt < kCrMaxContacts ? cnt : kCrMaxContacts) : 0u;
+    bcs[i] = bcs[i] + (cnt << 8);
+}
+
+// The CONTACT group of one iteration for particle i: ContactConstraint::
+// project (contactconstraint.cpp:14-40) with each immovable partner in index
+// order (the order the reference creates them), then the boundaries.  The
+// partner's position is read from E, before its own boundaries of this
+// iteration, as the reference's creation order has it; it does not move.
+__global__ __launch_bounds__(kBlock) void cr_contacts_k(const float4 *__restrict__ E,
+                                                        float4 *__restrict__ E2,
+                                                        const float4 *__restrict__ X,
+                                                        const uint32_t *__restrict__ bcs,
+                                                        const uint32_t *__restrict__ cnum,
+                                                        const uint32_t *__restrict__ clist,
+                                                        const float *__restrict__ tm,
+                                                        const DevStatus *__restrict__ st, CrConst c,
+                                                        uint32_t n, uint32_t it)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const float4 x = X[i];
+    const float4 ei = E[i];
+    const uint32_t bc = bcs[i];
+    float e[3] = {ei.x, ei.y, ei.z};
+    const uint32_t m = cnum[i];
+    if (m) {
+        const float ti = tm[i], cnt = (float)(bc >> 8), diam = 2.0f * c.radius;
+        for (uint32_t k = 0; k < m; k++) {
+            const uint32_t j = clist[(size_t)i * kCrMaxContacts + k];
+            const float4 ej = E[j];
+            // diff = ep(lower index) - ep(higher index); w sum = t + 0
+            const bool first = i < j;
+            float d[3];
+            if (first) {
+                d[0] = e[0] - ej.x;
+                d[1] = e[1] - ej.y;
+                d[2] = e[2] - ej.z;
+            } else {
+                d[0] = ej.x - e[0];
+                d[1] = ej.y - e[1];
+                d[2] = ej.z - e[2];
+            }
+            const float wsum = ti;  // t_i + 0 (the partner is immovable)
+            const float dist = sqrtf(((d[0] * d[0]) + (d[1] * d[1])) + (d[2] * d[2]));
+            const float mag = dist - diam;
+            if (mag > 0.0f) continue;
+            const float scale = mag / wsum;
+            for (int a = 0; a < 3; a++) {
+                const float dp = (scale / dist) * d[a];
+                e[a] = e[a] + (first ? (((-ti) * dp) / cnt) : ((ti * dp) / cnt));
+            }
+        }
+    }
+    const float p[3] = {x.x, x.y, x.z};
+    cr_boundaries(c, e, p, bc, st->iter + it, i);
+    E2[i] = make_float4(e[0], e[1], e[2], ei.w);
 }
 
 // TotalFluidConstraint::project, first loop (totalfluidconstraint.cpp:44-90):
@@ -231,6 +333,8 @@ __global__ __launch_bounds__(kBlock) void cr_lambda_k(const float4 *__restrict__
     if (wv >= count) return;  // whole wavefront
     const uint32_t i = first + wv;
     const float4 ei = E[i];
+    // a particle turned solid by a FluidEmitter has left the object
+    if (ei.w == 0.0f) return;  // whole wavefront
     float acc = 0.0f, den = 0.0f, gx = 0.0f, gy = 0.0f, gz = 0.0f;
     for (uint32_t base = 0; base < n; base += 64) {
         const uint32_t j = base + lane;
@@ -303,7 +407,7 @@ __global__ __launch_bounds__(kBlock) void cr_delta_k(const float4 *__restrict__ 
     if (i >= n) return;  // whole wavefront
     float4 ei = E[i];
     const uint32_t bc = bcs[i];
-    if (i - first < count) {
+    if (i - first < count && ei.w != 0.0f) {
         const float li = lam[i];
         const float kc = gas ? kGasKP : c.k_corr, wdq = gas ? c.g_wdq : c.w_dq;
         const uint32_t nc = gas ? kGasEP : c.n_corr;
@@ -495,9 +599,136 @@ __global__ __launch_bounds__(kBlock) void cr_markers_k(float4 *__restrict__ M,
     M[m] = q;
 }
 
+// FluidEmitter::tick's first loop (fluidemitter.cpp:13-62) over fluid
+// object [first, first + count), one workgroup.  Members are the particles
+// of the range that are not immovable; their position in the object's list
+// is their rank among members (block scan), and `lambdas[k]` is keyed by
+// particle index with that position k: the lambda of particle k when it was
+// a member at the object's last projection, else 0.  Pass 1 updates t and
+// marks the particles that turn solid; pass 2 turns them into immovable
+// solids (imass 0, SOLID, v = f = 0; ep = p follows at the next prediction).
+constexpr uint32_t kSolidifyBlock = 1024;
+
+__global__ __launch_bounds__(kSolidifyBlock) void cr_solidify_k(const float4 *__restrict__ X,
+                                                                float4 *__restrict__ V,
+                                                                Attr *__restrict__ A,
+                                                                float4 *__restrict__ F,
+                                                                const float *__restrict__ lam,
+                                                                float *__restrict__ tt,
+                                                                uint32_t *__restrict__ sflag,
+                                                                uint32_t first, uint32_t count,
+                                                                float dt)
+{
+    __shared__ uint32_t wave_sum[kSolidifyBlock / 64];
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < count; base += kSolidifyBlock) {
+        const uint32_t off = base + tid;
+        const uint32_t j = first + off;
+        const bool member = off < count && A[j].w != 0.0f;
+        const uint64_t mask = __ballot(member);
+        if (lane == 0) wave_sum[wave] = (uint32_t)__popcll(mask);
+        __syncthreads();
+        uint32_t before = carry, total = 0;
+        for (uint32_t w = 0; w < kSolidifyBlock / 64; w++) {
+            if (w < wave) before += wave_sum[w];
+            total += wave_sum[w];
+        }
+        const uint32_t k = before + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+        if (off < count) sflag[j] = 0u;
+        if (member) {
+            const float4 v = V[j], x = X[j];
+            const float speed = sqrtf(((v.x * v.x) + (v.y * v.y)) + (v.z * v.z));
+            if (speed < 0.06f && x.y <= 5.0f) {
+                // particle k was a member: inside the range and movable
+                // (pass 1 changes no attribute)
+                const bool was_member = k - first < count && A[k].w != 0.0f;
+                const float lk = was_member ? lam[k] : 0.0f;
+                float t = tt[j];
+                if (lk <= 0.0f) {
+                    t = t - 1.0f;
+                    if (t <= 0.0f) {
+                        t = 0.0f;
+                        sflag[j] = 1u;
+                    }
+                } else {
+                    t = t + dt;
+                    if (t > 3.0f) t = 3.0f;
+                }
+                tt[j] = t;
+            }
+        }
+        carry += total;
+        __syncthreads();  // wave_sum is rewritten by the next chunk
+    }
+    __syncthreads();
+    for (uint32_t off = tid; off < count; off += kSolidifyBlock) {
+        const uint32_t j = first + off;
+        if (!sflag[j]) continue;
+        Attr a = A[j];
+        a.w = 0.0f;
+        a.phase = PS_PHASE_SOLID;
+        A[j] = a;
+        V[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        F[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+}
+
+// FluidEmitter's emissions (fluidemitter.cpp:69-78): fluid particles of
+// mass 1 at pos with v = (u, 1), u the emission's frand() replacement:
+// SplitMix64 of seed + golden * (2^62 | emitter << 40 | emission) + 1, top
+// 24 bits.
+__global__ __launch_bounds__(64) void cr_emit_fluid_k(float4 *__restrict__ X, float4 *__restrict__ V,
+                                                      Attr *__restrict__ A, float4 *__restrict__ F,
+                                                      float *__restrict__ tt,
+                                                      uint32_t *__restrict__ slot_of, uint64_t seed,
+                                                      uint32_t first_part, uint32_t count, float px,
+                                                      float py, float pz, float rho0, uint32_t eidx,
+                                                      uint64_t emitted)
+{
+    const uint32_t k = threadIdx.x;
+    if (k >= count) return;
+    const uint32_t i = first_part + k;
+    const uint64_t ctr = (1ull << 62) | ((uint64_t)eidx << 40) | (emitted + k);
+    const uint64_t z = cr_mix64(seed + ctr * 0x9E3779B97F4A7C15ull + 1ull);
+    const float u = (float)(uint32_t)(z >> 40) * (1.0f / 16777216.0f);
+    X[i] = make_float4(px, py, pz, 0.0f);
+    V[i] = make_float4(u, 1.0f, 0.0f, 0.0f);
+    F[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    tt[i] = 4.0f;
+    Attr a;
+    a.w = 1.0f;
+    a.phase = PS_PHASE_FLUID;
+    a.rho0 = rho0;
+    a.orig = i;
+    A[i] = a;
+    slot_of[i] = i;
+}
+
 inline uint32_t blocks_for(uint32_t threads) { return (threads + kBlock - 1) / kBlock; }
 
 }  // namespace
+
+hipError_t launch_cr_solidify(hipStream_t st, const DevBuffers &b, uint32_t first, uint32_t count,
+                              float dt)
+{
+    if (!count) return hipSuccess;
+    hipLaunchKernelGGL(cr_solidify_k, dim3(1), dim3(kSolidifyBlock), 0, st, b.X, b.V, b.A, b.force,
+                       b.lam, b.ttimer, b.sflag, first, count, dt);
+    return hipGetLastError();
+}
+
+hipError_t launch_cr_emit_fluid(hipStream_t st, const DevBuffers &b, const CrConst &c,
+                                uint32_t first_part, uint32_t count, const float *pos, float rho0,
+                                uint32_t eidx, uint64_t emitted)
+{
+    if (!count) return hipSuccess;
+    if (count > 64) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(cr_emit_fluid_k, dim3(1), dim3(64), 0, st, b.X, b.V, b.A, b.force, b.ttimer,
+                       b.slot_of, c.seed, first_part, count, pos[0], pos[1], pos[2], rho0, eidx,
+                       emitted);
+    return hipGetLastError();
+}
 
 hipError_t launch_cr_emit(hipStream_t st, const DevBuffers &b, float4 *markers, uint32_t first_mark,
                           uint32_t first_part, uint32_t count, const float *pos, float rho0)
@@ -519,11 +750,28 @@ hipError_t launch_cr_markers(hipStream_t st, const DevBuffers &b, float4 *marker
 }
 
 hipError_t launch_cr_predict(hipStream_t st, const DevBuffers &b, const CrConst &c, uint32_t n,
-                             float dt)
+                             float dt, bool boundaries)
 {
     if (!n) return hipSuccess;
     hipLaunchKernelGGL(cr_predict_k, dim3(blocks_for(n)), dim3(kBlock), 0, st, b.X, b.V, b.A, b.P,
-                       b.bc, b.force, b.status, c, n, dt);
+                       b.bc, b.force, b.tmass, b.status, c, n, dt, boundaries ? 1 : 0);
+    return hipGetLastError();
+}
+
+hipError_t launch_cr_find_contacts(hipStream_t st, const DevBuffers &b, const CrConst &c, uint32_t n)
+{
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(cr_find_contacts_k, dim3((n + kWaves - 1) / kWaves), dim3(kBlock), 0, st, b.P,
+                       b.bc, b.cnum, b.clist, b.status, c, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_cr_contacts(hipStream_t st, const DevBuffers &b, const CrConst &c, uint32_t n,
+                              uint32_t it)
+{
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(cr_contacts_k, dim3(blocks_for(n)), dim3(kBlock), 0, st, b.P, b.Q, b.X, b.bc,
+                       b.cnum, b.clist, b.tmass, b.status, c, n, it);
     return hipGetLastError();
 }
 

@@ -61,6 +61,7 @@ struct KConst {
 // float on the host in the order the reference evaluates them.
 struct CrConst {
     float h, h2, poly6, spiky, w_dq, relax, k_corr, jitter, radius, sf, kf;
+    float touch;  // contact test: closer than a diameter less EPSILON (1e-4)
     float g_wdq;  // gas objects: poly6(DQ_P^2 H^2) of GasConstraint
     float gravity[3];
     float box_min[3], box_max[3];
@@ -157,8 +158,17 @@ struct DevBuffers {
     // CPU-reference mode only
     float4 *force;         // per particle, applied at the next prediction (gas)
     float *lam;            // lambda per particle
-    uint32_t *bc;          // boundary flags | boundary count << 8
+    uint32_t *bc;          // boundary flags | no-jitter bit | constraint count << 8
+    float *tmass;          // height-scaled mass of the contacts (per substep)
+    float *ttimer;         // FluidEmitter solidification timer t
+    uint32_t *cnum;        // contacts of a movable particle (<= kCrMaxContacts)
+    uint32_t *clist;       // their immovable partners, kCrMaxContacts per particle
+    uint32_t *sflag;       // scratch of the solidification pass
 };
+
+// Contacts kept per movable particle in the CPU-reference mode; more raise
+// PS_STATUS_NEIGHBOR_OVERFLOW and the rest are left out.
+constexpr uint32_t kCrMaxContacts = 32;
 
 // Per-particle attributes the reverse pass gathers from neighbours.
 struct __align__(16) AdjAttr {
@@ -238,7 +248,7 @@ hipError_t launch_adj(hipStream_t st, AdjKernel which, const DevBuffers &b, cons
 // CPU-reference mode (ps_cpuref.hip); storage in insertion order, iterate
 // ping-pong b.P -> b.Q per fluid object.
 hipError_t launch_cr_predict(hipStream_t st, const DevBuffers &b, const CrConst &c, uint32_t n,
-                             float dt);
+                             float dt, bool boundaries);
 // lambda, then delta + Jacobi update (b.P -> b.Q) of one fluid or gas
 // object [first, first + count); gas: 0 fluid, 1 closed gas, 2 open gas;
 // next_iter >= 0 also projects the boundaries of that iteration of the
@@ -262,5 +272,19 @@ hipError_t launch_export(hipStream_t st, const DevBuffers &b, uint32_t n, uint32
                          uint32_t first, uint32_t count, void *dst);
 hipError_t launch_cr_markers(hipStream_t st, const DevBuffers &b, float4 *markers, const CrConst &c,
                              uint32_t nmark, uint32_t n, float dt);
+// immovable solids present: contacts found after prediction (which then
+// leaves the boundaries to the contact pass), projected with the
+// boundaries at the start of every iteration, E -> E2 (b.P -> b.Q)
+hipError_t launch_cr_find_contacts(hipStream_t st, const DevBuffers &b, const CrConst &c, uint32_t n);
+hipError_t launch_cr_contacts(hipStream_t st, const DevBuffers &b, const CrConst &c, uint32_t n,
+                              uint32_t it);
+// FluidEmitter: solidification over the object [first, first+count), then
+// `count` emissions of emitter `eidx` (emission counter `emitted`) at
+// first_part
+hipError_t launch_cr_solidify(hipStream_t st, const DevBuffers &b, uint32_t first, uint32_t count,
+                              float dt);
+hipError_t launch_cr_emit_fluid(hipStream_t st, const DevBuffers &b, const CrConst &c,
+                                uint32_t first_part, uint32_t count, const float *pos, float rho0,
+                                uint32_t eidx, uint64_t emitted);
 
 }  // namespace ps

@@ -384,16 +384,22 @@ class FluidObjectsScene:
     cpu/src/simulation.cpp:431-449), and gas objects (positions, rest
     density, "gas" or "open gas"), added with ps_add_gas_object
     (Simulation::createGas, :408-428).  An object is (pos, rho0) or
-    (pos, rho0, kind).  Smoke emitters (Simulation::createSmokeEmitter,
+    (pos, rho0, kind); kind "static" adds immovable solids (new Particle(pos,
+    0), appended to the particle list as the volcano scene does, :1182-1185)
+    and takes no object index.  A fluid object may carry an inverse mass as
+    a fourth entry.  Smoke emitters (Simulation::createSmokeEmitter,
     :454-457) follow the objects, each (pos, particles_per_sec, object,
-    max_markers) with object the index of the gas object it feeds or -1."""
+    max_markers) with object the index of the gas object it feeds or -1;
+    then fluid emitters (createFluidEmitter, :459-461), each (pos,
+    particles_per_sec, object)."""
 
-    def __init__(self, params, objects, dt, name="", emitters=()):
+    def __init__(self, params, objects, dt, name="", emitters=(), fluid_emitters=()):
         self.params = params
         self.objects = objects
         self.dt = dt
         self.name = name
         self.emitters = list(emitters)
+        self.fluid_emitters = list(fluid_emitters)
 
     @property
     def n(self):
@@ -404,11 +410,15 @@ class FluidObjectsScene:
             pos, rho0 = o[0], o[1]
             kind = o[2] if len(o) > 2 else "fluid"
             if kind == "fluid":
-                sys_.add_fluid_object(pos, None, 1.0, rho0)
+                sys_.add_fluid_object(pos, None, o[3] if len(o) > 3 else 1.0, rho0)
+            elif kind == "static":
+                sys_.add_static_particles(pos)
             else:
                 sys_.add_gas_object(pos, None, 1.0, rho0, open=(kind == "open gas"))
         for e in self.emitters:
             sys_.add_smoke_emitter(*e)
+        for e in self.fluid_emitters:
+            sys_.add_fluid_emitter(*e)
 
 
 def cpu_reference_params(box_min, box_max, radius, h, iterations, max_particles, dims=2,
@@ -570,6 +580,44 @@ def reference_smoke(seed=0x5EED0009, dt=0.01, open_gas=True, emitter=False, max_
     return FluidObjectsScene(p, [(pos, 1.5, kind)], dt, emitters=emitters,
                              name="reference_smoke_%s%s" % ("open" if open_gas else "closed",
                                                             "_emitter" if emitter else ""))
+
+
+def reference_volcano(seed=0x5EED000A, dt=0.01, emitter=True):
+    """The CPU app's volcano, Simulation::initVolcano
+    (cpu/src/simulation.cpp:1178-1203): immovable solids along the cone
+    y = 10 - |x| (x from 1 while x <= 10, step .2, the -x one first), then
+    one fluid object of rest density 1 and mass 1.1 per particle (lattice
+    step .8: y from 0 while y < 9, x from 0 while x < 9 - y, the particles at
+    x and -x, each with .2 (frand() - .5) per axis), and a fluid emitter at
+    the origin, 40 per second for its first 5 s, that also turns slow fluid
+    into solid.  Box x in [-20, 20], y in [0, 100]; H = 2, radius .25,
+    3 iterations, dt .01."""
+    scale = 10.0
+    walls = []
+    x = 1.0
+    while x <= scale:
+        walls += [(-x, scale - x, 0.0), (x, scale - x, 0.0)]
+        x += 0.2
+    pts = []
+    y = 0.0
+    while y < scale - 1.0:
+        x = 0.0
+        while x < scale - y - 1.0:
+            pts += [(x, y), (-x, y)]
+            x += 0.8
+        y += 0.8
+    pts = np.array(pts, np.float64)
+    u = uniform24(seed, 2 * len(pts)).reshape(-1, 2)
+    fluid = np.zeros((len(pts), 3), np.float64)
+    fluid[:, :2] = pts + 0.2 * (u - 0.5)
+    walls = np.array(walls, np.float32)
+    n = walls.shape[0] + fluid.shape[0]
+    p = cpu_reference_params((-2.0 * scale, 0.0, -1.0), (2.0 * scale, 10.0 * scale, 1.0), 0.25, 2.0,
+                             3, n + (200 if emitter else 0))
+    objects = [(walls, 0.0, "static"),
+               (fluid.astype(np.float32), 1.0, "fluid", np.float32(1.0 / 1.1))]
+    fe = [((0.0, 0.0, 0.0), 4.0 * scale, 0)] if emitter else []
+    return FluidObjectsScene(p, objects, dt, fluid_emitters=fe, name="reference_volcano")
 
 
 def fluid_object_metrics(params, objects, pos):

@@ -28,6 +28,8 @@ def run_parity(sc, checkpoints):
             assert_bitexact(g.velocities(), ref.velocities(), "velocities after %d steps" % k)
             for e in range(len(getattr(sc, "emitters", ()))):
                 assert_bitexact(g.markers(e), ref.markers(e), "markers of %d after %d steps" % (e, k))
+            if getattr(sc, "fluid_emitters", None):
+                assert np.array_equal(g.phases(), ref.attrs()[0]), "phases after %d steps" % k
         st = g.status()
         assert st.flags == 0
         assert st.iterations == done * sc.params.iterations
@@ -190,6 +192,65 @@ def test_smoke_emitter_rules(gpu_available):
         with pytest.raises(capi.PsError) as e:
             g.add_smoke_emitter((0.0, 0.0, 0.0), 15.0, -1, 8)
         assert e.value.code == capi.PS_ERR_UNSUPPORTED
+
+
+def test_volcano_bitexact(gpu_available):
+    # Simulation::initVolcano: immovable cone, fluid of mass 1.1, a fluid
+    # emitter at 40/s for 5 s that turns slow fluid into solid
+    sc = scenes.reference_volcano()
+    assert [o[0].shape[0] for o in sc.objects] == [92, 156]
+    run_parity(sc, [1, 7, 50, 200, 520, 700])
+    with System(sc.params) as g:
+        sc.load_into(g)
+        g.step(sc.dt, 700)
+        ph = g.phases()
+        assert g.n == sc.n + 199                      # 5 s at 40/s, the 500th substep excluded
+        assert (ph == 3).sum() > 92                   # some fluid turned solid
+
+
+def test_static_obstacles_bitexact(gpu_available):
+    # fluid poured onto a slope of immovable solids: contacts with many
+    # partners per particle, and solids inside the boundary band
+    p = scenes.cpu_reference_params((-6.0, 0.0, -1.0), (6.0, 12.0, 1.0), 0.25, 2.0, 3, 600)
+    xs = np.arange(-6.0, 6.01, 0.3)
+    slope = np.stack([xs, 0.15 * (xs + 6.0), np.zeros_like(xs)], 1).astype(np.float32)
+    from particlesolver_amd.scenes import _reference_lattice, FluidObjectsScene
+    fluid = _reference_lattice(-3.0, 3.0, 4.0, 8.0, 0.5, 0x5EED0011)
+    sc = FluidObjectsScene(p, [(slope, 0.0, "static"), (fluid, 1.0, "fluid")], 0.01,
+                           name="slope")
+    run_parity(sc, [1, 10, 100, 300])
+
+
+def test_fluid_emitter_rules(gpu_available):
+    sc = scenes.reference_volcano(emitter=False)
+    roomy = sc.params.copy()
+    roomy.max_particles += 8
+    with System(roomy) as g:
+        sc.load_into(g)
+        with pytest.raises(capi.PsError) as e:           # a movable solid
+            g.add_particles(np.zeros((1, 3), np.float32), None, np.ones(1, np.float32),
+                            np.full(1, 3, np.int32), np.zeros(1, np.float32))
+        assert e.value.code == capi.PS_ERR_UNSUPPORTED
+        g.add_gas_object(np.array([[0.0, 50.0, 0.0]], np.float32))
+        with pytest.raises(capi.PsError) as e:           # a gas object
+            g.add_fluid_emitter((0.0, 0.0, 0.0), 40.0, 1)
+        assert e.value.code == capi.PS_ERR_INVALID_ARGUMENT
+        with pytest.raises(capi.PsError) as e:           # not the last object
+            g.add_fluid_emitter((0.0, 0.0, 0.0), 40.0, 0)
+        assert e.value.code == capi.PS_ERR_UNSUPPORTED
+    with System(sc.params) as g:
+        sc.load_into(g)
+        g.add_fluid_emitter((0.0, 0.0, 0.0), 40.0, 0)
+        with pytest.raises(capi.PsError) as e:           # nothing may follow it
+            g.add_static_particles(np.array([[5.0, 5.0, 0.0]]))
+        assert e.value.code == capi.PS_ERR_STATE
+        with pytest.raises(capi.PsError) as e:
+            g.step_graph(sc.dt, 12)
+        assert e.value.code == capi.PS_ERR_UNSUPPORTED
+        with pytest.raises(capi.PsError) as e:           # max_particles = the scene: no room
+            g.step(sc.dt, 3)
+        assert e.value.code == capi.PS_ERR_CAPACITY
+        assert g.status().iterations == 0
 
 
 def test_gas_objects_need_the_2d_cpu_reference_mode(gpu_available):
