@@ -77,6 +77,17 @@ typedef enum ps_error {
  * each workgroup's neighbour cells in LDS (uniform single-scene fluids).
  * Results are identical either way. */
 #define PS_FLAG_NO_LDS_STAGING 0x8u
+/* With PS_FLAG_CPU_REFERENCE: the stabilisation pre-pass the reference
+ * compiles under USE_STABILIZATION (cpu/src/simulation.h:17-18, off there):
+ * after the constraints are made, STABILIZATION_ITERATIONS = 2 passes
+ * project every boundary constraint with stabile = true
+ * (simulation.cpp:247-270), which moves the start position p with ep and
+ * applies no friction (boundaryconstraint.cpp:14-71); the fluid-and-gas
+ * jitter of these projections is drawn from the counter stream
+ * 2^63 | (iteration at the substep start mod 2^27) << 36 | pass << 35 |
+ * index << 3 | axis.  The passes count towards no particle's divisor.  In
+ * this mode's scope (immovable solids) no contact enters the pre-pass. */
+#define PS_FLAG_STABILIZATION 0x10u
 
 /* Status bits raised by the kernels, read with ps_get_status. */
 #define PS_STATUS_NEIGHBOR_OVERFLOW 0x1u /* a particle had more than max_neighbors

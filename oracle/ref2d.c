@@ -717,6 +717,35 @@ static void boundary(r2d_sys *s, uint32_t i, int axis, int greater, uint64_t ite
     }
 }
 
+/* BoundaryConstraint::project with stabile = true (boundaryconstraint.cpp:
+ * 14-71), the stabilisation pre-pass of USE_STABILIZATION: p moves with ep,
+ * no friction; jitter from the counter 2^63 | iteration << 36 | pass << 35
+ * | index << 3 | axis */
+static void stab_boundary(r2d_sys *s, uint32_t i, int axis, int greater, uint64_t iter, uint32_t pass)
+{
+    const real r = s->radius;
+    const real value = greater ? R(s->prm.box_min[axis]) : R(s->prm.box_max[axis]);
+    const int fl = s->phase[i] == PS_PHASE_FLUID || s->phase[i] == PS_PHASE_GAS;
+    real extra = R(0);
+    if (fl) {
+        uint64_t ctr = (1ull << 63) | (iter << 36) | ((uint64_t)pass << 35) | ((uint64_t)i << 3) |
+                       (uint64_t)axis;
+        uint64_t z = mix64(s->prm.seed + ctr * 0x9E3779B97F4A7C15ull + 1u);
+        extra = R((double)(uint32_t)(z >> 40) * (1.0 / 16777216.0)) * s->jit;
+    }
+    const real d = r + extra;
+    real *e = &s->ep[3 * i], *p = &s->p[3 * i];
+    if (greater) {
+        if (e[axis] >= value + r) return;
+        e[axis] = value + d;
+        p[axis] = value + d;
+    } else {
+        if (e[axis] <= value - r) return;
+        e[axis] = value - d;
+        p[axis] = value - d;
+    }
+}
+
 EXPORT int SFX(r2d_step)(r2d_sys *s, real dt, uint32_t steps)
 {
     const real gx = R(s->prm.gravity[0]), gy = R(s->prm.gravity[1]), gz = R(s->prm.gravity[2]);
@@ -774,6 +803,20 @@ EXPORT int SFX(r2d_step)(r2d_sys *s, real dt, uint32_t steps)
             }
         }
         s->cstart[s->n] = s->ncontact;
+        /* (10)-(15) the stabilisation pre-pass (simulation.cpp:247-270) */
+        if (s->prm.flags & PS_FLAG_STABILIZATION) {
+            const uint64_t it0 = s->iters & ((1ull << 27) - 1ull);
+            for (uint32_t q = 0; q < 2; q++)
+                for (uint32_t i = 0; i < s->n; i++) {
+                    uint32_t f = s->bflags[i];
+                    if (f & BX_LOW) stab_boundary(s, i, 0, 1, it0, q);
+                    if (f & BX_HIGH) stab_boundary(s, i, 0, 0, it0, q);
+                    if (f & BY_LOW) stab_boundary(s, i, 1, 1, it0, q);
+                    if (f & BY_HIGH) stab_boundary(s, i, 1, 0, it0, q);
+                    if (f & BZ_LOW) stab_boundary(s, i, 2, 1, it0, q);
+                    if (f & BZ_HIGH) stab_boundary(s, i, 2, 0, it0, q);
+                }
+        }
         /* (16)-(22) iterations: CONTACT group (per particle, the contacts
          * made while visiting it, then its boundaries x, y, z), STANDARD
          * group (fluid objects in order) */

@@ -38,6 +38,7 @@ PS_FIELD_PHASES = 4
 PS_FLAG_REBUILD_EVERY_ITERATION = 0x1
 PS_FLAG_SLEEPING = 0x2
 PS_FLAG_CPU_REFERENCE = 0x4
+PS_FLAG_STABILIZATION = 0x10
 PS_FLAG_NO_LDS_STAGING = 0x8
 
 PS_STATUS_NEIGHBOR_OVERFLOW = 0x1

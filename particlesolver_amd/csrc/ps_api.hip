@@ -329,6 +329,10 @@ int validate(const ps_params *p, std::string &why)
             why = "box_max must exceed box_min";
             return PS_ERR_INVALID_ARGUMENT;
         }
+    if ((p->flags & PS_FLAG_STABILIZATION) && !(p->flags & PS_FLAG_CPU_REFERENCE)) {
+        why = "PS_FLAG_STABILIZATION needs PS_FLAG_CPU_REFERENCE";
+        return PS_ERR_INVALID_ARGUMENT;
+    }
     return PS_OK;
 }
 
@@ -550,10 +554,13 @@ int cr_begin(ps_system *s, float dt)
         s->cr_swapped = false;
     }
     const bool contacts = cr_contacts(s);
-    LAUNCH(s, PS_K_PREDICT_HASH, ps::launch_cr_predict(s->stream, s->b, s->cr, s->n, dt, !contacts));
+    const bool stab = (s->prm.flags & PS_FLAG_STABILIZATION) != 0;
+    LAUNCH(s, PS_K_PREDICT_HASH,
+           ps::launch_cr_predict(s->stream, s->b, s->cr, s->n, dt, !contacts && !stab));
     if (contacts) LAUNCH(s, PS_K_NEIGHBORS, ps::launch_cr_find_contacts(s->stream, s->b, s->cr, s->n));
-    // without contacts, prediction projects the boundaries of iteration 0
-    s->cr_bnd_done = !contacts;
+    if (stab) LAUNCH(s, PS_K_SOLVE, ps::launch_cr_stabilize(s->stream, s->b, s->cr, s->n));
+    // with neither, prediction projects the boundaries of iteration 0
+    s->cr_bnd_done = !contacts && !stab;
     s->stage = 1;
     return PS_OK;
 }

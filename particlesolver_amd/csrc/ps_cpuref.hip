@@ -221,6 +221,54 @@ __global__ __launch_bounds__(kBlock) void cr_predict_k(const float4 *__restrict_
     if (tm) tm[i] = w != 0.0f ? 1.0f / ((1.0f / w) * ps_exp(-x.y)) : 0.0f;
 }
 
+// (10)-(15) the stabilisation pre-pass (simulation.cpp:247-270, compiled
+// with USE_STABILIZATION): STABILIZATION_ITERATIONS = 2 passes over the
+// particle's boundary constraints, x then y then z, each with stabile =
+// true (boundaryconstraint.cpp:14-71): ep and p both move to the push-out
+// value, no friction.  Jitter from the stream 2^63 | iteration << 36 |
+// pass << 35 | index << 3 | axis.
+__global__ __launch_bounds__(kBlock) void cr_stabilize_k(float4 *__restrict__ E,
+                                                         float4 *__restrict__ X,
+                                                         const uint32_t *__restrict__ bcs,
+                                                         const DevStatus *__restrict__ st,
+                                                         CrConst c, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t bc = bcs[i];
+    if (!(bc & 0x3Fu)) return;
+    float4 e = E[i], x = X[i];
+    float ev[3] = {e.x, e.y, e.z}, xv[3] = {x.x, x.y, x.z};
+    const uint64_t iter = st->iter & ((1ull << 27) - 1ull);
+    for (uint32_t pass = 0; pass < 2; pass++) {
+        for (uint32_t axis = 0; axis < 3; axis++) {
+            const uint32_t lo = 1u << (2 * axis), hi = 2u << (2 * axis);
+            if (!(bc & (lo | hi))) continue;
+            const bool greater = (bc & lo) != 0;
+            float extra = 0.0f;
+            if (!(bc & kNoJitter)) {
+                const uint64_t ctr = (1ull << 63) | (iter << 36) | ((uint64_t)pass << 35) |
+                                     ((uint64_t)i << 3) | (uint64_t)axis;
+                const uint64_t z = cr_mix64(c.seed + ctr * 0x9E3779B97F4A7C15ull + 1ull);
+                extra = ((float)(uint32_t)(z >> 40) * (1.0f / 16777216.0f)) * c.jitter;
+            }
+            const float r = c.radius, d = r + extra;
+            const float value = greater ? c.box_min[axis] : c.box_max[axis];
+            if (greater) {
+                if (ev[axis] >= value + r) continue;
+                ev[axis] = value + d;
+                xv[axis] = value + d;
+            } else {
+                if (ev[axis] <= value - r) continue;
+                ev[axis] = value - d;
+                xv[axis] = value - d;
+            }
+        }
+    }
+    E[i] = make_float4(ev[0], ev[1], ev[2], e.w);
+    X[i] = make_float4(xv[0], xv[1], xv[2], x.w);
+}
+
 // (7) contacts (simulation.cpp:164-199): a pair with exactly one immovable
 // particle (every solid of this mode is immovable, and every other particle
 // movable) whose predicted centres are closer than c.touch.  One wavefront
@@ -755,6 +803,14 @@ hipError_t launch_cr_predict(hipStream_t st, const DevBuffers &b, const CrConst 
     if (!n) return hipSuccess;
     hipLaunchKernelGGL(cr_predict_k, dim3(blocks_for(n)), dim3(kBlock), 0, st, b.X, b.V, b.A, b.P,
                        b.bc, b.force, b.tmass, b.status, c, n, dt, boundaries ? 1 : 0);
+    return hipGetLastError();
+}
+
+hipError_t launch_cr_stabilize(hipStream_t st, const DevBuffers &b, const CrConst &c, uint32_t n)
+{
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(cr_stabilize_k, dim3(blocks_for(n)), dim3(kBlock), 0, st, b.P, b.X, b.bc,
+                       b.status, c, n);
     return hipGetLastError();
 }
 

@@ -276,6 +276,9 @@ hipError_t launch_cr_markers(hipStream_t st, const DevBuffers &b, float4 *marker
 // leaves the boundaries to the contact pass), projected with the
 // boundaries at the start of every iteration, E -> E2 (b.P -> b.Q)
 hipError_t launch_cr_find_contacts(hipStream_t st, const DevBuffers &b, const CrConst &c, uint32_t n);
+// PS_FLAG_STABILIZATION: the two stabilisation passes over the boundaries
+// (b.P and b.X both move)
+hipError_t launch_cr_stabilize(hipStream_t st, const DevBuffers &b, const CrConst &c, uint32_t n);
 hipError_t launch_cr_contacts(hipStream_t st, const DevBuffers &b, const CrConst &c, uint32_t n,
                               uint32_t it);
 // FluidEmitter: solidification over the object [first, first+count), then

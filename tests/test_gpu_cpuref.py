@@ -221,6 +221,23 @@ def test_static_obstacles_bitexact(gpu_available):
     run_parity(sc, [1, 10, 100, 300])
 
 
+@pytest.mark.parametrize("make", [scenes.reference_two_fluids, scenes.reference_volcano,
+                                  lambda: scenes.reference_smoke(open_gas=True, emitter=True)])
+def test_stabilization_bitexact(gpu_available, make):
+    # the USE_STABILIZATION pre-pass (PS_FLAG_STABILIZATION) on three scenes
+    sc = make()
+    sc.params.flags |= capi.PS_FLAG_STABILIZATION
+    run_parity(sc, [1, 10, 100, 250])
+
+
+def test_stabilization_needs_the_cpu_reference_mode(gpu_available):
+    p = scenes.dam_break(4, 4, 4).params.copy()
+    p.flags |= capi.PS_FLAG_STABILIZATION
+    with pytest.raises(capi.PsError) as e:
+        System(p)
+    assert e.value.code == capi.PS_ERR_INVALID_ARGUMENT
+
+
 def test_fluid_emitter_rules(gpu_available):
     sc = scenes.reference_volcano(emitter=False)
     roomy = sc.params.copy()

@@ -14,7 +14,9 @@ Each check catches one plausible slip:
   otherwise, the solid has imass 0 and v = 0 and leaves the object, and
   `lambdas` is looked up with the position in the object's list, so a
   particle reads the lambda of the particle whose index equals its position
-  (0 when that one is outside the object).
+  (0 when that one is outside the object);
+- the stabilisation pre-pass (PS_FLAG_STABILIZATION): the start position
+  moves with the prediction, so the push-out leaves no velocity.
 
 The height-scaled contact mass tmass = m e^{-y} (particle.h:67-73) cancels
 from every contact with an immovable partner, the only contacts here, so
@@ -167,3 +169,28 @@ def test_fp32_tracks_fp64_on_the_volcano():
     b.step(np.float32(sc.dt), 10)
     assert a.n == b.n
     assert np.abs(a.positions().astype(np.float64) - b.positions()).max() < 1e-3
+
+
+def test_stabilization_moves_the_start_position_with_the_prediction():
+    # a particle predicted 0.1 below the floor: the pre-pass puts p and ep at
+    # the push-out height (r plus its jitter), so the substep ends at rest
+    # there; without the pre-pass the push-out shows up as velocity
+    dt = 0.01
+    out = {}
+    for stab in (False, True):
+        p = params(box=10.0)
+        p.box_min[1] = 0.0
+        if stab:
+            p.flags |= 0x10
+        r = Ref2D(p, "f64")
+        r.add_fluid_object(np.array([[0.0, 0.1, 0.0]]), np.array([[0.0, -20.0, 0.0]]),
+                           rest_density=1.0)
+        r.step(dt, 1)
+        out[stab] = (r.positions()[0], r.velocities()[0])
+    jit = float(np.float32(0.003 / 0.25)) * 0.25
+    y, vy = out[True][0][1], out[True][1][1]
+    assert 0.25 <= y < 0.25 + jit and vy == 0.0
+    y0, vy0 = out[False][0][1], out[False][1][1]
+    y_start = float(np.float32(0.1))
+    assert 0.25 <= y0 < 0.25 + jit and vy0 == pytest.approx((y0 - y_start) / dt, rel=1e-12)
+    assert y != y0                                # its own jitter draw
