@@ -302,9 +302,12 @@ __global__ __launch_bounds__(kMaxBlock) void reorder_k(DevBuffers b, uint32_t n)
 // itself, or its LDS index in a staged unit); pad is i's own entry, which
 // fills the last chunk.  Returns the number found (the table keeps the
 // first kmax).
+// p is taken by value: a reference into b.P would be re-read after every
+// table store (the compiler cannot rule out aliasing).  emit(t, j) is
+// j + emit(t, 0) (an offset per stencil row), taken once per row.
 template <typename Range, typename Emit>
 __device__ __forceinline__ uint32_t scan_stencil(const DevBuffers &b, const KConst &k, uint32_t i,
-                                                 uint32_t pad, const float4 &p, uint32_t kmax,
+                                                 uint32_t pad, const float4 p, uint32_t kmax,
                                                  Range range, Emit emit)
 {
     uint4 *out = const_cast<uint4 *>(nbr_lane(b, i));
@@ -314,6 +317,7 @@ __device__ __forceinline__ uint32_t scan_stencil(const DevBuffers &b, const KCon
     for (uint32_t t = 0; t < 9; t++) {
         uint32_t j0, j1;
         if (!range(t, j0, j1)) continue;
+        const uint32_t eoff = emit(t, 0u);
         for (uint32_t jb = j0; jb < j1; jb += 4) {
             float4 qv[4];
 #pragma unroll
@@ -328,7 +332,7 @@ __device__ __forceinline__ uint32_t scan_stencil(const DevBuffers &b, const KCon
                 const uint32_t j = jb + u;
                 if (j < j1 && j != i && r2[u] < r2max) {
                     if (cnt < kmax) {
-                        const uint32_t e = emit(t, j);
+                        const uint32_t e = j + eoff;
                         const uint32_t sl = cnt & 3u;
                         buf.x = sl == 0 ? e : buf.x;
                         buf.y = sl == 1 ? e : buf.y;
