@@ -218,7 +218,11 @@ class DeviceBuffer:
         self.ptr = p.value
 
     def fill(self, byte):
+        """set every byte; waits for the fill, which runs on the null stream
+        and so is not ordered with a system's (non-blocking) stream"""
         rc = self.hip.hipMemset(ctypes.c_void_p(self.ptr), int(byte), ctypes.c_size_t(self.nbytes))
+        if not rc:
+            rc = self.hip.hipDeviceSynchronize()
         if rc:
             raise PsError(PS_ERR_DEVICE, "hipMemset failed: %d" % rc)
 

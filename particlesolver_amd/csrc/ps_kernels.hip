@@ -332,16 +332,13 @@ __device__ __forceinline__ uint32_t scan_stencil(const DevBuffers &b, const KCon
                 const uint32_t j = jb + u;
                 if (j < j1 && j != i && r2[u] < r2max) {
                     if (cnt < kmax) {
-                        const uint32_t e = j + eoff;
-                        const uint32_t sl = cnt & 3u;
-                        buf.x = sl == 0 ? e : buf.x;
-                        buf.y = sl == 1 ? e : buf.y;
-                        buf.z = sl == 2 ? e : buf.z;
-                        buf.w = sl == 3 ? e : buf.w;
-                        if (sl == 3) {
-                            out[(size_t)(cnt >> 2) * 64] = buf;
-                            buf = make_uint4(pad, pad, pad, pad);
-                        }
+                        // shift in: after the 4th entry of a chunk buf holds
+                        // it in order (x first)
+                        buf.x = buf.y;
+                        buf.y = buf.z;
+                        buf.z = buf.w;
+                        buf.w = j + eoff;
+                        if ((cnt & 3u) == 3u) out[(size_t)(cnt >> 2) * 64] = buf;
                     }
                     cnt++;
                 }
@@ -349,7 +346,11 @@ __device__ __forceinline__ uint32_t scan_stencil(const DevBuffers &b, const KCon
         }
     }
     const uint32_t kept = cnt < kmax ? cnt : kmax;
-    if (kept & 3u) out[(size_t)(kept >> 2) * 64] = buf;
+    // a partial last chunk: its r entries sit at the right end of buf
+    const uint32_t r = kept & 3u;
+    if (r == 1u) out[(size_t)(kept >> 2) * 64] = make_uint4(buf.w, pad, pad, pad);
+    else if (r == 2u) out[(size_t)(kept >> 2) * 64] = make_uint4(buf.z, buf.w, pad, pad);
+    else if (r == 3u) out[(size_t)(kept >> 2) * 64] = make_uint4(buf.y, buf.z, buf.w, pad);
     b.ncnt[i] = kept;
     return cnt;
 }
