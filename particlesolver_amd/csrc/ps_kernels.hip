@@ -304,7 +304,8 @@ __global__ __launch_bounds__(kMaxBlock) void reorder_k(DevBuffers b, uint32_t n)
 // first kmax).
 // p is taken by value: a reference into b.P would be re-read after every
 // table store (the compiler cannot rule out aliasing).  emit(t, j) is
-// j + emit(t, 0) (an offset per stencil row), taken once per row.
+// j + emit(t, 0) (an offset per stencil row), taken once per row.  The
+// candidates are visited in the same order as a plain loop over each row.
 template <typename Range, typename Emit>
 __device__ __forceinline__ uint32_t scan_stencil(const DevBuffers &b, const KConst &k, uint32_t i,
                                                  uint32_t pad, const float4 p, uint32_t kmax,
@@ -314,34 +315,55 @@ __device__ __forceinline__ uint32_t scan_stencil(const DevBuffers &b, const KCon
     uint4 buf = make_uint4(pad, pad, pad, pad);
     uint32_t cnt = 0;
     const float r2max = k.search_r2;
+    // an accepted candidate: shifted into buf, which after the 4th entry of
+    // a chunk holds it in order (x first); chunks past kmax are not stored
+    auto accept = [&](uint32_t e) {
+        buf.x = buf.y;
+        buf.y = buf.z;
+        buf.z = buf.w;
+        buf.w = e;
+        if ((cnt & 3u) == 3u && cnt < kmax) out[(size_t)(cnt >> 2) * 64] = buf;
+        cnt++;
+    };
     for (uint32_t t = 0; t < 9; t++) {
         uint32_t j0, j1;
         if (!range(t, j0, j1)) continue;
         const uint32_t eoff = emit(t, 0u);
-        for (uint32_t jb = j0; jb < j1; jb += 4) {
-            float4 qv[4];
+        // the particle's own row (t = 4, which holds slot i) is scanned as
+        // [j0, i) and (i, j1): no candidate needs a test against i
+        const bool own = t == 4u;
+        for (uint32_t seg = 0; seg < (own ? 2u : 1u); seg++) {
+            const uint32_t a = (own && seg == 1u) ? i + 1u : j0;
+            const uint32_t e = (own && seg == 0u) ? i : j1;
+            uint32_t jb = a;
+            // whole groups of four: no bound test; the loads go ahead of the
+            // branches
+            for (; jb + 4u <= e; jb += 4u) {
+                // one address per group, immediate offsets for the four
+                const float4 *q = b.P + jb;
+                float4 qv[4];
 #pragma unroll
-            for (uint32_t u = 0; u < 4; u++) qv[u] = b.P[min(jb + u, j1 - 1)];
-            // the distances first, for all four: loads stay ahead of the branches
-            float r2[4];
+                for (uint32_t u = 0; u < 4; u++) qv[u] = q[u];
+                float r2[4];
 #pragma unroll
-            for (uint32_t u = 0; u < 4; u++)
-                r2[u] = dist2(p.x - qv[u].x, p.y - qv[u].y, p.z - qv[u].z);
+                for (uint32_t u = 0; u < 4; u++)
+                    r2[u] = dist2(p.x - qv[u].x, p.y - qv[u].y, p.z - qv[u].z);
 #pragma unroll
-            for (uint32_t u = 0; u < 4; u++) {
-                const uint32_t j = jb + u;
-                if (j < j1 && j != i && r2[u] < r2max) {
-                    if (cnt < kmax) {
-                        // shift in: after the 4th entry of a chunk buf holds
-                        // it in order (x first)
-                        buf.x = buf.y;
-                        buf.y = buf.z;
-                        buf.z = buf.w;
-                        buf.w = j + eoff;
-                        if ((cnt & 3u) == 3u) out[(size_t)(cnt >> 2) * 64] = buf;
-                    }
-                    cnt++;
-                }
+                for (uint32_t u = 0; u < 4; u++)
+                    if (r2[u] < r2max) accept(jb + u + eoff);
+            }
+            // the last one to three
+            if (jb < e) {
+                float4 qv[3];
+#pragma unroll
+                for (uint32_t u = 0; u < 3; u++) qv[u] = b.P[min(jb + u, e - 1u)];
+                float r2[3];
+#pragma unroll
+                for (uint32_t u = 0; u < 3; u++)
+                    r2[u] = dist2(p.x - qv[u].x, p.y - qv[u].y, p.z - qv[u].z);
+#pragma unroll
+                for (uint32_t u = 0; u < 3; u++)
+                    if (jb + u < e && r2[u] < r2max) accept(jb + u + eoff);
             }
         }
     }
