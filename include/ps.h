@@ -492,6 +492,11 @@ int ps_timing_enable(ps_system *sys, uint32_t kernel_mask);
 int ps_timing_reset(ps_system *sys);
 /* Total milliseconds and launch count for a kernel since the last reset. */
 int ps_timing_read(ps_system *sys, int kernel_id, double *total_ms, uint64_t *launches);
+/* Bracket only one launch in `period` of each enabled kernel (1, the
+ * default: every launch), counting from the call or the last
+ * ps_timing_reset; ps_timing_read then reports the bracketed launches.  A
+ * period coprime with the iteration count samples every iteration alike. */
+int ps_timing_sample(ps_system *sys, uint32_t period);
 const char *ps_kernel_name(int kernel_id);
 
 /* ---- autotuning -------------------------------------------------------- */

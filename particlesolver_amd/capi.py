@@ -147,6 +147,7 @@ PROTOTYPES = {
     "ps_read_markers": (_int, [_P, _u32, ctypes.POINTER(_u32), _P]),
     "ps_add_fluid_emitter": (_int, [_P, _P, ctypes.c_double, ctypes.c_int32, ctypes.POINTER(_u32)]),
     "ps_export_device": (_int, [_P, _u32, _u32, _u32, _P]),
+    "ps_timing_sample": (_int, [_P, _u32]),
     "ps_trajectory_open": (_int, [ctypes.c_char_p, _u32, _u32, ctypes.POINTER(_P)]),
     "ps_trajectory_write": (_int, [_P, _P]),
     "ps_trajectory_close": (_int, [_P]),
@@ -566,6 +567,10 @@ class System:
 
     def timing_reset(self):
         self._check(self.lib.ps_timing_reset(self.h))
+
+    def timing_sample(self, period):
+        """ps_timing_sample: events around one launch in `period`"""
+        self._check(self.lib.ps_timing_sample(self.h, period))
 
     def timing_read(self):
         out = {}

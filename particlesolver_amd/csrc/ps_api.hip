@@ -102,6 +102,8 @@ struct ps_system {
                                   256, 256, 256, 256, 256, 256, 256};  // reverse pass, export: fixed 256
     // timing
     uint32_t timing = 0;  // kernel mask
+    uint32_t timing_period = 1;     // bracket one launch in this many
+    uint64_t seen[PS_K_COUNT] = {};  // launches of each enabled kernel
     std::vector<TimedLaunch> pending;
     std::vector<hipEvent_t> free_events;
     double total_ms[PS_K_COUNT] = {};
@@ -396,6 +398,7 @@ int timed_begin(ps_system *s, int id, TimedLaunch &tl)
     tl.id = id;
     tl.start = tl.stop = nullptr;
     if (!(s->timing & (1u << id))) return PS_OK;
+    if (s->seen[id]++ % s->timing_period) return PS_OK;
     for (hipEvent_t *e : {&tl.start, &tl.stop}) {
         if (!s->free_events.empty()) {
             *e = s->free_events.back();
@@ -2225,7 +2228,18 @@ int ps_timing_reset(ps_system *s)
     for (int k = 0; k < PS_K_COUNT; k++) {
         s->total_ms[k] = 0.0;
         s->launches[k] = 0;
+        s->seen[k] = 0;
     }
+    return PS_OK;
+}
+
+int ps_timing_sample(ps_system *s, uint32_t period)
+{
+    if (int rc = check_sys(s)) return rc;
+    if (period == 0) return fail(s, PS_ERR_INVALID_ARGUMENT, "period must be >= 1");
+    if (int rc = timing_collect(s)) return rc;
+    s->timing_period = period;
+    for (int k = 0; k < PS_K_COUNT; k++) s->seen[k] = 0;
     return PS_OK;
 }
 
