@@ -601,7 +601,9 @@ __global__ __launch_bounds__(kStageUnit) void neighbors_staged_k(DevBuffers b, K
 template <bool UNIFORM, bool STAGED, bool TAPE = false>
 __global__ __launch_bounds__(kMaxBlock) void lambda_k(DevBuffers b, KConst k, uint32_t n, float mu)
 {
-    __shared__ float4 lds[STAGED ? kStageSlots : 1];
+    // staged: neighbour positions only, 12 bytes per slot (36 KB, so four
+    // 256-thread blocks fit in a CU's LDS where 16-byte slots allow three)
+    __shared__ float lds3[STAGED ? 3 * kStageSlots : 1];
     const uint32_t unit = xcd_block();
     const uint32_t i = unit * blockDim.x + threadIdx.x;
     bool staged = false;
@@ -610,7 +612,11 @@ __global__ __launch_bounds__(kMaxBlock) void lambda_k(DevBuffers b, KConst k, ui
         staged = d->staged != 0u;
         if (staged)
             stage_copy<6>(d, n, [&](uint32_t j) { return b.P[j]; },
-                          [&](uint32_t l, const float4 &v) { lds[l] = v; });
+                          [&](uint32_t l, const float4 &v) {
+                              lds3[3 * l] = v.x;
+                              lds3[3 * l + 1] = v.y;
+                              lds3[3 * l + 2] = v.z;
+                          });
     }
     if (i >= n) return;
     float *Pw = reinterpret_cast<float *>(b.P);
@@ -649,7 +655,8 @@ __global__ __launch_bounds__(kMaxBlock) void lambda_k(DevBuffers b, KConst k, ui
     };
     if (STAGED && staged) {
         for_neighbors(
-            b, i, c, [&](uint32_t l) { return lds[l]; },
+            b, i, c,
+            [&](uint32_t l) { return make_float4(lds3[3 * l], lds3[3 * l + 1], lds3[3 * l + 2], 0.0f); },
             [&](uint32_t, const float4 &pj) { body(pj, mu); });
     } else if (UNIFORM) {
         for_neighbors(
