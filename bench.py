@@ -200,6 +200,44 @@ def cpu_baseline(sc, budget_s):
                       "%d substeps after 1 warm-up, %.1f s" % (threads, sc.n, done, el)}
 
 
+# Algorithmic FP32 flops per accepted neighbour pair (distance, kernel
+# values and gradient, 1/r by the seeded Newton form, the tensile power as
+# two squarings), and the MI355X FP32 vector peak (packed FMA; the kernels
+# are built without packed FP32, so their own ceiling is half of it)
+ALGO_FLOPS_PER_PAIR = {"lambda": 25.0, "solve": 30.0}
+FP32_VECTOR_PEAK_TFLOPS = 157.3
+
+
+def copy_bandwidth(mib=512, reps=3):
+    """Device-to-device hipMemcpy of `mib` MiB (the runtime's copy kernel)
+    timed with HIP events: GB/s moved (read + write), best of `reps`."""
+    import ctypes
+    from particlesolver_amd import capi
+    capi.load()
+    hip = ctypes.CDLL("libamdhip64.so.7")      # the runtime the library links
+    n = mib << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    e0, e1 = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(n)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(n)) == 0
+    assert hip.hipEventCreate(ctypes.byref(e0)) == 0 and hip.hipEventCreate(ctypes.byref(e1)) == 0
+    assert hip.hipMemset(a, 1, ctypes.c_size_t(n)) == 0
+    best = 0.0
+    for _ in range(reps + 1):                    # the first is a warm-up
+        assert hip.hipEventRecord(e0, None) == 0
+        assert hip.hipMemcpyAsync(b, a, ctypes.c_size_t(n), 3, None) == 0
+        assert hip.hipEventRecord(e1, None) == 0
+        assert hip.hipEventSynchronize(e1) == 0
+        ms = ctypes.c_float()
+        assert hip.hipEventElapsedTime(ctypes.byref(ms), e0, e1) == 0
+        best = max(best, 2.0 * n / (ms.value * 1e-3) / 1e9)
+    hip.hipEventDestroy(e0)
+    hip.hipEventDestroy(e1)
+    hip.hipFree(a)
+    hip.hipFree(b)
+    return round(best, 1)
+
+
 def copy_calibration(mib):
     """Three hipMemcpy device-to-device copies of `mib` MiB: each is one
     dispatch of the runtime's copy kernel reading and writing exactly that
@@ -599,10 +637,20 @@ def main():
                 "traffic": quoted_traffic(dom, N),
                 "kernel": dom, "algo_bytes_per_launch": int(bytes_per_launch),
                 "K": round(K, 2)}
+        # the same launches against the FP32 vector peak (algorithmic flops)
+        flops = ALGO_FLOPS_PER_PAIR[dom] * K * N
+        tf = flops / per_launch_s / 1e12
+        roof["valu"] = {"algo_flops_per_pair": ALGO_FLOPS_PER_PAIR[dom],
+                        "achieved_TFLOPs": round(tf, 2), "peak_TFLOPs": FP32_VECTOR_PEAK_TFLOPS,
+                        "frac": round(tf / FP32_VECTOR_PEAK_TFLOPS, 4)}
         for k in kernels:
             if k in ALGO_BYTES:
                 b = ALGO_BYTES[k](K) * N
                 kernels[k]["algo_GBs"] = round(b / (kernels[k]["us_per_launch"] * 1e-6) / 1e9, 1)
+
+    if roof is not None:
+        # what a plain copy kernel reaches on this GPU, for scale
+        roof["copy_kernel_GBs"] = copy_bandwidth()
 
     cpu = None
     if not args.no_cpu:
