@@ -15,7 +15,16 @@
  * sleeping (Particle::confirmGuess, cpu/src/particle.h:60-65).  Gas objects
  * (2D, as the reference's dvec2 forms) also accumulate a per-particle force
  * (open-gas drag and the vorticity term) that the next prediction applies,
- * with gas gravity scaled by ALPHA (simulation.cpp:143-154, simulation.h:21).  The reference's compile-time constants
+ * with gas gravity scaled by ALPHA (simulation.cpp:143-154, simulation.h:21).
+ * Immovable solids (new Particle(pos, 0)) and their ContactConstraints with
+ * every other particle (simulation.cpp:164-199, contactconstraint.cpp:14-40,
+ * with Particle::scaleMass, particle.h:67-73), smoke emitters
+ * (opensmokeemitter.cpp:17-46, simulation.cpp:348-362), fluid emitters
+ * (fluidemitter.cpp:13-79) and, with PS_FLAG_STABILIZATION, the
+ * USE_STABILIZATION pre-pass over the boundaries (simulation.cpp:247-270).
+ * Parity unpinned: the height-scaled mass tmass = m e^{-y} of the contacts,
+ * which cancels from every contact with an immovable partner (the only
+ * contacts here), so no closed form sees more than its rounding.  The reference's compile-time constants
  * (PARTICLE_RAD, H, RELAXATION, K_P, E_P, DQ_P, S_SOLID) are runtime
  * parameters; frand() jitter becomes the counter RNG of ps_oracle.c; the
  * vector operations follow glm's order (normalize(r) = r * (1/sqrt(r.r))).
