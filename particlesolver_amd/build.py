@@ -14,9 +14,12 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libparticlesolver.so")
 
 SOURCES = ["ps_kernels.hip", "ps_adjoint.hip", "ps_cpuref.hip", "ps_export.hip", "ps_api.hip",
-           "ps_trajectory.hip", "particle_system.hip"]
+           "ps_trajectory.hip", "particle_system.hip", "simulation.hip"]
 DEMO_SRC = os.path.join(CSRC, "ps_demo.cpp")
 DEMO_BIN = os.path.join(HERE, "ps_demo")
+# programs over the C++ facades: (source, binary)
+DEMOS = [(DEMO_SRC, DEMO_BIN),
+         (os.path.join(CSRC, "sim_demo.cpp"), os.path.join(HERE, "sim_demo"))]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",
@@ -51,18 +54,21 @@ def build_library(verbose=True):
 
 
 def build_cpp_demo(verbose=True):
-    """C++ program that drives the ParticleSystem facade (reference method
-    names) through the library; skipped when its source is absent."""
-    if not os.path.exists(DEMO_SRC):
-        return None
-    cmd = [hipcc(), "-O2", "-std=c++17", "-I" + os.path.join(ROOT, "include"), "-I" + CSRC,
-           DEMO_SRC, "-o", DEMO_BIN + ".tmp", "-L" + HERE, "-lparticlesolver",
-           "-Wl,-rpath,$ORIGIN"]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    subprocess.run(cmd, check=True)
-    os.replace(DEMO_BIN + ".tmp", DEMO_BIN)
-    return DEMO_BIN
+    """C++ programs that drive the ParticleSystem and Simulation facades
+    (the reference's method names) through the library; a program whose
+    source is absent is skipped."""
+    built = []
+    for src, out in DEMOS:
+        if not os.path.exists(src):
+            continue
+        cmd = [hipcc(), "-O2", "-std=c++17", "-I" + os.path.join(ROOT, "include"), "-I" + CSRC,
+               src, "-o", out + ".tmp", "-L" + HERE, "-lparticlesolver", "-Wl,-rpath,$ORIGIN"]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+        os.replace(out + ".tmp", out)
+        built.append(out)
+    return built
 
 
 if __name__ == "__main__":
