@@ -15,6 +15,14 @@ __device__ __forceinline__ uint32_t xcd_block()
     return x * q + (x < r ? x : r) + (b >> 3);
 }
 
+// A status word as this kernel has updated it so far, read before an atomic
+// that may change nothing.  The words only grow (bits set, maxima raised)
+// within a kernel, so a stale value can only cost a redundant atomic.
+__device__ __forceinline__ uint32_t status_peek(const uint32_t *w)
+{
+    return __atomic_load_n(w, __ATOMIC_RELAXED);
+}
+
 // chunk m (entries 4m..4m+3) of particle i is nbr_lane(b, i)[m * 64]
 __device__ __forceinline__ const uint4 *nbr_lane(const DevBuffers &b, uint32_t i)
 {

@@ -153,11 +153,14 @@ __global__ __launch_bounds__(kMaxBlock) void predict_hash_k(DevBuffers b, KConst
             b.rank[s] = base + (lane - seg);
         }
         // status bits: one atomic per wavefront, not per particle (a splash
-        // can put thousands of predictions outside the grid at once)
+        // can put thousands of predictions outside the grid at once), and
+        // none once the bits are set (atomics on one address serialise)
         const uint64_t nf = __ballot((flags & PS_STATUS_NONFINITE) != 0u);
         const uint64_t og = __ballot((flags & PS_STATUS_OUTSIDE_GRID) != 0u);
-        if (lane == 0 && (nf | og))
-            atomicOr(&b.status->flags, (nf ? PS_STATUS_NONFINITE : 0u) | (og ? PS_STATUS_OUTSIDE_GRID : 0u));
+        if (lane == 0 && (nf | og)) {
+            const uint32_t want = (nf ? PS_STATUS_NONFINITE : 0u) | (og ? PS_STATUS_OUTSIDE_GRID : 0u);
+            if ((status_peek(&b.status->flags) & want) != want) atomicOr(&b.status->flags, want);
+        }
     }
 }
 
@@ -388,8 +391,10 @@ __device__ __forceinline__ void neighbor_status(const DevBuffers &b, uint32_t cn
     if (threadIdx.x == 0) {
         uint32_t mm = ldsw[0];
         for (uint32_t q = 1; q < (blockDim.x >> 6); q++) mm = ldsw[q] > mm ? ldsw[q] : mm;
-        if (mm > 0) atomicMax(&b.status->max_neighbors, mm);
-        if (mm > kmax) atomicOr(&b.status->flags, (uint32_t)PS_STATUS_NEIGHBOR_OVERFLOW);
+        // one address for every block: skip the atomics that change nothing
+        if (mm > 0 && mm > status_peek(&b.status->max_neighbors)) atomicMax(&b.status->max_neighbors, mm);
+        if (mm > kmax && !(status_peek(&b.status->flags) & PS_STATUS_NEIGHBOR_OVERFLOW))
+            atomicOr(&b.status->flags, (uint32_t)PS_STATUS_NEIGHBOR_OVERFLOW);
     }
 }
 
