@@ -274,18 +274,27 @@ __global__ __launch_bounds__(kMaxBlock) void reorder_k(DevBuffers b, uint32_t n)
     if (s >= n) return;
     const uint32_t c = b.key[s];
     const Attr a = b.A[s];
+    // the particle's own records go in flight before the count
+    const float4 p = b.P[s], xs = b.X[s], vs = b.V[s];
     const uint32_t b0 = b.cell_off[c], b1 = b.cell_off[c + 1];
     if (b1 > n || b0 >= b1 || a.orig >= n) {
         atomicOr(&b.status->flags, PS_STATUS_INTERNAL);
         return;
     }
+    // the count in groups of four independent loads (one wait per group
+    // instead of one per member)
     uint32_t r = 0;
-    for (uint32_t u = b0; u < b1; u++) r += b.tmp_orig[u] < a.orig ? 1u : 0u;
+    for (uint32_t u = b0; u < b1; u += 4) {
+        uint32_t t[4];
+#pragma unroll
+        for (uint32_t k = 0; k < 4; k++) t[k] = b.tmp_orig[min(u + k, b1 - 1u)];
+#pragma unroll
+        for (uint32_t k = 0; k < 4; k++) r += (u + k < b1 && t[k] < a.orig) ? 1u : 0u;
+    }
     const uint32_t d = b0 + r;
-    const float4 p = b.P[s];
-    b.X2[d] = b.X[s];
+    b.X2[d] = xs;
     b.Q[d] = make_float4(p.x, p.y, p.z, 0.0f);
-    b.V2[d] = b.V[s];
+    b.V2[d] = vs;
     b.A2[d] = a;
     b.key2[d] = c;
     if (b.sid) b.sid2[d] = b.sid[s];
