@@ -22,6 +22,8 @@ SCENES = {
     "fluid2d": (lambda: scenes.two_d_fluid(8, 5), 3),
     "dam_break_4096": (lambda: scenes.dam_break(16, 16, 16), 5),
     "cloth_over_pool_32x32_4096": (lambda: scenes.cloth_over_pool((32, 32), (16, 16, 16)), 6),
+    # the full adjoint workload (128 x 128 cloth over 64 x 16 x 64 fluid), 2 substeps
+    "cloth_over_pool_full_81920": (lambda: scenes.cloth_over_pool(), 2),
 }
 
 

@@ -1,0 +1,64 @@
+"""The 1,048,576-particle dam break (the benchmark's configuration) against
+the FP32 oracle at full size: the sort, the cell offsets, the neighbour table
+and two substeps bit for bit, and properties that hold at any size (the
+table is symmetric, its entries ascend, the cell offsets are the sorted
+keys' bounds).  The oracle runs a substep of this scene in about a quarter
+second on 16 host threads."""
+import numpy as np
+import pytest
+
+from particlesolver_amd import System, scenes
+from oracle import load_scene
+from test_gpu_parity import assert_bitexact
+
+pytestmark = pytest.mark.gpu
+
+
+def test_dam_break_1M_bitexact_and_table_properties(gpu_available):
+    sc = scenes.dam_break()                        # 128 x 64 x 128, h = 0.2 = cell
+    assert sc.n == 1 << 20
+    print("dam break", sc.n, flush=True)
+    ref = load_scene(sc, "f32")
+    with System(sc.params) as g:
+        sc.load_into(g)
+        g.begin(sc.dt)
+        ref.begin(sc.dt)
+        gi, gc, go = g.grid()
+        ri, rc, ro = ref.grid()
+        assert_bitexact(gi, ri, "sorted insertion index")
+        assert_bitexact(gc, rc, "cell keys")
+        assert_bitexact(go, ro, "cell offsets")
+        # sorted by (cell, insertion index); offsets bound the runs of keys
+        assert np.all(np.diff(gc.astype(np.int64)) >= 0)
+        same = np.diff(gc.astype(np.int64)) == 0
+        assert np.all(np.diff(gi.astype(np.int64))[same] > 0)
+        nc = go.shape[0] - 1
+        assert np.array_equal(go, np.searchsorted(gc, np.arange(nc + 1)).astype(np.uint32))
+        cnt, tab = g.neighbors()
+        rcnt, rtab = ref.neighbors()
+        assert_bitexact(cnt, rcnt, "neighbour counts")
+        k = int(cnt.max())
+        assert_bitexact(tab[:, :k], rtab[:, :k], "neighbour table")
+        assert g.status().flags & 1 == 0           # no overflow: every pair is listed
+        # symmetric, ascending, no self entries
+        rows = np.repeat(np.arange(sc.n, dtype=np.int64), cnt.astype(np.int64))
+        mask = np.arange(k)[None, :] < cnt[:, None]
+        cols = tab[:, :k][mask].astype(np.int64)
+        assert np.all(rows != cols)
+        fwd = np.sort(rows * sc.n + cols)
+        bwd = np.sort(cols * sc.n + rows)
+        assert np.array_equal(fwd, bwd)
+        per_row = np.split(cols, np.cumsum(cnt.astype(np.int64))[:-1])
+        assert all(np.all(np.diff(r) > 0) for r in per_row[::4096])
+        del rows, cols, fwd, bwd, per_row, tab, rtab
+        for it in range(sc.params.iterations):
+            g.iterate()
+            ref.iterate()
+        g.end()
+        ref.end()
+        assert_bitexact(g.positions(), ref.positions(), "positions after substep 1")
+        assert_bitexact(g.velocities(), ref.velocities(), "velocities after substep 1")
+        g.step(sc.dt, 1)
+        ref.step(sc.dt, 1)
+        assert_bitexact(g.positions(), ref.positions(), "positions after substep 2")
+        assert_bitexact(g.velocities(), ref.velocities(), "velocities after substep 2")
