@@ -22,6 +22,7 @@ summary kept under profiles/; the bench line then quotes the traffic of its
 dominant kernel from that summary when the workload matches.
 """
 import argparse
+import csv
 import json
 import os
 import sys
@@ -78,6 +79,10 @@ def parse():
                     help="substeps of an extra, separately timed run with events on every kernel")
     ap.add_argument("--no-staging", action="store_true",
                     help="neighbour reads from global memory instead of LDS-staged cells")
+    ap.add_argument("--trace-report", nargs=3, metavar=("STATS_CSV", "TRACE_CSV", "BENCH_LOG"),
+                    help="summarise a rocprofv3 --kernel-trace --stats run of this script "
+                         "(its kernel_stats and kernel_trace CSVs); BENCH_LOG holds that run's "
+                         "JSON line")
     ap.add_argument("--traffic-report", nargs=3, metavar=("FETCH_CSV", "WRITE_CSV", "BENCH_LOG"),
                     help="summarise two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) of this "
                          "script with --copy-calibration; BENCH_LOG holds that run's JSON line")
@@ -158,6 +163,52 @@ def traffic_report(fetch_csv, write_csv, log):
             k, fetch.get(k, 0.0), write.get(k, 0.0), t / 1e6,
             "%.1f" % (algo / 1e6) if algo else "-", "%.2f" % (t / algo) if algo else "-"))
     out += ["", "traffic_bytes_per_launch: " + json.dumps({"particles": N, "kernels": table})]
+    print("\n".join(out))
+
+
+def trace_report(stats_csv, trace_csv, log):
+    """Markdown summary of a kernel trace of this script (printed to stdout):
+    every kernel's statistics over the whole run, and the per-dispatch means
+    of the dominant kernels over the bench's timed window."""
+    line = None
+    for l in open(log):
+        l = l.strip()
+        if l.startswith("{") and '"metric"' in l:
+            line = json.loads(l)
+    rows = list(csv.DictReader(open(stats_csv)))
+    tot = sum(float(r["TotalDurationNs"]) for r in rows)
+    out = ["# Kernel durations: 1,048,576-particle PBF dam break (rocprofv3 kernel trace)", "",
+           "Commands (repository root on the GPU host as R, MI355X):", "", "```",
+           "cd /tmp && export TMPDIR=/tmp && rocprofv3 --kernel-trace --stats --output-format csv "
+           "-d /tmp/kt -o kt -- python3 $R/bench.py --no-cpu --no-adjoint --breakdown 0 "
+           "> $R/gpurun_out/kt.log 2>&1",
+           "cd $R && python3 bench.py --trace-report /tmp/kt/<host>/<pid>_kernel_stats.csv "
+           "/tmp/kt/<host>/<pid>_kernel_trace.csv gpurun_out/kt.log", "```", "",
+           "%d substeps (%d warm-up + %d timed) of the dam break from rest, then the exports of "
+           "the positions. Durations vary over the run: the collapse phase has the most "
+           "neighbours per particle." % (line["warmup"] + line["steps"], line["warmup"], line["steps"]),
+           "", "| kernel | calls | avg us | min us | max us | share |", "|---|---|---|---|---|---|"]
+    for r in sorted(rows, key=lambda r: -float(r["TotalDurationNs"])):
+        out.append("| `%s` | %s | %.1f | %.1f | %.1f | %.1f %% |" % (
+            r["Name"].split("(")[0].replace("void ", ""), r["Calls"], float(r["AverageNs"]) / 1e3,
+            float(r["MinNs"]) / 1e3, float(r["MaxNs"]) / 1e3, 100 * float(r["TotalDurationNs"]) / tot))
+    trace = sorted(csv.DictReader(open(trace_csv)), key=lambda r: int(r["Start_Timestamp"]))
+    warm, steps = line["warmup"], line["steps"]
+    out += ["", "Per-dispatch means over the bench's timed window (substeps %d-%d) from the same "
+            "trace, and the bench line this run printed:" % (warm + 1, warm + steps), ""]
+    for name, per in (("solve_k", 4), ("lambda_k", 4), ("neighbors_staged_k", 1),
+                      ("finalize_k", 1), ("reorder_k", 1), ("predict_hash_k", 1)):
+        ds = [(int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3
+              for r in trace if name in r["Kernel_Name"]]
+        win = ds[warm * per:(warm + steps) * per]
+        if win:
+            out.append("    %s: %d dispatches in the timed window, mean %.2f us (all %d: %.2f us)"
+                       % (name, len(win), sum(win) / len(win), len(ds), sum(ds) / len(ds)))
+    k = line["kernels"]
+    out.append("    bench line of this run: value %.4g, ms_per_step %.4f, solve %.2f us (%d sampled "
+               "launches), lambda %.2f us, roofline frac %.4f"
+               % (line["value"], line["ms_per_step"], k["solve"]["us_per_launch"],
+                  k["solve"]["launches"], k["lambda"]["us_per_launch"], line["roofline"]["frac"]))
     print("\n".join(out))
 
 
@@ -551,6 +602,8 @@ def adjoint_main(args):
 
 def main():
     args = parse()
+    if args.trace_report:
+        return trace_report(*args.trace_report)
     if args.traffic_report:
         return traffic_report(*args.traffic_report)
     if args.gpus != 1:

@@ -62,3 +62,30 @@ def test_dam_break_1M_bitexact_and_table_properties(gpu_available):
         ref.step(sc.dt, 1)
         assert_bitexact(g.positions(), ref.positions(), "positions after substep 2")
         assert_bitexact(g.velocities(), ref.velocities(), "velocities after substep 2")
+
+
+def test_ensemble_512_scenes_graph_bitexact_per_scene(gpu_available):
+    # the batched configuration at full size: 512 scenes, about 9M particles,
+    # an 8-substep captured loop with the tuned block sizes; every scene must
+    # match the oracle run on that scene alone (its own box, gravity scale,
+    # rope and seed).  The oracle takes about 5 s per substep over all scenes
+    # on 8 host threads.
+    b = scenes.ensemble()
+    assert len(b.scenes) == 512
+    print("ensemble", sum(s.n for s in b.scenes), flush=True)
+    with System(b.params, b.descs) as g:
+        b.load_into(g)
+        g.autotune(b.dt, 1)
+        g.step_graph(b.dt, 8)
+        x, v = g.positions(), g.velocities()
+        assert g.status().flags == 0
+    first = 0
+    for q, s in enumerate(b.scenes):
+        sc = b.plain(q)
+        ref = load_scene(sc, "f32")
+        ref.step(sc.dt, 8)
+        assert_bitexact(x[first:first + s.n], ref.positions(), "scene %d positions" % q)
+        assert_bitexact(v[first:first + s.n], ref.velocities(), "scene %d velocities" % q)
+        first += s.n
+        if q % 128 == 127:
+            print("scenes checked", q + 1, flush=True)
