@@ -637,6 +637,30 @@ def fluid_object_metrics(params, objects, pos):
     return rho, rho / rho0 - 1.0
 
 
+def pbf_density_metrics(sc, pos):
+    """FP64 densities of a 3D fluid scene at `pos`, rho_i = m_i W(0) +
+    sum_{j != i, r < h} m_j W_poly6(r) with m = 1/inverse mass (the density of
+    findLambdasD, integration_kernel.cuh:552-577), and the density constraint
+    C_i = rho_i / rho0_i - 1.  Pairs come from a k-d tree (scipy)."""
+    from scipy.spatial import cKDTree
+    ph = np.asarray(sc.phase)
+    w = np.asarray(sc.inv_mass, np.float64)
+    if np.any(ph != capi.PHASE_FLUID) or np.any(w <= 0.0):
+        raise ValueError("pbf_density_metrics takes movable fluid particles only")
+    x = np.asarray(pos, np.float64)
+    h = float(sc.params.h)
+    c = poly6_coef(h)
+    m = 1.0 / w
+    pairs = cKDTree(x, balanced_tree=False, compact_nodes=False).query_pairs(h, output_type="ndarray")
+    i, j = pairs[:, 0], pairs[:, 1]
+    d2 = ((x[i] - x[j]) ** 2).sum(1)
+    wij = np.where(d2 < h * h, c * (h * h - d2) ** 3, 0.0)
+    rho = m * c * h ** 6
+    rho += np.bincount(i, wij * m[j], minlength=x.shape[0])
+    rho += np.bincount(j, wij * m[i], minlength=x.shape[0])
+    return rho, rho / np.asarray(sc.rest_density, np.float64) - 1.0
+
+
 class Batch:
     """Scenes for ps_create_batch: a shared parameter set, one PsSceneDesc per
     scene, and each scene's particles/constraints (global indices are
