@@ -314,10 +314,13 @@ __global__ __launch_bounds__(kMaxBlock) void reorder_k(DevBuffers b, uint32_t n)
 // itself, or its LDS index in a staged unit); pad is i's own entry, which
 // fills the last chunk.  Returns the number found (the table keeps the
 // first kmax).
+// A row is tested in windows of 32 candidates without branches: each test
+// sets one bit of a mask (i's own bit is cleared), and the set bits are then
+// emitted lowest first, so the entries come out in ascending j as from a
+// plain loop over the row, and only accepted candidates pay for the table.
 // p is taken by value: a reference into b.P would be re-read after every
 // table store (the compiler cannot rule out aliasing).  emit(t, j) is
-// j + emit(t, 0) (an offset per stencil row), taken once per row.  The
-// candidates are visited in the same order as a plain loop over each row.
+// j + emit(t, 0) (an offset per stencil row), taken once per row.
 template <typename Range, typename Emit>
 __device__ __forceinline__ uint32_t scan_stencil(const DevBuffers &b, const KConst &k, uint32_t i,
                                                  uint32_t pad, const float4 p, uint32_t kmax,
@@ -341,41 +344,42 @@ __device__ __forceinline__ uint32_t scan_stencil(const DevBuffers &b, const KCon
         uint32_t j0, j1;
         if (!range(t, j0, j1)) continue;
         const uint32_t eoff = emit(t, 0u);
-        // the particle's own row (t = 4, which holds slot i) is scanned as
-        // [j0, i) and (i, j1): no candidate needs a test against i
-        const bool own = t == 4u;
-        for (uint32_t seg = 0; seg < (own ? 2u : 1u); seg++) {
-            const uint32_t a = (own && seg == 1u) ? i + 1u : j0;
-            const uint32_t e = (own && seg == 0u) ? i : j1;
-            uint32_t jb = a;
-            // whole groups of four: no bound test; the loads go ahead of the
-            // branches
-            for (; jb + 4u <= e; jb += 4u) {
+        for (uint32_t w0 = j0; w0 < j1; w0 += 32u) {
+            const uint32_t we = min(w0 + 32u, j1);
+            uint32_t m = 0;
+            uint32_t jb = w0;
+            // whole groups of four: no bound test; the loads go ahead of
+            // the tests
+            for (; jb + 4u <= we; jb += 4u) {
                 // one address per group, immediate offsets for the four
                 const float4 *q = b.P + jb;
                 float4 qv[4];
 #pragma unroll
                 for (uint32_t u = 0; u < 4; u++) qv[u] = q[u];
-                float r2[4];
+                uint32_t g = 0;
 #pragma unroll
                 for (uint32_t u = 0; u < 4; u++)
-                    r2[u] = dist2(p.x - qv[u].x, p.y - qv[u].y, p.z - qv[u].z);
-#pragma unroll
-                for (uint32_t u = 0; u < 4; u++)
-                    if (r2[u] < r2max) accept(jb + u + eoff);
+                    g |= (dist2(p.x - qv[u].x, p.y - qv[u].y, p.z - qv[u].z) < r2max ? 1u : 0u) << u;
+                m |= g << (jb - w0);
             }
             // the last one to three
-            if (jb < e) {
+            if (jb < we) {
                 float4 qv[3];
 #pragma unroll
-                for (uint32_t u = 0; u < 3; u++) qv[u] = b.P[min(jb + u, e - 1u)];
-                float r2[3];
+                for (uint32_t u = 0; u < 3; u++) qv[u] = b.P[min(jb + u, we - 1u)];
+                uint32_t g = 0;
 #pragma unroll
                 for (uint32_t u = 0; u < 3; u++)
-                    r2[u] = dist2(p.x - qv[u].x, p.y - qv[u].y, p.z - qv[u].z);
-#pragma unroll
-                for (uint32_t u = 0; u < 3; u++)
-                    if (jb + u < e && r2[u] < r2max) accept(jb + u + eoff);
+                    g |= (jb + u < we && dist2(p.x - qv[u].x, p.y - qv[u].y, p.z - qv[u].z) < r2max
+                              ? 1u : 0u) << u;
+                m |= g << (jb - w0);
+            }
+            // i itself (in its own row only)
+            if (i - w0 < 32u) m &= ~(1u << (i - w0));
+            while (m != 0u) {
+                const uint32_t u = (uint32_t)__builtin_ctz(m);
+                m &= m - 1u;
+                accept(w0 + u + eoff);
             }
         }
     }
