@@ -126,6 +126,9 @@ def traffic_report(fetch_csv, write_csv, log):
         if l.startswith("{") and '"metric"' in l:
             line = json.loads(l)
     N, K = line["config"]["particles"], line["config"]["mean_neighbors"]
+    if "adjoint" in line:
+        raise SystemExit("traffic report: the counted run must use --no-adjoint (the reverse-pass "
+                         "workload launches the same kernels on another scene)")
     cal = None
     for l in open(log):
         if l.startswith("copy calibration:"):
@@ -139,13 +142,14 @@ def traffic_report(fetch_csv, write_csv, log):
            "Commands (repository root on the GPU host, R = its absolute path):", "",
            "    cd /tmp && export TMPDIR=/tmp",
            "    rocprofv3 --pmc FETCH_SIZE --output-format csv -d $R/gpurun_out/pmc_fetch -o fetch "
-           "-- python3 $R/bench.py --steps 5 --warmup 2 --no-cpu --breakdown 0 "
+           "-- python3 $R/bench.py --steps 5 --warmup 2 --no-cpu --no-adjoint --breakdown 0 "
            "--copy-calibration 512 > $R/gpurun_out/pmc_fetch.log 2>&1",
            "    rocprofv3 --pmc WRITE_SIZE --output-format csv -d $R/gpurun_out/pmc_write -o write "
-           "-- python3 $R/bench.py --steps 5 --warmup 2 --no-cpu --breakdown 0 "
+           "-- python3 $R/bench.py --steps 5 --warmup 2 --no-cpu --no-adjoint --breakdown 0 "
            "--copy-calibration 512 > $R/gpurun_out/pmc_write.log 2>&1",
-           "    python bench.py --traffic-report gpurun_out/pmc_fetch/fetch_counter_collection.csv "
-           "gpurun_out/pmc_write/write_counter_collection.csv gpurun_out/pmc_fetch.log", "",
+           "    cd $R && python3 bench.py --traffic-report "
+           "gpurun_out/pmc_fetch/<host>/<pid>_fetch_counter_collection.csv "
+           "gpurun_out/pmc_write/<host>/<pid>_write_counter_collection.csv gpurun_out/pmc_fetch.log", "",
            "Counters: rocprofv3 FETCH_SIZE and WRITE_SIZE (KiB per dispatch), one run each.",
            "Calibration: the runtime copy kernel of a %d-byte hipMemcpy (device to device) in the "
            "same runs reads FETCH_SIZE %.1f KiB and WRITE_SIZE %.1f KiB for %.1f KiB moved each "
