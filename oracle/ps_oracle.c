@@ -84,6 +84,8 @@ __attribute__((visibility("default"))) real SFX(orc_rsq)(real x) { return RSQ(x)
 static real DIST2(real dx, real dy, real dz) { return FMA(dz, dz, FMA(dy, dy, dx * dx)); }
 /* below this squared distance a pair is degenerate (no direction) */
 #define DEGENERATE_R2 R(1e-8)
+/* the half of a neighbour-table entry k: its chunk of four entries, even or odd */
+#define HALF(k) ((int)(((k) >> 2) & 1u))
 
 #include "orc_exp.h"
 #define EXP orc_exp
@@ -614,10 +616,14 @@ static void lambda_pass(orc_sys *s)
         real px = s->P[4 * i], py = s->P[4 * i + 1], pz = s->P[4 * i + 2];
         real r0 = s->rho0[i];
         real sk = d->spiky * (R(1) / r0);
-        real rho = d->w_zero * (R(1) / s->w[i]);
-        real gix = R(0), giy = R(0), giz = R(0), den = R(0);
+        /* two halves of the table, [0] over its chunks of four entries
+         * 0, 2, 4, ... and [1] over chunks 1, 3, 5, ..., each in table order,
+         * added as [0] + [1] (HALF(k)); half 0 holds the self term */
+        real rho[2] = {d->w_zero * (R(1) / s->w[i]), R(0)};
+        real gix[2] = {R(0), R(0)}, giy[2] = {R(0), R(0)}, giz[2] = {R(0), R(0)}, den[2] = {R(0), R(0)};
         for (uint32_t k = 0; k < s->ncnt[i]; k++) {
             uint32_t j = s->nbr[(size_t)i * kmax + k];
+            const int hf = HALF(k);
             real wj = s->w[j];
             if (wj == R(0)) continue;
             real dx = px - s->P[4 * j], dy = py - s->P[4 * j + 1], dz = pz - s->P[4 * j + 2];
@@ -627,19 +633,21 @@ static void lambda_pass(orc_sys *s)
             real W = d->poly6 * ((t * t) * t);
             real m = R(1) / wj;
             if (s->phase[j] != PS_PHASE_FLUID) m = m * d->s_solid;
-            rho = FMA(W, m, rho);
+            rho[hf] = FMA(W, m, rho[hf]);
             if (r2 < DEGENERATE_R2) continue;
             real ri = RSQ(r2);
             real q = d->h - (r2 * ri);
             real a = (sk * (q * q)) * ri;
-            den = FMA(a * a, r2, den);
-            gix = FMA(a, dx, gix);
-            giy = FMA(a, dy, giy);
-            giz = FMA(a, dz, giz);
+            den[hf] = FMA(a * a, r2, den[hf]);
+            gix[hf] = FMA(a, dx, gix[hf]);
+            giy[hf] = FMA(a, dy, giy[hf]);
+            giz[hf] = FMA(a, dz, giz[hf]);
         }
-        den = den + DIST2(gix, giy, giz);
-        real C = (rho / r0) - R(1);
-        s->P[4 * i + 3] = -C / (den + d->relax);
+        real gx = gix[0] + gix[1], gy = giy[0] + giy[1], gz = giz[0] + giz[1];
+        real dn = (den[0] + den[1]) + DIST2(gx, gy, gz);
+        real rh = rho[0] + rho[1];
+        real C = (rh / r0) - R(1);
+        s->P[4 * i + 3] = -C / (dn + d->relax);
     }
 }
 
@@ -976,9 +984,11 @@ static void solve_pass(orc_sys *s)
          *   r^2 < 1e-8: gradW = (0, -0.001 spiky h^2, 0)   (:625-626) */
         if (ph == PS_PHASE_FLUID) {
             real li = s->P[4 * i + 3], ir0 = R(1) / s->rho0[i];
-            real fx = R(0), fy = R(0), fz = R(0);
+            /* in two halves of the table, as the lambda pass */
+            real hx[2] = {R(0), R(0)}, hy[2] = {R(0), R(0)}, hz[2] = {R(0), R(0)};
             for (uint32_t k = 0; k < s->ncnt[i]; k++) {
                 uint32_t j = s->nbr[(size_t)i * kmax + k];
+                const int hf = HALF(k);
                 if (s->w[j] == R(0)) continue;
                 real dx = px - s->P[4 * j], dy = py - s->P[4 * j + 1], dz = pz - s->P[4 * j + 2];
                 real r2 = DIST2(dx, dy, dz);
@@ -988,16 +998,17 @@ static void solve_pass(orc_sys *s)
                 real corr = -(d->k_corr * ipow(W * d->inv_w_dq, d->n_corr));
                 real coef = (li + s->P[4 * j + 3]) + corr;
                 if (r2 < DEGENERATE_R2) {
-                    fy = fy - (coef * ((d->spiky * (d->h * d->h)) * R(0.001)));
+                    hy[hf] = hy[hf] - (coef * ((d->spiky * (d->h * d->h)) * R(0.001)));
                 } else {
                     real ri = RSQ(r2);
                     real q = d->h - (r2 * ri);
                     real sc = coef * ((d->spiky * (q * q)) * ri);
-                    fx = FMA(-sc, dx, fx);
-                    fy = FMA(-sc, dy, fy);
-                    fz = FMA(-sc, dz, fz);
+                    hx[hf] = FMA(-sc, dx, hx[hf]);
+                    hy[hf] = FMA(-sc, dy, hy[hf]);
+                    hz[hf] = FMA(-sc, dz, hz[hf]);
                 }
             }
+            real fx = hx[0] + hx[1], fy = hy[0] + hy[1], fz = hz[0] + hz[1];
             ax = ax + (fx * ir0);
             ay = ay + (fy * ir0);
             az = az + (fz * ir0);

@@ -80,6 +80,56 @@ __device__ __forceinline__ void for_neighbors(const DevBuffers &b, uint32_t i, u
     }
 }
 
+// The fluid sums (density, λ's gradient sums, the Δp sum) are formed in two
+// halves, h = 0 over the table's chunks 0, 2, 4, ... and h = 1 over chunks
+// 1, 3, 5, ..., each in table order, and then added as half 0 + half 1.  The
+// staged kernels give each half its own lane (two lanes per particle, twice
+// the wavefronts for the same LDS); the others visit both halves in turn.
+// Visits the entries of half h of particle i's first c neighbours, pipelined
+// as for_neighbors.
+template <typename G, typename Fn>
+__device__ __forceinline__ void for_neighbors_half(const DevBuffers &b, uint32_t i, uint32_t c, uint32_t h,
+                                                   G gather, Fn fn)
+{
+    const uint32_t nch = (c + 3u) >> 2;  // chunks in use
+    if (h >= nch) return;
+    const uint4 *nb = nbr_lane(b, i);
+    const uint32_t last = ((nch - h + 1u) >> 1) - 1u;  // this half's chunks: 2m + h, m <= last
+    auto chunk = [&](uint32_t m) { return nb[(size_t)(2u * min(m, last) + h) * 64]; };
+    auto visit = [&](const uint4 &jj, uint32_t m, const auto &d0, const auto &d1, const auto &d2,
+                     const auto &d3) {
+        const uint32_t q = (2u * m + h) << 2;  // table position of jj.x
+        fn(jj.x, d0);
+        if (q + 1 < c) fn(jj.y, d1);
+        if (q + 2 < c) fn(jj.z, d2);
+        if (q + 3 < c) fn(jj.w, d3);
+    };
+    uint4 ja = chunk(0);
+    uint4 jb = chunk(1);
+    auto a0 = gather(ja.x);
+    auto a1 = gather(ja.y);
+    auto a2 = gather(ja.z);
+    auto a3 = gather(ja.w);
+    for (uint32_t m = 0;; m += 2) {
+        const uint4 jc = chunk(m + 2);
+        auto b0 = gather(jb.x);
+        auto b1 = gather(jb.y);
+        auto b2 = gather(jb.z);
+        auto b3 = gather(jb.w);
+        visit(ja, m, a0, a1, a2, a3);
+        if (m + 1 > last) break;
+        const uint4 jd = chunk(m + 3);
+        a0 = gather(jc.x);
+        a1 = gather(jc.y);
+        a2 = gather(jc.z);
+        a3 = gather(jc.w);
+        visit(jb, m + 1, b0, b1, b2, b3);
+        if (m + 2 > last) break;
+        ja = jc;
+        jb = jd;
+    }
+}
+
 // As for_neighbors, one chunk at a time: the four gathers of chunk m are
 // issued together (and the index load of chunk m+1), then processed.  Half
 // the records in flight, for kernels whose records or arithmetic are large.

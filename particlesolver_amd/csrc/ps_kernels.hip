@@ -442,15 +442,18 @@ __global__ __launch_bounds__(kMaxBlock) void neighbors_k(DevBuffers b, KConst k,
     neighbor_status(b, cnt, kmax, ldsw);
 }
 
-// Copy a staged unit's chunks of the current data into LDS.  Thread t
-// handles element t % 16 of chunks c = 16 k + t / 16, BATCH chunks' loads in
-// flight at a time; a chunk's elements past its interval read a clamped slot
-// and fill padding that no table entry refers to.  load(j) reads slot j,
-// store(l, v) writes LDS index l.  Every thread of the block calls it.
-template <uint32_t BATCH, typename Load, typename Store>
+// Copy a staged unit's chunks of the current data into LDS.  With THREADS
+// threads, thread t handles element t % 16 of chunks c = (THREADS/16) k +
+// t / 16, BATCH chunks' loads in flight at a time; a chunk's elements past
+// its interval read a clamped slot and fill padding that no table entry
+// refers to.  load(j) reads slot j, store(l, v) writes LDS index l.  Every
+// thread of the block calls it.
+template <uint32_t BATCH, uint32_t THREADS = kStageUnit, typename Load, typename Store>
 __device__ __forceinline__ void stage_copy(const StageDesc *d, uint32_t n, Load load, Store store)
 {
-    constexpr uint32_t kPerStep = kStageUnit / kStageChunk;  // chunks per step
+    static_assert(THREADS % kStageChunk == 0 && kStageChunks % (THREADS / kStageChunk) == 0,
+                  "whole chunks per step");
+    constexpr uint32_t kPerStep = THREADS / kStageChunk;  // chunks per step
     constexpr uint32_t kSteps = kStageChunks / kPerStep;
     const uint32_t nch = d->nchunks, sub = threadIdx.x % kStageChunk;
     const uint32_t c0 = threadIdx.x / kStageChunk;
@@ -620,21 +623,23 @@ template <bool UNIFORM, bool STAGED, bool TAPE = false>
 __global__ __launch_bounds__(kMaxBlock) void lambda_k(DevBuffers b, KConst k, uint32_t n, float mu)
 {
     // staged: neighbour positions only, 12 bytes per slot (36 KB, so four
-    // 256-thread blocks fit in a CU's LDS where 16-byte slots allow three)
+    // blocks fit in a CU's LDS where 16-byte slots allow three); two lanes
+    // per particle (2 kStageUnit threads), lane hh summing half hh
     __shared__ float lds3[STAGED ? 3 * kStageSlots : 1];
     const uint32_t unit = xcd_block();
-    const uint32_t i = unit * blockDim.x + threadIdx.x;
+    const uint32_t hh = STAGED ? (threadIdx.x & 1u) : 0u;
+    const uint32_t i = STAGED ? unit * kStageUnit + (threadIdx.x >> 1) : unit * blockDim.x + threadIdx.x;
     bool staged = false;
     if (STAGED) {
         const StageDesc *d = b.stage + unit;
         staged = d->staged != 0u;
         if (staged)
-            stage_copy<6>(d, n, [&](uint32_t j) { return b.P[j]; },
-                          [&](uint32_t l, const float4 &v) {
-                              lds3[3 * l] = v.x;
-                              lds3[3 * l + 1] = v.y;
-                              lds3[3 * l + 2] = v.z;
-                          });
+            stage_copy<6, 2 * kStageUnit>(d, n, [&](uint32_t j) { return b.P[j]; },
+                                          [&](uint32_t l, const float4 &v) {
+                                              lds3[3 * l] = v.x;
+                                              lds3[3 * l + 1] = v.y;
+                                              lds3[3 * l + 2] = v.z;
+                                          });
     }
     if (i >= n) return;
     float *Pw = reinterpret_cast<float *>(b.P);
@@ -650,48 +655,72 @@ __global__ __launch_bounds__(kMaxBlock) void lambda_k(DevBuffers b, KConst k, ui
     const float4 p = b.P[i];
     const float r0 = a.rho0;
     const float sk = k.spiky * (1.0f / r0);
-    float rho = k.w_zero * (UNIFORM ? mu : (1.0f / a.w));
-    float gix = 0.0f, giy = 0.0f, giz = 0.0f, den = 0.0f;
     const uint32_t c = b.ncnt[i];
+    // the sums of one half of the table (half 0 also holds the self term)
+    struct Sums {
+        float rho, gix, giy, giz, den;
+    };
+    const float self = k.w_zero * (UNIFORM ? mu : (1.0f / a.w));
     // mj: 1/w_j, times s_solid for a non-fluid neighbour
-    auto body = [&](const float4 &pj, float mj) {
+    auto body = [&](Sums &s, const float4 &pj, float mj) {
         float dx = p.x - pj.x, dy = p.y - pj.y, dz = p.z - pj.z;
         float r2 = dist2(dx, dy, dz);
         if (!(r2 < k.h2)) return;
         float t = k.h2 - r2;
         float W = k.poly6 * ((t * t) * t);
-        rho = __builtin_fmaf(W, mj, rho);
+        s.rho = __builtin_fmaf(W, mj, s.rho);
         const float ri = rsq(r2);
         const float qq = k.h - (r2 * ri);
         // a degenerate pair adds exact zeros: its d components are tiny and
         // finite, and sums that start at +0 never become -0, so x + 0 = x
         const float aa = r2 < kDegenerateR2 ? 0.0f : (sk * (qq * qq)) * ri;
-        den = __builtin_fmaf(aa * aa, r2, den);
-        gix = __builtin_fmaf(aa, dx, gix);
-        giy = __builtin_fmaf(aa, dy, giy);
-        giz = __builtin_fmaf(aa, dz, giz);
+        s.den = __builtin_fmaf(aa * aa, r2, s.den);
+        s.gix = __builtin_fmaf(aa, dx, s.gix);
+        s.giy = __builtin_fmaf(aa, dy, s.giy);
+        s.giz = __builtin_fmaf(aa, dz, s.giz);
     };
-    if (STAGED && staged) {
-        for_neighbors(
-            b, i, c,
-            [&](uint32_t l) { return make_float4(lds3[3 * l], lds3[3 * l + 1], lds3[3 * l + 2], 0.0f); },
-            [&](uint32_t, const float4 &pj) { body(pj, mu); });
-    } else if (UNIFORM) {
-        for_neighbors(
-            b, i, c, [&](uint32_t j) { return b.P[j]; },
-            [&](uint32_t, const float4 &pj) { body(pj, mu); });
+    auto half = [&](Sums &s, uint32_t h) {
+        if (STAGED && staged) {
+            for_neighbors_half(
+                b, i, c, h,
+                [&](uint32_t l) { return make_float4(lds3[3 * l], lds3[3 * l + 1], lds3[3 * l + 2], 0.0f); },
+                [&](uint32_t, const float4 &pj) { body(s, pj, mu); });
+        } else if (UNIFORM) {
+            for_neighbors_half(
+                b, i, c, h, [&](uint32_t j) { return b.P[j]; },
+                [&](uint32_t, const float4 &pj) { body(s, pj, mu); });
+        } else {
+            for_neighbors_half(
+                b, i, c, h, [&](uint32_t j) { return PA{b.P[j], b.A[j]}; },
+                [&](uint32_t, const PA &d) {
+                    if (d.a.w == 0.0f) return;
+                    float mj = 1.0f / d.a.w;
+                    if (d.a.phase != PS_PHASE_FLUID) mj = mj * k.s_solid;
+                    body(s, d.p, mj);
+                });
+        }
+    };
+    Sums s0 = {self, 0.0f, 0.0f, 0.0f, 0.0f}, s1 = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    if (STAGED) {
+        // this lane's half; the partner lane (lane ^ 1) holds the other.
+        // Both lanes form half 0 + half 1 (addition commutes)
+        Sums mine = {hh ? 0.0f : self, 0.0f, 0.0f, 0.0f, 0.0f};
+        half(mine, hh);
+        const Sums other = {__shfl_xor(mine.rho, 1, 64), __shfl_xor(mine.gix, 1, 64),
+                            __shfl_xor(mine.giy, 1, 64), __shfl_xor(mine.giz, 1, 64),
+                            __shfl_xor(mine.den, 1, 64)};
+        s0 = hh ? other : mine;
+        s1 = hh ? mine : other;
     } else {
-        for_neighbors(
-            b, i, c, [&](uint32_t j) { return PA{b.P[j], b.A[j]}; },
-            [&](uint32_t, const PA &d) {
-                if (d.a.w == 0.0f) return;
-                float mj = 1.0f / d.a.w;
-                if (d.a.phase != PS_PHASE_FLUID) mj = mj * k.s_solid;
-                body(d.p, mj);
-            });
+        half(s0, 0u);
+        half(s1, 1u);
     }
+    const float rho = s0.rho + s1.rho;
+    const float gix = s0.gix + s1.gix, giy = s0.giy + s1.giy, giz = s0.giz + s1.giz;
+    float den = s0.den + s1.den;
     den = den + dist2(gix, giy, giz);
     float C = (rho / r0) - 1.0f;
+    if (STAGED && hh) return;  // the pair's lane 0 writes
     Pw[4 * (size_t)i + 3] = -C / (den + k.relax);
     if (TAPE) {
         b.tape_lq[i] = make_float4(gix, giy, giz, den);
@@ -711,16 +740,19 @@ template <uint32_t F, bool UNIFORM, uint32_t NC, bool STAGED, bool TAPE = false>
 __global__ __launch_bounds__(kMaxBlock) void solve_k(DevBuffers b, KConst k, uint32_t n, float wu,
                                                   uint32_t it)
 {
+    // staged: two lanes per particle (2 kStageUnit threads), lane hh summing
+    // half hh of the fluid sum; lane 0 finishes the particle
     __shared__ float4 lds[STAGED ? kStageSlots : 1];
     const uint32_t unit = xcd_block();
-    const uint32_t i = unit * blockDim.x + threadIdx.x;
+    const uint32_t hh = STAGED ? (threadIdx.x & 1u) : 0u;
+    const uint32_t i = STAGED ? unit * kStageUnit + (threadIdx.x >> 1) : unit * blockDim.x + threadIdx.x;
     bool staged = false;
     if (STAGED) {
         const StageDesc *d = b.stage + unit;
         staged = d->staged != 0u;
         if (staged)
-            stage_copy<6>(d, n, [&](uint32_t j) { return b.P[j]; },
-                          [&](uint32_t l, const float4 &v) { lds[l] = v; });
+            stage_copy<6, 2 * kStageUnit>(d, n, [&](uint32_t j) { return b.P[j]; },
+                                          [&](uint32_t l, const float4 &v) { lds[l] = v; });
     }
     if (i >= n) return;
     const float4 p = b.P[i];
@@ -816,8 +848,11 @@ __global__ __launch_bounds__(kMaxBlock) void solve_k(DevBuffers b, KConst k, uin
             const float li = p.w, ir0 = 1.0f / a.rho0;
             // r^2 < 1e-8: gradient (0, -0.001 spiky h^2, 0) (integration_kernel.cuh:625-626)
             const float gdeg = (k.spiky * (k.h * k.h)) * 0.001f;
-            float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-            auto body = [&](const float4 &pj) {
+            // the sum of one half of the table
+            struct Sums {
+                float fx, fy, fz;
+            };
+            auto body = [&](Sums &s, const float4 &pj) {
                 float dx = px - pj.x, dy = py - pj.y, dz = pz - pj.z;
                 float r2 = dist2(dx, dy, dz);
                 if (!(r2 < k.h2)) return;
@@ -831,29 +866,45 @@ __global__ __launch_bounds__(kMaxBlock) void solve_k(DevBuffers b, KConst k, uin
                 // a degenerate pair (essentially never taken: one compare per
                 // pair) has only its y term
                 if (r2 < kDegenerateR2) {
-                    fy = fy - (coef * gdeg);
+                    s.fy = s.fy - (coef * gdeg);
                 } else {
-                    fx = __builtin_fmaf(-sc, dx, fx);
-                    fy = __builtin_fmaf(-sc, dy, fy);
-                    fz = __builtin_fmaf(-sc, dz, fz);
+                    s.fx = __builtin_fmaf(-sc, dx, s.fx);
+                    s.fy = __builtin_fmaf(-sc, dy, s.fy);
+                    s.fz = __builtin_fmaf(-sc, dz, s.fz);
                 }
             };
-            if (STAGED && staged) {
-                for_neighbors(
-                    b, i, c, [&](uint32_t l) { return lds[l]; },
-                    [&](uint32_t, const float4 &pj) { body(pj); });
-            } else if (UNIFORM) {
-                for_neighbors(
-                    b, i, c, [&](uint32_t j) { return b.P[j]; },
-                    [&](uint32_t, const float4 &pj) { body(pj); });
+            auto half = [&](Sums &s, uint32_t h) {
+                if (STAGED && staged) {
+                    for_neighbors_half(
+                        b, i, c, h, [&](uint32_t l) { return lds[l]; },
+                        [&](uint32_t, const float4 &pj) { body(s, pj); });
+                } else if (UNIFORM) {
+                    for_neighbors_half(
+                        b, i, c, h, [&](uint32_t j) { return b.P[j]; },
+                        [&](uint32_t, const float4 &pj) { body(s, pj); });
+                } else {
+                    for_neighbors_half(
+                        b, i, c, h, [&](uint32_t j) { return PA{b.P[j], b.A[j]}; },
+                        [&](uint32_t, const PA &d) {
+                            if (d.a.w == 0.0f) return;
+                            body(s, d.p);
+                        });
+                }
+            };
+            Sums s0 = {0.0f, 0.0f, 0.0f}, s1 = {0.0f, 0.0f, 0.0f};
+            if (STAGED) {
+                // this lane's half; the partner lane (lane ^ 1) holds the other
+                Sums mine = {0.0f, 0.0f, 0.0f};
+                half(mine, hh);
+                const Sums other = {__shfl_xor(mine.fx, 1, 64), __shfl_xor(mine.fy, 1, 64),
+                                    __shfl_xor(mine.fz, 1, 64)};
+                s0 = hh ? other : mine;
+                s1 = hh ? mine : other;
             } else {
-                for_neighbors(
-                    b, i, c, [&](uint32_t j) { return PA{b.P[j], b.A[j]}; },
-                    [&](uint32_t, const PA &d) {
-                        if (d.a.w == 0.0f) return;
-                        body(d.p);
-                    });
+                half(s0, 0u);
+                half(s1, 1u);
             }
+            const float fx = s0.fx + s1.fx, fy = s0.fy + s1.fy, fz = s0.fz + s1.fz;
             ax = ax + (fx * ir0);
             ay = ay + (fy * ir0);
             az = az + (fz * ir0);
@@ -864,6 +915,9 @@ __global__ __launch_bounds__(kMaxBlock) void solve_k(DevBuffers b, KConst k, uin
             }
         }
     }
+
+    // the staged variant is fluid-only: the pair's lane 0 finishes the particle
+    if (STAGED && hh) return;
 
     // a9: distance constraints, gathered over this particle's incident list
     if (F & F_DIST) {
@@ -1188,7 +1242,7 @@ hipError_t launch_lambda(hipStream_t st, const DevBuffers &b, const KConst &k, u
     else if (b.tape_lq)
         hipLaunchKernelGGL((lambda_k<false, false, true>), g, t, 0, st, b, k, n, mu);
     else if (uniform && b.staging)
-        hipLaunchKernelGGL((lambda_k<true, true>), dim3(blocks_for(n, kStageUnit)), dim3(kStageUnit),
+        hipLaunchKernelGGL((lambda_k<true, true>), dim3(blocks_for(n, kStageUnit)), dim3(2 * kStageUnit),
                            0, st, b, k, n, mu);
     else if (uniform)
         hipLaunchKernelGGL((lambda_k<true, false>), g, t, 0, st, b, k, n, mu);
@@ -1211,7 +1265,7 @@ static void solve_variant(hipStream_t st, const DevBuffers &b, const KConst &k, 
                           bool uniform, float wu, uint32_t iter, uint32_t bs)
 {
     const dim3 g(blocks_for(n, bs)), t(bs);
-    const dim3 gs(blocks_for(n, kStageUnit)), ts(kStageUnit);
+    const dim3 gs(blocks_for(n, kStageUnit)), ts(2 * kStageUnit);  // two lanes per particle
     const bool n4 = k.n_corr == 4u;
     // the staged variant exists for the uniform pure-fluid composition only
     constexpr bool CAN_STAGE = F == F_FLUID;
