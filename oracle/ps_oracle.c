@@ -1125,9 +1125,11 @@ ORC_EXPORT int SFX(orc_end)(orc_sys *s)
         real ux = px - s->X[3 * i], uy = py - s->X[3 * i + 1], uz = pz - s->X[3 * i + 2];
         real vx = ux / dt, vy = uy / dt, vz = uz / dt;
         if (s->phase[i] == PS_PHASE_FLUID && d->xsph_c != R(0)) {
-            real sx = R(0), sy = R(0), sz = R(0);
+            /* in two halves of the table, as the lambda pass */
+            real hx[2] = {R(0), R(0)}, hy[2] = {R(0), R(0)}, hz[2] = {R(0), R(0)};
             for (uint32_t k = 0; k < s->ncnt[i]; k++) {
                 uint32_t j = s->nbr[(size_t)i * kmax + k];
+                const int hf = HALF(k);
                 if (s->phase[j] != PS_PHASE_FLUID) continue;
                 real qx = s->P[4 * j], qy = s->P[4 * j + 1], qz = s->P[4 * j + 2];
                 real dx = px - qx, dy = py - qy, dz = pz - qz;
@@ -1137,10 +1139,11 @@ ORC_EXPORT int SFX(orc_end)(orc_sys *s)
                 real W = d->poly6 * ((t * t) * t);
                 real m = W * (R(1) / s->w[j]);
                 real ex = qx - s->X[3 * j], ey = qy - s->X[3 * j + 1], ez = qz - s->X[3 * j + 2];
-                sx = FMA(ex - ux, m, sx);
-                sy = FMA(ey - uy, m, sy);
-                sz = FMA(ez - uz, m, sz);
+                hx[hf] = FMA(ex - ux, m, hx[hf]);
+                hy[hf] = FMA(ey - uy, m, hy[hf]);
+                hz[hf] = FMA(ez - uz, m, hz[hf]);
             }
+            real sx = hx[0] + hx[1], sy = hy[0] + hy[1], sz = hz[0] + hz[1];
             real cr = d->xsph_c / s->rho0[i];
             vx = vx + ((cr * sx) / dt);
             vy = vy + ((cr * sy) / dt);

@@ -529,29 +529,34 @@ __global__ __launch_bounds__(kAdjBlock) void adj_xsph_k(DevBuffers b, KConst k, 
         const float4 vb = a.gv[ai.orig];
         const float vbn[3] = {vb.x, vb.y, vb.z};
         const float cr = k.xsph_c / ai.rho0;
-        float Sx = 0.0f, Sy = 0.0f, Sz = 0.0f;
-        auto body = [&](const float4 &pj, const float4 &xj, float wj) {
+        // S exactly as finalize_k forms it: two halves of the table
+        float Sh0[3] = {0.0f, 0.0f, 0.0f}, Sh1[3] = {0.0f, 0.0f, 0.0f};
+        auto body = [&](float *S, const float4 &pj, const float4 &xj, float wj) {
             const float r2 = dist2(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z);
             if (!(r2 < k.h2)) return;
             const float t = k.h2 - r2;
             const float m = (k.poly6 * ((t * t) * t)) * (1.0f / wj);
-            Sx = __builtin_fmaf((pj.x - xj.x) - ui[0], m, Sx);
-            Sy = __builtin_fmaf((pj.y - xj.y) - ui[1], m, Sy);
-            Sz = __builtin_fmaf((pj.z - xj.z) - ui[2], m, Sz);
+            S[0] = __builtin_fmaf((pj.x - xj.x) - ui[0], m, S[0]);
+            S[1] = __builtin_fmaf((pj.y - xj.y) - ui[1], m, S[1]);
+            S[2] = __builtin_fmaf((pj.z - xj.z) - ui[2], m, S[2]);
         };
         const uint32_t c = b.ncnt[i];
-        if (UNIFORM) {
-            for_neighbors(
-                b, i, c, [&](uint32_t j) { return PX{b.P[j], b.X[j]}; },
-                [&](uint32_t, const PX &d) { body(d.p, d.x, ai.w); });
-        } else {
-            for_neighbors(
-                b, i, c, [&](uint32_t j) { return GXA{b.P[j], b.X[j], b.A[j]}; },
-                [&](uint32_t, const GXA &d) {
-                    if (d.a.phase == PS_PHASE_FLUID) body(d.p, d.x, d.a.w);
-                });
-        }
-        const float S[3] = {Sx, Sy, Sz};
+        auto half = [&](float *S, uint32_t h) {
+            if (UNIFORM) {
+                for_neighbors_half(
+                    b, i, c, h, [&](uint32_t j) { return PX{b.P[j], b.X[j]}; },
+                    [&](uint32_t, const PX &d) { body(S, d.p, d.x, ai.w); });
+            } else {
+                for_neighbors_half(
+                    b, i, c, h, [&](uint32_t j) { return GXA{b.P[j], b.X[j], b.A[j]}; },
+                    [&](uint32_t, const GXA &d) {
+                        if (d.a.phase == PS_PHASE_FLUID) body(S, d.p, d.x, d.a.w);
+                    });
+            }
+        };
+        half(Sh0, 0u);
+        half(Sh1, 1u);
+        const float S[3] = {Sh0[0] + Sh1[0], Sh0[1] + Sh1[1], Sh0[2] + Sh1[2]};
         const float vS = dot3(vbn, S);
         r0 = r0 - (((k.xsph_c / (ai.rho0 * ai.rho0)) * vS) / dt);
         for (int q = 0; q < 3; q++) Sh[q] = (vbn[q] * cr) / dt;

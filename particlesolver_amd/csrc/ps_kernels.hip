@@ -1085,22 +1085,25 @@ __global__ __launch_bounds__(kMaxBlock) void finalize_k(DevBuffers b, KConst k, 
 {
     // staged: neighbour position and displacement u = p - x* (x, y, z, u.x)
     // and (u.y, u.z), formed with the same subtractions as the global path
+    // two lanes per particle when staged, as in lambda_k and solve_k
     __shared__ float4 lds_pu[STAGED ? kStageSlots : 1];
     __shared__ float2 lds_u[STAGED ? kStageSlots : 1];
     const uint32_t unit = xcd_block();
-    const uint32_t i = unit * blockDim.x + threadIdx.x;
+    const uint32_t hh = STAGED ? (threadIdx.x & 1u) : 0u;
+    const uint32_t i = STAGED ? unit * kStageUnit + (threadIdx.x >> 1) : unit * blockDim.x + threadIdx.x;
     bool staged = false;
     if (STAGED) {
         const StageDesc *d = b.stage + unit;
         staged = d->staged != 0u;
         if (staged)
-            stage_copy<4>(d, n, [&](uint32_t j) { return PX{b.P[j], b.X[j]}; },
-                          [&](uint32_t l, const PX &v) {
-                              lds_pu[l] = make_float4(v.p.x, v.p.y, v.p.z, v.p.x - v.x.x);
-                              lds_u[l] = make_float2(v.p.y - v.x.y, v.p.z - v.x.z);
-                          });
+            stage_copy<4, 2 * kStageUnit>(d, n, [&](uint32_t j) { return PX{b.P[j], b.X[j]}; },
+                                          [&](uint32_t l, const PX &v) {
+                                              lds_pu[l] = make_float4(v.p.x, v.p.y, v.p.z, v.p.x - v.x.x);
+                                              lds_u[l] = make_float2(v.p.y - v.x.y, v.p.z - v.x.z);
+                                          });
     }
-    if (i == 0) b.status->iter += its;  // every solve of this substep has completed
+    // every solve of this substep has completed
+    if (i == 0 && hh == 0) b.status->iter += its;
     if (i >= n) return;
     const float4 p = b.P[i];
     const float4 x = b.X[i];
@@ -1108,19 +1111,22 @@ __global__ __launch_bounds__(kMaxBlock) void finalize_k(DevBuffers b, KConst k, 
     float vx = ux / dt, vy = uy / dt, vz = uz / dt;
     const Attr a = b.A[i];
     if ((UNIFORM || a.phase == PS_PHASE_FLUID) && k.xsph_c != 0.0f) {
-        float sx = 0.0f, sy = 0.0f, sz = 0.0f;
         const uint32_t c = b.ncnt[i];
+        // the XSPH sum of one half of the table
+        struct Sums {
+            float sx, sy, sz;
+        };
         // qj: neighbour position, e: its displacement over the substep
-        auto body = [&](const float4 &qj, float ex, float ey, float ez, float mj) {
+        auto body = [&](Sums &s, const float4 &qj, float ex, float ey, float ez, float mj) {
             float dx = p.x - qj.x, dy = p.y - qj.y, dz = p.z - qj.z;
             float r2 = dist2(dx, dy, dz);
             if (!(r2 < k.h2)) return;
             float t = k.h2 - r2;
             float W = k.poly6 * ((t * t) * t);
             float m = W * mj;
-            sx = __builtin_fmaf(ex - ux, m, sx);
-            sy = __builtin_fmaf(ey - uy, m, sy);
-            sz = __builtin_fmaf(ez - uz, m, sz);
+            s.sx = __builtin_fmaf(ex - ux, m, s.sx);
+            s.sy = __builtin_fmaf(ey - uy, m, s.sy);
+            s.sz = __builtin_fmaf(ez - uz, m, s.sz);
         };
         struct PU {
             float4 pu;
@@ -1130,29 +1136,46 @@ __global__ __launch_bounds__(kMaxBlock) void finalize_k(DevBuffers b, KConst k, 
             float4 p, x;
             Attr a;
         };
-        if (STAGED && staged) {
-            for_neighbors(
-                b, i, c, [&](uint32_t l) { return PU{lds_pu[l], lds_u[l]}; },
-                [&](uint32_t, const PU &d) { body(d.pu, d.pu.w, d.u.x, d.u.y, mu); });
-        } else if (UNIFORM) {
-            for_neighbors(
-                b, i, c, [&](uint32_t j) { return PX{b.P[j], b.X[j]}; },
-                [&](uint32_t, const PX &d) {
-                    body(d.p, d.p.x - d.x.x, d.p.y - d.x.y, d.p.z - d.x.z, mu);
-                });
+        auto half = [&](Sums &s, uint32_t h) {
+            if (STAGED && staged) {
+                for_neighbors_half(
+                    b, i, c, h, [&](uint32_t l) { return PU{lds_pu[l], lds_u[l]}; },
+                    [&](uint32_t, const PU &d) { body(s, d.pu, d.pu.w, d.u.x, d.u.y, mu); });
+            } else if (UNIFORM) {
+                for_neighbors_half(
+                    b, i, c, h, [&](uint32_t j) { return PX{b.P[j], b.X[j]}; },
+                    [&](uint32_t, const PX &d) {
+                        body(s, d.p, d.p.x - d.x.x, d.p.y - d.x.y, d.p.z - d.x.z, mu);
+                    });
+            } else {
+                for_neighbors_half(
+                    b, i, c, h, [&](uint32_t j) { return PXA{b.P[j], b.X[j], b.A[j]}; },
+                    [&](uint32_t, const PXA &d) {
+                        if (d.a.phase != PS_PHASE_FLUID) return;
+                        body(s, d.p, d.p.x - d.x.x, d.p.y - d.x.y, d.p.z - d.x.z, 1.0f / d.a.w);
+                    });
+            }
+        };
+        Sums s0 = {0.0f, 0.0f, 0.0f}, s1 = {0.0f, 0.0f, 0.0f};
+        if (STAGED) {
+            // this lane's half; the partner lane (lane ^ 1) holds the other
+            Sums mine = {0.0f, 0.0f, 0.0f};
+            half(mine, hh);
+            const Sums other = {__shfl_xor(mine.sx, 1, 64), __shfl_xor(mine.sy, 1, 64),
+                                __shfl_xor(mine.sz, 1, 64)};
+            s0 = hh ? other : mine;
+            s1 = hh ? mine : other;
         } else {
-            for_neighbors(
-                b, i, c, [&](uint32_t j) { return PXA{b.P[j], b.X[j], b.A[j]}; },
-                [&](uint32_t, const PXA &d) {
-                    if (d.a.phase != PS_PHASE_FLUID) return;
-                    body(d.p, d.p.x - d.x.x, d.p.y - d.x.y, d.p.z - d.x.z, 1.0f / d.a.w);
-                });
+            half(s0, 0u);
+            half(s1, 1u);
         }
+        const float sx = s0.sx + s1.sx, sy = s0.sy + s1.sy, sz = s0.sz + s1.sz;
         const float cr = k.xsph_c / a.rho0;
         vx = vx + ((cr * sx) / dt);
         vy = vy + ((cr * sy) / dt);
         vz = vz + ((cr * sz) / dt);
     }
+    if (STAGED && hh) return;  // the pair's lane 0 writes
     b.V[i] = make_float4(vx, vy, vz, 0.0f);
 }
 
@@ -1318,7 +1341,7 @@ hipError_t launch_finalize(hipStream_t st, const DevBuffers &b, const KConst &k,
     const float mu = 1.0f / wu;
     if (uniform && b.staging)
         hipLaunchKernelGGL((finalize_k<true, true>), dim3(blocks_for(n, kStageUnit)),
-                           dim3(kStageUnit), 0, st, b, k, n, mu, dt, its);
+                           dim3(2 * kStageUnit), 0, st, b, k, n, mu, dt, its);
     else if (uniform)
         hipLaunchKernelGGL((finalize_k<true, false>), g, t, 0, st, b, k, n, mu, dt, its);
     else
