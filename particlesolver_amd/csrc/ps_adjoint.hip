@@ -541,21 +541,21 @@ __global__ __launch_bounds__(kAdjBlock) void adj_xsph_k(DevBuffers b, KConst k, 
             S[2] = __builtin_fmaf((pj.z - xj.z) - ui[2], m, S[2]);
         };
         const uint32_t c = b.ncnt[i];
-        auto half = [&](float *S, uint32_t h) {
-            if (UNIFORM) {
-                for_neighbors_half(
-                    b, i, c, h, [&](uint32_t j) { return PX{b.P[j], b.X[j]}; },
-                    [&](uint32_t, const PX &d) { body(S, d.p, d.x, ai.w); });
-            } else {
-                for_neighbors_half(
-                    b, i, c, h, [&](uint32_t j) { return GXA{b.P[j], b.X[j], b.A[j]}; },
-                    [&](uint32_t, const GXA &d) {
-                        if (d.a.phase == PS_PHASE_FLUID) body(S, d.p, d.x, d.a.w);
-                    });
-            }
-        };
-        half(Sh0, 0u);
-        half(Sh1, 1u);
+        if (UNIFORM) {
+            for_neighbors_split(
+                b, i, c, [&](uint32_t j) { return PX{b.P[j], b.X[j]}; },
+                [&](uint32_t, const PX &d) { body(Sh0, d.p, d.x, ai.w); },
+                [&](uint32_t, const PX &d) { body(Sh1, d.p, d.x, ai.w); });
+        } else {
+            for_neighbors_split(
+                b, i, c, [&](uint32_t j) { return GXA{b.P[j], b.X[j], b.A[j]}; },
+                [&](uint32_t, const GXA &d) {
+                    if (d.a.phase == PS_PHASE_FLUID) body(Sh0, d.p, d.x, d.a.w);
+                },
+                [&](uint32_t, const GXA &d) {
+                    if (d.a.phase == PS_PHASE_FLUID) body(Sh1, d.p, d.x, d.a.w);
+                });
+        }
         const float S[3] = {Sh0[0] + Sh1[0], Sh0[1] + Sh1[1], Sh0[2] + Sh1[2]};
         const float vS = dot3(vbn, S);
         r0 = r0 - (((k.xsph_c / (ai.rho0 * ai.rho0)) * vS) / dt);

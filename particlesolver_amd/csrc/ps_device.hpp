@@ -130,6 +130,52 @@ __device__ __forceinline__ void for_neighbors_half(const DevBuffers &b, uint32_t
     }
 }
 
+// Both halves in one pass, as for_neighbors: fn0 gets the entries of the
+// even chunks (half 0), fn1 those of the odd chunks (half 1), each half in
+// table order.  The pipelined loop's two record sets hold an even and an
+// odd chunk, so the half of every call is fixed where it is made.
+template <typename G, typename Fn0, typename Fn1>
+__device__ __forceinline__ void for_neighbors_split(const DevBuffers &b, uint32_t i, uint32_t c, G gather,
+                                                    Fn0 fn0, Fn1 fn1)
+{
+    if (c == 0) return;
+    const uint4 *nb = nbr_lane(b, i);
+    const uint32_t last = (c - 1) >> 2;
+    auto chunk = [&](uint32_t m) { return nb[(size_t)min(m, last) * 64]; };
+    auto visit = [&](auto &fn, const uint4 &jj, uint32_t q, const auto &d0, const auto &d1,
+                     const auto &d2, const auto &d3) {
+        fn(jj.x, d0);
+        if (q + 1 < c) fn(jj.y, d1);
+        if (q + 2 < c) fn(jj.z, d2);
+        if (q + 3 < c) fn(jj.w, d3);
+    };
+    uint4 ja = nb[0];
+    uint4 jb = chunk(1);
+    auto a0 = gather(ja.x);
+    auto a1 = gather(ja.y);
+    auto a2 = gather(ja.z);
+    auto a3 = gather(ja.w);
+    for (uint32_t m = 0;; m += 2) {
+        // a*: chunk m (even), b*: chunk m+1 (odd)
+        const uint4 jc = chunk(m + 2);
+        auto b0 = gather(jb.x);
+        auto b1 = gather(jb.y);
+        auto b2 = gather(jb.z);
+        auto b3 = gather(jb.w);
+        visit(fn0, ja, m << 2, a0, a1, a2, a3);
+        if (m + 1 > last) break;
+        const uint4 jd = chunk(m + 3);
+        a0 = gather(jc.x);
+        a1 = gather(jc.y);
+        a2 = gather(jc.z);
+        a3 = gather(jc.w);
+        visit(fn1, jb, (m + 1) << 2, b0, b1, b2, b3);
+        if (m + 2 > last) break;
+        ja = jc;
+        jb = jd;
+    }
+}
+
 // As for_neighbors, one chunk at a time: the four gathers of chunk m are
 // issued together (and the index load of chunk m+1), then processed.  Half
 // the records in flight, for kernels whose records or arithmetic are large.

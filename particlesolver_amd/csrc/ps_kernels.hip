@@ -679,41 +679,41 @@ __global__ __launch_bounds__(kMaxBlock) void lambda_k(DevBuffers b, KConst k, ui
         s.giy = __builtin_fmaf(aa, dy, s.giy);
         s.giz = __builtin_fmaf(aa, dz, s.giz);
     };
-    auto half = [&](Sums &s, uint32_t h) {
-        if (STAGED && staged) {
-            for_neighbors_half(
-                b, i, c, h,
-                [&](uint32_t l) { return make_float4(lds3[3 * l], lds3[3 * l + 1], lds3[3 * l + 2], 0.0f); },
-                [&](uint32_t, const float4 &pj) { body(s, pj, mu); });
-        } else if (UNIFORM) {
-            for_neighbors_half(
-                b, i, c, h, [&](uint32_t j) { return b.P[j]; },
-                [&](uint32_t, const float4 &pj) { body(s, pj, mu); });
-        } else {
-            for_neighbors_half(
-                b, i, c, h, [&](uint32_t j) { return PA{b.P[j], b.A[j]}; },
-                [&](uint32_t, const PA &d) {
-                    if (d.a.w == 0.0f) return;
-                    float mj = 1.0f / d.a.w;
-                    if (d.a.phase != PS_PHASE_FLUID) mj = mj * k.s_solid;
-                    body(s, d.p, mj);
-                });
-        }
-    };
     Sums s0 = {self, 0.0f, 0.0f, 0.0f, 0.0f}, s1 = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-    if (STAGED) {
-        // this lane's half; the partner lane (lane ^ 1) holds the other.
+    if constexpr (STAGED) {
+        // this lane's half (from LDS, or from global memory for a unit that
+        // was not staged); the partner lane (lane ^ 1) holds the other.
         // Both lanes form half 0 + half 1 (addition commutes)
         Sums mine = {hh ? 0.0f : self, 0.0f, 0.0f, 0.0f, 0.0f};
-        half(mine, hh);
+        if (staged)
+            for_neighbors_half(
+                b, i, c, hh,
+                [&](uint32_t l) { return make_float4(lds3[3 * l], lds3[3 * l + 1], lds3[3 * l + 2], 0.0f); },
+                [&](uint32_t, const float4 &pj) { body(mine, pj, mu); });
+        else
+            for_neighbors_half(
+                b, i, c, hh, [&](uint32_t j) { return b.P[j]; },
+                [&](uint32_t, const float4 &pj) { body(mine, pj, mu); });
         const Sums other = {__shfl_xor(mine.rho, 1, 64), __shfl_xor(mine.gix, 1, 64),
                             __shfl_xor(mine.giy, 1, 64), __shfl_xor(mine.giz, 1, 64),
                             __shfl_xor(mine.den, 1, 64)};
         s0 = hh ? other : mine;
         s1 = hh ? mine : other;
+    } else if (UNIFORM) {
+        for_neighbors_split(
+            b, i, c, [&](uint32_t j) { return b.P[j]; },
+            [&](uint32_t, const float4 &pj) { body(s0, pj, mu); },
+            [&](uint32_t, const float4 &pj) { body(s1, pj, mu); });
     } else {
-        half(s0, 0u);
-        half(s1, 1u);
+        auto gen = [&](Sums &s, const PA &d) {
+            if (d.a.w == 0.0f) return;
+            float mj = 1.0f / d.a.w;
+            if (d.a.phase != PS_PHASE_FLUID) mj = mj * k.s_solid;
+            body(s, d.p, mj);
+        };
+        for_neighbors_split(
+            b, i, c, [&](uint32_t j) { return PA{b.P[j], b.A[j]}; },
+            [&](uint32_t, const PA &d) { gen(s0, d); }, [&](uint32_t, const PA &d) { gen(s1, d); });
     }
     const float rho = s0.rho + s1.rho;
     const float gix = s0.gix + s1.gix, giy = s0.giy + s1.giy, giz = s0.giz + s1.giz;
@@ -873,36 +873,35 @@ __global__ __launch_bounds__(kMaxBlock) void solve_k(DevBuffers b, KConst k, uin
                     s.fz = __builtin_fmaf(-sc, dz, s.fz);
                 }
             };
-            auto half = [&](Sums &s, uint32_t h) {
-                if (STAGED && staged) {
-                    for_neighbors_half(
-                        b, i, c, h, [&](uint32_t l) { return lds[l]; },
-                        [&](uint32_t, const float4 &pj) { body(s, pj); });
-                } else if (UNIFORM) {
-                    for_neighbors_half(
-                        b, i, c, h, [&](uint32_t j) { return b.P[j]; },
-                        [&](uint32_t, const float4 &pj) { body(s, pj); });
-                } else {
-                    for_neighbors_half(
-                        b, i, c, h, [&](uint32_t j) { return PA{b.P[j], b.A[j]}; },
-                        [&](uint32_t, const PA &d) {
-                            if (d.a.w == 0.0f) return;
-                            body(s, d.p);
-                        });
-                }
-            };
             Sums s0 = {0.0f, 0.0f, 0.0f}, s1 = {0.0f, 0.0f, 0.0f};
-            if (STAGED) {
+            if constexpr (STAGED) {
                 // this lane's half; the partner lane (lane ^ 1) holds the other
                 Sums mine = {0.0f, 0.0f, 0.0f};
-                half(mine, hh);
+                if (staged)
+                    for_neighbors_half(
+                        b, i, c, hh, [&](uint32_t l) { return lds[l]; },
+                        [&](uint32_t, const float4 &pj) { body(mine, pj); });
+                else
+                    for_neighbors_half(
+                        b, i, c, hh, [&](uint32_t j) { return b.P[j]; },
+                        [&](uint32_t, const float4 &pj) { body(mine, pj); });
                 const Sums other = {__shfl_xor(mine.fx, 1, 64), __shfl_xor(mine.fy, 1, 64),
                                     __shfl_xor(mine.fz, 1, 64)};
                 s0 = hh ? other : mine;
                 s1 = hh ? mine : other;
+            } else if (UNIFORM) {
+                for_neighbors_split(
+                    b, i, c, [&](uint32_t j) { return b.P[j]; },
+                    [&](uint32_t, const float4 &pj) { body(s0, pj); },
+                    [&](uint32_t, const float4 &pj) { body(s1, pj); });
             } else {
-                half(s0, 0u);
-                half(s1, 1u);
+                auto gen = [&](Sums &s, const PA &d) {
+                    if (d.a.w == 0.0f) return;
+                    body(s, d.p);
+                };
+                for_neighbors_split(
+                    b, i, c, [&](uint32_t j) { return PA{b.P[j], b.A[j]}; },
+                    [&](uint32_t, const PA &d) { gen(s0, d); }, [&](uint32_t, const PA &d) { gen(s1, d); });
             }
             const float fx = s0.fx + s1.fx, fy = s0.fy + s1.fy, fz = s0.fz + s1.fz;
             ax = ax + (fx * ir0);
@@ -1136,38 +1135,37 @@ __global__ __launch_bounds__(kMaxBlock) void finalize_k(DevBuffers b, KConst k, 
             float4 p, x;
             Attr a;
         };
-        auto half = [&](Sums &s, uint32_t h) {
-            if (STAGED && staged) {
-                for_neighbors_half(
-                    b, i, c, h, [&](uint32_t l) { return PU{lds_pu[l], lds_u[l]}; },
-                    [&](uint32_t, const PU &d) { body(s, d.pu, d.pu.w, d.u.x, d.u.y, mu); });
-            } else if (UNIFORM) {
-                for_neighbors_half(
-                    b, i, c, h, [&](uint32_t j) { return PX{b.P[j], b.X[j]}; },
-                    [&](uint32_t, const PX &d) {
-                        body(s, d.p, d.p.x - d.x.x, d.p.y - d.x.y, d.p.z - d.x.z, mu);
-                    });
-            } else {
-                for_neighbors_half(
-                    b, i, c, h, [&](uint32_t j) { return PXA{b.P[j], b.X[j], b.A[j]}; },
-                    [&](uint32_t, const PXA &d) {
-                        if (d.a.phase != PS_PHASE_FLUID) return;
-                        body(s, d.p, d.p.x - d.x.x, d.p.y - d.x.y, d.p.z - d.x.z, 1.0f / d.a.w);
-                    });
-            }
+        auto glob = [&](Sums &s, const PX &d) {
+            body(s, d.p, d.p.x - d.x.x, d.p.y - d.x.y, d.p.z - d.x.z, mu);
         };
         Sums s0 = {0.0f, 0.0f, 0.0f}, s1 = {0.0f, 0.0f, 0.0f};
-        if (STAGED) {
+        if constexpr (STAGED) {
             // this lane's half; the partner lane (lane ^ 1) holds the other
             Sums mine = {0.0f, 0.0f, 0.0f};
-            half(mine, hh);
+            if (staged)
+                for_neighbors_half(
+                    b, i, c, hh, [&](uint32_t l) { return PU{lds_pu[l], lds_u[l]}; },
+                    [&](uint32_t, const PU &d) { body(mine, d.pu, d.pu.w, d.u.x, d.u.y, mu); });
+            else
+                for_neighbors_half(
+                    b, i, c, hh, [&](uint32_t j) { return PX{b.P[j], b.X[j]}; },
+                    [&](uint32_t, const PX &d) { glob(mine, d); });
             const Sums other = {__shfl_xor(mine.sx, 1, 64), __shfl_xor(mine.sy, 1, 64),
                                 __shfl_xor(mine.sz, 1, 64)};
             s0 = hh ? other : mine;
             s1 = hh ? mine : other;
+        } else if (UNIFORM) {
+            for_neighbors_split(
+                b, i, c, [&](uint32_t j) { return PX{b.P[j], b.X[j]}; },
+                [&](uint32_t, const PX &d) { glob(s0, d); }, [&](uint32_t, const PX &d) { glob(s1, d); });
         } else {
-            half(s0, 0u);
-            half(s1, 1u);
+            auto gen = [&](Sums &s, const PXA &d) {
+                if (d.a.phase != PS_PHASE_FLUID) return;
+                body(s, d.p, d.p.x - d.x.x, d.p.y - d.x.y, d.p.z - d.x.z, 1.0f / d.a.w);
+            };
+            for_neighbors_split(
+                b, i, c, [&](uint32_t j) { return PXA{b.P[j], b.X[j], b.A[j]}; },
+                [&](uint32_t, const PXA &d) { gen(s0, d); }, [&](uint32_t, const PXA &d) { gen(s1, d); });
         }
         const float sx = s0.sx + s1.sx, sy = s0.sy + s1.sy, sz = s0.sz + s1.sz;
         const float cr = k.xsph_c / a.rho0;
