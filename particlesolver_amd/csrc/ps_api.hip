@@ -1798,12 +1798,14 @@ int adj_taped_substep(ps_system *s, float dt, size_t t)
     const bool uni = uniform_fluid(s);
     for (uint32_t it = 0; !rc && e == hipSuccess && it < I; it++) {
         const size_t q = (t * I + it) * c;
+        // the taping lambda pass also writes the iterate after it to tape
         b.tape_lq = w.tLQ + q;
         b.tape_rho = w.tRho + q;
+        b.tape_pit = w.tPit + q;
         e = ps::launch_lambda(s->stream, b, s->kc, s->n, uni, s->wu, s->block[PS_K_LAMBDA]);
         b.tape_lq = nullptr;
         b.tape_rho = nullptr;
-        cp(w.tPit + q, b.P, sizeof(float4) * n);
+        b.tape_pit = nullptr;
         b.clamp_rec = w.tClamp + q;
         b.wall_rec = w.tW + q;
         b.tape_f = w.tF + q;

@@ -153,6 +153,7 @@ struct DevBuffers {
     float4 *wall_rec;      // with clamp_rec: position before wall friction,
                            // normal code (n_x+1)*9 + (n_y+1)*3 + (n_z+1) in .w
     float4 *tape_lq;       // lambda pass: S = sum_j a d (xyz), D (w)
+    float4 *tape_pit;      // lambda pass: the iterate after it (x, y, z, lambda)
     float *tape_rho;       // lambda pass: density
     float4 *tape_f;        // solve: fluid sum f (xyz), contact count bits (w)
     // CPU-reference mode only

@@ -647,8 +647,10 @@ __global__ __launch_bounds__(kMaxBlock) void lambda_k(DevBuffers b, KConst k, ui
     if (!UNIFORM && a.phase != PS_PHASE_FLUID) {
         Pw[4 * (size_t)i + 3] = 0.0f;
         if (TAPE) {
+            const float4 pn = b.P[i];
             b.tape_lq[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             b.tape_rho[i] = 0.0f;
+            b.tape_pit[i] = make_float4(pn.x, pn.y, pn.z, 0.0f);
         }
         return;
     }
@@ -721,8 +723,10 @@ __global__ __launch_bounds__(kMaxBlock) void lambda_k(DevBuffers b, KConst k, ui
     den = den + dist2(gix, giy, giz);
     float C = (rho / r0) - 1.0f;
     if (STAGED && hh) return;  // the pair's lane 0 writes
-    Pw[4 * (size_t)i + 3] = -C / (den + k.relax);
+    const float lam = -C / (den + k.relax);
+    Pw[4 * (size_t)i + 3] = lam;
     if (TAPE) {
+        b.tape_pit[i] = make_float4(p.x, p.y, p.z, lam);
         b.tape_lq[i] = make_float4(gix, giy, giz, den);
         b.tape_rho[i] = rho;
     }
