@@ -181,6 +181,8 @@ def trace_report(stats_csv, trace_csv, log):
             line = json.loads(l)
     rows = list(csv.DictReader(open(stats_csv)))
     tot = sum(float(r["TotalDurationNs"]) for r in rows)
+    if line["metric"].startswith("reverse-pass"):
+        return _trace_report_adjoint(rows, tot, line)
     out = ["# Kernel durations: 1,048,576-particle PBF dam break (rocprofv3 kernel trace)", "",
            "Commands (repository root on the GPU host as R, MI355X):", "", "```",
            "cd /tmp && export TMPDIR=/tmp && rocprofv3 --kernel-trace --stats --output-format csv "
@@ -213,6 +215,36 @@ def trace_report(stats_csv, trace_csv, log):
                "launches), lambda %.2f us, roofline frac %.4f"
                % (line["value"], line["ms_per_step"], k["solve"]["us_per_launch"],
                   k["solve"]["launches"], k["lambda"]["us_per_launch"], line["roofline"]["frac"]))
+    print("\n".join(out))
+
+
+def _trace_report_adjoint(rows, tot, line):
+    """The kernel-trace summary of a --workload adjoint_cloth_pool run."""
+    c = line["config"]
+    out = ["# Kernel durations: reverse pass on cloth over a fluid pool (rocprofv3 kernel trace)", "",
+           "Commands (repository root on the GPU host as R, MI355X):", "", "```",
+           "cd /tmp && export TMPDIR=/tmp && rocprofv3 --kernel-trace --stats --output-format csv "
+           "-d /tmp/kta -o kta -- python3 $R/bench.py --workload adjoint_cloth_pool --no-cpu "
+           "> $R/gpurun_out/kta.log 2>&1",
+           "cd $R && python3 bench.py --trace-report /tmp/kta/<host>/<pid>_kernel_stats.csv "
+           "/tmp/kta/<host>/<pid>_kernel_trace.csv gpurun_out/kta.log", "```", "",
+           "%s. The run holds one forward pass for the target (perturbed parameters), warm-up, "
+           "one timed forward pass, four reverse passes (two with checkpoints every %d "
+           "substeps, two with the whole horizon on tape) and one more with events on every "
+           "reverse kernel. Forward kernels appear in every pass (plain and taping variants); "
+           "the `adj_*` kernels only in the reverse passes; `__amd_rocclr_copyBuffer` is the "
+           "tape and checkpoint copies." % (c["workload"], c["checkpoint_every"]), "",
+           "| kernel | calls | avg us | min us | max us | share |", "|---|---|---|---|---|---|"]
+    for r in sorted(rows, key=lambda r: -float(r["TotalDurationNs"])):
+        out.append("| `%s` | %s | %.1f | %.1f | %.1f | %.1f %% |" % (
+            r["Name"].split("(")[0].replace("void ", ""), r["Calls"], float(r["AverageNs"]) / 1e3,
+            float(r["MinNs"]) / 1e3, float(r["MaxNs"]) / 1e3, 100 * float(r["TotalDurationNs"]) / tot))
+    out += ["", "The bench line of the same command (HIP events on the handle's stream): forward "
+            "%.2f ms per %d substeps; reverse pass %.2f ms with checkpoints every %d (%.3fx) and "
+            "%.2f ms with the whole horizon on tape (%.3fx)."
+            % (c["forward_ms"], c["substeps"], c["backward_ms"], c["checkpoint_every"],
+               c["backward_over_forward"], c["backward_ms_full_tape"],
+               c["backward_over_forward_full_tape"])]
     print("\n".join(out))
 
 
