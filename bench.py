@@ -63,12 +63,13 @@ def parse():
     ap.add_argument("--no-timing", action="store_true",
                     help="no per-kernel events in the timed region")
     ap.add_argument("--workload", default="dam_break",
-                    choices=["dam_break", "ensemble", "rayleigh_taylor_2d", "adjoint_cloth_pool"],
+                    choices=["dam_break", "ensemble", "rayleigh_taylor_2d", "adjoint_cloth_pool", "mixed"],
                     help="dam_break: the 1M-particle PBF metric; ensemble: 512 batched "
                          "scenes (one graph launch per 8 substeps) vs 512 per-scene systems; "
                          "rayleigh_taylor_2d: the 1,600-particle 2D scene in the reference's "
                          "CPU-solver semantics; adjoint_cloth_pool: the reverse pass "
-                         "(ps_step_backward) on 16,384 cloth + 65,536 fluid particles")
+                         "(ps_step_backward) on 16,384 cloth + 65,536 fluid particles; mixed: the "
+                         "765,536-particle cloth + rigid cubes + fluid scene")
     ap.add_argument("--no-adjoint", action="store_true",
                     help="dam_break: skip the reverse-pass summary")
     ap.add_argument("--scenes", type=int, default=512)
@@ -342,6 +343,46 @@ def copy_calibration(mib):
     hip.hipFree(a)
     hip.hipFree(b)
     print("copy calibration: 3 x %d bytes" % n, file=sys.stderr)
+
+
+def mixed_main(args):
+    """The 765,536-particle mixed-phase scene: a 256x256 cloth (structural,
+    shear and bend links, corners pinned), 3,125 rigid 4x4x4 cubes with
+    friction and 500,000 fluid particles of two rest densities, 4 iterations
+    per substep.  Timed like the dam break (warm-up, then `steps` substeps
+    between synchronisations); no per-kernel events."""
+    from particlesolver_amd import System, scenes
+    # the settling cubes and cloth pack up to ~210 neighbours within h: a
+    # table of 256 keeps every pair (no overflow)
+    sc = scenes.mixed(max_neighbors=256)
+    N, I = sc.n, sc.params.iterations
+    g = System(sc.params)
+    sc.load_into(g)
+    g.step(sc.dt, args.warmup)
+    g.synchronize()
+    t0 = time.perf_counter()
+    g.step(sc.dt, args.steps)
+    g.synchronize()
+    el = time.perf_counter() - t0
+    st = g.status()
+    k = float(g.neighbors()[0].mean())
+    g.close()
+    line = {
+        "metric": "particle-iterations/s, 3D mixed-phase scene (cloth, rigid cubes, fluid; 4 iterations/substep)",
+        "value": round(N * I * args.steps / el, 1), "unit": "particle-iterations/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(1000.0 * el / max(1, args.steps), 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded lattice jitter, seeded cube lattice and fluid densities)",
+        "config": {"workload": "mixed: 256x256 cloth (structural/shear/bend 1.0/0.5/0.1, corners "
+                               "pinned) + 3125 rigid 4x4x4 cubes (mu_s 0.6, mu_k 0.4) + 500000 "
+                               "fluid in two blocks of rest density 1.0 and 2.0 lattice sums = %d particles" % N,
+                   "particles": N, "iterations": I, "substeps_per_s": round(args.steps / el, 2),
+                   "mean_neighbors": round(k, 2), "max_neighbors_seen": st.max_neighbors,
+                   "status_flags": st.flags},
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(line))
 
 
 def ensemble_main(args):
@@ -652,6 +693,8 @@ def main():
         return rt2d_main(args)
     if args.workload == "adjoint_cloth_pool":
         return adjoint_main(args)
+    if args.workload == "mixed":
+        return mixed_main(args)
     from particlesolver_amd import System, scenes
     import numpy as np
 
