@@ -1784,17 +1784,20 @@ int adj_taped_substep(ps_system *s, float dt, size_t t)
     uint32_t *own_nbr = b.nbr, *own_cnt = b.ncnt;
     b.nbr = w.tN + t * w.tab;
     b.ncnt = w.tC + t * w.cap;
+    // the reorder writes the sorted start state, attributes and slot map
+    // into the tape slot as well
+    b.tape_x = w.tX + t * w.cap;
+    b.tape_a = w.tA + t * w.cap;
+    b.tape_s = w.tS + t * w.cap;
     int rc = ps_substep_begin(s, dt);
+    b.tape_x = nullptr;
+    b.tape_a = nullptr;
+    b.tape_s = nullptr;
     const size_t n = s->n, c = w.cap, I = s->prm.iterations;
     hipError_t e = hipSuccess;
     auto cp = [&](void *dst, const void *src, size_t bytes) {
         if (e == hipSuccess) e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, s->stream);
     };
-    if (!rc) {
-        cp(w.tX + t * c, b.X, sizeof(float4) * n);
-        cp(w.tA + t * c, b.A, sizeof(Attr) * n);
-        cp(w.tS + t * c, b.slot_of, sizeof(uint32_t) * n);
-    }
     const bool uni = uniform_fluid(s);
     for (uint32_t it = 0; !rc && e == hipSuccess && it < I; it++) {
         const size_t q = (t * I + it) * c;

@@ -154,6 +154,9 @@ struct DevBuffers {
                            // normal code (n_x+1)*9 + (n_y+1)*3 + (n_z+1) in .w
     float4 *tape_lq;       // lambda pass: S = sum_j a d (xyz), D (w)
     float4 *tape_pit;      // lambda pass: the iterate after it (x, y, z, lambda)
+    float4 *tape_x;        // reorder: the sorted start state, its attributes and
+    Attr *tape_a;          // (with track_slots) the slot map, also to tape
+    uint32_t *tape_s;
     float *tape_rho;       // lambda pass: density
     float4 *tape_f;        // solve: fluid sum f (xyz), contact count bits (w)
     // CPU-reference mode only

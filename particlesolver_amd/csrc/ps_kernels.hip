@@ -300,6 +300,12 @@ __global__ __launch_bounds__(kMaxBlock) void reorder_k(DevBuffers b, uint32_t n)
     if (b.sid) b.sid2[d] = b.sid[s];
     // a scatter by insertion index: skipped when no kernel reads it
     if (b.track_slots) b.slot_of[a.orig] = d;
+    // recording for the reverse pass: the same records into the tape slot
+    if (b.tape_x) {
+        b.tape_x[d] = xs;
+        b.tape_a[d] = a;
+        if (b.track_slots) b.tape_s[a.orig] = d;
+    }
 }
 
 
