@@ -1773,10 +1773,11 @@ int adj_substep(ps_system *s, float dt)
     return stage_end(s);
 }
 
-// One substep that also writes tape slot t: its neighbour table goes
-// straight into the slot; the sorted start state, attributes, slot map,
-// every iterate (with lambda), the clamp bits, the wall-friction records and
-// the final iterate are copied there.
+// One substep that also writes tape slot t: the kernels write straight into
+// the slot its neighbour table, the sorted start state, attributes and slot
+// map (reorder), every iterate with its lambda (the taping lambda pass), the
+// clamp bits, wall-friction records and fluid sums (the taping solve) and the
+// final iterate (the last solve).
 int adj_taped_substep(ps_system *s, float dt, size_t t)
 {
     ps_system::AdjWork &w = s->adj;
@@ -1793,11 +1794,8 @@ int adj_taped_substep(ps_system *s, float dt, size_t t)
     b.tape_x = nullptr;
     b.tape_a = nullptr;
     b.tape_s = nullptr;
-    const size_t n = s->n, c = w.cap, I = s->prm.iterations;
+    const size_t c = w.cap, I = s->prm.iterations;
     hipError_t e = hipSuccess;
-    auto cp = [&](void *dst, const void *src, size_t bytes) {
-        if (e == hipSuccess) e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, s->stream);
-    };
     const bool uni = uniform_fluid(s);
     for (uint32_t it = 0; !rc && e == hipSuccess && it < I; it++) {
         const size_t q = (t * I + it) * c;
@@ -1812,17 +1810,19 @@ int adj_taped_substep(ps_system *s, float dt, size_t t)
         b.clamp_rec = w.tClamp + q;
         b.wall_rec = w.tW + q;
         b.tape_f = w.tF + q;
+        // the last solve also writes the substep's final iterate to tape
+        b.tape_q = it + 1 == I ? w.tP + t * c : nullptr;
         if (e == hipSuccess)
             e = ps::launch_solve(s->stream, b, s->kc, s->n, features(s), uni, s->wu, s->it_sub,
                                  s->block[PS_K_SOLVE]);
         b.clamp_rec = nullptr;
         b.wall_rec = nullptr;
         b.tape_f = nullptr;
+        b.tape_q = nullptr;
         std::swap(b.P, b.Q);
         s->iters++;
         s->it_sub++;
     }
-    if (!rc) cp(w.tP + t * c, b.P, sizeof(float4) * n);
     if (!rc && e == hipSuccess) rc = stage_end(s);
     b.nbr = own_nbr;
     b.ncnt = own_cnt;

@@ -157,6 +157,7 @@ struct DevBuffers {
     float4 *tape_x;        // reorder: the sorted start state, its attributes and
     Attr *tape_a;          // (with track_slots) the slot map, also to tape
     uint32_t *tape_s;
+    float4 *tape_q;        // last solve of a taped substep: its result, also to tape
     float *tape_rho;       // lambda pass: density
     float4 *tape_f;        // solve: fluid sum f (xyz), contact count bits (w)
     // CPU-reference mode only

@@ -992,6 +992,7 @@ __global__ __launch_bounds__(kMaxBlock) void solve_k(DevBuffers b, KConst k, uin
         b.tape_f[i] = tape;
     }
     b.Q[i] = make_float4(nxp, nyp, nzp, 0.0f);
+    if (TAPE && b.tape_q) b.tape_q[i] = make_float4(nxp, nyp, nzp, 0.0f);
 }
 
 // ---------------------------------------------------------------------------
