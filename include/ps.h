@@ -88,6 +88,15 @@ typedef enum ps_error {
  * index << 3 | axis.  The passes count towards no particle's divisor.  In
  * this mode's scope (immovable solids) no contact enters the pre-pass. */
 #define PS_FLAG_STABILIZATION 0x10u
+/* The reference's tensile term, s_corr = -k_corr (W(r)/W(dq h))^n with k_corr
+ * taken as it is (lambdaCorr = -K_P pow(W/W(DQ_P H), E_P),
+ * gpu/src/cuda/integration_kernel.cuh:628-633; K_P in
+ * cpu/src/constraint/totalfluidconstraint.cpp:106).  Without it the 3D path
+ * scales k_corr by h^2, the units of lambda (a length^2), so that one
+ * dimensionless k works at any h: at h = 2 (the reference's H) the default
+ * form is 4x the reference's term, at h = 0.2 it is 1/25 of it.
+ * ps_params_default sets this flag. */
+#define PS_FLAG_UNSCALED_TENSILE 0x20u
 
 /* Status bits raised by the kernels, read with ps_get_status. */
 #define PS_STATUS_NEIGHBOR_OVERFLOW 0x1u /* a particle had more than max_neighbors
@@ -120,7 +129,8 @@ typedef struct ps_params {
     uint32_t grid[3];       /* cells per axis; positions outside clamp to the edge */
     uint32_t iterations;    /* solver iterations per substep                       */
     float relaxation;       /* lambda relaxation epsilon                           */
-    float k_corr;           /* tensile correction s_corr = -k h^2 (W(r)/W(dq h))^n */
+    float k_corr;           /* tensile correction s_corr = -k h^2 (W(r)/W(dq h))^n,
+                               -k (W(r)/W(dq h))^n with PS_FLAG_UNSCALED_TENSILE  */
     uint32_t n_corr;
     float dq_corr;          /* dq as a fraction of h                               */
     float xsph_c;           /* XSPH viscosity c (0 disables)                       */
@@ -146,7 +156,8 @@ typedef struct ps_derived {
     float w_zero;      /* poly6 * h^6 = W(0)                                      */
     float w_dq;        /* poly6 * (h^2 - (dq h)^2)^3 = W(dq h)                    */
     float inv_w_dq;    /* 1 / W(dq h)                                             */
-    float tensile_k;   /* k_corr * h^2: the tensile term in lambda's units (length^2) */
+    float tensile_k;   /* k_corr * h^2 (the tensile term in lambda's units, length^2),
+                          or k_corr with PS_FLAG_UNSCALED_TENSILE                   */
     float search_r2;   /* neighbour search radius^2 = max(h, collide_dist)^2      */
     float collide_r2;  /* collide_dist^2                                          */
     float wall_s;      /* sqrt(wall_mu_s) * r                                     */
@@ -317,6 +328,12 @@ int ps_read_markers(ps_system *sys, uint32_t emitter, uint32_t *count, float *xy
 int ps_add_distance_constraints(ps_system *sys, uint32_t m, const uint32_t *pairs,
                                 const float *rest, const float *stiffness);
 
+/* Distance constraints added so far (the reference keeps their count in
+ * its global constraint vectors, solver.cu:32-41); ps_step_backward writes
+ * this many dl_dstiffness entries.  A ps_add_distance_constraints call that
+ * fails adds none. */
+uint32_t ps_num_distance_constraints(const ps_system *sys);
+
 /* Replaces addPointConstraint (solver.cu:108-123): particle idx[k] is held
  * at xyz[3k..3k+2] after every iteration. */
 int ps_add_point_constraints(ps_system *sys, uint32_t m, const uint32_t *idx,
@@ -396,7 +413,9 @@ int ps_step_graph(ps_system *sys, float dt, uint32_t substeps);
  * CPU-reference mode; PS_ERR_CAPACITY if a neighbour table overflowed (the
  * transposed sums need every pair).  Device memory, kept by the handle:
  * (4 max_neighbors + 56 + 69 iterations) bytes per particle per taped
- * substep, plus 52 per particle per checkpoint.  Synchronous. */
+ * substep, plus 52 per particle per checkpoint.  Synchronous.  dl_dmu_k
+ * needs wall_mu_k > 0 (PS_ERR_INVALID_ARGUMENT otherwise: wall_k = sqrt(mu_k) r
+ * has no finite derivative at 0). */
 int ps_step_backward(ps_system *sys, float dt, uint32_t substeps, uint32_t checkpoint_every,
                      const float *dl_dx, const float *dl_dv, float *dl_dx0, float *dl_dv0,
                      float *dl_drho0, float *dl_dstiffness, float *dl_dmu_k);

@@ -13,13 +13,17 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 _libs = {}
 
 
-def _lib(prec):
-    if prec not in _libs:
-        path = os.path.join(HERE, "liboracle_%s.so" % prec)
+def _lib(prec, path=None):
+    """The oracle library of precision prec; `path` names another build of
+    the same sources (the mutation tests load deliberately broken copies)."""
+    key = (prec, path)
+    if key not in _libs:
+        if path is None:
+            path = os.path.join(HERE, "liboracle_%s.so" % prec)
         if not os.path.exists(path):
             raise RuntimeError("oracle library %s missing: run oracle/build.py" % path)
-        _libs[prec] = ctypes.CDLL(path)
-    return _libs[prec]
+        _libs[key] = ctypes.CDLL(path)
+    return _libs[key]
 
 
 def _ptr(a):
@@ -29,11 +33,11 @@ def _ptr(a):
 class Oracle:
     """CPU restatement of the substep, FP32 ('f32') or FP64 ('f64')."""
 
-    def __init__(self, params, prec="f32"):
+    def __init__(self, params, prec="f32", lib=None):
         self.prec = prec
         self.dtype = np.float32 if prec == "f32" else np.float64
         self.real = ctypes.c_float if prec == "f32" else ctypes.c_double
-        self.L = _lib(prec)
+        self.L = _lib(prec, lib)
         self.params = params.copy()
         s = self.prec
         self._f = lambda name: getattr(self.L, "%s_%s" % (name, s))

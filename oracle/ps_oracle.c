@@ -89,6 +89,8 @@ static real DIST2(real dx, real dy, real dz) { return FMA(dz, dz, FMA(dy, dy, dx
 
 #include "orc_exp.h"
 #define EXP orc_exp
+/* EXP exported for its accuracy test */
+__attribute__((visibility("default"))) real SFX(orc_exp)(real x) { return EXP(x); }
 
 #define ORC_EXPORT __attribute__((visibility("default")))
 
@@ -128,8 +130,10 @@ static void derive(const ps_params *p, orc_derived *d)
     d->radius = R(p->radius);
     d->relax = R(p->relaxation);
     /* s_corr is added to lambda (units length^2): the paper's dimensionless
-     * k measures lengths in units of h, so the term is k h^2 (...)^n */
-    d->k_corr = R((double)p->k_corr * h2);
+     * k measures lengths in units of h, so the term is k h^2 (...)^n; with
+     * PS_FLAG_UNSCALED_TENSILE k as it is, as the reference's
+     * lambdaCorr = -K_P pow(W/W(DQ_P H), E_P) (integration_kernel.cuh:628-633) */
+    d->k_corr = (p->flags & PS_FLAG_UNSCALED_TENSILE) ? R(p->k_corr) : R((double)p->k_corr * h2);
     d->xsph_c = R(p->xsph_c);
     d->s_solid = R(p->s_solid);
     d->mu_s = R(p->mu_s);

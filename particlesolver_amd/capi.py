@@ -40,6 +40,7 @@ PS_FLAG_SLEEPING = 0x2
 PS_FLAG_CPU_REFERENCE = 0x4
 PS_FLAG_STABILIZATION = 0x10
 PS_FLAG_NO_LDS_STAGING = 0x8
+PS_FLAG_UNSCALED_TENSILE = 0x20
 
 PS_STATUS_NEIGHBOR_OVERFLOW = 0x1
 PS_STATUS_NONFINITE = 0x2
@@ -152,6 +153,7 @@ PROTOTYPES = {
     "ps_trajectory_write": (_int, [_P, _P]),
     "ps_trajectory_close": (_int, [_P]),
     "ps_add_distance_constraints": (_int, [_P, _u32, _P, _P, _P]),
+    "ps_num_distance_constraints": (_u32, [_P]),
     "ps_add_point_constraints": (_int, [_P, _u32, _P, _P]),
     "ps_add_rigid_body": (_int, [_P, _u32, _P, _f32, ctypes.POINTER(_u32)]),
     "ps_add_rigid_bodies": (_int, [_P, _u32, _P, _P, _P, ctypes.POINTER(_u32)]),
@@ -414,7 +416,11 @@ class System:
         st = None if stiffness is None else np.ascontiguousarray(stiffness, dtype=np.float32)
         self._check(self.lib.ps_add_distance_constraints(self.h, pairs.shape[0], _ptr(pairs),
                                                          _ptr(rest), _ptr(st)))
-        self.n_dist = getattr(self, "n_dist", 0) + pairs.shape[0]
+
+    @property
+    def n_dist(self):
+        """distance constraints held by the system (ps_num_distance_constraints)"""
+        return self.lib.ps_num_distance_constraints(self.h)
 
     def add_point_constraints(self, idx, xyz):
         idx = np.ascontiguousarray(idx, dtype=np.uint32).reshape(-1)
@@ -462,13 +468,14 @@ class System:
         dx = np.empty((n, 3), np.float32)
         dv = np.empty((n, 3), np.float32)
         dr = np.empty(n, np.float32)
-        nd = getattr(self, "n_dist", 0)
+        nd = self.lib.ps_num_distance_constraints(self.h)
         ds = np.empty(max(nd, 1), np.float32)
-        dm = np.zeros(1, np.float32)
+        # dL/dwall_mu_k is None where wall_mu_k = 0 (no finite derivative there)
+        dm = np.zeros(1, np.float32) if self.params.wall_mu_k > 0.0 else None
         self._check(self.lib.ps_step_backward(self.h, dt, substeps, checkpoint_every, _ptr(gx),
                                               _ptr(gv), _ptr(dx), _ptr(dv), _ptr(dr), _ptr(ds),
                                               _ptr(dm)))
-        return dx, dv, dr, ds[:nd], float(dm[0])
+        return dx, dv, dr, ds[:nd], None if dm is None else float(dm[0])
 
     def begin(self, dt):
         self._check(self.lib.ps_substep_begin(self.h, dt))

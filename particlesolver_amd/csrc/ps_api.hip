@@ -177,8 +177,9 @@ void derive_constants(const ps_params &p, ps_derived &d)
     d.w_dq = (float)(poly6 * tq * tq * tq);
     d.inv_w_dq = (float)(1.0 / (poly6 * tq * tq * tq));
     // The tensile term is added to lambda (units length^2); the paper's
-    // dimensionless k measures lengths in units of h, hence k h^2.
-    d.tensile_k = (float)((double)p.k_corr * h2);
+    // dimensionless k measures lengths in units of h, hence k h^2.  The
+    // reference takes K_P as it is (integration_kernel.cuh:633).
+    d.tensile_k = (p.flags & PS_FLAG_UNSCALED_TENSILE) ? p.k_corr : (float)((double)p.k_corr * h2);
     d.search_r2 = (float)(sr * sr);
     d.collide_r2 = (float)((double)p.collide_dist * (double)p.collide_dist);
     d.wall_s = (float)(sqrt((double)p.wall_mu_s) * (double)p.radius);
@@ -749,6 +750,7 @@ int upload_constraints(ps_system *s)
         dev_free(b.dist_adj);
         dev_free(b.dist_pairs);
         dev_free(b.dist_rs);
+        s->dist_cap = 0;
         int rc;
         if ((rc = dev_alloc(s, &b.dist_adj, 2 * nc))) return rc;
         if ((rc = dev_alloc(s, &b.dist_pairs, nc))) return rc;
@@ -788,6 +790,7 @@ int upload_pins(ps_system *s)
     const size_t np = s->pin_xyz.size() / 3;
     if (np > s->pin_cap) {
         dev_free(b.pin_xyz);
+        s->pin_cap = 0;
         int rc = dev_alloc(s, &b.pin_xyz, np * 2);
         if (rc) return rc;
         s->pin_cap = np * 2;
@@ -879,7 +882,7 @@ int ps_params_default(ps_params *p)
     p->max_particles = 15000;
     p->max_neighbors = 128;
     p->seed = 1234;         // integration.cu:47
-    p->flags = 0;
+    p->flags = PS_FLAG_UNSCALED_TENSILE;  // lambdaCorr = -K_P (W/W(dq H))^E_P (:628-633)
     return PS_OK;
 }
 
@@ -1380,6 +1383,10 @@ int ps_add_distance_constraints(ps_system *s, uint32_t m, const uint32_t *pairs,
         if (stiffness && !(stiffness[k] > 0.0f && stiffness[k] <= 1.0f))
             return fail(s, PS_ERR_INVALID_ARGUMENT, "stiffness must be in (0, 1]");
     }
+    HIP_TRY(s, hipStreamSynchronize(s->stream));
+    drop_graph(s);
+    if (int rc = enable_slot_tracking(s)) return rc;
+    const size_t m0 = s->dist_rest.size();
     for (uint32_t k = 0; k < m; k++) {
         s->dist_pairs.push_back(pairs[2 * k]);
         s->dist_pairs.push_back(pairs[2 * k + 1]);
@@ -1388,10 +1395,28 @@ int ps_add_distance_constraints(ps_system *s, uint32_t m, const uint32_t *pairs,
         s->occ[pairs[2 * k]]++;
         s->occ[pairs[2 * k + 1]]++;
     }
-    HIP_TRY(s, hipStreamSynchronize(s->stream));
-    drop_graph(s);
-    if (int rc = enable_slot_tracking(s)) return rc;
-    return upload_constraints(s);
+    const int rc = upload_constraints(s);
+    if (rc) {
+        // keep the host sets equal to what the device last received: undo
+        // this call's constraints and re-upload the earlier ones (best effort;
+        // the first error is the one reported)
+        for (size_t c = m0; c < s->dist_rest.size(); c++) {
+            s->occ[s->dist_pairs[2 * c]]--;
+            s->occ[s->dist_pairs[2 * c + 1]]--;
+        }
+        s->dist_pairs.resize(2 * m0);
+        s->dist_rest.resize(m0);
+        s->dist_stiff.resize(m0);
+        const std::string why = s->err;
+        (void)upload_constraints(s);
+        s->err = why;
+    }
+    return rc;
+}
+
+uint32_t ps_num_distance_constraints(const ps_system *s)
+{
+    return s ? (uint32_t)s->dist_rest.size() : 0u;
 }
 
 int ps_add_point_constraints(ps_system *s, uint32_t m, const uint32_t *idx, const float *xyz)
@@ -1406,17 +1431,33 @@ int ps_add_point_constraints(ps_system *s, uint32_t m, const uint32_t *idx, cons
         for (int a = 0; a < 3; a++)
             if (!is_finite(xyz[3 * k + a])) return fail(s, PS_ERR_NOT_FINITE, "non-finite pin");
     }
+    HIP_TRY(s, hipStreamSynchronize(s->stream));
+    drop_graph(s);
+    const size_t np0 = s->pin_xyz.size();
+    std::vector<std::pair<uint32_t, uint32_t>> prev;  // (particle, its pin before this call)
     for (uint32_t k = 0; k < m; k++) {
         const uint32_t pid = (uint32_t)(s->pin_xyz.size() / 3);
         s->pin_xyz.push_back(xyz[3 * k]);
         s->pin_xyz.push_back(xyz[3 * k + 1]);
         s->pin_xyz.push_back(xyz[3 * k + 2]);
+        prev.emplace_back(idx[k], s->pin_of[idx[k]]);
         s->pin_of[idx[k]] = pid + 1;
         s->occ[idx[k]]++;
     }
-    HIP_TRY(s, hipStreamSynchronize(s->stream));
-    drop_graph(s);
-    return upload_pins(s);
+    const int rc = upload_pins(s);
+    if (rc) {
+        // undo this call's pins (in reverse, so a particle pinned twice gets
+        // its first value back) and re-upload the earlier ones (best effort)
+        for (size_t k = prev.size(); k-- > 0;) {
+            s->pin_of[prev[k].first] = prev[k].second;
+            s->occ[prev[k].first]--;
+        }
+        s->pin_xyz.resize(np0);
+        const std::string why = s->err;
+        (void)upload_pins(s);
+        s->err = why;
+    }
+    return rc;
 }
 
 int ps_add_rigid_bodies(ps_system *s, uint32_t nbody, const uint32_t *offsets,
@@ -2008,6 +2049,9 @@ int ps_step_backward(ps_system *s, float dt, uint32_t substeps, uint32_t checkpo
     if (s->prm.flags & (PS_FLAG_SLEEPING | PS_FLAG_REBUILD_EVERY_ITERATION))
         return fail(s, PS_ERR_UNSUPPORTED,
                     "ps_step_backward needs one neighbour table per substep and no sleeping");
+    if (dl_dmu_k && !(s->prm.wall_mu_k > 0.0f))
+        return fail(s, PS_ERR_INVALID_ARGUMENT,
+                    "dl_dmu_k needs wall_mu_k > 0 (d sqrt(mu_k)/d mu_k is unbounded at 0)");
     const uint32_t n = s->n;
     const float *in[2] = {dl_dx, dl_dv};
     for (const float *p : in)

@@ -185,7 +185,7 @@ def cloth_constraints(nx, nz, first, spacing, stiffness=(1.0, 0.5, 0.1)):
 
 
 def mixed(cloth_n=256, cubes=(25, 5, 25), fluid=(100, 50, 50), spacing=0.1, box=(30.0, 20.0, 30.0),
-          iterations=4, max_neighbors=96, seed=0x5EED0003, dt=1.0 / 60.0):
+          iterations=4, max_neighbors=256, seed=0x5EED0003, dt=1.0 / 60.0):
     """3D mixed-phase scene: a cloth_n^2 cloth (structural/shear/bend links,
     stiffness 1.0/0.5/0.1, corners pinned) above prod(cubes) rigid 4x4x4
     cubes (shape matching, mu_s 0.6 / mu_k 0.4) dropped from a seeded

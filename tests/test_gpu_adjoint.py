@@ -138,3 +138,26 @@ def test_rejected_scopes(gpu_available):
         with pytest.raises(capi.PsError) as e:
             g.step_backward(sc.dt, 1)
         assert e.value.code == capi.PS_ERR_CAPACITY
+
+
+def test_mu_k_gradient_needs_positive_mu_k_and_counts_come_from_the_library(gpu_available):
+    sc = scenes.rope_over_fluid()
+    p = sc.params.copy()
+    p.wall_mu_k = 0.0
+    gx, gv = cotangents(sc.n)
+    with System(p) as g:
+        sc.load_into(g)
+        nd = g.n_dist
+        assert nd == n_dist(sc) > 0
+        # a refused add leaves the count unchanged
+        with pytest.raises(capi.PsError):
+            g.add_distance_constraints([[0, 0]], [0.1])
+        assert g.n_dist == nd
+        # dL/dwall_mu_k has no finite value at wall_mu_k = 0: the C-ABI refuses
+        # the request, the binding leaves it out
+        dm = np.zeros(1, np.float32)
+        rc = g.lib.ps_step_backward(g.h, sc.dt, 1, 0, None, None, None, None, None, None,
+                                    dm.ctypes.data_as(capi.ctypes.c_void_p))
+        assert rc == capi.PS_ERR_INVALID_ARGUMENT
+        dx, dv, dr, ds, dmu = g.step_backward(sc.dt, 2, gx, gv)
+        assert dmu is None and ds.shape == (nd,) and np.isfinite(dx).all()

@@ -59,10 +59,22 @@ def test_mixed_trajectory_bitexact(gpu_available):
     assert np.isfinite(x).all()
 
 
-def test_mixed_full_size_first_substeps_bitexact(gpu_available):
-    sc = scenes.mixed()                 # 765,536 particles, 390,658 links, 3,125 bodies
-    assert sc.n == 765536
-    run_both(sc, 2)
+def test_mixed_full_size_bitexact(gpu_available):
+    # 765,536 particles, 390,658 links, 3,125 bodies: 30 substeps bit-exact,
+    # and the scene's neighbour table (256) holds every particle's candidates
+    sc = scenes.mixed()
+    assert sc.n == 765536 and sc.params.max_neighbors == 256
+    ref = load_scene(sc, "f32")
+    with System(sc.params) as g:
+        sc.load_into(g)
+        for done in (10, 20, 30):
+            g.step(sc.dt, 10)
+            ref.step(sc.dt, 10)
+            assert_bitexact(g.positions(), ref.positions(), "positions after %d substeps" % done)
+            assert_bitexact(g.velocities(), ref.velocities(), "velocities after %d substeps" % done)
+        gs, rs = g.status(), ref.status()
+        assert gs.flags == rs.flags and gs.max_neighbors == rs.max_neighbors
+        assert not gs.flags & capi.PS_STATUS_NEIGHBOR_OVERFLOW, gs.max_neighbors
 
 
 def test_two_d_fluid_bitexact(gpu_available):

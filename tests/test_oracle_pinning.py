@@ -169,8 +169,9 @@ def test_predict_closed_form():
     assert np.array_equal(it[idx[1], :3], x[1].astype(np.float64))   # static: w = 0
 
 
-def two_particle(r, h=0.2, rho0=1009.775, k=0.1, eps=600.0):
+def two_particle(r, h=0.2, rho0=1009.775, k=0.1, eps=600.0, flags=0):
     p = iso_params(2, h=h, k=k, prec_relax=eps)
+    p.flags |= flags
     o = Oracle(p, "f64")
     x = np.array([[4.0, 4.0, 4.0], [4.0 + r, 4.0, 4.0]], np.float32)
     o.add_particles(x, None, np.ones(2, np.float32), np.zeros(2, np.int32), np.full(2, rho0, np.float32))
@@ -208,6 +209,45 @@ def test_delta_p_two_particles_closed_form(r):
     assert np.allclose(it[0, 0] - x[0, 0], dp0, rtol=2e-6, atol=1e-15)
     assert np.allclose(it[1, 0] - x[1, 0], -dp0, rtol=2e-6, atol=1e-15)
     assert np.allclose(it[:, 1:3], x[:, 1:3], atol=0)
+
+
+@pytest.mark.parametrize("h,r", [(0.2, 0.07), (2.0, 0.9), (2.0, 1.5)])
+def test_delta_p_two_particles_unscaled_tensile(h, r):
+    """PS_FLAG_UNSCALED_TENSILE: s_corr = -k (W(r)/W(dq h))^n with k as
+    given, the form of Macklin and Mueller 2013 (eq. 13) and of the
+    reference's lambdaCorr = -K_P pow(W/W(DQ_P H), E_P)
+    (integration_kernel.cuh:628-633), at h = 2, the reference's H; the
+    default form is k h^2 (...)^n."""
+    from particlesolver_amd import capi
+    rho0, eps, k = np.float64(np.float32(1009.775 if h < 1 else 0.02)), 600.0, 0.1
+    p, o, x = two_particle(r, h, rho0, k, eps, flags=capi.PS_FLAG_UNSCALED_TENSILE)
+    lam = o.compute_lambda()
+    o.iterate()
+    it, _ = o.sorted_state()
+    rr = x[1, 0] - x[0, 0]
+    W = textbook_poly6(rr * rr, h)
+    Wdq = textbook_poly6((0.2 * h) ** 2, h)
+    s_corr = -k * (W / Wdq) ** 4
+    gradW = textbook_spiky_grad_mag(rr, h)
+    dp0 = (lam[0] + lam[1] + s_corr) * gradW / rho0
+    assert np.allclose(it[0, 0] - x[0, 0], dp0, rtol=2e-6, atol=1e-15)
+    assert np.allclose(it[1, 0] - x[1, 0], -dp0, rtol=2e-6, atol=1e-15)
+    # the derived constant: k itself, against k h^2 without the flag
+    d_ref = Oracle(p, "f64").derived()
+    p2 = p.copy()
+    p2.flags &= ~capi.PS_FLAG_UNSCALED_TENSILE
+    d_def = Oracle(p2, "f64").derived()
+    assert d_ref["tensile_k"] == np.float64(np.float32(k))
+    assert abs(d_def["tensile_k"] - np.float64(np.float32(k)) * np.float64(np.float32(h)) ** 2) < 1e-15
+
+
+def test_library_defaults_use_the_reference_tensile_form():
+    from particlesolver_amd import capi, derive
+    p = capi.default_params()
+    assert p.flags & capi.PS_FLAG_UNSCALED_TENSILE
+    d = derive(p)
+    assert np.float32(d.tensile_k) == np.float32(p.k_corr)
+    assert np.float32(Oracle(p, "f32").derived()["tensile_k"]) == np.float32(d.tensile_k)
 
 
 def test_rest_lattice_interior_lambda_vanishes():
