@@ -164,7 +164,8 @@ typedef struct ps_derived {
     float wall_k;      /* sqrt(wall_mu_k) * r                                     */
     float jitter;      /* wall_jitter * r                                         */
     uint32_t num_cells;
-    uint32_t reserved[3];
+    float w_ratio;     /* 1 / (h^2 - (dq h)^2)^3, so W(r)/W(dq h) = w_ratio (h^2 - r^2)^3 */
+    uint32_t reserved[2];
 } ps_derived;
 
 typedef struct ps_status {
@@ -517,6 +518,42 @@ int ps_timing_read(ps_system *sys, int kernel_id, double *total_ms, uint64_t *la
  * period coprime with the iteration count samples every iteration alike. */
 int ps_timing_sample(ps_system *sys, uint32_t period);
 const char *ps_kernel_name(int kernel_id);
+
+/* ---- stateless grid sort over caller device arrays --------------------- */
+
+/* The grid of a stateless sort: the grid fields of the reference's SimParams
+ * (worldOrigin, cellSize, gridSize; kernel.cuh:9-22). */
+typedef struct ps_grid_desc {
+    float origin[3];
+    float cell;        /* edge of a cubic cell, > 0                          */
+    uint32_t grid[3];  /* cells per axis, each > 0, product < 2^31           */
+    uint32_t reserved;
+} ps_grid_desc;
+
+/* Hash, stable sort and cell bounds of n positions, all in device memory,
+ * in one call: the reference's calcHash + sortParticles +
+ * reorderDataAndFindCellStart sequence (wrappers.cuh:30-46; integration.cu:
+ * 161-275) over the caller's arrays, with the cells and order of ps_step:
+ *   pos          n records (x, y, z, w) of float4, the reference's layout;
+ *   cell_key     out, n: the cell of each sorted slot, (z gy + y) gx + x with
+ *                each coordinate floor((p - origin) / cell) clamped into
+ *                the grid (the reference wraps, calcGridHash :197-205);
+ *   sorted_index out, n: the input index at each sorted slot, ordered by
+ *                (cell, input index) (a stable sort of the keys);
+ *   cell_start, cell_end  out, one per cell: sorted slots [start, end) of the
+ *                cell, both 0xffffffff for an empty cell (the reference sets
+ *                cellStart so and leaves cellEnd unset, integration.cu:198);
+ *   sorted_pos   out, n float4 records in sorted order, or NULL.
+ * stream is a hipStream_t (NULL: the null stream); the call returns after
+ * enqueueing its work.  Its workspace (12 n + 8 cells bytes and a little)
+ * is allocated and freed on the stream for each call, as the reference's
+ * reorderDataAndFindCellStart allocates a copy of the positions per call
+ * (integration.cu:216-220).  PS_ERR_INVALID_ARGUMENT for a null pointer
+ * (sorted_pos aside) or a bad grid; PS_ERR_OUT_OF_MEMORY if the workspace
+ * cannot be had. */
+int ps_grid_sort(const ps_grid_desc *grid, uint32_t n, const float *pos, uint32_t *cell_key,
+                 uint32_t *sorted_index, uint32_t *cell_start, uint32_t *cell_end,
+                 float *sorted_pos, void *stream);
 
 /* ---- autotuning -------------------------------------------------------- */
 

@@ -55,26 +55,32 @@ typedef float real;
 static double RSQ(double x) { return 1.0 / sqrt(x); }
 #else
 #define FMA fmaf
-/* 1/sqrt(x), x > 0 normal: the integer seed 0x5f375a86 - (bits(x) >> 1)
+/* 1/sqrt(x), x > 0 normal: the integer seed y0 = 0x5f375a86 - (bits(x) >> 1)
  * of the "fast inverse square root" (constant from C. Lomont, Fast Inverse
- * Square Root, 2003) and three Newton steps y <- y (3/2 - (x/2) y^2), each
- * as t = (x/2) y; u = fma(-t, y, 3/2); y = y u.  Relative error about 1e-7
- * (the seed's 3.4e-2 squares at every step).  Only IEEE operations, so the
- * kernels' copy of these steps gives the same float. */
+ * Square Root, 2003; relative error up to 3.4e-2), then one fourth-order
+ * correction.  With e = 1 - x y0^2, 1/sqrt(x) = y0 (1 - e)^(-1/2) =
+ * y0 (1 + e p(e)), p the binomial series 1/2 + 3/8 e + 5/16 e^2 +
+ * 35/128 e^3 + ..., truncated after e^3, its four coefficients re-fitted
+ * minimax over the seed's range of e (|e| < 0.07); y = fma(y0 e, p(e), y0),
+ * p by Horner in FMA.  Relative error below 1e-7 over [1e-8, 16].  Only
+ * IEEE operations, so the kernels' copy gives the same float. */
+static float f32_of_bits(uint32_t u)
+{
+    float f;
+    memcpy(&f, &u, sizeof(f));
+    return f;
+}
 static float RSQ(float x)
 {
     uint32_t i;
     memcpy(&i, &x, sizeof(i));
-    i = 0x5f375a86u - (i >> 1);
-    float y;
-    memcpy(&y, &i, sizeof(y));
-    const float hx = 0.5f * x;
-    for (int k = 0; k < 3; k++) {
-        const float t = hx * y;
-        const float u = fmaf(-t, y, 1.5f);
-        y = y * u;
-    }
-    return y;
+    const float y0 = f32_of_bits(0x5f375a86u - (i >> 1));
+    const float xy = x * y0;
+    const float e = fmaf(-xy, y0, 1.0f);
+    float p = fmaf(f32_of_bits(0x3e8c5420u), e, f32_of_bits(0x3ea0bc22u));
+    p = fmaf(p, e, f32_of_bits(0x3ec0000cu));
+    p = fmaf(p, e, f32_of_bits(0x3effffc8u));
+    return fmaf(y0 * e, p, y0);
 }
 #endif
 /* RSQ exported for its accuracy test */
@@ -99,6 +105,7 @@ __attribute__((visibility("default"))) real SFX(orc_exp)(real x) { return EXP(x)
 
 typedef struct orc_derived {
     real h, h2, poly6, spiky, w_zero, w_dq, inv_w_dq, search_r2, collide_r2, collide_dist;
+    real w_ratio; /* 1/(h^2 - (dq h)^2)^3: W(r)/W(dq h) = w_ratio (h^2 - r^2)^3 */
     real wall_s, wall_k, jitter, radius, relax, k_corr, xsph_c, s_solid, mu_s, mu_k;
     uint32_t n_corr, num_cells;
 } orc_derived;
@@ -121,6 +128,7 @@ static void derive(const ps_params *p, orc_derived *d)
     d->w_zero = R(poly6 * h6);
     d->w_dq = R(poly6 * tq * tq * tq);
     d->inv_w_dq = R(1.0 / (poly6 * tq * tq * tq));
+    d->w_ratio = R(1.0 / (tq * tq * tq));
     d->search_r2 = R(sr * sr);
     d->collide_r2 = R((double)p->collide_dist * (double)p->collide_dist);
     d->collide_dist = R(p->collide_dist);
@@ -143,15 +151,15 @@ static void derive(const ps_params *p, orc_derived *d)
 }
 
 /* Derived constants (FP64 values when ORC_DOUBLE) for pinning tests:
- * out[0..11] = h2, poly6, spiky, w_zero, w_dq, inv_w_dq, tensile_k,
- * search_r2, collide_r2, wall_s, wall_k, jitter. */
+ * out[0..12] = h2, poly6, spiky, w_zero, w_dq, inv_w_dq, tensile_k,
+ * search_r2, collide_r2, wall_s, wall_k, jitter, w_ratio. */
 ORC_EXPORT void SFX(orc_derive)(const ps_params *p, real *out)
 {
     orc_derived d;
     derive(p, &d);
     out[0] = d.h2; out[1] = d.poly6; out[2] = d.spiky; out[3] = d.w_zero; out[4] = d.w_dq;
     out[5] = d.inv_w_dq; out[6] = d.k_corr; out[7] = d.search_r2; out[8] = d.collide_r2;
-    out[9] = d.wall_s; out[10] = d.wall_k; out[11] = d.jitter;
+    out[9] = d.wall_s; out[10] = d.wall_k; out[11] = d.jitter; out[12] = d.w_ratio;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -602,14 +610,59 @@ ORC_EXPORT int SFX(orc_begin)(orc_sys *s, real dt)
  * TotalFluidConstraint::project first loop, totalfluidconstraint.cpp:41-90).
  *   rho_i = W(0)/w_i + sum_j s_j W(r_ij)/w_j   (s_j = 1 fluid, s_solid else)
  *   grad_j C_i = a d,  a = (spiky / rho0_i) (h-r)^2 / r,   d = p_i - p_j
- *   lambda_i = -(rho_i/rho0_i - 1) / (sum_j a^2 r^2 + |sum_j grad_j|^2 + eps)
- * with 1/r = RSQ(r^2), r = r^2 (1/r); degenerate pairs (r^2 < 1e-8) add
- * density only.  Static neighbours (w_j = 0) are skipped (cpu
+ *   lambda_i = -(rho_i/rho0_i - 1) / (sum_j |grad_j C_i|^2 + |sum_j grad_j C_i|^2 + eps)
+ * Per pair, with t = h^2 - r^2, 1/r = RSQ(r^2) and q = h - r =
+ * FMA(-r^2, 1/r, h): the density term t^3 m_j, the gradient term q^2 (1/r) d
+ * and |grad_j C_i|^2 as q^4 (times sk^2); poly6 and sk = spiky/rho0_i
+ * multiply the finished sums.  Degenerate pairs (r^2 < 1e-8) add density
+ * only.  Static neighbours (w_j = 0) are skipped (cpu
  * totalfluidconstraint.cpp:122). */
-static void lambda_pass(orc_sys *s)
+
+/* The sums of particle i (fluid) at iterate P: density rho, S = sum_j
+ * grad_j C_i and D = sum_j |grad_j C_i|^2 + |S|^2.  Two halves of the
+ * table, [0] over its chunks of four entries 0, 2, 4, ... and [1] over
+ * chunks 1, 3, 5, ..., each in table order, added as [0] + [1] (HALF(k)). */
+static void lambda_sums(const orc_sys *s, const real *P, uint32_t i, real *rho_out, real *S, real *D)
 {
     const orc_derived *d = &s->d;
     const uint32_t kmax = s->kmax;
+    real px = P[4 * i], py = P[4 * i + 1], pz = P[4 * i + 2];
+    real sk = d->spiky * (R(1) / s->rho0[i]);
+    real rho[2] = {R(0), R(0)};
+    real gix[2] = {R(0), R(0)}, giy[2] = {R(0), R(0)}, giz[2] = {R(0), R(0)}, den[2] = {R(0), R(0)};
+    for (uint32_t k = 0; k < s->ncnt[i]; k++) {
+        uint32_t j = s->nbr[(size_t)i * kmax + k];
+        const int hf = HALF(k);
+        real wj = s->w[j];
+        if (wj == R(0)) continue;
+        real dx = px - P[4 * j], dy = py - P[4 * j + 1], dz = pz - P[4 * j + 2];
+        real r2 = DIST2(dx, dy, dz);
+        if (!(r2 < d->h2)) continue;
+        real t = d->h2 - r2;
+        real m = R(1) / wj;
+        if (s->phase[j] != PS_PHASE_FLUID) m = m * d->s_solid;
+        rho[hf] = FMA((t * t) * t, m, rho[hf]);
+        if (r2 < DEGENERATE_R2) continue;
+        real ri = RSQ(r2);
+        real q = FMA(-r2, ri, d->h);
+        real q2 = q * q;
+        real a = q2 * ri;
+        den[hf] = FMA(q2, q2, den[hf]);
+        gix[hf] = FMA(a, dx, gix[hf]);
+        giy[hf] = FMA(a, dy, giy[hf]);
+        giz[hf] = FMA(a, dz, giz[hf]);
+    }
+    real gx = sk * (gix[0] + gix[1]), gy = sk * (giy[0] + giy[1]), gz = sk * (giz[0] + giz[1]);
+    *rho_out = FMA(d->poly6, rho[0] + rho[1], d->w_zero * (R(1) / s->w[i]));
+    S[0] = gx;
+    S[1] = gy;
+    S[2] = gz;
+    *D = FMA(sk * sk, den[0] + den[1], DIST2(gx, gy, gz));
+}
+
+static void lambda_pass(orc_sys *s)
+{
+    const orc_derived *d = &s->d;
 #pragma omp parallel for schedule(static)
     for (int64_t ii = 0; ii < (int64_t)s->n; ii++) {
         uint32_t i = (uint32_t)ii;
@@ -617,41 +670,10 @@ static void lambda_pass(orc_sys *s)
             s->P[4 * i + 3] = R(0);
             continue;
         }
-        real px = s->P[4 * i], py = s->P[4 * i + 1], pz = s->P[4 * i + 2];
-        real r0 = s->rho0[i];
-        real sk = d->spiky * (R(1) / r0);
-        /* two halves of the table, [0] over its chunks of four entries
-         * 0, 2, 4, ... and [1] over chunks 1, 3, 5, ..., each in table order,
-         * added as [0] + [1] (HALF(k)); half 0 holds the self term */
-        real rho[2] = {d->w_zero * (R(1) / s->w[i]), R(0)};
-        real gix[2] = {R(0), R(0)}, giy[2] = {R(0), R(0)}, giz[2] = {R(0), R(0)}, den[2] = {R(0), R(0)};
-        for (uint32_t k = 0; k < s->ncnt[i]; k++) {
-            uint32_t j = s->nbr[(size_t)i * kmax + k];
-            const int hf = HALF(k);
-            real wj = s->w[j];
-            if (wj == R(0)) continue;
-            real dx = px - s->P[4 * j], dy = py - s->P[4 * j + 1], dz = pz - s->P[4 * j + 2];
-            real r2 = DIST2(dx, dy, dz);
-            if (!(r2 < d->h2)) continue;
-            real t = d->h2 - r2;
-            real W = d->poly6 * ((t * t) * t);
-            real m = R(1) / wj;
-            if (s->phase[j] != PS_PHASE_FLUID) m = m * d->s_solid;
-            rho[hf] = FMA(W, m, rho[hf]);
-            if (r2 < DEGENERATE_R2) continue;
-            real ri = RSQ(r2);
-            real q = d->h - (r2 * ri);
-            real a = (sk * (q * q)) * ri;
-            den[hf] = FMA(a * a, r2, den[hf]);
-            gix[hf] = FMA(a, dx, gix[hf]);
-            giy[hf] = FMA(a, dy, giy[hf]);
-            giz[hf] = FMA(a, dz, giz[hf]);
-        }
-        real gx = gix[0] + gix[1], gy = giy[0] + giy[1], gz = giz[0] + giz[1];
-        real dn = (den[0] + den[1]) + DIST2(gx, gy, gz);
-        real rh = rho[0] + rho[1];
-        real C = (rh / r0) - R(1);
-        s->P[4 * i + 3] = -C / (dn + d->relax);
+        real rho, S[3], D;
+        lambda_sums(s, s->P, i, &rho, S, &D);
+        real C = (rho / s->rho0[i]) - R(1);
+        s->P[4 * i + 3] = -C / (D + d->relax);
     }
 }
 
@@ -669,6 +691,52 @@ static real ipow(real x, uint32_t n)
         if (n) b = b * b;
     }
     return r;
+}
+
+/* a6: the fluid sum of particle i at iterate P (lambda in P.w)
+ * (solveFluidsD, integration_kernel.cuh:596-642; cpu
+ * totalfluidconstraint.cpp:92-115):
+ *   f_i = sum_j (l_i + l_j + s_corr) gradW_ij,   (delta-p_i = f_i / rho0_i)
+ *   gradW_ij = -spiky (h-r)^2 d/r,  s_corr = -k (W(r)/W(dq h))^n  (k: k h^2,
+ *   or k with PS_FLAG_UNSCALED_TENSILE);
+ *   r^2 < 1e-8: gradW = (0, -0.001 spiky h^2, 0)   (:625-626).
+ * Per pair, with t = h^2 - r^2, 1/r = RSQ(r^2) and q = h - r =
+ * FMA(-r^2, 1/r, h): W(r)/W(dq h) = w_ratio t^3, coef = FMA(-k,
+ * (W/W(dq h))^n, l_i + l_j), the term -coef q^2 (1/r) d (a degenerate
+ * pair: -coef 0.001 h^2 in y); spiky multiplies the finished sum.  In two
+ * halves of the table, as lambda_sums.  Static neighbours are skipped. */
+static void fluid_sum(const orc_sys *s, const real *P, uint32_t i, real *f)
+{
+    const orc_derived *d = &s->d;
+    const uint32_t kmax = s->kmax;
+    real px = P[4 * i], py = P[4 * i + 1], pz = P[4 * i + 2];
+    real li = P[4 * i + 3];
+    real gdeg = (d->h * d->h) * R(0.001);
+    real hx[2] = {R(0), R(0)}, hy[2] = {R(0), R(0)}, hz[2] = {R(0), R(0)};
+    for (uint32_t k = 0; k < s->ncnt[i]; k++) {
+        uint32_t j = s->nbr[(size_t)i * kmax + k];
+        const int hf = HALF(k);
+        if (s->w[j] == R(0)) continue;
+        real dx = px - P[4 * j], dy = py - P[4 * j + 1], dz = pz - P[4 * j + 2];
+        real r2 = DIST2(dx, dy, dz);
+        if (!(r2 < d->h2)) continue;
+        real t = d->h2 - r2;
+        real x = d->w_ratio * ((t * t) * t);
+        real coef = FMA(-d->k_corr, ipow(x, d->n_corr), li + P[4 * j + 3]);
+        if (r2 < DEGENERATE_R2) {
+            hy[hf] = hy[hf] - (coef * gdeg);
+        } else {
+            real ri = RSQ(r2);
+            real q = FMA(-r2, ri, d->h);
+            real g = coef * ((q * q) * ri);
+            hx[hf] = FMA(-g, dx, hx[hf]);
+            hy[hf] = FMA(-g, dy, hy[hf]);
+            hz[hf] = FMA(-g, dz, hz[hf]);
+        }
+    }
+    f[0] = d->spiky * (hx[0] + hx[1]);
+    f[1] = d->spiky * (hy[0] + hy[1]);
+    f[2] = d->spiky * (hz[0] + hz[1]);
 }
 
 /* wall push-out + wall friction for one particle (collide_world_functor,
@@ -980,42 +1048,14 @@ static void solve_pass(orc_sys *s)
             }
         }
 
-        /* a6: delta-p with tensile correction (solveFluidsD,
-         * integration_kernel.cuh:596-642; cpu totalfluidconstraint.cpp:92-115)
-         *   dp_i = (1/rho0_i) sum_j (l_i + l_j + s_corr) gradW_ij,
-         *   gradW_ij = -spiky (h-r)^2 d/r,  s_corr = -k h^2 (W(r)/W(dq h))^n,
-         *   1/r = RSQ(r^2), r = r^2 (1/r);
-         *   r^2 < 1e-8: gradW = (0, -0.001 spiky h^2, 0)   (:625-626) */
+        /* a6: delta-p with tensile correction: f_i / rho0_i (fluid_sum) */
         if (ph == PS_PHASE_FLUID) {
-            real li = s->P[4 * i + 3], ir0 = R(1) / s->rho0[i];
-            /* in two halves of the table, as the lambda pass */
-            real hx[2] = {R(0), R(0)}, hy[2] = {R(0), R(0)}, hz[2] = {R(0), R(0)};
-            for (uint32_t k = 0; k < s->ncnt[i]; k++) {
-                uint32_t j = s->nbr[(size_t)i * kmax + k];
-                const int hf = HALF(k);
-                if (s->w[j] == R(0)) continue;
-                real dx = px - s->P[4 * j], dy = py - s->P[4 * j + 1], dz = pz - s->P[4 * j + 2];
-                real r2 = DIST2(dx, dy, dz);
-                if (!(r2 < d->h2)) continue;
-                real t = d->h2 - r2;
-                real W = d->poly6 * ((t * t) * t);
-                real corr = -(d->k_corr * ipow(W * d->inv_w_dq, d->n_corr));
-                real coef = (li + s->P[4 * j + 3]) + corr;
-                if (r2 < DEGENERATE_R2) {
-                    hy[hf] = hy[hf] - (coef * ((d->spiky * (d->h * d->h)) * R(0.001)));
-                } else {
-                    real ri = RSQ(r2);
-                    real q = d->h - (r2 * ri);
-                    real sc = coef * ((d->spiky * (q * q)) * ri);
-                    hx[hf] = FMA(-sc, dx, hx[hf]);
-                    hy[hf] = FMA(-sc, dy, hy[hf]);
-                    hz[hf] = FMA(-sc, dz, hz[hf]);
-                }
-            }
-            real fx = hx[0] + hx[1], fy = hy[0] + hy[1], fz = hz[0] + hz[1];
-            ax = ax + (fx * ir0);
-            ay = ay + (fy * ir0);
-            az = az + (fz * ir0);
+            real f[3];
+            fluid_sum(s, s->P, i, f);
+            real ir0 = R(1) / s->rho0[i];
+            ax = ax + (f[0] * ir0);
+            ay = ay + (f[1] * ir0);
+            az = az + (f[2] * ir0);
         }
 
         /* a9: distance constraints, mass-weighted with stiffness and
@@ -1115,12 +1155,42 @@ ORC_EXPORT int SFX(orc_iterate)(orc_sys *s)
  * from displacements u = x - x*:  v_i = u_i/dt + ((c/rho0_i) S)/dt with
  * S = sum_j (u_j - u_i) m_j W_ij; then x <- x_final (optionally sleeping,
  * cpu/src/particle.h:60-65).                                               */
+
+/* S of fluid particle i at iterate P with start positions X (3 per
+ * particle): per pair (u_j - u_i) m_j t^3, t = h^2 - r^2, in two halves of
+ * the table as lambda_sums; poly6 multiplies the finished sum. */
+static void xsph_sum(const orc_sys *s, const real *P, const real *X, uint32_t i, real *S)
+{
+    const orc_derived *d = &s->d;
+    const uint32_t kmax = s->kmax;
+    real px = P[4 * i], py = P[4 * i + 1], pz = P[4 * i + 2];
+    real ux = px - X[3 * i], uy = py - X[3 * i + 1], uz = pz - X[3 * i + 2];
+    real hx[2] = {R(0), R(0)}, hy[2] = {R(0), R(0)}, hz[2] = {R(0), R(0)};
+    for (uint32_t k = 0; k < s->ncnt[i]; k++) {
+        uint32_t j = s->nbr[(size_t)i * kmax + k];
+        const int hf = HALF(k);
+        if (s->phase[j] != PS_PHASE_FLUID) continue;
+        real qx = P[4 * j], qy = P[4 * j + 1], qz = P[4 * j + 2];
+        real dx = px - qx, dy = py - qy, dz = pz - qz;
+        real r2 = DIST2(dx, dy, dz);
+        if (!(r2 < d->h2)) continue;
+        real t = d->h2 - r2;
+        real m = ((t * t) * t) * (R(1) / s->w[j]);
+        real ex = qx - X[3 * j], ey = qy - X[3 * j + 1], ez = qz - X[3 * j + 2];
+        hx[hf] = FMA(ex - ux, m, hx[hf]);
+        hy[hf] = FMA(ey - uy, m, hy[hf]);
+        hz[hf] = FMA(ez - uz, m, hz[hf]);
+    }
+    S[0] = d->poly6 * (hx[0] + hx[1]);
+    S[1] = d->poly6 * (hy[0] + hy[1]);
+    S[2] = d->poly6 * (hz[0] + hz[1]);
+}
+
 ORC_EXPORT int SFX(orc_end)(orc_sys *s)
 {
     if (s->stage != 1) return PS_ERR_STATE;
     const orc_derived *d = &s->d;
     const real dt = s->dt;
-    const uint32_t kmax = s->kmax;
     real *Vn = s->tmpr; /* 3*n */
 #pragma omp parallel for schedule(static)
     for (int64_t ii = 0; ii < (int64_t)s->n; ii++) {
@@ -1129,29 +1199,12 @@ ORC_EXPORT int SFX(orc_end)(orc_sys *s)
         real ux = px - s->X[3 * i], uy = py - s->X[3 * i + 1], uz = pz - s->X[3 * i + 2];
         real vx = ux / dt, vy = uy / dt, vz = uz / dt;
         if (s->phase[i] == PS_PHASE_FLUID && d->xsph_c != R(0)) {
-            /* in two halves of the table, as the lambda pass */
-            real hx[2] = {R(0), R(0)}, hy[2] = {R(0), R(0)}, hz[2] = {R(0), R(0)};
-            for (uint32_t k = 0; k < s->ncnt[i]; k++) {
-                uint32_t j = s->nbr[(size_t)i * kmax + k];
-                const int hf = HALF(k);
-                if (s->phase[j] != PS_PHASE_FLUID) continue;
-                real qx = s->P[4 * j], qy = s->P[4 * j + 1], qz = s->P[4 * j + 2];
-                real dx = px - qx, dy = py - qy, dz = pz - qz;
-                real r2 = DIST2(dx, dy, dz);
-                if (!(r2 < d->h2)) continue;
-                real t = d->h2 - r2;
-                real W = d->poly6 * ((t * t) * t);
-                real m = W * (R(1) / s->w[j]);
-                real ex = qx - s->X[3 * j], ey = qy - s->X[3 * j + 1], ez = qz - s->X[3 * j + 2];
-                hx[hf] = FMA(ex - ux, m, hx[hf]);
-                hy[hf] = FMA(ey - uy, m, hy[hf]);
-                hz[hf] = FMA(ez - uz, m, hz[hf]);
-            }
-            real sx = hx[0] + hx[1], sy = hy[0] + hy[1], sz = hz[0] + hz[1];
+            real S[3];
+            xsph_sum(s, s->P, s->X, i, S);
             real cr = d->xsph_c / s->rho0[i];
-            vx = vx + ((cr * sx) / dt);
-            vy = vy + ((cr * sy) / dt);
-            vz = vz + ((cr * sz) / dt);
+            vx = vx + ((cr * S[0]) / dt);
+            vy = vy + ((cr * S[1]) / dt);
+            vz = vz + ((cr * S[2]) / dt);
         }
         Vn[3 * i] = vx;
         Vn[3 * i + 1] = vy;

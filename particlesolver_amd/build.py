@@ -13,7 +13,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libparticlesolver.so")
 
-SOURCES = ["ps_kernels.hip", "ps_adjoint.hip", "ps_cpuref.hip", "ps_export.hip", "ps_api.hip",
+SOURCES = ["ps_kernels.hip", "ps_adjoint.hip", "ps_cpuref.hip", "ps_export.hip", "ps_grid.hip", "ps_api.hip",
            "ps_trajectory.hip", "particle_system.hip", "simulation.hip"]
 DEMO_SRC = os.path.join(CSRC, "ps_demo.cpp")
 DEMO_BIN = os.path.join(HERE, "ps_demo")

@@ -104,13 +104,18 @@ class PsDerived(ctypes.Structure):
     _fields_ = [(n, ctypes.c_float) for n in
                 ("h2", "poly6", "spiky", "w_zero", "w_dq", "inv_w_dq", "tensile_k", "search_r2",
                  "collide_r2", "wall_s", "wall_k", "jitter")] + [
-        ("num_cells", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 3)]
+        ("num_cells", ctypes.c_uint32), ("w_ratio", ctypes.c_float), ("reserved", ctypes.c_uint32 * 2)]
 
 
 class PsSceneDesc(ctypes.Structure):
     _fields_ = [("box_min", ctypes.c_float * 3), ("box_max", ctypes.c_float * 3),
                 ("gravity_scale", ctypes.c_float), ("reserved0", ctypes.c_uint32),
                 ("seed", ctypes.c_uint64), ("reserved", ctypes.c_uint32 * 4)]
+
+
+class PsGridDesc(ctypes.Structure):
+    _fields_ = [("origin", ctypes.c_float * 3), ("cell", ctypes.c_float),
+                ("grid", ctypes.c_uint32 * 3), ("reserved", ctypes.c_uint32)]
 
 
 class PsStatus(ctypes.Structure):
@@ -182,6 +187,7 @@ PROTOTYPES = {
     "ps_autotune": (_int, [_P, _f32, _u32, _P]),
     "ps_get_tuning": (_int, [_P, _P]),
     "ps_set_tuning": (_int, [_P, _P]),
+    "ps_grid_sort": (_int, [ctypes.POINTER(PsGridDesc), _u32, _P, _P, _P, _P, _P, _P, _P]),
 }
 
 _lib = None
@@ -228,6 +234,15 @@ class DeviceBuffer:
             rc = self.hip.hipDeviceSynchronize()
         if rc:
             raise PsError(PS_ERR_DEVICE, "hipMemset failed: %d" % rc)
+
+    def upload(self, a):
+        """synchronous copy of a host array into the start of the buffer"""
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        rc = self.hip.hipMemcpy(ctypes.c_void_p(self.ptr), a.ctypes.data_as(ctypes.c_void_p),
+                                ctypes.c_size_t(a.nbytes), 1)       # hipMemcpyHostToDevice
+        if rc:
+            raise PsError(PS_ERR_DEVICE, "hipMemcpy failed: %d" % rc)
 
     def numpy(self, dtype, shape):
         """synchronous copy to the host (hipMemcpy waits for prior work)"""

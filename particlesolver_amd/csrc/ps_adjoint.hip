@@ -535,7 +535,7 @@ __global__ __launch_bounds__(kAdjBlock) void adj_xsph_k(DevBuffers b, KConst k, 
             const float r2 = dist2(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z);
             if (!(r2 < k.h2)) return;
             const float t = k.h2 - r2;
-            const float m = (k.poly6 * ((t * t) * t)) * (1.0f / wj);
+            const float m = ((t * t) * t) * (1.0f / wj);
             S[0] = __builtin_fmaf((pj.x - xj.x) - ui[0], m, S[0]);
             S[1] = __builtin_fmaf((pj.y - xj.y) - ui[1], m, S[1]);
             S[2] = __builtin_fmaf((pj.z - xj.z) - ui[2], m, S[2]);
@@ -556,7 +556,8 @@ __global__ __launch_bounds__(kAdjBlock) void adj_xsph_k(DevBuffers b, KConst k, 
                     if (d.a.phase == PS_PHASE_FLUID) body(Sh1, d.p, d.x, d.a.w);
                 });
         }
-        const float S[3] = {Sh0[0] + Sh1[0], Sh0[1] + Sh1[1], Sh0[2] + Sh1[2]};
+        const float S[3] = {k.poly6 * (Sh0[0] + Sh1[0]), k.poly6 * (Sh0[1] + Sh1[1]),
+                            k.poly6 * (Sh0[2] + Sh1[2])};
         const float vS = dot3(vbn, S);
         r0 = r0 - (((k.xsph_c / (ai.rho0 * ai.rho0)) * vS) / dt);
         for (int q = 0; q < 3; q++) Sh[q] = (vbn[q] * cr) / dt;

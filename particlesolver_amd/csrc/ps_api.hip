@@ -176,6 +176,7 @@ void derive_constants(const ps_params &p, ps_derived &d)
     d.w_zero = (float)(poly6 * h6);
     d.w_dq = (float)(poly6 * tq * tq * tq);
     d.inv_w_dq = (float)(1.0 / (poly6 * tq * tq * tq));
+    d.w_ratio = (float)(1.0 / (tq * tq * tq));
     // The tensile term is added to lambda (units length^2); the paper's
     // dimensionless k measures lengths in units of h, hence k h^2.  The
     // reference takes K_P as it is (integration_kernel.cuh:633).
@@ -198,6 +199,7 @@ void fill_kconst(const ps_params &p, const ps_derived &d, KConst &k)
     k.w_zero = d.w_zero;
     k.w_dq = d.w_dq;
     k.inv_w_dq = d.inv_w_dq;
+    k.w_ratio = d.w_ratio;
     k.search_r2 = d.search_r2;
     k.collide_r2 = d.collide_r2;
     k.collide_dist = p.collide_dist;
@@ -842,6 +844,10 @@ int read_sorted_positions(ps_system *s, std::vector<float4> &X, std::vector<Attr
 }  // namespace
 
 // ===========================================================================
+
+namespace ps {
+int fail_stateless(int code, const std::string &msg) { return fail(nullptr, code, msg); }
+}  // namespace ps
 
 extern "C" {
 

@@ -23,6 +23,25 @@ __device__ __forceinline__ uint32_t status_peek(const uint32_t *w)
     return __atomic_load_n(w, __ATOMIC_RELAXED);
 }
 
+// Cell coordinate of p on one axis (calcGridPos, integration_kernel.cuh:
+// 187-194): floor((p - o) / cell) by IEEE division, clamped into [0, g - 1]
+// of the dense grid (positions outside raise PS_STATUS_OUTSIDE_GRID in
+// flags, non-finite ones PS_STATUS_NONFINITE and land in cell 0).
+__device__ __forceinline__ uint32_t cell_axis(float p, float o, float cell, uint32_t g,
+                                              uint32_t &flags)
+{
+    float f = floorf((p - o) / cell);
+    if (!(f >= 0.0f)) {
+        flags |= (f != f) ? PS_STATUS_NONFINITE : PS_STATUS_OUTSIDE_GRID;
+        return 0u;
+    }
+    if (f >= (float)g) {
+        flags |= PS_STATUS_OUTSIDE_GRID;
+        return g - 1u;
+    }
+    return (uint32_t)f;
+}
+
 // chunk m (entries 4m..4m+3) of particle i is nbr_lane(b, i)[m * 64]
 __device__ __forceinline__ const uint4 *nbr_lane(const DevBuffers &b, uint32_t i)
 {
@@ -35,8 +54,8 @@ __device__ __forceinline__ const uint4 *nbr_lane(const DevBuffers &b, uint32_t i
 // are already in flight.  The loop is unrolled over two chunks whose record
 // sets swap roles, so no records are copied between registers.  Every entry
 // of a chunk at or below the last one is a valid slot (the builder pads the
-// last chunk with the particle's own slot), so all loads are unconditional;
-// only fn is guarded.
+// last chunk with the particle's own slot, or with the sentinel record in a
+// staged unit), so all loads are unconditional; only fn is guarded.
 template <typename G, typename Fn>
 __device__ __forceinline__ void for_neighbors(const DevBuffers &b, uint32_t i, uint32_t c, G gather,
                                               Fn fn)
@@ -86,8 +105,10 @@ __device__ __forceinline__ void for_neighbors(const DevBuffers &b, uint32_t i, u
 // staged kernels give each half its own lane (two lanes per particle, twice
 // the wavefronts for the same LDS); the others visit both halves in turn.
 // Visits the entries of half h of particle i's first c neighbours, pipelined
-// as for_neighbors.
-template <typename G, typename Fn>
+// as for_neighbors.  GUARD = false visits the padding of the last chunk as
+// well: for tables whose padding entries refer to the far-away sentinel
+// record of a staged unit (kStageSentinel), which fails every pair test.
+template <bool GUARD = true, typename G, typename Fn>
 __device__ __forceinline__ void for_neighbors_half(const DevBuffers &b, uint32_t i, uint32_t c, uint32_t h,
                                                    G gather, Fn fn)
 {
@@ -100,9 +121,9 @@ __device__ __forceinline__ void for_neighbors_half(const DevBuffers &b, uint32_t
                      const auto &d3) {
         const uint32_t q = (2u * m + h) << 2;  // table position of jj.x
         fn(jj.x, d0);
-        if (q + 1 < c) fn(jj.y, d1);
-        if (q + 2 < c) fn(jj.z, d2);
-        if (q + 3 < c) fn(jj.w, d3);
+        if (!GUARD || q + 1 < c) fn(jj.y, d1);
+        if (!GUARD || q + 2 < c) fn(jj.z, d2);
+        if (!GUARD || q + 3 < c) fn(jj.w, d3);
     };
     uint4 ja = chunk(0);
     uint4 jb = chunk(1);
@@ -128,6 +149,19 @@ __device__ __forceinline__ void for_neighbors_half(const DevBuffers &b, uint32_t
         ja = jc;
         jb = jd;
     }
+}
+
+// The entries of half h of particle i's first c neighbours in table order,
+// one at a time: for the rarely taken recomputation of one half.
+template <typename G, typename Fn>
+__device__ __forceinline__ void for_neighbors_half_plain(const DevBuffers &b, uint32_t i, uint32_t c,
+                                                         uint32_t h, G gather, Fn fn)
+{
+    for (uint32_t q0 = 4u * h; q0 < c; q0 += 8u)
+        for (uint32_t q = q0; q < q0 + 4u && q < c; q++) {
+            const uint32_t j = reinterpret_cast<const uint32_t *>(nbr_lane(b, i) + (size_t)(q >> 2) * 64)[q & 3u];
+            fn(j, gather(j));
+        }
 }
 
 // Both halves in one pass, as for_neighbors: fn0 gets the entries of the
@@ -220,7 +254,7 @@ __device__ __forceinline__ uint32_t nbr_at(const DevBuffers &b, uint32_t i, uint
 // ---------------------------------------------------------------------------
 // Pair arithmetic shared with the CPU oracle: squared distances and sums use
 // fused multiply-add, and 1/r comes from rsq() instead of sqrt + division
-// (an IEEE square root and division cost ~30 VALU per pair, rsq 12).
+// (an IEEE square root and division cost ~30 VALU per pair, rsq 9).
 
 // |d|^2 = fma(dz, dz, fma(dy, dy, dx dx))
 __device__ __forceinline__ float dist2(float dx, float dy, float dz)
@@ -228,21 +262,24 @@ __device__ __forceinline__ float dist2(float dx, float dy, float dz)
     return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
 }
 
-// 1/sqrt(x), x > 0 normal: integer seed 0x5f375a86 - (bits(x) >> 1) (the
-// "fast inverse square root" constant of C. Lomont, 2003) and three Newton
-// steps y <- y (3/2 - (x/2) y^2) as t = (x/2) y; u = fma(-t, y, 3/2);
-// y = y u.  Relative error about 1e-7.
+// 1/sqrt(x), x > 0 normal: integer seed y0 = 0x5f375a86 - (bits(x) >> 1)
+// (the "fast inverse square root" constant of C. Lomont, 2003; relative
+// error up to 3.4e-2), then one fourth-order correction.  With
+// e = 1 - x y0^2, 1/sqrt(x) = y0 (1 - e)^(-1/2) = y0 (1 + e p(e)), p the
+// binomial series 1/2 + 3/8 e + 5/16 e^2 + 35/128 e^3 + ..., truncated
+// after e^3 and its four coefficients re-fitted minimax over the seed's
+// range of e (|e| < 0.07): y = fma(y0 e, p(e), y0), p by Horner in FMA.
+// 9 operations; relative error below 1e-7 over [1e-8, 16] (the three
+// Newton steps it replaces took 12 for 1.3e-7).
 __device__ __forceinline__ float rsq(float x)
 {
-    float y = __uint_as_float(0x5f375a86u - (__float_as_uint(x) >> 1));
-    const float hx = 0.5f * x;
-#pragma unroll
-    for (int q = 0; q < 3; q++) {
-        const float t = hx * y;
-        const float u = __builtin_fmaf(-t, y, 1.5f);
-        y = y * u;
-    }
-    return y;
+    const float y0 = __uint_as_float(0x5f375a86u - (__float_as_uint(x) >> 1));
+    const float xy = x * y0;
+    const float e = __builtin_fmaf(-xy, y0, 1.0f);
+    float p = __builtin_fmaf(__uint_as_float(0x3e8c5420u), e, __uint_as_float(0x3ea0bc22u));
+    p = __builtin_fmaf(p, e, __uint_as_float(0x3ec0000cu));
+    p = __builtin_fmaf(p, e, __uint_as_float(0x3effffc8u));
+    return __builtin_fmaf(y0 * e, p, y0);
 }
 
 // below this squared distance a pair has no direction

@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <string>
+
 #include "../../include/ps.h"
 
 namespace ps {
@@ -45,6 +47,7 @@ struct __align__(16) SceneDev {
 // Constants handed to every kernel by value (kernarg segment).
 struct KConst {
     float h, h2, poly6, spiky, w_zero, w_dq, inv_w_dq, search_r2, collide_r2, collide_dist;
+    float w_ratio;  // 1/(h^2 - (dq h)^2)^3: W(r)/W(dq h) = w_ratio (h^2 - r^2)^3
     float wall_s, wall_k, jitter, radius, relax, k_corr, xsph_c, s_solid, mu_s, mu_k;
     float gravity[3];
     float box_min[3], box_max[3];
@@ -95,6 +98,11 @@ constexpr uint32_t kStageIv = 9 * kStageRows;
 constexpr uint32_t kStageSlots = 3072;
 constexpr uint32_t kStageChunk = 16;
 constexpr uint32_t kStageChunks = kStageSlots / kStageChunk;
+// LDS index of a staged unit's sentinel record, one past the staged slots:
+// a position at +infinity, whose squared distance to every particle fails
+// the pair tests.  A staged unit's table pads its last chunk with it, so
+// the staged kernels visit whole chunks without bound checks.
+constexpr uint32_t kStageSentinel = kStageSlots;
 
 struct StageDesc {
     uint32_t staged;              // 1: the unit's table entries are LDS indices
@@ -230,6 +238,12 @@ hipError_t launch_finalize(hipStream_t st, const DevBuffers &b, const KConst &k,
                            uint32_t bs);
 
 uint32_t scan_tiles(uint32_t num_cells);
+// cells per scan tile (launch_scan's partials: one per tile)
+constexpr uint32_t kScanTileCells = 4096;
+
+// Records msg as the calling thread's last error (ps_last_error(NULL)) and
+// returns code: for the entry points without a system handle.
+int fail_stateless(int code, const std::string &msg);
 
 // Reverse pass (ps_adjoint.hip), one kernel per call: velocity + XSPH
 // backwards (ADJ_XSPH then ADJ_FINALIZE; b.P the final iterate, b.X the

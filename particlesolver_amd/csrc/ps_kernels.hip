@@ -28,6 +28,7 @@ static constexpr uint32_t kBlock = 256;     // fixed-shape kernels (scan, scatte
 static constexpr uint32_t kMaxBlock = 512;  // tunable kernels run with 64..512 threads
 static constexpr uint32_t kScanPerThread = 16;
 static constexpr uint32_t kScanTile = kBlock * kScanPerThread;
+static_assert(kScanTile == kScanTileCells, "scan tile size");
 
 
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v)
@@ -65,21 +66,6 @@ __device__ __forceinline__ uint32_t wave_max(uint32_t v)
 // clamped grid.  The atomic's return value is the particle's arrival rank in
 // its cell, used only to place it provisionally; the final in-cell order is
 // fixed by insertion index in reorder_k.
-
-__device__ __forceinline__ uint32_t cell_axis(float p, float o, float cell, uint32_t g,
-                                              uint32_t &flags)
-{
-    float f = floorf((p - o) / cell);
-    if (!(f >= 0.0f)) {
-        flags |= (f != f) ? PS_STATUS_NONFINITE : PS_STATUS_OUTSIDE_GRID;
-        return 0u;
-    }
-    if (f >= (float)g) {
-        flags |= PS_STATUS_OUTSIDE_GRID;
-        return g - 1u;
-    }
-    return (uint32_t)f;
-}
 
 template <bool PRED, bool HASH>
 __global__ __launch_bounds__(kMaxBlock) void predict_hash_k(DevBuffers b, KConst k, uint32_t n,
@@ -317,9 +303,9 @@ __global__ __launch_bounds__(kMaxBlock) void reorder_k(DevBuffers b, uint32_t n)
 // Scan the nine stencil rows of particle i in order (t = (dz+1)*3 + dy+1)
 // and write its table: every slot j != i of range(t) = [j0, j1) with
 // |p - P[j]|^2 < search_r2, in ascending j, stored as emit(t, j) (the slot
-// itself, or its LDS index in a staged unit); pad is i's own entry, which
-// fills the last chunk.  Returns the number found (the table keeps the
-// first kmax).
+// itself, or its LDS index in a staged unit); pad fills the last chunk: i's
+// own slot, or a staged unit's sentinel record.  Returns the number found
+// (the table keeps the first kmax).
 // A row is tested in windows of 32 candidates without branches: each test
 // sets one bit of a mask (i's own bit is cleared), and the set bits are then
 // emitted lowest first, so the entries come out in ascending j as from a
@@ -611,7 +597,8 @@ __global__ __launch_bounds__(kStageUnit) void neighbors_staged_k(DevBuffers b, K
             const uint32_t iv = 9u * q + t;
             return staged ? j + (s_base[iv] - s_start[iv]) : j;
         };
-        cnt = scan_stencil(b, k, i, emit(4, i), b.P[i], kmax, range, emit);
+        // a staged unit pads with its sentinel record, a global one with i
+        cnt = scan_stencil(b, k, i, staged ? kStageSentinel : i, b.P[i], kmax, range, emit);
     }
     neighbor_status(b, cnt, kmax, ldsw);
 }
@@ -619,9 +606,14 @@ __global__ __launch_bounds__(kStageUnit) void neighbors_staged_k(DevBuffers b, K
 // ---------------------------------------------------------------------------
 // a5: PBF multiplier (findLambdasD, integration_kernel.cuh:521-593)
 //   rho_i = W(0)/w_i + sum_j s_j W(r)/w_j,  grad_j = a d,
-//   a = (spiky/rho0_i) (h-r)^2 / r,  1/r = rsq(r^2),  r = r^2 (1/r)
-//   lambda_i = -(rho_i/rho0_i - 1) / (sum a^2 r^2 + |sum grad_j|^2 + eps)
-// degenerate pairs (r^2 < 1e-8) add density only
+//   a = (spiky/rho0_i) (h-r)^2 / r,
+//   lambda_i = -(rho_i/rho0_i - 1) / (sum |grad_j|^2 + |sum grad_j|^2 + eps)
+// Per pair, with t = h^2 - r^2, 1/r = rsq(r^2) and q = h - r =
+// fma(-r^2, 1/r, h): the density term t^3 m_j, the gradient term q^2 (1/r) d
+// and |grad_j|^2 as q^4; the constants poly6 and sk = spiky/rho0_i multiply
+// the finished sums (rho = fma(poly6, sum, W(0)/w_i), grad sum sk S,
+// sum |grad_j|^2 = sk^2 sum q^4).  Degenerate pairs (r^2 < 1e-8) add density
+// only.
 // UNIFORM: every particle is fluid with inverse mass wu (no Attr gathers).
 
 // TAPE (reverse-pass recording): also store S, D and rho per particle.
@@ -629,9 +621,10 @@ template <bool UNIFORM, bool STAGED, bool TAPE = false>
 __global__ __launch_bounds__(kMaxBlock) void lambda_k(DevBuffers b, KConst k, uint32_t n, float mu)
 {
     // staged: neighbour positions only, 12 bytes per slot (36 KB, so four
-    // blocks fit in a CU's LDS where 16-byte slots allow three); two lanes
-    // per particle (2 kStageUnit threads), lane hh summing half hh
-    __shared__ float lds3[STAGED ? 3 * kStageSlots : 1];
+    // blocks fit in a CU's LDS where 16-byte slots allow three), and the
+    // sentinel record; two lanes per particle (2 kStageUnit threads), lane
+    // hh summing half hh
+    __shared__ float lds3[STAGED ? 3 * (kStageSlots + 1) : 1];
     const uint32_t unit = xcd_block();
     const uint32_t hh = STAGED ? (threadIdx.x & 1u) : 0u;
     const uint32_t i = STAGED ? unit * kStageUnit + (threadIdx.x >> 1) : unit * blockDim.x + threadIdx.x;
@@ -639,6 +632,7 @@ __global__ __launch_bounds__(kMaxBlock) void lambda_k(DevBuffers b, KConst k, ui
     if (STAGED) {
         const StageDesc *d = b.stage + unit;
         staged = d->staged != 0u;
+        if (staged && threadIdx.x < 3) lds3[3 * kStageSentinel + threadIdx.x] = __builtin_inff();
         if (staged)
             stage_copy<6, 2 * kStageUnit>(d, n, [&](uint32_t j) { return b.P[j]; },
                                           [&](uint32_t l, const float4 &v) {
@@ -664,7 +658,7 @@ __global__ __launch_bounds__(kMaxBlock) void lambda_k(DevBuffers b, KConst k, ui
     const float r0 = a.rho0;
     const float sk = k.spiky * (1.0f / r0);
     const uint32_t c = b.ncnt[i];
-    // the sums of one half of the table (half 0 also holds the self term)
+    // the sums of one half of the table
     struct Sums {
         float rho, gix, giy, giz, den;
     };
@@ -675,28 +669,32 @@ __global__ __launch_bounds__(kMaxBlock) void lambda_k(DevBuffers b, KConst k, ui
         float r2 = dist2(dx, dy, dz);
         if (!(r2 < k.h2)) return;
         float t = k.h2 - r2;
-        float W = k.poly6 * ((t * t) * t);
-        s.rho = __builtin_fmaf(W, mj, s.rho);
+        s.rho = __builtin_fmaf((t * t) * t, mj, s.rho);
+        if (r2 < kDegenerateR2) return;  // density only
         const float ri = rsq(r2);
-        const float qq = k.h - (r2 * ri);
-        // a degenerate pair adds exact zeros: its d components are tiny and
-        // finite, and sums that start at +0 never become -0, so x + 0 = x
-        const float aa = r2 < kDegenerateR2 ? 0.0f : (sk * (qq * qq)) * ri;
-        s.den = __builtin_fmaf(aa * aa, r2, s.den);
+        const float qq = __builtin_fmaf(-r2, ri, k.h);
+        const float q2 = qq * qq;
+        const float aa = q2 * ri;
+        s.den = __builtin_fmaf(q2, q2, s.den);
         s.gix = __builtin_fmaf(aa, dx, s.gix);
         s.giy = __builtin_fmaf(aa, dy, s.giy);
         s.giz = __builtin_fmaf(aa, dz, s.giz);
     };
-    Sums s0 = {self, 0.0f, 0.0f, 0.0f, 0.0f}, s1 = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    Sums s0 = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f}, s1 = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     if constexpr (STAGED) {
         // this lane's half (from LDS, or from global memory for a unit that
         // was not staged); the partner lane (lane ^ 1) holds the other.
         // Both lanes form half 0 + half 1 (addition commutes)
-        Sums mine = {hh ? 0.0f : self, 0.0f, 0.0f, 0.0f, 0.0f};
+        Sums mine = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
         if (staged)
-            for_neighbors_half(
+            for_neighbors_half<false>(
                 b, i, c, hh,
-                [&](uint32_t l) { return make_float4(lds3[3 * l], lds3[3 * l + 1], lds3[3 * l + 2], 0.0f); },
+                [&](uint32_t l) {
+                    // byte offset 12 l by the full-rate 24-bit multiply (l <= kStageSentinel)
+                    const float *q = reinterpret_cast<const float *>(reinterpret_cast<const char *>(lds3) +
+                                                                     __umul24(l, 12u));
+                    return make_float4(q[0], q[1], q[2], 0.0f);
+                },
                 [&](uint32_t, const float4 &pj) { body(mine, pj, mu); });
         else
             for_neighbors_half(
@@ -723,10 +721,9 @@ __global__ __launch_bounds__(kMaxBlock) void lambda_k(DevBuffers b, KConst k, ui
             b, i, c, [&](uint32_t j) { return PA{b.P[j], b.A[j]}; },
             [&](uint32_t, const PA &d) { gen(s0, d); }, [&](uint32_t, const PA &d) { gen(s1, d); });
     }
-    const float rho = s0.rho + s1.rho;
-    const float gix = s0.gix + s1.gix, giy = s0.giy + s1.giy, giz = s0.giz + s1.giz;
-    float den = s0.den + s1.den;
-    den = den + dist2(gix, giy, giz);
+    const float rho = __builtin_fmaf(k.poly6, s0.rho + s1.rho, self);
+    const float gix = sk * (s0.gix + s1.gix), giy = sk * (s0.giy + s1.giy), giz = sk * (s0.giz + s1.giz);
+    const float den = __builtin_fmaf(sk * sk, s0.den + s1.den, dist2(gix, giy, giz));
     float C = (rho / r0) - 1.0f;
     if (STAGED && hh) return;  // the pair's lane 0 writes
     const float lam = -C / (den + k.relax);
@@ -751,8 +748,9 @@ __global__ __launch_bounds__(kMaxBlock) void solve_k(DevBuffers b, KConst k, uin
                                                   uint32_t it)
 {
     // staged: two lanes per particle (2 kStageUnit threads), lane hh summing
-    // half hh of the fluid sum; lane 0 finishes the particle
-    __shared__ float4 lds[STAGED ? kStageSlots : 1];
+    // half hh of the fluid sum; lane 0 finishes the particle.  The LDS holds
+    // the unit's slots and the sentinel record
+    __shared__ float4 lds[STAGED ? kStageSlots + 1 : 1];
     const uint32_t unit = xcd_block();
     const uint32_t hh = STAGED ? (threadIdx.x & 1u) : 0u;
     const uint32_t i = STAGED ? unit * kStageUnit + (threadIdx.x >> 1) : unit * blockDim.x + threadIdx.x;
@@ -760,6 +758,8 @@ __global__ __launch_bounds__(kMaxBlock) void solve_k(DevBuffers b, KConst k, uin
     if (STAGED) {
         const StageDesc *d = b.stage + unit;
         staged = d->staged != 0u;
+        if (staged && threadIdx.x == 0)
+            lds[kStageSentinel] = make_float4(__builtin_inff(), __builtin_inff(), __builtin_inff(), 0.0f);
         if (staged)
             stage_copy<6, 2 * kStageUnit>(d, n, [&](uint32_t j) { return b.P[j]; },
                                           [&](uint32_t l, const float4 &v) { lds[l] = v; });
@@ -852,68 +852,116 @@ __global__ __launch_bounds__(kMaxBlock) void solve_k(DevBuffers b, KConst k, uin
     }
 
     // a6: PBF delta-p with tensile correction s_corr = -k h^2 (W/W(dq h))^n
-    // (solveFluidsD :596-642); k.k_corr already holds k h^2
+    // (solveFluidsD :596-642); k.k_corr already holds k h^2 (or k).  Per
+    // pair, with t = h^2 - r^2, 1/r = rsq(r^2) and q = h - r =
+    // fma(-r^2, 1/r, h): W/W(dq h) = w_ratio t^3, coef = fma(-k, (W/W(dq
+    // h))^n, l_i + l_j), and the term -coef q^2 (1/r) d; spiky multiplies
+    // the finished sum
     if (F & F_FLUID) {
         if (ph == PS_PHASE_FLUID) {
             const float li = p.w, ir0 = 1.0f / a.rho0;
             // r^2 < 1e-8: gradient (0, -0.001 spiky h^2, 0) (integration_kernel.cuh:625-626)
-            const float gdeg = (k.spiky * (k.h * k.h)) * 0.001f;
-            // the sum of one half of the table
+            const float gdeg = (k.h * k.h) * 0.001f;
+            // the sum of one half of the table; deg: a degenerate pair was
+            // left out, and the half must be formed again by exact()
             struct Sums {
                 float fx, fy, fz;
+                bool deg;
             };
-            auto body = [&](Sums &s, const float4 &pj) {
+            auto coef_of = [&](float r2, float lj) {
+                const float t = k.h2 - r2;
+                const float x = k.w_ratio * ((t * t) * t);
+                return __builtin_fmaf(-k.k_corr, ipow(x, NC ? NC : k.n_corr), li + lj);
+            };
+            auto term = [&](Sums &s, float coef, float r2, float dx, float dy, float dz) {
+                const float ri = rsq(r2);
+                const float qq = __builtin_fmaf(-r2, ri, k.h);
+                const float g = coef * ((qq * qq) * ri);
+                s.fx = __builtin_fmaf(-g, dx, s.fx);
+                s.fy = __builtin_fmaf(-g, dy, s.fy);
+                s.fz = __builtin_fmaf(-g, dz, s.fz);
+            };
+            // every pair but a degenerate one (essentially never present)
+            auto fast = [&](Sums &s, const float4 &pj) {
                 float dx = px - pj.x, dy = py - pj.y, dz = pz - pj.z;
                 float r2 = dist2(dx, dy, dz);
                 if (!(r2 < k.h2)) return;
-                float t = k.h2 - r2;
-                float W = k.poly6 * ((t * t) * t);
-                float corr = -(k.k_corr * ipow(W * k.inv_w_dq, NC ? NC : k.n_corr));
-                float coef = (li + pj.w) + corr;
-                const float ri = rsq(r2);
-                const float qq = k.h - (r2 * ri);
-                const float sc = coef * ((k.spiky * (qq * qq)) * ri);
-                // a degenerate pair (essentially never taken: one compare per
-                // pair) has only its y term
-                if (r2 < kDegenerateR2) {
-                    s.fy = s.fy - (coef * gdeg);
-                } else {
-                    s.fx = __builtin_fmaf(-sc, dx, s.fx);
-                    s.fy = __builtin_fmaf(-sc, dy, s.fy);
-                    s.fz = __builtin_fmaf(-sc, dz, s.fz);
-                }
+                const bool dg = r2 < kDegenerateR2;
+                s.deg = s.deg || dg;
+                if (dg) return;
+                term(s, coef_of(r2, pj.w), r2, dx, dy, dz);
             };
-            Sums s0 = {0.0f, 0.0f, 0.0f}, s1 = {0.0f, 0.0f, 0.0f};
+            // the reference's sum, degenerate pairs in their place
+            auto exact = [&](Sums &s, const float4 &pj) {
+                float dx = px - pj.x, dy = py - pj.y, dz = pz - pj.z;
+                float r2 = dist2(dx, dy, dz);
+                if (!(r2 < k.h2)) return;
+                const float coef = coef_of(r2, pj.w);
+                if (r2 < kDegenerateR2)
+                    s.fy = s.fy - (coef * gdeg);
+                else
+                    term(s, coef, r2, dx, dy, dz);
+            };
+            Sums s0 = {0.0f, 0.0f, 0.0f, false}, s1 = {0.0f, 0.0f, 0.0f, false};
             if constexpr (STAGED) {
                 // this lane's half; the partner lane (lane ^ 1) holds the other
-                Sums mine = {0.0f, 0.0f, 0.0f};
+                Sums mine = {0.0f, 0.0f, 0.0f, false};
+                auto from_lds = [&](uint32_t l) { return lds[l]; };
+                auto from_glob = [&](uint32_t j) { return b.P[j]; };
                 if (staged)
-                    for_neighbors_half(
-                        b, i, c, hh, [&](uint32_t l) { return lds[l]; },
-                        [&](uint32_t, const float4 &pj) { body(mine, pj); });
+                    for_neighbors_half<false>(b, i, c, hh, from_lds,
+                                              [&](uint32_t, const float4 &pj) { fast(mine, pj); });
                 else
-                    for_neighbors_half(
-                        b, i, c, hh, [&](uint32_t j) { return b.P[j]; },
-                        [&](uint32_t, const float4 &pj) { body(mine, pj); });
+                    for_neighbors_half(b, i, c, hh, from_glob,
+                                       [&](uint32_t, const float4 &pj) { fast(mine, pj); });
+                if (mine.deg) {
+                    mine = {0.0f, 0.0f, 0.0f, false};
+                    auto ex = [&](uint32_t, const float4 &pj) { exact(mine, pj); };
+                    if (staged)
+                        for_neighbors_half_plain(b, i, c, hh, from_lds, ex);
+                    else
+                        for_neighbors_half_plain(b, i, c, hh, from_glob, ex);
+                }
                 const Sums other = {__shfl_xor(mine.fx, 1, 64), __shfl_xor(mine.fy, 1, 64),
-                                    __shfl_xor(mine.fz, 1, 64)};
+                                    __shfl_xor(mine.fz, 1, 64), false};
                 s0 = hh ? other : mine;
                 s1 = hh ? mine : other;
             } else if (UNIFORM) {
+                auto gather = [&](uint32_t j) { return b.P[j]; };
                 for_neighbors_split(
-                    b, i, c, [&](uint32_t j) { return b.P[j]; },
-                    [&](uint32_t, const float4 &pj) { body(s0, pj); },
-                    [&](uint32_t, const float4 &pj) { body(s1, pj); });
+                    b, i, c, gather, [&](uint32_t, const float4 &pj) { fast(s0, pj); },
+                    [&](uint32_t, const float4 &pj) { fast(s1, pj); });
+                if (s0.deg || s1.deg) {
+                    s0 = {0.0f, 0.0f, 0.0f, false};
+                    s1 = {0.0f, 0.0f, 0.0f, false};
+                    for_neighbors_half_plain(b, i, c, 0u, gather,
+                                             [&](uint32_t, const float4 &pj) { exact(s0, pj); });
+                    for_neighbors_half_plain(b, i, c, 1u, gather,
+                                             [&](uint32_t, const float4 &pj) { exact(s1, pj); });
+                }
             } else {
-                auto gen = [&](Sums &s, const PA &d) {
-                    if (d.a.w == 0.0f) return;
-                    body(s, d.p);
-                };
+                auto gather = [&](uint32_t j) { return PA{b.P[j], b.A[j]}; };
                 for_neighbors_split(
-                    b, i, c, [&](uint32_t j) { return PA{b.P[j], b.A[j]}; },
-                    [&](uint32_t, const PA &d) { gen(s0, d); }, [&](uint32_t, const PA &d) { gen(s1, d); });
+                    b, i, c, gather,
+                    [&](uint32_t, const PA &d) {
+                        if (d.a.w != 0.0f) fast(s0, d.p);
+                    },
+                    [&](uint32_t, const PA &d) {
+                        if (d.a.w != 0.0f) fast(s1, d.p);
+                    });
+                if (s0.deg || s1.deg) {
+                    s0 = {0.0f, 0.0f, 0.0f, false};
+                    s1 = {0.0f, 0.0f, 0.0f, false};
+                    for_neighbors_half_plain(b, i, c, 0u, gather, [&](uint32_t, const PA &d) {
+                        if (d.a.w != 0.0f) exact(s0, d.p);
+                    });
+                    for_neighbors_half_plain(b, i, c, 1u, gather, [&](uint32_t, const PA &d) {
+                        if (d.a.w != 0.0f) exact(s1, d.p);
+                    });
+                }
             }
-            const float fx = s0.fx + s1.fx, fy = s0.fy + s1.fy, fz = s0.fz + s1.fz;
+            const float fx = k.spiky * (s0.fx + s1.fx), fy = k.spiky * (s0.fy + s1.fy),
+                        fz = k.spiky * (s0.fz + s1.fz);
             ax = ax + (fx * ir0);
             ay = ay + (fy * ir0);
             az = az + (fz * ir0);
@@ -1088,6 +1136,8 @@ __global__ __launch_bounds__(kBlock) void shape_k(DevBuffers b, uint32_t nbody)
 // a11: v = (x - x*)/dt (subtract_functor, integration_kernel.cuh:465-475)
 // plus XSPH over fluid neighbours from displacements u = x - x*:
 //   v_i = u_i/dt + ((c/rho0_i) sum_j (u_j - u_i) m_j W_ij)/dt
+// with the pair term (u_j - u_i) m_j t^3, t = h^2 - r^2, and poly6
+// multiplying the finished sum.
 
 template <bool UNIFORM, bool STAGED>
 __global__ __launch_bounds__(kMaxBlock) void finalize_k(DevBuffers b, KConst k, uint32_t n, float mu,
@@ -1096,8 +1146,9 @@ __global__ __launch_bounds__(kMaxBlock) void finalize_k(DevBuffers b, KConst k, 
     // staged: neighbour position and displacement u = p - x* (x, y, z, u.x)
     // and (u.y, u.z), formed with the same subtractions as the global path
     // two lanes per particle when staged, as in lambda_k and solve_k
-    __shared__ float4 lds_pu[STAGED ? kStageSlots : 1];
-    __shared__ float2 lds_u[STAGED ? kStageSlots : 1];
+    // (and the sentinel record)
+    __shared__ float4 lds_pu[STAGED ? kStageSlots + 1 : 1];
+    __shared__ float2 lds_u[STAGED ? kStageSlots + 1 : 1];
     const uint32_t unit = xcd_block();
     const uint32_t hh = STAGED ? (threadIdx.x & 1u) : 0u;
     const uint32_t i = STAGED ? unit * kStageUnit + (threadIdx.x >> 1) : unit * blockDim.x + threadIdx.x;
@@ -1105,6 +1156,10 @@ __global__ __launch_bounds__(kMaxBlock) void finalize_k(DevBuffers b, KConst k, 
     if (STAGED) {
         const StageDesc *d = b.stage + unit;
         staged = d->staged != 0u;
+        if (staged && threadIdx.x == 0) {
+            lds_pu[kStageSentinel] = make_float4(__builtin_inff(), __builtin_inff(), __builtin_inff(), 0.0f);
+            lds_u[kStageSentinel] = make_float2(0.0f, 0.0f);
+        }
         if (staged)
             stage_copy<4, 2 * kStageUnit>(d, n, [&](uint32_t j) { return PX{b.P[j], b.X[j]}; },
                                           [&](uint32_t l, const PX &v) {
@@ -1132,8 +1187,7 @@ __global__ __launch_bounds__(kMaxBlock) void finalize_k(DevBuffers b, KConst k, 
             float r2 = dist2(dx, dy, dz);
             if (!(r2 < k.h2)) return;
             float t = k.h2 - r2;
-            float W = k.poly6 * ((t * t) * t);
-            float m = W * mj;
+            float m = ((t * t) * t) * mj;
             s.sx = __builtin_fmaf(ex - ux, m, s.sx);
             s.sy = __builtin_fmaf(ey - uy, m, s.sy);
             s.sz = __builtin_fmaf(ez - uz, m, s.sz);
@@ -1154,7 +1208,7 @@ __global__ __launch_bounds__(kMaxBlock) void finalize_k(DevBuffers b, KConst k, 
             // this lane's half; the partner lane (lane ^ 1) holds the other
             Sums mine = {0.0f, 0.0f, 0.0f};
             if (staged)
-                for_neighbors_half(
+                for_neighbors_half<false>(
                     b, i, c, hh, [&](uint32_t l) { return PU{lds_pu[l], lds_u[l]}; },
                     [&](uint32_t, const PU &d) { body(mine, d.pu, d.pu.w, d.u.x, d.u.y, mu); });
             else
@@ -1178,7 +1232,8 @@ __global__ __launch_bounds__(kMaxBlock) void finalize_k(DevBuffers b, KConst k, 
                 b, i, c, [&](uint32_t j) { return PXA{b.P[j], b.X[j], b.A[j]}; },
                 [&](uint32_t, const PXA &d) { gen(s0, d); }, [&](uint32_t, const PXA &d) { gen(s1, d); });
         }
-        const float sx = s0.sx + s1.sx, sy = s0.sy + s1.sy, sz = s0.sz + s1.sz;
+        const float sx = k.poly6 * (s0.sx + s1.sx), sy = k.poly6 * (s0.sy + s1.sy),
+                    sz = k.poly6 * (s0.sz + s1.sz);
         const float cr = k.xsph_c / a.rho0;
         vx = vx + ((cr * sx) / dt);
         vy = vy + ((cr * sy) / dt);

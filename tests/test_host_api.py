@@ -128,3 +128,24 @@ def test_create_without_gpu_reports_no_device_or_succeeds():
         assert lib.ps_destroy(h) == capi.PS_OK
     else:
         assert b"device" in lib.ps_last_error(None).lower()
+
+
+def test_grid_sort_rejects_bad_arguments_before_any_device_work():
+    """ps_grid_sort checks its arguments on the host first: null outputs, an
+    empty grid axis, a non-positive or non-finite cell, 2^31 cells."""
+    lib = capi.load()
+    g = capi.PsGridDesc()
+    g.origin[:] = (0.0, 0.0, 0.0)
+    g.cell = 1.0
+    g.grid[:] = (4, 4, 4)
+    dummy = 0x1000  # never dereferenced: every call below fails its checks
+    assert lib.ps_grid_sort(None, 0, None, None, None, dummy, dummy, None, None) == capi.PS_ERR_INVALID_ARGUMENT
+    assert lib.ps_grid_sort(ctypes.byref(g), 8, None, dummy, dummy, dummy, dummy, None, None) == \
+        capi.PS_ERR_INVALID_ARGUMENT
+    assert b"null pointer" in lib.ps_last_error(None)
+    for grid, cell in (((4, 0, 4), 1.0), ((4, 4, 4), 0.0), ((4, 4, 4), float("nan")),
+                       ((2048, 2048, 1024), 1.0)):
+        g.grid[:] = grid
+        g.cell = cell
+        assert lib.ps_grid_sort(ctypes.byref(g), 1, dummy, dummy, dummy, dummy, dummy, None, None) == \
+            capi.PS_ERR_INVALID_ARGUMENT, (grid, cell)
