@@ -555,6 +555,20 @@ int ps_grid_sort(const ps_grid_desc *grid, uint32_t n, const float *pos, uint32_
                  uint32_t *sorted_index, uint32_t *cell_start, uint32_t *cell_end,
                  float *sorted_pos, void *stream);
 
+/* ps_grid_sort without any allocation: the workspace is a fixed device array
+ * inside the library (one per GPU, held until the process ends), for callers
+ * that cannot allocate (code under HIP graph capture, tight serving loops).
+ * No counterpart in the reference.  Limits: n <= PS_GRID_NOALLOC_MAX_PARTICLES
+ * and grid cells <= PS_GRID_NOALLOC_MAX_CELLS, else PS_ERR_CAPACITY.  Every
+ * call on one GPU shares that GPU's array, so calls that overlap on one GPU
+ * (on different streams, or from different threads) race; calls ordered on
+ * one stream do not. */
+#define PS_GRID_NOALLOC_MAX_PARTICLES (1u << 20)
+#define PS_GRID_NOALLOC_MAX_CELLS (1u << 21)
+int ps_grid_sort_noalloc(const ps_grid_desc *grid, uint32_t n, const float *pos, uint32_t *cell_key,
+                         uint32_t *sorted_index, uint32_t *cell_start, uint32_t *cell_end,
+                         float *sorted_pos, void *stream);
+
 /* ---- autotuning -------------------------------------------------------- */
 
 /* Choose the workgroup size (64, 128, 256 or 512) of each per-particle

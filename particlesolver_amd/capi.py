@@ -23,6 +23,8 @@ PHASE_RIGID = 4
 PS_OK = 0
 PS_ERR_INVALID_ARGUMENT = -1
 PS_ERR_CAPACITY = -2
+PS_GRID_NOALLOC_MAX_PARTICLES = 1 << 20
+PS_GRID_NOALLOC_MAX_CELLS = 1 << 21
 PS_ERR_DEVICE = -3
 PS_ERR_NO_DEVICE = -4
 PS_ERR_STATE = -5
@@ -188,6 +190,7 @@ PROTOTYPES = {
     "ps_get_tuning": (_int, [_P, _P]),
     "ps_set_tuning": (_int, [_P, _P]),
     "ps_grid_sort": (_int, [ctypes.POINTER(PsGridDesc), _u32, _P, _P, _P, _P, _P, _P, _P]),
+    "ps_grid_sort_noalloc": (_int, [ctypes.POINTER(PsGridDesc), _u32, _P, _P, _P, _P, _P, _P, _P]),
 }
 
 _lib = None

@@ -16,8 +16,14 @@
 //      cell with a smaller index (a stable sort), the outputs gathered there;
 //   then grid_bounds_k turns the offsets into cell_start / cell_end.
 // Arrival order depends on scheduling; the final order does not.
+//
+// ps_grid_sort_noalloc runs the same launches over `scratch`, a static
+// device array of the library (one copy per GPU), so that it allocates
+// nothing; its address on each GPU is looked up once with
+// hipGetSymbolAddress and kept.
 
 #include <cmath>
+#include <mutex>
 
 #include "ps_device.hpp"
 
@@ -190,6 +196,48 @@ hipError_t grid_run(hipStream_t st, const GridK &g, uint32_t cells, uint32_t n, 
 }  // namespace
 }  // namespace ps
 
+// The allocation-free variant's workspace, sized for
+// PS_GRID_NOALLOC_MAX_PARTICLES inputs on PS_GRID_NOALLOC_MAX_CELLS cells
+// (28 MiB).  In host code `scratch` names a host-side stand-in, not the
+// device address: only the pointer from hipGetSymbolAddress is used.
+static constexpr size_t kNoAllocBytes =
+    ps::grid_work_bytes(PS_GRID_NOALLOC_MAX_PARTICLES, PS_GRID_NOALLOC_MAX_CELLS);
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+namespace ps {
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+// The address of `scratch` on the current device, looked up on the first
+// call there (with that device current) and kept.
+int scratch_on_current_device(void **out)
+{
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess)
+        return fail_stateless(PS_ERR_NO_DEVICE, std::string("ps_grid_sort_noalloc: hipGetDevice: ") +
+                                                    hipGetErrorString(e));
+    if (dev < 0 || dev >= kMaxDevices)
+        return fail_stateless(PS_ERR_DEVICE, "ps_grid_sort_noalloc: device index beyond 63");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail_stateless(PS_ERR_DEVICE, std::string("ps_grid_sort_noalloc: hipGetSymbolAddress: ") +
+                                                     hipGetErrorString(e));
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return PS_OK;
+}
+
+}  // namespace
+}  // namespace ps
+
 using namespace ps;
 
 extern "C" int ps_grid_sort(const ps_grid_desc *grid, uint32_t n, const float *pos, uint32_t *cell_key,
@@ -212,5 +260,30 @@ extern "C" int ps_grid_sort(const ps_grid_desc *grid, uint32_t n, const float *p
     if (e == hipSuccess) e = ef;
     if (e != hipSuccess)
         return fail_stateless(PS_ERR_DEVICE, std::string("ps_grid_sort: ") + hipGetErrorString(e));
+    return PS_OK;
+}
+
+extern "C" int ps_grid_sort_noalloc(const ps_grid_desc *grid, uint32_t n, const float *pos, uint32_t *cell_key,
+                                    uint32_t *sorted_index, uint32_t *cell_start, uint32_t *cell_end,
+                                    float *sorted_pos, void *stream)
+{
+    GridK g;
+    uint32_t cells = 0;
+    if (int rc = grid_check(grid, n, pos, cell_key, sorted_index, cell_start, cell_end, g, cells)) return rc;
+    if (n > PS_GRID_NOALLOC_MAX_PARTICLES)
+        return fail_stateless(PS_ERR_CAPACITY, "ps_grid_sort_noalloc: more than PS_GRID_NOALLOC_MAX_PARTICLES "
+                                               "(1048576) inputs; use ps_grid_sort");
+    if (cells > PS_GRID_NOALLOC_MAX_CELLS)
+        return fail_stateless(PS_ERR_CAPACITY, "ps_grid_sort_noalloc: more than PS_GRID_NOALLOC_MAX_CELLS "
+                                               "(2097152) cells; use ps_grid_sort");
+    void *ws = nullptr;
+    if (int rc = scratch_on_current_device(&ws)) return rc;
+    // the layout for (n, cells) is a prefix-sized carve of the one for the
+    // limits: grid_work_bytes grows with both
+    const hipError_t e = grid_run(static_cast<hipStream_t>(stream), g, cells, n,
+                                  reinterpret_cast<const float4 *>(pos), carve(ws, n, cells), cell_key,
+                                  sorted_index, cell_start, cell_end, reinterpret_cast<float4 *>(sorted_pos));
+    if (e != hipSuccess)
+        return fail_stateless(PS_ERR_DEVICE, std::string("ps_grid_sort_noalloc: ") + hipGetErrorString(e));
     return PS_OK;
 }
