@@ -164,8 +164,10 @@ typedef struct ps_derived {
     float wall_k;      /* sqrt(wall_mu_k) * r                                     */
     float jitter;      /* wall_jitter * r                                         */
     uint32_t num_cells;
-    float w_ratio;     /* 1 / (h^2 - (dq h)^2)^3, so W(r)/W(dq h) = w_ratio (h^2 - r^2)^3 */
-    uint32_t reserved[2];
+    float w_scale;     /* 1 / (h^2 - (dq h)^2)                                    */
+    float w_scale_h2;  /* h^2 / (h^2 - (dq h)^2): s = w_scale_h2 - w_scale r^2 gives
+                          W(r)/W(dq h) = s^3                                      */
+    uint32_t reserved[1];
 } ps_derived;
 
 typedef struct ps_status {

@@ -188,10 +188,11 @@ class Oracle:
         return s
 
     def derived(self):
-        out = np.empty(13, self.dtype)
+        out = np.empty(14, self.dtype)
         self._f("orc_derive")(ctypes.byref(self.params), _ptr(out))
         return dict(zip(["h2", "poly6", "spiky", "w_zero", "w_dq", "inv_w_dq", "tensile_k",
-                         "search_r2", "collide_r2", "wall_s", "wall_k", "jitter", "w_ratio"], out))
+                         "search_r2", "collide_r2", "wall_s", "wall_k", "jitter", "w_scale", "w_scale_h2"],
+                        out))
 
     def set_state(self, x=None, v=None):
         """positions / velocities in insertion order at the oracle's precision"""

@@ -105,7 +105,9 @@ __attribute__((visibility("default"))) real SFX(orc_exp)(real x) { return EXP(x)
 
 typedef struct orc_derived {
     real h, h2, poly6, spiky, w_zero, w_dq, inv_w_dq, search_r2, collide_r2, collide_dist;
-    real w_ratio; /* 1/(h^2 - (dq h)^2)^3: W(r)/W(dq h) = w_ratio (h^2 - r^2)^3 */
+    /* 1/(h^2 - (dq h)^2) and h^2 times it: with s = FMA(-w_scale, r^2,
+     * w_scale_h2), W(r)/W(dq h) = s^3 */
+    real w_scale, w_scale_h2;
     real wall_s, wall_k, jitter, radius, relax, k_corr, xsph_c, s_solid, mu_s, mu_k;
     uint32_t n_corr, num_cells;
 } orc_derived;
@@ -128,7 +130,8 @@ static void derive(const ps_params *p, orc_derived *d)
     d->w_zero = R(poly6 * h6);
     d->w_dq = R(poly6 * tq * tq * tq);
     d->inv_w_dq = R(1.0 / (poly6 * tq * tq * tq));
-    d->w_ratio = R(1.0 / (tq * tq * tq));
+    d->w_scale = R(1.0 / tq);
+    d->w_scale_h2 = R(h2 / tq);
     d->search_r2 = R(sr * sr);
     d->collide_r2 = R((double)p->collide_dist * (double)p->collide_dist);
     d->collide_dist = R(p->collide_dist);
@@ -151,15 +154,16 @@ static void derive(const ps_params *p, orc_derived *d)
 }
 
 /* Derived constants (FP64 values when ORC_DOUBLE) for pinning tests:
- * out[0..12] = h2, poly6, spiky, w_zero, w_dq, inv_w_dq, tensile_k,
- * search_r2, collide_r2, wall_s, wall_k, jitter, w_ratio. */
+ * out[0..13] = h2, poly6, spiky, w_zero, w_dq, inv_w_dq, tensile_k,
+ * search_r2, collide_r2, wall_s, wall_k, jitter, w_scale, w_scale_h2. */
 ORC_EXPORT void SFX(orc_derive)(const ps_params *p, real *out)
 {
     orc_derived d;
     derive(p, &d);
     out[0] = d.h2; out[1] = d.poly6; out[2] = d.spiky; out[3] = d.w_zero; out[4] = d.w_dq;
     out[5] = d.inv_w_dq; out[6] = d.k_corr; out[7] = d.search_r2; out[8] = d.collide_r2;
-    out[9] = d.wall_s; out[10] = d.wall_k; out[11] = d.jitter; out[12] = d.w_ratio;
+    out[9] = d.wall_s; out[10] = d.wall_k; out[11] = d.jitter; out[12] = d.w_scale;
+    out[13] = d.w_scale_h2;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -701,7 +705,8 @@ static real ipow(real x, uint32_t n)
  *   or k with PS_FLAG_UNSCALED_TENSILE);
  *   r^2 < 1e-8: gradW = (0, -0.001 spiky h^2, 0)   (:625-626).
  * Per pair, with t = h^2 - r^2, 1/r = RSQ(r^2) and q = h - r =
- * FMA(-r^2, 1/r, h): W(r)/W(dq h) = w_ratio t^3, coef = FMA(-k,
+ * FMA(-r^2, 1/r, h): W(r)/W(dq h) = s^3 with s = (h^2 - r^2)/(h^2 -
+ * (dq h)^2) = FMA(-w_scale, r^2, w_scale_h2), coef = FMA(-k,
  * (W/W(dq h))^n, l_i + l_j), the term -coef q^2 (1/r) d (a degenerate
  * pair: -coef 0.001 h^2 in y); spiky multiplies the finished sum.  In two
  * halves of the table, as lambda_sums.  Static neighbours are skipped. */
@@ -720,8 +725,8 @@ static void fluid_sum(const orc_sys *s, const real *P, uint32_t i, real *f)
         real dx = px - P[4 * j], dy = py - P[4 * j + 1], dz = pz - P[4 * j + 2];
         real r2 = DIST2(dx, dy, dz);
         if (!(r2 < d->h2)) continue;
-        real t = d->h2 - r2;
-        real x = d->w_ratio * ((t * t) * t);
+        real sw = FMA(-d->w_scale, r2, d->w_scale_h2);
+        real x = (sw * sw) * sw;
         real coef = FMA(-d->k_corr, ipow(x, d->n_corr), li + P[4 * j + 3]);
         if (r2 < DEGENERATE_R2) {
             hy[hf] = hy[hf] - (coef * gdeg);

@@ -106,7 +106,8 @@ class PsDerived(ctypes.Structure):
     _fields_ = [(n, ctypes.c_float) for n in
                 ("h2", "poly6", "spiky", "w_zero", "w_dq", "inv_w_dq", "tensile_k", "search_r2",
                  "collide_r2", "wall_s", "wall_k", "jitter")] + [
-        ("num_cells", ctypes.c_uint32), ("w_ratio", ctypes.c_float), ("reserved", ctypes.c_uint32 * 2)]
+        ("num_cells", ctypes.c_uint32), ("w_scale", ctypes.c_float), ("w_scale_h2", ctypes.c_float),
+        ("reserved", ctypes.c_uint32 * 1)]
 
 
 class PsSceneDesc(ctypes.Structure):

@@ -47,7 +47,9 @@ struct __align__(16) SceneDev {
 // Constants handed to every kernel by value (kernarg segment).
 struct KConst {
     float h, h2, poly6, spiky, w_zero, w_dq, inv_w_dq, search_r2, collide_r2, collide_dist;
-    float w_ratio;  // 1/(h^2 - (dq h)^2)^3: W(r)/W(dq h) = w_ratio (h^2 - r^2)^3
+    // 1/(h^2 - (dq h)^2) and h^2 times it: with s = fma(-w_scale, r^2,
+    // w_scale_h2), W(r)/W(dq h) = s^3
+    float w_scale, w_scale_h2;
     float wall_s, wall_k, jitter, radius, relax, k_corr, xsph_c, s_solid, mu_s, mu_k;
     float gravity[3];
     float box_min[3], box_max[3];

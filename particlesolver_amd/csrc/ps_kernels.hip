@@ -854,9 +854,10 @@ __global__ __launch_bounds__(kMaxBlock) void solve_k(DevBuffers b, KConst k, uin
     // a6: PBF delta-p with tensile correction s_corr = -k h^2 (W/W(dq h))^n
     // (solveFluidsD :596-642); k.k_corr already holds k h^2 (or k).  Per
     // pair, with t = h^2 - r^2, 1/r = rsq(r^2) and q = h - r =
-    // fma(-r^2, 1/r, h): W/W(dq h) = w_ratio t^3, coef = fma(-k, (W/W(dq
-    // h))^n, l_i + l_j), and the term -coef q^2 (1/r) d; spiky multiplies
-    // the finished sum
+    // fma(-r^2, 1/r, h): W/W(dq h) = s^3 with s = (h^2 - r^2)/(h^2 - (dq h)^2)
+    // = fma(-w_scale, r^2, w_scale_h2), coef = fma(-k, (W/W(dq h))^n,
+    // l_i + l_j), and the term -coef q^2 (1/r) d; spiky multiplies the
+    // finished sum
     if (F & F_FLUID) {
         if (ph == PS_PHASE_FLUID) {
             const float li = p.w, ir0 = 1.0f / a.rho0;
@@ -869,8 +870,8 @@ __global__ __launch_bounds__(kMaxBlock) void solve_k(DevBuffers b, KConst k, uin
                 bool deg;
             };
             auto coef_of = [&](float r2, float lj) {
-                const float t = k.h2 - r2;
-                const float x = k.w_ratio * ((t * t) * t);
+                const float sw = __builtin_fmaf(-k.w_scale, r2, k.w_scale_h2);
+                const float x = (sw * sw) * sw;
                 return __builtin_fmaf(-k.k_corr, ipow(x, NC ? NC : k.n_corr), li + lj);
             };
             auto term = [&](Sums &s, float coef, float r2, float dx, float dy, float dz) {

@@ -65,13 +65,13 @@ def test_derived_constants_library_equals_oracle():
         d = derive(sc.params)
         o = Oracle(sc.params, "f32").derived()
         for k in ("h2", "poly6", "spiky", "w_zero", "w_dq", "inv_w_dq", "tensile_k", "search_r2",
-                  "collide_r2", "wall_s", "wall_k", "jitter", "w_ratio"):
+                  "collide_r2", "wall_s", "wall_k", "jitter", "w_scale", "w_scale_h2"):
             assert np.float32(getattr(d, k)) == np.float32(o[k]), k
 
 
 def test_tensile_ratio_constant():
-    """w_ratio (h^2 - r^2)^3 is W(r)/W(dq h): against poly6 and W(dq h)
-    formed independently in FP64, at several r."""
+    """(w_scale_h2 - w_scale r^2)^3 is W(r)/W(dq h): against poly6 and
+    W(dq h) formed independently in FP64, at several r."""
     p = scenes.dam_break(8, 4, 8).params
     d = Oracle(p, "f64").derived()
     h = float(np.float32(p.h))
@@ -80,8 +80,10 @@ def test_tensile_ratio_constant():
     w_dq = poly6 * (h * h - dq * dq) ** 3
     for r in (0.0, 0.3 * h, dq, 0.9 * h):
         w = poly6 * (h * h - r * r) ** 3
-        assert abs(d["w_ratio"] * (h * h - r * r) ** 3 - w / w_dq) <= 1e-12 * max(1.0, w / w_dq)
-    assert abs(d["w_ratio"] * (h * h - dq * dq) ** 3 - 1.0) < 1e-12
+        s = d["w_scale_h2"] - d["w_scale"] * r * r
+        assert abs(s ** 3 - w / w_dq) <= 1e-12 * max(1.0, w / w_dq)
+    assert abs((d["w_scale_h2"] - d["w_scale"] * dq * dq) ** 3 - 1.0) < 1e-12
+    assert abs(d["w_scale_h2"] - d["w_scale"] * h * h) < 1e-12
 
 
 def test_wall_uniform_is_splitmix64():
