@@ -318,14 +318,17 @@ __global__ __launch_bounds__(kAdjBlock) void adj_pre_k(DevBuffers b, KConst k, A
     }
 }
 
-// dL/dl -> dL/dC = -lb / (D + eps), dL/dD = -lb l / (D + eps); rho0 terms
+// dL/dl -> dL/dC = -lb / (D + eps), dL/dD = -lb l / (D + eps); rho0 terms.
+// Two lanes per particle: lane hh sums lb over half hh of the table (its
+// chunks 2m + hh), then lb = half 0 + half 1, and lane 0 finishes.
 template <bool UNIFORM>
 __global__ __launch_bounds__(kAdjBlock) void adj_lambda_k(DevBuffers b, KConst k, AdjBuffers a, uint32_t n)
 {
-    const uint32_t i = blockIdx.x * kAdjBlock + threadIdx.x;
-    if (i >= n) return;
+    const uint32_t i = blockIdx.x * (kAdjBlock / 2) + (threadIdx.x >> 1);
+    const uint32_t hh = threadIdx.x & 1u;
+    if (i >= n) return;  // both lanes of a particle leave together
     if (!UNIFORM && b.A[i].phase != PS_PHASE_FLUID) {
-        a.CD[i] = make_float2(0.0f, 0.0f);
+        if (!hh) a.CD[i] = make_float2(0.0f, 0.0f);
         return;
     }
     const float4 pi = a.Pit[i];
@@ -352,14 +355,17 @@ __global__ __launch_bounds__(kAdjBlock) void adj_lambda_k(DevBuffers b, KConst k
     };
     const uint32_t c = b.ncnt[i];
     if (UNIFORM) {
-        for_neighbors(
-            b, i, c, [&](uint32_t j) { return GLam{a.Pit[j], a.M[j]}; },
+        for_neighbors_half(
+            b, i, c, hh, [&](uint32_t j) { return GLam{a.Pit[j], a.M[j]}; },
             [&](uint32_t, const GLam &d) { body(d.p, d.m, 1.0f, PS_PHASE_FLUID); });
     } else {
-        for_neighbors(
-            b, i, c, [&](uint32_t j) { return GLamAttr{a.Pit[j], a.M[j], a.AB[j]}; },
+        for_neighbors_half(
+            b, i, c, hh, [&](uint32_t j) { return GLamAttr{a.Pit[j], a.M[j], a.AB[j]}; },
             [&](uint32_t, const GLamAttr &d) { body(d.p, d.m, d.a.w, d.a.phase); });
     }
+    const float lo = __shfl_xor(lb, 1, 64);
+    if (hh) return;
+    lb = lb + lo;  // half 0 + half 1
     const float4 sd = a.SD[i];
     const float den = sd.w + k.relax;
     const float lam = pi.w;
@@ -372,16 +378,19 @@ __global__ __launch_bounds__(kAdjBlock) void adj_lambda_k(DevBuffers b, KConst k
     a.r0b[i] = r;
 }
 
-// dL/dp before the iteration
+// dL/dp before the iteration.  Two lanes per particle: lane hh sums the
+// pair and contact terms of half hh of the table (half 0 starts from dL/dy),
+// then acc = half 0 + half 1, and lane 0 adds the distance constraints.
 template <bool UNIFORM>
 __global__ __launch_bounds__(kAdjBlock) void adj_position_k(DevBuffers b, KConst k, AdjBuffers a, uint32_t n)
 {
-    const uint32_t p = blockIdx.x * kAdjBlock + threadIdx.x;
-    if (p >= n) return;
+    const uint32_t p = blockIdx.x * (kAdjBlock / 2) + (threadIdx.x >> 1);
+    const uint32_t hh = threadIdx.x & 1u;
+    if (p >= n) return;  // both lanes of a particle leave together
     const float4 pp = a.Pit[p];
     const Attr ap = b.A[p];
     const float4 Yp = a.Y[p];
-    float acc[3] = {Yp.x, Yp.y, Yp.z};
+    float acc[3] = {hh ? 0.0f : Yp.x, hh ? 0.0f : Yp.y, hh ? 0.0f : Yp.z};
     float ydp[3];
     yd_of(k, Yp, ydp);
     const bool pf = UNIFORM || ap.phase == PS_PHASE_FLUID;
@@ -440,15 +449,21 @@ __global__ __launch_bounds__(kAdjBlock) void adj_position_k(DevBuffers b, KConst
     };
     const uint32_t c = b.ncnt[p];
     if (UNIFORM) {
-        for_neighbors_chunked(
-            b, p, c, [&](uint32_t j) { return GPos{a.Pit[j], a.M[j], a.SD[j], a.CD[j]}; },
+        for_neighbors_half_chunked(
+            b, p, c, hh, [&](uint32_t j) { return GPos{a.Pit[j], a.M[j], a.SD[j], a.CD[j]}; },
             [&](uint32_t, const GPos &d) { body(d.p, d.m, d.sd, d.cd, ap.w, PS_PHASE_FLUID, 0u, massp); });
     } else {
-        for_neighbors_chunked(
-            b, p, c, [&](uint32_t j) { return GPosAttr{a.Pit[j], a.M[j], a.SD[j], a.CD[j], a.AB[j]}; },
+        for_neighbors_half_chunked(
+            b, p, c, hh, [&](uint32_t j) { return GPosAttr{a.Pit[j], a.M[j], a.SD[j], a.CD[j], a.AB[j]}; },
             [&](uint32_t, const GPosAttr &d) {
                 body(d.p, d.m, d.sd, d.cd, d.a.w, d.a.phase, d.a.nc, d.a.mass);
             });
+    }
+    {
+        float lo[3];
+        for (int q2 = 0; q2 < 3; q2++) lo[q2] = __shfl_xor(acc[q2], 1, 64);
+        if (hh) return;
+        for (int q2 = 0; q2 < 3; q2++) acc[q2] = acc[q2] + lo[q2];  // half 0 + half 1
     }
     // distance constraints incident to p
     if (a.dist) {
@@ -656,17 +671,27 @@ hipError_t launch_adj(hipStream_t st, AdjKernel which, const DevBuffers &b, cons
         else                                                                                       \
             hipLaunchKernelGGL(kern<false>, grid, block, 0, st, __VA_ARGS__);                      \
     } while (0)
+    // two lanes per particle: kAdjBlock / 2 particles per block
+    const dim3 grid2((count + kAdjBlock / 2 - 1) / (kAdjBlock / 2));
+#define PS_ADJ_LAUNCH2(kern, ...)                                                                 \
+    do {                                                                                           \
+        if (uniform)                                                                               \
+            hipLaunchKernelGGL(kern<true>, grid2, block, 0, st, __VA_ARGS__);                      \
+        else                                                                                       \
+            hipLaunchKernelGGL(kern<false>, grid2, block, 0, st, __VA_ARGS__);                     \
+    } while (0)
     switch (which) {
     case ADJ_XSPH: PS_ADJ_LAUNCH(adj_xsph_k, b, k, a, count, dt); break;
     case ADJ_FINALIZE: PS_ADJ_LAUNCH(adj_finalize_k, b, k, a, count, dt); break;
     case ADJ_PRE: PS_ADJ_LAUNCH(adj_pre_k, b, k, a, count); break;
-    case ADJ_LAMBDA: PS_ADJ_LAUNCH(adj_lambda_k, b, k, a, count); break;
-    case ADJ_POSITION: PS_ADJ_LAUNCH(adj_position_k, b, k, a, count); break;
+    case ADJ_LAMBDA: PS_ADJ_LAUNCH2(adj_lambda_k, b, k, a, count); break;
+    case ADJ_POSITION: PS_ADJ_LAUNCH2(adj_position_k, b, k, a, count); break;
     case ADJ_STIFFNESS: hipLaunchKernelGGL(adj_stiffness_k, grid, block, 0, st, b, k, a, count); break;
     case ADJ_PREDICT: hipLaunchKernelGGL(adj_predict_k, grid, block, 0, st, b, k, a, count, dt); break;
     default: return hipErrorInvalidValue;
     }
 #undef PS_ADJ_LAUNCH
+#undef PS_ADJ_LAUNCH2
     return hipGetLastError();
 }
 

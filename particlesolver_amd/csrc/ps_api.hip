@@ -36,6 +36,11 @@ struct ps_system {
     KConst kc;
     DevBuffers b;
     hipStream_t stream = nullptr;
+    // the reverse pass's own stream (created by the first ps_step_backward):
+    // it reverses one segment while `stream` recomputes the one before
+    hipStream_t rstream = nullptr;
+    // the stream timed launches are enqueued on (null: `stream`)
+    hipStream_t tstream = nullptr;
     uint32_t n = 0, cap = 0, ncell = 0, kmax = 0, tiles = 0;
     // scene composition
     bool uniform = true;     // every particle fluid with one inverse mass
@@ -176,7 +181,8 @@ void derive_constants(const ps_params &p, ps_derived &d)
     d.w_zero = (float)(poly6 * h6);
     d.w_dq = (float)(poly6 * tq * tq * tq);
     d.inv_w_dq = (float)(1.0 / (poly6 * tq * tq * tq));
-    d.w_ratio = (float)(1.0 / (tq * tq * tq));
+    d.w_scale = (float)(1.0 / tq);
+    d.w_scale_h2 = (float)(h2 / tq);
     // The tensile term is added to lambda (units length^2); the paper's
     // dimensionless k measures lengths in units of h, hence k h^2.  The
     // reference takes K_P as it is (integration_kernel.cuh:633).
@@ -199,7 +205,8 @@ void fill_kconst(const ps_params &p, const ps_derived &d, KConst &k)
     k.w_zero = d.w_zero;
     k.w_dq = d.w_dq;
     k.inv_w_dq = d.inv_w_dq;
-    k.w_ratio = d.w_ratio;
+    k.w_scale = d.w_scale;
+    k.w_scale_h2 = d.w_scale_h2;
     k.search_r2 = d.search_r2;
     k.collide_r2 = d.collide_r2;
     k.collide_dist = p.collide_dist;
@@ -410,14 +417,14 @@ int timed_begin(ps_system *s, int id, TimedLaunch &tl)
             HIP_TRY(s, hipEventCreate(e));
         }
     }
-    HIP_TRY(s, hipEventRecord(tl.start, s->stream));
+    HIP_TRY(s, hipEventRecord(tl.start, s->tstream ? s->tstream : s->stream));
     return PS_OK;
 }
 
 int timed_end(ps_system *s, TimedLaunch &tl)
 {
     if (!tl.start) return PS_OK;
-    HIP_TRY(s, hipEventRecord(tl.stop, s->stream));
+    HIP_TRY(s, hipEventRecord(tl.stop, s->tstream ? s->tstream : s->stream));
     s->pending.push_back(tl);
     return PS_OK;
 }
@@ -426,6 +433,7 @@ int timing_collect(ps_system *s)
 {
     if (s->pending.empty()) return PS_OK;
     HIP_TRY(s, hipStreamSynchronize(s->stream));
+    if (s->rstream) HIP_TRY(s, hipStreamSynchronize(s->rstream));
     for (TimedLaunch &tl : s->pending) {
         float ms = 0.0f;
         HIP_TRY(s, hipEventElapsedTime(&ms, tl.start, tl.stop));
@@ -1072,6 +1080,7 @@ int ps_destroy(ps_system *s)
     for (hipEvent_t e : s->free_events) (void)hipEventDestroy(e);
     drop_graph(s);
     free_all(s);
+    if (s->rstream) (void)hipStreamDestroy(s->rstream);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
     return PS_OK;
@@ -1878,10 +1887,12 @@ int adj_taped_substep(ps_system *s, float dt, size_t t)
     return rc;
 }
 
-// Reverse of the substep on tape slot t: insertion-order gx/gv of its end
-// state in, of its start state out; rho0, stiffness and wall_k terms
-// accumulate.
-int adj_reverse(ps_system *s, float dt, size_t t)
+// Reverse of the substep on tape slot t, enqueued on st: insertion-order
+// gx/gv of its end state in, of its start state out; rho0, stiffness and
+// wall_k terms accumulate.  It reads the tape slot, the constraint tables
+// and its own work arrays, never the live particle state, so a recompute
+// may run beside it on another stream.
+int adj_reverse(ps_system *s, float dt, size_t t, hipStream_t st)
 {
     ps_system::AdjWork &w = s->adj;
     const size_t c = w.cap, I = s->prm.iterations;
@@ -1912,8 +1923,13 @@ int adj_reverse(ps_system *s, float dt, size_t t)
     a.pb = cur;
     const uint32_t n = s->n, ndist = (uint32_t)s->dist_rest.size();
     const bool uni = s->uniform;
-    LAUNCH(s, PS_K_ADJ_XSPH, ps::launch_adj(s->stream, ps::ADJ_XSPH, bt, s->kc, a, n, dt, uni));
-    LAUNCH(s, PS_K_ADJ_FINALIZE, ps::launch_adj(s->stream, ps::ADJ_FINALIZE, bt, s->kc, a, n, dt, uni));
+    s->tstream = st;  // LAUNCH times these launches on st
+    struct Restore {
+        ps_system *s;
+        ~Restore() { s->tstream = nullptr; }
+    } restore{s};
+    LAUNCH(s, PS_K_ADJ_XSPH, ps::launch_adj(st, ps::ADJ_XSPH, bt, s->kc, a, n, dt, uni));
+    LAUNCH(s, PS_K_ADJ_FINALIZE, ps::launch_adj(st, ps::ADJ_FINALIZE, bt, s->kc, a, n, dt, uni));
     for (size_t it = I; it-- > 0;) {
         const size_t q = (t * I + it) * c;
         a.Pit = w.tPit + q;
@@ -1924,16 +1940,16 @@ int adj_reverse(ps_system *s, float dt, size_t t)
         a.tf = w.tF + q;
         a.pbn = cur;
         a.pb = nxt;
-        LAUNCH(s, PS_K_ADJ_PRE, ps::launch_adj(s->stream, ps::ADJ_PRE, bt, s->kc, a, n, dt, uni));
-        LAUNCH(s, PS_K_ADJ_LAMBDA, ps::launch_adj(s->stream, ps::ADJ_LAMBDA, bt, s->kc, a, n, dt, uni));
-        LAUNCH(s, PS_K_ADJ_POSITION, ps::launch_adj(s->stream, ps::ADJ_POSITION, bt, s->kc, a, n, dt, uni));
+        LAUNCH(s, PS_K_ADJ_PRE, ps::launch_adj(st, ps::ADJ_PRE, bt, s->kc, a, n, dt, uni));
+        LAUNCH(s, PS_K_ADJ_LAMBDA, ps::launch_adj(st, ps::ADJ_LAMBDA, bt, s->kc, a, n, dt, uni));
+        LAUNCH(s, PS_K_ADJ_POSITION, ps::launch_adj(st, ps::ADJ_POSITION, bt, s->kc, a, n, dt, uni));
         if (ndist)
             LAUNCH(s, PS_K_ADJ_STIFFNESS,
-                   ps::launch_adj(s->stream, ps::ADJ_STIFFNESS, bt, s->kc, a, ndist, dt, uni));
+                   ps::launch_adj(st, ps::ADJ_STIFFNESS, bt, s->kc, a, ndist, dt, uni));
         std::swap(cur, nxt);
     }
     a.pb = cur;
-    LAUNCH(s, PS_K_ADJ_PREDICT, ps::launch_adj(s->stream, ps::ADJ_PREDICT, bt, s->kc, a, n, dt, uni));
+    LAUNCH(s, PS_K_ADJ_PREDICT, ps::launch_adj(st, ps::ADJ_PREDICT, bt, s->kc, a, n, dt, uni));
     return PS_OK;
 }
 
@@ -1944,12 +1960,17 @@ int step_backward_impl(ps_system *s, float dt, uint32_t S, uint32_t K, const flo
     const uint32_t n = s->n;
     // segments of K substeps (K = 0: one segment, the whole horizon); the
     // last is taped by the forward pass, the others recomputed from their
-    // checkpoints and taped when the reverse pass reaches them
+    // checkpoints and taped when the reverse pass reaches them.  With more
+    // than one segment the tape holds two: segment g is taped into buffer
+    // g % 2, so the recompute of segment g - 1 (on `stream`) runs while the
+    // reverse of segment g (on `rstream`) reads the other buffer
     const uint32_t seg = K ? K : S;
     const uint32_t nseg = (S + seg - 1) / seg;
     const size_t fin = nseg;  // checkpoint of the final state
+    const size_t nbuf = nseg > 1 ? 2 : 1;
     int rc;
-    if ((rc = adj_reserve(s, nseg + 1, seg, s->prm.iterations ? s->prm.iterations : 1))) return rc;
+    if ((rc = adj_reserve(s, nseg + 1, nbuf * seg, s->prm.iterations ? s->prm.iterations : 1))) return rc;
+    if (!s->rstream) HIP_TRY(s, hipStreamCreateWithFlags(&s->rstream, hipStreamNonBlocking));
     ps_system::AdjWork &w = s->adj;
     std::vector<uint64_t> sub(nseg + 1), its(nseg + 1);
     auto save = [&](size_t k) {
@@ -1965,7 +1986,7 @@ int step_backward_impl(ps_system *s, float dt, uint32_t S, uint32_t K, const flo
     auto tape_segment = [&](uint32_t g) {
         const uint32_t s0 = g * seg, s1 = std::min(S, s0 + seg);
         for (uint32_t t = s0; t < s1; t++)
-            if (int r = adj_taped_substep(s, dt, t - s0)) return r;
+            if (int r = adj_taped_substep(s, dt, (g % nbuf) * seg + (t - s0))) return r;
         return (int)PS_OK;
     };
     for (uint32_t g = 0; g < nseg; g++) {
@@ -1998,15 +2019,42 @@ int step_backward_impl(ps_system *s, float dt, uint32_t S, uint32_t K, const flo
     HIP_TRY(s, hipMemsetAsync(w.dwk, 0, sizeof(float) * n, s->stream));
     if (!s->dist_rest.empty())
         HIP_TRY(s, hipMemsetAsync(w.sb, 0, sizeof(float) * s->dist_rest.size(), s->stream));
+    // tape_done[g]: segment g is on tape (stream); rev_done[g]: reversed (rstream)
+    struct Events {
+        std::vector<hipEvent_t> e;
+        ~Events()
+        {
+            for (hipEvent_t x : e)
+                if (x) (void)hipEventDestroy(x);
+        }
+    } tape_done, rev_done;
+    tape_done.e.assign(nseg, nullptr);
+    rev_done.e.assign(nseg, nullptr);
+    for (uint32_t g = 0; g < nseg; g++) {
+        HIP_TRY(s, hipEventCreateWithFlags(&tape_done.e[g], hipEventDisableTiming));
+        HIP_TRY(s, hipEventCreateWithFlags(&rev_done.e[g], hipEventDisableTiming));
+    }
+    auto recompute = [&](uint32_t g) {
+        if (int r = load(g)) return r;
+        if (int r = tape_segment(g)) return r;
+        HIP_TRY(s, hipEventRecord(tape_done.e[g], s->stream));
+        return (int)PS_OK;
+    };
+    HIP_TRY(s, hipEventRecord(tape_done.e[nseg - 1], s->stream));
+    if (nseg >= 2 && (rc = recompute(nseg - 2))) return rc;
     for (uint32_t g = nseg; g-- > 0;) {
         const uint32_t s0 = g * seg, s1 = std::min(S, s0 + seg);
-        if (g + 1 < nseg) {
-            if ((rc = load(g))) return rc;
-            if ((rc = tape_segment(g))) return rc;
-        }
+        HIP_TRY(s, hipStreamWaitEvent(s->rstream, tape_done.e[g], 0));
         for (uint32_t t = s1; t-- > s0;)
-            if ((rc = adj_reverse(s, dt, t - s0))) return rc;
+            if ((rc = adj_reverse(s, dt, (g % nbuf) * seg + (t - s0), s->rstream))) return rc;
+        HIP_TRY(s, hipEventRecord(rev_done.e[g], s->rstream));
+        if (g >= 2) {
+            // segment g - 2 goes into the buffer segment g has just been read from
+            HIP_TRY(s, hipStreamWaitEvent(s->stream, rev_done.e[g], 0));
+            if ((rc = recompute(g - 2))) return rc;
+        }
     }
+    HIP_TRY(s, hipStreamWaitEvent(s->stream, rev_done.e[0], 0));
     if ((rc = load(fin))) return rc;
     s->stage = 0;
     s->it_sub = 0;
@@ -2080,6 +2128,9 @@ int ps_step_backward(ps_system *s, float dt, uint32_t substeps, uint32_t checkpo
     const int rc = step_backward_impl(s, dt, substeps, K, dl_dx, dl_dv, dl_dx0, dl_dv0, dl_drho0,
                                       dl_dstiffness, dl_dmu_k);
     s->adj_active = false;
+    // a failure can leave reverse work queued: let it drain before the
+    // workspace is reused
+    if (rc && s->rstream) (void)hipStreamSynchronize(s->rstream);
     s->b.clamp_rec = nullptr;
     s->b.wall_rec = nullptr;
     s->b.tape_lq = nullptr;

@@ -151,6 +151,32 @@ __device__ __forceinline__ void for_neighbors_half(const DevBuffers &b, uint32_t
     }
 }
 
+// Half h of particle i's first c neighbours, one chunk at a time as
+// for_neighbors_chunked (four records in flight): for a kernel whose
+// records or arithmetic are large that gives each half its own lane.
+template <typename G, typename Fn>
+__device__ __forceinline__ void for_neighbors_half_chunked(const DevBuffers &b, uint32_t i, uint32_t c,
+                                                           uint32_t h, G gather, Fn fn)
+{
+    const uint32_t nch = (c + 3u) >> 2;  // chunks in use; half h has 2m + h
+    if (h >= nch) return;
+    const uint4 *nb = nbr_lane(b, i);
+    uint4 jj = nb[(size_t)h * 64];
+    for (uint32_t m = h; m < nch; m += 2u) {
+        const uint4 jn = nb[(size_t)(m + 2u < nch ? m + 2u : m) * 64];
+        auto d0 = gather(jj.x);
+        auto d1 = gather(jj.y);
+        auto d2 = gather(jj.z);
+        auto d3 = gather(jj.w);
+        const uint32_t q = m << 2;
+        fn(jj.x, d0);
+        if (q + 1 < c) fn(jj.y, d1);
+        if (q + 2 < c) fn(jj.z, d2);
+        if (q + 3 < c) fn(jj.w, d3);
+        jj = jn;
+    }
+}
+
 // The entries of half h of particle i's first c neighbours in table order,
 // one at a time: for the rarely taken recomputation of one half.
 template <typename G, typename Fn>
