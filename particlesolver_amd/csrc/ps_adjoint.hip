@@ -533,8 +533,10 @@ template <bool UNIFORM>
 __global__ __launch_bounds__(kAdjBlock) void adj_xsph_k(DevBuffers b, KConst k, AdjBuffers a, uint32_t n,
                                                      float dt)
 {
-    const uint32_t i = blockIdx.x * kAdjBlock + threadIdx.x;
-    if (i >= n) return;
+    // two lanes per particle: lane hh forms half hh of S, lane 0 finishes
+    const uint32_t i = blockIdx.x * (kAdjBlock / 2) + (threadIdx.x >> 1);
+    const uint32_t hh = threadIdx.x & 1u;
+    if (i >= n) return;  // both lanes of a particle leave together
     const Attr ai = b.A[i];
     float r0 = 0.0f;
     float Sh[3] = {0.0f, 0.0f, 0.0f};
@@ -545,53 +547,57 @@ __global__ __launch_bounds__(kAdjBlock) void adj_xsph_k(DevBuffers b, KConst k, 
         const float vbn[3] = {vb.x, vb.y, vb.z};
         const float cr = k.xsph_c / ai.rho0;
         // S exactly as finalize_k forms it: two halves of the table
-        float Sh0[3] = {0.0f, 0.0f, 0.0f}, Sh1[3] = {0.0f, 0.0f, 0.0f};
-        auto body = [&](float *S, const float4 &pj, const float4 &xj, float wj) {
+        float Sm[3] = {0.0f, 0.0f, 0.0f};
+        auto body = [&](const float4 &pj, const float4 &xj, float wj) {
             const float r2 = dist2(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z);
             if (!(r2 < k.h2)) return;
             const float t = k.h2 - r2;
             const float m = ((t * t) * t) * (1.0f / wj);
-            S[0] = __builtin_fmaf((pj.x - xj.x) - ui[0], m, S[0]);
-            S[1] = __builtin_fmaf((pj.y - xj.y) - ui[1], m, S[1]);
-            S[2] = __builtin_fmaf((pj.z - xj.z) - ui[2], m, S[2]);
+            Sm[0] = __builtin_fmaf((pj.x - xj.x) - ui[0], m, Sm[0]);
+            Sm[1] = __builtin_fmaf((pj.y - xj.y) - ui[1], m, Sm[1]);
+            Sm[2] = __builtin_fmaf((pj.z - xj.z) - ui[2], m, Sm[2]);
         };
         const uint32_t c = b.ncnt[i];
         if (UNIFORM) {
-            for_neighbors_split(
-                b, i, c, [&](uint32_t j) { return PX{b.P[j], b.X[j]}; },
-                [&](uint32_t, const PX &d) { body(Sh0, d.p, d.x, ai.w); },
-                [&](uint32_t, const PX &d) { body(Sh1, d.p, d.x, ai.w); });
+            for_neighbors_half(
+                b, i, c, hh, [&](uint32_t j) { return PX{b.P[j], b.X[j]}; },
+                [&](uint32_t, const PX &d) { body(d.p, d.x, ai.w); });
         } else {
-            for_neighbors_split(
-                b, i, c, [&](uint32_t j) { return GXA{b.P[j], b.X[j], b.A[j]}; },
+            for_neighbors_half(
+                b, i, c, hh, [&](uint32_t j) { return GXA{b.P[j], b.X[j], b.A[j]}; },
                 [&](uint32_t, const GXA &d) {
-                    if (d.a.phase == PS_PHASE_FLUID) body(Sh0, d.p, d.x, d.a.w);
-                },
-                [&](uint32_t, const GXA &d) {
-                    if (d.a.phase == PS_PHASE_FLUID) body(Sh1, d.p, d.x, d.a.w);
+                    if (d.a.phase == PS_PHASE_FLUID) body(d.p, d.x, d.a.w);
                 });
         }
-        const float S[3] = {k.poly6 * (Sh0[0] + Sh1[0]), k.poly6 * (Sh0[1] + Sh1[1]),
-                            k.poly6 * (Sh0[2] + Sh1[2])};
+        float lo[3];
+        for (int q = 0; q < 3; q++) lo[q] = __shfl_xor(Sm[q], 1, 64);
+        if (hh) return;
+        const float S[3] = {k.poly6 * (Sm[0] + lo[0]), k.poly6 * (Sm[1] + lo[1]), k.poly6 * (Sm[2] + lo[2])};
         const float vS = dot3(vbn, S);
         r0 = r0 - (((k.xsph_c / (ai.rho0 * ai.rho0)) * vS) / dt);
         for (int q = 0; q < 3; q++) Sh[q] = (vbn[q] * cr) / dt;
     }
+    if (hh) return;
     a.r0b[i] = r0;
     a.wkb[i] = 0.0f;
     a.Y[i] = make_float4(Sh[0], Sh[1], Sh[2], 0.0f);
 }
 
+// Two lanes per particle: lane hh sums the pair terms of half hh of the
+// table (half 0 starts from dL/dx and dL/dv / dt), then half 0 + half 1.
 template <bool UNIFORM>
 __global__ __launch_bounds__(kAdjBlock) void adj_finalize_k(DevBuffers b, KConst k, AdjBuffers a, uint32_t n,
                                                          float dt)
 {
-    const uint32_t p = blockIdx.x * kAdjBlock + threadIdx.x;
-    if (p >= n) return;
+    const uint32_t p = blockIdx.x * (kAdjBlock / 2) + (threadIdx.x >> 1);
+    const uint32_t hh = threadIdx.x & 1u;
+    if (p >= n) return;  // both lanes of a particle leave together
     const Attr ap = b.A[p];
     const float4 vb = a.gv[ap.orig], xb = a.gx[ap.orig];
     float ub[3] = {vb.x / dt, vb.y / dt, vb.z / dt};
     float acc[3] = {xb.x, xb.y, xb.z};
+    if (hh)
+        for (int q = 0; q < 3; q++) ub[q] = acc[q] = 0.0f;
     if (k.xsph_c != 0.0f && (UNIFORM || ap.phase == PS_PHASE_FLUID)) {
         const float4 pp = b.P[p], xp = b.X[p];
         const float up[3] = {pp.x - xp.x, pp.y - xp.y, pp.z - xp.z};
@@ -621,16 +627,26 @@ __global__ __launch_bounds__(kAdjBlock) void adj_finalize_k(DevBuffers b, KConst
         };
         const uint32_t c = b.ncnt[p];
         if (UNIFORM) {
-            for_neighbors(
-                b, p, c, [&](uint32_t j) { return GXY{b.P[j], b.X[j], a.Y[j]}; },
+            for_neighbors_half(
+                b, p, c, hh, [&](uint32_t j) { return GXY{b.P[j], b.X[j], a.Y[j]}; },
                 [&](uint32_t, const GXY &d) { body(d.p, d.x, d.y, ap.w); });
         } else {
-            for_neighbors(
-                b, p, c, [&](uint32_t j) { return GXYA{b.P[j], b.X[j], a.Y[j], b.A[j]}; },
+            for_neighbors_half(
+                b, p, c, hh, [&](uint32_t j) { return GXYA{b.P[j], b.X[j], a.Y[j], b.A[j]}; },
                 [&](uint32_t, const GXYA &d) {
                     if (d.a.phase == PS_PHASE_FLUID) body(d.p, d.x, d.y, d.a.w);
                 });
         }
+    }
+    float lu[3], la[3];
+    for (int q = 0; q < 3; q++) {
+        lu[q] = __shfl_xor(ub[q], 1, 64);
+        la[q] = __shfl_xor(acc[q], 1, 64);
+    }
+    if (hh) return;
+    for (int q = 0; q < 3; q++) {
+        ub[q] = ub[q] + lu[q];  // half 0 + half 1
+        acc[q] = acc[q] + la[q];
     }
     a.pb[p] = make_float4(acc[0] + ub[0], acc[1] + ub[1], acc[2] + ub[2], 0.0f);
     a.xsb[p] = make_float4(-ub[0], -ub[1], -ub[2], 0.0f);
@@ -681,8 +697,8 @@ hipError_t launch_adj(hipStream_t st, AdjKernel which, const DevBuffers &b, cons
             hipLaunchKernelGGL(kern<false>, grid2, block, 0, st, __VA_ARGS__);                     \
     } while (0)
     switch (which) {
-    case ADJ_XSPH: PS_ADJ_LAUNCH(adj_xsph_k, b, k, a, count, dt); break;
-    case ADJ_FINALIZE: PS_ADJ_LAUNCH(adj_finalize_k, b, k, a, count, dt); break;
+    case ADJ_XSPH: PS_ADJ_LAUNCH2(adj_xsph_k, b, k, a, count, dt); break;
+    case ADJ_FINALIZE: PS_ADJ_LAUNCH2(adj_finalize_k, b, k, a, count, dt); break;
     case ADJ_PRE: PS_ADJ_LAUNCH(adj_pre_k, b, k, a, count); break;
     case ADJ_LAMBDA: PS_ADJ_LAUNCH2(adj_lambda_k, b, k, a, count); break;
     case ADJ_POSITION: PS_ADJ_LAUNCH2(adj_position_k, b, k, a, count); break;
