@@ -184,16 +184,25 @@ def trace_report(stats_csv, trace_csv, log):
     tot = sum(float(r["TotalDurationNs"]) for r in rows)
     if line["metric"].startswith("reverse-pass"):
         return _trace_report_adjoint(rows, tot, line)
-    out = ["# Kernel durations: 1,048,576-particle PBF dam break (rocprofv3 kernel trace)", "",
+    mixed = "mixed-phase" in line["metric"]
+    if mixed:
+        title = "765,536-particle mixed scene: cloth, rigid cubes, fluid"
+        run = "--workload mixed --no-cpu --breakdown 0 --steps %d --warmup %d" % (line["steps"], line["warmup"])
+        what = ("%d substeps (%d warm-up + %d timed) of the mixed scene from its initial state "
+                "(cloth and cubes falling onto the fluid)." % (line["warmup"] + line["steps"],
+                                                                line["warmup"], line["steps"]))
+    else:
+        title = "1,048,576-particle PBF dam break"
+        run = "--no-cpu --no-adjoint --breakdown 0"
+        what = ("%d substeps (%d warm-up + %d timed) of the dam break from rest, then the exports of "
+                "the positions. Durations vary over the run: the collapse phase has the most "
+                "neighbours per particle." % (line["warmup"] + line["steps"], line["warmup"], line["steps"]))
+    out = ["# Kernel durations: %s (rocprofv3 kernel trace)" % title, "",
            "Commands (repository root on the GPU host as R, MI355X):", "", "```",
            "cd /tmp && export TMPDIR=/tmp && rocprofv3 --kernel-trace --stats --output-format csv "
-           "-d /tmp/kt -o kt -- python3 $R/bench.py --no-cpu --no-adjoint --breakdown 0 "
-           "> $R/gpurun_out/kt.log 2>&1",
+           "-d /tmp/kt -o kt -- python3 $R/bench.py %s > $R/gpurun_out/kt.log 2>&1" % run,
            "cd $R && python3 bench.py --trace-report /tmp/kt/<host>/<pid>_kernel_stats.csv "
-           "/tmp/kt/<host>/<pid>_kernel_trace.csv gpurun_out/kt.log", "```", "",
-           "%d substeps (%d warm-up + %d timed) of the dam break from rest, then the exports of "
-           "the positions. Durations vary over the run: the collapse phase has the most "
-           "neighbours per particle." % (line["warmup"] + line["steps"], line["warmup"], line["steps"]),
+           "/tmp/kt/<host>/<pid>_kernel_trace.csv gpurun_out/kt.log", "```", "", what,
            "", "| kernel | calls | avg us | min us | max us | share |", "|---|---|---|---|---|---|"]
     for r in sorted(rows, key=lambda r: -float(r["TotalDurationNs"])):
         out.append("| `%s` | %s | %.1f | %.1f | %.1f | %.1f %% |" % (
@@ -203,7 +212,8 @@ def trace_report(stats_csv, trace_csv, log):
     warm, steps = line["warmup"], line["steps"]
     out += ["", "Per-dispatch means over the bench's timed window (substeps %d-%d) from the same "
             "trace, and the bench line this run printed:" % (warm + 1, warm + steps), ""]
-    for name, per in (("solve_k", 4), ("lambda_k", 4), ("neighbors_staged_k", 1),
+    for name, per in (("solve_k", 4), ("lambda_k", 4), ("shape_k", 4),
+                      ("neighbors_k" if mixed else "neighbors_staged_k", 1),
                       ("finalize_k", 1), ("reorder_k", 1), ("predict_hash_k", 1)):
         ds = [(int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3
               for r in trace if name in r["Kernel_Name"]]
@@ -263,7 +273,7 @@ def quoted_traffic(kernel, n):
     return None
 
 
-def cpu_baseline(sc, budget_s):
+def cpu_baseline(sc, budget_s, what="dam break"):
     """FP32 CPU oracle on the same scene: as many substeps as fit the budget
     (at least one), timed after one warm-up substep."""
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
@@ -284,8 +294,8 @@ def cpu_baseline(sc, budget_s):
             break
     rate = sc.n * sc.params.iterations * done / el
     return {"value": rate, "unit": "particle-iterations/s", "cores": threads, "kind": "port",
-            "sample": "CPU oracle (C, FP32, OpenMP %d threads) on the same %d-particle dam break: "
-                      "%d substeps after 1 warm-up, %.1f s" % (threads, sc.n, done, el)}
+            "sample": "CPU oracle (C, FP32, OpenMP %d threads) on the same %d-particle %s: "
+                      "%d substeps after 1 warm-up, %.1f s" % (threads, sc.n, what, done, el)}
 
 
 # Algorithmic FP32 flops per accepted neighbour pair (distance, kernel
@@ -345,12 +355,30 @@ def copy_calibration(mib):
     print("copy calibration: 3 x %d bytes" % n, file=sys.stderr)
 
 
+def mixed_solve_bytes(K, n, n_nonfluid, n_cloth, n_links, n_rigid):
+    """Algorithmic bytes of one solve launch on the mixed scene: every
+    particle's iterate 16 + attributes 16 + count 4 + K indices + result 16;
+    the contact phases' start position 16 (friction and walls); per cloth
+    particle its constraint range 8 and incidence count 4; per link its
+    pair 8 and (rest, stiffness) 8, gathered from both ends with the id 4
+    and the partner's slot 4 (48 per link); per body member its body, member
+    index and goal 24.  Neighbour records count inside the own reads."""
+    return ((52.0 + 4.0 * K) * n + 16.0 * n_nonfluid + 12.0 * n_cloth + 48.0 * n_links
+            + 24.0 * n_rigid)
+
+
+MIXED_TIMED = ("lambda", "solve", "shape", "neighbors", "finalize")
+
+
 def mixed_main(args):
     """The 765,536-particle mixed-phase scene: a 256x256 cloth (structural,
     shear and bend links, corners pinned), 3,125 rigid 4x4x4 cubes with
     friction and 500,000 fluid particles of two rest densities, 4 iterations
     per substep.  Timed like the dam break (warm-up, then `steps` substeps
-    between synchronisations); no per-kernel events."""
+    between synchronisations, events around one launch in TIMING_PERIOD of
+    the main kernels); then an all-events breakdown and the CPU oracle on
+    the same scene."""
+    import numpy as np
     from particlesolver_amd import System, scenes
     # the settling cubes and cloth pack up to ~210 neighbours within h: a
     # table of 256 keeps every pair (no overflow)
@@ -360,13 +388,49 @@ def mixed_main(args):
     sc.load_into(g)
     g.step(sc.dt, args.warmup)
     g.synchronize()
+    k_start = float(g.neighbors()[0].mean())
+    if not args.no_timing:
+        g.timing(MIXED_TIMED)
+        g.timing_sample(TIMING_PERIOD)
+        g.timing_reset()
     t0 = time.perf_counter()
     g.step(sc.dt, args.steps)
     g.synchronize()
     el = time.perf_counter() - t0
+    kt = g.timing_read() if not args.no_timing else {}
     st = g.status()
-    k = float(g.neighbors()[0].mean())
+    K = 0.5 * (k_start + float(g.neighbors()[0].mean()))
+    breakdown = {}
+    if not args.no_timing and args.breakdown:
+        g.timing(True)
+        g.timing_sample(1)
+        g.timing_reset()
+        g.step(sc.dt, args.breakdown)
+        g.synchronize()
+        for name, (ms, launches) in g.timing_read().items():
+            if launches:
+                breakdown[name] = round(1000.0 * ms / launches, 2)
+        g.timing(False)
     g.close()
+    n_fluid = int((np.asarray(sc.phase) == 0).sum())
+    n_rigid = int(sum(len(i) for i, _ in sc.bodies))
+    n_links = int(sc.distance[0].shape[0]) if sc.distance is not None else 0
+    n_cloth = int(np.unique(sc.distance[0]).size) if n_links else 0  # particles with links
+    kernels, roof = {}, None
+    for name, (ms, launches) in kt.items():
+        if launches:
+            kernels[name] = {"ms_total": round(ms, 4), "launches": launches,
+                             "us_per_launch": round(1000.0 * ms / launches, 3)}
+    if "solve" in kernels:
+        b = mixed_solve_bytes(K, N, N - n_fluid, n_cloth, n_links, n_rigid)
+        ach = b / (kernels["solve"]["us_per_launch"] * 1e-6) / 1e9
+        roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "solve",
+                "algo_bytes_per_launch": int(b), "K": round(K, 2)}
+        kernels["solve"]["algo_GBs"] = round(ach, 1)
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(scenes.mixed(max_neighbors=256), args.cpu_seconds, "mixed-phase scene")
     line = {
         "metric": "particle-iterations/s, 3D mixed-phase scene (cloth, rigid cubes, fluid; 4 iterations/substep)",
         "value": round(N * I * args.steps / el, 1), "unit": "particle-iterations/s", "n_gpus": 1,
@@ -378,9 +442,10 @@ def mixed_main(args):
                                "pinned) + 3125 rigid 4x4x4 cubes (mu_s 0.6, mu_k 0.4) + 500000 "
                                "fluid in two blocks of rest density 1.0 and 2.0 lattice sums = %d particles" % N,
                    "particles": N, "iterations": I, "substeps_per_s": round(args.steps / el, 2),
-                   "mean_neighbors": round(k, 2), "max_neighbors_seen": st.max_neighbors,
-                   "status_flags": st.flags},
-        "roofline": None, "cpu_baseline": None,
+                   "mean_neighbors": round(K, 2), "max_neighbors_seen": st.max_neighbors,
+                   "status_flags": st.flags, "links": n_links, "bodies": len(sc.bodies)},
+        "roofline": roof, "cpu_baseline": cpu, "kernels": kernels,
+        "breakdown_us_per_launch_all_events": breakdown,
     }
     print(json.dumps(line))
 

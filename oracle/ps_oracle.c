@@ -916,33 +916,51 @@ static void body_init(orc_sys *s, uint32_t b)
 }
 
 /* goals of every body from the current iterate */
+/* A body's sums over its members in the order of a 64-lane wavefront:
+ * member k (0-based within the body) goes to lane k % 64, each lane sums
+ * its members in order, then the lanes are added pairwise, v[l] + v[l + s]
+ * for s = 32, 16, ..., 1.  lane[64][w]: the per-lane partial sums of w
+ * quantities; the totals go to out[w]. */
+static void lane_tree_sum(real (*lane)[9], int w, real *out)
+{
+    for (uint32_t st = 32; st > 0; st >>= 1)
+        for (uint32_t l = 0; l < st; l++)
+            for (int e = 0; e < w; e++) lane[l][e] = lane[l][e] + lane[l + st][e];
+    for (int e = 0; e < w; e++) out[e] = lane[0][e];
+}
+
 static void shape_pass(orc_sys *s)
 {
 #pragma omp parallel for schedule(static)
     for (int64_t bb = 0; bb < (int64_t)s->nbody; bb++) {
         uint32_t b = (uint32_t)bb, a = s->body_off[b], e = s->body_off[b + 1];
-        real cx = R(0), cy = R(0), cz = R(0);
+        real lane[64][9];
+        memset(lane, 0, sizeof(lane));
         for (uint32_t k = a; k < e; k++) {
             uint32_t t = s->slot_of[s->body_idx[k]];
             real m = R(1) / s->w[t];
-            cx = cx + (m * s->P[4 * t]);
-            cy = cy + (m * s->P[4 * t + 1]);
-            cz = cz + (m * s->P[4 * t + 2]);
+            real *v = lane[(k - a) % 64];
+            v[0] = v[0] + (m * s->P[4 * t]);
+            v[1] = v[1] + (m * s->P[4 * t + 1]);
+            v[2] = v[2] + (m * s->P[4 * t + 2]);
         }
-        cx = cx / s->body_mass[b];
-        cy = cy / s->body_mass[b];
-        cz = cz / s->body_mass[b];
-        real A[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        real c[3];
+        lane_tree_sum(lane, 3, c);
+        real cx = c[0] / s->body_mass[b], cy = c[1] / s->body_mass[b], cz = c[2] / s->body_mass[b];
+        memset(lane, 0, sizeof(lane));
         for (uint32_t k = a; k < e; k++) {
             uint32_t t = s->slot_of[s->body_idx[k]];
             real m = R(1) / s->w[t];
             real px = m * (s->P[4 * t] - cx), py = m * (s->P[4 * t + 1] - cy),
                  pz = m * (s->P[4 * t + 2] - cz);
             real qx = s->body_rest[3 * k], qy = s->body_rest[3 * k + 1], qz = s->body_rest[3 * k + 2];
+            real *A = lane[(k - a) % 64];
             A[0] = A[0] + (px * qx); A[1] = A[1] + (px * qy); A[2] = A[2] + (px * qz);
             A[3] = A[3] + (py * qx); A[4] = A[4] + (py * qy); A[5] = A[5] + (py * qz);
             A[6] = A[6] + (pz * qx); A[7] = A[7] + (pz * qy); A[8] = A[8] + (pz * qz);
         }
+        real A[9];
+        lane_tree_sum(lane, 9, A);
         extract_rotation(A, &s->body_q[4 * b], 10);
         real m[9];
         quat_to_mat(&s->body_q[4 * b], m);

@@ -777,33 +777,32 @@ __global__ __launch_bounds__(kMaxBlock) void solve_k(DevBuffers b, KConst k, uin
     // a7: particle contacts + friction (collideD, integration_kernel.cuh:358-462)
     if (F & F_CONTACT) {
         if (ph >= PS_PHASE_CLOTH && wi != 0.0f) {
+            // both walks over the table are pipelined a chunk at a time
+            auto pa = [&](uint32_t j) { return PA{b.P[j], b.A[j]}; };
             uint32_t nc = 0;
-            for (uint32_t q = 0; q < c; q++) {
-                const uint32_t j = nbr_at(b, i, q);
-                const int32_t pj = b.A[j].phase;
-                if (ph > PS_PHASE_SOLID && ph == pj) continue;
-                const float4 qj = b.P[j];
+            for_neighbors_chunked(b, i, c, pa, [&](uint32_t, const PA &d) {
+                if (ph > PS_PHASE_SOLID && ph == d.a.phase) return;
+                const float4 qj = d.p;
                 float dx = px - qj.x, dy = py - qj.y, dz = pz - qj.z;
                 float r2 = ((dx * dx) + (dy * dy)) + (dz * dz);
                 if (r2 < k.collide_r2) nc++;
-            }
+            });
             if (TAPE) tape.w = __uint_as_float(nc);
             if (nc > 0) {
                 const float fnc = (float)nc;
                 const float ewi = wi * ps_exp(py);
                 const float4 xi = b.X[i];
                 float cx = 0.0f, cy = 0.0f, cz = 0.0f;
-                for (uint32_t q = 0; q < c; q++) {
-                    const uint32_t j = nbr_at(b, i, q);
-                    const Attr aj = b.A[j];
+                for_neighbors_chunked(b, i, c, pa, [&](uint32_t j, const PA &d) {
+                    const Attr aj = d.a;
                     const int32_t pj = aj.phase;
-                    if (ph > PS_PHASE_SOLID && ph == pj) continue;
-                    const float4 qj = b.P[j];
+                    if (ph > PS_PHASE_SOLID && ph == pj) return;
+                    const float4 qj = d.p;
                     float dx = px - qj.x, dy = py - qj.y, dz = pz - qj.z;
                     float r2 = ((dx * dx) + (dy * dy)) + (dz * dz);
-                    if (!(r2 < k.collide_r2)) continue;
+                    if (!(r2 < k.collide_r2)) return;
                     float dist = sqrtf(r2);
-                    if (dist < 1e-6f) continue;
+                    if (dist < 1e-6f) return;
                     const float wj = aj.w;
                     float cw = wi, cw2 = wj;
                     const bool solid = ph >= PS_PHASE_SOLID && pj >= PS_PHASE_SOLID;
@@ -812,14 +811,14 @@ __global__ __launch_bounds__(kMaxBlock) void solve_k(DevBuffers b, KConst k, uin
                         cw2 = wj * ps_exp(qj.y);
                     }
                     float wsum = cw + cw2;
-                    if (wsum == 0.0f) continue;
+                    if (wsum == 0.0f) return;
                     float mag = dist - k.collide_dist;
                     float sc = (((mag / wsum) / dist) * cw) / fnc;
                     float ex = -(sc * dx), ey = -(sc * dy), ez = -(sc * dz);
                     cx = cx + ex;
                     cy = cy + ey;
                     cz = cz + ez;
-                    if (!solid) continue;
+                    if (!solid) return;
                     float sc2 = (((mag / wsum) / dist) * cw2) / fnc;
                     float fx = sc2 * dx, fy = sc2 * dy, fz = sc2 * dz;
                     const float4 xj = b.X[j];
@@ -830,7 +829,7 @@ __global__ __launch_bounds__(kMaxBlock) void solve_k(DevBuffers b, KConst k, uin
                     float dn = ((rx * nx) + (ry * ny)) + (rz * nz);
                     float tx = rx - (dn * nx), ty = ry - (dn * ny), tz = rz - (dn * nz);
                     float lt = sqrtf(((tx * tx) + (ty * ty)) + (tz * tz));
-                    if (lt < 0.001f) continue;
+                    if (lt < 0.001f) return;
                     float share = cw / wsum;
                     if (lt < (k.mu_s * dist)) {
                         cx = cx - (tx * share);
@@ -843,7 +842,7 @@ __global__ __launch_bounds__(kMaxBlock) void solve_k(DevBuffers b, KConst k, uin
                         cy = cy - ((ty * f) * share);
                         cz = cz - ((tz * f) * share);
                     }
-                }
+                });
                 ax = ax + cx;
                 ay = ay + cy;
                 az = az + cz;
@@ -1045,7 +1044,7 @@ __global__ __launch_bounds__(kMaxBlock) void solve_k(DevBuffers b, KConst k, uin
 }
 
 // ---------------------------------------------------------------------------
-// a12: shape matching per body (one thread per body): centre of mass, A_pq,
+// a12: shape matching per body (one wavefront per body): centre of mass, A_pq,
 // rotation by the fixed-point quaternion iteration (Mueller, Bender,
 // Chentanez, Macklin, MIG 2016) with a first-order quaternion step, goals.
 
@@ -1062,14 +1061,28 @@ __device__ __forceinline__ void quat_to_mat(float w, float x, float y, float z, 
     m[8] = 1.0f - (2.0f * ((x * x) + (y * y)));
 }
 
+// The sum of v over the 64 lanes in pairwise order, v[l] + v[l + s] for
+// s = 32, 16, ..., 1 (the oracle's lane_tree_sum), returned to every lane.
+__device__ __forceinline__ float wave_tree_sum(float v)
+{
+#pragma unroll
+    for (uint32_t s = 32; s > 0; s >>= 1) v = v + __shfl_down(v, s, 64);
+    return __shfl(v, 0, 64);
+}
+
+// One wavefront per body: member q (of a0..a1) belongs to lane (q - a0) %
+// 64, each lane sums its members in order and the lanes are added pairwise;
+// every lane then runs the same rotation iteration on the same sums, and
+// the members' goals are written lane-parallel.
 __global__ __launch_bounds__(kBlock) void shape_k(DevBuffers b, uint32_t nbody)
 {
-    const uint32_t bd = blockIdx.x * kBlock + threadIdx.x;
-    if (bd >= nbody) return;
+    const uint32_t bd = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+    const uint32_t lane = threadIdx.x & 63u;
+    if (bd >= nbody) return;  // the whole wavefront leaves together
     const uint32_t a0 = b.body_off[bd], a1 = b.body_off[bd + 1];
     const float M = b.body_mass[bd];
     float cx = 0.0f, cy = 0.0f, cz = 0.0f;
-    for (uint32_t q = a0; q < a1; q++) {
+    for (uint32_t q = a0 + lane; q < a1; q += 64u) {
         const uint32_t t = b.slot_of[b.body_idx[q]];
         const float m = 1.0f / b.A[t].w;
         const float4 p = b.P[t];
@@ -1077,11 +1090,11 @@ __global__ __launch_bounds__(kBlock) void shape_k(DevBuffers b, uint32_t nbody)
         cy = cy + (m * p.y);
         cz = cz + (m * p.z);
     }
-    cx = cx / M;
-    cy = cy / M;
-    cz = cz / M;
+    cx = wave_tree_sum(cx) / M;
+    cy = wave_tree_sum(cy) / M;
+    cz = wave_tree_sum(cz) / M;
     float A[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    for (uint32_t q = a0; q < a1; q++) {
+    for (uint32_t q = a0 + lane; q < a1; q += 64u) {
         const uint32_t t = b.slot_of[b.body_idx[q]];
         const float m = 1.0f / b.A[t].w;
         const float4 p = b.P[t];
@@ -1091,6 +1104,8 @@ __global__ __launch_bounds__(kBlock) void shape_k(DevBuffers b, uint32_t nbody)
         A[3] = A[3] + (uy * r.x); A[4] = A[4] + (uy * r.y); A[5] = A[5] + (uy * r.z);
         A[6] = A[6] + (uz * r.x); A[7] = A[7] + (uz * r.y); A[8] = A[8] + (uz * r.z);
     }
+#pragma unroll
+    for (int e = 0; e < 9; e++) A[e] = wave_tree_sum(A[e]);
     float4 qt = b.body_q[bd];
     float qw = qt.x, qx = qt.y, qy = qt.z, qz = qt.w;
     for (int it = 0; it < 10; it++) {
@@ -1110,7 +1125,7 @@ __global__ __launch_bounds__(kBlock) void shape_k(DevBuffers b, uint32_t nbody)
         oy = oy / den;
         oz = oz / den;
         float wl2 = ((ox * ox) + (oy * oy)) + (oz * oz);
-        if (wl2 < 1e-18f) break;
+        if (wl2 < 1e-18f) break;  // the same on every lane
         float dw = 1.0f, dx = ox * 0.5f, dy = oy * 0.5f, dz = oz * 0.5f;
         float nw = (((dw * qw) - (dx * qx)) - (dy * qy)) - (dz * qz);
         float nx = (((dw * qx) + (dx * qw)) + (dy * qz)) - (dz * qy);
@@ -1122,10 +1137,10 @@ __global__ __launch_bounds__(kBlock) void shape_k(DevBuffers b, uint32_t nbody)
         qy = ny / nl;
         qz = nz / nl;
     }
-    b.body_q[bd] = make_float4(qw, qx, qy, qz);
+    if (lane == 0) b.body_q[bd] = make_float4(qw, qx, qy, qz);
     float m[9];
     quat_to_mat(qw, qx, qy, qz, m);
-    for (uint32_t q = a0; q < a1; q++) {
+    for (uint32_t q = a0 + lane; q < a1; q += 64u) {
         const float4 r = b.body_rest[q];
         b.body_goal[q] = make_float4((((m[0] * r.x) + (m[1] * r.y)) + (m[2] * r.z)) + cx,
                                      (((m[3] * r.x) + (m[4] * r.y)) + (m[5] * r.z)) + cy,
@@ -1342,7 +1357,8 @@ hipError_t launch_lambda(hipStream_t st, const DevBuffers &b, const KConst &k, u
 hipError_t launch_shape(hipStream_t st, const DevBuffers &b, uint32_t nbody)
 {
     if (nbody == 0) return hipSuccess;
-    hipLaunchKernelGGL(shape_k, dim3(blocks_for(nbody)), dim3(kBlock), 0, st, b, nbody);
+    // one wavefront per body
+    hipLaunchKernelGGL(shape_k, dim3(blocks_for(nbody, kBlock / 64)), dim3(kBlock), 0, st, b, nbody);
     return hipGetLastError();
 }
 
