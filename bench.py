@@ -73,6 +73,8 @@ def parse():
     ap.add_argument("--no-adjoint", action="store_true",
                     help="dam_break: skip the reverse-pass summary")
     ap.add_argument("--scenes", type=int, default=512)
+    ap.add_argument("--no-batched", action="store_true",
+                    help="leave the batched-ensemble object out of the default line")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also run 3 device-to-device copies of MIB MiB (the runtime's copy "
                          "kernel), to scale HBM counters against a known byte count")
@@ -84,6 +86,8 @@ def parse():
                     help="summarise a rocprofv3 --kernel-trace --stats run of this script "
                          "(its kernel_stats and kernel_trace CSVs); BENCH_LOG holds that run's "
                          "JSON line")
+    ap.add_argument("--counter-report", nargs=3, metavar=("PMCA_CSV", "PMCB_CSV", "BENCH_LOG"),
+                    help="summarise two rocprofv3 --pmc runs of SQ counters of this script")
     ap.add_argument("--traffic-report", nargs=3, metavar=("FETCH_CSV", "WRITE_CSV", "BENCH_LOG"),
                     help="summarise two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) of this "
                          "script with --copy-calibration; BENCH_LOG holds that run's JSON line")
@@ -127,9 +131,9 @@ def traffic_report(fetch_csv, write_csv, log):
         if l.startswith("{") and '"metric"' in l:
             line = json.loads(l)
     N, K = line["config"]["particles"], line["config"]["mean_neighbors"]
-    if "adjoint" in line:
-        raise SystemExit("traffic report: the counted run must use --no-adjoint (the reverse-pass "
-                         "workload launches the same kernels on another scene)")
+    if "adjoint" in line or "batched" in line:
+        raise SystemExit("traffic report: the counted run must use --no-adjoint --no-batched (the "
+                         "reverse-pass and ensemble workloads launch the same kernels on other scenes)")
     cal = None
     for l in open(log):
         if l.startswith("copy calibration:"):
@@ -143,10 +147,10 @@ def traffic_report(fetch_csv, write_csv, log):
            "Commands (repository root on the GPU host, R = its absolute path):", "",
            "    cd /tmp && export TMPDIR=/tmp",
            "    rocprofv3 --pmc FETCH_SIZE --output-format csv -d $R/gpurun_out/pmc_fetch -o fetch "
-           "-- python3 $R/bench.py --steps 5 --warmup 2 --no-cpu --no-adjoint --breakdown 0 "
+           "-- python3 $R/bench.py --steps 5 --warmup 2 --no-cpu --no-adjoint --no-batched --breakdown 0 "
            "--copy-calibration 512 > $R/gpurun_out/pmc_fetch.log 2>&1",
            "    rocprofv3 --pmc WRITE_SIZE --output-format csv -d $R/gpurun_out/pmc_write -o write "
-           "-- python3 $R/bench.py --steps 5 --warmup 2 --no-cpu --no-adjoint --breakdown 0 "
+           "-- python3 $R/bench.py --steps 5 --warmup 2 --no-cpu --no-adjoint --no-batched --breakdown 0 "
            "--copy-calibration 512 > $R/gpurun_out/pmc_write.log 2>&1",
            "    cd $R && python3 bench.py --traffic-report "
            "gpurun_out/pmc_fetch/<host>/<pid>_fetch_counter_collection.csv "
@@ -168,6 +172,65 @@ def traffic_report(fetch_csv, write_csv, log):
             k, fetch.get(k, 0.0), write.get(k, 0.0), t / 1e6,
             "%.1f" % (algo / 1e6) if algo else "-", "%.2f" % (t / algo) if algo else "-"))
     out += ["", "traffic_bytes_per_launch: " + json.dumps({"particles": N, "kernels": table})]
+    print("\n".join(out))
+
+
+# MI355X: 256 CUs x 4 SIMDs, 2.4 GHz; a wave64 FP32 VALU instruction (no
+# packed FP32) occupies a SIMD for 4 cycles
+SIMDS, CLOCK_GHZ, VALU_CYCLES = 1024, 2.4, 4.0
+
+
+def counter_report(a_csv, b_csv, log):
+    """Markdown summary of two rocprofv3 --pmc passes of SQ counters over
+    this script (printed to stdout): per kernel, the counts per wavefront and
+    the VALU issue floor, INSTS_VALU x 4 cycles / (1024 SIMDs x 2.4 GHz),
+    against the dispatch's own duration in the same runs."""
+    rows = {}
+    for f in (a_csv, b_csv):
+        for r in csv.DictReader(open(f)):
+            k = r["Kernel_Name"]
+            if "rocclr" in k:
+                continue
+            k = k.replace("(anonymous namespace)::", "").split("(")[0].replace("void ", "")
+            d = rows.setdefault(k, {"dur": [], "c": {}})
+            d["c"].setdefault(r["Counter_Name"], []).append(float(r["Counter_Value"]))
+            d["dur"].append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
+    line = None
+    for l in open(log):
+        l = l.strip()
+        if l.startswith("{") and '"metric"' in l:
+            line = json.loads(l)
+    K = line["config"]["mean_neighbors"]
+    cmd = ("timeout -s KILL 150 rocprofv3 --pmc %s --output-format csv -d $R/gpurun_out/pmc%s -o pmc%s -- "
+           "python3 $R/bench.py --steps 10 --warmup 3 --no-cpu --no-adjoint --no-batched --breakdown 0 "
+           "> $R/gpurun_out/pmc%s.log 2>&1")
+    out = ["# SQ counters per wavefront: 1,048,576-particle PBF dam break", "",
+           "Commands (repository root on the GPU host as R, MI355X):", "", "```",
+           "cd /tmp && export TMPDIR=/tmp",
+           cmd % ("SQ_WAVES SQ_INSTS_VALU SQ_INSTS_SALU SQ_INSTS_LDS SQ_LDS_BANK_CONFLICT SQ_WAIT_INST_LDS "
+                  "SQ_ACTIVE_INST_VALU SQ_WAVE_CYCLES", "A", "A", "A"),
+           cmd % ("SQ_BUSY_CYCLES SQ_WAIT_INST_ANY SQ_ACTIVE_INST_LDS SQ_ACTIVE_INST_ANY SQ_ACTIVE_INST_SCA "
+                  "SQ_WAIT_ANY SQ_INSTS_SMEM SQ_INSTS_VMEM_RD", "B", "B", "B"),
+           "cd $R && python3 bench.py --counter-report gpurun_out/pmcA/<host>/<pid>_pmcA_counter_collection.csv "
+           "gpurun_out/pmcB/<host>/<pid>_pmcB_counter_collection.csv gpurun_out/pmcA.log", "```", "",
+           "13 substeps from rest (K = %.2f neighbours per particle). Counts are means per dispatch divided "
+           "by SQ_WAVES. The VALU floor is the time to issue the dispatch's VALU instructions at one "
+           "wave64 instruction per SIMD every %d cycles on all %d SIMDs at %.1f GHz (FP32 without packed "
+           "math); a kernel whose duration is close to it is VALU-issue-bound. Durations are the "
+           "dispatches' own under the counters (kernels run one at a time)." % (K, VALU_CYCLES, SIMDS, CLOCK_GHZ),
+           "", "| kernel | waves | VALU / wave | SALU / wave | LDS / wave | VMEM rd / wave | "
+           "LDS bank conflict cycles / wave | mean us | VALU floor us | floor / duration |",
+           "|---|---:|---:|---:|---:|---:|---:|---:|---:|---:|"]
+    for k, d in sorted(rows.items(), key=lambda kv: -sum(kv[1]["dur"])):
+        c = {n: sum(v) / len(v) for n, v in d["c"].items()}
+        w = c.get("SQ_WAVES", 0.0)
+        if w <= 0:
+            continue
+        dur = sum(d["dur"]) / len(d["dur"])
+        floor = c.get("SQ_INSTS_VALU", 0.0) * VALU_CYCLES / (SIMDS * CLOCK_GHZ * 1e3)
+        out.append("| `%s` | %.0f | %.1f | %.1f | %.1f | %.1f | %.1f | %.1f | %.1f | %.2f |" % (
+            k, w, c.get("SQ_INSTS_VALU", 0) / w, c.get("SQ_INSTS_SALU", 0) / w, c.get("SQ_INSTS_LDS", 0) / w,
+            c.get("SQ_INSTS_VMEM_RD", 0) / w, c.get("SQ_LDS_BANK_CONFLICT", 0) / w, dur, floor, floor / dur))
     print("\n".join(out))
 
 
@@ -193,7 +256,7 @@ def trace_report(stats_csv, trace_csv, log):
                                                                 line["warmup"], line["steps"]))
     else:
         title = "1,048,576-particle PBF dam break"
-        run = "--no-cpu --no-adjoint --breakdown 0"
+        run = "--no-cpu --no-adjoint --no-batched --breakdown 0"
         what = ("%d substeps (%d warm-up + %d timed) of the dam break from rest, then the exports of "
                 "the positions. Durations vary over the run: the collapse phase has the most "
                 "neighbours per particle." % (line["warmup"] + line["steps"], line["warmup"], line["steps"]))
@@ -206,7 +269,7 @@ def trace_report(stats_csv, trace_csv, log):
            "", "| kernel | calls | avg us | min us | max us | share |", "|---|---|---|---|---|---|"]
     for r in sorted(rows, key=lambda r: -float(r["TotalDurationNs"])):
         out.append("| `%s` | %s | %.1f | %.1f | %.1f | %.1f %% |" % (
-            r["Name"].split("(")[0].replace("void ", ""), r["Calls"], float(r["AverageNs"]) / 1e3,
+            r["Name"].replace("(anonymous namespace)::", "").split("(")[0].replace("void ", ""), r["Calls"], float(r["AverageNs"]) / 1e3,
             float(r["MinNs"]) / 1e3, float(r["MaxNs"]) / 1e3, 100 * float(r["TotalDurationNs"]) / tot))
     trace = sorted(csv.DictReader(open(trace_csv)), key=lambda r: int(r["Start_Timestamp"]))
     warm, steps = line["warmup"], line["steps"]
@@ -248,7 +311,7 @@ def _trace_report_adjoint(rows, tot, line):
            "| kernel | calls | avg us | min us | max us | share |", "|---|---|---|---|---|---|"]
     for r in sorted(rows, key=lambda r: -float(r["TotalDurationNs"])):
         out.append("| `%s` | %s | %.1f | %.1f | %.1f | %.1f %% |" % (
-            r["Name"].split("(")[0].replace("void ", ""), r["Calls"], float(r["AverageNs"]) / 1e3,
+            r["Name"].replace("(anonymous namespace)::", "").split("(")[0].replace("void ", ""), r["Calls"], float(r["AverageNs"]) / 1e3,
             float(r["MinNs"]) / 1e3, float(r["MaxNs"]) / 1e3, 100 * float(r["TotalDurationNs"]) / tot))
     out += ["", "The bench line of the same command (HIP events on the handle's stream): forward "
             "%.2f ms per %d substeps; reverse pass %.2f ms with checkpoints every %d (%.3fx) and "
@@ -450,20 +513,22 @@ def mixed_main(args):
     print(json.dumps(line))
 
 
-def ensemble_main(args):
+def ensemble_measure(nscenes=512, K=8, reps=None, steps=64):
     """512-scene batched ensemble: one handle, every launch covers all scenes,
-    8 substeps per captured graph, vs the same scenes as 512 separate
-    handles, each with its own launches: issued back to back (their streams
-    overlap on the GPU: speedup_vs_per_scene) and one scene after another
-    (each finished before the next: speedup_vs_sequential_scenes)."""
+    K substeps per captured graph, against the same scenes as nscenes
+    separate handles, each with its own launches: issued back to back (their
+    streams overlap on the GPU) and one scene after another (each finished
+    before the next starts, as the reference's single legacy stream orders
+    its calls).  Returns the rates and the wall times of the phases."""
     from particlesolver_amd import System, scenes
-    b = scenes.ensemble(nscenes=args.scenes)
-    N, I, K = b.n, b.params.iterations, 8
+    t_all = time.perf_counter()
+    b = scenes.ensemble(nscenes=nscenes)
+    N, I = b.n, b.params.iterations
     g = System(b.params, b.descs)
     b.load_into(g)
     g.step_graph(b.dt, K)           # capture + first replay (warm-up)
     g.synchronize()
-    reps = max(1, args.steps // K)
+    reps = reps or max(1, steps // K)
     t0 = time.perf_counter()
     for _ in range(reps):
         g.step_graph(b.dt, K)
@@ -471,9 +536,6 @@ def ensemble_main(args):
     el_b = time.perf_counter() - t0
     st = g.status()
     g.close()
-    batched = N * I * K * reps / el_b
-    print("batched: %d scenes %d particles %.2f ms per %d substeps" % (len(b.scenes), N, 1e3 * el_b / reps, K),
-          file=sys.stderr, flush=True)
     plains = []
     for q in range(len(b.scenes)):
         sc = b.plain(q)
@@ -490,8 +552,6 @@ def ensemble_main(args):
     for h in plains:
         h.synchronize()
     el_p = time.perf_counter() - t0
-    # the same calls one scene after another, each finished before the next
-    # starts (the reference's single legacy stream orders them so)
     t0 = time.perf_counter()
     for h in plains:
         h.step(b.dt, K)
@@ -499,22 +559,32 @@ def ensemble_main(args):
     el_s = time.perf_counter() - t0
     for h in plains:
         h.close()
+    batched = N * I * K * reps / el_b
     plain = N * I * K / el_p
     sequential = N * I * K / el_s
+    return {"scenes": len(b.scenes), "particles": N, "iterations": I, "graph_substeps": K,
+            "timed_substeps": K * reps, "value": round(batched, 1),
+            "ms_per_substep": round(1e3 * el_b / (K * reps), 4),
+            "plain_value": round(plain, 1), "speedup_vs_per_scene": round(batched / plain, 2),
+            "plain_ms_per_8_substeps_all_scenes": round(1e3 * el_p, 2),
+            "sequential_value": round(sequential, 1),
+            "speedup_vs_sequential_scenes": round(batched / sequential, 2),
+            "sequential_ms_per_8_substeps_all_scenes": round(1e3 * el_s, 2),
+            "status_flags": st.flags, "max_neighbors_seen": st.max_neighbors,
+            "wall_s": round(time.perf_counter() - t_all, 2)}
+
+
+def ensemble_main(args):
+    r = ensemble_measure(args.scenes, steps=max(args.steps, 8))
     line = {
         "metric": "particle-iterations/s, 512-scene batched ensemble (8-substep graph) vs per-scene launches",
-        "value": round(batched, 1), "unit": "particle-iterations/s", "n_gpus": 1,
-        "steps": K * reps, "warmup": K, "ms_per_step": round(1e3 * el_b / (K * reps), 4),
+        "value": r["value"], "unit": "particle-iterations/s", "n_gpus": 1,
+        "steps": r["timed_substeps"], "warmup": r["graph_substeps"], "ms_per_step": r["ms_per_substep"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (seeded SplitMix64 scene sizes, densities, stiffness, gravity)",
-        "config": {"workload": "%d scenes, %d particles (log-uniform 2k-64k each: fluid block + "
-                               "64-link rope), 4 iterations/substep, dt=1/60" % (len(b.scenes), N),
-                   "plain_value": round(plain, 1), "speedup_vs_per_scene": round(batched / plain, 2),
-                   "plain_ms_per_8_substeps_all_scenes": round(1e3 * el_p, 2),
-                   "sequential_value": round(sequential, 1),
-                   "speedup_vs_sequential_scenes": round(batched / sequential, 2),
-                   "sequential_ms_per_8_substeps_all_scenes": round(1e3 * el_s, 2),
-                   "status_flags": st.flags, "max_neighbors_seen": st.max_neighbors},
+        "config": dict({"workload": "%d scenes, %d particles (log-uniform 2k-64k each: fluid block + "
+                                    "64-link rope), 4 iterations/substep, dt=1/60" % (r["scenes"], r["particles"])},
+                       **{k: v for k, v in r.items() if k != "value"}),
     }
     print(json.dumps(line))
 
@@ -748,6 +818,8 @@ def main():
         return trace_report(*args.trace_report)
     if args.traffic_report:
         return traffic_report(*args.traffic_report)
+    if args.counter_report:
+        return counter_report(*args.counter_report)
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
               "(--gpus %d requested)" % args.gpus, file=sys.stderr)
@@ -894,6 +966,13 @@ def main():
         "breakdown_us_per_launch_all_events": breakdown,
         "export": export,
     }
+    if not args.no_batched:
+        # the batched ensemble: one handle for all scenes against the same
+        # scenes as separate handles
+        r = ensemble_measure(args.scenes)
+        line["batched"] = dict({"workload": "%d scenes, %d particles (log-uniform 2k-64k each: fluid "
+                                            "block + 64-link rope), 4 iterations/substep, one captured "
+                                            "8-substep graph" % (r["scenes"], r["particles"])}, **r)
     if not args.no_adjoint:
         r = adjoint_measure(args, cpu=False)
         line["adjoint"] = {k: r[k] for k in ("workload", "forward_ms", "backward_ms",

@@ -575,11 +575,16 @@ int ps_grid_sort_noalloc(const ps_grid_desc *grid, uint32_t n, const float *pos,
 
 /* Choose the workgroup size (64, 128, 256 or 512) of each per-particle
  * kernel (predict_hash, reorder, neighbors, lambda, solve, finalize) by
- * timing `substeps` substeps of this system per candidate with events.  The
- * simulation state is restored afterwards.  Choices are cached per
- * (kernel, log2 particle count, scene composition) for the process and, if
- * the environment names a file in PS_AUTOTUNE_CACHE, across processes.
- * blocks_out (PS_K_COUNT entries, may be NULL) receives the sizes in use. */
+ * timing `substeps` substeps of this system per candidate with events, and,
+ * for a uniform single-scene fluid, whether the lambda, solve and finalize
+ * kernels stage each workgroup's neighbour cells in LDS (whole substeps
+ * with staging off and on, alternately twice; the choice shows in
+ * ps_status.staged_units, zero when off).  Neither choice changes any
+ * result.  The simulation state is restored afterwards.  Choices are
+ * cached per (kernel, log2 particle count, scene composition) for the
+ * process and, if the environment names a file in PS_AUTOTUNE_CACHE,
+ * across processes.  blocks_out (PS_K_COUNT entries, may be NULL) receives
+ * the sizes in use. */
 int ps_autotune(ps_system *sys, float dt, uint32_t substeps, uint32_t *blocks_out);
 int ps_get_tuning(const ps_system *sys, uint32_t *blocks_out);
 int ps_set_tuning(ps_system *sys, const uint32_t *blocks);

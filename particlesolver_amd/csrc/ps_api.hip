@@ -103,6 +103,10 @@ struct ps_system {
     bool cr_bnd_done = false;  // boundaries of iteration it_sub already projected
     bool cr_swapped = false;   // iterate lives in b.Q (odd number of object passes)
     // workgroup size of each kernel (ps_autotune)
+    // LDS staging of neighbour cells as ps_autotune chose it for this
+    // composition: -1 not tuned (staged where the kernels allow it), 0 off,
+    // 1 on
+    int stage_choice = -1;
     uint32_t block[PS_K_COUNT] = {256, 256, 256, 256, 256, 256, 256, 256, 256, 256,
                                   256, 256, 256, 256, 256, 256, 256};  // reverse pass, export: fixed 256
     // timing
@@ -503,7 +507,7 @@ int stage_sort(ps_system *s)
 bool staging_ok(const ps_system *s)
 {
     return !s->cpuref && !s->adj_active && !(s->prm.flags & PS_FLAG_NO_LDS_STAGING) &&
-           s->kc.nscenes == 1 && uniform_fluid(s);
+           s->kc.nscenes == 1 && uniform_fluid(s) && s->stage_choice != 0;
 }
 
 int stage_neighbors(ps_system *s)
@@ -2367,6 +2371,8 @@ namespace {
 const int kTunable[] = {PS_K_PREDICT_HASH, PS_K_REORDER, PS_K_NEIGHBORS, PS_K_LAMBDA, PS_K_SOLVE,
                         PS_K_FINALIZE};
 const uint32_t kCandidates[] = {64, 128, 256, 512};
+// the tuning-table entry of the staging choice (value 0 off, 1 on)
+constexpr int kTuneStaging = 1000;
 
 // (kernel, log2 N bucket, scene features) -> block size, shared by the
 // handles of this process and, with PS_AUTOTUNE_CACHE=<file>, across runs
@@ -2393,7 +2399,9 @@ void tune_cache_load()
     int k;
     unsigned b, ft, bs;
     while (fscanf(f, "%d %u %u %u", &k, &b, &ft, &bs) == 4)
-        if (bs >= 64 && bs <= 512 && bs % 64 == 0 && k >= 0 && k < PS_K_COUNT) g_tune[{k, b, ft}] = bs;
+        if ((bs >= 64 && bs <= 512 && bs % 64 == 0 && k >= 0 && k < PS_K_COUNT) ||
+            (k == kTuneStaging && bs <= 1u))
+            g_tune[{k, b, ft}] = bs;
     fclose(f);
 }
 
@@ -2432,6 +2440,16 @@ int ps_autotune(ps_system *s, float dt, uint32_t substeps, uint32_t *blocks_out)
         auto it = g_tune.find({k, bucket, feat});
         if (it != g_tune.end())
             s->block[k] = it->second;
+        else
+            need = true;
+    }
+    // the second knob: LDS staging of each unit's neighbour cells (uniform
+    // single-scene fluids, unless the caller turned it off by flag)
+    const bool stageable = s->kc.nscenes == 1 && uniform_fluid(s) && !(s->prm.flags & PS_FLAG_NO_LDS_STAGING);
+    if (stageable) {
+        auto it = g_tune.find({kTuneStaging, bucket, feat});
+        if (it != g_tune.end())
+            s->stage_choice = (int)it->second;
         else
             need = true;
     }
@@ -2498,6 +2516,36 @@ int ps_autotune(ps_system *s, float dt, uint32_t substeps, uint32_t *blocks_out)
             s->block[k] = best;
             if (rc) break;
             g_tune[{k, bucket, feat}] = best;
+        }
+        if (!rc && stageable && !g_tune.count({kTuneStaging, bucket, feat})) {
+            // whole substeps with staging off and on, alternating twice (the
+            // scene moves on between trials), timed with events on the stream
+            hipEvent_t e0 = nullptr, e1 = nullptr;
+            double tot[2] = {0.0, 0.0};
+            e = hipEventCreate(&e0);
+            if (e == hipSuccess) e = hipEventCreate(&e1);
+            for (int round = 0; round < 4 && !rc && e == hipSuccess; round++) {
+                const int choice = round & 1;
+                s->stage_choice = choice;
+                s->timing = 0;
+                if ((rc = ps_step(s, dt, 1))) break;  // warm-up of this choice
+                e = hipEventRecord(e0, s->stream);
+                if (e == hipSuccess && !(rc = ps_step(s, dt, substeps))) {
+                    e = hipEventRecord(e1, s->stream);
+                    if (e == hipSuccess) e = hipEventSynchronize(e1);
+                    float ms = 0.0f;
+                    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+                    tot[choice] += ms;
+                }
+            }
+            if (e0) (void)hipEventDestroy(e0);
+            if (e1) (void)hipEventDestroy(e1);
+            if (!rc && e == hipSuccess) {
+                s->stage_choice = tot[1] <= tot[0] ? 1 : 0;
+                g_tune[{kTuneStaging, bucket, feat}] = (uint32_t)s->stage_choice;
+            } else if (!rc) {
+                rc = fail(s, PS_ERR_DEVICE, std::string("autotune staging trial: ") + hipGetErrorString(e));
+            }
         }
         s->timing = timing0;
         memcpy(s->total_ms, ms0, sizeof(ms0));

@@ -868,8 +868,12 @@ __global__ __launch_bounds__(kMaxBlock) void solve_k(DevBuffers b, KConst k, uin
                 float fx, fy, fz;
                 bool deg;
             };
+            // w_scale_h2 held in a vector register for the loop: an FMA may
+            // read one scalar operand only, and w_scale is the other
+            float wsh2 = k.w_scale_h2;
+            asm volatile("" : "+v"(wsh2));
             auto coef_of = [&](float r2, float lj) {
-                const float sw = __builtin_fmaf(-k.w_scale, r2, k.w_scale_h2);
+                const float sw = __builtin_fmaf(-k.w_scale, r2, wsh2);
                 const float x = (sw * sw) * sw;
                 return __builtin_fmaf(-k.k_corr, ipow(x, NC ? NC : k.n_corr), li + lj);
             };

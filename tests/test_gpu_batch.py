@@ -121,5 +121,10 @@ def test_autotune_restores_state_and_block_size_never_changes_results(gpu_availa
             ref.step(sc.dt, 2)
             assert_bitexact(g.positions(), ref.positions(), "positions with blocks %s" % bs)
             assert_bitexact(g.velocities(), ref.velocities(), "velocities with blocks %s" % bs)
+            # the tuner's staging choice: every unit staged (as far as the
+            # slots allow) or none
+            st = g.status()
+            assert st.units > 0 and (st.staged_units == 0 or st.staged_units > st.units // 2), \
+                (st.units, st.staged_units)
         with pytest.raises(capi.PsError):
             g.set_tuning({k: 96 for k in TUNABLE})

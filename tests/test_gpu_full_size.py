@@ -65,6 +65,25 @@ def test_dam_break_1M_bitexact_and_table_properties(gpu_available):
         assert_bitexact(g.velocities(), ref.velocities(), "velocities after substep 2")
 
 
+def test_dam_break_1M_60_substeps_bitexact(gpu_available):
+    # the collapse from rest through 60 substeps (a second of simulated
+    # time): positions and velocities bit for bit at 20, 40 and 60, with the
+    # benchmark's table of 192 entries (no overflow)
+    sc = scenes.dam_break(max_neighbors=192)
+    ref = load_scene(sc, "f32")
+    with System(sc.params) as g:
+        sc.load_into(g)
+        for done in (20, 40, 60):
+            g.step(sc.dt, 20)
+            ref.step(sc.dt, 20)
+            print("dam break 1M: %d substeps" % done, flush=True)
+            assert_bitexact(g.positions(), ref.positions(), "positions after %d substeps" % done)
+            assert_bitexact(g.velocities(), ref.velocities(), "velocities after %d substeps" % done)
+        gs, rs = g.status(), ref.status()
+        assert gs.flags == rs.flags and gs.max_neighbors == rs.max_neighbors
+        assert not gs.flags & 1, gs.max_neighbors
+
+
 def test_ensemble_512_scenes_graph_bitexact_per_scene(gpu_available):
     # the batched configuration at full size: 512 scenes, about 9M particles,
     # an 8-substep captured loop with the tuned block sizes; every scene must
