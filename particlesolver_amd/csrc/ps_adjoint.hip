@@ -683,18 +683,18 @@ hipError_t launch_adj(hipStream_t st, AdjKernel which, const DevBuffers &b, cons
 #define PS_ADJ_LAUNCH(kern, ...)                                                                  \
     do {                                                                                           \
         if (uniform)                                                                               \
-            hipLaunchKernelGGL(kern<true>, grid, block, 0, st, __VA_ARGS__);                       \
+            PS_LAUNCH(kern<true>, grid, block, 0, st, __VA_ARGS__);                       \
         else                                                                                       \
-            hipLaunchKernelGGL(kern<false>, grid, block, 0, st, __VA_ARGS__);                      \
+            PS_LAUNCH(kern<false>, grid, block, 0, st, __VA_ARGS__);                      \
     } while (0)
     // two lanes per particle: kAdjBlock / 2 particles per block
     const dim3 grid2((count + kAdjBlock / 2 - 1) / (kAdjBlock / 2));
 #define PS_ADJ_LAUNCH2(kern, ...)                                                                 \
     do {                                                                                           \
         if (uniform)                                                                               \
-            hipLaunchKernelGGL(kern<true>, grid2, block, 0, st, __VA_ARGS__);                      \
+            PS_LAUNCH(kern<true>, grid2, block, 0, st, __VA_ARGS__);                      \
         else                                                                                       \
-            hipLaunchKernelGGL(kern<false>, grid2, block, 0, st, __VA_ARGS__);                     \
+            PS_LAUNCH(kern<false>, grid2, block, 0, st, __VA_ARGS__);                     \
     } while (0)
     switch (which) {
     case ADJ_XSPH: PS_ADJ_LAUNCH2(adj_xsph_k, b, k, a, count, dt); break;
@@ -702,8 +702,8 @@ hipError_t launch_adj(hipStream_t st, AdjKernel which, const DevBuffers &b, cons
     case ADJ_PRE: PS_ADJ_LAUNCH(adj_pre_k, b, k, a, count); break;
     case ADJ_LAMBDA: PS_ADJ_LAUNCH2(adj_lambda_k, b, k, a, count); break;
     case ADJ_POSITION: PS_ADJ_LAUNCH2(adj_position_k, b, k, a, count); break;
-    case ADJ_STIFFNESS: hipLaunchKernelGGL(adj_stiffness_k, grid, block, 0, st, b, k, a, count); break;
-    case ADJ_PREDICT: hipLaunchKernelGGL(adj_predict_k, grid, block, 0, st, b, k, a, count, dt); break;
+    case ADJ_STIFFNESS: PS_LAUNCH(adj_stiffness_k, grid, block, 0, st, b, k, a, count); break;
+    case ADJ_PREDICT: PS_LAUNCH(adj_predict_k, grid, block, 0, st, b, k, a, count, dt); break;
     default: return hipErrorInvalidValue;
     }
 #undef PS_ADJ_LAUNCH

@@ -26,6 +26,7 @@ namespace {
 struct TimedLaunch {
     int id;
     hipEvent_t start, stop;
+    bool by_dispatch;  // the events ride on the kernel's dispatch (PS_LAUNCH)
 };
 
 }  // namespace
@@ -411,6 +412,7 @@ int timed_begin(ps_system *s, int id, TimedLaunch &tl)
 {
     tl.id = id;
     tl.start = tl.stop = nullptr;
+    tl.by_dispatch = false;
     if (!(s->timing & (1u << id))) return PS_OK;
     if (s->seen[id]++ % s->timing_period) return PS_OK;
     for (hipEvent_t *e : {&tl.start, &tl.stop}) {
@@ -421,14 +423,27 @@ int timed_begin(ps_system *s, int id, TimedLaunch &tl)
             HIP_TRY(s, hipEventCreate(e));
         }
     }
-    HIP_TRY(s, hipEventRecord(tl.start, s->tstream ? s->tstream : s->stream));
+    // single-kernel launchers take the pair on their dispatch; the scan's
+    // two kernels are bracketed on the stream
+    tl.by_dispatch = id != PS_K_SCAN;
+    if (tl.by_dispatch)
+        ps::g_launch_events = ps::LaunchEvents{tl.start, tl.stop};
+    else
+        HIP_TRY(s, hipEventRecord(tl.start, s->tstream ? s->tstream : s->stream));
     return PS_OK;
 }
 
 int timed_end(ps_system *s, TimedLaunch &tl)
 {
     if (!tl.start) return PS_OK;
-    HIP_TRY(s, hipEventRecord(tl.stop, s->tstream ? s->tstream : s->stream));
+    if (tl.by_dispatch && ps::g_launch_events.start) {
+        // no kernel took the pair (nothing to launch): no measurement
+        ps::g_launch_events = ps::LaunchEvents{};
+        s->free_events.push_back(tl.start);
+        s->free_events.push_back(tl.stop);
+        return PS_OK;
+    }
+    if (!tl.by_dispatch) HIP_TRY(s, hipEventRecord(tl.stop, s->tstream ? s->tstream : s->stream));
     s->pending.push_back(tl);
     return PS_OK;
 }

@@ -51,15 +51,15 @@ hipError_t launch_export(hipStream_t st, const DevBuffers &b, uint32_t n, uint32
     const dim3 grid((n + kExportBlock - 1) / kExportBlock);
     switch (field) {
     case PS_FIELD_POSITIONS:
-        hipLaunchKernelGGL(export_vec_k, grid, dim3(kExportBlock), 0, st, b.X, b.A, n, first, count,
+        PS_LAUNCH(export_vec_k, grid, dim3(kExportBlock), 0, st, b.X, b.A, n, first, count,
                            (float4 *)dst, 1.0f);
         break;
     case PS_FIELD_VELOCITIES:
-        hipLaunchKernelGGL(export_vec_k, grid, dim3(kExportBlock), 0, st, b.V, b.A, n, first, count,
+        PS_LAUNCH(export_vec_k, grid, dim3(kExportBlock), 0, st, b.V, b.A, n, first, count,
                            (float4 *)dst, 0.0f);
         break;
     case PS_FIELD_PHASES:
-        hipLaunchKernelGGL(export_phase_k, grid, dim3(kExportBlock), 0, st, b.A, n, first, count,
+        PS_LAUNCH(export_phase_k, grid, dim3(kExportBlock), 0, st, b.A, n, first, count,
                            (int32_t *)dst);
         break;
     default:

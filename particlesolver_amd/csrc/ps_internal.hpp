@@ -218,6 +218,31 @@ struct AdjBuffers {
     uint32_t dist;         // the scene has distance constraints
 };
 
+// Timing of one launch by the dispatch itself: the host puts an event pair
+// here before calling a launcher, and the launcher's kernel, started with
+// PS_LAUNCH, takes it (hipExtLaunchKernelGGL records the pair at the
+// kernel's own start and end, so no stream interval outside the kernel lies
+// between them) and clears it.  Launchers of two kernels (the scan) leave it.
+struct LaunchEvents {
+    hipEvent_t start = nullptr, stop = nullptr;
+};
+extern thread_local LaunchEvents g_launch_events;
+}  // namespace ps
+
+#include <hip/hip_ext.h>
+
+#define PS_LAUNCH(kern, grid, block, lds, st, ...)                                               \
+    do {                                                                                         \
+        const ps::LaunchEvents ev_ = ps::g_launch_events;                                        \
+        ps::g_launch_events = ps::LaunchEvents{};                                                \
+        if (ev_.start)                                                                           \
+            hipExtLaunchKernelGGL(kern, grid, block, lds, st, ev_.start, ev_.stop, 0u, __VA_ARGS__); \
+        else                                                                                     \
+            hipLaunchKernelGGL(kern, grid, block, lds, st, __VA_ARGS__);                         \
+    } while (0)
+
+namespace ps {
+
 // Kernel launchers (ps_kernels.hip).  All enqueue on `st` and return the
 // launch error.
 // bs: workgroup size (64..512, a multiple of 64) of the tunable kernels

@@ -1282,6 +1282,8 @@ __global__ __launch_bounds__(kBlock) void sleep_k(DevBuffers b, uint32_t n)
 // ---------------------------------------------------------------------------
 // launchers
 
+thread_local LaunchEvents g_launch_events;
+
 static inline uint32_t blocks_for(uint32_t n, uint32_t bs = kBlock) { return (n + bs - 1) / bs; }
 
 uint32_t scan_tiles(uint32_t num_cells) { return (num_cells + kScanTile - 1) / kScanTile; }
@@ -1292,11 +1294,11 @@ hipError_t launch_predict_hash(hipStream_t st, const DevBuffers &b, const KConst
     if (n == 0) return hipSuccess;
     const dim3 g(blocks_for(n, bs)), t(bs);
     if (predict && hash)
-        hipLaunchKernelGGL((predict_hash_k<true, true>), g, t, 0, st, b, k, n, dt);
+        PS_LAUNCH((predict_hash_k<true, true>), g, t, 0, st, b, k, n, dt);
     else if (predict)
-        hipLaunchKernelGGL((predict_hash_k<true, false>), g, t, 0, st, b, k, n, dt);
+        PS_LAUNCH((predict_hash_k<true, false>), g, t, 0, st, b, k, n, dt);
     else if (hash)
-        hipLaunchKernelGGL((predict_hash_k<false, true>), g, t, 0, st, b, k, n, dt);
+        PS_LAUNCH((predict_hash_k<false, true>), g, t, 0, st, b, k, n, dt);
     return hipGetLastError();
 }
 
@@ -1315,14 +1317,14 @@ hipError_t launch_scan(hipStream_t st, const DevBuffers &b, uint32_t num_cells)
 hipError_t launch_scatter(hipStream_t st, const DevBuffers &b, uint32_t n)
 {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(scatter_k, dim3(blocks_for(n)), dim3(kBlock), 0, st, b, n);
+    PS_LAUNCH(scatter_k, dim3(blocks_for(n)), dim3(kBlock), 0, st, b, n);
     return hipGetLastError();
 }
 
 hipError_t launch_reorder(hipStream_t st, const DevBuffers &b, uint32_t n, uint32_t bs)
 {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(reorder_k, dim3(blocks_for(n, bs)), dim3(bs), 0, st, b, n);
+    PS_LAUNCH(reorder_k, dim3(blocks_for(n, bs)), dim3(bs), 0, st, b, n);
     return hipGetLastError();
 }
 
@@ -1331,10 +1333,10 @@ hipError_t launch_neighbors(hipStream_t st, const DevBuffers &b, const KConst &k
 {
     if (n == 0) return hipSuccess;
     if (b.staging)
-        hipLaunchKernelGGL(neighbors_staged_k, dim3(blocks_for(n, kStageUnit)), dim3(kStageUnit), 0,
+        PS_LAUNCH(neighbors_staged_k, dim3(blocks_for(n, kStageUnit)), dim3(kStageUnit), 0,
                            st, b, k, n, kmax);
     else
-        hipLaunchKernelGGL(neighbors_k, dim3(blocks_for(n, bs)), dim3(bs), 0, st, b, k, n, kmax);
+        PS_LAUNCH(neighbors_k, dim3(blocks_for(n, bs)), dim3(bs), 0, st, b, k, n, kmax);
     return hipGetLastError();
 }
 
@@ -1345,16 +1347,16 @@ hipError_t launch_lambda(hipStream_t st, const DevBuffers &b, const KConst &k, u
     const dim3 g(blocks_for(n, bs)), t(bs);
     const float mu = 1.0f / wu;
     if (b.tape_lq && uniform)  // recording for the reverse pass (never staged)
-        hipLaunchKernelGGL((lambda_k<true, false, true>), g, t, 0, st, b, k, n, mu);
+        PS_LAUNCH((lambda_k<true, false, true>), g, t, 0, st, b, k, n, mu);
     else if (b.tape_lq)
-        hipLaunchKernelGGL((lambda_k<false, false, true>), g, t, 0, st, b, k, n, mu);
+        PS_LAUNCH((lambda_k<false, false, true>), g, t, 0, st, b, k, n, mu);
     else if (uniform && b.staging)
-        hipLaunchKernelGGL((lambda_k<true, true>), dim3(blocks_for(n, kStageUnit)), dim3(2 * kStageUnit),
+        PS_LAUNCH((lambda_k<true, true>), dim3(blocks_for(n, kStageUnit)), dim3(2 * kStageUnit),
                            0, st, b, k, n, mu);
     else if (uniform)
-        hipLaunchKernelGGL((lambda_k<true, false>), g, t, 0, st, b, k, n, mu);
+        PS_LAUNCH((lambda_k<true, false>), g, t, 0, st, b, k, n, mu);
     else
-        hipLaunchKernelGGL((lambda_k<false, false>), g, t, 0, st, b, k, n, mu);
+        PS_LAUNCH((lambda_k<false, false>), g, t, 0, st, b, k, n, mu);
     return hipGetLastError();
 }
 
@@ -1362,7 +1364,7 @@ hipError_t launch_shape(hipStream_t st, const DevBuffers &b, uint32_t nbody)
 {
     if (nbody == 0) return hipSuccess;
     // one wavefront per body
-    hipLaunchKernelGGL(shape_k, dim3(blocks_for(nbody, kBlock / 64)), dim3(kBlock), 0, st, b, nbody);
+    PS_LAUNCH(shape_k, dim3(blocks_for(nbody, kBlock / 64)), dim3(kBlock), 0, st, b, nbody);
     return hipGetLastError();
 }
 
@@ -1379,27 +1381,27 @@ static void solve_variant(hipStream_t st, const DevBuffers &b, const KConst &k, 
     constexpr bool CAN_STAGE = F == F_FLUID;
     if (b.tape_f) {  // recording for the reverse pass (never staged)
         if (uniform && n4)
-            hipLaunchKernelGGL((solve_k<F, true, 4, false, true>), g, t, 0, st, b, k, n, wu, iter);
+            PS_LAUNCH((solve_k<F, true, 4, false, true>), g, t, 0, st, b, k, n, wu, iter);
         else if (uniform)
-            hipLaunchKernelGGL((solve_k<F, true, 0, false, true>), g, t, 0, st, b, k, n, wu, iter);
+            PS_LAUNCH((solve_k<F, true, 0, false, true>), g, t, 0, st, b, k, n, wu, iter);
         else if (n4)
-            hipLaunchKernelGGL((solve_k<F, false, 4, false, true>), g, t, 0, st, b, k, n, wu, iter);
+            PS_LAUNCH((solve_k<F, false, 4, false, true>), g, t, 0, st, b, k, n, wu, iter);
         else
-            hipLaunchKernelGGL((solve_k<F, false, 0, false, true>), g, t, 0, st, b, k, n, wu, iter);
+            PS_LAUNCH((solve_k<F, false, 0, false, true>), g, t, 0, st, b, k, n, wu, iter);
         return;
     }
     if (CAN_STAGE && uniform && b.staging && n4)
-        hipLaunchKernelGGL((solve_k<F, true, 4, CAN_STAGE>), gs, ts, 0, st, b, k, n, wu, iter);
+        PS_LAUNCH((solve_k<F, true, 4, CAN_STAGE>), gs, ts, 0, st, b, k, n, wu, iter);
     else if (CAN_STAGE && uniform && b.staging)
-        hipLaunchKernelGGL((solve_k<F, true, 0, CAN_STAGE>), gs, ts, 0, st, b, k, n, wu, iter);
+        PS_LAUNCH((solve_k<F, true, 0, CAN_STAGE>), gs, ts, 0, st, b, k, n, wu, iter);
     else if (uniform && n4)
-        hipLaunchKernelGGL((solve_k<F, true, 4, false>), g, t, 0, st, b, k, n, wu, iter);
+        PS_LAUNCH((solve_k<F, true, 4, false>), g, t, 0, st, b, k, n, wu, iter);
     else if (uniform)
-        hipLaunchKernelGGL((solve_k<F, true, 0, false>), g, t, 0, st, b, k, n, wu, iter);
+        PS_LAUNCH((solve_k<F, true, 0, false>), g, t, 0, st, b, k, n, wu, iter);
     else if (n4)
-        hipLaunchKernelGGL((solve_k<F, false, 4, false>), g, t, 0, st, b, k, n, wu, iter);
+        PS_LAUNCH((solve_k<F, false, 4, false>), g, t, 0, st, b, k, n, wu, iter);
     else
-        hipLaunchKernelGGL((solve_k<F, false, 0, false>), g, t, 0, st, b, k, n, wu, iter);
+        PS_LAUNCH((solve_k<F, false, 0, false>), g, t, 0, st, b, k, n, wu, iter);
 }
 
 hipError_t launch_solve(hipStream_t st, const DevBuffers &b, const KConst &k, uint32_t n,
@@ -1425,12 +1427,12 @@ hipError_t launch_finalize(hipStream_t st, const DevBuffers &b, const KConst &k,
     const dim3 g(blocks_for(n, bs)), t(bs);
     const float mu = 1.0f / wu;
     if (uniform && b.staging)
-        hipLaunchKernelGGL((finalize_k<true, true>), dim3(blocks_for(n, kStageUnit)),
+        PS_LAUNCH((finalize_k<true, true>), dim3(blocks_for(n, kStageUnit)),
                            dim3(2 * kStageUnit), 0, st, b, k, n, mu, dt, its);
     else if (uniform)
-        hipLaunchKernelGGL((finalize_k<true, false>), g, t, 0, st, b, k, n, mu, dt, its);
+        PS_LAUNCH((finalize_k<true, false>), g, t, 0, st, b, k, n, mu, dt, its);
     else
-        hipLaunchKernelGGL((finalize_k<false, false>), g, t, 0, st, b, k, n, mu, dt, its);
+        PS_LAUNCH((finalize_k<false, false>), g, t, 0, st, b, k, n, mu, dt, its);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || !sleeping) return e;
     hipLaunchKernelGGL(sleep_k, dim3(blocks_for(n)), dim3(kBlock), 0, st, b, n);
