@@ -61,7 +61,7 @@ def build_cpp_demo(verbose=True):
     for src, out in DEMOS:
         if not os.path.exists(src):
             continue
-        cmd = [hipcc(), "-O2", "-std=c++17", "-I" + os.path.join(ROOT, "include"), "-I" + CSRC,
+        cmd = [hipcc(), "-O2", "-std=c++17", "-ffp-contract=off", "-I" + os.path.join(ROOT, "include"), "-I" + CSRC,
                src, "-o", out + ".tmp", "-L" + HERE, "-lparticlesolver", "-Wl,-rpath,$ORIGIN"]
         if verbose:
             print(" ".join(cmd), flush=True)

@@ -84,7 +84,8 @@ void ParticleSystem::update(float deltaTime)
         for (int a = 0; a < 3; a++) { vel[a] = m_particlesToAdd.front(); m_particlesToAdd.pop_front(); }
         w[0] = 1.0f / m_particlesToAdd.front();
         m_particlesToAdd.pop_front();
-        addParticleMultiple(pos, vel, w, ro, std::vector<int32_t>(1, PS_PHASE_SOLID));
+        if (getNumParticles() == m_params.max_particles) continue;  // particlesystem.cpp:303-304
+        addParticleMultiple(pos, vel, w, ro, std::vector<int32_t>(1, PS_PHASE_SOLID), true);
     }
     if (m_fluidsToAdd.size() >= 5) {
         std::vector<float> pos, vel, w, ro;
@@ -102,16 +103,21 @@ void ParticleSystem::update(float deltaTime)
     }
 }
 
-void ParticleSystem::addParticleMultiple(const std::vector<float> &pos,
+// particlesystem.cpp:333-347: a block that would reach maxParticles is
+// dropped whole, without an error, as there (false then; the builders then
+// add none of its constraints)
+bool ParticleSystem::addParticleMultiple(const std::vector<float> &pos,
                                          const std::vector<float> &vel,
                                          const std::vector<float> &w,
                                          const std::vector<float> &ro,
-                                         const std::vector<int32_t> &phase)
+                                         const std::vector<int32_t> &phase, bool single)
 {
     const uint32_t n = (uint32_t)w.size();
-    if (n == 0) return;
+    if (n == 0) return true;
+    if (!single && (uint64_t)getNumParticles() + n >= m_params.max_particles) return false;
     check(ps_add_particles(m_sys, n, pos.data(), vel.data(), w.data(), phase.data(), ro.data(),
                            nullptr));
+    return true;
 }
 
 // particlesystem.cpp:357-405: lattice spacing 2.5 r, jitter r * 0.01
@@ -208,12 +214,13 @@ void ParticleSystem::addHorizCloth(Int2 ll, Int2 ur, Float3 spacing, Float2 dist
             ro.push_back(1.0f);
             ph.push_back(PS_PHASE_RIGID + m_rigidIndex);
         }
-    addParticleMultiple(pos, vel, w, ro, ph);
-    if (!pinIdx.empty())
-        check(ps_add_point_constraints(m_sys, (uint32_t)pinIdx.size(), pinIdx.data(), pins.data()));
-    if (!rest.empty())
-        check(ps_add_distance_constraints(m_sys, (uint32_t)rest.size(), pairs.data(), rest.data(),
-                                          nullptr));
+    if (addParticleMultiple(pos, vel, w, ro, ph) && !w.empty()) {
+        if (!pinIdx.empty())
+            check(ps_add_point_constraints(m_sys, (uint32_t)pinIdx.size(), pinIdx.data(), pins.data()));
+        if (!rest.empty())
+            check(ps_add_distance_constraints(m_sys, (uint32_t)rest.size(), pairs.data(), rest.data(),
+                                              nullptr));
+    }
     m_rigidIndex++;
 }
 
@@ -236,13 +243,14 @@ void ParticleSystem::addRope(Float3 start, Float3 spacing, float dist, int numLi
             rest.push_back(dist);
         }
     }
-    addParticleMultiple(pos, vel, w, ro, ph);
-    if (!rest.empty())
-        check(ps_add_distance_constraints(m_sys, (uint32_t)rest.size(), pairs.data(), rest.data(),
-                                          nullptr));
-    if (constrainStart) {
-        const float p[3] = {start.x, start.y, start.z};
-        check(ps_add_point_constraints(m_sys, 1, &s0, p));
+    if (addParticleMultiple(pos, vel, w, ro, ph)) {
+        if (!rest.empty())
+            check(ps_add_distance_constraints(m_sys, (uint32_t)rest.size(), pairs.data(), rest.data(),
+                                              nullptr));
+        if (constrainStart) {
+            const float p[3] = {start.x, start.y, start.z};
+            check(ps_add_point_constraints(m_sys, 1, &s0, p));
+        }
     }
     m_rigidIndex++;
 }
@@ -275,8 +283,7 @@ void ParticleSystem::addStaticSphere(Int3 ll, Int3 ur, float spacing)
                     ph.push_back(PS_PHASE_RIGID + m_rigidIndex);
                 }
             }
-    addParticleMultiple(pos, vel, w, ro, ph);
-    if (!idx.empty())
+    if (addParticleMultiple(pos, vel, w, ro, ph) && !idx.empty())
         check(ps_add_point_constraints(m_sys, (uint32_t)idx.size(), idx.data(), pos.data()));
     m_rigidIndex++;
 }

@@ -8,7 +8,9 @@
 //    metadata (colours) is not kept;
 //  - the grid is dense, with cell = max(h, 2r) and sized to the bounds, instead
 //    of a wrapped 64^3 hash (integration_kernel.cuh:199-201);
-//  - errors throw ps::Error instead of printing and exiting;
+//  - errors throw ps::Error instead of printing and exiting; additions past
+//    maxParticles are dropped silently as there (particlesystem.cpp:303,335),
+//    and a builder whose particles were dropped adds no constraints;
 //  - random jitter comes from a seeded SplitMix64 stream instead of rand().
 #pragma once
 
@@ -80,9 +82,9 @@ public:
 
 private:
     void check(int rc) const;
-    void addParticleMultiple(const std::vector<float> &pos, const std::vector<float> &vel,
+    bool addParticleMultiple(const std::vector<float> &pos, const std::vector<float> &vel,
                              const std::vector<float> &w, const std::vector<float> &ro,
-                             const std::vector<int32_t> &phase);
+                             const std::vector<int32_t> &phase, bool single = false);
     float frand();
 
     ps_system *m_sys = nullptr;
