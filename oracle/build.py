@@ -6,11 +6,18 @@ CPU solver, Simulation::tick) are compiled together twice with gcc: FP32
 FP32 build to exactly the IEEE operations written in the source.  OpenMP
 parallelises the per-particle loops (each particle is a pure gather, so the
 results do not depend on the thread count).
+
+    python oracle/build.py              # the two libraries, next to this file
+    python oracle/build.py --sanitize   # ASan+UBSan builds in oracle/_san, then
+                                        # the oracle's CPU tests against them
 """
 import os
 import subprocess
+import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+SAN_DIR = os.path.join(HERE, "_san")
 SRCS = [os.path.join(HERE, "ps_oracle.c"), os.path.join(HERE, "ref2d.c")]
 
 # -mfma turns the explicit fmaf/fma calls into the FMA instruction (the same
@@ -20,12 +27,26 @@ CFLAGS = ["-std=c99", "-O2", "-fPIC", "-shared", "-ffp-contract=off", "-fno-fast
           "-mfma", "-fopenmp", "-Wall", "-Wno-unknown-pragmas"]
 
 
-def build_oracle(verbose=True):
+# AddressSanitizer + UndefinedBehaviorSanitizer build (host code only: the
+# oracle is plain C), every UB report fatal
+SAN_FLAGS = ["-O1", "-g", "-fno-omit-frame-pointer", "-fsanitize=address,undefined",
+             "-fno-sanitize-recover=all"]
+
+# the CPU tests that exercise the oracle, run against the sanitizer build
+SAN_TESTS = ["tests/test_oracle_pinning.py", "tests/test_oracle_adjoint.py", "tests/test_oracle_friction.py",
+             "tests/test_ref2d_pinning.py", "tests/test_ref2d_emitter.py", "tests/test_ref2d_gas.py",
+             "tests/test_ref2d_volcano.py"]
+
+
+def build_oracle(verbose=True, sanitize=False):
     outs = []
+    outdir = SAN_DIR if sanitize else HERE
+    os.makedirs(outdir, exist_ok=True)
+    flags = CFLAGS + SAN_FLAGS if sanitize else CFLAGS
     for dbl, name in ((0, "liboracle_f32.so"), (1, "liboracle_f64.so")):
-        out = os.path.join(HERE, name)
+        out = os.path.join(outdir, name)
         tmp = out + ".tmp"
-        cmd = ["gcc"] + CFLAGS + ["-DORC_DOUBLE=%d" % dbl] + SRCS + ["-o", tmp, "-lm"]
+        cmd = ["gcc"] + flags + ["-DORC_DOUBLE=%d" % dbl] + SRCS + ["-o", tmp, "-lm"]
         if verbose:
             print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True)
@@ -34,5 +55,22 @@ def build_oracle(verbose=True):
     return outs
 
 
+def run_sanitized_tests(extra=()):
+    """The oracle's CPU tests with the sanitizer build loaded (libasan
+    preloaded into the interpreter, leak checks off: CPython's own
+    allocations would dominate them)."""
+    asan = subprocess.run(["gcc", "-print-file-name=libasan.so"], check=True, capture_output=True,
+                          text=True).stdout.strip()
+    env = dict(os.environ, PS_ORACLE_DIR=SAN_DIR, LD_PRELOAD=asan,
+               ASAN_OPTIONS="detect_leaks=0:abort_on_error=1",
+               UBSAN_OPTIONS="print_stacktrace=1:halt_on_error=1")
+    cmd = [sys.executable, "-m", "pytest", "-q", "-x", "-p", "no:cacheprovider"] + SAN_TESTS + list(extra)
+    print(" ".join(cmd), flush=True)
+    return subprocess.run(cmd, cwd=ROOT, env=env).returncode
+
+
 if __name__ == "__main__":
+    if "--sanitize" in sys.argv[1:]:
+        build_oracle(sanitize=True)
+        sys.exit(run_sanitized_tests())
     build_oracle()

@@ -15,11 +15,13 @@ _libs = {}
 
 def _lib(prec, path=None):
     """The oracle library of precision prec; `path` names another build of
-    the same sources (the mutation tests load deliberately broken copies)."""
+    the same sources (the mutation tests load deliberately broken copies).
+    PS_ORACLE_DIR, when set, names the folder of the default builds (the
+    sanitizer builds of `oracle/build.py --sanitize` live in oracle/_san)."""
     key = (prec, path)
     if key not in _libs:
         if path is None:
-            path = os.path.join(HERE, "liboracle_%s.so" % prec)
+            path = os.path.join(os.environ.get("PS_ORACLE_DIR") or HERE, "liboracle_%s.so" % prec)
         if not os.path.exists(path):
             raise RuntimeError("oracle library %s missing: run oracle/build.py" % path)
         _libs[key] = ctypes.CDLL(path)
