@@ -153,8 +153,8 @@ def traffic_report(fetch_csv, write_csv, log):
            "-- python3 $R/bench.py --steps 5 --warmup 2 --no-cpu --no-adjoint --no-batched --breakdown 0 "
            "--copy-calibration 512 > $R/gpurun_out/pmc_write.log 2>&1",
            "    cd $R && python3 bench.py --traffic-report "
-           "gpurun_out/pmc_fetch/<host>/<pid>_fetch_counter_collection.csv "
-           "gpurun_out/pmc_write/<host>/<pid>_write_counter_collection.csv gpurun_out/pmc_fetch.log", "",
+           "gpurun_out/pmc_fetch/fetch_counter_collection.csv "
+           "gpurun_out/pmc_write/write_counter_collection.csv gpurun_out/pmc_fetch.log", "",
            "Counters: rocprofv3 FETCH_SIZE and WRITE_SIZE (KiB per dispatch), one run each.",
            "Calibration: the runtime copy kernel of a %d-byte hipMemcpy (device to device) in the "
            "same runs reads FETCH_SIZE %.1f KiB and WRITE_SIZE %.1f KiB for %.1f KiB moved each "
@@ -211,8 +211,8 @@ def counter_report(a_csv, b_csv, log):
                   "SQ_ACTIVE_INST_VALU SQ_WAVE_CYCLES", "A", "A", "A"),
            cmd % ("SQ_BUSY_CYCLES SQ_WAIT_INST_ANY SQ_ACTIVE_INST_LDS SQ_ACTIVE_INST_ANY SQ_ACTIVE_INST_SCA "
                   "SQ_WAIT_ANY SQ_INSTS_SMEM SQ_INSTS_VMEM_RD", "B", "B", "B"),
-           "cd $R && python3 bench.py --counter-report gpurun_out/pmcA/<host>/<pid>_pmcA_counter_collection.csv "
-           "gpurun_out/pmcB/<host>/<pid>_pmcB_counter_collection.csv gpurun_out/pmcA.log", "```", "",
+           "cd $R && python3 bench.py --counter-report gpurun_out/pmcA/pmcA_counter_collection.csv "
+           "gpurun_out/pmcB/pmcB_counter_collection.csv gpurun_out/pmcA.log", "```", "",
            "13 substeps from rest (K = %.2f neighbours per particle). Counts are means per dispatch divided "
            "by SQ_WAVES. The VALU floor is the time to issue the dispatch's VALU instructions at one "
            "wave64 instruction per SIMD every %d cycles on all %d SIMDs at %.1f GHz (FP32 without packed "
@@ -247,6 +247,8 @@ def trace_report(stats_csv, trace_csv, log):
     tot = sum(float(r["TotalDurationNs"]) for r in rows)
     if line["metric"].startswith("reverse-pass"):
         return _trace_report_adjoint(rows, tot, line)
+    if "batched ensemble" in line["metric"]:
+        return _trace_report_ensemble(rows, tot, line, trace_csv)
     mixed = "mixed-phase" in line["metric"]
     if mixed:
         title = "765,536-particle mixed scene: cloth, rigid cubes, fluid"
@@ -264,8 +266,8 @@ def trace_report(stats_csv, trace_csv, log):
            "Commands (repository root on the GPU host as R, MI355X):", "", "```",
            "cd /tmp && export TMPDIR=/tmp && rocprofv3 --kernel-trace --stats --output-format csv "
            "-d /tmp/kt -o kt -- python3 $R/bench.py %s > $R/gpurun_out/kt.log 2>&1" % run,
-           "cd $R && python3 bench.py --trace-report /tmp/kt/<host>/<pid>_kernel_stats.csv "
-           "/tmp/kt/<host>/<pid>_kernel_trace.csv gpurun_out/kt.log", "```", "", what,
+           "cd $R && python3 bench.py --trace-report /tmp/kt/kt_kernel_stats.csv "
+           "/tmp/kt/kt_kernel_trace.csv gpurun_out/kt.log", "```", "", what,
            "", "| kernel | calls | avg us | min us | max us | share |", "|---|---|---|---|---|---|"]
     for r in sorted(rows, key=lambda r: -float(r["TotalDurationNs"])):
         out.append("| `%s` | %s | %.1f | %.1f | %.1f | %.1f %% |" % (
@@ -292,6 +294,67 @@ def trace_report(stats_csv, trace_csv, log):
     print("\n".join(out))
 
 
+def _trace_report_ensemble(rows, tot, line, trace_csv):
+    """The kernel-trace summary of a --workload ensemble run: the kernels of
+    the batched handle's graph replays (its stream is the one of the first
+    dispatch of predict_hash_k), their busy time against the replays' span,
+    and the idle gaps between consecutive kernels -- what one persistent
+    launch could at most recover."""
+    c = line["config"]
+    out = ["# Kernel durations: 512-scene batched ensemble (rocprofv3 kernel trace)", "",
+           "Commands (repository root on the GPU host as R, MI355X):", "", "```",
+           "cd /tmp && export TMPDIR=/tmp && rocprofv3 --kernel-trace --stats --output-format csv "
+           "-d /tmp/kte -o kte -- python3 $R/bench.py --workload ensemble --steps %d "
+           "> $R/gpurun_out/kte.log 2>&1" % c["timed_substeps"],
+           "cd $R && python3 bench.py --trace-report /tmp/kte/kte_kernel_stats.csv "
+           "/tmp/kte/kte_kernel_trace.csv gpurun_out/kte.log", "```", "",
+           "%s. One handle steps all scenes with one launch per stage, %d substeps per captured "
+           "graph: one warm-up replay, then %d timed replays. Then the same scenes as %d separate "
+           "handles, twice issued back to back and once one after another (the rows below "
+           "include those runs)." % (c["workload"], c["graph_substeps"],
+                                    c["timed_substeps"] // c["graph_substeps"], c["scenes"]),
+           "", "| kernel | calls | avg us | min us | max us | share |", "|---|---|---|---|---|---|"]
+    for r in sorted(rows, key=lambda r: -float(r["TotalDurationNs"])):
+        out.append("| `%s` | %s | %.1f | %.1f | %.1f | %.1f %% |" % (
+            r["Name"].replace("(anonymous namespace)::", "").split("(")[0].replace("void ", ""), r["Calls"],
+            float(r["AverageNs"]) / 1e3, float(r["MinNs"]) / 1e3, float(r["MaxNs"]) / 1e3,
+            100 * float(r["TotalDurationNs"]) / tot))
+    trace = sorted(csv.DictReader(open(trace_csv)), key=lambda r: int(r["Start_Timestamp"]))
+    first = next(r for r in trace if "predict_hash_k" in r["Kernel_Name"])
+    # the batched phase ends where a predict_hash_k of another grid size (a
+    # per-scene handle) appears
+    stop = next((int(r["Start_Timestamp"]) for r in trace if "predict_hash_k" in r["Kernel_Name"]
+                 and r["Grid_Size_X"] != first["Grid_Size_X"]), None)
+    mine = [r for r in trace if r["Stream_Id"] == first["Stream_Id"]
+            and int(first["Start_Timestamp"]) <= int(r["Start_Timestamp"]) < (stop or 1 << 63)]
+    starts = [i for i, r in enumerate(mine) if "predict_hash_k" in r["Kernel_Name"]]
+    K = c["graph_substeps"]
+    if len(starts) >= 2 * K:
+        timed = mine[starts[K]:]                       # after the warm-up replay
+        t0 = int(timed[0]["Start_Timestamp"])
+        t1 = max(int(r["End_Timestamp"]) for r in timed)
+        busy, gaps, end = 0, [], t0
+        for r in timed:
+            a, b = int(r["Start_Timestamp"]), int(r["End_Timestamp"])
+            if a > end:
+                gaps.append(a - end)
+            busy += max(0, b - max(a, end))
+            end = max(end, b)
+        subs = len(starts) - K
+        out += ["", "The batched handle's timed replays in the same trace: %d substeps, %d dispatches "
+                "(%.1f per substep) over %.2f ms, %.2f ms per substep. Kernels are busy %.2f%% of that "
+                "span; the %d gaps between consecutive kernels sum to %.3f ms (mean %.2f us, max %.2f us), "
+                "%.3f ms per substep." % (subs, len(timed), len(timed) / subs, (t1 - t0) / 1e6,
+                                          (t1 - t0) / 1e6 / subs, 100.0 * busy / (t1 - t0), len(gaps),
+                                          sum(gaps) / 1e6, sum(gaps) / max(len(gaps), 1) / 1e3,
+                                          max(gaps or [0]) / 1e3, sum(gaps) / 1e6 / subs)]
+    out += ["", "The bench line of the same command: batched %.4g particle-iterations/s (%.3f ms per "
+            "substep); per-scene handles issued back to back %.4g (%.2fx), one after another %.4g (%.2fx)."
+            % (line["value"], line["ms_per_step"], c["plain_value"], c["speedup_vs_per_scene"],
+               c["sequential_value"], c["speedup_vs_sequential_scenes"])]
+    print("\n".join(out))
+
+
 def _trace_report_adjoint(rows, tot, line):
     """The kernel-trace summary of a --workload adjoint_cloth_pool run."""
     c = line["config"]
@@ -300,8 +363,8 @@ def _trace_report_adjoint(rows, tot, line):
            "cd /tmp && export TMPDIR=/tmp && rocprofv3 --kernel-trace --stats --output-format csv "
            "-d /tmp/kta -o kta -- python3 $R/bench.py --workload adjoint_cloth_pool --no-cpu "
            "> $R/gpurun_out/kta.log 2>&1",
-           "cd $R && python3 bench.py --trace-report /tmp/kta/<host>/<pid>_kernel_stats.csv "
-           "/tmp/kta/<host>/<pid>_kernel_trace.csv gpurun_out/kta.log", "```", "",
+           "cd $R && python3 bench.py --trace-report /tmp/kta/kta_kernel_stats.csv "
+           "/tmp/kta/kta_kernel_trace.csv gpurun_out/kta.log", "```", "",
            "%s. The run holds one forward pass for the target (perturbed parameters), warm-up, "
            "one timed forward pass, four reverse passes (two with checkpoints every %d "
            "substeps, two with the whole horizon on tape) and one more with events on every "
