@@ -386,7 +386,7 @@ void free_all(ps_system *s)
     DevBuffers &b = s->b;
     dev_free(b.X); dev_free(b.X2); dev_free(b.P); dev_free(b.Q); dev_free(b.V); dev_free(b.V2);
     dev_free(b.A); dev_free(b.A2); dev_free(b.key); dev_free(b.key2); dev_free(b.rank);
-    dev_free(b.tmp_orig); dev_free(b.slot_of); dev_free(b.cell_count); dev_free(b.cell_off);
+    dev_free(b.tmp_orig); dev_free(b.slot_of); dev_free(b.mj); dev_free(b.cell_count); dev_free(b.cell_off);
     dev_free(b.scan_part); dev_free(b.nbr); dev_free(b.ncnt); dev_free(b.dist_off);
     dev_free(b.dist_adj); dev_free(b.dist_pairs); dev_free(b.dist_rs); dev_free(b.occ);
     dev_free(b.pin_of); dev_free(b.pin_xyz); dev_free(b.body_off); dev_free(b.body_idx);
@@ -505,7 +505,9 @@ int stage_sort(ps_system *s)
 {
     LAUNCH(s, PS_K_SCAN, ps::launch_scan(s->stream, s->b, s->ncell));
     LAUNCH(s, PS_K_SCATTER, ps::launch_scatter(s->stream, s->b, s->n));
-    LAUNCH(s, PS_K_REORDER, ps::launch_reorder(s->stream, s->b, s->n, s->block[PS_K_REORDER]));
+    // the neighbour masses for the fluid sums of scenes that are not a uniform fluid
+    LAUNCH(s, PS_K_REORDER, ps::launch_reorder(s->stream, s->b, s->n, s->block[PS_K_REORDER],
+                                               uniform_fluid(s) ? -1.0f : s->kc.s_solid));
     DevBuffers &b = s->b;
     std::swap(b.X, b.X2);
     std::swap(b.P, b.Q);
@@ -1024,7 +1026,7 @@ static int create_impl(const ps_params *params, uint32_t nscenes, const ps_scene
     } while (0)
     ALLOC(b.X, c); ALLOC(b.X2, c); ALLOC(b.P, c); ALLOC(b.Q, c); ALLOC(b.V, c); ALLOC(b.V2, c);
     ALLOC(b.A, c); ALLOC(b.A2, c); ALLOC(b.key, c); ALLOC(b.key2, c); ALLOC(b.rank, c);
-    ALLOC(b.tmp_orig, c); ALLOC(b.slot_of, c);
+    ALLOC(b.tmp_orig, c); ALLOC(b.slot_of, c); ALLOC(b.mj, c);
     ALLOC(b.cell_count, s->ncell); ALLOC(b.cell_off, (size_t)s->ncell + 1);
     ALLOC(b.scan_part, s->tiles);
     ALLOC(b.nbr, groups * 64 * s->kmax); ALLOC(b.ncnt, c);

@@ -271,6 +271,12 @@ struct PX {
     float4 p, x;
 };
 
+// position and neighbour density mass (DevBuffers::mj)
+struct PM {
+    float4 p;
+    float m;
+};
+
 // entry q of particle i (single-entry access for the less travelled loops)
 __device__ __forceinline__ uint32_t nbr_at(const DevBuffers &b, uint32_t i, uint32_t q)
 {

@@ -125,6 +125,10 @@ struct DevBuffers {
                            // track_slots: distance constraints or bodies exist)
     uint32_t track_slots;
     uint32_t *sid, *sid2;  // scene of each slot (batches only)
+    float *mj;             // per slot, written by the sort for scenes that are not
+                           // a uniform fluid: the particle's density mass as a
+                           // neighbour, 1/w (times s_solid if not fluid), or -1
+                           // for w = 0 (left out of the fluid sums)
     SceneDev *scenes;
     // grid
     uint32_t *cell_count;  // num_cells, zero between builds
@@ -250,7 +254,7 @@ hipError_t launch_predict_hash(hipStream_t st, const DevBuffers &b, const KConst
                                float dt, bool predict, bool hash, uint32_t bs);
 hipError_t launch_scan(hipStream_t st, const DevBuffers &b, uint32_t num_cells);
 hipError_t launch_scatter(hipStream_t st, const DevBuffers &b, uint32_t n);
-hipError_t launch_reorder(hipStream_t st, const DevBuffers &b, uint32_t n, uint32_t bs);
+hipError_t launch_reorder(hipStream_t st, const DevBuffers &b, uint32_t n, uint32_t bs, float mj_solid);
 hipError_t launch_neighbors(hipStream_t st, const DevBuffers &b, const KConst &k, uint32_t n,
                             uint32_t kmax, uint32_t bs);
 hipError_t launch_lambda(hipStream_t st, const DevBuffers &b, const KConst &k, uint32_t n,

@@ -254,7 +254,9 @@ __global__ __launch_bounds__(kBlock) void scatter_k(DevBuffers b, uint32_t n)
 // index): a stable sort of insertion-ordered input, as thrust::sort_by_key
 // of (hash, index) in sortParticles (integration.cu:270-275) is meant to
 // give.  Every per-particle array moves with the particle.
-__global__ __launch_bounds__(kMaxBlock) void reorder_k(DevBuffers b, uint32_t n)
+// mj_solid >= 0: also write mj (the neighbour density mass) with s_solid =
+// mj_solid.
+__global__ __launch_bounds__(kMaxBlock) void reorder_k(DevBuffers b, uint32_t n, float mj_solid)
 {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
@@ -283,6 +285,15 @@ __global__ __launch_bounds__(kMaxBlock) void reorder_k(DevBuffers b, uint32_t n)
     b.V2[d] = vs;
     b.A2[d] = a;
     b.key2[d] = c;
+    if (mj_solid >= 0.0f) {
+        // the neighbour mass of the fluid sums, as they form it per pair
+        float m = -1.0f;
+        if (a.w != 0.0f) {
+            m = 1.0f / a.w;
+            if (a.phase != PS_PHASE_FLUID) m = m * mj_solid;
+        }
+        b.mj[d] = m;
+    }
     if (b.sid) b.sid2[d] = b.sid[s];
     // a scatter by insertion index: skipped when no kernel reads it
     if (b.track_slots) b.slot_of[a.orig] = d;
@@ -711,15 +722,14 @@ __global__ __launch_bounds__(kMaxBlock) void lambda_k(DevBuffers b, KConst k, ui
             [&](uint32_t, const float4 &pj) { body(s0, pj, mu); },
             [&](uint32_t, const float4 &pj) { body(s1, pj, mu); });
     } else {
-        auto gen = [&](Sums &s, const PA &d) {
-            if (d.a.w == 0.0f) return;
-            float mj = 1.0f / d.a.w;
-            if (d.a.phase != PS_PHASE_FLUID) mj = mj * k.s_solid;
-            body(s, d.p, mj);
+        // the neighbour's mass from the sort's mj (-1: w = 0, left out)
+        auto gen = [&](Sums &s, const PM &d) {
+            if (d.m < 0.0f) return;
+            body(s, d.p, d.m);
         };
         for_neighbors_split(
-            b, i, c, [&](uint32_t j) { return PA{b.P[j], b.A[j]}; },
-            [&](uint32_t, const PA &d) { gen(s0, d); }, [&](uint32_t, const PA &d) { gen(s1, d); });
+            b, i, c, [&](uint32_t j) { return PM{b.P[j], b.mj[j]}; },
+            [&](uint32_t, const PM &d) { gen(s0, d); }, [&](uint32_t, const PM &d) { gen(s1, d); });
     }
     const float rho = __builtin_fmaf(k.poly6, s0.rho + s1.rho, self);
     const float gix = sk * (s0.gix + s1.gix), giy = sk * (s0.giy + s1.giy), giz = sk * (s0.giz + s1.giz);
@@ -944,23 +954,24 @@ __global__ __launch_bounds__(kMaxBlock) void solve_k(DevBuffers b, KConst k, uin
                                              [&](uint32_t, const float4 &pj) { exact(s1, pj); });
                 }
             } else {
-                auto gather = [&](uint32_t j) { return PA{b.P[j], b.A[j]}; };
+                // neighbours with w = 0 are left out (mj < 0)
+                auto gather = [&](uint32_t j) { return PM{b.P[j], b.mj[j]}; };
                 for_neighbors_split(
                     b, i, c, gather,
-                    [&](uint32_t, const PA &d) {
-                        if (d.a.w != 0.0f) fast(s0, d.p);
+                    [&](uint32_t, const PM &d) {
+                        if (d.m >= 0.0f) fast(s0, d.p);
                     },
-                    [&](uint32_t, const PA &d) {
-                        if (d.a.w != 0.0f) fast(s1, d.p);
+                    [&](uint32_t, const PM &d) {
+                        if (d.m >= 0.0f) fast(s1, d.p);
                     });
                 if (s0.deg || s1.deg) {
                     s0 = {0.0f, 0.0f, 0.0f, false};
                     s1 = {0.0f, 0.0f, 0.0f, false};
-                    for_neighbors_half_plain(b, i, c, 0u, gather, [&](uint32_t, const PA &d) {
-                        if (d.a.w != 0.0f) exact(s0, d.p);
+                    for_neighbors_half_plain(b, i, c, 0u, gather, [&](uint32_t, const PM &d) {
+                        if (d.m >= 0.0f) exact(s0, d.p);
                     });
-                    for_neighbors_half_plain(b, i, c, 1u, gather, [&](uint32_t, const PA &d) {
-                        if (d.a.w != 0.0f) exact(s1, d.p);
+                    for_neighbors_half_plain(b, i, c, 1u, gather, [&](uint32_t, const PM &d) {
+                        if (d.m >= 0.0f) exact(s1, d.p);
                     });
                 }
             }
@@ -1321,10 +1332,10 @@ hipError_t launch_scatter(hipStream_t st, const DevBuffers &b, uint32_t n)
     return hipGetLastError();
 }
 
-hipError_t launch_reorder(hipStream_t st, const DevBuffers &b, uint32_t n, uint32_t bs)
+hipError_t launch_reorder(hipStream_t st, const DevBuffers &b, uint32_t n, uint32_t bs, float mj_solid)
 {
     if (n == 0) return hipSuccess;
-    PS_LAUNCH(reorder_k, dim3(blocks_for(n, bs)), dim3(bs), 0, st, b, n);
+    PS_LAUNCH(reorder_k, dim3(blocks_for(n, bs)), dim3(bs), 0, st, b, n, mj_solid);
     return hipGetLastError();
 }
 
