@@ -106,6 +106,11 @@ constexpr uint32_t kStageChunks = kStageSlots / kStageChunk;
 // the staged kernels visit whole chunks without bound checks.
 constexpr uint32_t kStageSentinel = kStageSlots;
 
+// Contacts a particle of the global-slot solve lists in LDS during its
+// counting walk; the corrections visit the listed ones, or walk the whole
+// table again when there are more.
+constexpr uint32_t kContactList = 16;
+
 struct StageDesc {
     uint32_t staged;              // 1: the unit's table entries are LDS indices
     uint32_t nchunks;             // chunks in use
@@ -114,7 +119,8 @@ struct StageDesc {
 
 struct DevBuffers {
     // particle state, sorted storage order
-    float4 *X, *X2;  // positions at substep start (x*), .w unused
+    float4 *X, *X2;  // positions at substep start (x*); .w: in scenes that are not
+                     // a uniform fluid, the sort's XSPH mass (1/w fluid, -1 other)
     float4 *P, *Q;   // iterate (x, y, z, lambda) ping-pong
     float4 *V, *V2;  // velocities
     Attr *A, *A2;

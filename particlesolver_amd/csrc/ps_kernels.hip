@@ -280,7 +280,11 @@ __global__ __launch_bounds__(kMaxBlock) void reorder_k(DevBuffers b, uint32_t n,
         for (uint32_t k = 0; k < 4; k++) r += (u + k < b1 && t[k] < a.orig) ? 1u : 0u;
     }
     const uint32_t d = b0 + r;
-    b.X2[d] = xs;
+    // scenes that are not a uniform fluid: x*.w carries the particle's XSPH
+    // mass as a neighbour, 1/w for a fluid particle, -1 for any other
+    b.X2[d] = mj_solid >= 0.0f
+                  ? make_float4(xs.x, xs.y, xs.z, a.phase == PS_PHASE_FLUID ? 1.0f / a.w : -1.0f)
+                  : xs;
     b.Q[d] = make_float4(p.x, p.y, p.z, 0.0f);
     b.V2[d] = vs;
     b.A2[d] = a;
@@ -761,6 +765,9 @@ __global__ __launch_bounds__(kMaxBlock) void solve_k(DevBuffers b, KConst k, uin
     // half hh of the fluid sum; lane 0 finishes the particle.  The LDS holds
     // the unit's slots and the sentinel record
     __shared__ float4 lds[STAGED ? kStageSlots + 1 : 1];
+    // the contact lists of the block (global-slot variants with contacts):
+    // kContactList entries per thread, entry q of thread t at q blockDim + t
+    extern __shared__ uint32_t contact_lds[];
     const uint32_t unit = xcd_block();
     const uint32_t hh = STAGED ? (threadIdx.x & 1u) : 0u;
     const uint32_t i = STAGED ? unit * kStageUnit + (threadIdx.x >> 1) : unit * blockDim.x + threadIdx.x;
@@ -787,15 +794,23 @@ __global__ __launch_bounds__(kMaxBlock) void solve_k(DevBuffers b, KConst k, uin
     // a7: particle contacts + friction (collideD, integration_kernel.cuh:358-462)
     if (F & F_CONTACT) {
         if (ph >= PS_PHASE_CLOTH && wi != 0.0f) {
-            // both walks over the table are pipelined a chunk at a time
+            // the counting walk over the table is pipelined a chunk at a time
+            // and lists the first kContactList contacts (slots, table order)
+            // in this thread's column of the block's LDS; the corrections
+            // then visit only those, or walk the table again past that many
             auto pa = [&](uint32_t j) { return PA{b.P[j], b.A[j]}; };
+            uint32_t *clist = contact_lds + threadIdx.x;
+            const uint32_t cstride = STAGED ? 0u : blockDim.x;
             uint32_t nc = 0;
-            for_neighbors_chunked(b, i, c, pa, [&](uint32_t, const PA &d) {
+            for_neighbors_chunked(b, i, c, pa, [&](uint32_t j, const PA &d) {
                 if (ph > PS_PHASE_SOLID && ph == d.a.phase) return;
                 const float4 qj = d.p;
                 float dx = px - qj.x, dy = py - qj.y, dz = pz - qj.z;
                 float r2 = ((dx * dx) + (dy * dy)) + (dz * dz);
-                if (r2 < k.collide_r2) nc++;
+                if (r2 < k.collide_r2) {
+                    if (!STAGED && nc < kContactList) clist[nc * cstride] = j;
+                    nc++;
+                }
             });
             if (TAPE) tape.w = __uint_as_float(nc);
             if (nc > 0) {
@@ -803,7 +818,7 @@ __global__ __launch_bounds__(kMaxBlock) void solve_k(DevBuffers b, KConst k, uin
                 const float ewi = wi * ps_exp(py);
                 const float4 xi = b.X[i];
                 float cx = 0.0f, cy = 0.0f, cz = 0.0f;
-                for_neighbors_chunked(b, i, c, pa, [&](uint32_t j, const PA &d) {
+                auto correct = [&](uint32_t j, const PA &d) {
                     const Attr aj = d.a;
                     const int32_t pj = aj.phase;
                     if (ph > PS_PHASE_SOLID && ph == pj) return;
@@ -852,7 +867,23 @@ __global__ __launch_bounds__(kMaxBlock) void solve_k(DevBuffers b, KConst k, uin
                         cy = cy - ((ty * f) * share);
                         cz = cz - ((tz * f) * share);
                     }
-                });
+                };
+                if (!STAGED && nc <= kContactList) {
+                    // the listed contacts in order, four records in flight
+                    for (uint32_t q0 = 0; q0 < nc; q0 += 4u) {
+                        uint32_t jq[4];
+                        PA dq[4];
+#pragma unroll
+                        for (uint32_t u = 0; u < 4; u++) jq[u] = clist[min(q0 + u, nc - 1u) * cstride];
+#pragma unroll
+                        for (uint32_t u = 0; u < 4; u++) dq[u] = pa(jq[u]);
+#pragma unroll
+                        for (uint32_t u = 0; u < 4; u++)
+                            if (q0 + u < nc) correct(jq[u], dq[u]);
+                    }
+                } else {
+                    for_neighbors_chunked(b, i, c, pa, correct);
+                }
                 ax = ax + cx;
                 ay = ay + cy;
                 az = az + cz;
@@ -1227,10 +1258,6 @@ __global__ __launch_bounds__(kMaxBlock) void finalize_k(DevBuffers b, KConst k, 
             float4 pu;
             float2 u;
         };
-        struct PXA {
-            float4 p, x;
-            Attr a;
-        };
         auto glob = [&](Sums &s, const PX &d) {
             body(s, d.p, d.p.x - d.x.x, d.p.y - d.x.y, d.p.z - d.x.z, mu);
         };
@@ -1255,13 +1282,15 @@ __global__ __launch_bounds__(kMaxBlock) void finalize_k(DevBuffers b, KConst k, 
                 b, i, c, [&](uint32_t j) { return PX{b.P[j], b.X[j]}; },
                 [&](uint32_t, const PX &d) { glob(s0, d); }, [&](uint32_t, const PX &d) { glob(s1, d); });
         } else {
-            auto gen = [&](Sums &s, const PXA &d) {
-                if (d.a.phase != PS_PHASE_FLUID) return;
-                body(s, d.p, d.p.x - d.x.x, d.p.y - d.x.y, d.p.z - d.x.z, 1.0f / d.a.w);
+            // the neighbour's mass from x*.w (the sort's 1/w for a fluid
+            // particle, -1 for any other: left out)
+            auto gen = [&](Sums &s, const PX &d) {
+                if (d.x.w < 0.0f) return;
+                body(s, d.p, d.p.x - d.x.x, d.p.y - d.x.y, d.p.z - d.x.z, d.x.w);
             };
             for_neighbors_split(
-                b, i, c, [&](uint32_t j) { return PXA{b.P[j], b.X[j], b.A[j]}; },
-                [&](uint32_t, const PXA &d) { gen(s0, d); }, [&](uint32_t, const PXA &d) { gen(s1, d); });
+                b, i, c, [&](uint32_t j) { return PX{b.P[j], b.X[j]}; },
+                [&](uint32_t, const PX &d) { gen(s0, d); }, [&](uint32_t, const PX &d) { gen(s1, d); });
         }
         const float sx = k.poly6 * (s0.sx + s1.sx), sy = k.poly6 * (s0.sy + s1.sy),
                     sz = k.poly6 * (s0.sz + s1.sz);
@@ -1388,17 +1417,19 @@ static void solve_variant(hipStream_t st, const DevBuffers &b, const KConst &k, 
     const dim3 g(blocks_for(n, bs)), t(bs);
     const dim3 gs(blocks_for(n, kStageUnit)), ts(2 * kStageUnit);  // two lanes per particle
     const bool n4 = k.n_corr == 4u;
+    // the contact lists of a block (the global-slot variants)
+    const size_t cl = (F & F_CONTACT) ? sizeof(uint32_t) * kContactList * bs : 0;
     // the staged variant exists for the uniform pure-fluid composition only
     constexpr bool CAN_STAGE = F == F_FLUID;
     if (b.tape_f) {  // recording for the reverse pass (never staged)
         if (uniform && n4)
-            PS_LAUNCH((solve_k<F, true, 4, false, true>), g, t, 0, st, b, k, n, wu, iter);
+            PS_LAUNCH((solve_k<F, true, 4, false, true>), g, t, cl, st, b, k, n, wu, iter);
         else if (uniform)
-            PS_LAUNCH((solve_k<F, true, 0, false, true>), g, t, 0, st, b, k, n, wu, iter);
+            PS_LAUNCH((solve_k<F, true, 0, false, true>), g, t, cl, st, b, k, n, wu, iter);
         else if (n4)
-            PS_LAUNCH((solve_k<F, false, 4, false, true>), g, t, 0, st, b, k, n, wu, iter);
+            PS_LAUNCH((solve_k<F, false, 4, false, true>), g, t, cl, st, b, k, n, wu, iter);
         else
-            PS_LAUNCH((solve_k<F, false, 0, false, true>), g, t, 0, st, b, k, n, wu, iter);
+            PS_LAUNCH((solve_k<F, false, 0, false, true>), g, t, cl, st, b, k, n, wu, iter);
         return;
     }
     if (CAN_STAGE && uniform && b.staging && n4)
@@ -1406,13 +1437,13 @@ static void solve_variant(hipStream_t st, const DevBuffers &b, const KConst &k, 
     else if (CAN_STAGE && uniform && b.staging)
         PS_LAUNCH((solve_k<F, true, 0, CAN_STAGE>), gs, ts, 0, st, b, k, n, wu, iter);
     else if (uniform && n4)
-        PS_LAUNCH((solve_k<F, true, 4, false>), g, t, 0, st, b, k, n, wu, iter);
+        PS_LAUNCH((solve_k<F, true, 4, false>), g, t, cl, st, b, k, n, wu, iter);
     else if (uniform)
-        PS_LAUNCH((solve_k<F, true, 0, false>), g, t, 0, st, b, k, n, wu, iter);
+        PS_LAUNCH((solve_k<F, true, 0, false>), g, t, cl, st, b, k, n, wu, iter);
     else if (n4)
-        PS_LAUNCH((solve_k<F, false, 4, false>), g, t, 0, st, b, k, n, wu, iter);
+        PS_LAUNCH((solve_k<F, false, 4, false>), g, t, cl, st, b, k, n, wu, iter);
     else
-        PS_LAUNCH((solve_k<F, false, 0, false>), g, t, 0, st, b, k, n, wu, iter);
+        PS_LAUNCH((solve_k<F, false, 0, false>), g, t, cl, st, b, k, n, wu, iter);
 }
 
 hipError_t launch_solve(hipStream_t st, const DevBuffers &b, const KConst &k, uint32_t n,
