@@ -1012,8 +1012,12 @@ static void solve_pass(orc_sys *s)
             if (nc > 0) {
                 real fnc = (real)nc;
                 real ewi = wi * EXP(py);
-                real cx = R(0), cy = R(0), cz = R(0);
+                /* the corrections are summed in the two halves of the table
+                 * (HALF), each in table order, then added half 0 + half 1,
+                 * as the fluid sums are */
+                real ch[2][3] = {{R(0), R(0), R(0)}, {R(0), R(0), R(0)}};
                 for (uint32_t k = 0; k < s->ncnt[i]; k++) {
+                    real *c3 = ch[HALF(k)];
                     uint32_t j = s->nbr[(size_t)i * kmax + k];
                     int32_t pj = s->phase[j];
                     if (ph > PS_PHASE_SOLID && ph == pj) continue;
@@ -1036,9 +1040,9 @@ static void solve_pass(orc_sys *s)
                     /* dp1 = -cw * (mag / wsum) * d / dist / nc */
                     real sc = (((mag / wsum) / dist) * cw) / fnc;
                     real ex = -(sc * dx), ey = -(sc * dy), ez = -(sc * dz);
-                    cx = cx + ex;
-                    cy = cy + ey;
-                    cz = cz + ez;
+                    c3[0] = c3[0] + ex;
+                    c3[1] = c3[1] + ey;
+                    c3[2] = c3[2] + ez;
                     if (!solid) continue;
                     real sc2 = (((mag / wsum) / dist) * cw2) / fnc;
                     real fx = sc2 * dx, fy = sc2 * dy, fz = sc2 * dz;
@@ -1054,17 +1058,18 @@ static void solve_pass(orc_sys *s)
                     if (lt < R(0.001)) continue;
                     real share = cw / wsum;
                     if (lt < (d->mu_s * dist)) {
-                        cx = cx - (tx * share);
-                        cy = cy - (ty * share);
-                        cz = cz - (tz * share);
+                        c3[0] = c3[0] - (tx * share);
+                        c3[1] = c3[1] - (ty * share);
+                        c3[2] = c3[2] - (tz * share);
                     } else {
                         real f = (d->mu_k * dist) / lt;
                         if (f > R(1)) f = R(1);
-                        cx = cx - ((tx * f) * share);
-                        cy = cy - ((ty * f) * share);
-                        cz = cz - ((tz * f) * share);
+                        c3[0] = c3[0] - ((tx * f) * share);
+                        c3[1] = c3[1] - ((ty * f) * share);
+                        c3[2] = c3[2] - ((tz * f) * share);
                     }
                 }
+                real cx = ch[0][0] + ch[1][0], cy = ch[0][1] + ch[1][1], cz = ch[0][2] + ch[1][2];
                 ax = ax + cx;
                 ay = ay + cy;
                 az = az + cz;

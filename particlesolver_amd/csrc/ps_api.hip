@@ -46,6 +46,11 @@ struct ps_system {
     // scene composition
     bool uniform = true;     // every particle fluid with one inverse mass
     float wu = 1.0f;
+    // every fluid particle of one inverse mass wf (any other phases): the
+    // condition for staging a scene that is not a uniform fluid
+    bool fluid_mass_uniform = true;
+    bool any_fluid = false;
+    float wf = 1.0f;
     bool any_contact_phase = false;
     bool any_solid_phase = false;  // SOLID or RIGID particles present
     // host copies of the constraint sets (insertion indices)
@@ -519,13 +524,16 @@ int stage_sort(ps_system *s)
     return PS_OK;
 }
 
-// Neighbour cells are staged in LDS for uniform single-scene fluids (the
-// composition whose kernels have staged variants).
-bool staging_ok(const ps_system *s)
+// Neighbour cells can be staged in LDS for a single scene, unless the
+// caller turned it off by flag; the autotuner may turn it off too.  (Staged
+// XSPH takes the fluid mass as a constant: with fluid masses that differ,
+// finalize gathers from global memory through the units' chunk tables.)
+bool can_stage(const ps_system *s)
 {
-    return !s->cpuref && !s->adj_active && !(s->prm.flags & PS_FLAG_NO_LDS_STAGING) &&
-           s->kc.nscenes == 1 && uniform_fluid(s) && s->stage_choice != 0;
+    return !s->cpuref && !(s->prm.flags & PS_FLAG_NO_LDS_STAGING) && s->kc.nscenes == 1;
 }
+
+bool staging_ok(const ps_system *s) { return can_stage(s) && !s->adj_active && s->stage_choice != 0; }
 
 int stage_neighbors(ps_system *s)
 {
@@ -564,7 +572,8 @@ int stage_end(ps_system *s)
     const bool uni = uniform_fluid(s);
     LAUNCH(s, PS_K_FINALIZE,
            ps::launch_finalize(s->stream, s->b, s->kc, s->n, s->dt,
-                               (s->prm.flags & PS_FLAG_SLEEPING) != 0, uni, s->wu, s->it_sub,
+                               (s->prm.flags & PS_FLAG_SLEEPING) != 0, uni, s->wu,
+                               s->fluid_mass_uniform ? s->wf : -1.0f, s->it_sub,
                                s->block[PS_K_FINALIZE]));
     s->it_sub = 0;
     std::swap(s->b.X, s->b.P);
@@ -1342,6 +1351,8 @@ static int add_particles_impl(ps_system *s, uint32_t scene, uint32_t n, const fl
     bool uni = s->uniform;
     float wu = s->n ? s->wu : inv_mass[0];
     bool contact = s->any_contact_phase, solid = s->any_solid_phase;
+    bool fmu = s->fluid_mass_uniform, anyf = s->any_fluid;
+    float wf = s->wf;
     for (uint32_t k = 0; k < n; k++) {
         for (int a = 0; a < 3; a++) {
             if (!is_finite(pos[3 * k + a]) || (vel && !is_finite(vel[3 * k + a])))
@@ -1364,6 +1375,11 @@ static int add_particles_impl(ps_system *s, uint32_t scene, uint32_t n, const fl
         A[k].orig = s->n + k;
         slot[k] = s->n + k;
         if (phase[k] != PS_PHASE_FLUID || inv_mass[k] != wu) uni = false;
+        if (phase[k] == PS_PHASE_FLUID) {
+            if (!anyf) wf = inv_mass[k];
+            else if (inv_mass[k] != wf) fmu = false;
+            anyf = true;
+        }
         if (phase[k] >= PS_PHASE_CLOTH) contact = true;
         if (phase[k] >= PS_PHASE_SOLID) solid = true;
     }
@@ -1397,6 +1413,9 @@ static int add_particles_impl(ps_system *s, uint32_t scene, uint32_t n, const fl
     s->n += n;
     s->uniform = uni;
     s->wu = wu;
+    s->fluid_mass_uniform = fmu;
+    s->any_fluid = anyf;
+    s->wf = wf;
     s->any_contact_phase = contact;
     s->any_solid_phase = solid;
     s->nbr_valid = false;
@@ -2460,9 +2479,10 @@ int ps_autotune(ps_system *s, float dt, uint32_t substeps, uint32_t *blocks_out)
         else
             need = true;
     }
-    // the second knob: LDS staging of each unit's neighbour cells (uniform
-    // single-scene fluids, unless the caller turned it off by flag)
-    const bool stageable = s->kc.nscenes == 1 && uniform_fluid(s) && !(s->prm.flags & PS_FLAG_NO_LDS_STAGING);
+    // the second knob: LDS staging of each unit's neighbour cells (single
+    // scenes whose fluid particles share one mass, unless the caller turned
+    // it off by flag)
+    const bool stageable = can_stage(s);
     if (stageable) {
         auto it = g_tune.find({kTuneStaging, bucket, feat});
         if (it != g_tune.end())

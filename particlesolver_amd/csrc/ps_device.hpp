@@ -262,6 +262,31 @@ __device__ __forceinline__ void for_neighbors_chunked(const DevBuffers &b, uint3
     }
 }
 
+// As for_neighbors_chunked, and fn(h, j, record) is told the half h of the
+// table the entry lies in (its chunk's parity).
+template <typename G, typename Fn>
+__device__ __forceinline__ void for_neighbors_chunked_h(const DevBuffers &b, uint32_t i, uint32_t c, G gather,
+                                                        Fn fn)
+{
+    if (c == 0) return;
+    const uint4 *nb = nbr_lane(b, i);
+    const uint32_t last = (c - 1) >> 2;
+    uint4 jj = nb[0];
+    for (uint32_t m = 0; m <= last; m++) {
+        const uint4 jn = nb[(size_t)min(m + 1, last) * 64];
+        auto d0 = gather(jj.x);
+        auto d1 = gather(jj.y);
+        auto d2 = gather(jj.z);
+        auto d3 = gather(jj.w);
+        const uint32_t q = m << 2, h = m & 1u;
+        fn(h, jj.x, d0);
+        if (q + 1 < c) fn(h, jj.y, d1);
+        if (q + 2 < c) fn(h, jj.z, d2);
+        if (q + 3 < c) fn(h, jj.w, d3);
+        jj = jn;
+    }
+}
+
 struct PA {
     float4 p;
     Attr a;

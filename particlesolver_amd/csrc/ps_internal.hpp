@@ -110,6 +110,9 @@ constexpr uint32_t kStageSentinel = kStageSlots;
 // counting walk; the corrections visit the listed ones, or walk the whole
 // table again when there are more.
 constexpr uint32_t kContactList = 16;
+// the same per lane in the staged solve (two lanes per particle, each over
+// one half of the table)
+constexpr uint32_t kHalfContactList = 8;
 
 struct StageDesc {
     uint32_t staged;              // 1: the unit's table entries are LDS indices
@@ -271,7 +274,7 @@ hipError_t launch_shape(hipStream_t st, const DevBuffers &b, uint32_t nbody);
 hipError_t launch_solve(hipStream_t st, const DevBuffers &b, const KConst &k, uint32_t n,
                         uint32_t features, bool uniform, float wu, uint32_t it, uint32_t bs);
 hipError_t launch_finalize(hipStream_t st, const DevBuffers &b, const KConst &k, uint32_t n,
-                           float dt, bool sleeping, bool uniform, float wu, uint32_t its,
+                           float dt, bool sleeping, bool uniform, float wu, float wf, uint32_t its,
                            uint32_t bs);
 
 uint32_t scan_tiles(uint32_t num_cells);

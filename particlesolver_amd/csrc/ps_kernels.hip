@@ -635,11 +635,14 @@ __global__ __launch_bounds__(kStageUnit) void neighbors_staged_k(DevBuffers b, K
 template <bool UNIFORM, bool STAGED, bool TAPE = false>
 __global__ __launch_bounds__(kMaxBlock) void lambda_k(DevBuffers b, KConst k, uint32_t n, float mu)
 {
-    // staged: neighbour positions only, 12 bytes per slot (36 KB, so four
-    // blocks fit in a CU's LDS where 16-byte slots allow three), and the
-    // sentinel record; two lanes per particle (2 kStageUnit threads), lane
-    // hh summing half hh
-    __shared__ float lds3[STAGED ? 3 * (kStageSlots + 1) : 1];
+    // staged: a uniform fluid stages neighbour positions only, 12 bytes
+    // per slot (36 KB, so four blocks fit in a CU's LDS where 16-byte slots
+    // allow three); other scenes stage (x, y, z, mj), a particle with w = 0
+    // as a position at infinity, which every pair test rejects as the
+    // global path leaves it out.  Then the sentinel record; two lanes per
+    // particle (2 kStageUnit threads), lane hh summing half hh
+    __shared__ float lds3[STAGED && UNIFORM ? 3 * (kStageSlots + 1) : 1];
+    __shared__ float4 lds4[STAGED && !UNIFORM ? kStageSlots + 1 : 1];
     const uint32_t unit = xcd_block();
     const uint32_t hh = STAGED ? (threadIdx.x & 1u) : 0u;
     const uint32_t i = STAGED ? unit * kStageUnit + (threadIdx.x >> 1) : unit * blockDim.x + threadIdx.x;
@@ -647,14 +650,25 @@ __global__ __launch_bounds__(kMaxBlock) void lambda_k(DevBuffers b, KConst k, ui
     if (STAGED) {
         const StageDesc *d = b.stage + unit;
         staged = d->staged != 0u;
-        if (staged && threadIdx.x < 3) lds3[3 * kStageSentinel + threadIdx.x] = __builtin_inff();
-        if (staged)
-            stage_copy<6, 2 * kStageUnit>(d, n, [&](uint32_t j) { return b.P[j]; },
-                                          [&](uint32_t l, const float4 &v) {
-                                              lds3[3 * l] = v.x;
-                                              lds3[3 * l + 1] = v.y;
-                                              lds3[3 * l + 2] = v.z;
-                                          });
+        if (UNIFORM) {
+            if (staged && threadIdx.x < 3) lds3[3 * kStageSentinel + threadIdx.x] = __builtin_inff();
+            if (staged)
+                stage_copy<6, 2 * kStageUnit>(d, n, [&](uint32_t j) { return b.P[j]; },
+                                              [&](uint32_t l, const float4 &v) {
+                                                  lds3[3 * l] = v.x;
+                                                  lds3[3 * l + 1] = v.y;
+                                                  lds3[3 * l + 2] = v.z;
+                                              });
+        } else {
+            const float inf = __builtin_inff();
+            if (staged && threadIdx.x == 0) lds4[kStageSentinel] = make_float4(inf, inf, inf, 0.0f);
+            if (staged)
+                stage_copy<6, 2 * kStageUnit>(d, n, [&](uint32_t j) { return PM{b.P[j], b.mj[j]}; },
+                                              [&](uint32_t l, const PM &v) {
+                                                  lds4[l] = v.m < 0.0f ? make_float4(inf, inf, inf, 0.0f)
+                                                                       : make_float4(v.p.x, v.p.y, v.p.z, v.m);
+                                              });
+        }
     }
     if (i >= n) return;
     float *Pw = reinterpret_cast<float *>(b.P);
@@ -701,7 +715,7 @@ __global__ __launch_bounds__(kMaxBlock) void lambda_k(DevBuffers b, KConst k, ui
         // was not staged); the partner lane (lane ^ 1) holds the other.
         // Both lanes form half 0 + half 1 (addition commutes)
         Sums mine = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-        if (staged)
+        if (UNIFORM && staged)
             for_neighbors_half<false>(
                 b, i, c, hh,
                 [&](uint32_t l) {
@@ -711,10 +725,19 @@ __global__ __launch_bounds__(kMaxBlock) void lambda_k(DevBuffers b, KConst k, ui
                     return make_float4(q[0], q[1], q[2], 0.0f);
                 },
                 [&](uint32_t, const float4 &pj) { body(mine, pj, mu); });
-        else
+        else if (UNIFORM)
             for_neighbors_half(
                 b, i, c, hh, [&](uint32_t j) { return b.P[j]; },
                 [&](uint32_t, const float4 &pj) { body(mine, pj, mu); });
+        else if (staged)
+            for_neighbors_half<false>(b, i, c, hh, [&](uint32_t l) { return lds4[l]; },
+                                      [&](uint32_t, const float4 &pj) { body(mine, pj, pj.w); });
+        else
+            for_neighbors_half(
+                b, i, c, hh, [&](uint32_t j) { return PM{b.P[j], b.mj[j]}; },
+                [&](uint32_t, const PM &d) {
+                    if (d.m >= 0.0f) body(mine, d.p, d.m);
+                });
         const Sums other = {__shfl_xor(mine.rho, 1, 64), __shfl_xor(mine.gix, 1, 64),
                             __shfl_xor(mine.giy, 1, 64), __shfl_xor(mine.giz, 1, 64),
                             __shfl_xor(mine.den, 1, 64)};
@@ -764,7 +787,12 @@ __global__ __launch_bounds__(kMaxBlock) void solve_k(DevBuffers b, KConst k, uin
     // staged: two lanes per particle (2 kStageUnit threads), lane hh summing
     // half hh of the fluid sum; lane 0 finishes the particle.  The LDS holds
     // the unit's slots and the sentinel record
+    // In a scene that is not a uniform fluid a particle with w = 0 is
+    // staged as a position at infinity (the fluid sum leaves it out, as the
+    // global path does); the contact walk reads global memory, through the
+    // unit's chunk table (s_src) in a staged unit.
     __shared__ float4 lds[STAGED ? kStageSlots + 1 : 1];
+    __shared__ uint32_t s_src[STAGED && (F & F_CONTACT) ? kStageChunks : 1];
     // the contact lists of the block (global-slot variants with contacts):
     // kContactList entries per thread, entry q of thread t at q blockDim + t
     extern __shared__ uint32_t contact_lds[];
@@ -775,12 +803,25 @@ __global__ __launch_bounds__(kMaxBlock) void solve_k(DevBuffers b, KConst k, uin
     if (STAGED) {
         const StageDesc *d = b.stage + unit;
         staged = d->staged != 0u;
-        if (staged && threadIdx.x == 0)
-            lds[kStageSentinel] = make_float4(__builtin_inff(), __builtin_inff(), __builtin_inff(), 0.0f);
-        if (staged)
+        const float inf = __builtin_inff();
+        if (staged && threadIdx.x == 0) lds[kStageSentinel] = make_float4(inf, inf, inf, 0.0f);
+        if ((F & F_CONTACT) && staged)
+            for (uint32_t q = threadIdx.x; q < d->nchunks; q += blockDim.x) s_src[q] = d->src[q];
+        if (UNIFORM && staged)
             stage_copy<6, 2 * kStageUnit>(d, n, [&](uint32_t j) { return b.P[j]; },
                                           [&](uint32_t l, const float4 &v) { lds[l] = v; });
+        else if (staged)
+            stage_copy<6, 2 * kStageUnit>(d, n, [&](uint32_t j) { return PM{b.P[j], b.mj[j]}; },
+                                          [&](uint32_t l, const PM &v) {
+                                              lds[l] = v.m < 0.0f ? make_float4(inf, inf, inf, 0.0f) : v.p;
+                                          });
     }
+    // the storage slot of table entry e (an LDS index in a staged unit; its
+    // sentinel padding, never visited, maps to i)
+    auto slot = [&](uint32_t e) -> uint32_t {
+        if (!(STAGED && (F & F_CONTACT)) || !staged) return e;
+        return e >= kStageSlots ? i : s_src[e >> 4] + (e & (kStageChunk - 1u));
+    };
     if (i >= n) return;
     const float4 p = b.P[i];
     const Attr a = b.A[i];
@@ -794,31 +835,57 @@ __global__ __launch_bounds__(kMaxBlock) void solve_k(DevBuffers b, KConst k, uin
     // a7: particle contacts + friction (collideD, integration_kernel.cuh:358-462)
     if (F & F_CONTACT) {
         if (ph >= PS_PHASE_CLOTH && wi != 0.0f) {
-            // the counting walk over the table is pipelined a chunk at a time
-            // and lists the first kContactList contacts (slots, table order)
-            // in this thread's column of the block's LDS; the corrections
-            // then visit only those, or walk the table again past that many
-            auto pa = [&](uint32_t j) { return PA{b.P[j], b.A[j]}; };
-            uint32_t *clist = contact_lds + threadIdx.x;
-            const uint32_t cstride = STAGED ? 0u : blockDim.x;
-            uint32_t nc = 0;
-            for_neighbors_chunked(b, i, c, pa, [&](uint32_t j, const PA &d) {
-                if (ph > PS_PHASE_SOLID && ph == d.a.phase) return;
+            // The corrections are summed in the two halves of the table, each
+            // in table order, then added half 0 + half 1 (the oracle's
+            // order).  The counting walk lists the contacts in the block's
+            // LDS (entry m of thread t at contact_lds[m blockDim + t]) and
+            // the corrections visit only those, or walk the table again when
+            // the list overflowed.  Global-slot variants: one lane walks both
+            // halves (pipelined two chunks at a time), listing slot | half
+            // << 31.  Staged: each of the pair's two lanes counts and
+            // corrects its own half, listing LDS indices (global slots in a
+            // unit that was not staged), and the halves meet by a shuffle.
+            auto pa = [&](uint32_t e) {
+                const uint32_t j = slot(e);
+                return PA{b.P[j], b.A[j]};
+            };
+            auto is_contact = [&](const PA &d) {
+                if (ph > PS_PHASE_SOLID && ph == d.a.phase) return false;
                 const float4 qj = d.p;
                 float dx = px - qj.x, dy = py - qj.y, dz = pz - qj.z;
                 float r2 = ((dx * dx) + (dy * dy)) + (dz * dz);
-                if (r2 < k.collide_r2) {
-                    if (!STAGED && nc < kContactList) clist[nc * cstride] = j;
-                    nc++;
-                }
-            });
+                return r2 < k.collide_r2;
+            };
+            uint32_t *clist = contact_lds + threadIdx.x;
+            const uint32_t cstride = blockDim.x;
+            constexpr uint32_t kList = STAGED ? kHalfContactList : kContactList;
+            uint32_t nc = 0;
+            if constexpr (STAGED) {
+                for_neighbors_half_chunked(b, i, c, hh, pa, [&](uint32_t e, const PA &d) {
+                    if (is_contact(d)) {
+                        if (nc < kList) clist[nc * cstride] = e;
+                        nc++;
+                    }
+                });
+            } else {
+                for_neighbors_chunked_h(b, i, c, pa, [&](uint32_t h, uint32_t j, const PA &d) {
+                    if (is_contact(d)) {
+                        if (nc < kList) clist[nc * cstride] = j | (h << 31);
+                        nc++;
+                    }
+                });
+            }
+            const uint32_t nmine = nc;  // contacts this lane found (and listed up to kList)
+            if (STAGED) nc = nc + __shfl_xor(nc, 1, 64);
             if (TAPE) tape.w = __uint_as_float(nc);
             if (nc > 0) {
                 const float fnc = (float)nc;
                 const float ewi = wi * ps_exp(py);
                 const float4 xi = b.X[i];
-                float cx = 0.0f, cy = 0.0f, cz = 0.0f;
-                auto correct = [&](uint32_t j, const PA &d) {
+                struct C3 {
+                    float x, y, z;
+                };
+                auto correct = [&](C3 &cs, uint32_t e, const PA &d) {
                     const Attr aj = d.a;
                     const int32_t pj = aj.phase;
                     if (ph > PS_PHASE_SOLID && ph == pj) return;
@@ -840,13 +907,13 @@ __global__ __launch_bounds__(kMaxBlock) void solve_k(DevBuffers b, KConst k, uin
                     float mag = dist - k.collide_dist;
                     float sc = (((mag / wsum) / dist) * cw) / fnc;
                     float ex = -(sc * dx), ey = -(sc * dy), ez = -(sc * dz);
-                    cx = cx + ex;
-                    cy = cy + ey;
-                    cz = cz + ez;
+                    cs.x = cs.x + ex;
+                    cs.y = cs.y + ey;
+                    cs.z = cs.z + ez;
                     if (!solid) return;
                     float sc2 = (((mag / wsum) / dist) * cw2) / fnc;
                     float fx = sc2 * dx, fy = sc2 * dy, fz = sc2 * dz;
-                    const float4 xj = b.X[j];
+                    const float4 xj = b.X[slot(e)];
                     float rx = ((px + ex) - xi.x) - ((qj.x + fx) - xj.x);
                     float ry = ((py + ey) - xi.y) - ((qj.y + fy) - xj.y);
                     float rz = ((pz + ez) - xi.z) - ((qj.z + fz) - xj.z);
@@ -857,36 +924,60 @@ __global__ __launch_bounds__(kMaxBlock) void solve_k(DevBuffers b, KConst k, uin
                     if (lt < 0.001f) return;
                     float share = cw / wsum;
                     if (lt < (k.mu_s * dist)) {
-                        cx = cx - (tx * share);
-                        cy = cy - (ty * share);
-                        cz = cz - (tz * share);
+                        cs.x = cs.x - (tx * share);
+                        cs.y = cs.y - (ty * share);
+                        cs.z = cs.z - (tz * share);
                     } else {
                         float f = (k.mu_k * dist) / lt;
                         if (f > 1.0f) f = 1.0f;
-                        cx = cx - ((tx * f) * share);
-                        cy = cy - ((ty * f) * share);
-                        cz = cz - ((tz * f) * share);
+                        cs.x = cs.x - ((tx * f) * share);
+                        cs.y = cs.y - ((ty * f) * share);
+                        cs.z = cs.z - ((tz * f) * share);
                     }
                 };
-                if (!STAGED && nc <= kContactList) {
+                C3 ch0 = {0.0f, 0.0f, 0.0f}, ch1 = {0.0f, 0.0f, 0.0f};  // the halves
+                if (nmine <= kList) {
                     // the listed contacts in order, four records in flight
-                    for (uint32_t q0 = 0; q0 < nc; q0 += 4u) {
+                    for (uint32_t q0 = 0; q0 < nmine; q0 += 4u) {
                         uint32_t jq[4];
                         PA dq[4];
 #pragma unroll
-                        for (uint32_t u = 0; u < 4; u++) jq[u] = clist[min(q0 + u, nc - 1u) * cstride];
+                        for (uint32_t u = 0; u < 4; u++) jq[u] = clist[min(q0 + u, nmine - 1u) * cstride];
 #pragma unroll
-                        for (uint32_t u = 0; u < 4; u++) dq[u] = pa(jq[u]);
+                        for (uint32_t u = 0; u < 4; u++) dq[u] = pa(STAGED ? jq[u] : (jq[u] & 0x7fffffffu));
 #pragma unroll
                         for (uint32_t u = 0; u < 4; u++)
-                            if (q0 + u < nc) correct(jq[u], dq[u]);
+                            if (q0 + u < nmine) {
+                                if (STAGED || !(jq[u] >> 31))
+                                    correct(ch0, jq[u] & (STAGED ? ~0u : 0x7fffffffu), dq[u]);
+                                else
+                                    correct(ch1, jq[u] & 0x7fffffffu, dq[u]);
+                            }
                     }
+                } else if (STAGED) {
+                    for_neighbors_half_chunked(b, i, c, hh, pa,
+                                               [&](uint32_t e, const PA &d) { correct(ch0, e, d); });
                 } else {
-                    for_neighbors_chunked(b, i, c, pa, correct);
+                    for_neighbors_chunked_h(b, i, c, pa, [&](uint32_t h, uint32_t j, const PA &d) {
+                        if (h)
+                            correct(ch1, j, d);
+                        else
+                            correct(ch0, j, d);
+                    });
                 }
-                ax = ax + cx;
-                ay = ay + cy;
-                az = az + cz;
+                if (STAGED) {
+                    // this lane's half is in ch0; the partner lane holds the other
+                    const C3 other = {__shfl_xor(ch0.x, 1, 64), __shfl_xor(ch0.y, 1, 64), __shfl_xor(ch0.z, 1, 64)};
+                    if (hh) {
+                        ch1 = ch0;
+                        ch0 = other;
+                    } else {
+                        ch1 = other;
+                    }
+                }
+                ax = ax + (ch0.x + ch1.x);
+                ay = ay + (ch0.y + ch1.y);
+                az = az + (ch0.z + ch1.z);
             }
         }
     }
@@ -953,19 +1044,29 @@ __global__ __launch_bounds__(kMaxBlock) void solve_k(DevBuffers b, KConst k, uin
                 Sums mine = {0.0f, 0.0f, 0.0f, false};
                 auto from_lds = [&](uint32_t l) { return lds[l]; };
                 auto from_glob = [&](uint32_t j) { return b.P[j]; };
+                // a unit that was not staged, in a scene that is not a uniform fluid
+                auto from_glob_m = [&](uint32_t j) { return PM{b.P[j], b.mj[j]}; };
                 if (staged)
                     for_neighbors_half<false>(b, i, c, hh, from_lds,
                                               [&](uint32_t, const float4 &pj) { fast(mine, pj); });
-                else
+                else if (UNIFORM)
                     for_neighbors_half(b, i, c, hh, from_glob,
                                        [&](uint32_t, const float4 &pj) { fast(mine, pj); });
+                else
+                    for_neighbors_half(b, i, c, hh, from_glob_m, [&](uint32_t, const PM &d) {
+                        if (d.m >= 0.0f) fast(mine, d.p);
+                    });
                 if (mine.deg) {
                     mine = {0.0f, 0.0f, 0.0f, false};
                     auto ex = [&](uint32_t, const float4 &pj) { exact(mine, pj); };
                     if (staged)
                         for_neighbors_half_plain(b, i, c, hh, from_lds, ex);
-                    else
+                    else if (UNIFORM)
                         for_neighbors_half_plain(b, i, c, hh, from_glob, ex);
+                    else
+                        for_neighbors_half_plain(b, i, c, hh, from_glob_m, [&](uint32_t, const PM &d) {
+                            if (d.m >= 0.0f) exact(mine, d.p);
+                        });
                 }
                 const Sums other = {__shfl_xor(mine.fx, 1, 64), __shfl_xor(mine.fy, 1, 64),
                                     __shfl_xor(mine.fz, 1, 64), false};
@@ -1019,7 +1120,7 @@ __global__ __launch_bounds__(kMaxBlock) void solve_k(DevBuffers b, KConst k, uin
         }
     }
 
-    // the staged variant is fluid-only: the pair's lane 0 finishes the particle
+    // staged: the pair's lane 0 finishes the particle
     if (STAGED && hh) return;
 
     // a9: distance constraints, gathered over this particle's incident list
@@ -1208,24 +1309,39 @@ __global__ __launch_bounds__(kMaxBlock) void finalize_k(DevBuffers b, KConst k, 
     // staged: neighbour position and displacement u = p - x* (x, y, z, u.x)
     // and (u.y, u.z), formed with the same subtractions as the global path
     // two lanes per particle when staged, as in lambda_k and solve_k
-    // (and the sentinel record)
+    // (and the sentinel record).  In a scene that is not a uniform fluid,
+    // with every fluid particle of one mass (mu >= 0), a particle that is
+    // not fluid is staged at infinity: the XSPH sum leaves it out, as the
+    // global path does by its x*.w < 0.  With fluid masses that differ
+    // (mu < 0) nothing is staged: the neighbours are gathered from global
+    // memory through the unit's chunk table (s_src)
     __shared__ float4 lds_pu[STAGED ? kStageSlots + 1 : 1];
     __shared__ float2 lds_u[STAGED ? kStageSlots + 1 : 1];
+    __shared__ uint32_t s_src[STAGED && !UNIFORM ? kStageChunks : 1];
     const uint32_t unit = xcd_block();
     const uint32_t hh = STAGED ? (threadIdx.x & 1u) : 0u;
     const uint32_t i = STAGED ? unit * kStageUnit + (threadIdx.x >> 1) : unit * blockDim.x + threadIdx.x;
-    bool staged = false;
+    bool staged = false, mapped = false;
     if (STAGED) {
         const StageDesc *d = b.stage + unit;
         staged = d->staged != 0u;
+        mapped = !UNIFORM && staged && mu < 0.0f;
+        if (mapped) {
+            for (uint32_t q = threadIdx.x; q < d->nchunks; q += blockDim.x) s_src[q] = d->src[q];
+            __syncthreads();
+        }
+        staged = staged && !mapped;
+        const float inf = __builtin_inff();
         if (staged && threadIdx.x == 0) {
-            lds_pu[kStageSentinel] = make_float4(__builtin_inff(), __builtin_inff(), __builtin_inff(), 0.0f);
+            lds_pu[kStageSentinel] = make_float4(inf, inf, inf, 0.0f);
             lds_u[kStageSentinel] = make_float2(0.0f, 0.0f);
         }
         if (staged)
             stage_copy<4, 2 * kStageUnit>(d, n, [&](uint32_t j) { return PX{b.P[j], b.X[j]}; },
                                           [&](uint32_t l, const PX &v) {
-                                              lds_pu[l] = make_float4(v.p.x, v.p.y, v.p.z, v.p.x - v.x.x);
+                                              lds_pu[l] = !UNIFORM && v.x.w < 0.0f
+                                                              ? make_float4(inf, inf, inf, 0.0f)
+                                                              : make_float4(v.p.x, v.p.y, v.p.z, v.p.x - v.x.x);
                                               lds_u[l] = make_float2(v.p.y - v.x.y, v.p.z - v.x.z);
                                           });
     }
@@ -1261,6 +1377,12 @@ __global__ __launch_bounds__(kMaxBlock) void finalize_k(DevBuffers b, KConst k, 
         auto glob = [&](Sums &s, const PX &d) {
             body(s, d.p, d.p.x - d.x.x, d.p.y - d.x.y, d.p.z - d.x.z, mu);
         };
+        // the neighbour's mass from x*.w (the sort's 1/w for a fluid
+        // particle, -1 for any other: left out)
+        auto gen = [&](Sums &s, const PX &d) {
+            if (d.x.w < 0.0f) return;
+            body(s, d.p, d.p.x - d.x.x, d.p.y - d.x.y, d.p.z - d.x.z, d.x.w);
+        };
         Sums s0 = {0.0f, 0.0f, 0.0f}, s1 = {0.0f, 0.0f, 0.0f};
         if constexpr (STAGED) {
             // this lane's half; the partner lane (lane ^ 1) holds the other
@@ -1269,10 +1391,25 @@ __global__ __launch_bounds__(kMaxBlock) void finalize_k(DevBuffers b, KConst k, 
                 for_neighbors_half<false>(
                     b, i, c, hh, [&](uint32_t l) { return PU{lds_pu[l], lds_u[l]}; },
                     [&](uint32_t, const PU &d) { body(mine, d.pu, d.pu.w, d.u.x, d.u.y, mu); });
+            else if (mapped)
+                // table entries are LDS indices (the sentinel padding, never
+                // visited, maps to i)
+                for_neighbors_half(
+                    b, i, c, hh,
+                    [&](uint32_t e) {
+                        const uint32_t j = e >= kStageSlots ? i : s_src[e >> 4] + (e & (kStageChunk - 1u));
+                        return PX{b.P[j], b.X[j]};
+                    },
+                    [&](uint32_t, const PX &d) { gen(mine, d); });
             else
                 for_neighbors_half(
                     b, i, c, hh, [&](uint32_t j) { return PX{b.P[j], b.X[j]}; },
-                    [&](uint32_t, const PX &d) { glob(mine, d); });
+                    [&](uint32_t, const PX &d) {
+                        if (UNIFORM)
+                            glob(mine, d);
+                        else
+                            gen(mine, d);
+                    });
             const Sums other = {__shfl_xor(mine.sx, 1, 64), __shfl_xor(mine.sy, 1, 64),
                                 __shfl_xor(mine.sz, 1, 64)};
             s0 = hh ? other : mine;
@@ -1282,12 +1419,6 @@ __global__ __launch_bounds__(kMaxBlock) void finalize_k(DevBuffers b, KConst k, 
                 b, i, c, [&](uint32_t j) { return PX{b.P[j], b.X[j]}; },
                 [&](uint32_t, const PX &d) { glob(s0, d); }, [&](uint32_t, const PX &d) { glob(s1, d); });
         } else {
-            // the neighbour's mass from x*.w (the sort's 1/w for a fluid
-            // particle, -1 for any other: left out)
-            auto gen = [&](Sums &s, const PX &d) {
-                if (d.x.w < 0.0f) return;
-                body(s, d.p, d.p.x - d.x.x, d.p.y - d.x.y, d.p.z - d.x.z, d.x.w);
-            };
             for_neighbors_split(
                 b, i, c, [&](uint32_t j) { return PX{b.P[j], b.X[j]}; },
                 [&](uint32_t, const PX &d) { gen(s0, d); }, [&](uint32_t, const PX &d) { gen(s1, d); });
@@ -1393,6 +1524,9 @@ hipError_t launch_lambda(hipStream_t st, const DevBuffers &b, const KConst &k, u
     else if (uniform && b.staging)
         PS_LAUNCH((lambda_k<true, true>), dim3(blocks_for(n, kStageUnit)), dim3(2 * kStageUnit),
                            0, st, b, k, n, mu);
+    else if (b.staging)
+        PS_LAUNCH((lambda_k<false, true>), dim3(blocks_for(n, kStageUnit)), dim3(2 * kStageUnit),
+                           0, st, b, k, n, mu);
     else if (uniform)
         PS_LAUNCH((lambda_k<true, false>), g, t, 0, st, b, k, n, mu);
     else
@@ -1417,10 +1551,11 @@ static void solve_variant(hipStream_t st, const DevBuffers &b, const KConst &k, 
     const dim3 g(blocks_for(n, bs)), t(bs);
     const dim3 gs(blocks_for(n, kStageUnit)), ts(2 * kStageUnit);  // two lanes per particle
     const bool n4 = k.n_corr == 4u;
-    // the contact lists of a block (the global-slot variants)
+    // the contact lists of a block
     const size_t cl = (F & F_CONTACT) ? sizeof(uint32_t) * kContactList * bs : 0;
-    // the staged variant exists for the uniform pure-fluid composition only
-    constexpr bool CAN_STAGE = F == F_FLUID;
+    const size_t cls = (F & F_CONTACT) ? sizeof(uint32_t) * kHalfContactList * 2 * kStageUnit : 0;  // per lane
+    // staged variants for every composition (uniform fluid or not)
+    constexpr bool CAN_STAGE = true;
     if (b.tape_f) {  // recording for the reverse pass (never staged)
         if (uniform && n4)
             PS_LAUNCH((solve_k<F, true, 4, false, true>), g, t, cl, st, b, k, n, wu, iter);
@@ -1436,6 +1571,10 @@ static void solve_variant(hipStream_t st, const DevBuffers &b, const KConst &k, 
         PS_LAUNCH((solve_k<F, true, 4, CAN_STAGE>), gs, ts, 0, st, b, k, n, wu, iter);
     else if (CAN_STAGE && uniform && b.staging)
         PS_LAUNCH((solve_k<F, true, 0, CAN_STAGE>), gs, ts, 0, st, b, k, n, wu, iter);
+    else if (CAN_STAGE && b.staging && n4)
+        PS_LAUNCH((solve_k<F, false, 4, CAN_STAGE>), gs, ts, cls, st, b, k, n, wu, iter);
+    else if (CAN_STAGE && b.staging)
+        PS_LAUNCH((solve_k<F, false, 0, CAN_STAGE>), gs, ts, cls, st, b, k, n, wu, iter);
     else if (uniform && n4)
         PS_LAUNCH((solve_k<F, true, 4, false>), g, t, cl, st, b, k, n, wu, iter);
     else if (uniform)
@@ -1462,7 +1601,7 @@ hipError_t launch_solve(hipStream_t st, const DevBuffers &b, const KConst &k, ui
 }
 
 hipError_t launch_finalize(hipStream_t st, const DevBuffers &b, const KConst &k, uint32_t n,
-                           float dt, bool sleeping, bool uniform, float wu, uint32_t its,
+                           float dt, bool sleeping, bool uniform, float wu, float wf, uint32_t its,
                            uint32_t bs)
 {
     if (n == 0) return hipSuccess;
@@ -1471,6 +1610,9 @@ hipError_t launch_finalize(hipStream_t st, const DevBuffers &b, const KConst &k,
     if (uniform && b.staging)
         PS_LAUNCH((finalize_k<true, true>), dim3(blocks_for(n, kStageUnit)),
                            dim3(2 * kStageUnit), 0, st, b, k, n, mu, dt, its);
+    else if (b.staging)  // wf >= 0: every fluid particle of inverse mass wf
+        PS_LAUNCH((finalize_k<false, true>), dim3(blocks_for(n, kStageUnit)),
+                           dim3(2 * kStageUnit), 0, st, b, k, n, wf >= 0.0f ? 1.0f / wf : -1.0f, dt, its);
     else if (uniform)
         PS_LAUNCH((finalize_k<true, false>), g, t, 0, st, b, k, n, mu, dt, its);
     else

@@ -238,13 +238,13 @@ MUTATIONS = [
     ("mu_s/mu_k swapped in particle friction",
      "if (lt < (d->mu_s * dist)) {", "if (lt < (d->mu_k * dist)) {", ["static_pair"]),
     ("kinetic particle friction without the mass share",
-     "cx = cx - ((tx * f) * share);", "cx = cx - (tx * f);", ["kinetic_pair"]),
+     "c3[0] = c3[0] - ((tx * f) * share);", "c3[0] = c3[0] - (tx * f);", ["kinetic_pair"]),
     ("e^{-y} mass scaling",
      "real ewi = wi * EXP(py);", "real ewi = wi * EXP(-py);", ["height_scaled_pair"]),
     ("particle i's height for both (the GPU reference's form)",
      "cw2 = wj * EXP(qy);", "cw2 = wj * EXP(py);", ["height_scaled_pair"]),
     ("no clamp of the kinetic factor",
-     "if (f > R(1)) f = R(1);\n                        cx", "cx", ["clamped_pair"]),
+     "if (f > R(1)) f = R(1);\n                        c3[0]", "c3[0]", ["clamped_pair"]),
     ("height scaling also for cloth",
      "int solid = ph >= PS_PHASE_SOLID && pj >= PS_PHASE_SOLID;",
      "int solid = ph >= PS_PHASE_CLOTH && pj >= PS_PHASE_CLOTH;", ["cloth_pair_is_not_height_scaled"]),
