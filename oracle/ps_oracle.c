@@ -92,6 +92,9 @@ static real DIST2(real dx, real dy, real dz) { return FMA(dz, dz, FMA(dy, dy, dx
 #define DEGENERATE_R2 R(1e-8)
 /* the half of a neighbour-table entry k: its chunk of four entries, even or odd */
 #define HALF(k) ((int)(((k) >> 2) & 1u))
+/* the quarter of table entry k: its chunk of four entries modulo 4 (the
+ * reverse pass's position sums) */
+#define QUARTER(k) ((int)(((k) >> 2) & 3u))
 
 #include "orc_exp.h"
 #define EXP orc_exp

@@ -378,15 +378,16 @@ __global__ __launch_bounds__(kAdjBlock) void adj_lambda_k(DevBuffers b, KConst k
     a.r0b[i] = r;
 }
 
-// dL/dp before the iteration.  Two lanes per particle: lane hh sums the
-// pair and contact terms of half hh of the table (half 0 starts from dL/dy),
-// then acc = half 0 + half 1, and lane 0 adds the distance constraints.
+// dL/dp before the iteration.  Four lanes per particle: lane hh sums the
+// pair and contact terms of quarter hh of the table (chunks hh, hh + 4, ...;
+// quarter 0 starts from dL/dy), then acc = (q0 + q2) + (q1 + q3), and lane 0
+// adds the distance constraints.
 template <bool UNIFORM>
 __global__ __launch_bounds__(kAdjBlock) void adj_position_k(DevBuffers b, KConst k, AdjBuffers a, uint32_t n)
 {
-    const uint32_t p = blockIdx.x * (kAdjBlock / 2) + (threadIdx.x >> 1);
-    const uint32_t hh = threadIdx.x & 1u;
-    if (p >= n) return;  // both lanes of a particle leave together
+    const uint32_t p = blockIdx.x * (kAdjBlock / 4) + (threadIdx.x >> 2);
+    const uint32_t hh = threadIdx.x & 3u;
+    if (p >= n) return;  // the four lanes of a particle leave together
     const float4 pp = a.Pit[p];
     const Attr ap = b.A[p];
     const float4 Yp = a.Y[p];
@@ -449,21 +450,23 @@ __global__ __launch_bounds__(kAdjBlock) void adj_position_k(DevBuffers b, KConst
     };
     const uint32_t c = b.ncnt[p];
     if (UNIFORM) {
-        for_neighbors_half_chunked(
-            b, p, c, hh, [&](uint32_t j) { return GPos{a.Pit[j], a.M[j], a.SD[j], a.CD[j]}; },
+        for_neighbors_strided_chunked(
+            b, p, c, hh, 4u, [&](uint32_t j) { return GPos{a.Pit[j], a.M[j], a.SD[j], a.CD[j]}; },
             [&](uint32_t, const GPos &d) { body(d.p, d.m, d.sd, d.cd, ap.w, PS_PHASE_FLUID, 0u, massp); });
     } else {
-        for_neighbors_half_chunked(
-            b, p, c, hh, [&](uint32_t j) { return GPosAttr{a.Pit[j], a.M[j], a.SD[j], a.CD[j], a.AB[j]}; },
+        for_neighbors_strided_chunked(
+            b, p, c, hh, 4u, [&](uint32_t j) { return GPosAttr{a.Pit[j], a.M[j], a.SD[j], a.CD[j], a.AB[j]}; },
             [&](uint32_t, const GPosAttr &d) {
                 body(d.p, d.m, d.sd, d.cd, d.a.w, d.a.phase, d.a.nc, d.a.mass);
             });
     }
     {
+        // lanes 0 and 1 form q0 + q2 and q1 + q3, lane 0 their sum
+        for (int q2 = 0; q2 < 3; q2++) acc[q2] = acc[q2] + __shfl_xor(acc[q2], 2, 64);
         float lo[3];
         for (int q2 = 0; q2 < 3; q2++) lo[q2] = __shfl_xor(acc[q2], 1, 64);
         if (hh) return;
-        for (int q2 = 0; q2 < 3; q2++) acc[q2] = acc[q2] + lo[q2];  // half 0 + half 1
+        for (int q2 = 0; q2 < 3; q2++) acc[q2] = acc[q2] + lo[q2];
     }
     // distance constraints incident to p
     if (a.dist) {
@@ -701,7 +704,15 @@ hipError_t launch_adj(hipStream_t st, AdjKernel which, const DevBuffers &b, cons
     case ADJ_FINALIZE: PS_ADJ_LAUNCH2(adj_finalize_k, b, k, a, count, dt); break;
     case ADJ_PRE: PS_ADJ_LAUNCH(adj_pre_k, b, k, a, count); break;
     case ADJ_LAMBDA: PS_ADJ_LAUNCH2(adj_lambda_k, b, k, a, count); break;
-    case ADJ_POSITION: PS_ADJ_LAUNCH2(adj_position_k, b, k, a, count); break;
+    case ADJ_POSITION: {
+        // four lanes per particle: kAdjBlock / 4 particles per block
+        const dim3 grid4((count + kAdjBlock / 4 - 1) / (kAdjBlock / 4));
+        if (uniform)
+            PS_LAUNCH(adj_position_k<true>, grid4, block, 0, st, b, k, a, count);
+        else
+            PS_LAUNCH(adj_position_k<false>, grid4, block, 0, st, b, k, a, count);
+        break;
+    }
     case ADJ_STIFFNESS: PS_LAUNCH(adj_stiffness_k, grid, block, 0, st, b, k, a, count); break;
     case ADJ_PREDICT: PS_LAUNCH(adj_predict_k, grid, block, 0, st, b, k, a, count, dt); break;
     default: return hipErrorInvalidValue;

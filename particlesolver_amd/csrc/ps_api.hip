@@ -524,16 +524,14 @@ int stage_sort(ps_system *s)
     return PS_OK;
 }
 
-// Neighbour cells can be staged in LDS for a single scene, unless the
-// caller turned it off by flag; the autotuner may turn it off too.  (Staged
-// XSPH takes the fluid mass as a constant: with fluid masses that differ,
-// finalize gathers from global memory through the units' chunk tables.)
-bool can_stage(const ps_system *s)
-{
-    return !s->cpuref && !(s->prm.flags & PS_FLAG_NO_LDS_STAGING) && s->kc.nscenes == 1;
-}
+// Neighbour cells can be staged in LDS (in a batch, the units that lie in
+// one scene), unless the caller turned it off by flag; the autotuner may
+// turn it off too.  (Staged XSPH takes the fluid mass as a constant: with
+// fluid masses that differ, finalize gathers from global memory through
+// the units' chunk tables.)
+bool can_stage(const ps_system *s) { return !s->cpuref && !(s->prm.flags & PS_FLAG_NO_LDS_STAGING); }
 
-bool staging_ok(const ps_system *s) { return can_stage(s) && !s->adj_active && s->stage_choice != 0; }
+bool staging_ok(const ps_system *s) { return can_stage(s) && s->stage_choice != 0; }
 
 int stage_neighbors(ps_system *s)
 {
@@ -1879,7 +1877,12 @@ int adj_taped_substep(ps_system *s, float dt, size_t t)
     ps_system::AdjWork &w = s->adj;
     DevBuffers &b = s->b;
     uint32_t *own_nbr = b.nbr, *own_cnt = b.ncnt;
-    b.nbr = w.tN + t * w.tab;
+    // a staged substep keeps its LDS-index table and writes the global-slot
+    // copy the reverse kernels read into the slot
+    if (staging_ok(s))
+        b.nbr_glob = w.tN + t * w.tab;
+    else
+        b.nbr = w.tN + t * w.tab;
     b.ncnt = w.tC + t * w.cap;
     // the reorder writes the sorted start state, attributes and slot map
     // into the tape slot as well
@@ -1922,6 +1925,7 @@ int adj_taped_substep(ps_system *s, float dt, size_t t)
     if (!rc && e == hipSuccess) rc = stage_end(s);
     b.nbr = own_nbr;
     b.ncnt = own_cnt;
+    b.nbr_glob = nullptr;
     s->nbr_valid = false;
     if (!rc && e != hipSuccess) rc = fail(s, PS_ERR_DEVICE, std::string("taped substep: ") + hipGetErrorString(e));
     return rc;
@@ -2479,9 +2483,8 @@ int ps_autotune(ps_system *s, float dt, uint32_t substeps, uint32_t *blocks_out)
         else
             need = true;
     }
-    // the second knob: LDS staging of each unit's neighbour cells (single
-    // scenes whose fluid particles share one mass, unless the caller turned
-    // it off by flag)
+    // the second knob: LDS staging of each unit's neighbour cells (unless
+    // the caller turned it off by flag)
     const bool stageable = can_stage(s);
     if (stageable) {
         auto it = g_tune.find({kTuneStaging, bucket, feat});

@@ -177,6 +177,33 @@ __device__ __forceinline__ void for_neighbors_half_chunked(const DevBuffers &b, 
     }
 }
 
+// The chunks first, first + stride, first + 2 stride, ... of particle i's
+// first c neighbours, one chunk at a time (four records in flight), each in
+// table order: quarter q of the table is first = q, stride = 4.
+template <typename G, typename Fn>
+__device__ __forceinline__ void for_neighbors_strided_chunked(const DevBuffers &b, uint32_t i, uint32_t c,
+                                                              uint32_t first, uint32_t stride, G gather,
+                                                              Fn fn)
+{
+    const uint32_t nch = (c + 3u) >> 2;  // chunks in use
+    if (first >= nch) return;
+    const uint4 *nb = nbr_lane(b, i);
+    uint4 jj = nb[(size_t)first * 64];
+    for (uint32_t m = first; m < nch; m += stride) {
+        const uint4 jn = nb[(size_t)(m + stride < nch ? m + stride : m) * 64];
+        auto d0 = gather(jj.x);
+        auto d1 = gather(jj.y);
+        auto d2 = gather(jj.z);
+        auto d3 = gather(jj.w);
+        const uint32_t q = m << 2;
+        fn(jj.x, d0);
+        if (q + 1 < c) fn(jj.y, d1);
+        if (q + 2 < c) fn(jj.z, d2);
+        if (q + 3 < c) fn(jj.w, d3);
+        jj = jn;
+    }
+}
+
 // The entries of half h of particle i's first c neighbours in table order,
 // one at a time: for the rarely taken recomputation of one half.
 template <typename G, typename Fn>

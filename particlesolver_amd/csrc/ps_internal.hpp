@@ -146,6 +146,8 @@ struct DevBuffers {
     // neighbours: sliced ELL in chunks of four (see ps_kernels.hip)
     uint32_t *nbr;
     uint32_t *ncnt;
+    uint32_t *nbr_glob;    // recording a substep for the reverse pass with staging:
+                           // the same table with global slots (else null)
     uint32_t kq;  // max_neighbors / 4
     // distance constraints (insertion-index CSR)
     uint32_t *dist_off;    // max_particles + 1

@@ -328,23 +328,38 @@ __global__ __launch_bounds__(kMaxBlock) void reorder_k(DevBuffers b, uint32_t n,
 // p is taken by value: a reference into b.P would be re-read after every
 // table store (the compiler cannot rule out aliasing).  emit(t, j) is
 // j + emit(t, 0) (an offset per stencil row), taken once per row.
-template <typename Range, typename Emit>
+// GLOB: also write the table with global slots (entries j, padding i) to
+// b.nbr_glob in the same layout (the copy a recording for the reverse pass
+// keeps while the staged kernels read LDS indices).
+template <bool GLOB = false, typename Range, typename Emit>
 __device__ __forceinline__ uint32_t scan_stencil(const DevBuffers &b, const KConst &k, uint32_t i,
                                                  uint32_t pad, const float4 p, uint32_t kmax,
                                                  Range range, Emit emit)
 {
     uint4 *out = const_cast<uint4 *>(nbr_lane(b, i));
+    uint4 *out2 = GLOB ? reinterpret_cast<uint4 *>(b.nbr_glob) + (size_t)(i >> 6) * b.kq * 64 + (i & 63)
+                       : nullptr;
     uint4 buf = make_uint4(pad, pad, pad, pad);
+    uint4 buf2 = make_uint4(i, i, i, i);
     uint32_t cnt = 0;
     const float r2max = k.search_r2;
     // an accepted candidate: shifted into buf, which after the 4th entry of
     // a chunk holds it in order (x first); chunks past kmax are not stored
-    auto accept = [&](uint32_t e) {
+    auto accept = [&](uint32_t e, uint32_t j) {
         buf.x = buf.y;
         buf.y = buf.z;
         buf.z = buf.w;
         buf.w = e;
-        if ((cnt & 3u) == 3u && cnt < kmax) out[(size_t)(cnt >> 2) * 64] = buf;
+        if (GLOB) {
+            buf2.x = buf2.y;
+            buf2.y = buf2.z;
+            buf2.z = buf2.w;
+            buf2.w = j;
+        }
+        if ((cnt & 3u) == 3u && cnt < kmax) {
+            out[(size_t)(cnt >> 2) * 64] = buf;
+            if (GLOB) out2[(size_t)(cnt >> 2) * 64] = buf2;
+        }
         cnt++;
     };
     for (uint32_t t = 0; t < 9; t++) {
@@ -386,7 +401,7 @@ __device__ __forceinline__ uint32_t scan_stencil(const DevBuffers &b, const KCon
             while (m != 0u) {
                 const uint32_t u = (uint32_t)__builtin_ctz(m);
                 m &= m - 1u;
-                accept(w0 + u + eoff);
+                accept(w0 + u + eoff, w0 + u);
             }
         }
     }
@@ -396,6 +411,11 @@ __device__ __forceinline__ uint32_t scan_stencil(const DevBuffers &b, const KCon
     if (r == 1u) out[(size_t)(kept >> 2) * 64] = make_uint4(buf.w, pad, pad, pad);
     else if (r == 2u) out[(size_t)(kept >> 2) * 64] = make_uint4(buf.z, buf.w, pad, pad);
     else if (r == 3u) out[(size_t)(kept >> 2) * 64] = make_uint4(buf.y, buf.z, buf.w, pad);
+    if (GLOB) {
+        if (r == 1u) out2[(size_t)(kept >> 2) * 64] = make_uint4(buf2.w, i, i, i);
+        else if (r == 2u) out2[(size_t)(kept >> 2) * 64] = make_uint4(buf2.z, buf2.w, i, i);
+        else if (r == 3u) out2[(size_t)(kept >> 2) * 64] = make_uint4(buf2.y, buf2.z, buf2.w, i);
+    }
     b.ncnt[i] = kept;
     return cnt;
 }
@@ -485,13 +505,14 @@ __device__ __forceinline__ void stage_copy(const StageDesc *d, uint32_t n, Load 
     __syncthreads();
 }
 
-// Staged build (one single-scene grid): plan the unit's intervals and its
+// Staged build (the unit's scene grid): plan the unit's intervals and its
 // chunk table for the kernels that copy them into LDS, then scan the
 // stencil in global memory (full occupancy; this kernel reads each
 // candidate once per particle, so LDS would not save it traffic) and store
 // each neighbour as its LDS index j + base - start of its interval.  A unit
 // spanning more than kStageRows rows or kStageSlots slots keeps global slots
 // (staged = 0).
+template <bool GLOB>
 __global__ __launch_bounds__(kStageUnit) void neighbors_staged_k(DevBuffers b, KConst k, uint32_t n,
                                                               uint32_t kmax)
 {
@@ -505,8 +526,21 @@ __global__ __launch_bounds__(kStageUnit) void neighbors_staged_k(DevBuffers b, K
     const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
     const uint32_t i = unit * kStageUnit + tid;
     const bool valid = i < n;  // the valid threads are a prefix of the unit
-    const uint32_t gx = k.grid[0], gy = k.grid[1], gz = k.grid[2];
-    const uint32_t key = valid ? b.key[i] : 0u;
+    // the particle's grid: in a batch its scene's (cells cbase ..), and a
+    // unit whose particles lie in two scenes keeps global slots
+    uint32_t gx = k.grid[0], gy = k.grid[1], gz = k.grid[2], cbase = 0;
+    bool one_scene = true;
+    if (k.nscenes > 1u) {
+        const uint32_t first = unit * kStageUnit, lastv = min(first + kStageUnit, n) - 1u;
+        const uint32_t su = b.sid[first];
+        one_scene = b.sid[lastv] == su;
+        const SceneDev &sc = b.scenes[valid ? b.sid[i] : su];
+        gx = sc.grid[0];
+        gy = sc.grid[1];
+        gz = sc.grid[2];
+        cbase = sc.cell_base;
+    }
+    const uint32_t key = valid ? b.key[i] - cbase : 0u;
     const uint32_t row = key / gx, x = key % gx;  // row = z gy + y
     const uint32_t ry = row % gy, rz = row / gy;
     const uint32_t x0 = x > 0 ? x - 1 : 0, x1 = x + 1 < gx ? x + 1 : gx - 1;
@@ -529,6 +563,7 @@ __global__ __launch_bounds__(kStageUnit) void neighbors_staged_k(DevBuffers b, K
         before += q < w ? s_wheads[q] : 0u;
         nrows += s_wheads[q];
     }
+    if (!one_scene) nrows = kStageRows + 1u;  // not staged
     const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
     const uint32_t q = before + (uint32_t)__popcll(hm & upto) - 1u;  // this particle's row
     if (valid && nrows <= kStageRows) {
@@ -546,7 +581,7 @@ __global__ __launch_bounds__(kStageUnit) void neighbors_staged_k(DevBuffers b, K
         const int z = (int)rrz + (int)(t / 3) - 1, y = (int)rry + (int)(t % 3) - 1;
         uint32_t st = 0, len = 0;
         if (z >= 0 && z < (int)gz && y >= 0 && y < (int)gy) {
-            const uint32_t r2 = ((uint32_t)z * gy + (uint32_t)y) * gx;
+            const uint32_t r2 = cbase + ((uint32_t)z * gy + (uint32_t)y) * gx;
             const uint32_t xa = s_xa[qq], xb = s_xb[qq];
             st = b.cell_off[r2 + (xa > 0 ? xa - 1 : 0)];
             const uint32_t en = min(b.cell_off[r2 + (xb + 1 < gx ? xb + 1 : gx - 1) + 1], n);
@@ -593,7 +628,7 @@ __global__ __launch_bounds__(kStageUnit) void neighbors_staged_k(DevBuffers b, K
         auto range = [&](uint32_t t, uint32_t &j0, uint32_t &j1) {
             const int z = (int)rz + (int)(t / 3) - 1, y = (int)ry + (int)(t % 3) - 1;
             if (z < 0 || z >= (int)gz || y < 0 || y >= (int)gy) return false;
-            const uint32_t r2 = ((uint32_t)z * gy + (uint32_t)y) * gx;
+            const uint32_t r2 = cbase + ((uint32_t)z * gy + (uint32_t)y) * gx;
             j0 = b.cell_off[r2 + x0];
             j1 = min(b.cell_off[r2 + x1 + 1], n);
             if (j0 >= j1) return false;
@@ -613,7 +648,7 @@ __global__ __launch_bounds__(kStageUnit) void neighbors_staged_k(DevBuffers b, K
             return staged ? j + (s_base[iv] - s_start[iv]) : j;
         };
         // a staged unit pads with its sentinel record, a global one with i
-        cnt = scan_stencil(b, k, i, staged ? kStageSentinel : i, b.P[i], kmax, range, emit);
+        cnt = scan_stencil<GLOB>(b, k, i, staged ? kStageSentinel : i, b.P[i], kmax, range, emit);
     }
     neighbor_status(b, cnt, kmax, ldsw);
 }
@@ -1503,8 +1538,11 @@ hipError_t launch_neighbors(hipStream_t st, const DevBuffers &b, const KConst &k
                             uint32_t kmax, uint32_t bs)
 {
     if (n == 0) return hipSuccess;
-    if (b.staging)
-        PS_LAUNCH(neighbors_staged_k, dim3(blocks_for(n, kStageUnit)), dim3(kStageUnit), 0,
+    if (b.staging && b.nbr_glob)  // recording: the global-slot copy as well
+        PS_LAUNCH(neighbors_staged_k<true>, dim3(blocks_for(n, kStageUnit)), dim3(kStageUnit), 0,
+                           st, b, k, n, kmax);
+    else if (b.staging)
+        PS_LAUNCH(neighbors_staged_k<false>, dim3(blocks_for(n, kStageUnit)), dim3(kStageUnit), 0,
                            st, b, k, n, kmax);
     else
         PS_LAUNCH(neighbors_k, dim3(blocks_for(n, bs)), dim3(bs), 0, st, b, k, n, kmax);
@@ -1517,7 +1555,12 @@ hipError_t launch_lambda(hipStream_t st, const DevBuffers &b, const KConst &k, u
     if (n == 0) return hipSuccess;
     const dim3 g(blocks_for(n, bs)), t(bs);
     const float mu = 1.0f / wu;
-    if (b.tape_lq && uniform)  // recording for the reverse pass (never staged)
+    const dim3 gs(blocks_for(n, kStageUnit)), ts(2 * kStageUnit);  // staged: two lanes per particle
+    if (b.tape_lq && b.staging && uniform)  // recording for the reverse pass
+        PS_LAUNCH((lambda_k<true, true, true>), gs, ts, 0, st, b, k, n, mu);
+    else if (b.tape_lq && b.staging)
+        PS_LAUNCH((lambda_k<false, true, true>), gs, ts, 0, st, b, k, n, mu);
+    else if (b.tape_lq && uniform)
         PS_LAUNCH((lambda_k<true, false, true>), g, t, 0, st, b, k, n, mu);
     else if (b.tape_lq)
         PS_LAUNCH((lambda_k<false, false, true>), g, t, 0, st, b, k, n, mu);
@@ -1556,7 +1599,18 @@ static void solve_variant(hipStream_t st, const DevBuffers &b, const KConst &k, 
     const size_t cls = (F & F_CONTACT) ? sizeof(uint32_t) * kHalfContactList * 2 * kStageUnit : 0;  // per lane
     // staged variants for every composition (uniform fluid or not)
     constexpr bool CAN_STAGE = true;
-    if (b.tape_f) {  // recording for the reverse pass (never staged)
+    if (b.tape_f && b.staging) {  // recording for the reverse pass
+        if (uniform && n4)
+            PS_LAUNCH((solve_k<F, true, 4, true, true>), gs, ts, cls, st, b, k, n, wu, iter);
+        else if (uniform)
+            PS_LAUNCH((solve_k<F, true, 0, true, true>), gs, ts, cls, st, b, k, n, wu, iter);
+        else if (n4)
+            PS_LAUNCH((solve_k<F, false, 4, true, true>), gs, ts, cls, st, b, k, n, wu, iter);
+        else
+            PS_LAUNCH((solve_k<F, false, 0, true, true>), gs, ts, cls, st, b, k, n, wu, iter);
+        return;
+    }
+    if (b.tape_f) {  // recording for the reverse pass
         if (uniform && n4)
             PS_LAUNCH((solve_k<F, true, 4, false, true>), g, t, cl, st, b, k, n, wu, iter);
         else if (uniform)
