@@ -1,4 +1,5 @@
-"""LDS-staged neighbour cells against the FP32 oracle.
+"""LDS-staged neighbour cells against the FP32 oracle (uniform fluids; other
+compositions and batches: test_gpu_staging_mixed.py).
 
 Uniform single-scene fluids run their neighbour build, lambda, solve and
 XSPH kernels from neighbour positions copied into LDS per 256-particle unit
