@@ -262,12 +262,13 @@ def trace_report(stats_csv, trace_csv, log):
         what = ("%d substeps (%d warm-up + %d timed) of the dam break from rest, then the exports of "
                 "the positions. Durations vary over the run: the collapse phase has the most "
                 "neighbours per particle." % (line["warmup"] + line["steps"], line["warmup"], line["steps"]))
+    d = "ktm" if mixed else "kt"
     out = ["# Kernel durations: %s (rocprofv3 kernel trace)" % title, "",
            "Commands (repository root on the GPU host as R, MI355X):", "", "```",
            "cd /tmp && export TMPDIR=/tmp && rocprofv3 --kernel-trace --stats --output-format csv "
-           "-d /tmp/kt -o kt -- python3 $R/bench.py %s > $R/gpurun_out/kt.log 2>&1" % run,
-           "cd $R && python3 bench.py --trace-report /tmp/kt/kt_kernel_stats.csv "
-           "/tmp/kt/kt_kernel_trace.csv gpurun_out/kt.log", "```", "", what,
+           "-d /tmp/%s -o kt -- python3 $R/bench.py %s > $R/gpurun_out/%s.log 2>&1" % (d, run, d),
+           "cd $R && python3 bench.py --trace-report /tmp/%s/kt_kernel_stats.csv "
+           "/tmp/%s/kt_kernel_trace.csv gpurun_out/%s.log" % (d, d, d), "```", "", what,
            "", "| kernel | calls | avg us | min us | max us | share |", "|---|---|---|---|---|---|"]
     for r in sorted(rows, key=lambda r: -float(r["TotalDurationNs"])):
         out.append("| `%s` | %s | %.1f | %.1f | %.1f | %.1f %% |" % (
@@ -349,7 +350,9 @@ def _trace_report_ensemble(rows, tot, line, trace_csv):
                                           sum(gaps) / 1e6, sum(gaps) / max(len(gaps), 1) / 1e3,
                                           max(gaps or [0]) / 1e3, sum(gaps) / 1e6 / subs)]
     out += ["", "The bench line of the same command: batched %.4g particle-iterations/s (%.3f ms per "
-            "substep); per-scene handles issued back to back %.4g (%.2fx), one after another %.4g (%.2fx)."
+            "substep); per-scene handles issued back to back %.4g (%.2fx), one after another %.4g (%.2fx). "
+            "The tracer adds its cost to each of the per-scene runs' many small launches, so their rates "
+            "here are lower than without it (the `batched` object of the default bench line)."
             % (line["value"], line["ms_per_step"], c["plain_value"], c["speedup_vs_per_scene"],
                c["sequential_value"], c["speedup_vs_sequential_scenes"])]
     print("\n".join(out))
