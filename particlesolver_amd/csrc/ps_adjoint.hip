@@ -319,14 +319,14 @@ __global__ __launch_bounds__(kAdjBlock) void adj_pre_k(DevBuffers b, KConst k, A
 }
 
 // dL/dl -> dL/dC = -lb / (D + eps), dL/dD = -lb l / (D + eps); rho0 terms.
-// Two lanes per particle: lane hh sums lb over half hh of the table (its
-// chunks 2m + hh), then lb = half 0 + half 1, and lane 0 finishes.
+// Four lanes per particle: lane hh sums lb over quarter hh of the table (its
+// chunks 4m + hh), then lb = (q0 + q2) + (q1 + q3), and lane 0 finishes.
 template <bool UNIFORM>
 __global__ __launch_bounds__(kAdjBlock) void adj_lambda_k(DevBuffers b, KConst k, AdjBuffers a, uint32_t n)
 {
-    const uint32_t i = blockIdx.x * (kAdjBlock / 2) + (threadIdx.x >> 1);
-    const uint32_t hh = threadIdx.x & 1u;
-    if (i >= n) return;  // both lanes of a particle leave together
+    const uint32_t i = blockIdx.x * (kAdjBlock / 4) + (threadIdx.x >> 2);
+    const uint32_t hh = threadIdx.x & 3u;
+    if (i >= n) return;  // the four lanes of a particle leave together
     if (!UNIFORM && b.A[i].phase != PS_PHASE_FLUID) {
         if (!hh) a.CD[i] = make_float2(0.0f, 0.0f);
         return;
@@ -355,17 +355,18 @@ __global__ __launch_bounds__(kAdjBlock) void adj_lambda_k(DevBuffers b, KConst k
     };
     const uint32_t c = b.ncnt[i];
     if (UNIFORM) {
-        for_neighbors_half(
-            b, i, c, hh, [&](uint32_t j) { return GLam{a.Pit[j], a.M[j]}; },
+        for_neighbors_strided_chunked(
+            b, i, c, hh, 4u, [&](uint32_t j) { return GLam{a.Pit[j], a.M[j]}; },
             [&](uint32_t, const GLam &d) { body(d.p, d.m, 1.0f, PS_PHASE_FLUID); });
     } else {
-        for_neighbors_half(
-            b, i, c, hh, [&](uint32_t j) { return GLamAttr{a.Pit[j], a.M[j], a.AB[j]}; },
+        for_neighbors_strided_chunked(
+            b, i, c, hh, 4u, [&](uint32_t j) { return GLamAttr{a.Pit[j], a.M[j], a.AB[j]}; },
             [&](uint32_t, const GLamAttr &d) { body(d.p, d.m, d.a.w, d.a.phase); });
     }
+    lb = lb + __shfl_xor(lb, 2, 64);  // lanes 0 and 1: q0 + q2, q1 + q3
     const float lo = __shfl_xor(lb, 1, 64);
     if (hh) return;
-    lb = lb + lo;  // half 0 + half 1
+    lb = lb + lo;
     const float4 sd = a.SD[i];
     const float den = sd.w + k.relax;
     const float lam = pi.w;
@@ -586,15 +587,16 @@ __global__ __launch_bounds__(kAdjBlock) void adj_xsph_k(DevBuffers b, KConst k, 
     a.Y[i] = make_float4(Sh[0], Sh[1], Sh[2], 0.0f);
 }
 
-// Two lanes per particle: lane hh sums the pair terms of half hh of the
-// table (half 0 starts from dL/dx and dL/dv / dt), then half 0 + half 1.
+// Four lanes per particle: lane hh sums the pair terms of quarter hh of the
+// table (quarter 0 starts from dL/dx and dL/dv / dt), then
+// (q0 + q2) + (q1 + q3).
 template <bool UNIFORM>
 __global__ __launch_bounds__(kAdjBlock) void adj_finalize_k(DevBuffers b, KConst k, AdjBuffers a, uint32_t n,
                                                          float dt)
 {
-    const uint32_t p = blockIdx.x * (kAdjBlock / 2) + (threadIdx.x >> 1);
-    const uint32_t hh = threadIdx.x & 1u;
-    if (p >= n) return;  // both lanes of a particle leave together
+    const uint32_t p = blockIdx.x * (kAdjBlock / 4) + (threadIdx.x >> 2);
+    const uint32_t hh = threadIdx.x & 3u;
+    if (p >= n) return;  // the four lanes of a particle leave together
     const Attr ap = b.A[p];
     const float4 vb = a.gv[ap.orig], xb = a.gx[ap.orig];
     float ub[3] = {vb.x / dt, vb.y / dt, vb.z / dt};
@@ -630,16 +632,21 @@ __global__ __launch_bounds__(kAdjBlock) void adj_finalize_k(DevBuffers b, KConst
         };
         const uint32_t c = b.ncnt[p];
         if (UNIFORM) {
-            for_neighbors_half(
-                b, p, c, hh, [&](uint32_t j) { return GXY{b.P[j], b.X[j], a.Y[j]}; },
+            for_neighbors_strided_chunked(
+                b, p, c, hh, 4u, [&](uint32_t j) { return GXY{b.P[j], b.X[j], a.Y[j]}; },
                 [&](uint32_t, const GXY &d) { body(d.p, d.x, d.y, ap.w); });
         } else {
-            for_neighbors_half(
-                b, p, c, hh, [&](uint32_t j) { return GXYA{b.P[j], b.X[j], a.Y[j], b.A[j]}; },
+            for_neighbors_strided_chunked(
+                b, p, c, hh, 4u, [&](uint32_t j) { return GXYA{b.P[j], b.X[j], a.Y[j], b.A[j]}; },
                 [&](uint32_t, const GXYA &d) {
                     if (d.a.phase == PS_PHASE_FLUID) body(d.p, d.x, d.y, d.a.w);
                 });
         }
+    }
+    // lanes 0 and 1 form q0 + q2 and q1 + q3, lane 0 their sum
+    for (int q = 0; q < 3; q++) {
+        ub[q] = ub[q] + __shfl_xor(ub[q], 2, 64);
+        acc[q] = acc[q] + __shfl_xor(acc[q], 2, 64);
     }
     float lu[3], la[3];
     for (int q = 0; q < 3; q++) {
@@ -648,7 +655,7 @@ __global__ __launch_bounds__(kAdjBlock) void adj_finalize_k(DevBuffers b, KConst
     }
     if (hh) return;
     for (int q = 0; q < 3; q++) {
-        ub[q] = ub[q] + lu[q];  // half 0 + half 1
+        ub[q] = ub[q] + lu[q];
         acc[q] = acc[q] + la[q];
     }
     a.pb[p] = make_float4(acc[0] + ub[0], acc[1] + ub[1], acc[2] + ub[2], 0.0f);
@@ -699,26 +706,28 @@ hipError_t launch_adj(hipStream_t st, AdjKernel which, const DevBuffers &b, cons
         else                                                                                       \
             PS_LAUNCH(kern<false>, grid2, block, 0, st, __VA_ARGS__);                     \
     } while (0)
+    // four lanes per particle: kAdjBlock / 4 particles per block
+    const dim3 grid4((count + kAdjBlock / 4 - 1) / (kAdjBlock / 4));
+#define PS_ADJ_LAUNCH4(kern, ...)                                                                 \
+    do {                                                                                           \
+        if (uniform)                                                                               \
+            PS_LAUNCH(kern<true>, grid4, block, 0, st, __VA_ARGS__);                      \
+        else                                                                                       \
+            PS_LAUNCH(kern<false>, grid4, block, 0, st, __VA_ARGS__);                     \
+    } while (0)
     switch (which) {
     case ADJ_XSPH: PS_ADJ_LAUNCH2(adj_xsph_k, b, k, a, count, dt); break;
-    case ADJ_FINALIZE: PS_ADJ_LAUNCH2(adj_finalize_k, b, k, a, count, dt); break;
+    case ADJ_FINALIZE: PS_ADJ_LAUNCH4(adj_finalize_k, b, k, a, count, dt); break;
     case ADJ_PRE: PS_ADJ_LAUNCH(adj_pre_k, b, k, a, count); break;
-    case ADJ_LAMBDA: PS_ADJ_LAUNCH2(adj_lambda_k, b, k, a, count); break;
-    case ADJ_POSITION: {
-        // four lanes per particle: kAdjBlock / 4 particles per block
-        const dim3 grid4((count + kAdjBlock / 4 - 1) / (kAdjBlock / 4));
-        if (uniform)
-            PS_LAUNCH(adj_position_k<true>, grid4, block, 0, st, b, k, a, count);
-        else
-            PS_LAUNCH(adj_position_k<false>, grid4, block, 0, st, b, k, a, count);
-        break;
-    }
+    case ADJ_LAMBDA: PS_ADJ_LAUNCH4(adj_lambda_k, b, k, a, count); break;
+    case ADJ_POSITION: PS_ADJ_LAUNCH4(adj_position_k, b, k, a, count); break;
     case ADJ_STIFFNESS: PS_LAUNCH(adj_stiffness_k, grid, block, 0, st, b, k, a, count); break;
     case ADJ_PREDICT: PS_LAUNCH(adj_predict_k, grid, block, 0, st, b, k, a, count, dt); break;
     default: return hipErrorInvalidValue;
     }
 #undef PS_ADJ_LAUNCH
 #undef PS_ADJ_LAUNCH2
+#undef PS_ADJ_LAUNCH4
     return hipGetLastError();
 }
 
