@@ -74,8 +74,9 @@ typedef enum ps_error {
  * too for the reference's behaviour (it always sleeps). */
 #define PS_FLAG_CPU_REFERENCE 0x4u
 /* Read neighbours from global memory even where the kernels could stage
- * each workgroup's neighbour cells in LDS (uniform single-scene fluids).
- * Results are identical either way. */
+ * each workgroup's neighbour cells in LDS (every 3D composition; in a batch
+ * the workgroups whose particles lie in one scene).  Results are identical
+ * either way. */
 #define PS_FLAG_NO_LDS_STAGING 0x8u
 /* With PS_FLAG_CPU_REFERENCE: the stabilisation pre-pass the reference
  * compiles under USE_STABILIZATION (cpu/src/simulation.h:17-18, off there):
@@ -575,9 +576,9 @@ int ps_grid_sort_noalloc(const ps_grid_desc *grid, uint32_t n, const float *pos,
 
 /* Choose the workgroup size (64, 128, 256 or 512) of each per-particle
  * kernel (predict_hash, reorder, neighbors, lambda, solve, finalize) by
- * timing `substeps` substeps of this system per candidate with events, and,
- * for a uniform single-scene fluid, whether the lambda, solve and finalize
- * kernels stage each workgroup's neighbour cells in LDS (whole substeps
+ * timing `substeps` substeps of this system per candidate with events, and
+ * whether the lambda, solve and finalize kernels stage each workgroup's
+ * neighbour cells in LDS (whole substeps
  * with staging off and on, alternately twice; the choice shows in
  * ps_status.staged_units, zero when off).  Neither choice changes any
  * result.  The simulation state is restored afterwards.  Choices are

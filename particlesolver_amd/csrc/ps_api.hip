@@ -46,8 +46,8 @@ struct ps_system {
     // scene composition
     bool uniform = true;     // every particle fluid with one inverse mass
     float wu = 1.0f;
-    // every fluid particle of one inverse mass wf (any other phases): the
-    // condition for staging a scene that is not a uniform fluid
+    // every fluid particle of one inverse mass wf (any other phases): staged
+    // XSPH then takes the fluid mass as a constant
     bool fluid_mass_uniform = true;
     bool any_fluid = false;
     float wf = 1.0f;

@@ -169,7 +169,7 @@ struct DevBuffers {
     uint32_t *body_of;     // per particle: body + 1
     uint32_t *member_of;   // per particle: member index
     DevStatus *status;
-    // LDS staging (uniform single-scene fluid): one descriptor per unit
+    // LDS staging of neighbour cells: one descriptor per unit
     StageDesc *stage;
     uint32_t staging;      // 1: the last neighbour build staged its units
     // adjoint recompute only (else null): per particle, the coordinates the
