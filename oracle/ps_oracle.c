@@ -93,7 +93,7 @@ static real DIST2(real dx, real dy, real dz) { return FMA(dz, dz, FMA(dy, dy, dx
 /* the half of a neighbour-table entry k: its chunk of four entries, even or odd */
 #define HALF(k) ((int)(((k) >> 2) & 1u))
 /* the quarter of table entry k: its chunk of four entries modulo 4 (the
- * reverse pass's position sums) */
+ * XSPH sum and the reverse pass's sums) */
 #define QUARTER(k) ((int)(((k) >> 2) & 3u))
 
 #include "orc_exp.h"
@@ -1188,18 +1188,20 @@ ORC_EXPORT int SFX(orc_iterate)(orc_sys *s)
  * cpu/src/particle.h:60-65).                                               */
 
 /* S of fluid particle i at iterate P with start positions X (3 per
- * particle): per pair (u_j - u_i) m_j t^3, t = h^2 - r^2, in two halves of
- * the table as lambda_sums; poly6 multiplies the finished sum. */
+ * particle): per pair (u_j - u_i) m_j t^3, t = h^2 - r^2, in four quarters
+ * of the table (QUARTER: chunks of four entries in turn, each in table
+ * order), added as ([0] + [2]) + ([1] + [3]); poly6 multiplies the
+ * finished sum. */
 static void xsph_sum(const orc_sys *s, const real *P, const real *X, uint32_t i, real *S)
 {
     const orc_derived *d = &s->d;
     const uint32_t kmax = s->kmax;
     real px = P[4 * i], py = P[4 * i + 1], pz = P[4 * i + 2];
     real ux = px - X[3 * i], uy = py - X[3 * i + 1], uz = pz - X[3 * i + 2];
-    real hx[2] = {R(0), R(0)}, hy[2] = {R(0), R(0)}, hz[2] = {R(0), R(0)};
+    real hx[4] = {R(0), R(0), R(0), R(0)}, hy[4] = {R(0), R(0), R(0), R(0)}, hz[4] = {R(0), R(0), R(0), R(0)};
     for (uint32_t k = 0; k < s->ncnt[i]; k++) {
         uint32_t j = s->nbr[(size_t)i * kmax + k];
-        const int hf = HALF(k);
+        const int hf = QUARTER(k);
         if (s->phase[j] != PS_PHASE_FLUID) continue;
         real qx = P[4 * j], qy = P[4 * j + 1], qz = P[4 * j + 2];
         real dx = px - qx, dy = py - qy, dz = pz - qz;
@@ -1212,9 +1214,9 @@ static void xsph_sum(const orc_sys *s, const real *P, const real *X, uint32_t i,
         hy[hf] = FMA(ey - uy, m, hy[hf]);
         hz[hf] = FMA(ez - uz, m, hz[hf]);
     }
-    S[0] = d->poly6 * (hx[0] + hx[1]);
-    S[1] = d->poly6 * (hy[0] + hy[1]);
-    S[2] = d->poly6 * (hz[0] + hz[1]);
+    S[0] = d->poly6 * ((hx[0] + hx[2]) + (hx[1] + hx[3]));
+    S[1] = d->poly6 * ((hy[0] + hy[2]) + (hy[1] + hy[3]));
+    S[2] = d->poly6 * ((hz[0] + hz[2]) + (hz[1] + hz[3]));
 }
 
 ORC_EXPORT int SFX(orc_end)(orc_sys *s)

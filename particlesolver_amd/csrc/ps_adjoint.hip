@@ -537,10 +537,10 @@ template <bool UNIFORM>
 __global__ __launch_bounds__(kAdjBlock) void adj_xsph_k(DevBuffers b, KConst k, AdjBuffers a, uint32_t n,
                                                      float dt)
 {
-    // two lanes per particle: lane hh forms half hh of S, lane 0 finishes
-    const uint32_t i = blockIdx.x * (kAdjBlock / 2) + (threadIdx.x >> 1);
-    const uint32_t hh = threadIdx.x & 1u;
-    if (i >= n) return;  // both lanes of a particle leave together
+    // four lanes per particle: lane hh forms quarter hh of S, lane 0 finishes
+    const uint32_t i = blockIdx.x * (kAdjBlock / 4) + (threadIdx.x >> 2);
+    const uint32_t hh = threadIdx.x & 3u;
+    if (i >= n) return;  // the four lanes of a particle leave together
     const Attr ai = b.A[i];
     float r0 = 0.0f;
     float Sh[3] = {0.0f, 0.0f, 0.0f};
@@ -550,7 +550,7 @@ __global__ __launch_bounds__(kAdjBlock) void adj_xsph_k(DevBuffers b, KConst k, 
         const float4 vb = a.gv[ai.orig];
         const float vbn[3] = {vb.x, vb.y, vb.z};
         const float cr = k.xsph_c / ai.rho0;
-        // S exactly as finalize_k forms it: two halves of the table
+        // S exactly as finalize_k forms it: four quarters of the table
         float Sm[3] = {0.0f, 0.0f, 0.0f};
         auto body = [&](const float4 &pj, const float4 &xj, float wj) {
             const float r2 = dist2(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z);
@@ -563,16 +563,18 @@ __global__ __launch_bounds__(kAdjBlock) void adj_xsph_k(DevBuffers b, KConst k, 
         };
         const uint32_t c = b.ncnt[i];
         if (UNIFORM) {
-            for_neighbors_half(
-                b, i, c, hh, [&](uint32_t j) { return PX{b.P[j], b.X[j]}; },
+            for_neighbors_strided_chunked(
+                b, i, c, hh, 4u, [&](uint32_t j) { return PX{b.P[j], b.X[j]}; },
                 [&](uint32_t, const PX &d) { body(d.p, d.x, ai.w); });
         } else {
-            for_neighbors_half(
-                b, i, c, hh, [&](uint32_t j) { return GXA{b.P[j], b.X[j], b.A[j]}; },
+            for_neighbors_strided_chunked(
+                b, i, c, hh, 4u, [&](uint32_t j) { return GXA{b.P[j], b.X[j], b.A[j]}; },
                 [&](uint32_t, const GXA &d) {
                     if (d.a.phase == PS_PHASE_FLUID) body(d.p, d.x, d.a.w);
                 });
         }
+        // (q0 + q2) + (q1 + q3)
+        for (int q = 0; q < 3; q++) Sm[q] = Sm[q] + __shfl_xor(Sm[q], 2, 64);
         float lo[3];
         for (int q = 0; q < 3; q++) lo[q] = __shfl_xor(Sm[q], 1, 64);
         if (hh) return;
@@ -697,15 +699,6 @@ hipError_t launch_adj(hipStream_t st, AdjKernel which, const DevBuffers &b, cons
         else                                                                                       \
             PS_LAUNCH(kern<false>, grid, block, 0, st, __VA_ARGS__);                      \
     } while (0)
-    // two lanes per particle: kAdjBlock / 2 particles per block
-    const dim3 grid2((count + kAdjBlock / 2 - 1) / (kAdjBlock / 2));
-#define PS_ADJ_LAUNCH2(kern, ...)                                                                 \
-    do {                                                                                           \
-        if (uniform)                                                                               \
-            PS_LAUNCH(kern<true>, grid2, block, 0, st, __VA_ARGS__);                      \
-        else                                                                                       \
-            PS_LAUNCH(kern<false>, grid2, block, 0, st, __VA_ARGS__);                     \
-    } while (0)
     // four lanes per particle: kAdjBlock / 4 particles per block
     const dim3 grid4((count + kAdjBlock / 4 - 1) / (kAdjBlock / 4));
 #define PS_ADJ_LAUNCH4(kern, ...)                                                                 \
@@ -716,7 +709,7 @@ hipError_t launch_adj(hipStream_t st, AdjKernel which, const DevBuffers &b, cons
             PS_LAUNCH(kern<false>, grid4, block, 0, st, __VA_ARGS__);                     \
     } while (0)
     switch (which) {
-    case ADJ_XSPH: PS_ADJ_LAUNCH2(adj_xsph_k, b, k, a, count, dt); break;
+    case ADJ_XSPH: PS_ADJ_LAUNCH4(adj_xsph_k, b, k, a, count, dt); break;
     case ADJ_FINALIZE: PS_ADJ_LAUNCH4(adj_finalize_k, b, k, a, count, dt); break;
     case ADJ_PRE: PS_ADJ_LAUNCH(adj_pre_k, b, k, a, count); break;
     case ADJ_LAMBDA: PS_ADJ_LAUNCH4(adj_lambda_k, b, k, a, count); break;
@@ -726,7 +719,6 @@ hipError_t launch_adj(hipStream_t st, AdjKernel which, const DevBuffers &b, cons
     default: return hipErrorInvalidValue;
     }
 #undef PS_ADJ_LAUNCH
-#undef PS_ADJ_LAUNCH2
 #undef PS_ADJ_LAUNCH4
     return hipGetLastError();
 }

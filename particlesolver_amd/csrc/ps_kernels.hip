@@ -78,13 +78,15 @@ __global__ __launch_bounds__(kMaxBlock) void predict_hash_k(DevBuffers b, KConst
     float px = 0.0f, py = 0.0f, pz = 0.0f;
     if (valid) {
         if (PRED) {
+            // the three loads go out together (a static particle's velocity
+            // is read and not used)
             const float4 x = b.X[s];
             const Attr a = b.A[s];
+            const float4 v = b.V[s];
             px = x.x;
             py = x.y;
             pz = x.z;
             if (a.w != 0.0f) {
-                const float4 v = b.V[s];
                 const float gx = multi ? sc->gravity[0] : k.gravity[0];
                 const float gy = multi ? sc->gravity[1] : k.gravity[1];
                 const float gz = multi ? sc->gravity[2] : k.gravity[2];
@@ -1335,15 +1337,21 @@ __global__ __launch_bounds__(kBlock) void shape_k(DevBuffers b, uint32_t nbody)
 // plus XSPH over fluid neighbours from displacements u = x - x*:
 //   v_i = u_i/dt + ((c/rho0_i) sum_j (u_j - u_i) m_j W_ij)/dt
 // with the pair term (u_j - u_i) m_j t^3, t = h^2 - r^2, and poly6
-// multiplying the finished sum.
+// multiplying the finished sum.  Four lanes per particle: lane hh sums
+// quarter hh of the table (its chunks hh, hh + 4, ..., in table order), and
+// two shuffles add the quarters as (q0 + q2) + (q1 + q3).  Its one sum is
+// short (about 17 VALU per pair), so the kernel is bound by latency more
+// than by issue: four lanes give the 256 particles of a staged unit (24
+// bytes of LDS per slot, two units per CU) eight wavefronts per SIMD.
+
+static constexpr uint32_t kFinalizeStagedBlock = 4u * kStageUnit;
 
 template <bool UNIFORM, bool STAGED>
-__global__ __launch_bounds__(kMaxBlock) void finalize_k(DevBuffers b, KConst k, uint32_t n, float mu,
-                                                     float dt, uint32_t its)
+__global__ __launch_bounds__(STAGED ? kFinalizeStagedBlock : kMaxBlock) void finalize_k(
+    DevBuffers b, KConst k, uint32_t n, float mu, float dt, uint32_t its)
 {
     // staged: neighbour position and displacement u = p - x* (x, y, z, u.x)
     // and (u.y, u.z), formed with the same subtractions as the global path
-    // two lanes per particle when staged, as in lambda_k and solve_k
     // (and the sentinel record).  In a scene that is not a uniform fluid,
     // with every fluid particle of one mass (mu >= 0), a particle that is
     // not fluid is staged at infinity: the XSPH sum leaves it out, as the
@@ -1354,8 +1362,8 @@ __global__ __launch_bounds__(kMaxBlock) void finalize_k(DevBuffers b, KConst k, 
     __shared__ float2 lds_u[STAGED ? kStageSlots + 1 : 1];
     __shared__ uint32_t s_src[STAGED && !UNIFORM ? kStageChunks : 1];
     const uint32_t unit = xcd_block();
-    const uint32_t hh = STAGED ? (threadIdx.x & 1u) : 0u;
-    const uint32_t i = STAGED ? unit * kStageUnit + (threadIdx.x >> 1) : unit * blockDim.x + threadIdx.x;
+    const uint32_t hh = threadIdx.x & 3u;
+    const uint32_t i = STAGED ? unit * kStageUnit + (threadIdx.x >> 2) : (unit * blockDim.x + threadIdx.x) >> 2;
     bool staged = false, mapped = false;
     if (STAGED) {
         const StageDesc *d = b.stage + unit;
@@ -1372,17 +1380,17 @@ __global__ __launch_bounds__(kMaxBlock) void finalize_k(DevBuffers b, KConst k, 
             lds_u[kStageSentinel] = make_float2(0.0f, 0.0f);
         }
         if (staged)
-            stage_copy<4, 2 * kStageUnit>(d, n, [&](uint32_t j) { return PX{b.P[j], b.X[j]}; },
-                                          [&](uint32_t l, const PX &v) {
-                                              lds_pu[l] = !UNIFORM && v.x.w < 0.0f
-                                                              ? make_float4(inf, inf, inf, 0.0f)
-                                                              : make_float4(v.p.x, v.p.y, v.p.z, v.p.x - v.x.x);
-                                              lds_u[l] = make_float2(v.p.y - v.x.y, v.p.z - v.x.z);
-                                          });
+            stage_copy<3, kFinalizeStagedBlock>(d, n, [&](uint32_t j) { return PX{b.P[j], b.X[j]}; },
+                                                [&](uint32_t l, const PX &v) {
+                                                    lds_pu[l] = !UNIFORM && v.x.w < 0.0f
+                                                                    ? make_float4(inf, inf, inf, 0.0f)
+                                                                    : make_float4(v.p.x, v.p.y, v.p.z, v.p.x - v.x.x);
+                                                    lds_u[l] = make_float2(v.p.y - v.x.y, v.p.z - v.x.z);
+                                                });
     }
     // every solve of this substep has completed
     if (i == 0 && hh == 0) b.status->iter += its;
-    if (i >= n) return;
+    if (i >= n) return;  // the four lanes of a particle leave together
     const float4 p = b.P[i];
     const float4 x = b.X[i];
     const float ux = p.x - x.x, uy = p.y - x.y, uz = p.z - x.z;
@@ -1390,12 +1398,13 @@ __global__ __launch_bounds__(kMaxBlock) void finalize_k(DevBuffers b, KConst k, 
     const Attr a = b.A[i];
     if ((UNIFORM || a.phase == PS_PHASE_FLUID) && k.xsph_c != 0.0f) {
         const uint32_t c = b.ncnt[i];
-        // the XSPH sum of one half of the table
+        // the XSPH sum of this lane's quarter of the table
         struct Sums {
             float sx, sy, sz;
         };
+        Sums s = {0.0f, 0.0f, 0.0f};
         // qj: neighbour position, e: its displacement over the substep
-        auto body = [&](Sums &s, const float4 &qj, float ex, float ey, float ez, float mj) {
+        auto body = [&](const float4 &qj, float ex, float ey, float ez, float mj) {
             float dx = p.x - qj.x, dy = p.y - qj.y, dz = p.z - qj.z;
             float r2 = dist2(dx, dy, dz);
             if (!(r2 < k.h2)) return;
@@ -1409,63 +1418,48 @@ __global__ __launch_bounds__(kMaxBlock) void finalize_k(DevBuffers b, KConst k, 
             float4 pu;
             float2 u;
         };
-        auto glob = [&](Sums &s, const PX &d) {
-            body(s, d.p, d.p.x - d.x.x, d.p.y - d.x.y, d.p.z - d.x.z, mu);
-        };
+        auto glob = [&](const PX &d) { body(d.p, d.p.x - d.x.x, d.p.y - d.x.y, d.p.z - d.x.z, mu); };
         // the neighbour's mass from x*.w (the sort's 1/w for a fluid
         // particle, -1 for any other: left out)
-        auto gen = [&](Sums &s, const PX &d) {
+        auto gen = [&](const PX &d) {
             if (d.x.w < 0.0f) return;
-            body(s, d.p, d.p.x - d.x.x, d.p.y - d.x.y, d.p.z - d.x.z, d.x.w);
+            body(d.p, d.p.x - d.x.x, d.p.y - d.x.y, d.p.z - d.x.z, d.x.w);
         };
-        Sums s0 = {0.0f, 0.0f, 0.0f}, s1 = {0.0f, 0.0f, 0.0f};
-        if constexpr (STAGED) {
-            // this lane's half; the partner lane (lane ^ 1) holds the other
-            Sums mine = {0.0f, 0.0f, 0.0f};
-            if (staged)
-                for_neighbors_half<false>(
-                    b, i, c, hh, [&](uint32_t l) { return PU{lds_pu[l], lds_u[l]}; },
-                    [&](uint32_t, const PU &d) { body(mine, d.pu, d.pu.w, d.u.x, d.u.y, mu); });
-            else if (mapped)
-                // table entries are LDS indices (the sentinel padding, never
-                // visited, maps to i)
-                for_neighbors_half(
-                    b, i, c, hh,
-                    [&](uint32_t e) {
-                        const uint32_t j = e >= kStageSlots ? i : s_src[e >> 4] + (e & (kStageChunk - 1u));
-                        return PX{b.P[j], b.X[j]};
-                    },
-                    [&](uint32_t, const PX &d) { gen(mine, d); });
-            else
-                for_neighbors_half(
-                    b, i, c, hh, [&](uint32_t j) { return PX{b.P[j], b.X[j]}; },
-                    [&](uint32_t, const PX &d) {
-                        if (UNIFORM)
-                            glob(mine, d);
-                        else
-                            gen(mine, d);
-                    });
-            const Sums other = {__shfl_xor(mine.sx, 1, 64), __shfl_xor(mine.sy, 1, 64),
-                                __shfl_xor(mine.sz, 1, 64)};
-            s0 = hh ? other : mine;
-            s1 = hh ? mine : other;
-        } else if (UNIFORM) {
-            for_neighbors_split(
-                b, i, c, [&](uint32_t j) { return PX{b.P[j], b.X[j]}; },
-                [&](uint32_t, const PX &d) { glob(s0, d); }, [&](uint32_t, const PX &d) { glob(s1, d); });
-        } else {
-            for_neighbors_split(
-                b, i, c, [&](uint32_t j) { return PX{b.P[j], b.X[j]}; },
-                [&](uint32_t, const PX &d) { gen(s0, d); }, [&](uint32_t, const PX &d) { gen(s1, d); });
-        }
-        const float sx = k.poly6 * (s0.sx + s1.sx), sy = k.poly6 * (s0.sy + s1.sy),
-                    sz = k.poly6 * (s0.sz + s1.sz);
+        if (STAGED && staged)
+            for_neighbors_strided_chunked(
+                b, i, c, hh, 4u, [&](uint32_t l) { return PU{lds_pu[l], lds_u[l]}; },
+                [&](uint32_t, const PU &d) { body(d.pu, d.pu.w, d.u.x, d.u.y, mu); });
+        else if (STAGED && mapped)
+            // table entries are LDS indices (the sentinel padding, never
+            // visited, maps to i)
+            for_neighbors_strided_chunked(
+                b, i, c, hh, 4u,
+                [&](uint32_t e) {
+                    const uint32_t j = e >= kStageSlots ? i : s_src[e >> 4] + (e & (kStageChunk - 1u));
+                    return PX{b.P[j], b.X[j]};
+                },
+                [&](uint32_t, const PX &d) { gen(d); });
+        else
+            for_neighbors_strided_chunked(
+                b, i, c, hh, 4u, [&](uint32_t j) { return PX{b.P[j], b.X[j]}; },
+                [&](uint32_t, const PX &d) {
+                    if (UNIFORM)
+                        glob(d);
+                    else
+                        gen(d);
+                });
+        // lanes 0 and 1 form q0 + q2 and q1 + q3, lane 0 their sum
+        s.sx = s.sx + __shfl_xor(s.sx, 2, 64);
+        s.sy = s.sy + __shfl_xor(s.sy, 2, 64);
+        s.sz = s.sz + __shfl_xor(s.sz, 2, 64);
+        const Sums o = {__shfl_xor(s.sx, 1, 64), __shfl_xor(s.sy, 1, 64), __shfl_xor(s.sz, 1, 64)};
+        const float sx = k.poly6 * (s.sx + o.sx), sy = k.poly6 * (s.sy + o.sy), sz = k.poly6 * (s.sz + o.sz);
         const float cr = k.xsph_c / a.rho0;
         vx = vx + ((cr * sx) / dt);
         vy = vy + ((cr * sy) / dt);
         vz = vz + ((cr * sz) / dt);
     }
-    if (STAGED && hh) return;  // the pair's lane 0 writes
+    if (hh) return;  // the particle's lane 0 writes
     b.V[i] = make_float4(vx, vy, vz, 0.0f);
 }
 
@@ -1659,14 +1653,15 @@ hipError_t launch_finalize(hipStream_t st, const DevBuffers &b, const KConst &k,
                            uint32_t bs)
 {
     if (n == 0) return hipSuccess;
-    const dim3 g(blocks_for(n, bs)), t(bs);
+    // four lanes per particle (bs is a multiple of 64)
+    const dim3 g(blocks_for(n, bs / 4u)), t(bs);
     const float mu = 1.0f / wu;
     if (uniform && b.staging)
         PS_LAUNCH((finalize_k<true, true>), dim3(blocks_for(n, kStageUnit)),
-                           dim3(2 * kStageUnit), 0, st, b, k, n, mu, dt, its);
+                           dim3(kFinalizeStagedBlock), 0, st, b, k, n, mu, dt, its);
     else if (b.staging)  // wf >= 0: every fluid particle of inverse mass wf
         PS_LAUNCH((finalize_k<false, true>), dim3(blocks_for(n, kStageUnit)),
-                           dim3(2 * kStageUnit), 0, st, b, k, n, wf >= 0.0f ? 1.0f / wf : -1.0f, dt, its);
+                           dim3(kFinalizeStagedBlock), 0, st, b, k, n, wf >= 0.0f ? 1.0f / wf : -1.0f, dt, its);
     else if (uniform)
         PS_LAUNCH((finalize_k<true, false>), g, t, 0, st, b, k, n, mu, dt, its);
     else
