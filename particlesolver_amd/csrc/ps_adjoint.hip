@@ -29,6 +29,9 @@
 
 namespace ps {
 
+// The kernels that gather over the neighbour table map blocks with
+// xcd_block(), so each XCD reverses one contiguous range of sorted particles
+// and its L2 holds that region's records.
 static constexpr uint32_t kAdjBlock = 256;
 
 __device__ __forceinline__ float dot3(const float *a, const float *b)
@@ -58,58 +61,6 @@ __device__ __forceinline__ void adj_grad_w6(const KConst &k, const float *dv, fl
     const float c = -((6.0f * k.poly6) * (t * t));
     for (int a = 0; a < 3; a++) g[a] = c * dv[a];
 }
-
-// tensile term s(d) = -k (W / W(dq h))^n and its gradient in d
-__device__ __forceinline__ float adj_corr(const KConst &k, const float *dv, float r2, float *grad)
-{
-    const float t = k.h2 - r2;
-    const float W = k.poly6 * ((t * t) * t);
-    const float x = W * k.inv_w_dq;
-    const uint32_t n = k.n_corr;
-    const float s = -(k.k_corr * ipow(x, n));
-    const float xn1 = n > 1u ? ipow(x, n - 1u) : 1.0f;
-    const float c = (((k.k_corr * (float)n) * xn1) * k.inv_w_dq) * ((6.0f * k.poly6) * (t * t));
-    for (int a = 0; a < 3; a++) grad[a] = c * dv[a];
-    return s;
-}
-
-// E = d/dd [mu . (l_i + l_j + s(d)) G(d)] for owner i, d = p_i - p_j;
-// J_G = -(b I + (b'/r) d d^T), b' = -spiky (h - r)(h + r) / r^2
-__device__ __forceinline__ void adj_pair(const KConst &k, const float *dv, float r2, float li, float lj,
-                                         const float *mu, float *E)
-{
-    float gc[3], g[3];
-    const float s = adj_corr(k, dv, r2, gc);
-    const float coef = (li + lj) + s;
-    adj_gamma(k, dv, r2, g);
-    const float mg = dot3(mu, g);
-    for (int a = 0; a < 3; a++) E[a] = mg * gc[a];
-    if (r2 < kDegenerateR2) return;
-    const float ri = rsq(r2);
-    const float r = r2 * ri;
-    const float q = k.h - r;
-    const float bb = (k.spiky * (q * q)) * ri;
-    const float bp = -(((k.spiky * q) * (k.h + r)) * (ri * ri));
-    const float dm = dot3(dv, mu);
-    const float c2 = (bp * ri) * dm;
-    for (int a = 0; a < 3; a++) E[a] = E[a] - (coef * ((bb * mu[a]) + (c2 * dv[a])));
-}
-
-// dD_i/dd for owner i (non-degenerate pair), sk = spiky / rho0_i
-__device__ __forceinline__ void adj_dD(const KConst &k, const float *dv, float r2, float sk, const float *S,
-                                       float *out)
-{
-    const float ri = rsq(r2);
-    const float r = r2 * ri;
-    const float q = k.h - r;
-    const float aa = (sk * (q * q)) * ri;
-    const float ap = -(((sk * q) * (k.h + r)) * (ri * ri));
-    const float ds = dot3(dv, S);
-    // d(a^2 r^2)/dd = -4 sk^2 (h - r)^3 d / r
-    const float c1 = -((((4.0f * sk) * sk) * ((q * q) * q)) * ri);
-    for (int a = 0; a < 3; a++) out[a] = (c1 * dv[a]) + (2.0f * ((aa * S[a]) + (((ap * ri) * ds) * dv[a])));
-}
-
 
 // Cloth contact of owner i with j (collideD without friction: a distance
 // constraint of rest collide_dist, stiffness 1, count nc): M^T y for
@@ -148,64 +99,65 @@ __device__ __forceinline__ void yd_of(const KConst &k, const float4 &Y, float *y
     yd[2] = k.dims == 2u ? 0.0f : Y.z;
 }
 
-// Quantities of one pair shared by every term of both owners, evaluated
-// once (each is the value the term-by-term helpers above compute).
-struct PairQ {
-    float t, tt, s, cc, c6;  // h^2 - r^2, its square, tensile term, d s/dd and dW/dd per unit d
-    bool deg;
-    float ri, r, q, qq, bb, bp;  // 1/r, r, h - r, (h - r)^2, b, b' (non-degenerate pairs)
-};
-
-__device__ __forceinline__ PairQ pair_q(const KConst &k, float r2)
+// Both owners' fluid terms of the pair (p, j) in dL/dp_p, d = p_p - p_j:
+// E_p - E_j, with E_i = d/dd [mu_i . (l_i + l_j + s(d)) G(d)] for owner i
+// (J_G = -(b I + (b'/r) d d^T), b' = -spiky (h - r)(h + r) / r^2), plus
+// cm_p g6(d) - cm_j g6(-d) (the density term, g6 = dW/dd) and
+// Db_p dD_p(d) - Db_j dD_j(-d) (dD_i/dd of the denominator D_i).  Folded
+// into one expression,
+//   (alpha - K mD + cms c6) d + 2A dV - cb dmu,
+//   dmu = mu_p - mu_j, mD = d.dmu, K = b cc + coef b'/r, cb = coef b,
+//   dV = V_p - V_j, alpha = C us - 2 Bq (d.dV), us = u_p + u_j,
+//   A = (h - r)^2 / r, Bq = (h - r)(h + r) / r^3, C = -4 (h - r)^3 / r,
+//   cms = Cr_p m_j + Cr_j m_p,
+// with the per-particle factors of adj_lambda_k (MC: mu, Cr; KV: V, u).  An
+// owner that does not take part (op, oj false) enters with zeros.  cc and
+// c6 are the tensile term's and W's gradient factors (dW/dd = c6 d).  A
+// degenerate pair (G the constant (0, g0y, 0) for both owners) adds
+// (cc g0y (mu_p.y + mu_j.y) + cms c6) d.  Adds the terms to acc.
+__device__ __forceinline__ void adj_pair_sum(const KConst &k, const float *dv, float r2, float lsum, bool op,
+                                             bool oj, const float4 &mcp, const float4 &mcj, const float4 &kvp,
+                                             const float4 &kvj, float massp, float massj, float *acc)
 {
-    PairQ Q;
-    Q.t = k.h2 - r2;
-    Q.tt = Q.t * Q.t;
-    const float W = k.poly6 * (Q.tt * Q.t);
+    const float t = k.h2 - r2;
+    const float tt = t * t;
+    const float W = k.poly6 * (tt * t);
     const float x = W * k.inv_w_dq;
     const uint32_t n = k.n_corr;
-    Q.s = -(k.k_corr * ipow(x, n));
+    const float s = -(k.k_corr * ipow(x, n));
     const float xn1 = n > 1u ? ipow(x, n - 1u) : 1.0f;
-    Q.cc = (((k.k_corr * (float)n) * xn1) * k.inv_w_dq) * ((6.0f * k.poly6) * Q.tt);
-    Q.c6 = -((6.0f * k.poly6) * Q.tt);
-    Q.deg = r2 < kDegenerateR2;
-    Q.ri = rsq(r2);
-    Q.r = r2 * Q.ri;
-    Q.q = k.h - Q.r;
-    Q.qq = Q.q * Q.q;
-    Q.bb = (k.spiky * Q.qq) * Q.ri;
-    Q.bp = -(((k.spiky * Q.q) * (k.h + Q.r)) * (Q.ri * Q.ri));
-    return Q;
-}
-
-// adj_pair from shared quantities (coef = (l_i + l_j) + s)
-__device__ __forceinline__ void pair_E(const KConst &k, const PairQ &Q, const float *dv, float coef,
-                                       const float *mu, float *E)
-{
-    float g[3];
-    if (Q.deg) {
-        g[0] = 0.0f;
-        g[1] = -((k.spiky * (k.h * k.h)) * 0.001f);
-        g[2] = 0.0f;
-    } else {
-        for (int a = 0; a < 3; a++) g[a] = -(Q.bb * dv[a]);
+    const float cc = (((k.k_corr * (float)n) * xn1) * k.inv_w_dq) * ((6.0f * k.poly6) * tt);
+    const float c6 = -((6.0f * k.poly6) * tt);
+    const float coef = lsum + s;
+    const float mup[3] = {op ? mcp.x : 0.0f, op ? mcp.y : 0.0f, op ? mcp.z : 0.0f};
+    const float muj[3] = {oj ? mcj.x : 0.0f, oj ? mcj.y : 0.0f, oj ? mcj.z : 0.0f};
+    const float cms = (op ? mcp.w * massj : 0.0f) + (oj ? mcj.w * massp : 0.0f);
+    if (r2 < kDegenerateR2) {
+        const float g0y = -((k.spiky * (k.h * k.h)) * 0.001f);
+        const float be = (cc * (g0y * (mup[1] + muj[1]))) + (cms * c6);
+        for (int a = 0; a < 3; a++) acc[a] = acc[a] + (be * dv[a]);
+        return;
     }
-    const float mg = dot3(mu, g);
-    for (int a = 0; a < 3; a++) E[a] = mg * (Q.cc * dv[a]);
-    if (Q.deg) return;
-    const float c2 = (Q.bp * Q.ri) * dot3(dv, mu);
-    for (int a = 0; a < 3; a++) E[a] = E[a] - (coef * ((Q.bb * mu[a]) + (c2 * dv[a])));
-}
-
-// adj_dD from shared quantities (non-degenerate pair)
-__device__ __forceinline__ void pair_dD(const KConst &k, const PairQ &Q, const float *dv, float sk, const float *S,
-                                        float *out)
-{
-    const float aa = (sk * Q.qq) * Q.ri;
-    const float ap = -(((sk * Q.q) * (k.h + Q.r)) * (Q.ri * Q.ri));
-    const float ds = dot3(dv, S);
-    const float c1 = -((((4.0f * sk) * sk) * (Q.qq * Q.q)) * Q.ri);
-    for (int a = 0; a < 3; a++) out[a] = (c1 * dv[a]) + (2.0f * ((aa * S[a]) + (((ap * Q.ri) * ds) * dv[a])));
+    const float ri = rsq(r2);
+    const float r = r2 * ri;
+    const float q = k.h - r;
+    const float qq = q * q;
+    const float bb = (k.spiky * qq) * ri;
+    const float bp = -(((k.spiky * q) * (k.h + r)) * (ri * ri));
+    const float dmu[3] = {mup[0] - muj[0], mup[1] - muj[1], mup[2] - muj[2]};
+    const float mD = dot3(dv, dmu);
+    const float K = (bb * cc) + (coef * (bp * ri));
+    const float cb = coef * bb;
+    const float A2 = 2.0f * (qq * ri);
+    const float Bq = ((q * (k.h + r)) * (ri * ri)) * ri;
+    const float C = -((4.0f * (qq * q)) * ri);
+    const float us = (op ? kvp.w : 0.0f) + (oj ? kvj.w : 0.0f);
+    const float dV[3] = {(op ? kvp.x : 0.0f) - (oj ? kvj.x : 0.0f), (op ? kvp.y : 0.0f) - (oj ? kvj.y : 0.0f),
+                         (op ? kvp.z : 0.0f) - (oj ? kvj.z : 0.0f)};
+    const float vD = dot3(dv, dV);
+    const float al = (C * us) - ((2.0f * Bq) * vD);
+    const float be = (al - (K * mD)) + (cms * c6);
+    for (int a = 0; a < 3; a++) acc[a] = acc[a] + ((be * dv[a]) + ((A2 * dV[a]) - (cb * dmu[a])));
 }
 
 // density multiplier of a neighbour in rho_i
@@ -226,12 +178,10 @@ struct GLamAttr {
     AdjAttr a;
 };
 struct GPos {
-    float4 p, m, sd;
-    float2 cd;
+    float4 p, mc, kv;
 };
 struct GPosAttr {
-    float4 p, m, sd;
-    float2 cd;
+    float4 p, mc, kv;
     AdjAttr a;
 };
 struct GXA {
@@ -324,15 +274,18 @@ __global__ __launch_bounds__(kAdjBlock) void adj_pre_k(DevBuffers b, KConst k, A
 template <bool UNIFORM>
 __global__ __launch_bounds__(kAdjBlock) void adj_lambda_k(DevBuffers b, KConst k, AdjBuffers a, uint32_t n)
 {
-    const uint32_t i = blockIdx.x * (kAdjBlock / 4) + (threadIdx.x >> 2);
+    const uint32_t i = xcd_block() * (kAdjBlock / 4) + (threadIdx.x >> 2);
     const uint32_t hh = threadIdx.x & 3u;
     if (i >= n) return;  // the four lanes of a particle leave together
+    const float4 mi = a.M[i];
     if (!UNIFORM && b.A[i].phase != PS_PHASE_FLUID) {
-        if (!hh) a.CD[i] = make_float2(0.0f, 0.0f);
+        if (!hh) {
+            a.MC[i] = make_float4(mi.x, mi.y, mi.z, 0.0f);
+            a.KV[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
         return;
     }
     const float4 pi = a.Pit[i];
-    const float4 mi = a.M[i];
     const float mu[3] = {mi.x, mi.y, mi.z};
     const float ir0 = mi.w;
     float lb = 0.0f;
@@ -372,11 +325,15 @@ __global__ __launch_bounds__(kAdjBlock) void adj_lambda_k(DevBuffers b, KConst k
     const float lam = pi.w;
     const float Cb = -(lb / den);
     const float Db = -((lb * lam) / den);
-    a.CD[i] = make_float2(Cb, Db);
     float r = a.r0b[i];
     r = r - (Cb * (a.rho[i] * (ir0 * ir0)));
     r = r - (Db * ((2.0f * sd.w) * ir0));
     a.r0b[i] = r;
+    // the factors adj_position_k combines per pair
+    const float sk = k.spiky * ir0;
+    const float dk = Db * sk;
+    a.MC[i] = make_float4(mu[0], mu[1], mu[2], Cb * ir0);
+    a.KV[i] = make_float4(dk * sd.x, dk * sd.y, dk * sd.z, Db * (sk * sk));
 }
 
 // dL/dp before the iteration.  Four lanes per particle: lane hh sums the
@@ -386,7 +343,7 @@ __global__ __launch_bounds__(kAdjBlock) void adj_lambda_k(DevBuffers b, KConst k
 template <bool UNIFORM>
 __global__ __launch_bounds__(kAdjBlock) void adj_position_k(DevBuffers b, KConst k, AdjBuffers a, uint32_t n)
 {
-    const uint32_t p = blockIdx.x * (kAdjBlock / 4) + (threadIdx.x >> 2);
+    const uint32_t p = xcd_block() * (kAdjBlock / 4) + (threadIdx.x >> 2);
     const uint32_t hh = threadIdx.x & 3u;
     if (p >= n) return;  // the four lanes of a particle leave together
     const float4 pp = a.Pit[p];
@@ -396,70 +353,44 @@ __global__ __launch_bounds__(kAdjBlock) void adj_position_k(DevBuffers b, KConst
     float ydp[3];
     yd_of(k, Yp, ydp);
     const bool pf = UNIFORM || ap.phase == PS_PHASE_FLUID;
-    const float4 mp4 = a.M[p];
-    const float mup[3] = {mp4.x, mp4.y, mp4.z};
-    const float irp = mp4.w;
-    const float2 cdp = a.CD[p];
-    const float4 sdp = a.SD[p];
-    const float Sp[3] = {sdp.x, sdp.y, sdp.z};
+    const float4 mcp = a.MC[p];  // mu_p, Cr_p
+    const float4 kvp = a.KV[p];  // V_p, u_p
     const uint32_t ncp = UNIFORM ? 0u : a.nc[p];
     const bool contacts = !UNIFORM && a.contacts;
     const float massp = adj_mass(k, ap.w, ap.phase);  // p as a neighbour of j
-    auto body = [&](const float4 &pj, const float4 &mj, const float4 &sdj, const float2 &cdj, float wj,
-                    int32_t phj, uint32_t ncj, float massj) {
+    auto body = [&](const float4 &pj, const float4 &mcj, const float4 &kvj, float wj, int32_t phj, uint32_t ncj,
+                    float massj) {
         const float dv[3] = {pp.x - pj.x, pp.y - pj.y, pp.z - pj.z};
-        const float dj[3] = {-dv[0], -dv[1], -dv[2]};
         const float r2 = dist2(dv[0], dv[1], dv[2]);
         if (r2 < k.h2) {
-            const PairQ Q = pair_q(k, r2);
-            const float coef = (pp.w + pj.w) + Q.s;
-            float E[3], dd[3];
-            if (pf && wj != 0.0f) {
-                // p owns the pair
-                pair_E(k, Q, dv, coef, mup, E);
-                const float cm = (cdp.x * massj) * irp;
-                for (int q2 = 0; q2 < 3; q2++) acc[q2] = (acc[q2] + E[q2]) + (cm * (Q.c6 * dv[q2]));
-                if (!Q.deg) {
-                    pair_dD(k, Q, dv, k.spiky * irp, Sp, dd);
-                    for (int q2 = 0; q2 < 3; q2++) acc[q2] = acc[q2] + (cdp.y * dd[q2]);
-                }
-            }
-            if (phj == PS_PHASE_FLUID && ap.w != 0.0f) {
-                // j owns the pair (d_jp = -d)
-                const float muj[3] = {mj.x, mj.y, mj.z};
-                const float irj = mj.w;
-                pair_E(k, Q, dj, coef, muj, E);
-                const float cm = (cdj.x * massp) * irj;
-                for (int q2 = 0; q2 < 3; q2++) acc[q2] = (acc[q2] - E[q2]) - (cm * (Q.c6 * dj[q2]));
-                if (!Q.deg) {
-                    const float Sj[3] = {sdj.x, sdj.y, sdj.z};
-                    pair_dD(k, Q, dj, k.spiky * irj, Sj, dd);
-                    for (int q2 = 0; q2 < 3; q2++) acc[q2] = acc[q2] - (cdj.y * dd[q2]);
-                }
-            }
+            // p owns the pair when fluid and j moves, j when fluid and p moves
+            const bool op = pf && wj != 0.0f;
+            const bool oj = (UNIFORM || phj == PS_PHASE_FLUID) && ap.w != 0.0f;
+            if (op || oj) adj_pair_sum(k, dv, r2, pp.w + pj.w, op, oj, mcp, mcj, kvp, kvj, massp, massj, acc);
         }
         if (contacts) {
             // cloth contacts: p's own, then p as the partner of j's (for a
-            // non-fluid j, M holds its masked dL/dy)
+            // non-fluid j, MC holds its masked dL/dy)
+            const float dj[3] = {-dv[0], -dv[1], -dv[2]};
             float m[3];
             if (adj_contact(k, ap.w, ap.phase, wj, phj, ncp, dv, ydp, m))
                 for (int q2 = 0; q2 < 3; q2++) acc[q2] = acc[q2] + m[q2];
-            const float ydj[3] = {mj.x, mj.y, mj.z};
+            const float ydj[3] = {mcj.x, mcj.y, mcj.z};
             if (adj_contact(k, wj, phj, ap.w, ap.phase, ncj, dj, ydj, m))
                 for (int q2 = 0; q2 < 3; q2++) acc[q2] = acc[q2] - m[q2];
         }
     };
     const uint32_t c = b.ncnt[p];
+    // two records (48 or 64 bytes each) in flight per lane: four would hold
+    // the kernel to four wavefronts per SIMD
     if (UNIFORM) {
-        for_neighbors_strided_chunked(
-            b, p, c, hh, 4u, [&](uint32_t j) { return GPos{a.Pit[j], a.M[j], a.SD[j], a.CD[j]}; },
-            [&](uint32_t, const GPos &d) { body(d.p, d.m, d.sd, d.cd, ap.w, PS_PHASE_FLUID, 0u, massp); });
+        for_neighbors_strided_pairs(
+            b, p, c, hh, 4u, [&](uint32_t j) { return GPos{a.Pit[j], a.MC[j], a.KV[j]}; },
+            [&](uint32_t, const GPos &d) { body(d.p, d.mc, d.kv, ap.w, PS_PHASE_FLUID, 0u, massp); });
     } else {
-        for_neighbors_strided_chunked(
-            b, p, c, hh, 4u, [&](uint32_t j) { return GPosAttr{a.Pit[j], a.M[j], a.SD[j], a.CD[j], a.AB[j]}; },
-            [&](uint32_t, const GPosAttr &d) {
-                body(d.p, d.m, d.sd, d.cd, d.a.w, d.a.phase, d.a.nc, d.a.mass);
-            });
+        for_neighbors_strided_pairs(
+            b, p, c, hh, 4u, [&](uint32_t j) { return GPosAttr{a.Pit[j], a.MC[j], a.KV[j], a.AB[j]}; },
+            [&](uint32_t, const GPosAttr &d) { body(d.p, d.mc, d.kv, d.a.w, d.a.phase, d.a.nc, d.a.mass); });
     }
     {
         // lanes 0 and 1 form q0 + q2 and q1 + q3, lane 0 their sum
@@ -538,7 +469,7 @@ __global__ __launch_bounds__(kAdjBlock) void adj_xsph_k(DevBuffers b, KConst k, 
                                                      float dt)
 {
     // four lanes per particle: lane hh forms quarter hh of S, lane 0 finishes
-    const uint32_t i = blockIdx.x * (kAdjBlock / 4) + (threadIdx.x >> 2);
+    const uint32_t i = xcd_block() * (kAdjBlock / 4) + (threadIdx.x >> 2);
     const uint32_t hh = threadIdx.x & 3u;
     if (i >= n) return;  // the four lanes of a particle leave together
     const Attr ai = b.A[i];
@@ -596,7 +527,7 @@ template <bool UNIFORM>
 __global__ __launch_bounds__(kAdjBlock) void adj_finalize_k(DevBuffers b, KConst k, AdjBuffers a, uint32_t n,
                                                          float dt)
 {
-    const uint32_t p = blockIdx.x * (kAdjBlock / 4) + (threadIdx.x >> 2);
+    const uint32_t p = xcd_block() * (kAdjBlock / 4) + (threadIdx.x >> 2);
     const uint32_t hh = threadIdx.x & 3u;
     if (p >= n) return;  // the four lanes of a particle leave together
     const Attr ap = b.A[p];

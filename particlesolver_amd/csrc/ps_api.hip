@@ -145,7 +145,7 @@ struct ps_system {
         float4 *pb = nullptr, *pb2 = nullptr, *Y = nullptr, *M = nullptr, *xsb = nullptr;
         ps::AdjAttr *AB = nullptr;
         float4 *gx = nullptr, *gv = nullptr;
-        float2 *CD = nullptr;
+        float4 *MC = nullptr, *KV = nullptr;
         uint32_t *nc = nullptr;
         float *r0b = nullptr, *wkb = nullptr, *drho = nullptr, *dwk = nullptr, *sb = nullptr;
     } adj;
@@ -381,7 +381,7 @@ void adj_free(ps_system *s)
     dev_free(w.tN); dev_free(w.tC); dev_free(w.tS); dev_free(w.tX); dev_free(w.tP); dev_free(w.tPit);
     dev_free(w.tW); dev_free(w.tA); dev_free(w.tClamp); dev_free(w.tLQ); dev_free(w.tF); dev_free(w.tRho);
     dev_free(w.pb); dev_free(w.pb2); dev_free(w.Y); dev_free(w.M); dev_free(w.AB);
-    dev_free(w.xsb); dev_free(w.gx); dev_free(w.gv); dev_free(w.CD); dev_free(w.nc); dev_free(w.r0b);
+    dev_free(w.xsb); dev_free(w.gx); dev_free(w.gv); dev_free(w.MC); dev_free(w.KV); dev_free(w.nc); dev_free(w.r0b);
     dev_free(w.wkb); dev_free(w.drho); dev_free(w.dwk); dev_free(w.sb);
     w.cap = w.cks = w.tslots = w.iters = w.ndist = w.tab = 0;
 }
@@ -1821,7 +1821,7 @@ int adj_reserve(ps_system *s, size_t cks, size_t tslots, size_t iters)
         (rc = dev_alloc(s, &w.pb, c)) || (rc = dev_alloc(s, &w.pb2, c)) || (rc = dev_alloc(s, &w.Y, c)) ||
         (rc = dev_alloc(s, &w.M, c)) || (rc = dev_alloc(s, &w.AB, c)) ||
         (rc = dev_alloc(s, &w.xsb, c)) || (rc = dev_alloc(s, &w.gx, c)) || (rc = dev_alloc(s, &w.gv, c)) ||
-        (rc = dev_alloc(s, &w.CD, c)) || (rc = dev_alloc(s, &w.nc, c)) || (rc = dev_alloc(s, &w.r0b, c)) ||
+        (rc = dev_alloc(s, &w.MC, c)) || (rc = dev_alloc(s, &w.KV, c)) || (rc = dev_alloc(s, &w.nc, c)) || (rc = dev_alloc(s, &w.r0b, c)) ||
         (rc = dev_alloc(s, &w.wkb, c)) || (rc = dev_alloc(s, &w.drho, c)) || (rc = dev_alloc(s, &w.dwk, c)) ||
         (rc = dev_alloc(s, &w.sb, ndist))) {
         adj_free(s);
@@ -1951,7 +1951,8 @@ int adj_reverse(ps_system *s, float dt, size_t t, hipStream_t st)
     a.Y = w.Y;
     a.M = w.M;
     a.AB = w.AB;
-    a.CD = w.CD;
+    a.MC = w.MC;
+    a.KV = w.KV;
     a.nc = w.nc;
     a.r0b = w.r0b;
     a.wkb = w.wkb;

@@ -204,6 +204,38 @@ __device__ __forceinline__ void for_neighbors_strided_chunked(const DevBuffers &
     }
 }
 
+// As for_neighbors_strided_chunked with two records in flight instead of
+// four: the gathers of a chunk's entries 2 and 3 go out once entries 0 and
+// 1 are done.  For records so large (tens of bytes) that four of them in
+// registers would cost a wavefront per SIMD.
+template <typename G, typename Fn>
+__device__ __forceinline__ void for_neighbors_strided_pairs(const DevBuffers &b, uint32_t i, uint32_t c,
+                                                            uint32_t first, uint32_t stride, G gather,
+                                                            Fn fn)
+{
+    const uint32_t nch = (c + 3u) >> 2;  // chunks in use
+    if (first >= nch) return;
+    const uint4 *nb = nbr_lane(b, i);
+    uint4 jj = nb[(size_t)first * 64];
+    for (uint32_t m = first; m < nch; m += stride) {
+        const uint4 jn = nb[(size_t)(m + stride < nch ? m + stride : m) * 64];
+        const uint32_t q = m << 2;
+        {
+            const auto d0 = gather(jj.x);
+            const auto d1 = gather(jj.y);
+            fn(jj.x, d0);
+            if (q + 1 < c) fn(jj.y, d1);
+        }
+        if (q + 2 < c) {
+            const auto d2 = gather(jj.z);
+            const auto d3 = gather(jj.w);
+            fn(jj.z, d2);
+            if (q + 3 < c) fn(jj.w, d3);
+        }
+        jj = jn;
+    }
+}
+
 // The entries of half h of particle i's first c neighbours in table order,
 // one at a time: for the rarely taken recomputation of one half.
 template <typename G, typename Fn>

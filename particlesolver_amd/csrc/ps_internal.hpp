@@ -222,7 +222,8 @@ struct AdjBuffers {
     float4 *Y;             // masked dL/dy (xyz); finalize: dL/dS
     float4 *M;             // mu = dL/dy / rho0 (fluid) or dL/dy (xyz), 1/rho0 (w)
     AdjAttr *AB;           // attributes with contact counts (non-uniform scenes)
-    float2 *CD;            // dL/dC, dL/dD
+    float4 *MC;            // mu as in M (xyz), Cr = dL/dC / rho0 (w); Cr = 0 if not fluid
+    float4 *KV;            // V = dL/dD sk S (xyz), u = dL/dD sk^2 (w), sk = spiky / rho0
     uint32_t *nc;          // contact count
     float *r0b, *wkb;      // dL/drho0 and dL/dwall_k of this substep
     float4 *xsb;           // dL/dx* of this substep

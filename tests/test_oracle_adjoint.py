@@ -22,6 +22,7 @@ over fluid (distance constraints with per-constraint stiffness, a pin,
 fluid-rope density coupling), cloth on the floor beside fluid (cloth-cloth
 and cloth-fluid contacts, kinetic wall friction, constraints, a pin) and a
 2D fluid (z held)."""
+import ctypes
 import math
 
 import numpy as np
@@ -191,3 +192,54 @@ def test_rejects_what_it_does_not_cover():
     o = load_scene(sc, "f64")
     with pytest.raises(RuntimeError):
         o.step_backward(sc.dt, 1, None, None)           # sleeping
+
+
+def _pair_check(lib, prec, prm, q, v, op, oj):
+    dt = np.float64 if prec == "f64" else np.float32
+    f = getattr(lib, "orc_adj_pair_check_" + prec)
+    f.restype = None
+    qa, va = np.ascontiguousarray(q, dt), np.ascontiguousarray(v, dt)
+    own, tot = np.zeros(3, dt), np.zeros(3, dt)
+    f(ctypes.byref(prm), qa.ctypes.data_as(ctypes.c_void_p), va.ctypes.data_as(ctypes.c_void_p),
+      ctypes.c_int(op), ctypes.c_int(oj), own.ctypes.data_as(ctypes.c_void_p),
+      tot.ctypes.data_as(ctypes.c_void_p))
+    return own, tot
+
+
+@pytest.mark.parametrize("n_corr", [1, 4])
+def test_pair_terms_folded_equal_owner_by_owner(n_corr):
+    """The reverse pass forms a pair's fluid terms in dL/dp_p (both owners'
+    E, density and denominator terms) as one folded expression
+    (adj_pair_sum).  In FP64 it must equal the owner-by-owner form (each
+    owner's adj_pair, adj_grad_w6 and adj_dD, with d and -d) to rounding:
+    over random pairs inside the kernel radius, with either owner alone and
+    both, and for degenerate pairs.  A sign, a factor or an owner swapped in
+    the folding shows up at O(1) relative error."""
+    from oracle import _lib
+    sc = scenes.dam_break(4, 4, 4)
+    prm = sc.params
+    prm.n_corr = n_corr
+    h = prm.h
+    lib = _lib("f64")
+    rng = np.random.default_rng(11 + n_corr)
+    worst = 0.0
+    for trial in range(400):
+        degenerate = trial % 40 == 0
+        u = rng.standard_normal(3)
+        u /= np.linalg.norm(u)
+        d = u * (rng.uniform(0.0, 5e-5) if degenerate else rng.uniform(0.05, 0.999) * h)
+        lam = rng.standard_normal(2) * 1e-2
+        cb = rng.standard_normal(2) * 1e2
+        db = rng.standard_normal(2) * 1e-3
+        ir0 = 1.0 / rng.uniform(500.0, 1500.0, 2)
+        mass = rng.uniform(0.5, 2.0, 2)
+        q = [lam[0], lam[1], cb[0], cb[1], db[0], db[1], ir0[0], ir0[1], mass[0], mass[1]]
+        v = np.concatenate([d, rng.standard_normal(3) * 1e-3, rng.standard_normal(3) * 1e-3,
+                            rng.standard_normal(3) * 10.0, rng.standard_normal(3) * 10.0])
+        for op, oj in ((1, 1), (1, 0), (0, 1)):
+            own, tot = _pair_check(lib, "f64", prm, q, v, op, oj)
+            scale = max(np.abs(own).max(), 1e-300)
+            err = np.abs(own - tot).max() / scale
+            worst = max(worst, err)
+            assert err < 1e-11, (trial, op, oj, own, tot)
+    assert worst > 0.0  # the two forms round differently, so this is no self-comparison
