@@ -811,7 +811,7 @@ def adjoint_measure(args, cloth=(128, 128), fluid=(64, 16, 64), T=60, every=10, 
     with System(sc.params) as g:
         sc.load_into(g)
         g.timing([k for k in ("lambda", "solve") + tuple(ADJ_ALGO_BYTES) +
-                  ("adj_xsph", "adj_finalize", "adj_stiffness", "adj_predict")])
+                  ("adj_xsph", "adj_finalize", "adj_predict")])
         g.timing_reset()
         g.step_backward(sc.dt, T, gx, None, checkpoint_every=every)
         for name, (ms, launches) in g.timing_read().items():

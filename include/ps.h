@@ -500,7 +500,7 @@ typedef enum ps_kernel_id {
     PS_K_ADJ_PRE,
     PS_K_ADJ_LAMBDA,
     PS_K_ADJ_POSITION,
-    PS_K_ADJ_STIFFNESS,
+    PS_K_ADJ_STIFFNESS, /* no launches: adj_position forms dL/dstiffness */
     PS_K_ADJ_PREDICT,
     /* insertion-order export (ps_export_device, trajectory files) */
     PS_K_EXPORT,

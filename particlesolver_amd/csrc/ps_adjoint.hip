@@ -21,7 +21,7 @@
 // Kernels per reversed iteration: adj_pre_k (walls, pins and wall friction
 // backwards, contact counts, rho/S/D, f), adj_lambda_k (dL/dl -> dL/dC,
 // dL/dD), adj_position_k (dL/dp: pair, density, D, contact and distance
-// terms), adj_stiffness_k (per constraint).  Per substep: adj_xsph_k and
+// terms, and dL/dstiffness per constraint).  Per substep: adj_xsph_k and
 // adj_finalize_k (velocity + XSPH backwards) before the iterations,
 // adj_predict_k after them.
 
@@ -397,65 +397,75 @@ __global__ __launch_bounds__(kAdjBlock) void adj_position_k(DevBuffers b, KConst
         for (int q2 = 0; q2 < 3; q2++) acc[q2] = acc[q2] + __shfl_xor(acc[q2], 2, 64);
         float lo[3];
         for (int q2 = 0; q2 < 3; q2++) lo[q2] = __shfl_xor(acc[q2], 1, 64);
-        if (hh) return;
-        for (int q2 = 0; q2 < 3; q2++) acc[q2] = acc[q2] + lo[q2];
+        for (int q2 = 0; q2 < 3; q2++) acc[q2] = acc[q2] + lo[q2];  // used in lane 0
     }
-    // distance constraints incident to p
+    // distance constraints incident to p, in their order; for a constraint
+    // whose first particle p is, also its dL/dstiffness (one writer per
+    // constraint): T_a / st = -(w_a / (wsum occ_a)) (1 - L/r) d_ab, and T_b
+    // its mirror.  The particle's four lanes take four constraints at a
+    // time (each a chain of dependent loads); lane 0 adds their terms in
+    // constraint order.
     if (a.dist) {
         const uint32_t o = ap.orig;
-        for (uint32_t e = b.dist_off[o]; e < b.dist_off[o + 1]; e++) {
-            const uint32_t cid = b.dist_adj[e];
-            const uint2 pr = b.dist_pairs[cid];
-            const uint32_t j = b.slot_of[pr.x == o ? pr.y : pr.x];
-            const float4 pj = a.Pit[j];
-            const float dv[3] = {pp.x - pj.x, pp.y - pj.y, pp.z - pj.z};
-            const float dist = sqrtf(((dv[0] * dv[0]) + (dv[1] * dv[1])) + (dv[2] * dv[2]));
-            const Attr aj = b.A[j];
-            const float wsum = ap.w + aj.w;
-            if (dist < 1e-6f || wsum == 0.0f) continue;
-            const float2 rs = b.dist_rs[cid];
-            const float L = rs.x, st = rs.y;
-            const float k1 = 1.0f - (L / dist), k2 = L / ((dist * dist) * dist);
-            // M(d) = -(w st / (wsum occ)) [(1 - L/r) I + (L/r^3) d d^T]
-            if (ap.w != 0.0f) {
-                const float cf = -(((ap.w * st) / wsum) / (float)b.occ[o]);
-                const float dy = dot3(dv, ydp);
-                for (int q2 = 0; q2 < 3; q2++) acc[q2] = acc[q2] + (cf * ((k1 * ydp[q2]) + ((k2 * dy) * dv[q2])));
+        const uint32_t e0 = b.dist_off[o], e1 = b.dist_off[o + 1];
+        const uint32_t quad = threadIdx.x & ~3u;
+        for (uint32_t base = e0; base < e1; base += 4u) {  // the same trip count in the four lanes
+            const uint32_t e = base + hh;
+            // this lane's constraint: the p term X and the j term Z
+            float X[3] = {0.0f, 0.0f, 0.0f}, Z[3] = {0.0f, 0.0f, 0.0f};
+            uint32_t fl = 0;  // bit 0: X is added, bit 1: Z is subtracted
+            if (e < e1) {
+                const uint32_t cid = b.dist_adj[e];
+                const uint2 pr = b.dist_pairs[cid];
+                const uint32_t j = b.slot_of[pr.x == o ? pr.y : pr.x];
+                const float4 pj = a.Pit[j];
+                const float dv[3] = {pp.x - pj.x, pp.y - pj.y, pp.z - pj.z};
+                const float dist = sqrtf(((dv[0] * dv[0]) + (dv[1] * dv[1])) + (dv[2] * dv[2]));
+                const Attr aj = b.A[j];
+                const float wsum = ap.w + aj.w;
+                if (!(dist < 1e-6f || wsum == 0.0f)) {
+                    const float2 rs = b.dist_rs[cid];
+                    const float L = rs.x, st = rs.y;
+                    const float k1 = 1.0f - (L / dist), k2 = L / ((dist * dist) * dist);
+                    const bool first = pr.x == o;  // d = p_a - p_b
+                    float gs = 0.0f;
+                    // M(d) = -(w st / (wsum occ)) [(1 - L/r) I + (L/r^3) d d^T]
+                    if (ap.w != 0.0f) {
+                        const float cf = -(((ap.w * st) / wsum) / (float)b.occ[o]);
+                        const float dy = dot3(dv, ydp);
+                        for (int q2 = 0; q2 < 3; q2++) X[q2] = cf * ((k1 * ydp[q2]) + ((k2 * dy) * dv[q2]));
+                        fl |= 1u;
+                        if (first) gs = gs - (((ap.w / wsum) / (float)b.occ[o]) * (k1 * dy));
+                    }
+                    if (aj.w != 0.0f) {
+                        float ydj[3];
+                        yd_of(k, a.Y[j], ydj);
+                        const float cf = -(((aj.w * st) / wsum) / (float)b.occ[aj.orig]);
+                        const float dy = dot3(dv, ydj);
+                        for (int q2 = 0; q2 < 3; q2++) Z[q2] = cf * ((k1 * ydj[q2]) + ((k2 * dy) * dv[q2]));
+                        fl |= 2u;
+                        if (first) gs = gs + (((aj.w / wsum) / (float)b.occ[aj.orig]) * (k1 * dy));
+                    }
+                    if (first) a.sb[cid] = a.sb[cid] + gs;
+                }
             }
-            if (aj.w != 0.0f) {
-                float ydj[3];
-                yd_of(k, a.Y[j], ydj);
-                const float cf = -(((aj.w * st) / wsum) / (float)b.occ[aj.orig]);
-                const float dy = dot3(dv, ydj);
-                for (int q2 = 0; q2 < 3; q2++) acc[q2] = acc[q2] - (cf * ((k1 * ydj[q2]) + ((k2 * dy) * dv[q2])));
+            for (uint32_t l = 0; l < 4u; l++) {
+                const int src = (int)(quad + l);
+                const uint32_t fll = (uint32_t)__shfl((int)fl, src, 64);
+                float Xl[3], Zl[3];
+                for (int q2 = 0; q2 < 3; q2++) {
+                    Xl[q2] = __shfl(X[q2], src, 64);
+                    Zl[q2] = __shfl(Z[q2], src, 64);
+                }
+                if (fll & 1u)
+                    for (int q2 = 0; q2 < 3; q2++) acc[q2] = acc[q2] + Xl[q2];
+                if (fll & 2u)
+                    for (int q2 = 0; q2 < 3; q2++) acc[q2] = acc[q2] - Zl[q2];
             }
         }
     }
+    if (hh) return;
     a.pb[p] = make_float4(acc[0], acc[1], acc[2], 0.0f);
-}
-
-// dL/dstiffness: Delta_a / st = -(w_a / (wsum occ_a)) (1 - L/r) d_ab
-__global__ __launch_bounds__(kAdjBlock) void adj_stiffness_k(DevBuffers b, KConst k, AdjBuffers a, uint32_t m)
-{
-    const uint32_t c = blockIdx.x * kAdjBlock + threadIdx.x;
-    if (c >= m) return;
-    const uint2 pr = b.dist_pairs[c];
-    const uint32_t ia = b.slot_of[pr.x], ib = b.slot_of[pr.y];
-    const float4 pa = a.Pit[ia], pb = a.Pit[ib];
-    const float dv[3] = {pa.x - pb.x, pa.y - pb.y, pa.z - pb.z};
-    const float dist = sqrtf(((dv[0] * dv[0]) + (dv[1] * dv[1])) + (dv[2] * dv[2]));
-    const float wa = b.A[ia].w, wb = b.A[ib].w, wsum = wa + wb;
-    if (dist < 1e-6f || wsum == 0.0f) return;
-    const float k1 = 1.0f - (b.dist_rs[c].x / dist);
-    float yda[3], ydb[3];
-    yd_of(k, a.Y[ia], yda);
-    yd_of(k, a.Y[ib], ydb);
-    const float ya = dot3(dv, yda);
-    const float ybv = dot3(dv, ydb);
-    float g = 0.0f;
-    if (wa != 0.0f) g = g - (((wa / wsum) / (float)b.occ[pr.x]) * (k1 * ya));
-    if (wb != 0.0f) g = g + (((wb / wsum) / (float)b.occ[pr.y]) * (k1 * ybv));
-    a.sb[c] = a.sb[c] + g;
 }
 
 // ---------------------------------------------------------------------------
@@ -645,7 +655,6 @@ hipError_t launch_adj(hipStream_t st, AdjKernel which, const DevBuffers &b, cons
     case ADJ_PRE: PS_ADJ_LAUNCH(adj_pre_k, b, k, a, count); break;
     case ADJ_LAMBDA: PS_ADJ_LAUNCH4(adj_lambda_k, b, k, a, count); break;
     case ADJ_POSITION: PS_ADJ_LAUNCH4(adj_position_k, b, k, a, count); break;
-    case ADJ_STIFFNESS: PS_LAUNCH(adj_stiffness_k, grid, block, 0, st, b, k, a, count); break;
     case ADJ_PREDICT: PS_LAUNCH(adj_predict_k, grid, block, 0, st, b, k, a, count, dt); break;
     default: return hipErrorInvalidValue;
     }

@@ -1966,7 +1966,7 @@ int adj_reverse(ps_system *s, float dt, size_t t, hipStream_t st)
     a.dist = s->dist_rest.empty() ? 0u : 1u;
     float4 *cur = w.pb, *nxt = w.pb2;
     a.pb = cur;
-    const uint32_t n = s->n, ndist = (uint32_t)s->dist_rest.size();
+    const uint32_t n = s->n;
     const bool uni = s->uniform;
     s->tstream = st;  // LAUNCH times these launches on st
     struct Restore {
@@ -1988,9 +1988,6 @@ int adj_reverse(ps_system *s, float dt, size_t t, hipStream_t st)
         LAUNCH(s, PS_K_ADJ_PRE, ps::launch_adj(st, ps::ADJ_PRE, bt, s->kc, a, n, dt, uni));
         LAUNCH(s, PS_K_ADJ_LAMBDA, ps::launch_adj(st, ps::ADJ_LAMBDA, bt, s->kc, a, n, dt, uni));
         LAUNCH(s, PS_K_ADJ_POSITION, ps::launch_adj(st, ps::ADJ_POSITION, bt, s->kc, a, n, dt, uni));
-        if (ndist)
-            LAUNCH(s, PS_K_ADJ_STIFFNESS,
-                   ps::launch_adj(st, ps::ADJ_STIFFNESS, bt, s->kc, a, ndist, dt, uni));
         std::swap(cur, nxt);
     }
     a.pb = cur;
@@ -2003,12 +2000,13 @@ int step_backward_impl(ps_system *s, float dt, uint32_t S, uint32_t K, const flo
                        float *dl_dstiffness, float *dl_dmu_k)
 {
     const uint32_t n = s->n;
-    // segments of K substeps (K = 0: one segment, the whole horizon); the
-    // last is taped by the forward pass, the others recomputed from their
-    // checkpoints and taped when the reverse pass reaches them.  With more
-    // than one segment the tape holds two: segment g is taped into buffer
-    // g % 2, so the recompute of segment g - 1 (on `stream`) runs while the
-    // reverse of segment g (on `rstream`) reads the other buffer
+    // segments of K substeps (K = 0: one segment, the whole horizon).  With
+    // more than one segment the tape holds two: segment g is taped into
+    // buffer g % 2.  The forward pass tapes the last two segments (one
+    // buffer each); the others are recomputed from their checkpoints when
+    // the reverse pass reaches them, segment g - 2 (on `stream`) into the
+    // buffer the reverse of segment g (on `rstream`) has just finished
+    // with, while the reverse of segment g - 1 reads the other
     const uint32_t seg = K ? K : S;
     const uint32_t nseg = (S + seg - 1) / seg;
     const size_t fin = nseg;  // checkpoint of the final state
@@ -2036,7 +2034,7 @@ int step_backward_impl(ps_system *s, float dt, uint32_t S, uint32_t K, const flo
     };
     for (uint32_t g = 0; g < nseg; g++) {
         if ((rc = save(g))) return rc;
-        if (g + 1 < nseg) {
+        if (g + nbuf < nseg) {
             for (uint32_t t = g * seg; t < std::min(S, (g + 1) * seg); t++)
                 if ((rc = adj_substep(s, dt))) return rc;
         } else if ((rc = tape_segment(g))) {
@@ -2085,8 +2083,8 @@ int step_backward_impl(ps_system *s, float dt, uint32_t S, uint32_t K, const flo
         HIP_TRY(s, hipEventRecord(tape_done.e[g], s->stream));
         return (int)PS_OK;
     };
-    HIP_TRY(s, hipEventRecord(tape_done.e[nseg - 1], s->stream));
-    if (nseg >= 2 && (rc = recompute(nseg - 2))) return rc;
+    // the forward pass has taped the last nbuf segments
+    for (uint32_t g = nseg - (uint32_t)nbuf; g < nseg; g++) HIP_TRY(s, hipEventRecord(tape_done.e[g], s->stream));
     for (uint32_t g = nseg; g-- > 0;) {
         const uint32_t s0 = g * seg, s1 = std::min(S, s0 + seg);
         HIP_TRY(s, hipStreamWaitEvent(s->rstream, tape_done.e[g], 0));

@@ -291,16 +291,15 @@ int fail_stateless(int code, const std::string &msg);
 // Reverse pass (ps_adjoint.hip), one kernel per call: velocity + XSPH
 // backwards (ADJ_XSPH then ADJ_FINALIZE; b.P the final iterate, b.X the
 // substep start), one iteration backwards (ADJ_PRE, ADJ_LAMBDA,
-// ADJ_POSITION, then ADJ_STIFFNESS over count = the distance constraints;
+// ADJ_POSITION, which also adds dL/dstiffness per distance constraint;
 // a.Pit, a.clamp, a.wrec of that iteration, a.pbn -> a.pb), predict
-// backwards (ADJ_PREDICT).  count: particles, or constraints.
+// backwards (ADJ_PREDICT).  count: particles.
 enum AdjKernel : uint32_t {
     ADJ_XSPH,
     ADJ_FINALIZE,
     ADJ_PRE,
     ADJ_LAMBDA,
     ADJ_POSITION,
-    ADJ_STIFFNESS,
     ADJ_PREDICT,
 };
 // uniform: every particle fluid with one inverse mass (no attribute gathers)
