@@ -277,15 +277,22 @@ __global__ __launch_bounds__(kAdjBlock) void adj_lambda_k(DevBuffers b, KConst k
     const uint32_t i = xcd_block() * (kAdjBlock / 4) + (threadIdx.x >> 2);
     const uint32_t hh = threadIdx.x & 3u;
     if (i >= n) return;  // the four lanes of a particle leave together
+    // every per-particle load goes out before the first wait (lane 0's
+    // finish would otherwise wait on three more after the sum)
     const float4 mi = a.M[i];
-    if (!UNIFORM && b.A[i].phase != PS_PHASE_FLUID) {
+    const int32_t phi = UNIFORM ? (int32_t)PS_PHASE_FLUID : b.A[i].phase;
+    const float4 pi = a.Pit[i];
+    const uint32_t c = b.ncnt[i];
+    const float4 sd = a.SD[i];
+    const float rhoi = a.rho[i];
+    float r = a.r0b[i];
+    if (!UNIFORM && phi != PS_PHASE_FLUID) {
         if (!hh) {
             a.MC[i] = make_float4(mi.x, mi.y, mi.z, 0.0f);
             a.KV[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         }
         return;
     }
-    const float4 pi = a.Pit[i];
     const float mu[3] = {mi.x, mi.y, mi.z};
     const float ir0 = mi.w;
     float lb = 0.0f;
@@ -306,7 +313,6 @@ __global__ __launch_bounds__(kAdjBlock) void adj_lambda_k(DevBuffers b, KConst k
             lb = lb + dot3(muj, gj);
         }
     };
-    const uint32_t c = b.ncnt[i];
     if (UNIFORM) {
         for_neighbors_strided_chunked(
             b, i, c, hh, 4u, [&](uint32_t j) { return GLam{a.Pit[j], a.M[j]}; },
@@ -320,13 +326,11 @@ __global__ __launch_bounds__(kAdjBlock) void adj_lambda_k(DevBuffers b, KConst k
     const float lo = __shfl_xor(lb, 1, 64);
     if (hh) return;
     lb = lb + lo;
-    const float4 sd = a.SD[i];
     const float den = sd.w + k.relax;
     const float lam = pi.w;
     const float Cb = -(lb / den);
     const float Db = -((lb * lam) / den);
-    float r = a.r0b[i];
-    r = r - (Cb * (a.rho[i] * (ir0 * ir0)));
+    r = r - (Cb * (rhoi * (ir0 * ir0)));
     r = r - (Db * ((2.0f * sd.w) * ir0));
     a.r0b[i] = r;
     // the factors adj_position_k combines per pair
@@ -482,11 +486,13 @@ __global__ __launch_bounds__(kAdjBlock) void adj_xsph_k(DevBuffers b, KConst k, 
     const uint32_t i = xcd_block() * (kAdjBlock / 4) + (threadIdx.x >> 2);
     const uint32_t hh = threadIdx.x & 3u;
     if (i >= n) return;  // the four lanes of a particle leave together
+    // the particle's own loads go out together
     const Attr ai = b.A[i];
+    const float4 pi = b.P[i], xi = b.X[i];
+    const uint32_t c = b.ncnt[i];
     float r0 = 0.0f;
     float Sh[3] = {0.0f, 0.0f, 0.0f};
     if (k.xsph_c != 0.0f && (UNIFORM || ai.phase == PS_PHASE_FLUID)) {
-        const float4 pi = b.P[i], xi = b.X[i];
         const float ui[3] = {pi.x - xi.x, pi.y - xi.y, pi.z - xi.z};
         const float4 vb = a.gv[ai.orig];
         const float vbn[3] = {vb.x, vb.y, vb.z};
@@ -502,7 +508,6 @@ __global__ __launch_bounds__(kAdjBlock) void adj_xsph_k(DevBuffers b, KConst k, 
             Sm[1] = __builtin_fmaf((pj.y - xj.y) - ui[1], m, Sm[1]);
             Sm[2] = __builtin_fmaf((pj.z - xj.z) - ui[2], m, Sm[2]);
         };
-        const uint32_t c = b.ncnt[i];
         if (UNIFORM) {
             for_neighbors_strided_chunked(
                 b, i, c, hh, 4u, [&](uint32_t j) { return PX{b.P[j], b.X[j]}; },
@@ -540,16 +545,18 @@ __global__ __launch_bounds__(kAdjBlock) void adj_finalize_k(DevBuffers b, KConst
     const uint32_t p = xcd_block() * (kAdjBlock / 4) + (threadIdx.x >> 2);
     const uint32_t hh = threadIdx.x & 3u;
     if (p >= n) return;  // the four lanes of a particle leave together
+    // the particle's own loads go out together
     const Attr ap = b.A[p];
+    const float4 pp = b.P[p], xp = b.X[p];
+    const float4 shp4 = a.Y[p];
+    const uint32_t c = b.ncnt[p];
     const float4 vb = a.gv[ap.orig], xb = a.gx[ap.orig];
     float ub[3] = {vb.x / dt, vb.y / dt, vb.z / dt};
     float acc[3] = {xb.x, xb.y, xb.z};
     if (hh)
         for (int q = 0; q < 3; q++) ub[q] = acc[q] = 0.0f;
     if (k.xsph_c != 0.0f && (UNIFORM || ap.phase == PS_PHASE_FLUID)) {
-        const float4 pp = b.P[p], xp = b.X[p];
         const float up[3] = {pp.x - xp.x, pp.y - xp.y, pp.z - xp.z};
-        const float4 shp4 = a.Y[p];
         const float Shp[3] = {shp4.x, shp4.y, shp4.z};
         const float mp = 1.0f / ap.w;
         auto body = [&](const float4 &pj, const float4 &xj, const float4 &shj4, float wj) {
@@ -573,7 +580,6 @@ __global__ __launch_bounds__(kAdjBlock) void adj_finalize_k(DevBuffers b, KConst
                 acc[q2] = (acc[q2] + (wb * g6[q2])) + (wb2 * g6[q2]);
             }
         };
-        const uint32_t c = b.ncnt[p];
         if (UNIFORM) {
             for_neighbors_strided_chunked(
                 b, p, c, hh, 4u, [&](uint32_t j) { return GXY{b.P[j], b.X[j], a.Y[j]}; },
