@@ -747,11 +747,12 @@ ADJ_ALGO_BYTES = {
     # + attribute record out 16
     "adj_pre": lambda K: 105.0,
     # attributes 16 + iterate 16 + mu 16 + recorded S,D 16 + recorded rho 4
-    # + count 4 + K indices + rho0 term r/w 8 + dL/dC,dL/dD out 8
-    "adj_lambda": lambda K: 88.0 + 4.0 * K,
-    # iterate 16 + attributes 16 + dL/dy 16 + mu 16 + dL/dC,dD 8 + S,D 16
-    # + count 4 + contact count 4 + K indices + dL/dp out 16
-    "adj_position": lambda K: 112.0 + 4.0 * K,
+    # + count 4 + K indices + rho0 term r/w 8 + the pair factors out
+    # (mu,Cr 16 and V,u 16)
+    "adj_lambda": lambda K: 112.0 + 4.0 * K,
+    # iterate 16 + attributes 16 + dL/dy 16 + pair factors 32 + count 4
+    # + contact count 4 + K indices + dL/dp out 16
+    "adj_position": lambda K: 104.0 + 4.0 * K,
 }
 
 
