@@ -1157,30 +1157,53 @@ __global__ __launch_bounds__(kMaxBlock) void solve_k(DevBuffers b, KConst k, uin
         }
     }
 
-    // staged: the pair's lane 0 finishes the particle
-    if (STAGED && hh) return;
-
     // a9: distance constraints, gathered over this particle's incident list
+    // in order.  Each is a chain of dependent loads (index, pair, slot,
+    // position, attributes); staged, the particle's two lanes take two at a
+    // time and lane 0 subtracts both terms in constraint order
     if (F & F_DIST) {
         const uint32_t o = a.orig;
         const uint32_t a0 = b.dist_off[o], a1 = b.dist_off[o + 1];
         if (a1 > a0 && wi != 0.0f) {
             float cx = 0.0f, cy = 0.0f, cz = 0.0f;
-            for (uint32_t q = a0; q < a1; q++) {
-                const uint32_t cid = b.dist_adj[q];
-                const uint2 pr = b.dist_pairs[cid];
-                const uint32_t other = pr.x == o ? pr.y : pr.x;
-                const uint32_t j = b.slot_of[other];
-                const float4 pj = b.P[j];
-                float dx = px - pj.x, dy = py - pj.y, dz = pz - pj.z;
-                float dist = sqrtf(((dx * dx) + (dy * dy)) + (dz * dz));
-                float wsum = wi + b.A[j].w;
-                if (dist < 1e-6f || wsum == 0.0f) continue;
-                const float2 rs = b.dist_rs[cid];
-                float sc = ((((dist - rs.x) / wsum) / dist) * wi) * rs.y;
-                cx = cx - (sc * dx);
-                cy = cy - (sc * dy);
-                cz = cz - (sc * dz);
+            constexpr uint32_t kStep = STAGED ? 2u : 1u;
+            for (uint32_t q0 = a0; q0 < a1; q0 += kStep) {  // the same trip count in both lanes
+                const uint32_t q = q0 + (STAGED ? hh : 0u);
+                float tx = 0.0f, ty = 0.0f, tz = 0.0f;
+                bool use = false;
+                if (q < a1) {
+                    const uint32_t cid = b.dist_adj[q];
+                    const uint2 pr = b.dist_pairs[cid];
+                    const uint32_t other = pr.x == o ? pr.y : pr.x;
+                    const uint32_t j = b.slot_of[other];
+                    const float4 pj = b.P[j];
+                    float dx = px - pj.x, dy = py - pj.y, dz = pz - pj.z;
+                    float dist = sqrtf(((dx * dx) + (dy * dy)) + (dz * dz));
+                    float wsum = wi + b.A[j].w;
+                    if (!(dist < 1e-6f || wsum == 0.0f)) {
+                        const float2 rs = b.dist_rs[cid];
+                        float sc = ((((dist - rs.x) / wsum) / dist) * wi) * rs.y;
+                        tx = sc * dx;
+                        ty = sc * dy;
+                        tz = sc * dz;
+                        use = true;
+                    }
+                }
+                if (use) {
+                    cx = cx - tx;
+                    cy = cy - ty;
+                    cz = cz - tz;
+                }
+                if (STAGED) {
+                    // lane 0: then its partner's constraint, q0 + 1
+                    const float ox = __shfl_xor(tx, 1, 64), oy = __shfl_xor(ty, 1, 64),
+                                oz = __shfl_xor(tz, 1, 64);
+                    if (__shfl_xor(use ? 1 : 0, 1, 64)) {
+                        cx = cx - ox;
+                        cy = cy - oy;
+                        cz = cz - oz;
+                    }
+                }
             }
             const float occ = (float)b.occ[o];
             ax = ax + (cx / occ);
@@ -1188,6 +1211,9 @@ __global__ __launch_bounds__(kMaxBlock) void solve_k(DevBuffers b, KConst k, uin
             az = az + (cz / occ);
         }
     }
+
+    // staged: the pair's lane 0 finishes the particle
+    if (STAGED && hh) return;
 
     // a12: shape-matching goal pull
     if (F & F_SHAPE) {
