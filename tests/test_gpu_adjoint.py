@@ -55,6 +55,25 @@ def test_matches_fp32_oracle_bitexact(gpu_available, name):
         assert_bitexact(g.velocities(), ref.velocities(), "%s final velocities" % name)
 
 
+def test_full_workload_20_substeps_checkpointed_bitexact(gpu_available):
+    """The full adjoint workload (81,920 particles) over 20 substeps with
+    checkpoints every 6: four segments, the forward pass taping the last two
+    and the reverse pass recomputing the other two on the second stream.
+    Every gradient and the final state bit for bit against the FP32 oracle
+    (about 14 s of oracle time)."""
+    sc = scenes.cloth_over_pool()
+    gx, gv = cotangents(sc.n, seed=9)
+    ref = load_scene(sc, "f32")
+    r = ref.step_backward(sc.dt, 20, gx, gv, n_dist(sc), with_mu=True)
+    with System(sc.params) as g:
+        sc.load_into(g)
+        out = g.step_backward(sc.dt, 20, gx, gv, checkpoint_every=6)
+        for a, b, what in zip(out[:4], r[:4], ("dL/dx0", "dL/dv0", "dL/drho0", "dL/dstiffness")):
+            assert_bitexact(a, b, "full workload %s" % what)
+        assert np.float32(out[4]).view(np.uint32) == np.float32(r[4]).view(np.uint32), (out[4], r[4])
+        assert_bitexact(g.positions(), ref.positions(), "full workload final positions")
+
+
 def test_checkpoint_spacing_changes_no_bit(gpu_available):
     sc = scenes.cloth_and_fluid()
     gx, gv = cotangents(sc.n)
