@@ -391,7 +391,7 @@ void free_all(ps_system *s)
     DevBuffers &b = s->b;
     dev_free(b.X); dev_free(b.X2); dev_free(b.P); dev_free(b.Q); dev_free(b.V); dev_free(b.V2);
     dev_free(b.A); dev_free(b.A2); dev_free(b.key); dev_free(b.key2); dev_free(b.rank);
-    dev_free(b.tmp_orig); dev_free(b.slot_of); dev_free(b.mj); dev_free(b.cell_count); dev_free(b.cell_off);
+    dev_free(b.tmp_orig); dev_free(b.slot_of); dev_free(b.mj); dev_free(b.phs); dev_free(b.cell_count); dev_free(b.cell_off);
     dev_free(b.scan_part); dev_free(b.nbr); dev_free(b.ncnt); dev_free(b.dist_off);
     dev_free(b.dist_adj); dev_free(b.dist_pairs); dev_free(b.dist_rs); dev_free(b.occ);
     dev_free(b.pin_of); dev_free(b.pin_xyz); dev_free(b.body_off); dev_free(b.body_idx);
@@ -1033,7 +1033,7 @@ static int create_impl(const ps_params *params, uint32_t nscenes, const ps_scene
     } while (0)
     ALLOC(b.X, c); ALLOC(b.X2, c); ALLOC(b.P, c); ALLOC(b.Q, c); ALLOC(b.V, c); ALLOC(b.V2, c);
     ALLOC(b.A, c); ALLOC(b.A2, c); ALLOC(b.key, c); ALLOC(b.key2, c); ALLOC(b.rank, c);
-    ALLOC(b.tmp_orig, c); ALLOC(b.slot_of, c); ALLOC(b.mj, c);
+    ALLOC(b.tmp_orig, c); ALLOC(b.slot_of, c); ALLOC(b.mj, c); ALLOC(b.phs, c);
     ALLOC(b.cell_count, s->ncell); ALLOC(b.cell_off, (size_t)s->ncell + 1);
     ALLOC(b.scan_part, s->tiles);
     ALLOC(b.nbr, groups * 64 * s->kmax); ALLOC(b.ncnt, c);

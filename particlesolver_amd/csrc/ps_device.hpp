@@ -361,6 +361,19 @@ struct PM {
     float m;
 };
 
+// the same and the phase (a staged unit's contact records)
+struct PMH {
+    float4 p;
+    float m;
+    int32_t ph;
+};
+
+// a staged position and phase
+struct PH {
+    float4 p;
+    int32_t ph;
+};
+
 // entry q of particle i (single-entry access for the less travelled loops)
 __device__ __forceinline__ uint32_t nbr_at(const DevBuffers &b, uint32_t i, uint32_t q)
 {

@@ -138,6 +138,8 @@ struct DevBuffers {
                            // a uniform fluid: the particle's density mass as a
                            // neighbour, 1/w (times s_solid if not fluid), or -1
                            // for w = 0 (left out of the fluid sums)
+    int32_t *phs;          // per slot, written with mj: the phase, its sign bit
+                           // set for w = 0 (the staged solve's contact records)
     SceneDev *scenes;
     // grid
     uint32_t *cell_count;  // num_cells, zero between builds

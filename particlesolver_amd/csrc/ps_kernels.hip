@@ -299,6 +299,8 @@ __global__ __launch_bounds__(kMaxBlock) void reorder_k(DevBuffers b, uint32_t n,
             if (a.phase != PS_PHASE_FLUID) m = m * mj_solid;
         }
         b.mj[d] = m;
+        // the phase as the staged contact walk reads it
+        b.phs[d] = a.w != 0.0f ? a.phase : (int32_t)((uint32_t)a.phase | 0x80000000u);
     }
     if (b.sid) b.sid2[d] = b.sid[s];
     // a scatter by insertion index: skipped when no kernel reads it
@@ -826,9 +828,14 @@ __global__ __launch_bounds__(kMaxBlock) void solve_k(DevBuffers b, KConst k, uin
     // the unit's slots and the sentinel record
     // In a scene that is not a uniform fluid a particle with w = 0 is
     // staged as a position at infinity (the fluid sum leaves it out, as the
-    // global path does); the contact walk reads global memory, through the
-    // unit's chunk table (s_src) in a staged unit.
+    // global path does).  With contacts, in a unit where some particle
+    // walks its table for them, each slot's phase is staged as well (lds_ph,
+    // from the sort's phs: the sign bit set for a particle with w = 0), so
+    // the counting walk runs from LDS, reading the true position of such a
+    // particle from global memory; the corrections gather through the
+    // unit's chunk table (s_src).
     __shared__ float4 lds[STAGED ? kStageSlots + 1 : 1];
+    __shared__ int32_t lds_ph[STAGED && (F & F_CONTACT) ? kStageSlots + 1 : 1];
     __shared__ uint32_t s_src[STAGED && (F & F_CONTACT) ? kStageChunks : 1];
     // the contact lists of the block (global-slot variants with contacts):
     // kContactList entries per thread, entry q of thread t at q blockDim + t
@@ -840,13 +847,31 @@ __global__ __launch_bounds__(kMaxBlock) void solve_k(DevBuffers b, KConst k, uin
     if (STAGED) {
         const StageDesc *d = b.stage + unit;
         staged = d->staged != 0u;
+        // with contacts: whether a particle of the unit walks its table for
+        // them (only then are the phases staged)
+        bool walk = false;
+        if (F & F_CONTACT) {
+            bool mine = false;
+            if (i < n) {
+                const Attr ai = b.A[i];
+                mine = ai.phase >= PS_PHASE_CLOTH && ai.w != 0.0f;
+            }
+            walk = __syncthreads_or(mine ? 1 : 0) != 0;
+        }
         const float inf = __builtin_inff();
         if (staged && threadIdx.x == 0) lds[kStageSentinel] = make_float4(inf, inf, inf, 0.0f);
+        if ((F & F_CONTACT) && staged && threadIdx.x == 0) lds_ph[kStageSentinel] = 0;
         if ((F & F_CONTACT) && staged)
             for (uint32_t q = threadIdx.x; q < d->nchunks; q += blockDim.x) s_src[q] = d->src[q];
         if (UNIFORM && staged)
             stage_copy<6, 2 * kStageUnit>(d, n, [&](uint32_t j) { return b.P[j]; },
                                           [&](uint32_t l, const float4 &v) { lds[l] = v; });
+        else if ((F & F_CONTACT) && staged && walk)
+            stage_copy<6, 2 * kStageUnit>(d, n, [&](uint32_t j) { return PMH{b.P[j], b.mj[j], b.phs[j]}; },
+                                          [&](uint32_t l, const PMH &v) {
+                                              lds[l] = v.m < 0.0f ? make_float4(inf, inf, inf, 0.0f) : v.p;
+                                              lds_ph[l] = v.ph;
+                                          });
         else if (staged)
             stage_copy<6, 2 * kStageUnit>(d, n, [&](uint32_t j) { return PM{b.P[j], b.mj[j]}; },
                                           [&](uint32_t l, const PM &v) {
@@ -898,12 +923,41 @@ __global__ __launch_bounds__(kMaxBlock) void solve_k(DevBuffers b, KConst k, uin
             constexpr uint32_t kList = STAGED ? kHalfContactList : kContactList;
             uint32_t nc = 0;
             if constexpr (STAGED) {
-                for_neighbors_half_chunked(b, i, c, hh, pa, [&](uint32_t e, const PA &d) {
-                    if (is_contact(d)) {
-                        if (nc < kList) clist[nc * cstride] = e;
-                        nc++;
-                    }
-                });
+                if (staged) {
+                    // from the staged records: position and phase (a
+                    // particle with w = 0 is staged at infinity and marked
+                    // by the phase's sign bit: its position from memory)
+                    for_neighbors_half_chunked(
+                        b, i, c, hh, [&](uint32_t e) { return PH{lds[e], lds_ph[e]}; },
+                        [&](uint32_t e, const PH &d) {
+                            const int32_t phj = d.ph & 0x7fffffff;
+                            if (ph > PS_PHASE_SOLID && ph == phj) return;
+                            float qx = d.p.x, qy = d.p.y, qz = d.p.z;
+                            if (d.ph < 0) {
+                                // a rare path; read through a volatile
+                                // pointer, which the compiler cannot merge
+                                // with the staged record's (merged, the
+                                // records went through scratch memory)
+                                const volatile float *g = &b.P[slot(e)].x;
+                                qx = g[0];
+                                qy = g[1];
+                                qz = g[2];
+                            }
+                            float dx = px - qx, dy = py - qy, dz = pz - qz;
+                            float r2 = ((dx * dx) + (dy * dy)) + (dz * dz);
+                            if (r2 < k.collide_r2) {
+                                if (nc < kList) clist[nc * cstride] = e;
+                                nc++;
+                            }
+                        });
+                } else {
+                    for_neighbors_half_chunked(b, i, c, hh, pa, [&](uint32_t e, const PA &d) {
+                        if (is_contact(d)) {
+                            if (nc < kList) clist[nc * cstride] = e;
+                            nc++;
+                        }
+                    });
+                }
             } else {
                 for_neighbors_chunked_h(b, i, c, pa, [&](uint32_t h, uint32_t j, const PA &d) {
                     if (is_contact(d)) {
