@@ -1415,6 +1415,7 @@ static int add_particles_impl(ps_system *s, uint32_t scene, uint32_t n, const fl
     s->any_fluid = anyf;
     s->wf = wf;
     s->any_contact_phase = contact;
+    s->b.contacts = contact ? 1u : 0u;
     s->any_solid_phase = solid;
     s->nbr_valid = false;
     return PS_OK;

@@ -117,6 +117,7 @@ constexpr uint32_t kHalfContactList = 8;
 struct StageDesc {
     uint32_t staged;              // 1: the unit's table entries are LDS indices
     uint32_t nchunks;             // chunks in use
+    uint32_t walk;                // 1: a particle of the unit walks its table for contacts
     uint32_t src[kStageChunks];   // global slot of each chunk's first element
 };
 
@@ -140,6 +141,7 @@ struct DevBuffers {
                            // for w = 0 (left out of the fluid sums)
     int32_t *phs;          // per slot, written with mj: the phase, its sign bit
                            // set for w = 0 (the staged solve's contact records)
+    uint32_t contacts;     // the scene has contact phases (then phs is written)
     SceneDev *scenes;
     // grid
     uint32_t *cell_count;  // num_cells, zero between builds

@@ -522,7 +522,7 @@ __global__ __launch_bounds__(kStageUnit) void neighbors_staged_k(DevBuffers b, K
 {
     __shared__ uint32_t s_start[kStageIv], s_base[kStageIv + 1], s_len[kStageIv];
     __shared__ uint32_t s_row[kStageRows], s_xa[kStageRows], s_xb[kStageRows];
-    __shared__ uint32_t s_wheads[kStageUnit / 64], s_wlast[kStageUnit / 64];
+    __shared__ uint32_t s_wheads[kStageUnit / 64], s_wlast[kStageUnit / 64], s_wwalk[kStageUnit / 64];
     __shared__ uint32_t s_staged;
     __shared__ StageDesc s_desc;
     __shared__ uint32_t ldsw[kStageUnit / 64];
@@ -545,6 +545,9 @@ __global__ __launch_bounds__(kStageUnit) void neighbors_staged_k(DevBuffers b, K
         cbase = sc.cell_base;
     }
     const uint32_t key = valid ? b.key[i] - cbase : 0u;
+    // whether the particle walks its table for contacts (phase from cloth
+    // up, w != 0: the sign bit of phs clear), loaded beside the key
+    const bool walker = valid && b.contacts && b.phs[i] >= PS_PHASE_CLOTH;
     const uint32_t row = key / gx, x = key % gx;  // row = z gy + y
     const uint32_t ry = row % gy, rz = row / gy;
     const uint32_t x0 = x > 0 ? x - 1 : 0, x1 = x + 1 < gx ? x + 1 : gx - 1;
@@ -560,12 +563,17 @@ __global__ __launch_bounds__(kStageUnit) void neighbors_staged_k(DevBuffers b, K
     if (lane == 0 && w > 0) prev = s_wlast[w - 1];
     const bool head = valid && (tid == 0 || row != prev);
     const uint64_t hm = __ballot(head);
-    if (lane == 0) s_wheads[w] = (uint32_t)__popcll(hm);
+    const uint64_t wm = __ballot(walker);
+    if (lane == 0) {
+        s_wheads[w] = (uint32_t)__popcll(hm);
+        s_wwalk[w] = wm != 0ull ? 1u : 0u;
+    }
     __syncthreads();
-    uint32_t before = 0, nrows = 0;
+    uint32_t before = 0, nrows = 0, walk = 0;
     for (uint32_t q = 0; q < kStageUnit / 64; q++) {
         before += q < w ? s_wheads[q] : 0u;
         nrows += s_wheads[q];
+        walk |= s_wwalk[q];
     }
     if (!one_scene) nrows = kStageRows + 1u;  // not staged
     const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
@@ -624,6 +632,7 @@ __global__ __launch_bounds__(kStageUnit) void neighbors_staged_k(DevBuffers b, K
     if (tid == 0) {
         desc->staged = s_desc.staged;
         desc->nchunks = nch;
+        desc->walk = walk;
     }
     for (uint32_t c = tid; c < nch; c += kStageUnit) desc->src[c] = s_desc.src[c];
 
@@ -829,11 +838,11 @@ __global__ __launch_bounds__(kMaxBlock) void solve_k(DevBuffers b, KConst k, uin
     // In a scene that is not a uniform fluid a particle with w = 0 is
     // staged as a position at infinity (the fluid sum leaves it out, as the
     // global path does).  With contacts, in a unit where some particle
-    // walks its table for them, each slot's phase is staged as well (lds_ph,
-    // from the sort's phs: the sign bit set for a particle with w = 0), so
-    // the counting walk runs from LDS, reading the true position of such a
-    // particle from global memory; the corrections gather through the
-    // unit's chunk table (s_src).
+    // walks its table for them (the neighbour build's StageDesc::walk), each
+    // slot's phase is staged as well (lds_ph, from the sort's contiguous phs:
+    // the sign bit set for a particle with w = 0), so the counting walk runs
+    // from LDS, reading the true position of such a particle from global
+    // memory; the corrections gather through the unit's chunk table (s_src).
     __shared__ float4 lds[STAGED ? kStageSlots + 1 : 1];
     __shared__ int32_t lds_ph[STAGED && (F & F_CONTACT) ? kStageSlots + 1 : 1];
     __shared__ uint32_t s_src[STAGED && (F & F_CONTACT) ? kStageChunks : 1];
@@ -847,17 +856,6 @@ __global__ __launch_bounds__(kMaxBlock) void solve_k(DevBuffers b, KConst k, uin
     if (STAGED) {
         const StageDesc *d = b.stage + unit;
         staged = d->staged != 0u;
-        // with contacts: whether a particle of the unit walks its table for
-        // them (only then are the phases staged)
-        bool walk = false;
-        if (F & F_CONTACT) {
-            bool mine = false;
-            if (i < n) {
-                const Attr ai = b.A[i];
-                mine = ai.phase >= PS_PHASE_CLOTH && ai.w != 0.0f;
-            }
-            walk = __syncthreads_or(mine ? 1 : 0) != 0;
-        }
         const float inf = __builtin_inff();
         if (staged && threadIdx.x == 0) lds[kStageSentinel] = make_float4(inf, inf, inf, 0.0f);
         if ((F & F_CONTACT) && staged && threadIdx.x == 0) lds_ph[kStageSentinel] = 0;
@@ -866,7 +864,7 @@ __global__ __launch_bounds__(kMaxBlock) void solve_k(DevBuffers b, KConst k, uin
         if (UNIFORM && staged)
             stage_copy<6, 2 * kStageUnit>(d, n, [&](uint32_t j) { return b.P[j]; },
                                           [&](uint32_t l, const float4 &v) { lds[l] = v; });
-        else if ((F & F_CONTACT) && staged && walk)
+        else if ((F & F_CONTACT) && staged && d->walk)
             stage_copy<6, 2 * kStageUnit>(d, n, [&](uint32_t j) { return PMH{b.P[j], b.mj[j], b.phs[j]}; },
                                           [&](uint32_t l, const PMH &v) {
                                               lds[l] = v.m < 0.0f ? make_float4(inf, inf, inf, 0.0f) : v.p;
