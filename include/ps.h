@@ -497,7 +497,7 @@ typedef enum ps_kernel_id {
     /* reverse pass (ps_step_backward) */
     PS_K_ADJ_XSPH,
     PS_K_ADJ_FINALIZE,
-    PS_K_ADJ_PRE,
+    PS_K_ADJ_PRE,       /* no launches: the pre-step ends adj_finalize and adj_position */
     PS_K_ADJ_LAMBDA,
     PS_K_ADJ_POSITION,
     PS_K_ADJ_STIFFNESS, /* no launches: adj_position forms dL/dstiffness */

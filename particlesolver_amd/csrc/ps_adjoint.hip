@@ -18,11 +18,12 @@
 //   b = spiky (h - r)^2 / r (G = (0, -gdeg, 0) for a degenerate pair),
 //   l_i = -C_i / (D_i + eps), C_i = rho_i / rho0_i - 1,
 //   D_i = sum_j a_ij^2 r^2 + |S_i|^2, S_i = sum_j a_ij d_ij, a = b / rho0_i.
-// Kernels per reversed iteration: adj_pre_k (walls, pins and wall friction
-// backwards, contact counts, rho/S/D, f), adj_lambda_k (dL/dl -> dL/dC,
-// dL/dD), adj_position_k (dL/dp: pair, density, D, contact and distance
-// terms, and dL/dstiffness per constraint).  Per substep: adj_xsph_k and
-// adj_finalize_k (velocity + XSPH backwards) before the iterations,
+// Kernels per reversed iteration: adj_lambda_k (dL/dl -> dL/dC, dL/dD),
+// adj_position_k (dL/dp: pair, density, D, contact and distance terms, and
+// dL/dstiffness per constraint), which ends with the next iteration's
+// pre-step (adj_pre_one: walls, pins and wall friction backwards, the rho0
+// term of f).  Per substep: adj_xsph_k and adj_finalize_k (velocity + XSPH
+// backwards, then the last iteration's pre-step) before the iterations,
 // adj_predict_k after them.
 
 #include "ps_device.hpp"
@@ -198,27 +199,45 @@ struct GXYA {
 
 // ---------------------------------------------------------------------------
 // Pins, walls and wall friction backwards, and the fluid sum's rho0 term,
-// from the recorded forward quantities (rho, S, D of the lambda pass; f and
-// the contact count of the solve).  Wall friction (walls(): p'' = p' - t f,
-// t = e - (e.n) n, e = p' - x*): static, or kinetic clamped to f = 1:
-// dp''/dp' = n n^T, dp''/dx* = I - n n^T; kinetic with f = wall_k / l < 1:
+// of one particle (the pre-step of a reversed iteration, from the recorded
+// forward quantities: f and the contact count of the solve, the clamp bits
+// and wall-friction records of walls()), for g = dL/dp after the iteration.
+// Wall friction (walls(): p'' = p' - t f, t = e - (e.n) n, e = p' - x*):
+// static, or kinetic clamped to f = 1: dp''/dp' = n n^T,
+// dp''/dx* = I - n n^T; kinetic with f = wall_k / l < 1:
 // dp''/dp' = I - wall_k K (I - n n^T), K = (I - t^ t^T) / l, the x* part its
-// negative second term, and dp''/dwall_k = -t / l.  Writes M = (mu =
-// yd / rho0 for fluid, yd otherwise; 1 / rho0) and the attribute record the
-// later passes gather.
-template <bool UNIFORM>
-__global__ __launch_bounds__(kAdjBlock) void adj_pre_k(DevBuffers b, KConst k, AdjBuffers a, uint32_t n)
+// negative second term, and dp''/dwall_k = -t / l.  Writes the iteration's
+// Yn (masked dL/dy), M = (mu = yd / rho0 for fluid, yd otherwise; 1 / rho0)
+// and the attribute record ABn the later kernels gather; updates xsb, wkb
+// and r0b.  Run at the end of the kernel that forms g (adj_finalize_k,
+// adj_position_k), by the particle's lane 0.
+// The caller loads the records (AdjPreIn), early where registers allow, so
+// that this tail waits on no load unless the particle has wall friction.
+struct AdjPreIn {
+    uint32_t cl;  // clamp bits
+    float4 tf;    // recorded fluid sum and contact count
+    float r0b;    // dL/drho0 so far
+};
+
+__device__ __forceinline__ AdjPreIn adj_pre_load(const AdjBuffers &a, uint32_t i)
 {
-    const uint32_t i = blockIdx.x * kAdjBlock + threadIdx.x;
-    if (i >= n) return;
-    const Attr ai = b.A[i];
-    float g[3];
-    load3(a.pbn[i], g);
-    const uint32_t cl = a.clamp[i];
+    AdjPreIn q;
+    q.cl = a.clampn[i];
+    q.tf = a.tfn[i];
+    q.r0b = a.r0b[i];
+    return q;
+}
+
+template <bool UNIFORM>
+__device__ __forceinline__ void adj_pre_one(const DevBuffers &b, const KConst &k, const AdjBuffers &a,
+                                            uint32_t i, const Attr &ai, const AdjPreIn &in, float *g)
+{
+    const uint32_t cl = in.cl;
+    const float4 tf = in.tf;
     if (cl & kPinned) {
         g[0] = g[1] = g[2] = 0.0f;
     } else if (cl & kWallFriction) {
-        const float4 rec = a.wrec[i];
+        const float4 rec = a.wrecn[i];
         const float4 xs = b.X[i];
         const int code = (int)rec.w;
         const float nn[3] = {(float)(code / 9 - 1), (float)((code / 3) % 3 - 1), (float)(code % 3 - 1)};
@@ -254,17 +273,16 @@ __global__ __launch_bounds__(kAdjBlock) void adj_pre_k(DevBuffers b, KConst k, A
     const float yd[3] = {g[0], g[1], k.dims == 2u ? 0.0f : g[2]};
     const bool fl = UNIFORM || ai.phase == PS_PHASE_FLUID;
     const float ir0 = 1.0f / ai.rho0;
-    const float4 tf = a.tf[i];
     if (fl) {
         const float f[3] = {tf.x, tf.y, tf.z};
-        a.r0b[i] = a.r0b[i] - (dot3(yd, f) * (ir0 * ir0));
+        a.r0b[i] = in.r0b - (dot3(yd, f) * (ir0 * ir0));
     }
-    a.Y[i] = make_float4(g[0], g[1], g[2], 0.0f);
+    a.Yn[i] = make_float4(g[0], g[1], g[2], 0.0f);
     a.M[i] = fl ? make_float4(yd[0] * ir0, yd[1] * ir0, yd[2] * ir0, ir0) : make_float4(yd[0], yd[1], yd[2], ir0);
     if (!UNIFORM) {
         const uint32_t nc = __float_as_uint(tf.w);
         a.nc[i] = nc;
-        a.AB[i] = AdjAttr{ai.w, ai.phase, nc, adj_mass(k, ai.w, ai.phase)};
+        a.ABn[i] = AdjAttr{ai.w, ai.phase, nc, adj_mass(k, ai.w, ai.phase)};
     }
 }
 
@@ -469,7 +487,12 @@ __global__ __launch_bounds__(kAdjBlock) void adj_position_k(DevBuffers b, KConst
         }
     }
     if (hh) return;
-    a.pb[p] = make_float4(acc[0], acc[1], acc[2], 0.0f);
+    // the next iteration's pre-step (its records loaded here: held from the
+    // start they would cost this kernel a wavefront per SIMD)
+    if (a.clampn)
+        adj_pre_one<UNIFORM>(b, k, a, p, ap, adj_pre_load(a, p), acc);
+    else
+        a.pb[p] = make_float4(acc[0], acc[1], acc[2], 0.0f);
 }
 
 // ---------------------------------------------------------------------------
@@ -545,11 +568,14 @@ __global__ __launch_bounds__(kAdjBlock) void adj_finalize_k(DevBuffers b, KConst
     const uint32_t p = xcd_block() * (kAdjBlock / 4) + (threadIdx.x >> 2);
     const uint32_t hh = threadIdx.x & 3u;
     if (p >= n) return;  // the four lanes of a particle leave together
-    // the particle's own loads go out together
+    // the particle's own loads go out together (and the last iteration's
+    // pre-step records)
     const Attr ap = b.A[p];
     const float4 pp = b.P[p], xp = b.X[p];
     const float4 shp4 = a.Y[p];
     const uint32_t c = b.ncnt[p];
+    AdjPreIn pre{};
+    if (a.clampn && hh == 0) pre = adj_pre_load(a, p);
     const float4 vb = a.gv[ap.orig], xb = a.gx[ap.orig];
     float ub[3] = {vb.x / dt, vb.y / dt, vb.z / dt};
     float acc[3] = {xb.x, xb.y, xb.z};
@@ -607,8 +633,12 @@ __global__ __launch_bounds__(kAdjBlock) void adj_finalize_k(DevBuffers b, KConst
         ub[q] = ub[q] + lu[q];
         acc[q] = acc[q] + la[q];
     }
-    a.pb[p] = make_float4(acc[0] + ub[0], acc[1] + ub[1], acc[2] + ub[2], 0.0f);
     a.xsb[p] = make_float4(-ub[0], -ub[1], -ub[2], 0.0f);
+    float g[3] = {acc[0] + ub[0], acc[1] + ub[1], acc[2] + ub[2]};
+    if (a.clampn)
+        adj_pre_one<UNIFORM>(b, k, a, p, ap, pre, g);  // the last iteration's pre-step
+    else
+        a.pb[p] = make_float4(g[0], g[1], g[2], 0.0f);
 }
 
 // predict backwards: p0 = x* + dt (v + dt g) for w != 0 (z held in 2D);
@@ -639,13 +669,6 @@ hipError_t launch_adj(hipStream_t st, AdjKernel which, const DevBuffers &b, cons
 {
     if (count == 0) return hipSuccess;
     const dim3 grid(adj_blocks(count)), block(kAdjBlock);
-#define PS_ADJ_LAUNCH(kern, ...)                                                                  \
-    do {                                                                                           \
-        if (uniform)                                                                               \
-            PS_LAUNCH(kern<true>, grid, block, 0, st, __VA_ARGS__);                       \
-        else                                                                                       \
-            PS_LAUNCH(kern<false>, grid, block, 0, st, __VA_ARGS__);                      \
-    } while (0)
     // four lanes per particle: kAdjBlock / 4 particles per block
     const dim3 grid4((count + kAdjBlock / 4 - 1) / (kAdjBlock / 4));
 #define PS_ADJ_LAUNCH4(kern, ...)                                                                 \
@@ -658,13 +681,11 @@ hipError_t launch_adj(hipStream_t st, AdjKernel which, const DevBuffers &b, cons
     switch (which) {
     case ADJ_XSPH: PS_ADJ_LAUNCH4(adj_xsph_k, b, k, a, count, dt); break;
     case ADJ_FINALIZE: PS_ADJ_LAUNCH4(adj_finalize_k, b, k, a, count, dt); break;
-    case ADJ_PRE: PS_ADJ_LAUNCH(adj_pre_k, b, k, a, count); break;
     case ADJ_LAMBDA: PS_ADJ_LAUNCH4(adj_lambda_k, b, k, a, count); break;
     case ADJ_POSITION: PS_ADJ_LAUNCH4(adj_position_k, b, k, a, count); break;
     case ADJ_PREDICT: PS_LAUNCH(adj_predict_k, grid, block, 0, st, b, k, a, count, dt); break;
     default: return hipErrorInvalidValue;
     }
-#undef PS_ADJ_LAUNCH
 #undef PS_ADJ_LAUNCH4
     return hipGetLastError();
 }

@@ -142,8 +142,8 @@ struct ps_system {
         Attr *tA = nullptr;
         uint8_t *tClamp = nullptr;
         // reverse-pass work arrays
-        float4 *pb = nullptr, *pb2 = nullptr, *Y = nullptr, *M = nullptr, *xsb = nullptr;
-        ps::AdjAttr *AB = nullptr;
+        float4 *pb = nullptr, *Y = nullptr, *Y2 = nullptr, *M = nullptr, *xsb = nullptr;
+        ps::AdjAttr *AB = nullptr, *AB2 = nullptr;
         float4 *gx = nullptr, *gv = nullptr;
         float4 *MC = nullptr, *KV = nullptr;
         uint32_t *nc = nullptr;
@@ -380,7 +380,7 @@ void adj_free(ps_system *s)
     dev_free(w.ckX); dev_free(w.ckV); dev_free(w.ckA); dev_free(w.ckS); dev_free(w.ckSt);
     dev_free(w.tN); dev_free(w.tC); dev_free(w.tS); dev_free(w.tX); dev_free(w.tP); dev_free(w.tPit);
     dev_free(w.tW); dev_free(w.tA); dev_free(w.tClamp); dev_free(w.tLQ); dev_free(w.tF); dev_free(w.tRho);
-    dev_free(w.pb); dev_free(w.pb2); dev_free(w.Y); dev_free(w.M); dev_free(w.AB);
+    dev_free(w.pb); dev_free(w.Y); dev_free(w.Y2); dev_free(w.M); dev_free(w.AB); dev_free(w.AB2);
     dev_free(w.xsb); dev_free(w.gx); dev_free(w.gv); dev_free(w.MC); dev_free(w.KV); dev_free(w.nc); dev_free(w.r0b);
     dev_free(w.wkb); dev_free(w.drho); dev_free(w.dwk); dev_free(w.sb);
     w.cap = w.cks = w.tslots = w.iters = w.ndist = w.tab = 0;
@@ -1819,8 +1819,8 @@ int adj_reserve(ps_system *s, size_t cks, size_t tslots, size_t iters)
         (rc = dev_alloc(s, &w.tW, ti * c)) || (rc = dev_alloc(s, &w.tClamp, ti * c)) ||
         (rc = dev_alloc(s, &w.tLQ, ti * c)) || (rc = dev_alloc(s, &w.tF, ti * c)) ||
         (rc = dev_alloc(s, &w.tRho, ti * c)) ||
-        (rc = dev_alloc(s, &w.pb, c)) || (rc = dev_alloc(s, &w.pb2, c)) || (rc = dev_alloc(s, &w.Y, c)) ||
-        (rc = dev_alloc(s, &w.M, c)) || (rc = dev_alloc(s, &w.AB, c)) ||
+        (rc = dev_alloc(s, &w.pb, c)) || (rc = dev_alloc(s, &w.Y, c)) || (rc = dev_alloc(s, &w.Y2, c)) ||
+        (rc = dev_alloc(s, &w.M, c)) || (rc = dev_alloc(s, &w.AB, c)) || (rc = dev_alloc(s, &w.AB2, c)) ||
         (rc = dev_alloc(s, &w.xsb, c)) || (rc = dev_alloc(s, &w.gx, c)) || (rc = dev_alloc(s, &w.gv, c)) ||
         (rc = dev_alloc(s, &w.MC, c)) || (rc = dev_alloc(s, &w.KV, c)) || (rc = dev_alloc(s, &w.nc, c)) || (rc = dev_alloc(s, &w.r0b, c)) ||
         (rc = dev_alloc(s, &w.wkb, c)) || (rc = dev_alloc(s, &w.drho, c)) || (rc = dev_alloc(s, &w.dwk, c)) ||
@@ -1949,9 +1949,13 @@ int adj_reverse(ps_system *s, float dt, size_t t, hipStream_t st)
     bt.slot_of = w.tS + t * c;
     bt.P = w.tP + t * c;
     ps::AdjBuffers a{};
-    a.Y = w.Y;
+    // dL/dy and the attribute records alternate between two buffers by
+    // iteration: iteration it uses buffer it % 2, and the kernel that forms
+    // its input (adj_position_k of iteration it + 1, or adj_finalize_k, whose
+    // dL/dS is in buffer I % 2) writes them while its own are still read
+    float4 *Yb[2] = {w.Y, w.Y2};
+    ps::AdjAttr *ABb[2] = {w.AB, w.AB2};
     a.M = w.M;
-    a.AB = w.AB;
     a.MC = w.MC;
     a.KV = w.KV;
     a.nc = w.nc;
@@ -1965,8 +1969,7 @@ int adj_reverse(ps_system *s, float dt, size_t t, hipStream_t st)
     a.dwk = w.dwk;
     a.contacts = s->any_contact_phase ? 1u : 0u;
     a.dist = s->dist_rest.empty() ? 0u : 1u;
-    float4 *cur = w.pb, *nxt = w.pb2;
-    a.pb = cur;
+    a.pb = w.pb;
     const uint32_t n = s->n;
     const bool uni = s->uniform;
     s->tstream = st;  // LAUNCH times these launches on st
@@ -1974,24 +1977,35 @@ int adj_reverse(ps_system *s, float dt, size_t t, hipStream_t st)
         ps_system *s;
         ~Restore() { s->tstream = nullptr; }
     } restore{s};
+    // the pre-step of iteration it, run at the end of the kernel forming
+    // its input (none past the first iteration: that kernel writes pb)
+    auto next_pre = [&](size_t it, bool any) {
+        if (!any) {
+            a.clampn = nullptr;
+            return;
+        }
+        const size_t q = (t * I + it) * c;
+        a.clampn = w.tClamp + q;
+        a.wrecn = w.tW + q;
+        a.tfn = w.tF + q;
+        a.Yn = Yb[it % 2];
+        a.ABn = ABb[it % 2];
+    };
+    a.Y = Yb[I % 2];  // dL/dS
     LAUNCH(s, PS_K_ADJ_XSPH, ps::launch_adj(st, ps::ADJ_XSPH, bt, s->kc, a, n, dt, uni));
+    next_pre(I ? I - 1 : 0, I > 0);
     LAUNCH(s, PS_K_ADJ_FINALIZE, ps::launch_adj(st, ps::ADJ_FINALIZE, bt, s->kc, a, n, dt, uni));
     for (size_t it = I; it-- > 0;) {
         const size_t q = (t * I + it) * c;
         a.Pit = w.tPit + q;
-        a.clamp = w.tClamp + q;
-        a.wrec = w.tW + q;
         a.SD = w.tLQ + q;
         a.rho = w.tRho + q;
-        a.tf = w.tF + q;
-        a.pbn = cur;
-        a.pb = nxt;
-        LAUNCH(s, PS_K_ADJ_PRE, ps::launch_adj(st, ps::ADJ_PRE, bt, s->kc, a, n, dt, uni));
+        a.Y = Yb[it % 2];
+        a.AB = ABb[it % 2];
         LAUNCH(s, PS_K_ADJ_LAMBDA, ps::launch_adj(st, ps::ADJ_LAMBDA, bt, s->kc, a, n, dt, uni));
+        next_pre(it ? it - 1 : 0, it > 0);
         LAUNCH(s, PS_K_ADJ_POSITION, ps::launch_adj(st, ps::ADJ_POSITION, bt, s->kc, a, n, dt, uni));
-        std::swap(cur, nxt);
     }
-    a.pb = cur;
     LAUNCH(s, PS_K_ADJ_PREDICT, ps::launch_adj(st, ps::ADJ_PREDICT, bt, s->kc, a, n, dt, uni));
     return PS_OK;
 }

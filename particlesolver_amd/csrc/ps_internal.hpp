@@ -216,13 +216,9 @@ struct __align__(16) AdjAttr {
 // noted.
 struct AdjBuffers {
     const float4 *Pit;     // iterate of the iteration being reversed (x, y, z, lambda)
-    const uint8_t *clamp;  // its walls()/pin bits
-    const float4 *wrec;    // its wall-friction records
-    const float4 *pbn;     // dL/dp after the iteration
     float4 *pb;            // dL/dp before it (finalize: of the final iterate)
     const float4 *SD;      // recorded lambda pass: S = sum_j a d (xyz), D (w)
     const float *rho;      // recorded lambda pass: density
-    const float4 *tf;      // recorded solve: fluid sum f (xyz), contact count bits (w)
     float4 *Y;             // masked dL/dy (xyz); finalize: dL/dS
     float4 *M;             // mu = dL/dy / rho0 (fluid) or dL/dy (xyz), 1/rho0 (w)
     AdjAttr *AB;           // attributes with contact counts (non-uniform scenes)
@@ -236,6 +232,16 @@ struct AdjBuffers {
     float *drho, *dwk;     // insertion order: accumulated dL/drho0, dL/dwall_k
     uint32_t contacts;     // the scene has contact phases
     uint32_t dist;         // the scene has distance constraints
+    // The pre-step of the next reversed iteration (pins, walls and wall
+    // friction backwards), run per particle by adj_finalize_k and
+    // adj_position_k on the dL/dp they form, with that iteration's records;
+    // it writes Yn and ABn (the other buffer of a pair, as the kernel's own
+    // neighbours still read Y and AB), M, nc, and updates xsb, wkb, r0b.
+    // clampn null: no next iteration, pb is written instead.
+    const uint8_t *clampn;
+    const float4 *wrecn, *tfn;
+    float4 *Yn;
+    AdjAttr *ABn;
 };
 
 // Timing of one launch by the dispatch itself: the host puts an event pair
@@ -294,14 +300,14 @@ int fail_stateless(int code, const std::string &msg);
 
 // Reverse pass (ps_adjoint.hip), one kernel per call: velocity + XSPH
 // backwards (ADJ_XSPH then ADJ_FINALIZE; b.P the final iterate, b.X the
-// substep start), one iteration backwards (ADJ_PRE, ADJ_LAMBDA,
-// ADJ_POSITION, which also adds dL/dstiffness per distance constraint;
-// a.Pit, a.clamp, a.wrec of that iteration, a.pbn -> a.pb), predict
-// backwards (ADJ_PREDICT).  count: particles.
+// substep start), one iteration backwards (ADJ_LAMBDA, ADJ_POSITION, which
+// also adds dL/dstiffness per distance constraint; a.Pit, a.SD, a.rho of
+// that iteration), predict backwards (ADJ_PREDICT).  ADJ_FINALIZE and
+// ADJ_POSITION end with the next iteration's pre-step (AdjBuffers::clampn),
+// or write a.pb.  count: particles.
 enum AdjKernel : uint32_t {
     ADJ_XSPH,
     ADJ_FINALIZE,
-    ADJ_PRE,
     ADJ_LAMBDA,
     ADJ_POSITION,
     ADJ_PREDICT,
