@@ -425,65 +425,91 @@ __global__ __launch_bounds__(kAdjBlock) void adj_position_k(DevBuffers b, KConst
     // whose first particle p is, also its dL/dstiffness (one writer per
     // constraint): T_a / st = -(w_a / (wsum occ_a)) (1 - L/r) d_ab, and T_b
     // its mirror.  The particle's four lanes take four constraints at a
-    // time (each a chain of dependent loads); lane 0 adds their terms in
-    // constraint order.
+    // time, kR such rounds at once: each constraint is a chain of dependent
+    // loads (index, pair, slot, records), and each level of the chain goes
+    // out for all kR rounds together.  Lane 0 adds the terms in constraint
+    // order.
     if (a.dist) {
+        constexpr uint32_t kR = 3;
         const uint32_t o = ap.orig;
         const uint32_t e0 = b.dist_off[o], e1 = b.dist_off[o + 1];
         const uint32_t quad = threadIdx.x & ~3u;
-        for (uint32_t base = e0; base < e1; base += 4u) {  // the same trip count in the four lanes
-            const uint32_t e = base + hh;
-            // this lane's constraint: the p term X and the j term Z
-            float X[3] = {0.0f, 0.0f, 0.0f}, Z[3] = {0.0f, 0.0f, 0.0f};
-            uint32_t fl = 0;  // bit 0: X is added, bit 1: Z is subtracted
-            if (e < e1) {
-                const uint32_t cid = b.dist_adj[e];
-                const uint2 pr = b.dist_pairs[cid];
-                const uint32_t j = b.slot_of[pr.x == o ? pr.y : pr.x];
-                const float4 pj = a.Pit[j];
-                const float dv[3] = {pp.x - pj.x, pp.y - pj.y, pp.z - pj.z};
+        const float occo = (float)b.occ[o];
+        for (uint32_t base = e0; base < e1; base += 4u * kR) {  // the same trip count in the four lanes
+            uint32_t cid[kR], j[kR];
+            bool ok[kR];
+            uint2 pr[kR];
+#pragma unroll
+            for (uint32_t r = 0; r < kR; r++) {
+                const uint32_t e = base + 4u * r + hh;
+                ok[r] = e < e1;
+                cid[r] = b.dist_adj[ok[r] ? e : e0];
+            }
+#pragma unroll
+            for (uint32_t r = 0; r < kR; r++) pr[r] = b.dist_pairs[cid[r]];
+#pragma unroll
+            for (uint32_t r = 0; r < kR; r++) j[r] = b.slot_of[pr[r].x == o ? pr[r].y : pr[r].x];
+            float4 pj[kR], yj[kR];
+            float2 rs[kR];
+            Attr aj[kR];
+#pragma unroll
+            for (uint32_t r = 0; r < kR; r++) {
+                pj[r] = a.Pit[j[r]];
+                aj[r] = b.A[j[r]];
+                yj[r] = a.Y[j[r]];
+                rs[r] = b.dist_rs[cid[r]];
+            }
+            // each round's p term X and j term Z; fl bit 0: X is added,
+            // bit 1: Z is subtracted
+            float X[kR][3], Z[kR][3];
+            uint32_t fl[kR];
+#pragma unroll
+            for (uint32_t r = 0; r < kR; r++) {
+                fl[r] = 0;
+                for (int q2 = 0; q2 < 3; q2++) X[r][q2] = Z[r][q2] = 0.0f;
+                const float dv[3] = {pp.x - pj[r].x, pp.y - pj[r].y, pp.z - pj[r].z};
                 const float dist = sqrtf(((dv[0] * dv[0]) + (dv[1] * dv[1])) + (dv[2] * dv[2]));
-                const Attr aj = b.A[j];
-                const float wsum = ap.w + aj.w;
-                if (!(dist < 1e-6f || wsum == 0.0f)) {
-                    const float2 rs = b.dist_rs[cid];
-                    const float L = rs.x, st = rs.y;
-                    const float k1 = 1.0f - (L / dist), k2 = L / ((dist * dist) * dist);
-                    const bool first = pr.x == o;  // d = p_a - p_b
-                    float gs = 0.0f;
-                    // M(d) = -(w st / (wsum occ)) [(1 - L/r) I + (L/r^3) d d^T]
-                    if (ap.w != 0.0f) {
-                        const float cf = -(((ap.w * st) / wsum) / (float)b.occ[o]);
-                        const float dy = dot3(dv, ydp);
-                        for (int q2 = 0; q2 < 3; q2++) X[q2] = cf * ((k1 * ydp[q2]) + ((k2 * dy) * dv[q2]));
-                        fl |= 1u;
-                        if (first) gs = gs - (((ap.w / wsum) / (float)b.occ[o]) * (k1 * dy));
-                    }
-                    if (aj.w != 0.0f) {
-                        float ydj[3];
-                        yd_of(k, a.Y[j], ydj);
-                        const float cf = -(((aj.w * st) / wsum) / (float)b.occ[aj.orig]);
-                        const float dy = dot3(dv, ydj);
-                        for (int q2 = 0; q2 < 3; q2++) Z[q2] = cf * ((k1 * ydj[q2]) + ((k2 * dy) * dv[q2]));
-                        fl |= 2u;
-                        if (first) gs = gs + (((aj.w / wsum) / (float)b.occ[aj.orig]) * (k1 * dy));
-                    }
-                    if (first) a.sb[cid] = a.sb[cid] + gs;
+                const float wsum = ap.w + aj[r].w;
+                if (!ok[r] || dist < 1e-6f || wsum == 0.0f) continue;
+                const float L = rs[r].x, st = rs[r].y;
+                const float k1 = 1.0f - (L / dist), k2 = L / ((dist * dist) * dist);
+                const bool first = pr[r].x == o;  // d = p_a - p_b
+                float gs = 0.0f;
+                // M(d) = -(w st / (wsum occ)) [(1 - L/r) I + (L/r^3) d d^T]
+                if (ap.w != 0.0f) {
+                    const float cf = -(((ap.w * st) / wsum) / occo);
+                    const float dy = dot3(dv, ydp);
+                    for (int q2 = 0; q2 < 3; q2++) X[r][q2] = cf * ((k1 * ydp[q2]) + ((k2 * dy) * dv[q2]));
+                    fl[r] |= 1u;
+                    if (first) gs = gs - (((ap.w / wsum) / occo) * (k1 * dy));
                 }
-            }
-            for (uint32_t l = 0; l < 4u; l++) {
-                const int src = (int)(quad + l);
-                const uint32_t fll = (uint32_t)__shfl((int)fl, src, 64);
-                float Xl[3], Zl[3];
-                for (int q2 = 0; q2 < 3; q2++) {
-                    Xl[q2] = __shfl(X[q2], src, 64);
-                    Zl[q2] = __shfl(Z[q2], src, 64);
+                if (aj[r].w != 0.0f) {
+                    float ydj[3];
+                    yd_of(k, yj[r], ydj);
+                    const float occj = (float)b.occ[aj[r].orig];
+                    const float cf = -(((aj[r].w * st) / wsum) / occj);
+                    const float dy = dot3(dv, ydj);
+                    for (int q2 = 0; q2 < 3; q2++) Z[r][q2] = cf * ((k1 * ydj[q2]) + ((k2 * dy) * dv[q2]));
+                    fl[r] |= 2u;
+                    if (first) gs = gs + (((aj[r].w / wsum) / occj) * (k1 * dy));
                 }
-                if (fll & 1u)
-                    for (int q2 = 0; q2 < 3; q2++) acc[q2] = acc[q2] + Xl[q2];
-                if (fll & 2u)
-                    for (int q2 = 0; q2 < 3; q2++) acc[q2] = acc[q2] - Zl[q2];
+                if (first) a.sb[cid[r]] = a.sb[cid[r]] + gs;
             }
+#pragma unroll
+            for (uint32_t r = 0; r < kR; r++)
+                for (uint32_t l = 0; l < 4u; l++) {
+                    const int src = (int)(quad + l);
+                    const uint32_t fll = (uint32_t)__shfl((int)fl[r], src, 64);
+                    float Xl[3], Zl[3];
+                    for (int q2 = 0; q2 < 3; q2++) {
+                        Xl[q2] = __shfl(X[r][q2], src, 64);
+                        Zl[q2] = __shfl(Z[r][q2], src, 64);
+                    }
+                    if (fll & 1u)
+                        for (int q2 = 0; q2 < 3; q2++) acc[q2] = acc[q2] + Xl[q2];
+                    if (fll & 2u)
+                        for (int q2 = 0; q2 < 3; q2++) acc[q2] = acc[q2] - Zl[q2];
+                }
         }
     }
     if (hh) return;

@@ -1212,48 +1212,66 @@ __global__ __launch_bounds__(kMaxBlock) void solve_k(DevBuffers b, KConst k, uin
     // a9: distance constraints, gathered over this particle's incident list
     // in order.  Each is a chain of dependent loads (index, pair, slot,
     // position, attributes); staged, the particle's two lanes take two at a
-    // time and lane 0 subtracts both terms in constraint order
+    // time, kR such rounds at once, each level of the chains going out for
+    // all of them together, and lane 0 subtracts the terms in constraint
+    // order
     if (F & F_DIST) {
         const uint32_t o = a.orig;
         const uint32_t a0 = b.dist_off[o], a1 = b.dist_off[o + 1];
         if (a1 > a0 && wi != 0.0f) {
             float cx = 0.0f, cy = 0.0f, cz = 0.0f;
-            constexpr uint32_t kStep = STAGED ? 2u : 1u;
-            for (uint32_t q0 = a0; q0 < a1; q0 += kStep) {  // the same trip count in both lanes
-                const uint32_t q = q0 + (STAGED ? hh : 0u);
-                float tx = 0.0f, ty = 0.0f, tz = 0.0f;
-                bool use = false;
-                if (q < a1) {
-                    const uint32_t cid = b.dist_adj[q];
-                    const uint2 pr = b.dist_pairs[cid];
-                    const uint32_t other = pr.x == o ? pr.y : pr.x;
-                    const uint32_t j = b.slot_of[other];
-                    const float4 pj = b.P[j];
-                    float dx = px - pj.x, dy = py - pj.y, dz = pz - pj.z;
+            constexpr uint32_t kStep = STAGED ? 2u : 1u, kR = 3;
+            for (uint32_t q0 = a0; q0 < a1; q0 += kStep * kR) {  // the same trip count in both lanes
+                uint32_t cid[kR], j[kR];
+                bool ok[kR];
+#pragma unroll
+                for (uint32_t r = 0; r < kR; r++) {
+                    const uint32_t q = q0 + kStep * r + (STAGED ? hh : 0u);
+                    ok[r] = q < a1;
+                    cid[r] = b.dist_adj[ok[r] ? q : a0];
+                }
+                uint2 pr[kR];
+#pragma unroll
+                for (uint32_t r = 0; r < kR; r++) pr[r] = b.dist_pairs[cid[r]];
+#pragma unroll
+                for (uint32_t r = 0; r < kR; r++) j[r] = b.slot_of[pr[r].x == o ? pr[r].y : pr[r].x];
+                float4 pj[kR];
+                float wj[kR];
+                float2 rs[kR];
+#pragma unroll
+                for (uint32_t r = 0; r < kR; r++) {
+                    pj[r] = b.P[j[r]];
+                    wj[r] = b.A[j[r]].w;
+                    rs[r] = b.dist_rs[cid[r]];
+                }
+#pragma unroll
+                for (uint32_t r = 0; r < kR; r++) {
+                    float tx = 0.0f, ty = 0.0f, tz = 0.0f;
+                    bool use = false;
+                    float dx = px - pj[r].x, dy = py - pj[r].y, dz = pz - pj[r].z;
                     float dist = sqrtf(((dx * dx) + (dy * dy)) + (dz * dz));
-                    float wsum = wi + b.A[j].w;
-                    if (!(dist < 1e-6f || wsum == 0.0f)) {
-                        const float2 rs = b.dist_rs[cid];
-                        float sc = ((((dist - rs.x) / wsum) / dist) * wi) * rs.y;
+                    float wsum = wi + wj[r];
+                    if (ok[r] && !(dist < 1e-6f || wsum == 0.0f)) {
+                        float sc = ((((dist - rs[r].x) / wsum) / dist) * wi) * rs[r].y;
                         tx = sc * dx;
                         ty = sc * dy;
                         tz = sc * dz;
                         use = true;
                     }
-                }
-                if (use) {
-                    cx = cx - tx;
-                    cy = cy - ty;
-                    cz = cz - tz;
-                }
-                if (STAGED) {
-                    // lane 0: then its partner's constraint, q0 + 1
-                    const float ox = __shfl_xor(tx, 1, 64), oy = __shfl_xor(ty, 1, 64),
-                                oz = __shfl_xor(tz, 1, 64);
-                    if (__shfl_xor(use ? 1 : 0, 1, 64)) {
-                        cx = cx - ox;
-                        cy = cy - oy;
-                        cz = cz - oz;
+                    if (use) {
+                        cx = cx - tx;
+                        cy = cy - ty;
+                        cz = cz - tz;
+                    }
+                    if (STAGED) {
+                        // lane 0: then its partner's constraint of the round
+                        const float ox = __shfl_xor(tx, 1, 64), oy = __shfl_xor(ty, 1, 64),
+                                    oz = __shfl_xor(tz, 1, 64);
+                        if (__shfl_xor(use ? 1 : 0, 1, 64)) {
+                            cx = cx - ox;
+                            cy = cy - oy;
+                            cz = cz - oz;
+                        }
                     }
                 }
             }
