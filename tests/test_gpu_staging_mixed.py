@@ -2,9 +2,12 @@
 
 The staged kernels read a unit's neighbour cells from LDS: lambda stages
 (x, y, z, mj) with a particle of w = 0 at infinity, the solve (x, y, z,
-lambda) the same way while its contact walks read global memory through the
-unit's chunk table, and XSPH stages when all fluid particles share one mass
-(else it gathers through the chunk table).  A batch stages the units that lie
+lambda) the same way, and in a unit holding a contact-walking particle also
+each slot's phase, so the counting walk runs from LDS (a partner with w = 0,
+staged at infinity, is read from memory: the rope below lands on a static
+slab) while the corrections gather through the unit's chunk table; XSPH
+stages when all fluid particles share one mass (else it gathers through the
+chunk table).  A batch stages the units that lie
 in one scene.  None of this may change a bit: every run below is compared
 with the FP32 oracle and with the same system built with staging off
 (PS_FLAG_NO_LDS_STAGING), and checks that units did stage."""
