@@ -52,6 +52,7 @@ struct ps_system {
     bool any_fluid = false;
     float wf = 1.0f;
     bool any_contact_phase = false;
+    uint64_t n_walk = 0;  // particles that walk their table for contacts (phase >= cloth, w != 0)
     bool any_solid_phase = false;  // SOLID or RIGID particles present
     // host copies of the constraint sets (insertion indices)
     std::vector<uint32_t> dist_pairs;
@@ -1349,6 +1350,7 @@ static int add_particles_impl(ps_system *s, uint32_t scene, uint32_t n, const fl
     bool uni = s->uniform;
     float wu = s->n ? s->wu : inv_mass[0];
     bool contact = s->any_contact_phase, solid = s->any_solid_phase;
+    uint64_t nwalk = s->n_walk;
     bool fmu = s->fluid_mass_uniform, anyf = s->any_fluid;
     float wf = s->wf;
     for (uint32_t k = 0; k < n; k++) {
@@ -1379,6 +1381,7 @@ static int add_particles_impl(ps_system *s, uint32_t scene, uint32_t n, const fl
             anyf = true;
         }
         if (phase[k] >= PS_PHASE_CLOTH) contact = true;
+        if (phase[k] >= PS_PHASE_CLOTH && inv_mass[k] != 0.0f) nwalk++;
         if (phase[k] >= PS_PHASE_SOLID) solid = true;
     }
     const size_t o = s->n;
@@ -1416,6 +1419,12 @@ static int add_particles_impl(ps_system *s, uint32_t scene, uint32_t n, const fl
     s->wf = wf;
     s->any_contact_phase = contact;
     s->b.contacts = contact ? 1u : 0u;
+    // the solve runs units without contact-walking particles through the
+    // variant without contacts when such particles are rare (at most 1 in
+    // 32: ropes in fluid); with more of them (cloth, rigid bodies) most
+    // units hold one and a second launch over the others would not pay
+    s->n_walk = nwalk;
+    s->b.split_solve = (contact && nwalk * 32u <= (uint64_t)s->n) ? 1u : 0u;
     s->any_solid_phase = solid;
     s->nbr_valid = false;
     return PS_OK;

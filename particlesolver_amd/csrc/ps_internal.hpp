@@ -142,6 +142,9 @@ struct DevBuffers {
     int32_t *phs;          // per slot, written with mj: the phase, its sign bit
                            // set for w = 0 (the staged solve's contact records)
     uint32_t contacts;     // the scene has contact phases (then phs is written)
+    uint32_t solve_units;  // staged solve launches: 0 every unit, 1 the units without a
+                           // contact-walking particle, 2 the units with one
+    uint32_t split_solve;  // such particles are rare: launch the two classes apart
     SceneDev *scenes;
     // grid
     uint32_t *cell_count;  // num_cells, zero between builds
@@ -248,7 +251,9 @@ struct AdjBuffers {
 // here before calling a launcher, and the launcher's kernel, started with
 // PS_LAUNCH, takes it (hipExtLaunchKernelGGL records the pair at the
 // kernel's own start and end, so no stream interval outside the kernel lies
-// between them) and clears it.  Launchers of two kernels (the scan) leave it.
+// between them) and clears it.  Launchers of two kernels leave it (the scan)
+// or split it (the solve of a scene with contacts: the first launch records
+// the start, the second the end).
 struct LaunchEvents {
     hipEvent_t start = nullptr, stop = nullptr;
 };
@@ -261,7 +266,7 @@ extern thread_local LaunchEvents g_launch_events;
     do {                                                                                         \
         const ps::LaunchEvents ev_ = ps::g_launch_events;                                        \
         ps::g_launch_events = ps::LaunchEvents{};                                                \
-        if (ev_.start)                                                                           \
+        if (ev_.start || ev_.stop)                                                               \
             hipExtLaunchKernelGGL(kern, grid, block, lds, st, ev_.start, ev_.stop, 0u, __VA_ARGS__); \
         else                                                                                     \
             hipLaunchKernelGGL(kern, grid, block, lds, st, __VA_ARGS__);                         \
