@@ -349,6 +349,15 @@ def _trace_report_ensemble(rows, tot, line, trace_csv):
                                           (t1 - t0) / 1e6 / subs, 100.0 * busy / (t1 - t0), len(gaps),
                                           sum(gaps) / 1e6, sum(gaps) / max(len(gaps), 1) / 1e3,
                                           max(gaps or [0]) / 1e3, sum(gaps) / 1e6 / subs)]
+        per = {}
+        for r in timed:
+            nm = r["Kernel_Name"].replace("(anonymous namespace)::", "").split("(")[0].replace("void ", "")
+            per.setdefault(nm, []).append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
+        out += ["", "The same replays by kernel (dispatches per substep, mean duration, time per substep):",
+                "", "| kernel | per substep | mean us | us per substep |", "|---|---|---|---|"]
+        for nm, ds in sorted(per.items(), key=lambda kv: -sum(kv[1])):
+            out.append("| `%s` | %.0f | %.1f | %.1f |" % (nm, len(ds) / subs, sum(ds) / len(ds),
+                                                      sum(ds) / subs))
     out += ["", "The bench line of the same command: batched %.4g particle-iterations/s (%.3f ms per "
             "substep); per-scene handles issued back to back %.4g (%.2fx), one after another %.4g (%.2fx). "
             "The tracer adds its cost to each of the per-scene runs' many small launches, so their rates "
