@@ -278,8 +278,7 @@ def trace_report(stats_csv, trace_csv, log):
     warm, steps = line["warmup"], line["steps"]
     out += ["", "Per-dispatch means over the bench's timed window (substeps %d-%d) from the same "
             "trace, and the bench line this run printed:" % (warm + 1, warm + steps), ""]
-    for name, per in (("solve_k", 4), ("lambda_k", 4), ("shape_k", 4),
-                      ("neighbors_k" if mixed else "neighbors_staged_k", 1),
+    for name, per in (("solve_k", 4), ("lambda_k", 4), ("shape_k", 4), ("neighbors_", 1),
                       ("finalize_k", 1), ("reorder_k", 1), ("predict_hash_k", 1)):
         ds = [(int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3
               for r in trace if name in r["Kernel_Name"]]

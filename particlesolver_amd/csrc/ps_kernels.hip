@@ -1444,20 +1444,22 @@ __global__ __launch_bounds__(kBlock) void shape_k(DevBuffers b, uint32_t nbody)
 
 static constexpr uint32_t kFinalizeStagedBlock = 4u * kStageUnit;
 
-template <bool UNIFORM, bool STAGED>
+template <bool UNIFORM, bool STAGED, bool MASS = false>
 __global__ __launch_bounds__(STAGED ? kFinalizeStagedBlock : kMaxBlock) void finalize_k(
     DevBuffers b, KConst k, uint32_t n, float mu, float dt, uint32_t its)
 {
     // staged: neighbour position and displacement u = p - x* (x, y, z, u.x)
     // and (u.y, u.z), formed with the same subtractions as the global path
-    // (and the sentinel record).  In a scene that is not a uniform fluid,
-    // with every fluid particle of one mass (mu >= 0), a particle that is
-    // not fluid is staged at infinity: the XSPH sum leaves it out, as the
-    // global path does by its x*.w < 0.  With fluid masses that differ
-    // (mu < 0) nothing is staged: the neighbours are gathered from global
-    // memory through the unit's chunk table (s_src)
+    // (and the sentinel record).  In a scene that is not a uniform fluid a
+    // particle that is not fluid is staged at infinity: the XSPH sum leaves
+    // it out, as the global path does by its x*.w < 0.  With every fluid
+    // particle of one mass (mu >= 0) the mass is mu; with fluid masses that
+    // differ (MASS) each slot's mass (x*.w) is staged as well, 12 KB more
+    // LDS (one block per CU instead of two).  A unit that was not staged
+    // gathers from global memory through the unit's chunk table (s_src).
     __shared__ float4 lds_pu[STAGED ? kStageSlots + 1 : 1];
     __shared__ float2 lds_u[STAGED ? kStageSlots + 1 : 1];
+    __shared__ float lds_m[STAGED && MASS ? kStageSlots + 1 : 1];
     __shared__ uint32_t s_src[STAGED && !UNIFORM ? kStageChunks : 1];
     const uint32_t unit = xcd_block();
     const uint32_t hh = threadIdx.x & 3u;
@@ -1466,7 +1468,7 @@ __global__ __launch_bounds__(STAGED ? kFinalizeStagedBlock : kMaxBlock) void fin
     if (STAGED) {
         const StageDesc *d = b.stage + unit;
         staged = d->staged != 0u;
-        mapped = !UNIFORM && staged && mu < 0.0f;
+        mapped = !UNIFORM && !MASS && staged && mu < 0.0f;
         if (mapped) {
             for (uint32_t q = threadIdx.x; q < d->nchunks; q += blockDim.x) s_src[q] = d->src[q];
             __syncthreads();
@@ -1476,8 +1478,18 @@ __global__ __launch_bounds__(STAGED ? kFinalizeStagedBlock : kMaxBlock) void fin
         if (staged && threadIdx.x == 0) {
             lds_pu[kStageSentinel] = make_float4(inf, inf, inf, 0.0f);
             lds_u[kStageSentinel] = make_float2(0.0f, 0.0f);
+            if (MASS) lds_m[kStageSentinel] = 0.0f;
         }
-        if (staged)
+        if (MASS && staged)
+            stage_copy<3, kFinalizeStagedBlock>(d, n, [&](uint32_t j) { return PX{b.P[j], b.X[j]}; },
+                                                [&](uint32_t l, const PX &v) {
+                                                    lds_pu[l] = v.x.w < 0.0f
+                                                                    ? make_float4(inf, inf, inf, 0.0f)
+                                                                    : make_float4(v.p.x, v.p.y, v.p.z, v.p.x - v.x.x);
+                                                    lds_u[l] = make_float2(v.p.y - v.x.y, v.p.z - v.x.z);
+                                                    lds_m[l] = v.x.w;
+                                                });
+        else if (staged)
             stage_copy<3, kFinalizeStagedBlock>(d, n, [&](uint32_t j) { return PX{b.P[j], b.X[j]}; },
                                                 [&](uint32_t l, const PX &v) {
                                                     lds_pu[l] = !UNIFORM && v.x.w < 0.0f
@@ -1523,7 +1535,16 @@ __global__ __launch_bounds__(STAGED ? kFinalizeStagedBlock : kMaxBlock) void fin
             if (d.x.w < 0.0f) return;
             body(d.p, d.p.x - d.x.x, d.p.y - d.x.y, d.p.z - d.x.z, d.x.w);
         };
-        if (STAGED && staged)
+        struct PUM {
+            float4 pu;
+            float2 u;
+            float m;
+        };
+        if (STAGED && MASS && staged)
+            for_neighbors_strided_chunked(
+                b, i, c, hh, 4u, [&](uint32_t l) { return PUM{lds_pu[l], lds_u[l], lds_m[l]}; },
+                [&](uint32_t, const PUM &d) { body(d.pu, d.pu.w, d.u.x, d.u.y, d.m); });
+        else if (STAGED && staged)
             for_neighbors_strided_chunked(
                 b, i, c, hh, 4u, [&](uint32_t l) { return PU{lds_pu[l], lds_u[l]}; },
                 [&](uint32_t, const PU &d) { body(d.pu, d.pu.w, d.u.x, d.u.y, mu); });
@@ -1780,9 +1801,12 @@ hipError_t launch_finalize(hipStream_t st, const DevBuffers &b, const KConst &k,
     if (uniform && b.staging)
         PS_LAUNCH((finalize_k<true, true>), dim3(blocks_for(n, kStageUnit)),
                            dim3(kFinalizeStagedBlock), 0, st, b, k, n, mu, dt, its);
-    else if (b.staging)  // wf >= 0: every fluid particle of inverse mass wf
+    else if (b.staging && wf >= 0.0f)  // every fluid particle of inverse mass wf
         PS_LAUNCH((finalize_k<false, true>), dim3(blocks_for(n, kStageUnit)),
-                           dim3(kFinalizeStagedBlock), 0, st, b, k, n, wf >= 0.0f ? 1.0f / wf : -1.0f, dt, its);
+                           dim3(kFinalizeStagedBlock), 0, st, b, k, n, 1.0f / wf, dt, its);
+    else if (b.staging)  // fluid masses that differ: staged with each slot's mass
+        PS_LAUNCH((finalize_k<false, true, true>), dim3(blocks_for(n, kStageUnit)),
+                           dim3(kFinalizeStagedBlock), 0, st, b, k, n, -1.0f, dt, its);
     else if (uniform)
         PS_LAUNCH((finalize_k<true, false>), g, t, 0, st, b, k, n, mu, dt, its);
     else
