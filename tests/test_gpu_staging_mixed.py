@@ -6,13 +6,14 @@ lambda) the same way, and in a unit holding a contact-walking particle also
 each slot's phase, so the counting walk runs from LDS (a partner with w = 0,
 staged at infinity, is read from memory: the rope below lands on a static
 slab) while the corrections gather through the unit's chunk table; XSPH
-stages when all fluid particles share one mass (else it gathers through the
-chunk table).  Where contact-walking particles are
-at most 1 in 32 (the rope scene below: 12 of 3,532) the solve is two
-launches, the units without such a particle through the variant without
-contacts.  A batch stages the units that lie in one scene.  None of this may change a bit: every run below is compared
-with the FP32 oracle and with the same system built with staging off
-(PS_FLAG_NO_LDS_STAGING), and checks that units did stage."""
+stages the neighbours as well, with each slot's mass where fluid masses
+differ (the scene below has two).  Where contact-walking particles are at
+most 1 in 32 (the rope scene below: 12 of 3,532) the solve is two launches,
+the units without such a particle through the variant without contacts.  A
+batch stages the units that lie in one scene.  None of this may change a
+bit: every run below is compared with the FP32 oracle and with the same
+system built with staging off (PS_FLAG_NO_LDS_STAGING), and checks that
+units did stage."""
 import numpy as np
 import pytest
 
