@@ -1422,7 +1422,11 @@ static int add_particles_impl(ps_system *s, uint32_t scene, uint32_t n, const fl
     // the solve runs units without contact-walking particles through the
     // variant without contacts when such particles are rare (at most 1 in
     // 32: ropes in fluid); with more of them (cloth, rigid bodies) most
-    // units hold one and a second launch over the others would not pay
+    // units hold one and a second launch over the others would not pay.
+    // Only the launch plan depends on this count: the neighbour build
+    // classifies the units each substep from the particles' phases and
+    // masses as they are then, so a count made stale by a later change of
+    // w or phase costs time, never a different result.
     s->n_walk = nwalk;
     s->b.split_solve = (contact && nwalk * 32u <= (uint64_t)s->n) ? 1u : 0u;
     s->any_solid_phase = solid;
