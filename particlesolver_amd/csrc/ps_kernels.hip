@@ -335,10 +335,20 @@ __global__ __launch_bounds__(kMaxBlock) void reorder_k(DevBuffers b, uint32_t n,
 // GLOB: also write the table with global slots (entries j, padding i) to
 // b.nbr_glob in the same layout (the copy a recording for the reverse pass
 // keeps while the staged kernels read LDS indices).
-template <bool GLOB = false, typename Range, typename Emit>
+// WAVE (the staged build; every lane of the wavefront calls it, valid or
+// not): per stencil row the wavefront's lanes, consecutive sorted particles,
+// mostly share one grid row, so their ranges lie in [lo, hi), lo the range
+// start of lane 0 and hi the range end of the last valid lane (lastv).
+// When that span fits kWaveRowSlots the wavefront copies it into its own
+// LDS (wl) with coalesced loads, and a lane whose range lies inside it
+// reads its candidates there instead of one global load per candidate and
+// lane (the loads that bound this kernel).  Any other lane or row reads
+// global memory as without WAVE; the table is the same.
+template <bool GLOB = false, bool WAVE = false, typename Range, typename Emit>
 __device__ __forceinline__ uint32_t scan_stencil(const DevBuffers &b, const KConst &k, uint32_t i,
                                                  uint32_t pad, const float4 p, uint32_t kmax,
-                                                 Range range, Emit emit)
+                                                 Range range, Emit emit, bool valid = true,
+                                                 float4 *wl = nullptr, uint32_t lastv = 63u)
 {
     uint4 *out = const_cast<uint4 *>(nbr_lane(b, i));
     uint4 *out2 = GLOB ? reinterpret_cast<uint4 *>(b.nbr_glob) + (size_t)(i >> 6) * b.kq * 64 + (i & 63)
@@ -366,10 +376,8 @@ __device__ __forceinline__ uint32_t scan_stencil(const DevBuffers &b, const KCon
         }
         cnt++;
     };
-    for (uint32_t t = 0; t < 9; t++) {
-        uint32_t j0, j1;
-        if (!range(t, j0, j1)) continue;
-        const uint32_t eoff = emit(t, 0u);
+    // the candidates [j0, j1) of one stencil row; q0 + (j - j0) holds slot j
+    auto scan_row = [&](uint32_t j0, uint32_t j1, uint32_t eoff, const float4 *q0) {
         for (uint32_t w0 = j0; w0 < j1; w0 += 32u) {
             const uint32_t we = min(w0 + 32u, j1);
             uint32_t m = 0;
@@ -378,7 +386,7 @@ __device__ __forceinline__ uint32_t scan_stencil(const DevBuffers &b, const KCon
             // the tests
             for (; jb + 4u <= we; jb += 4u) {
                 // one address per group, immediate offsets for the four
-                const float4 *q = b.P + jb;
+                const float4 *q = q0 + (jb - j0);
                 float4 qv[4];
 #pragma unroll
                 for (uint32_t u = 0; u < 4; u++) qv[u] = q[u];
@@ -392,7 +400,7 @@ __device__ __forceinline__ uint32_t scan_stencil(const DevBuffers &b, const KCon
             if (jb < we) {
                 float4 qv[3];
 #pragma unroll
-                for (uint32_t u = 0; u < 3; u++) qv[u] = b.P[min(jb + u, we - 1u)];
+                for (uint32_t u = 0; u < 3; u++) qv[u] = q0[min(jb + u, we - 1u) - j0];
                 uint32_t g = 0;
 #pragma unroll
                 for (uint32_t u = 0; u < 3; u++)
@@ -408,6 +416,34 @@ __device__ __forceinline__ uint32_t scan_stencil(const DevBuffers &b, const KCon
                 accept(w0 + u + eoff, w0 + u);
             }
         }
+    };
+    for (uint32_t t = 0; t < 9; t++) {
+        uint32_t j0 = 0, j1 = 0;
+        const bool ok = valid && range(t, j0, j1);
+        if constexpr (WAVE) {
+            // every lane is here: the span and the copy are the wavefront's
+            const uint32_t lo = __shfl(j0, 0, 64), hi = __shfl(j1, (int)lastv, 64);
+            const bool rowl = lo < hi && hi - lo <= kWaveRowSlots;
+            if (rowl) {
+                // the previous row's LDS reads were issued before these
+                // writes, and a wavefront's LDS operations run in order
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                for (uint32_t s = threadIdx.x & 63u; s < hi - lo; s += 64u) wl[s] = b.P[lo + s];
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            }
+            if (!ok) continue;
+            const uint32_t eoff = emit(t, 0u);
+            if (rowl && j0 >= lo && j1 <= hi)
+                scan_row(j0, j1, eoff, wl + (j0 - lo));
+            else
+                scan_row(j0, j1, eoff, b.P + j0);
+        } else {
+            if (!ok) continue;
+            scan_row(j0, j1, emit(t, 0u), b.P + j0);
+        }
     }
     const uint32_t kept = cnt < kmax ? cnt : kmax;
     // a partial last chunk: its r entries sit at the right end of buf
@@ -420,7 +456,7 @@ __device__ __forceinline__ uint32_t scan_stencil(const DevBuffers &b, const KCon
         else if (r == 2u) out2[(size_t)(kept >> 2) * 64] = make_uint4(buf2.z, buf2.w, i, i);
         else if (r == 3u) out2[(size_t)(kept >> 2) * 64] = make_uint4(buf2.y, buf2.z, buf2.w, i);
     }
-    b.ncnt[i] = kept;
+    if (valid) b.ncnt[i] = kept;
     return cnt;
 }
 
@@ -526,6 +562,7 @@ __global__ __launch_bounds__(kStageUnit) void neighbors_staged_k(DevBuffers b, K
     __shared__ uint32_t s_staged;
     __shared__ StageDesc s_desc;
     __shared__ uint32_t ldsw[kStageUnit / 64];
+    __shared__ float4 s_wrow[kStageUnit / 64][kWaveRowSlots];  // each wavefront's stencil row
     const uint32_t unit = xcd_block();
     const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
     const uint32_t i = unit * kStageUnit + tid;
@@ -637,7 +674,9 @@ __global__ __launch_bounds__(kStageUnit) void neighbors_staged_k(DevBuffers b, K
     for (uint32_t c = tid; c < nch; c += kStageUnit) desc->src[c] = s_desc.src[c];
 
     uint32_t cnt = 0;
-    if (valid) {
+    // the wavefront's first particle and its last valid lane
+    const uint32_t wfirst = unit * kStageUnit + (tid & ~63u);
+    if (wfirst < n) {
         auto range = [&](uint32_t t, uint32_t &j0, uint32_t &j1) {
             const int z = (int)rz + (int)(t / 3) - 1, y = (int)ry + (int)(t % 3) - 1;
             if (z < 0 || z >= (int)gz || y < 0 || y >= (int)gy) return false;
@@ -661,7 +700,10 @@ __global__ __launch_bounds__(kStageUnit) void neighbors_staged_k(DevBuffers b, K
             return staged ? j + (s_base[iv] - s_start[iv]) : j;
         };
         // a staged unit pads with its sentinel record, a global one with i
-        cnt = scan_stencil<GLOB>(b, k, i, staged ? kStageSentinel : i, b.P[i], kmax, range, emit);
+        const uint32_t lastv = min(63u, n - 1u - wfirst);
+        cnt = scan_stencil<GLOB, true>(b, k, i, staged ? kStageSentinel : i,
+                                       valid ? b.P[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f), kmax,
+                                       range, emit, valid, s_wrow[w], lastv);
     }
     neighbor_status(b, cnt, kmax, ldsw);
 }
