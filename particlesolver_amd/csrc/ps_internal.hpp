@@ -99,8 +99,8 @@ constexpr uint32_t kStageRows = 3;
 constexpr uint32_t kStageIv = 9 * kStageRows;
 constexpr uint32_t kStageSlots = 3072;
 // the staged neighbour build: slots of one stencil row a wavefront copies
-// into its LDS (3 KB)
-constexpr uint32_t kWaveRowSlots = 192;
+// into its LDS (x, y, z: 3 KB)
+constexpr uint32_t kWaveRowSlots = 256;
 constexpr uint32_t kStageChunk = 16;
 constexpr uint32_t kStageChunks = kStageSlots / kStageChunk;
 // LDS index of a staged unit's sentinel record, one past the staged slots:

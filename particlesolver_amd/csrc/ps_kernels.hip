@@ -348,7 +348,7 @@ template <bool GLOB = false, bool WAVE = false, typename Range, typename Emit>
 __device__ __forceinline__ uint32_t scan_stencil(const DevBuffers &b, const KConst &k, uint32_t i,
                                                  uint32_t pad, const float4 p, uint32_t kmax,
                                                  Range range, Emit emit, bool valid = true,
-                                                 float4 *wl = nullptr, uint32_t lastv = 63u)
+                                                 float *wl = nullptr, uint32_t lastv = 63u)
 {
     uint4 *out = const_cast<uint4 *>(nbr_lane(b, i));
     uint4 *out2 = GLOB ? reinterpret_cast<uint4 *>(b.nbr_glob) + (size_t)(i >> 6) * b.kq * 64 + (i & 63)
@@ -376,8 +376,8 @@ __device__ __forceinline__ uint32_t scan_stencil(const DevBuffers &b, const KCon
         }
         cnt++;
     };
-    // the candidates [j0, j1) of one stencil row; q0 + (j - j0) holds slot j
-    auto scan_row = [&](uint32_t j0, uint32_t j1, uint32_t eoff, const float4 *q0) {
+    // the candidates [j0, j1) of one stencil row; at(j - j0) reads slot j
+    auto scan_row = [&](uint32_t j0, uint32_t j1, uint32_t eoff, auto at) {
         for (uint32_t w0 = j0; w0 < j1; w0 += 32u) {
             const uint32_t we = min(w0 + 32u, j1);
             uint32_t m = 0;
@@ -386,10 +386,9 @@ __device__ __forceinline__ uint32_t scan_stencil(const DevBuffers &b, const KCon
             // the tests
             for (; jb + 4u <= we; jb += 4u) {
                 // one address per group, immediate offsets for the four
-                const float4 *q = q0 + (jb - j0);
                 float4 qv[4];
 #pragma unroll
-                for (uint32_t u = 0; u < 4; u++) qv[u] = q[u];
+                for (uint32_t u = 0; u < 4; u++) qv[u] = at(jb - j0 + u);
                 uint32_t g = 0;
 #pragma unroll
                 for (uint32_t u = 0; u < 4; u++)
@@ -400,7 +399,7 @@ __device__ __forceinline__ uint32_t scan_stencil(const DevBuffers &b, const KCon
             if (jb < we) {
                 float4 qv[3];
 #pragma unroll
-                for (uint32_t u = 0; u < 3; u++) qv[u] = q0[min(jb + u, we - 1u) - j0];
+                for (uint32_t u = 0; u < 3; u++) qv[u] = at(min(jb + u, we - 1u) - j0);
                 uint32_t g = 0;
 #pragma unroll
                 for (uint32_t u = 0; u < 3; u++)
@@ -429,20 +428,31 @@ __device__ __forceinline__ uint32_t scan_stencil(const DevBuffers &b, const KCon
                 // writes, and a wavefront's LDS operations run in order
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
-                for (uint32_t s = threadIdx.x & 63u; s < hi - lo; s += 64u) wl[s] = b.P[lo + s];
+                for (uint32_t s = threadIdx.x & 63u; s < hi - lo; s += 64u) {
+                    const float4 v = b.P[lo + s];
+                    wl[3u * s] = v.x;
+                    wl[3u * s + 1u] = v.y;
+                    wl[3u * s + 2u] = v.z;
+                }
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             }
             if (!ok) continue;
             const uint32_t eoff = emit(t, 0u);
-            if (rowl && j0 >= lo && j1 <= hi)
-                scan_row(j0, j1, eoff, wl + (j0 - lo));
-            else
-                scan_row(j0, j1, eoff, b.P + j0);
+            if (rowl && j0 >= lo && j1 <= hi) {
+                const float *w3 = wl + 3u * (j0 - lo);
+                scan_row(j0, j1, eoff, [&](uint32_t d) {
+                    return make_float4(w3[3u * d], w3[3u * d + 1u], w3[3u * d + 2u], 0.0f);
+                });
+            } else {
+                const float4 *q0 = b.P + j0;
+                scan_row(j0, j1, eoff, [&](uint32_t d) { return q0[d]; });
+            }
         } else {
             if (!ok) continue;
-            scan_row(j0, j1, emit(t, 0u), b.P + j0);
+            const float4 *q0 = b.P + j0;
+            scan_row(j0, j1, emit(t, 0u), [&](uint32_t d) { return q0[d]; });
         }
     }
     const uint32_t kept = cnt < kmax ? cnt : kmax;
@@ -562,7 +572,7 @@ __global__ __launch_bounds__(kStageUnit) void neighbors_staged_k(DevBuffers b, K
     __shared__ uint32_t s_staged;
     __shared__ StageDesc s_desc;
     __shared__ uint32_t ldsw[kStageUnit / 64];
-    __shared__ float4 s_wrow[kStageUnit / 64][kWaveRowSlots];  // each wavefront's stencil row
+    __shared__ float s_wrow[kStageUnit / 64][3 * kWaveRowSlots];  // each wavefront's stencil row (x, y, z)
     const uint32_t unit = xcd_block();
     const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
     const uint32_t i = unit * kStageUnit + tid;
